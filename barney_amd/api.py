@@ -1,0 +1,277 @@
+"""Python host layer over the C ABI: device arrays, worlds, volumes, renders.
+
+This mirrors the object model barney's host core drives through rtcore
+(rtc::Device buffers, Geom/Group/InstanceGroup builds, trace and compute
+launches); every call goes to libbnamd.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check
+
+_initialized = False
+
+
+def init(device=0):
+    global _initialized
+    lib = capi.load()
+    if not _initialized:
+        check(lib.bnamd_init(device))
+        _initialized = True
+    return lib
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a numpy array."""
+
+    def __init__(self, arr=None, nbytes=None, dtype=np.float32, shape=None):
+        lib = init()
+        if arr is not None:
+            arr = np.ascontiguousarray(arr)
+            self.dtype, self.shape, nbytes = arr.dtype, arr.shape, arr.nbytes
+        else:
+            self.dtype, self.shape = np.dtype(dtype), shape
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(lib.bnamd_device_malloc(C.byref(self.ptr), max(self.nbytes, 1)))
+        if arr is not None and self.nbytes:
+            check(lib.bnamd_memcpy_h2d(self.ptr, arr.ctypes.data, self.nbytes))
+
+    @classmethod
+    def empty(cls, shape, dtype=np.float32):
+        dt = np.dtype(dtype)
+        return cls(nbytes=int(np.prod(shape)) * dt.itemsize, dtype=dt, shape=tuple(shape))
+
+    def zero(self):
+        check(capi.load().bnamd_memset(self.ptr, 0, self.nbytes))
+        return self
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            check(capi.load().bnamd_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            capi.load().bnamd_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+GEOM_TRIANGLES, GEOM_SPHERES, GEOM_CYLINDERS = 0, 1, 2
+MAT_MATTE, MAT_PBR = 0, 1
+
+IDENTITY = (1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0)
+
+
+class Geom:
+    """Host description of one geometry (triangles / spheres / cylinders)."""
+
+    def __init__(self, kind, vertices, indices=None, radii=None, colors=None, default_radius=1.0,
+                 material=0, user_id=0):
+        self.kind = kind
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        self.indices = None if indices is None else np.ascontiguousarray(indices, dtype=np.int32)
+        self.radii = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+        self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 3)
+        self.default_radius = float(default_radius)
+        self.material = int(material)
+        self.user_id = int(user_id)
+
+    @property
+    def num_prims(self):
+        if self.kind == GEOM_TRIANGLES:
+            return len(self.indices.reshape(-1, 3))
+        if self.kind == GEOM_CYLINDERS:
+            return len(self.indices.reshape(-1, 2))
+        return len(self.vertices)
+
+
+def matte(color):
+    m = capi.bnamd_material()
+    m.type = MAT_MATTE
+    m.color[:] = [float(c) for c in color]
+    m.opacity = 1.0
+    return m
+
+
+class World:
+    """Acceleration structure over one instance group (identity instances)."""
+
+    def __init__(self, groups, materials=None, instances=None):
+        lib = init()
+        self._keep = []
+        cgroups = (capi.bnamd_group * len(groups))()
+        for gi, geoms in enumerate(groups):
+            arr = (capi.bnamd_geom * max(len(geoms), 1))()
+            for k, g in enumerate(geoms):
+                dv = DeviceArray(g.vertices)
+                di = DeviceArray(g.indices) if g.indices is not None else None
+                dr = DeviceArray(g.radii) if g.radii is not None else None
+                dc = DeviceArray(g.colors) if g.colors is not None else None
+                self._keep += [x for x in (dv, di, dr, dc) if x is not None]
+                e = arr[k]
+                e.kind, e.numPrims, e.numVertices = g.kind, g.num_prims, len(g.vertices)
+                e.materialID, e.userID = g.material, g.user_id
+                e.vertices = dv.addr
+                e.indices = di.addr if di else None
+                e.radii = dr.addr if dr else None
+                e.colors = dc.addr if dc else None
+                e.defaultRadius = g.default_radius
+            cgroups[gi].geoms = C.cast(arr, C.POINTER(capi.bnamd_geom))
+            cgroups[gi].numGeoms = len(geoms)
+            self._keep.append(arr)
+        if instances is None:
+            instances = [(gi, 0, IDENTITY) for gi in range(len(groups))]
+        cinst = (capi.bnamd_instance * max(len(instances), 1))()
+        for k, (g, iid, xfm) in enumerate(instances):
+            cinst[k].group, cinst[k].instID = g, iid
+            cinst[k].xfm[:] = [float(x) for x in xfm]
+        materials = materials or [matte((0.8, 0.8, 0.8))]
+        cmats = (capi.bnamd_material * len(materials))(*materials)
+        self.handle = C.c_void_p()
+        check(lib.bnamd_world_build(cgroups, len(groups), cinst, len(instances), cmats, len(materials), None,
+                                    C.byref(self.handle)))
+
+    def info(self):
+        inf = capi.bnamd_world_info()
+        check(capi.load().bnamd_world_info_get(self.handle, C.byref(inf)))
+        return {"numPrims": inf.numPrims, "numNodes": inf.numNodes, "depth": inf.depth,
+                "buildMs": inf.buildMs, "lower": list(inf.sceneLower), "upper": list(inf.sceneUpper)}
+
+    def trace(self, rays):
+        """rays: (n, 8) float32 [org, tmin, dir, tmax] -> structured hit array."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = DeviceArray(rays)
+        dh = DeviceArray.empty((n, 10), np.float32)
+        check(capi.load().bnamd_trace(self.handle, dr.addr, dh.addr, n, None))
+        return hits_from_raw(dh.numpy())
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            capi.load().bnamd_world_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def hits_from_raw(raw):
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    ints = raw.view(np.int32)
+    return {"t": raw[:, 0].copy(), "u": raw[:, 1].copy(), "v": raw[:, 2].copy(),
+            "primID": ints[:, 3].copy(), "geomID": ints[:, 4].copy(), "instID": ints[:, 5].copy(),
+            "N": raw[:, 6:9].copy(), "instIndex": ints[:, 9].copy()}
+
+
+def camera(position, direction, up=(0, 1, 0), fovy=60.0):
+    c = capi.bnamd_camera()
+    c.position[:] = [float(x) for x in position]
+    c.direction[:] = [float(x) for x in direction]
+    c.up[:] = [float(x) for x in up]
+    c.fovy = float(fovy)
+    return c
+
+
+def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
+                  max_diffuse=1, russian_roulette=0):
+    p = capi.bnamd_render_params()
+    p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
+    p.ambientRadiance = float(ambient)
+    p.bgColor[:] = [float(x) for x in bg]
+    p.maxPathDepth, p.maxDiffuseBounces, p.russianRoulette = max_depth, max_diffuse, russian_roulette
+    return p
+
+
+def render_surface(world, cam, params, with_ids=False, accum=None):
+    n = params.width * params.height
+    acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+    ids = DeviceArray.empty((params.height, params.width, 4), np.int32) if with_ids else None
+    st = capi.bnamd_render_stats()
+    check(capi.load().bnamd_render_surface(world.handle, C.byref(cam), C.byref(params), acc.addr,
+                                           ids.addr if ids else None, None, C.byref(st)))
+    out = {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays, "ms": st.ms},
+           "pixels": n}
+    if ids is not None:
+        out["ids"] = ids.numpy()
+    return out
+
+
+class Volume:
+    """Structured scalar field + macrocell grid + transfer function."""
+
+    def __init__(self, scalars, origin=(0, 0, 0), spacing=(1, 1, 1), cells_per_mc=8):
+        lib = init()
+        scalars = np.ascontiguousarray(scalars, dtype=np.float32)
+        nz, ny, nx = scalars.shape
+        self.dims = (nx, ny, nz)
+        self.field = DeviceArray(scalars)
+        d = capi.bnamd_structured_desc()
+        d.scalars = self.field.addr
+        d.dims[:] = [nx, ny, nz]
+        d.gridOrigin[:] = [float(x) for x in origin]
+        d.gridSpacing[:] = [float(x) for x in spacing]
+        d.cellsPerMC = int(cells_per_mc)
+        self.handle = C.c_void_p()
+        check(lib.bnamd_volume_create(C.byref(d), C.byref(self.handle)))
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 4)
+        check(capi.load().bnamd_volume_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]),
+                                              float(domain[1]), float(base_density)))
+
+    def mc_dims(self):
+        d = (C.c_int * 3)()
+        check(capi.load().bnamd_volume_mc_dims(self.handle, d))
+        return tuple(d)
+
+    def macrocells(self):
+        mx, my, mz = self.mc_dims()
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        check(capi.load().bnamd_volume_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data))
+        return ranges, maj
+
+    def sample_rays(self, rays, seeds, exact_log=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        n = len(rays)
+        dr, ds = DeviceArray(rays), DeviceArray(seeds)
+        do = DeviceArray.empty((n, 4), np.float32)
+        check(capi.load().bnamd_volume_sample_rays(self.handle, dr.addr, ds.addr, do.addr, n, int(exact_log), None))
+        return do.numpy()
+
+    def render(self, cam, params, accum=None):
+        acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+        st = capi.bnamd_volume_stats()
+        check(capi.load().bnamd_volume_render(self.handle, C.byref(cam), C.byref(params), acc.addr, None,
+                                              C.byref(st)))
+        return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
+                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            capi.load().bnamd_volume_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,119 @@
+"""ctypes declarations for libbnamd.so (include/bnamd.h).
+
+The library is the product: if it is missing or fails to load this module
+raises, there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbnamd.so")
+
+
+class bnamd_geom(C.Structure):
+    _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("numVertices", C.c_int),
+                ("materialID", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
+                ("radii", C.c_void_p), ("colors", C.c_void_p), ("defaultRadius", C.c_float),
+                ("userID", C.c_int)]
+
+
+class bnamd_group(C.Structure):
+    _fields_ = [("geoms", C.POINTER(bnamd_geom)), ("numGeoms", C.c_int)]
+
+
+class bnamd_instance(C.Structure):
+    _fields_ = [("group", C.c_int), ("instID", C.c_int), ("xfm", C.c_float * 12)]
+
+
+class bnamd_material(C.Structure):
+    _fields_ = [("type", C.c_int), ("color", C.c_float * 3), ("opacity", C.c_float),
+                ("metallic", C.c_float), ("roughness", C.c_float), ("transmission", C.c_float),
+                ("ior", C.c_float), ("specular", C.c_float)]
+
+
+class bnamd_world_info(C.Structure):
+    _fields_ = [("numPrims", C.c_longlong), ("numNodes", C.c_longlong), ("depth", C.c_int),
+                ("buildMs", C.c_float), ("sceneLower", C.c_float * 3), ("sceneUpper", C.c_float * 3)]
+
+
+class bnamd_camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float)]
+
+
+class bnamd_render_params(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int),
+                ("pathsPerPixel", C.c_int), ("ambientRadiance", C.c_float),
+                ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
+                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int)]
+
+
+class bnamd_render_stats(C.Structure):
+    _fields_ = [("raysTraced", C.c_longlong), ("primaryRays", C.c_longlong), ("ms", C.c_float)]
+
+
+class bnamd_structured_desc(C.Structure):
+    _fields_ = [("scalars", C.c_void_p), ("dims", C.c_int * 3), ("gridOrigin", C.c_float * 3),
+                ("gridSpacing", C.c_float * 3), ("cellsPerMC", C.c_int)]
+
+
+class bnamd_volume_stats(C.Structure):
+    _fields_ = [("samples", C.c_longlong), ("ddaSteps", C.c_longlong), ("raysTraced", C.c_longlong),
+                ("ms", C.c_float)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "bnamd_init": (C.c_int, [C.c_int]),
+    "bnamd_last_error": (C.c_char_p, []),
+    "bnamd_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "bnamd_device_free": (C.c_int, [C.c_void_p]),
+    "bnamd_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "bnamd_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "bnamd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "bnamd_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
+    "bnamd_synchronize": (C.c_int, []),
+    "bnamd_world_build": (C.c_int, [C.POINTER(bnamd_group), C.c_int, C.POINTER(bnamd_instance), C.c_int,
+                                    C.POINTER(bnamd_material), C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bnamd_world_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_world_info_get": (C.c_int, [C.c_void_p, C.POINTER(bnamd_world_info)]),
+    "bnamd_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_render_surface": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(bnamd_render_stats)]),
+    "bnamd_volume_create": (C.c_int, [C.POINTER(bnamd_structured_desc), C.POINTER(C.c_void_p)]),
+    "bnamd_volume_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_volume_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "bnamd_volume_mc_dims": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "bnamd_volume_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_volume_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                      C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
+    "bnamd_volume_sample_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                           C.c_void_p]),
+}
+
+_lib = None
+
+
+class BnamdError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libbnamd.so and declares every entry point; raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BnamdError(f"{LIB_PATH} is not built (run python -m barney_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        raise BnamdError(load().bnamd_last_error().decode())

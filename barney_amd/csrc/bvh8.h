@@ -1,0 +1,344 @@
+// Compressed 8-wide BVH for gfx950: node layout, primitive records and the
+// closest-hit traversal with a per-wavefront LDS stack.
+//
+// Replaces the acceleration structures built by rtcore/cuda/cudaGroup.cu
+// (cuBQL binary BVHs, :144-562) and walked by
+// rtcore/cuda/TraceInterface.h:102-261; the closest-hit semantics follow the
+// embree backend (rtcore/embree/Device.cpp:107-196, Group.cpp:34-79).
+#pragma once
+#include "common.h"
+
+namespace bnamd {
+
+enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2 };
+
+// One node = one 128-byte line. Child boxes are quantized to 16 bits per
+// plane relative to the node origin with a power-of-two scale per axis.
+//   n0: ox oy oz | ex ey ez imask
+//   n1: childBase | primBase | meta[0..3] | meta[4..7]
+//   n2..n7: qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8] qhi.z[8] (u16)
+// meta[slot]: leaf child -> (count << 5) | offset into the node's prim block;
+// internal child -> imask bit set, node = childBase + rank among internal
+// slots; empty -> 0 with an inverted (empty) box.
+struct alignas(128) BNode8 {
+  float ox, oy, oz;
+  uint8_t ex, ey, ez, imask;
+  uint32_t childBase;
+  uint32_t primBase;
+  uint8_t meta[8];
+  uint16_t q[6][8];
+};
+static_assert(sizeof(BNode8) == 128, "BNode8 must be one 128B line");
+
+// Primitive record (48 B, three float4) stored in BVH leaf order.
+//   triangle: a = v0 | flat,  b = e1 = v1-v0 | -, c = e2 = v2-v0 | type
+//   sphere:   a = center | flat, b = (radius,0,0) | -, c = 0 | type
+//   cylinder: a = v0 | flat, b = v1 | -, c = (radius,0,0) | type
+// 'flat' is the primitive's index in instance-major, geom-major, prim-minor
+// order; it doubles as the deterministic tie-break key at equal t.
+struct PrimRec {
+  float4 a, b, c;
+};
+
+struct HitRec {
+  float t;
+  float u, v;      // triangle barycentrics; sphere: u = hit t in the shifted frame
+  uint32_t flat;   // 0xffffffff = no hit
+};
+
+// Deterministic acceptance: closest t wins, ties at equal t go to the
+// smallest (instance, geom, prim) tuple, i.e. the smallest flat index.
+BN_HD bool better(float t, uint32_t flat, const HitRec &best) {
+  return t < best.t || (t == best.t && flat < best.flat);
+}
+
+// ---------------------------------------------------------------------------
+// Primitive intersectors. Each computes the candidate t exactly as the
+// reference program does (same fp32 operation order, no FMA contraction),
+// tested against the ray's original [tmin, tmax]; acceptance against the
+// current closest hit then uses better().
+// ---------------------------------------------------------------------------
+
+// rtcore/cuda/TraceInterface.h:32-98 (barney's own plane + 2D-projection
+// triangle test).
+BN_HD void isect_triangle(vec3f org, vec3f dir, float tmax, vec3f v0, vec3f e1, vec3f e2,
+                          uint32_t flat, HitRec &best) {
+#pragma clang fp contract(off)
+  const vec3f N = cross(e1, e2);
+  const float dd = dot(dir, N);
+  if (fabsf(dd) < 1e-12f) return;
+  if (dd == 0.f) return;
+  const float t = -dot(org - v0, N) / dd;
+  if (t <= 0.f || t >= tmax) return;
+  if (t > best.t) return;
+  const vec3f P = org - v0 + t * dir;
+  float e1u, e2u, Pu, e1v, e2v, Pv;
+  if (fabsf(N.x) >= fmaxf(fabsf(N.y), fabsf(N.z))) {
+    e1u = e1.y; e2u = e2.y; Pu = P.y;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else if (fabsf(N.y) > fabsf(N.z)) {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.y; e2v = e2.y; Pv = P.y;
+  }
+  const float den = e1u * e2v - e2u * e1v;
+  if (den == 0.f) return;
+  const float u = (Pu * e2v - Pv * e2u) / den;
+  const float v = (e1u * Pv - e1v * Pu) / den;
+  if ((u < 0.f) || (v < 0.f) || ((u + v) >= 1.f)) return;
+  if (!better(t, flat, best)) return;
+  best.t = t;
+  best.u = u;
+  best.v = v;
+  best.flat = flat;
+}
+
+// barney/geometry/Spheres.dev.cu:116-170 ("move the origin" trick).
+// Returns the shifted-frame hit in best.u so the closest-hit epilogue can
+// rebuild the object-space point bit-identically.
+BN_HD void isect_sphere(vec3f org, vec3f dir, float tmin, float tmax, vec3f center, float radius,
+                        uint32_t flat, HitRec &best) {
+#pragma clang fp contract(off)
+  const float t_move = fmaxf(0.f, length(center - org) - 3.f * radius);
+  const vec3f o = org + t_move * dir;
+  const float t_max = tmax - t_move;
+  if (t_max < 0.f) return;
+  float hit_t = t_max;
+  const float t_min = fmaxf(0.f, tmin - t_move);
+  const vec3f oc = o - center;
+  const float a = dot(dir, dir);
+  const float b = dot(oc, dir);
+  const float c = dot(oc, oc) - radius * radius;
+  const float disc = b * b - a * c;
+  if (disc < 0.f) return;
+  const float sq = sqrtf(disc);
+  {
+    const float tt = (-b - sq) / a;
+    if (tt < hit_t && tt > t_min) hit_t = tt;
+  }
+  {
+    const float tt = (-b + sq) / a;
+    if (tt < hit_t && tt > t_min) hit_t = tt;
+  }
+  if (!(hit_t < t_max)) return;
+  const float t = hit_t + t_move;
+  if (!better(t, flat, best)) return;
+  best.t = t;
+  best.u = hit_t;
+  best.v = t_move;
+  best.flat = flat;
+}
+
+// barney/geometry/Cylinders.dev.cu:101-216 (capped cylinder). Writes the
+// object-space normal of the chosen side when N != nullptr.
+BN_HD bool cylinder_hit(vec3f ray_org, vec3f ray_dir, float ray_tmin, float ray_tmax, vec3f v0,
+                        vec3f v1, float radius, float &t_out, vec3f *N) {
+#pragma clang fp contract(off)
+  const vec3f d = ray_dir;
+  const vec3f s = v1 - v0;
+  const vec3f sxd = cross(s, d);
+  const float a = dot(sxd, sxd);
+  if (a == 0.f) return false;
+  const vec3f f = v0 - ray_org;
+  const vec3f sxf = cross(s, f);
+  const float ra = 1.f / a;
+  const float ts = dot(sxd, sxf) * ra;
+  const vec3f fp = f - ts * d;
+  const float s2 = dot(s, s);
+  const vec3f perp = cross(s, fp);
+  const float c = radius * radius * s2 - dot(perp, perp);
+  if (c < 0.f) return false;
+  float td = sqrtf(c * ra);
+  const float tube_t0 = ts - td;
+  const float tube_t1 = ts + td;
+  const float sf = dot(s, f);
+  const float sd = dot(s, d);
+  float cap_t0 = -1e20f, cap_t1 = +1e20f;
+  float cap_t_v0 = cap_t0, cap_t_v1 = cap_t1;
+  if (sd == 0.f) {
+    if (dot(ray_org - v0, v1 - v0) < 0.f) return false;
+    if (dot(ray_org - v1, v0 - v1) < 0.f) return false;
+  } else {
+    const float rsd = 1.f / sd;
+    cap_t_v0 = sf * rsd;
+    cap_t_v1 = cap_t_v0 + s2 * rsd;
+    cap_t0 = fminf(cap_t_v0, cap_t_v1);
+    cap_t1 = fmaxf(cap_t_v0, cap_t_v1);
+  }
+  const float t0 = fmaxf(cap_t0, tube_t0);
+  const float t1 = fminf(cap_t1, tube_t1);
+  if (t0 > t1) return false;
+  float thit;
+  if (ray_tmin <= t0 && t0 <= ray_tmax) {
+    thit = t0;
+    if (N) {
+      td *= -1.f;
+      const float hit_surf_u = (thit * sd - sf) * (1.f / s2);
+      if (t0 == cap_t0)
+        *N = (cap_t0 == cap_t_v0) ? -s : s;
+      else
+        *N = td * d - fp - hit_surf_u * s;
+    }
+  } else if (ray_tmin <= t1 && t1 <= ray_tmax) {
+    thit = t1;
+    if (N) {
+      const float hit_surf_u = (thit * sd - sf) * (1.f / s2);
+      // Cylinders.dev.cu:221 compares t0 (not t1) against cap_t1 here;
+      // kept as-is so exit-side cap hits get the tube normal formula.
+      if (t0 == cap_t1)
+        *N = (cap_t0 == cap_t_v0) ? -s : s;
+      else
+        *N = td * d - fp - hit_surf_u * s;
+    }
+  } else {
+    return false;
+  }
+  t_out = thit;
+  return true;
+}
+
+BN_HD void isect_cylinder(vec3f org, vec3f dir, float tmin, float tmax, vec3f v0, vec3f v1,
+                          float radius, uint32_t flat, HitRec &best) {
+  float t;
+  if (!cylinder_hit(org, dir, tmin, tmax, v0, v1, radius, t, nullptr)) return;
+  if (!better(t, flat, best)) return;
+  best.t = t;
+  best.u = 0.f;
+  best.v = 0.f;
+  best.flat = flat;
+}
+
+// barney clamps exactly-zero direction components before tracing
+// (barney/kernels/traceRays.dev.cu:42-44).
+BN_HD vec3f fix_dir(vec3f d) {
+  if (d.x == 0.f) d.x = 1e-6f;
+  if (d.y == 0.f) d.y = 1e-6f;
+  if (d.z == 0.f) d.z = 1e-6f;
+  return d;
+}
+
+
+BN_D void test_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, vec3f dir,
+                    float tmin, float tmax, HitRec &best) {
+  const float4 a = prims[idx].a;
+  const float4 b = prims[idx].b;
+  const float4 c = prims[idx].c;
+  const uint32_t type = __float_as_uint(c.w);
+  const uint32_t flat = __float_as_uint(a.w);
+  const vec3f p0 = {a.x, a.y, a.z};
+  if (type == PRIM_TRIANGLE) {
+    isect_triangle(org, dir, tmax, p0, {b.x, b.y, b.z}, {c.x, c.y, c.z}, flat, best);
+  } else if (type == PRIM_SPHERE) {
+    isect_sphere(org, dir, tmin, tmax, p0, b.x, flat, best);
+  } else {
+    isect_cylinder(org, dir, tmin, tmax, p0, {b.x, b.y, b.z}, c.x, flat, best);
+  }
+}
+
+// Stack entry: node group = {childBase, (hitbits | imask << 8)}.
+// The per-lane stack lives in LDS as stack[depth][lanes] so consecutive
+// lanes hit consecutive banks.
+struct TraverseCtx {
+  uint2 *stack;    // LDS base for this thread (already offset by lane)
+  int stride;      // lanes per stack level
+  int maxDepth;
+};
+
+// Closest-hit traversal of the whole world BVH8. Returns false if the stack
+// would have overflowed (the builder reports depth, so this does not happen
+// when the kernel is launched with that depth).
+BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims,
+                           vec3f org, vec3f dir, float tmin, float tmax, HitRec &best,
+                           const TraverseCtx &ctx) {
+  best.t = tmax;
+  best.flat = 0xffffffffu;
+  best.u = best.v = 0.f;
+  const vec3f inv = rcp(dir);
+  const vec3f oi = org * inv;
+  const uint32_t oct = (dir.x < 0.f ? 1u : 0u) | (dir.y < 0.f ? 2u : 0u) | (dir.z < 0.f ? 4u : 0u);
+
+  uint32_t grpBase = 0;
+  uint32_t grpBits = (1u << oct) | (1u << 8);  // root = slot 0 (k = 0 ^ oct)
+  uint32_t primBase = 0, primMask = 0;
+  int sp = 0;
+  while (true) {
+    if ((grpBits & 0xffu) != 0) {
+      const uint32_t k = __builtin_ctz(grpBits);
+      grpBits &= ~(1u << k);
+      const uint32_t slot = k ^ oct;
+      const uint32_t imask = grpBits >> 8;
+      const uint32_t nodeIdx = grpBase + __builtin_popcount(imask & ((1u << slot) - 1u));
+      if ((grpBits & 0xffu) != 0) {
+        if (sp >= ctx.maxDepth) return false;
+        ctx.stack[sp * ctx.stride] = make_uint2(grpBase, grpBits);
+        ++sp;
+      }
+      const uint4 *np = (const uint4 *)(nodes + nodeIdx);
+      const uint4 n0 = np[0];
+      const uint4 n1 = np[1];
+      const uint4 qx0 = np[2], qy0 = np[3], qz0 = np[4];
+      const uint4 qx1 = np[5], qy1 = np[6], qz1 = np[7];
+      const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
+      const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
+      const uint32_t nimask = n0.w >> 24;
+      const float sx = __uint_as_float(ex << 23) * inv.x;
+      const float sy = __uint_as_float(ey << 23) * inv.y;
+      const float sz = __uint_as_float(ez << 23) * inv.z;
+      const float bx = ox * inv.x - oi.x;
+      const float by = oy * inv.y - oi.y;
+      const float bz = oz * inv.z - oi.z;
+      const float tcap = best.t;
+      uint32_t hitInner = 0, hitPrims = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t wlx = (&qx0.x)[c >> 1], wly = (&qy0.x)[c >> 1], wlz = (&qz0.x)[c >> 1];
+        const uint32_t whx = (&qx1.x)[c >> 1], why = (&qy1.x)[c >> 1], whz = (&qz1.x)[c >> 1];
+        const int sh = (c & 1) * 16;
+        const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
+                    lz = (float)((wlz >> sh) & 0xffffu);
+        const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
+                    hz = (float)((whz >> sh) & 0xffffu);
+        const float tlx = fmaf(lx, sx, bx), thx = fmaf(hx, sx, bx);
+        const float tly = fmaf(ly, sy, by), thy = fmaf(hy, sy, by);
+        const float tlz = fmaf(lz, sz, bz), thz = fmaf(hz, sz, bz);
+        float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), tmin));
+        float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fmaxf(tlz, thz));
+        // conservative widening against rounding in the quantized slab test
+        tn = tn - fabsf(tn) * 2e-6f;
+        tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
+        const bool hit = (tn <= tf) && (lx <= hx);
+        if (hit) {
+          const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
+          if (nimask & (1u << c)) {
+            hitInner |= 1u << (c ^ oct);
+          } else {
+            const uint32_t cnt = meta >> 5, off = meta & 31u;
+            hitPrims |= ((1u << cnt) - 1u) << off;
+          }
+        }
+      }
+      grpBase = n1.x;
+      grpBits = hitInner | (nimask << 8);
+      primBase = n1.y;
+      primMask = hitPrims;
+    }
+    while (primMask) {
+      const uint32_t i = __builtin_ctz(primMask);
+      primMask &= primMask - 1u;
+      test_prim(prims, primBase + i, org, dir, tmin, tmax, best);
+    }
+    if ((grpBits & 0xffu) == 0) {
+      if (sp == 0) break;
+      --sp;
+      const uint2 e = ctx.stack[sp * ctx.stride];
+      grpBase = e.x;
+      grpBits = e.y;
+    }
+  }
+  return true;
+}
+
+
+}  // namespace bnamd

@@ -1,0 +1,176 @@
+// Plain per-pixel megakernel: for each pixel and path, generate the camera
+// ray, then trace / shade / trace-shadow until the path ends, accumulating
+// into the pixel. This fuses barney's generateRays -> {traceRays, shadeRays}*
+// generation loop (barney/Context.cpp:142-185) into one thread per pixel:
+// each ray of a path only depends on its own RNG state, so per-pixel
+// depth-first order computes the same values as the queue-based order.
+#include "shade.h"
+
+namespace bnamd {
+
+size_t trace_lds_bytes(int depth, int block);
+
+namespace {
+
+// Pixel tiles of 8x8 per wavefront, 4 wavefronts per workgroup (16x16).
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
+// receives a contiguous band of tiles and keeps its L2 working set local.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  const int per = (nb + 7) >> 3;
+  const int xcd = bid & 7, local = bid >> 3;
+  const int t = xcd * per + local;
+  return t < nb ? t : -1;
+}
+
+__device__ __forceinline__ ShadeHit shade_surface(const WorldDev &w, vec3f org, vec3f dir, float tmax,
+                                                  const HitRec &h) {
+  ShadeHit sh;
+  if (h.flat == 0xffffffffu) {
+    sh.hit = false;
+    return sh;
+  }
+  const SurfaceHit s = surface_epilogue(w, org, dir, 0.f, tmax, h);
+  const GeomDev &g = w.geoms[s.geom];
+  const MaterialDev &m = w.materials[g.materialID];
+  vec3f color = {m.color[0], m.color[1], m.color[2]};
+  if (g.colors) color = load3(g.colors + 3 * s.primID);
+  sh.hit = true;
+  sh.P = s.P;
+  sh.N = s.Ng;
+  sh.t = h.t;
+  sh.bsdf = matte_bsdf(color);
+  return sh;
+}
+
+// shadeRays.cu:1030-1037 adds only positive channels of a clamped fragment.
+__device__ __forceinline__ void add_fragment(vec3f &sum, vec3f frag, float clampMax) {
+  frag = vmin(frag, clampMax);
+  if (frag.x > 0.f) sum.x += frag.x;
+  if (frag.y > 0.f) sum.y += frag.y;
+  if (frag.z > 0.f) sum.z += frag.z;
+}
+
+__global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam, RenderDD rp, int spp,
+                                                        float4 *__restrict accum, int4 *__restrict ids,
+                                                        unsigned long long *__restrict counters) {
+  extern __shared__ uint2 lds_stack[];
+  const int tilesX = (rp.width + 15) >> 4, tilesY = (rp.height + 15) >> 4;
+  const int tile = xcd_remap(blockIdx.x, tilesX * tilesY);
+  if (tile < 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ix = (tile % tilesX) * 16 + (wave & 1) * 8 + (lane & 7);
+  const int iy = (tile / tilesX) * 16 + (wave >> 1) * 8 + (lane >> 3);
+  const bool active = ix < rp.width && iy < rp.height;
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
+  unsigned long long nrays = 0;
+  if (active) {
+    const int pix = ix + iy * rp.width;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int accum0 = rp.accumID;
+    for (int p = 0; p < spp; ++p) {
+      RenderDD rpp = rp;
+      rpp.accumID = accum0 + p;
+      PathRay ray = camera_ray(cam, rpp, ix, iy);
+      const float clampMax = 10.f * (1 + rpp.accumID);
+      vec3f sum = vec3f(0.f);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const vec3f dir = fix_dir(ray.dir);
+        HitRec h;
+        traverse_closest(w.nodes, w.prims, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        ++nrays;
+        const ShadeHit sh = shade_surface(w, ray.org, dir, ray.tMax, h);
+        if (gen == 0) {
+          alpha = sh.hit ? 1.f : ray.missAlpha;
+          if (ids && p == 0 && accum0 == 0) {
+            int4 rec = make_int4(-1, -1, -1, __float_as_int(BN_INF));
+            if (sh.hit) {
+              const SurfaceHit s = surface_epilogue(w, ray.org, dir, 0.f, ray.tMax, h);
+              const GeomDev &g = w.geoms[s.geom];
+              // Spheres.dev.cu:192-195 writes 13+rank as the instance ID.
+              rec = make_int4(s.primID, g.kind == BNAMD_GEOM_SPHERES ? 13 : g.instID, g.userID,
+                              __float_as_int(h.t));
+            }
+            ids[pix] = rec;
+          }
+        }
+        PathRay shadow;
+        vec3f frag = bounce(rpp, ray, sh, shadow, gen);
+        add_fragment(sum, frag, clampMax);
+        if (shadow.tMax > 0.f) {
+          const vec3f sdir = fix_dir(shadow.dir);
+          HitRec hs;
+          traverse_closest(w.nodes, w.prims, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx);
+          ++nrays;
+          ShadeHit shs;
+          shs.hit = hs.flat != 0xffffffffu;
+          PathRay dummy;
+          frag = bounce(rpp, shadow, shs, dummy, gen + 1);
+          add_fragment(sum, frag, clampMax);
+        }
+      }
+      acc.x += sum.x;
+      acc.y += sum.y;
+      acc.z += sum.z;
+      acc.w += alpha;
+    }
+    if (accum0 == 0) {
+      accum[pix] = acc;
+    } else {
+      float4 o = accum[pix];
+      o.x += acc.x; o.y += acc.y; o.z += acc.z; o.w += acc.w;
+      accum[pix] = o;
+    }
+  }
+  // one atomic per wavefront for the ray counter
+  for (int off = 32; off > 0; off >>= 1) nrays += __shfl_xor(nrays, off);
+  if (lane == 0) atomicAdd(counters, nrays);
+}
+
+}  // namespace
+
+const World &world_of(const bnamd_world *w);
+
+}  // namespace bnamd
+
+using namespace bnamd;
+
+extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera *camera,
+                                    const bnamd_render_params *params, float *accum, int *ids, void *stream,
+                                    bnamd_render_stats *stats) {
+  return guarded([&] {
+    BN_REQUIRE(world && camera && params && accum, "bnamd_render_surface: NULL argument");
+    BN_REQUIRE(params->width > 0 && params->height > 0, "bnamd_render_surface: empty frame");
+    BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_render_surface: pathsPerPixel must be >= 1");
+    const World &w = world_of(world);
+    const CameraDD cam = make_camera(*camera);
+    const RenderDD rp = make_render_dd(*params);
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf<unsigned long long> counter(1);
+    BN_HIP(hipMemsetAsync(counter.ptr, 0, sizeof(unsigned long long), s));
+    const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
+    const int block = 256;
+    const size_t lds = trace_lds_bytes(w.depth, block);
+    BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_surface: BVH too deep for the LDS stack");
+    hipEvent_t e0, e1;
+    BN_HIP(hipEventCreate(&e0));
+    BN_HIP(hipEventCreate(&e1));
+    BN_HIP(hipEventRecord(e0, s));
+    k_render_surface<<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+                                               (int4 *)ids, counter.ptr);
+    BN_HIP(hipGetLastError());
+    BN_HIP(hipEventRecord(e1, s));
+    BN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long n = 0;
+    BN_HIP(hipMemcpy(&n, counter.ptr, sizeof(n), hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->raysTraced = (long long)n;
+      stats->primaryRays = (long long)params->width * params->height * params->pathsPerPixel;
+      stats->ms = ms;
+    }
+  });
+}

@@ -1,0 +1,66 @@
+// Device/memory entry points of the C ABI (the rtc::Device memory surface of
+// rtcore/cudaCommon/Device.h:34-64) and the last-error slot.
+#include "runtime.h"
+#include "../../include/bnamd.h"
+#include <mutex>
+
+namespace bnamd {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace bnamd
+
+using namespace bnamd;
+
+extern "C" const char *bnamd_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int bnamd_init(int device) {
+  return guarded([&] {
+    int n = 0;
+    BN_HIP(hipGetDeviceCount(&n));
+    BN_REQUIRE(n > 0, "bnamd_init: no HIP device available");
+    BN_REQUIRE(device >= 0 && device < n, "bnamd_init: device index out of range");
+    BN_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BN_HIP(hipGetDeviceProperties(&prop, device));
+    BN_REQUIRE(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
+               std::string("bnamd_init: built for gfx950, device is ") + prop.gcnArchName);
+  });
+}
+
+extern "C" int bnamd_device_malloc(void **dptr, size_t bytes) {
+  return guarded([&] {
+    BN_REQUIRE(dptr, "bnamd_device_malloc: NULL out pointer");
+    *dptr = nullptr;
+    if (bytes) BN_HIP(hipMalloc(dptr, bytes));
+  });
+}
+extern "C" int bnamd_device_free(void *dptr) {
+  return guarded([&] {
+    if (dptr) BN_HIP(hipFree(dptr));
+  });
+}
+extern "C" int bnamd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+extern "C" int bnamd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+extern "C" int bnamd_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  });
+}
+extern "C" int bnamd_memset(void *dst, int value, size_t bytes) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemset(dst, value, bytes));
+  });
+}
+extern "C" int bnamd_synchronize(void) {
+  return guarded([&] { BN_HIP(hipDeviceSynchronize()); });
+}

@@ -1,0 +1,266 @@
+// Structured-volume sampling with a macrocell majorant grid and delta
+// (Woodcock) tracking, restating:
+//   StructuredDataSampler::DD::sample    barney/volume/StructuredData.h:104-115
+//   tex3D LINEAR + CLAMP (embree oracle) rtcore/embree/Texture.cpp:324-416
+//   TransferFunction::DD::{map,majorant} barney/volume/TransferFunction.h:67-116
+//   fastLog / Woodcock::sampleRange      barney/volume/Volume.h:174-223
+//   dda::dda3                            barney/volume/DDA.h:65-206
+//   MCVolumeAccel::isProg                barney/volume/MCAccelerator.h:569-708
+//   boxTest                              barney/render/Ray.h:176-197
+#pragma once
+#include "common.h"
+
+namespace bnamd {
+
+struct TFDD {
+  const float4 *values;
+  int numValues;
+  float domLo, domHi;
+  float baseDensity;
+};
+
+struct VolumeDD {
+  const float *scalars;
+  int nx, ny, nz;          // numScalars
+  vec3f origin, spacing;   // cell grid
+  vec3f worldLo, worldHi;  // origin + spacing * numCells
+  // macrocell grid
+  const float *majorants;
+  int mx, my, mz;
+  vec3f mcOrigin, mcSpacing;
+  TFDD tf;
+};
+
+struct VolSample {
+  float r, g, b, w;
+};
+
+// Volume.h:174-184: 3-term atanh series for log.
+BN_HD float fast_log(float f) {
+#pragma clang fp contract(off)
+  f = (f - 1.f) / (f + 1.f);
+  const float f2 = f * f;
+  float s = f;
+  f *= f2;
+  s += (1.f / 3.f) * f;
+  f *= f2;
+  s += (1.f / 5.f) * f;
+  return s + s;
+}
+
+// embree Texture.cpp:326-335 (lerpCoords_notNormalized)
+BN_HD void lerp_coords(int N, float tc, int &i0, int &i1, float &f) {
+#pragma clang fp contract(off)
+  if (tc - .5f <= 0.f) { i0 = i1 = 0; f = 0.f; return; }
+  if (tc - .5f >= N - 1) { i0 = i1 = N - 1; f = 0.f; return; }
+  f = (tc - .5f) - int(tc - .5f);
+  i0 = int(tc - .5f);
+  i1 = i0 + 1;
+}
+
+BN_HD float lerp_l(float f, float a, float b) {
+#pragma clang fp contract(off)
+  return (1.f - f) * a + f * b;
+}
+
+// Trilinear sample of the scalar field at world point P; NaN outside.
+BN_HD float sample_field(const VolumeDD &v, vec3f P) {
+#pragma clang fp contract(off)
+  const vec3f rel = (P - v.origin) * rcp(v.spacing);
+  if (rel.x < 0.f) return __builtin_nanf("");
+  if (rel.y < 0.f) return __builtin_nanf("");
+  if (rel.z < 0.f) return __builtin_nanf("");
+  if (rel.x >= (float)(v.nx - 1)) return __builtin_nanf("");
+  if (rel.y >= (float)(v.ny - 1)) return __builtin_nanf("");
+  if (rel.z >= (float)(v.nz - 1)) return __builtin_nanf("");
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  lerp_coords(v.nx, rel.x + .5f, x0, x1, fx);
+  lerp_coords(v.ny, rel.y + .5f, y0, y1, fy);
+  lerp_coords(v.nz, rel.z + .5f, z0, z1, fz);
+  const size_t sx = 1, sy = (size_t)v.nx, sz = (size_t)v.nx * v.ny;
+  const float *s = v.scalars;
+  const float v000 = s[x0 * sx + y0 * sy + z0 * sz];
+  const float v001 = s[x1 * sx + y0 * sy + z0 * sz];
+  const float v010 = s[x0 * sx + y1 * sy + z0 * sz];
+  const float v011 = s[x1 * sx + y1 * sy + z0 * sz];
+  const float v100 = s[x0 * sx + y0 * sy + z1 * sz];
+  const float v101 = s[x1 * sx + y0 * sy + z1 * sz];
+  const float v110 = s[x0 * sx + y1 * sy + z1 * sz];
+  const float v111 = s[x1 * sx + y1 * sy + z1 * sz];
+  const float v00 = lerp_l(fx, v000, v001);
+  const float v01 = lerp_l(fx, v010, v011);
+  const float v10 = lerp_l(fx, v100, v101);
+  const float v11 = lerp_l(fx, v110, v111);
+  const float v0 = lerp_l(fy, v00, v01);
+  const float v1 = lerp_l(fy, v10, v11);
+  return lerp_l(fz, v0, v1);
+}
+
+BN_HD VolSample tf_map(const TFDD &tf, float s) {
+#pragma clang fp contract(off)
+  float f = (s - tf.domLo) / (tf.domHi - tf.domLo);
+  f = fminf(fmaxf(f, 0.f), 1.f);
+  f *= (tf.numValues - 1);
+  int idx = int(f);
+  idx = idx < 0 ? 0 : (idx > tf.numValues - 2 ? tf.numValues - 2 : idx);
+  f -= idx;
+  const float4 a = tf.values[idx], b = tf.values[idx + 1];
+  VolSample r;
+  r.r = (1.f - f) * a.x + f * b.x;
+  r.g = (1.f - f) * a.y + f * b.y;
+  r.b = (1.f - f) * a.z + f * b.z;
+  r.w = (1.f - f) * a.w + f * b.w;
+  r.w *= tf.baseDensity;
+  return r;
+}
+
+BN_HD float tf_majorant(const TFDD &tf, float lo, float hi) {
+#pragma clang fp contract(off)
+  if (lo > hi) return 0.f;
+  const float span = tf.domHi - tf.domLo;
+  float f_lo = (lo - tf.domLo) / span;
+  float f_hi = (hi - tf.domLo) / span;
+  f_lo = fminf(fmaxf(f_lo, 0.f), 1.f);
+  f_hi = fminf(fmaxf(f_hi, 0.f), 1.f);
+  f_lo *= (tf.numValues - 1);
+  f_hi *= (tf.numValues - 1);
+  auto clampi = [&](int i) { return i < 0 ? 0 : (i > tf.numValues - 2 ? tf.numValues - 2 : i); };
+  const int i_lo = clampi(int(f_lo));
+  const int i_hi = clampi(int(f_hi));
+  f_lo -= i_lo;
+  f_hi -= i_hi;
+  const float lerp_lo = (1.f - f_lo) * tf.values[i_lo].w + f_lo * tf.values[i_lo + 1].w;
+  const float lerp_hi = (1.f - f_hi) * tf.values[i_hi].w + f_hi * tf.values[i_hi + 1].w;
+  float m = 0.f;
+  m = fmaxf(m, lerp_lo);
+  m = fmaxf(m, lerp_hi);
+  for (int i = i_lo + 1; i <= i_hi; ++i) m = fmaxf(m, tf.values[i].w);
+  return m * tf.baseDensity;
+}
+
+BN_HD VolSample sample_and_map(const VolumeDD &v, vec3f P) {
+  const float f = sample_field(v, P);
+  if (f != f) return VolSample{0.f, 0.f, 0.f, 0.f};
+  return tf_map(v.tf, f);
+}
+
+struct VolCounters {
+  uint32_t samples;
+  uint32_t ddaSteps;
+};
+
+struct VolHit {
+  bool hit;
+  float t;
+  vec3f rgb;
+};
+
+BN_HD int vfloor_i(float f) { return (int)floorf(f); }
+
+// MCVolumeAccel::isProg for one ray segment [tmin, tmax]: box test, DDA over
+// macrocells, Woodcock tracking in each cell with a non-zero majorant. The
+// ray's RNG seed is advanced as Random2(seed, hash(inst, geom, 0)) does.
+BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmin, float tmax,
+                              float rayTMax, RNGSeed &seed, bool exactLog, VolCounters &cnt) {
+#pragma clang fp contract(off)
+  VolHit res;
+  res.hit = false;
+  float t0 = tmin, t1 = tmax;
+  {
+    const vec3f t_lo = (v.worldLo - org) * rcp(dir);
+    const vec3f t_hi = (v.worldHi - org) * rcp(dir);
+    const vec3f t_nr = vmin(t_lo, t_hi);
+    const vec3f t_fr = vmax(t_lo, t_hi);
+    t0 = fmaxf(t0, reduce_max(t_nr));
+    t1 = fminf(t1, reduce_min(t_fr));
+    if (!(t0 <= t1)) return res;
+  }
+  const vec3f dorg = (org - v.mcOrigin) * rcp(v.mcSpacing);
+  const vec3f ddir = dir * rcp(v.mcSpacing);
+  Random2 rng(seed, bn_hash3(0u, 0u, 0u));
+
+  // dda::dda3(dorg, ddir, t1, mcDims, lambda)
+  const float tMax = t1;
+  const vec3f f_size = {(float)v.mx, (float)v.my, (float)v.mz};
+  const vec3f rcp_dir = rcp(ddir);
+  vec3f t_lo = (vec3f(0.f) - dorg) * rcp(ddir);
+  vec3f t_hi = (f_size - dorg) * rcp(ddir);
+  vec3f t_nr = vmin(t_lo, t_hi);
+  vec3f t_fr = vmax(t_lo, t_hi);
+  if (ddir.x == 0.f) {
+    if (dorg.x < 0.f || dorg.x > f_size.x) return res;
+    t_nr.x = -BN_INF; t_fr.x = BN_INF;
+  }
+  if (ddir.y == 0.f) {
+    if (dorg.y < 0.f || dorg.y > f_size.y) return res;
+    t_nr.y = -BN_INF; t_fr.y = BN_INF;
+  }
+  if (ddir.z == 0.f) {
+    if (dorg.z < 0.f || dorg.z > f_size.z) return res;
+    t_nr.z = -BN_INF; t_fr.z = BN_INF;
+  }
+  const float ray_t0 = fmaxf(0.f, reduce_max(t_nr));
+  const float ray_t1 = fminf(tMax, reduce_min(t_fr));
+  if (ray_t0 > ray_t1) return res;
+  const vec3f oiv = dorg + ray_t0 * ddir;
+  const vec3f f_cell = vmax(vec3f(0.f), vmin(f_size - vec3f(1.f), vec3f(floorf(oiv.x), floorf(oiv.y), floorf(oiv.z))));
+  const vec3f f_cell_end = {ddir.x > 0.f ? f_cell.x + 1.f : f_cell.x, ddir.y > 0.f ? f_cell.y + 1.f : f_cell.y,
+                            ddir.z > 0.f ? f_cell.z + 1.f : f_cell.z};
+  const vec3f t_step = vabs(rcp_dir);
+  vec3f t_next = {(ddir.x == 0.f) ? BN_INF : (fabsf(f_cell_end.x - oiv.x) * t_step.x),
+                  (ddir.y == 0.f) ? BN_INF : (fabsf(f_cell_end.y - oiv.y) * t_step.y),
+                  (ddir.z == 0.f) ? BN_INF : (fabsf(f_cell_end.z - oiv.z) * t_step.z)};
+  const int stopx = ddir.x > 0.f ? v.mx : -1, stopy = ddir.y > 0.f ? v.my : -1, stopz = ddir.z > 0.f ? v.mz : -1;
+  const int dx = ddir.x > 0.f ? 1 : -1, dy = ddir.y > 0.f ? 1 : -1, dz = ddir.z > 0.f ? 1 : -1;
+  int cx = (int)f_cell.x, cy = (int)f_cell.y, cz = (int)f_cell.z;
+  float next_cell_begin = 0.f;
+  while (true) {
+    const float t_closest = reduce_min(t_next);
+    const float cell_t0 = ray_t0 + next_cell_begin;
+    const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
+    cnt.ddaSteps++;
+    // ---- lambda body (MCAccelerator.h:619-705) ----
+    const float majorant = v.majorants[cx + v.mx * (cy + v.my * cz)];
+    if (majorant != 0.f) {
+      const float lower = cell_t0;
+      const float upper = fminf(cell_t1, rayTMax);
+      // Woodcock::sampleRange (Volume.h:188-223)
+      float t = lower;
+      while (true) {
+        const float r = rng();
+        const float dt = exactLog ? -logf(1.f - r) / majorant : -fast_log(1.f - r) / majorant;
+        t += dt;
+        if (t >= upper) break;
+        const vec3f P = org + t * dir;
+        const VolSample s = sample_and_map(v, P);
+        cnt.samples++;
+        if (s.w >= rng() * majorant) {
+          res.hit = true;
+          res.t = t;
+          res.rgb = {s.r, s.g, s.b};
+          return res;
+        }
+      }
+    }
+    // ---- DDA step ----
+    next_cell_begin = t_closest;
+    if (t_next.x == t_closest) {
+      t_next.x += t_step.x;
+      cx += dx;
+      if (cx == stopx) return res;
+    }
+    if (t_next.y == t_closest) {
+      t_next.y += t_step.y;
+      cy += dy;
+      if (cy == stopy) return res;
+    }
+    if (t_next.z == t_closest) {
+      t_next.z += t_step.z;
+      cz += dz;
+      if (cz == stopz) return res;
+    }
+  }
+}
+
+}  // namespace bnamd

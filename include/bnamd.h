@@ -1,0 +1,219 @@
+/* bnamd.h -- C ABI of barney-amd, an MI355X-native (gfx950) ray-tracing core
+ * that sits behind barney's rtcore TraceInterface / ComputeInterface.
+ *
+ * Conventions
+ *  - Every function returns BNAMD_OK (0) or BNAMD_ERROR (-1); on error
+ *    bnamd_last_error() returns a message (the C++ layer rethrows it as
+ *    std::runtime_error, as rtcore/cudaCommon/cuda-helper.h:41-60 does).
+ *  - Pointers documented as "device" point into GPU memory (from
+ *    bnamd_device_malloc, hipMalloc or a torch tensor); everything else is
+ *    host memory. Streams are hipStream_t passed as void* (NULL = default).
+ *  - Plain C types only: no torch or C++ types cross this boundary.
+ *  - Results are complete when the call returns unless a stream is passed,
+ *    in which case work is enqueued on it.
+ */
+#ifndef BNAMD_H
+#define BNAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BNAMD_OK 0
+#define BNAMD_ERROR (-1)
+
+/* ------------------------------------------------------------------------
+ * Runtime / memory.  Replaces rtc::Device (rtcore/cudaCommon/Device.h:34-64:
+ * allocMem/freeMem/copy/copyAsync/memsetAsync/sync/setActive).
+ * --------------------------------------------------------------------- */
+int bnamd_init(int device);
+const char *bnamd_last_error(void);
+int bnamd_device_malloc(void **dptr, size_t bytes);
+int bnamd_device_free(void *dptr);
+int bnamd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int bnamd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int bnamd_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int bnamd_memset(void *dst, int value, size_t bytes);
+int bnamd_synchronize(void);
+
+/* ------------------------------------------------------------------------
+ * Geometry and acceleration structures.
+ * bnamd_geom mirrors the device data (DD) of barney's Triangles
+ * (barney/geometry/Triangles.h), Spheres (Spheres.h) and Cylinders
+ * (Cylinders.h) geoms as handed to rtc::Geom::setDD.  A group is the
+ * geometry list of rtc::Device::createTrianglesGroup / createUserGeomsGroup
+ * (rtcore/cuda/Device.h:36-70); bnamd_world_build is
+ * rtc::Device::createInstanceGroup + Group::buildAccel
+ * (rtcore/cuda/cudaGroup.cu:144-562) and returns the AccelHandle analogue.
+ * --------------------------------------------------------------------- */
+#define BNAMD_GEOM_TRIANGLES 0
+#define BNAMD_GEOM_SPHERES 1
+#define BNAMD_GEOM_CYLINDERS 2
+
+typedef struct {
+  int kind;              /* BNAMD_GEOM_* */
+  int numPrims;          /* triangles / spheres / cylinders */
+  int numVertices;       /* entries in 'vertices' */
+  int materialID;        /* index into the world's material table */
+  const float *vertices; /* device float3[]: tri/cyl vertices, sphere centers */
+  const int *indices;    /* device int3[] (triangles) / int2[] (cylinders); unused for spheres */
+  const float *radii;    /* device float[numPrims] or NULL (spheres: defaultRadius) */
+  const float *colors;   /* device float3[numPrims] per-primitive color or NULL */
+  float defaultRadius;
+  int userID;            /* 'objID' written to the ID channel */
+} bnamd_geom;
+
+typedef struct {
+  const bnamd_geom *geoms; /* host array */
+  int numGeoms;
+} bnamd_group;
+
+typedef struct {
+  int group;      /* index into the group array */
+  int instID;     /* user instance ID (InstanceGroup 'instIDs') */
+  float xfm[12];  /* affine3f: vx, vy, vz, p (object to world) */
+} bnamd_instance;
+
+/* Material table entries: ANARI matte (barney/material/AnariMatte.h) and
+ * PBR (AnariPBR.h) reduced to the parameters the BSDFs read. */
+#define BNAMD_MAT_MATTE 0
+#define BNAMD_MAT_PBR 1
+typedef struct {
+  int type;
+  float color[3];
+  float opacity;
+  float metallic;
+  float roughness;
+  float transmission;
+  float ior;
+  float specular;
+} bnamd_material;
+
+typedef struct bnamd_world_s bnamd_world;
+
+typedef struct {
+  long long numPrims;
+  long long numNodes;
+  int depth;
+  float buildMs;
+  float sceneLower[3];
+  float sceneUpper[3];
+} bnamd_world_info;
+
+int bnamd_world_build(const bnamd_group *groups, int numGroups,
+                      const bnamd_instance *instances, int numInstances,
+                      const bnamd_material *materials, int numMaterials,
+                      void *stream, bnamd_world **out);
+int bnamd_world_destroy(bnamd_world *world);
+int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info);
+
+/* ------------------------------------------------------------------------
+ * Trace: closest hit for a batch of rays (rtc::TraceInterface::traceRay
+ * semantics, embree oracle rtcore/embree/Device.cpp:107-196).  At equal t the
+ * smallest (instance index, geomID, primID) wins.
+ * --------------------------------------------------------------------- */
+typedef struct {
+  float org[3];
+  float tmin;
+  float dir[3];
+  float tmax;
+} bnamd_ray; /* 32 B */
+
+typedef struct {
+  float t;        /* tmax of the ray if no hit */
+  float u, v;     /* triangle barycentrics */
+  int primID;     /* -1 if no hit */
+  int geomID;
+  int instID;     /* user instance ID */
+  float N[3];     /* normalized world-space geometric normal */
+  int instIndex;  /* flattened instance index */
+} bnamd_hit; /* 40 B */
+
+int bnamd_trace(const bnamd_world *world, const bnamd_ray *rays, bnamd_hit *hits, int numRays,
+                void *stream);
+
+/* ------------------------------------------------------------------------
+ * Surface path tracing (plain per-pixel megakernel): generateRays +
+ * traceRays + shadeRays of barney/kernels/ (generateRays.cu, traceRays.cu,
+ * shadeRays.cu) fused per pixel.
+ * --------------------------------------------------------------------- */
+typedef struct {
+  float position[3];
+  float direction[3];
+  float up[3];
+  float fovy; /* degrees */
+} bnamd_camera;
+
+typedef struct {
+  int width, height;
+  int accumID;          /* progressive frame index (jitter + seed) */
+  int pathsPerPixel;
+  float ambientRadiance;
+  float bgColor[4];     /* w < 0: barney's vertical gradient */
+  int maxPathDepth;     /* barney: 10 */
+  int maxDiffuseBounces;/* barney: 1 */
+  int russianRoulette;  /* 0: off (barney); >0: start depth for RR */
+} bnamd_render_params;
+
+typedef struct {
+  long long raysTraced; /* all rays traced: primary, shadow and bounce */
+  long long primaryRays;
+  float ms;
+} bnamd_render_stats;
+
+/* accum: device float4[width*height] (RGBA accumulation, as the accum
+ * channel of barney/fb/TiledFB.h:21-24 but linear); ids: device int4 per
+ * pixel {primID, instID, objID, depth-as-float-bits} or NULL. */
+int bnamd_render_surface(const bnamd_world *world, const bnamd_camera *camera,
+                         const bnamd_render_params *params, float *accum, int *ids,
+                         void *stream, bnamd_render_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Structured volumes: StructuredData + MCGrid + TransferFunction +
+ * MCVolumeAccel (barney/volume/).
+ * --------------------------------------------------------------------- */
+typedef struct {
+  const float *scalars; /* device float[dims.x*dims.y*dims.z], x fastest */
+  int dims[3];          /* number of scalars (cells = dims-1) */
+  float gridOrigin[3];
+  float gridSpacing[3];
+  int cellsPerMC;       /* barney: 8 (StructuredData.cu:25) */
+} bnamd_structured_desc;
+
+typedef struct bnamd_volume_s bnamd_volume;
+
+int bnamd_volume_create(const bnamd_structured_desc *desc, bnamd_volume **out);
+int bnamd_volume_destroy(bnamd_volume *vol);
+/* values: host float4[numValues]; recomputes majorants (MCGrid.cu:94-116). */
+int bnamd_volume_set_tf(bnamd_volume *vol, const float *values, int numValues,
+                        float domainLo, float domainHi, float baseDensity);
+/* mc grid dims, and host copies of ranges (float2 per cell) / majorants */
+int bnamd_volume_mc_dims(const bnamd_volume *vol, int dims[3]);
+int bnamd_volume_get_mcs(const bnamd_volume *vol, float *ranges, float *majorants);
+
+/* Delta-tracking render with single scattering (SCI_VIS_MODE shade path). */
+typedef struct {
+  long long samples;  /* sampleAndMap evaluations */
+  long long ddaSteps; /* macrocells visited */
+  long long raysTraced;
+  float ms;
+} bnamd_volume_stats;
+
+int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *camera,
+                        const bnamd_render_params *params, float *accum, void *stream,
+                        bnamd_volume_stats *stats);
+
+/* Per-ray delta-tracking query for parity checks: for each ray returns the
+ * first accepted Woodcock sample {t, r, g, b} (t = +inf if none), using the
+ * ray seeds given (uint64 per ray). */
+int bnamd_volume_sample_rays(const bnamd_volume *vol, const bnamd_ray *rays,
+                             const uint64_t *seeds, float *out, int numRays, int exactLog,
+                             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BNAMD_H */

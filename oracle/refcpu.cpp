@@ -1,0 +1,1179 @@
+// refcpu.cpp -- CPU oracle (test infrastructure only; see refcpu.h).
+//
+// A multithreaded C++ restatement of barney's embree-backend path: camera
+// rays, closest-hit tracing with barney's geometry programs, the env-light
+// shade/bounce step, and structured-volume delta tracking. Compiled with
+// -ffp-contract=off so every fp32 expression rounds exactly as written.
+#include "refcpu.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+namespace {
+
+const float kInf = std::numeric_limits<float>::infinity();
+const float kPi = (float)M_PI;
+
+struct V3 {
+  float x, y, z;
+};
+inline V3 mk(float a, float b, float c) { return V3{a, b, c}; }
+inline V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 mul(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 scl(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+inline V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline V3 nrm(V3 a) { return scl(1.f / std::sqrt(dot(a, a)), a); }
+inline float len(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 vmin(V3 a, V3 b) { return mk(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)); }
+inline V3 vmax(V3 a, V3 b) { return mk(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
+inline V3 vmins(V3 a, float b) { return mk(std::fmin(a.x, b), std::fmin(a.y, b), std::fmin(a.z, b)); }
+inline float maxc(V3 a) { return std::fmax(a.x, std::fmax(a.y, a.z)); }
+inline float minc(V3 a) { return std::fmin(a.x, std::fmin(a.y, a.z)); }
+inline V3 inv(V3 a) { return mk(1.f / a.x, 1.f / a.y, 1.f / a.z); }
+inline V3 vabs(V3 a) { return mk(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+inline bool eq(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+inline V3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+inline uint32_t fbits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+// ---- RNG: barney-common.h:65-111 -------------------------------------------
+struct Rng {
+  uint64_t s = 0xcafef00dd15ea5e5ull;
+  uint32_t next() {
+    uint64_t x = s;
+    unsigned c = (unsigned)(x >> 61);
+    s = x * 6364136223846793005ull;
+    x ^= x >> 22;
+    return (uint32_t)(x >> (22 + c));
+  }
+  uint32_t mix(uint64_t h) {
+    s ^= (h << 1);
+    return next();
+  }
+  void seed(uint32_t a, uint32_t b) {
+    s ^= 2ull * a + 1ull;
+    next();
+    s ^= 2ull * b;
+    next();
+  }
+  float uni() { return (float)(next() & 0xFFFFFFu) * (1.f / 16777216.f); }
+};
+// Random2(RNGSeed &parent, b)
+inline Rng child(Rng &parent, uint64_t b) {
+  Rng r;
+  r.s = parent.s;
+  parent.next();
+  r.mix(b ? b : 290374ull);
+  return r;
+}
+inline uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c) {
+  uint64_t h = 0xcbf29ce484222325ull ^ (0x100000001b3ull * (uint64_t)a);
+  h = h * 0x100000001b3ull ^ b;
+  h = h * 0x100000001b3ull ^ c;
+  return h;
+}
+
+template <typename F>
+void parallel_for(int n, int threads, F f) {
+  if (threads <= 1 || n < 64) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      while (true) {
+        const int b = next.fetch_add(64);
+        if (b >= n) break;
+        const int e = std::min(n, b + 64);
+        for (int i = b; i < e; ++i) f(i);
+      }
+    });
+  for (auto &th : pool) th.join();
+}
+int default_threads(int t) { return t > 0 ? t : (int)std::max(1u, std::thread::hardware_concurrency()); }
+
+// ---- primitives --------------------------------------------------------------
+struct Prim {
+  int geom, prim;
+  uint32_t key;  // tie-break: position in (geom, prim) order
+  float lo[3], hi[3];
+};
+
+struct Best {
+  float t;
+  float u, v;       // tri barycentrics; sphere: shifted-frame t, t_move
+  uint32_t key;
+  int geom, prim;
+};
+inline bool better(float t, uint32_t key, const Best &b) { return t < b.t || (t == b.t && key < b.key); }
+
+}  // namespace
+
+struct ref_scene_s {
+  std::vector<ref_geom> geoms;
+  std::vector<Prim> prims;  // reordered into BVH order
+  struct Node {
+    float lo[3], hi[3];
+    int left;  // internal: left child index (right = left+1); leaf: -1
+    int first, count;
+  };
+  std::vector<Node> nodes;
+};
+
+namespace {
+
+// rtcore/cuda/TraceInterface.h:32-98 (barney's plane + 2D-projection test)
+void hit_triangle(V3 org, V3 dir, float tmax, V3 v0, V3 v1, V3 v2, uint32_t key, int g, int p, Best &b) {
+  const V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+  const V3 N = cross(e1, e2);
+  const float dd = dot(dir, N);
+  if (std::fabs(dd) < 1e-12f) return;
+  if (dd == 0.f) return;
+  const float t = -dot(sub(org, v0), N) / dd;
+  if (t <= 0.f || t >= tmax) return;
+  const V3 P = add(sub(org, v0), scl(t, dir));
+  float e1u, e2u, Pu, e1v, e2v, Pv;
+  const float ax = std::fabs(N.x), ay = std::fabs(N.y), az = std::fabs(N.z);
+  if (ax >= std::fmax(ay, az)) {
+    e1u = e1.y; e2u = e2.y; Pu = P.y; e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else if (ay > az) {
+    e1u = e1.x; e2u = e2.x; Pu = P.x; e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else {
+    e1u = e1.x; e2u = e2.x; Pu = P.x; e1v = e1.y; e2v = e2.y; Pv = P.y;
+  }
+  // det(a,b,c,d) = a*d - c*b
+  if (e1u * e2v - e2u * e1v == 0.f) return;
+  const float den = e1u * e2v - e1v * e2u;
+  const float u = (Pu * e2v - Pv * e2u) / den;
+  const float v = (e1u * Pv - e1v * Pu) / den;
+  if (u < 0.f || v < 0.f || (u + v) >= 1.f) return;
+  if (!better(t, key, b)) return;
+  b.t = t; b.u = u; b.v = v; b.key = key; b.geom = g; b.prim = p;
+}
+
+// barney/geometry/Spheres.dev.cu:116-170
+void hit_sphere(V3 org, V3 dir, float tmin, float tmax, V3 c, float r, uint32_t key, int g, int p, Best &b) {
+  const float t_move = std::fmax(0.f, len(sub(c, org)) - 3.f * r);
+  const V3 o = add(org, scl(t_move, dir));
+  const float t_max = tmax - t_move;
+  if (t_max < 0.f) return;
+  float hit_t = t_max;
+  const float t_min = std::fmax(0.f, tmin - t_move);
+  const V3 oc = sub(o, c);
+  const float A = dot(dir, dir), B = dot(oc, dir), C = dot(oc, oc) - r * r;
+  const float disc = B * B - A * C;
+  if (disc < 0.f) return;
+  float tt = (-B - std::sqrt(disc)) / A;
+  if (tt < hit_t && tt > t_min) hit_t = tt;
+  tt = (-B + std::sqrt(disc)) / A;
+  if (tt < hit_t && tt > t_min) hit_t = tt;
+  if (!(hit_t < t_max)) return;
+  const float t = hit_t + t_move;
+  if (!better(t, key, b)) return;
+  b.t = t; b.u = hit_t; b.v = t_move; b.key = key; b.geom = g; b.prim = p;
+}
+
+// barney/geometry/Cylinders.dev.cu:101-243
+bool cyl(V3 ro, V3 rd, float tmin, float tmax, V3 v0, V3 v1, float radius, float &tout, V3 *Nout) {
+  const V3 d = rd, s = sub(v1, v0);
+  const V3 sxd = cross(s, d);
+  const float a = dot(sxd, sxd);
+  if (a == 0.f) return false;
+  const V3 f = sub(v0, ro);
+  const V3 sxf = cross(s, f);
+  const float ra = 1.f / a;
+  const float ts = dot(sxd, sxf) * ra;
+  const V3 fp = sub(f, scl(ts, d));
+  const float s2 = dot(s, s);
+  const V3 perp = cross(s, fp);
+  const float c = radius * radius * s2 - dot(perp, perp);
+  if (c < 0.f) return false;
+  float td = std::sqrt(c * ra);
+  const float tube0 = ts - td, tube1 = ts + td;
+  const float sf = dot(s, f), sd = dot(s, d);
+  float cap0 = -1e20f, cap1 = 1e20f, capv0 = cap0, capv1 = cap1;
+  if (sd == 0.f) {
+    if (dot(sub(ro, v0), sub(v1, v0)) < 0.f) return false;
+    if (dot(sub(ro, v1), sub(v0, v1)) < 0.f) return false;
+  } else {
+    const float rsd = 1.f / sd;
+    capv0 = sf * rsd;
+    capv1 = capv0 + s2 * rsd;
+    cap0 = std::fmin(capv0, capv1);
+    cap1 = std::fmax(capv0, capv1);
+  }
+  const float t0 = std::fmax(cap0, tube0), t1 = std::fmin(cap1, tube1);
+  if (t0 > t1) return false;
+  V3 N = mk(0, 0, 0);
+  float th;
+  if (tmin <= t0 && t0 <= tmax) {
+    th = t0;
+    td *= -1.f;
+    const float u = (th * sd - sf) * (1.f / s2);
+    if (t0 == cap0) N = (cap0 == capv0) ? neg(s) : s;
+    else N = sub(sub(scl(td, d), fp), scl(u, s));
+  } else if (tmin <= t1 && t1 <= tmax) {
+    th = t1;
+    const float u = (th * sd - sf) * (1.f / s2);
+    if (t0 == cap1) N = (cap0 == capv0) ? neg(s) : s;  // as in Cylinders.dev.cu:221
+    else N = sub(sub(scl(td, d), fp), scl(u, s));
+  } else {
+    return false;
+  }
+  tout = th;
+  if (Nout) *Nout = N;
+  return true;
+}
+
+void test_prim(const ref_scene *sc, const Prim &pr, V3 org, V3 dir, float tmin, float tmax, Best &b) {
+  const ref_geom &g = sc->geoms[pr.geom];
+  const int p = pr.prim;
+  if (g.kind == 0) {
+    const int *ix = g.indices + 3 * p;
+    hit_triangle(org, dir, tmax, ld3(g.vertices + 3 * ix[0]), ld3(g.vertices + 3 * ix[1]),
+                 ld3(g.vertices + 3 * ix[2]), pr.key, pr.geom, p, b);
+  } else if (g.kind == 1) {
+    const float r = g.radii ? g.radii[p] : g.defaultRadius;
+    hit_sphere(org, dir, tmin, tmax, ld3(g.vertices + 3 * p), r, pr.key, pr.geom, p, b);
+  } else {
+    const int *ix = g.indices + 2 * p;
+    float t;
+    if (cyl(org, dir, tmin, tmax, ld3(g.vertices + 3 * ix[0]), ld3(g.vertices + 3 * ix[1]), g.radii[p], t, nullptr) &&
+        better(t, pr.key, b)) {
+      b.t = t; b.u = 0.f; b.v = 0.f; b.key = pr.key; b.geom = pr.geom; b.prim = p;
+    }
+  }
+}
+
+// Conservative slab test (double precision, widened) so that no primitive
+// whose box the ray touches is culled.
+inline bool box_hit(const float *lo, const float *hi, const double *o, const double *id, double tmin, double tmax) {
+  double t0 = tmin, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double a = ((double)lo[k] - o[k]) * id[k], c = ((double)hi[k] - o[k]) * id[k];
+    if (a > c) std::swap(a, c);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, c);
+  }
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-9;
+}
+
+Best closest(const ref_scene *sc, V3 org, V3 dir, float tmin, float tmax, bool brute, long long *tests) {
+  Best b;
+  b.t = tmax; b.u = b.v = 0.f; b.key = 0xffffffffu; b.geom = -1; b.prim = -1;
+  if (brute) {
+    for (const Prim &p : sc->prims) test_prim(sc, p, org, dir, tmin, tmax, b);
+    if (tests) *tests += (long long)sc->prims.size();
+    return b;
+  }
+  if (sc->nodes.empty()) return b;
+  const double o[3] = {org.x, org.y, org.z};
+  const double id[3] = {1.0 / dir.x, 1.0 / dir.y, 1.0 / dir.z};
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const auto &n = sc->nodes[stack[--sp]];
+    if (!box_hit(n.lo, n.hi, o, id, tmin, b.t)) continue;
+    if (n.left < 0) {
+      for (int i = 0; i < n.count; ++i) test_prim(sc, sc->prims[n.first + i], org, dir, tmin, tmax, b);
+      if (tests) *tests += n.count;
+    } else {
+      stack[sp++] = n.left + 1;
+      stack[sp++] = n.left;
+    }
+  }
+  return b;
+}
+
+// Binned SAH build (16 bins), leaves of <= 4 prims.
+void build_bvh(ref_scene *s) {
+  auto &P = s->prims;
+  auto &N = s->nodes;
+  N.clear();
+  if (P.empty()) return;
+  N.reserve(2 * P.size());
+  N.push_back({});
+  struct Task { int node, first, count; };
+  std::vector<Task> st{{0, 0, (int)P.size()}};
+  while (!st.empty()) {
+    Task t = st.back();
+    st.pop_back();
+    auto &nd = N[t.node];
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+    float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+    for (int i = t.first; i < t.first + t.count; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], P[i].lo[k]);
+        hi[k] = std::max(hi[k], P[i].hi[k]);
+        const float c = 0.5f * (P[i].lo[k] + P[i].hi[k]);
+        clo[k] = std::min(clo[k], c);
+        chi[k] = std::max(chi[k], c);
+      }
+    for (int k = 0; k < 3; ++k) { N[t.node].lo[k] = lo[k]; N[t.node].hi[k] = hi[k]; }
+    (void)nd;
+    if (t.count <= 4) {
+      N[t.node].left = -1; N[t.node].first = t.first; N[t.node].count = t.count;
+      continue;
+    }
+    const int B = 16;
+    double bestCost = 1e300;
+    int bestAxis = -1, bestBin = -1;
+    for (int k = 0; k < 3; ++k) {
+      const float ext = chi[k] - clo[k];
+      if (!(ext > 0.f)) continue;
+      int cnt[B] = {};
+      float blo[B][3], bhi[B][3];
+      for (int b = 0; b < B; ++b)
+        for (int j = 0; j < 3; ++j) { blo[b][j] = kInf; bhi[b][j] = -kInf; }
+      for (int i = t.first; i < t.first + t.count; ++i) {
+        const float c = 0.5f * (P[i].lo[k] + P[i].hi[k]);
+        int b = std::min(B - 1, (int)((c - clo[k]) / ext * B));
+        ++cnt[b];
+        for (int j = 0; j < 3; ++j) {
+          blo[b][j] = std::min(blo[b][j], P[i].lo[j]);
+          bhi[b][j] = std::max(bhi[b][j], P[i].hi[j]);
+        }
+      }
+      auto area = [](const float *l, const float *h) {
+        const double dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+        return (dx < 0) ? 0.0 : dx * dy + dy * dz + dz * dx;
+      };
+      double rA[B];
+      int rC[B];
+      float rl[3] = {kInf, kInf, kInf}, rh[3] = {-kInf, -kInf, -kInf};
+      int c = 0;
+      for (int b = B - 1; b > 0; --b) {
+        c += cnt[b];
+        for (int j = 0; j < 3; ++j) { rl[j] = std::min(rl[j], blo[b][j]); rh[j] = std::max(rh[j], bhi[b][j]); }
+        rA[b] = area(rl, rh);
+        rC[b] = c;
+      }
+      float ll[3] = {kInf, kInf, kInf}, lh[3] = {-kInf, -kInf, -kInf};
+      int lc = 0;
+      for (int b = 0; b < B - 1; ++b) {
+        lc += cnt[b];
+        for (int j = 0; j < 3; ++j) { ll[j] = std::min(ll[j], blo[b][j]); lh[j] = std::max(lh[j], bhi[b][j]); }
+        if (lc == 0 || rC[b + 1] == 0) continue;
+        const double cost = area(ll, lh) * lc + rA[b + 1] * rC[b + 1];
+        if (cost < bestCost) { bestCost = cost; bestAxis = k; bestBin = b; }
+      }
+    }
+    int mid;
+    if (bestAxis < 0) {
+      mid = t.first + t.count / 2;
+    } else {
+      const int k = bestAxis;
+      const float ext = chi[k] - clo[k];
+      auto it = std::partition(P.begin() + t.first, P.begin() + t.first + t.count, [&](const Prim &p) {
+        const float c = 0.5f * (p.lo[k] + p.hi[k]);
+        return std::min(B - 1, (int)((c - clo[k]) / ext * B)) <= bestBin;
+      });
+      mid = (int)(it - P.begin());
+      if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+    }
+    const int l = (int)N.size();
+    N.push_back({});
+    N.push_back({});
+    N[t.node].left = l;
+    N[t.node].first = t.first;
+    N[t.node].count = t.count;
+    st.push_back({l, t.first, mid - t.first});
+    st.push_back({l + 1, mid, t.first + t.count - mid});
+  }
+}
+
+V3 fix_dir(V3 d) {
+  if (d.x == 0.f) d.x = 1e-6f;
+  if (d.y == 0.f) d.y = 1e-6f;
+  if (d.z == 0.f) d.z = 1e-6f;
+  return d;
+}
+
+// Closest-hit epilogue: Triangles.dev.cu:57-74, Spheres.dev.cu:57-90,
+// Cylinders.dev.cu:208-216.
+void hit_point_normal(const ref_scene *sc, const Best &b, V3 org, V3 dir, float tmin, float tmax, V3 &P, V3 &N) {
+  const ref_geom &g = sc->geoms[b.geom];
+  const int p = b.prim;
+  if (g.kind == 0) {
+    const int *ix = g.indices + 3 * p;
+    const V3 v0 = ld3(g.vertices + 3 * ix[0]), v1 = ld3(g.vertices + 3 * ix[1]), v2 = ld3(g.vertices + 3 * ix[2]);
+    N = nrm(cross(sub(v1, v0), sub(v2, v0)));
+    P = add(add(scl(1.f - b.u - b.v, v0), scl(b.u, v1)), scl(b.v, v2));
+  } else if (g.kind == 1) {
+    const V3 c = ld3(g.vertices + 3 * p);
+    const V3 o = add(org, scl(b.v, dir));
+    P = add(o, scl(b.u, dir));
+    N = nrm(eq(P, c) ? mk(1, 0, 0) : sub(P, c));
+  } else {
+    const int *ix = g.indices + 2 * p;
+    float t;
+    V3 n = mk(0, 0, 0);
+    cyl(org, dir, tmin, tmax, ld3(g.vertices + 3 * ix[0]), ld3(g.vertices + 3 * ix[1]), g.radii[p], t, &n);
+    P = add(add(org, scl(t, dir)), scl(1e-4f, nrm(n)));
+    N = nrm(n);
+  }
+}
+
+// ---- camera (Camera.cu:81-100, generateRays.cu:73-129) ----------------------
+struct Cam {
+  V3 org, d00, du, dv;
+};
+Cam make_cam(const ref_camera *c) {
+  Cam r;
+  r.org = ld3(c->position);
+  V3 d = nrm(nrm(ld3(c->direction)));
+  r.du = nrm(cross(d, ld3(c->up)));
+  r.dv = nrm(cross(r.du, d));
+  r.d00 = scl((float)(1.f / (2.0f * std::tan((0.5f * c->fovy) * (float)M_PI / 180.0f))), d);
+  return r;
+}
+
+struct PRay {
+  V3 org, dir;
+  float tMax;
+  Rng rng;
+  V3 tp;
+  float mis;
+  int nDiff;
+  bool shadow;
+  V3 miss;
+  float missA;
+};
+
+PRay cam_ray(const Cam &c, const ref_render_params *p, int ix, int iy, int accumID) {
+  PRay r;
+  Rng rnd;
+  rnd.seed((uint32_t)(ix + p->width * accumID), (uint32_t)(iy + p->height * accumID));
+  r.rng = Rng();
+  r.rng.seed((uint32_t)(ix + accumID * p->width), (uint32_t)iy);
+  const float pu = accumID == 0 ? .5f : rnd.uni();
+  const float pv = accumID == 0 ? .5f : rnd.uni();
+  const float iu = (ix + pu) / float(p->width), iv = (iy + pv) / float(p->height);
+  const float aspect = p->width / float(p->height);
+  const V3 d = add(add(c.d00, scl(1.f * aspect * (iu - .5f), c.du)), scl(1.f * (iv - .5f), c.dv));
+  r.org = c.org;
+  r.dir = nrm(d);
+  r.tMax = 1e30f;
+  r.tp = mk(1, 1, 1);
+  r.mis = 0.f;
+  r.nDiff = 0;
+  r.shadow = false;
+  const float t = (iy + .5f) / float(p->height);
+  if (p->bgColor[3] >= 0.f) {
+    r.miss = ld3(p->bgColor);
+    r.missA = p->bgColor[3];
+  } else {
+    r.miss = add(scl(1.0f - t, mk(0.9f, 0.9f, 0.9f)), scl(t, mk(0.15f, 0.25f, .8f)));
+    r.missA = (1.0f - t) * 1.f + t * 1.f;
+  }
+  return r;
+}
+
+// ---- shading: shadeRays.cu bounce() with the env light -------------------
+struct HitInfo {
+  bool hit;
+  V3 P, N;
+  float t;
+  int type;  // 1 lambert, 2 phase
+  V3 albedo;
+};
+
+V3 frame_xfm(V3 N, V3 l) {  // published embree/ospray frame(N)
+  const V3 dx0 = mk(0.f, N.z, -N.y), dx1 = mk(-N.z, 0.f, N.x);
+  const V3 dx = nrm(dot(dx0, dx0) > dot(dx1, dx1) ? dx0 : dx1);
+  const V3 dy = nrm(cross(N, dx));
+  return add(add(scl(l.x, dx), scl(l.y, dy)), scl(l.z, N));
+}
+
+V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int depth, int maxDepth, int maxDiff) {
+  sh.tMax = -1.f;
+  V3 frag = mk(0, 0, 0);
+  const V3 tp0 = ray.tp;
+  if (ray.shadow) {
+    if (!h.hit) frag = scl(ray.mis, ray.tp);
+    ray.tMax = -1.f;
+    return frag;
+  }
+  V3 Ng = h.hit ? h.N : mk(0, 0, 0);
+  const bool vol = eq(Ng, mk(0, 0, 0));
+  if (!vol) Ng = nrm(Ng);
+  const V3 Ngff = dot(ray.dir, Ng) < 0.f ? Ng : neg(Ng);
+  if (!h.hit) {
+    frag = depth == 0 ? ray.miss : scl(ray.mis, mul(ray.tp, mk(p->ambientRadiance, p->ambientRadiance, p->ambientRadiance)));
+    if (depth != 0) frag = scl(ray.mis, mul(ray.tp, mk(p->ambientRadiance, p->ambientRadiance, p->ambientRadiance)));
+    ray.tMax = -1.f;
+    return frag;
+  }
+  Rng rnd = child(ray.rng, (uint64_t)fbits(h.t));
+  const V3 P = h.P;
+  const V3 wo = neg(nrm(ray.dir));
+  const float eps = std::fmax(1e-4f, 1e-6f * maxc(vabs(P)));
+  const V3 off = vol ? wo : Ngff;
+  // sampleEnvLight: randomDirection (DG.h:184-194)
+  V3 ld;
+  while (true) {
+    V3 v;
+    v.x = 1.f - 2.f * rnd.uni();
+    v.y = 1.f - 2.f * rnd.uni();
+    v.z = 1.f - 2.f * rnd.uni();
+    if (dot(v, v) <= 1.f) { ld = nrm(v); break; }
+  }
+  V3 lrad;
+  float lpdf;
+  if (!eq(Ngff, mk(0, 0, 0))) {
+    lrad = mk(2.f * kPi * p->ambientRadiance, 2.f * kPi * p->ambientRadiance, 2.f * kPi * p->ambientRadiance);
+    if (dot(ld, Ngff) < 0.f) ld = neg(ld);
+    lpdf = 1.f / (2.f * kPi);
+  } else {
+    const float r4 = 4.f * kPi * p->ambientRadiance;
+    lrad = mk(r4, r4, r4);
+    lpdf = 1.f / (4.f * kPi);
+  }
+  bool needMIS = false, have = false;
+  {
+    float w = maxc(lrad);
+    const float sum = w;
+    if (sum != 0.f) {
+      w *= 1.f / sum;
+      (void)rnd.uni();  // light-selection draw
+      lrad = scl(1.f / w, lrad);
+      needMIS = true;
+      have = !(lpdf <= 0.f) && !std::isnan(lpdf);
+    }
+  }
+  auto eval = [&](V3 wi, V3 &val) -> float {
+    if (h.type == 1) {
+      const float c = std::fmax(dot(wi, Ng), 0.f);
+      val = scl(c, scl(1.f / kPi, h.albedo));
+      val = mk(h.albedo.x * (1.f / kPi) * c, h.albedo.y * (1.f / kPi) * c, h.albedo.z * (1.f / kPi) * c);
+      return c * (1.f / kPi);
+    }
+    const float d = 1.f / (4.f * kPi);
+    val = scl(d, h.albedo);
+    return d;
+  };
+  auto pdf = [&](V3 wi) -> float {
+    if (h.type == 1) {
+      V3 N = Ng;
+      if (dot(wi, N) < 0.f) N = neg(N);
+      return std::fmax(dot(wi, N), 0.f) * (1.f / kPi);
+    }
+    return 1.f / (4.f * kPi);
+  };
+  if (have) {
+    V3 fv;
+    const float fp = eval(ld, fv);
+    if (!(fp > 0.f) || maxc(fv) < 1e-4f) {
+      sh.tMax = -1.f;
+    } else {
+      fv = vmins(fv, 13.f);
+      const float s1 = std::isinf(lpdf) ? 1.f : 1.f / lpdf;
+      const float s2 = vol ? 1.f : ((1.f / kPi) * std::fabs(dot(Ng, ld)));
+      const V3 tsr = scl(s2, mul(mul(scl(s1, tp0), fv), lrad));
+      sh.shadow = true;
+      sh.dir = ld;
+      sh.org = add(P, scl(eps, off));
+      sh.tMax = kInf * (1.f - 2.f * eps);
+      sh.tp = tsr;
+      sh.rng = ray.rng;
+      ray.rng.mix((uint64_t)fbits(h.t));
+      sh.mis = 1.f;
+      sh.nDiff = ray.nDiff;
+      if (needMIS) {
+        const float pl = 1.f / (4.f * kPi);
+        const float pb = pdf(ld);
+        sh.mis = pl / (pl + pb + 1e-10f);
+        if (sh.mis < 1e-5f) sh.tMax = -1.f;
+      }
+    }
+  }
+  if (depth >= maxDepth) {
+    ray.tMax = -1.f;
+    return frag;
+  }
+  ray.tMax = kInf;
+  V3 sdir, fr;
+  float spdf;
+  bool diffuse;
+  if (h.type == 1) {
+    V3 N = Ng;
+    if (dot(wo, N) < 0.f) N = neg(N);
+    const float sx = rnd.uni(), sy = rnd.uni();
+    const float phi = (2.f * kPi) * sx;
+    const float ct = std::sqrt(sy), st = std::sqrt(1.0f - sy);
+    const V3 l = mk(std::cos(phi) * st, std::sin(phi) * st, ct);
+    sdir = frame_xfm(N, l);
+    spdf = l.z * (1.f / kPi);
+    fr = h.albedo;
+    diffuse = true;
+  } else {
+    (void)rnd.uni();
+    (void)rnd.uni();
+    spdf = 1.f / (4.f * kPi);
+    fr = scl(spdf, h.albedo);
+    sdir = mk(0, 0, 1);
+    diffuse = false;
+  }
+  if (!(spdf > 0.f) || spdf <= 1e-6f) return frag;
+  if (!diffuse) {
+    ray.tMax = -1.f;
+    return frag;
+  }
+  if (ray.nDiff >= maxDiff) {
+    ray.tMax = -1.f;
+    return frag;
+  }
+  ray.nDiff += 1;
+  ray.org = add(P, scl(1.f * eps, off));
+  ray.dir = nrm(sdir);
+  const float cosf_ = vol ? 1.f : std::fabs(dot(Ng, nrm(ray.dir)));
+  const float den = std::isinf(spdf) ? 1.f : (spdf + 1e-10f);
+  V3 sf = mk(fr.x * cosf_ / den, fr.y * cosf_ / den, fr.z * cosf_ / den);
+  sf = vmins(sf, 2.5f);
+  ray.tp = mul(ray.tp, sf);
+  if (needMIS && !std::isinf(spdf)) {
+    const float pb = pdf(ray.dir), pl = 1.f / (4.f * kPi);
+    ray.mis = pb / (pb + pl);
+  } else {
+    ray.mis = 1.f;
+  }
+  return frag;
+}
+
+void add_frag(V3 &sum, V3 f, float cmax) {
+  f = vmins(f, cmax);
+  if (f.x > 0.f) sum.x += f.x;
+  if (f.y > 0.f) sum.y += f.y;
+  if (f.z > 0.f) sum.z += f.z;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+uint64_t ref_rng_seed(uint64_t state, uint32_t a, uint32_t b) {
+  Rng r;
+  r.s = state;
+  r.seed(a, b);
+  return r.s;
+}
+void ref_rng_sequence(uint64_t *state, uint32_t *out, int n) {
+  Rng r;
+  r.s = *state;
+  for (int i = 0; i < n; ++i) out[i] = r.next();
+  *state = r.s;
+}
+void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n) {
+  Rng r;
+  r.seed(a, b);
+  for (int i = 0; i < n; ++i) out[i] = r.uni();
+}
+uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c) { return fnv3(a, b, c); }
+
+// Volume.h:174-184
+float ref_fast_log(float f) {
+  f = (f - 1.f) / (f + 1.f);
+  const float f2 = f * f;
+  float s = f;
+  f *= f2;
+  s += (1.f / 3.f) * f;
+  f *= f2;
+  s += (1.f / 5.f) * f;
+  return s + s;
+}
+
+ref_scene *ref_scene_create(const ref_geom *geoms, int numGeoms) {
+  auto *s = new ref_scene_s;
+  s->geoms.assign(geoms, geoms + numGeoms);
+  uint32_t key = 0;
+  for (int g = 0; g < numGeoms; ++g) {
+    const ref_geom &gd = geoms[g];
+    for (int p = 0; p < gd.numPrims; ++p, ++key) {
+      Prim pr;
+      pr.geom = g;
+      pr.prim = p;
+      pr.key = key;
+      V3 lo, hi;
+      if (gd.kind == 0) {
+        const int *ix = gd.indices + 3 * p;
+        const V3 a = ld3(gd.vertices + 3 * ix[0]), b = ld3(gd.vertices + 3 * ix[1]), c = ld3(gd.vertices + 3 * ix[2]);
+        lo = vmin(a, vmin(b, c));
+        hi = vmax(a, vmax(b, c));
+      } else if (gd.kind == 1) {
+        const V3 c = ld3(gd.vertices + 3 * p);
+        const float r = gd.radii ? gd.radii[p] : gd.defaultRadius;
+        lo = sub(c, mk(r, r, r));
+        hi = add(c, mk(r, r, r));
+      } else {
+        const int *ix = gd.indices + 2 * p;
+        const V3 a = ld3(gd.vertices + 3 * ix[0]), b = ld3(gd.vertices + 3 * ix[1]);
+        const float r = gd.radii[p];
+        lo = vmin(sub(a, mk(r, r, r)), sub(b, mk(r, r, r)));
+        hi = vmax(add(a, mk(r, r, r)), add(b, mk(r, r, r)));
+      }
+      pr.lo[0] = lo.x; pr.lo[1] = lo.y; pr.lo[2] = lo.z;
+      pr.hi[0] = hi.x; pr.hi[1] = hi.y; pr.hi[2] = hi.z;
+      s->prims.push_back(pr);
+    }
+  }
+  build_bvh(s);
+  return s;
+}
+
+void ref_scene_destroy(ref_scene *s) { delete s; }
+
+void ref_trace(const ref_scene *s, const float *rays, ref_hit *hits, int n, int brute, int threads, long long *primTests) {
+  std::atomic<long long> tests(0);
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * i;
+    const V3 org = ld3(r), dir = fix_dir(ld3(r + 4));
+    long long tc = 0;
+    const Best b = closest(s, org, dir, r[3], r[7], brute != 0, &tc);
+    tests += tc;
+    ref_hit &h = hits[i];
+    h.t = b.t; h.u = b.u; h.v = b.v;
+    if (b.geom < 0) {
+      h.primID = h.geomID = h.instID = h.instIndex = -1;
+      h.N[0] = h.N[1] = h.N[2] = 0.f;
+    } else {
+      V3 P, N;
+      hit_point_normal(s, b, org, dir, r[3], r[7], P, N);
+      h.primID = b.prim; h.geomID = b.geom; h.instID = 0; h.instIndex = 0;
+      h.N[0] = N.x; h.N[1] = N.y; h.N[2] = N.z;
+    }
+  });
+  if (primTests) *primTests = tests.load();
+}
+
+double ref_triangle_t_f64(const double *o, const double *d, const double *a, const double *b, const double *c) {
+  // Moller-Trumbore in double precision
+  double e1[3], e2[3], p[3], q[3], s[3];
+  for (int k = 0; k < 3; ++k) { e1[k] = b[k] - a[k]; e2[k] = c[k] - a[k]; s[k] = o[k] - a[k]; }
+  p[0] = d[1] * e2[2] - d[2] * e2[1]; p[1] = d[2] * e2[0] - d[0] * e2[2]; p[2] = d[0] * e2[1] - d[1] * e2[0];
+  const double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+  if (std::fabs(det) < 1e-300) return -1.0;
+  const double u = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) / det;
+  if (u < 0.0 || u > 1.0) return -1.0;
+  q[0] = s[1] * e1[2] - s[2] * e1[1]; q[1] = s[2] * e1[0] - s[0] * e1[2]; q[2] = s[0] * e1[1] - s[1] * e1[0];
+  const double v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) / det;
+  if (v < 0.0 || u + v > 1.0) return -1.0;
+  return (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) / det;
+}
+
+void ref_camera_ray(const ref_camera *cam, const ref_render_params *p, int ix, int iy, float *org, float *dir,
+                    uint64_t *seed) {
+  const Cam c = make_cam(cam);
+  const PRay r = cam_ray(c, p, ix, iy, p->accumID);
+  org[0] = r.org.x; org[1] = r.org.y; org[2] = r.org.z;
+  dir[0] = r.dir.x; dir[1] = r.dir.y; dir[2] = r.dir.z;
+  *seed = r.rng.s;
+}
+
+long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const ref_render_params *p, float *accum,
+                             int *ids, int y0, int y1, int threads) {
+  const Cam c = make_cam(cam);
+  const int W = p->width;
+  const int rows = y1 - y0;
+  std::atomic<long long> nrays(0);
+  const int maxDepth = p->maxPathDepth > 0 ? p->maxPathDepth : 10;
+  const int maxDiff = p->maxDiffuseBounces >= 0 ? p->maxDiffuseBounces : 1;
+  parallel_for(rows * W, default_threads(threads), [&](int li) {
+    const int ix = li % W, iy = y0 + li / W;
+    const int pix = ix + iy * W;
+    float acc[4] = {0, 0, 0, 0};
+    long long nr = 0;
+    for (int pp = 0; pp < p->pathsPerPixel; ++pp) {
+      const int a = p->accumID + pp;
+      PRay ray = cam_ray(c, p, ix, iy, a);
+      const float cmax = 10.f * (1 + a);
+      V3 sum = mk(0, 0, 0);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const V3 d = fix_dir(ray.dir);
+        const Best b = closest(s, ray.org, d, 0.f, ray.tMax, false, nullptr);
+        ++nr;
+        HitInfo hi;
+        hi.hit = b.geom >= 0;
+        if (hi.hit) {
+          hit_point_normal(s, b, ray.org, d, 0.f, ray.tMax, hi.P, hi.N);
+          hi.t = b.t;
+          hi.type = 1;
+          const ref_geom &g = s->geoms[b.geom];
+          const V3 col = g.colors ? ld3(g.colors + 3 * b.prim) : ld3(g.color);
+          hi.albedo = scl(1.f / kPi, scl(.85f, col));
+          hi.albedo = mk(.85f * col.x * (1.f / kPi), .85f * col.y * (1.f / kPi), .85f * col.z * (1.f / kPi));
+        }
+        if (gen == 0) {
+          alpha = hi.hit ? 1.f : ray.missA;
+          if (ids && pp == 0 && p->accumID == 0) {
+            int *rec = ids + 4 * pix;
+            if (hi.hit) {
+              const ref_geom &g = s->geoms[b.geom];
+              rec[0] = b.prim;
+              rec[1] = g.kind == 1 ? 13 : 0;
+              rec[2] = g.userID;
+              std::memcpy(&rec[3], &b.t, 4);
+            } else {
+              rec[0] = rec[1] = rec[2] = -1;
+              const float inf = kInf;
+              std::memcpy(&rec[3], &inf, 4);
+            }
+          }
+        }
+        PRay sh;
+        add_frag(sum, bounce(p, ray, hi, sh, gen, maxDepth, maxDiff), cmax);
+        if (sh.tMax > 0.f) {
+          const Best bs = closest(s, sh.org, fix_dir(sh.dir), 0.f, sh.tMax, false, nullptr);
+          ++nr;
+          HitInfo hs;
+          hs.hit = bs.geom >= 0;
+          PRay dummy;
+          add_frag(sum, bounce(p, sh, hs, dummy, gen + 1, maxDepth, maxDiff), cmax);
+        }
+      }
+      acc[0] += sum.x; acc[1] += sum.y; acc[2] += sum.z; acc[3] += alpha;
+    }
+    float *o = accum + 4 * pix;
+    if (p->accumID == 0) {
+      o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+    } else {
+      o[0] += acc[0]; o[1] += acc[1]; o[2] += acc[2]; o[3] += acc[3];
+    }
+    nrays += nr;
+  });
+  return nrays.load();
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Structured volumes (barney/volume)
+// ============================================================================
+struct ref_volume_s {
+  std::vector<float> scalars;
+  int n[3];
+  V3 origin, spacing, wlo, whi;
+  int cpm;
+  int m[3];
+  V3 mcOrigin, mcSpacing;
+  std::vector<float> rlo, rhi, maj;
+  std::vector<float> tf;  // float4
+  int numTF = 0;
+  float dlo = 0.f, dhi = 1.f, base = 1.f;
+};
+
+namespace {
+
+// embree Texture.cpp:326-335
+void lcoords(int N, float tc, int &i0, int &i1, float &f) {
+  if (tc - .5f <= 0.f) { i0 = i1 = 0; f = 0.f; return; }
+  if (tc - .5f >= N - 1) { i0 = i1 = N - 1; f = 0.f; return; }
+  f = (tc - .5f) - int(tc - .5f);
+  i0 = int(tc - .5f);
+  i1 = i0 + 1;
+}
+inline float lerpl(float f, float a, float b) { return (1.f - f) * a + f * b; }
+
+// StructuredData.h:104-115 + embree Texture.cpp:338-409
+float vsample(const ref_volume_s *v, V3 P) {
+  const V3 rel = mul(sub(P, v->origin), inv(v->spacing));
+  if (rel.x < 0.f || rel.y < 0.f || rel.z < 0.f) return NAN;
+  if (rel.x >= v->n[0] - 1 || rel.y >= v->n[1] - 1 || rel.z >= v->n[2] - 1) return NAN;
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  lcoords(v->n[0], rel.x + .5f, x0, x1, fx);
+  lcoords(v->n[1], rel.y + .5f, y0, y1, fy);
+  lcoords(v->n[2], rel.z + .5f, z0, z1, fz);
+  auto at = [&](int x, int y, int z) { return v->scalars[(size_t)x + (size_t)v->n[0] * ((size_t)y + (size_t)v->n[1] * z)]; };
+  const float a = lerpl(fx, at(x0, y0, z0), at(x1, y0, z0));
+  const float b = lerpl(fx, at(x0, y1, z0), at(x1, y1, z0));
+  const float c = lerpl(fx, at(x0, y0, z1), at(x1, y0, z1));
+  const float d = lerpl(fx, at(x0, y1, z1), at(x1, y1, z1));
+  return lerpl(fz, lerpl(fy, a, b), lerpl(fy, c, d));
+}
+
+// TransferFunction.h:67-86
+void tfmap(const ref_volume_s *v, float s, float *o) {
+  float f = (s - v->dlo) / (v->dhi - v->dlo);
+  f = std::min(std::max(f, 0.f), 1.f);
+  f *= (v->numTF - 1);
+  const int i = std::min(std::max(int(f), 0), v->numTF - 2);
+  f -= i;
+  for (int k = 0; k < 4; ++k) o[k] = (1.f - f) * v->tf[4 * i + k] + f * v->tf[4 * (i + 1) + k];
+  o[3] *= v->base;
+}
+
+// TransferFunction.h:92-116
+float tfmaj(const ref_volume_s *v, float lo, float hi) {
+  if (lo > hi) return 0.f;
+  const float span = v->dhi - v->dlo;
+  float a = std::min(std::max((lo - v->dlo) / span, 0.f), 1.f) * (v->numTF - 1);
+  float b = std::min(std::max((hi - v->dlo) / span, 0.f), 1.f) * (v->numTF - 1);
+  const int ia = std::min(std::max(int(a), 0), v->numTF - 2);
+  const int ib = std::min(std::max(int(b), 0), v->numTF - 2);
+  a -= ia;
+  b -= ib;
+  const float la = (1.f - a) * v->tf[4 * ia + 3] + a * v->tf[4 * (ia + 1) + 3];
+  const float lb = (1.f - b) * v->tf[4 * ib + 3] + b * v->tf[4 * (ib + 1) + 3];
+  float m = std::fmax(std::fmax(0.f, la), lb);
+  for (int i = ia + 1; i <= ib; ++i) m = std::fmax(m, v->tf[4 * i + 3]);
+  return m * v->base;
+}
+
+// DDA.h:65-206 (callback returns false to stop)
+template <typename F>
+void dda3(V3 org, V3 dir, float tMax, const int *g, F &&cb) {
+  const V3 fs = mk((float)g[0], (float)g[1], (float)g[2]);
+  const V3 rd = inv(dir);
+  V3 tlo = mul(sub(mk(0, 0, 0), org), inv(dir)), thi = mul(sub(fs, org), inv(dir));
+  V3 tnr = vmin(tlo, thi), tfr = vmax(tlo, thi);
+  const float *o = &org.x, *d = &dir.x, *fsz = &fs.x;
+  float *pn = &tnr.x, *pf = &tfr.x;
+  for (int k = 0; k < 3; ++k)
+    if (d[k] == 0.f) {
+      if (o[k] < 0.f || o[k] > fsz[k]) return;
+      pn[k] = -kInf;
+      pf[k] = kInf;
+    }
+  const float t0 = std::fmax(0.f, maxc(tnr));
+  const float t1 = std::fmin(tMax, minc(tfr));
+  if (t0 > t1) return;
+  const V3 oiv = add(org, scl(t0, dir));
+  V3 fc = vmax(mk(0, 0, 0), vmin(sub(fs, mk(1, 1, 1)), mk(std::floor(oiv.x), std::floor(oiv.y), std::floor(oiv.z))));
+  const float fce[3] = {d[0] > 0.f ? fc.x + 1.f : fc.x, d[1] > 0.f ? fc.y + 1.f : fc.y, d[2] > 0.f ? fc.z + 1.f : fc.z};
+  const V3 ts = vabs(rd);
+  const float *pts = &ts.x, *po = &oiv.x;
+  float tn[3];
+  int stop[3], delta[3], cell[3] = {(int)fc.x, (int)fc.y, (int)fc.z};
+  for (int k = 0; k < 3; ++k) {
+    tn[k] = d[k] == 0.f ? kInf : std::fabs(fce[k] - po[k]) * pts[k];
+    stop[k] = d[k] > 0.f ? g[k] : -1;
+    delta[k] = d[k] > 0.f ? 1 : -1;
+  }
+  float ncb = 0.f;
+  while (true) {
+    const float tc = std::fmin(tn[0], std::fmin(tn[1], tn[2]));
+    const float c0 = t0 + ncb;
+    const float c1 = t0 + std::fmin(tc, tMax);
+    if (!cb(cell, c0, c1)) return;
+    ncb = tc;
+    for (int k = 0; k < 3; ++k)
+      if (tn[k] == tc) {
+        tn[k] += pts[k];
+        cell[k] += delta[k];
+        if (cell[k] == stop[k]) return;
+      }
+  }
+}
+
+struct VHit {
+  bool hit;
+  float t;
+  float rgb[3];
+};
+
+// MCAccelerator.h:569-708 + Volume.h:188-223
+VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed, bool exactLog,
+               long long &samples) {
+  VHit res;
+  res.hit = false;
+  float t0 = tmin, t1 = tmax;
+  {
+    const V3 a = mul(sub(v->wlo, org), inv(dir)), b = mul(sub(v->whi, org), inv(dir));
+    t0 = std::fmax(t0, maxc(vmin(a, b)));
+    t1 = std::fmin(t1, minc(vmax(a, b)));
+    if (!(t0 <= t1)) return res;
+  }
+  const V3 dorg = mul(sub(org, v->mcOrigin), inv(v->mcSpacing));
+  const V3 ddir = mul(dir, inv(v->mcSpacing));
+  Rng rng = child(seed, fnv3(0, 0, 0));
+  dda3(dorg, ddir, t1, v->m, [&](const int *c, float c0, float c1) -> bool {
+    const float maj = v->maj[(size_t)c[0] + (size_t)v->m[0] * ((size_t)c[1] + (size_t)v->m[1] * c[2])];
+    if (maj == 0.f) return true;
+    const float upper = std::fmin(c1, rayTMax);
+    float t = c0;
+    while (true) {
+      const float r = rng.uni();
+      const float dt = exactLog ? -std::log(1.f - r) / maj : -ref_fast_log(1.f - r) / maj;
+      t += dt;
+      if (t >= upper) return true;
+      const V3 P = add(org, scl(t, dir));
+      const float f = vsample(v, P);
+      float s[4] = {0, 0, 0, 0};
+      if (!std::isnan(f)) tfmap(v, f, s);
+      ++samples;
+      if (s[3] >= rng.uni() * maj) {
+        res.hit = true;
+        res.t = t;
+        res.rgb[0] = s[0]; res.rgb[1] = s[1]; res.rgb[2] = s[2];
+        return false;
+      }
+    }
+  });
+  return res;
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_volume *ref_volume_create(const float *scalars, const int *dims, const float *origin, const float *spacing,
+                              int cpm) {
+  auto *v = new ref_volume_s;
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  v->scalars.assign(scalars, scalars + n);
+  for (int k = 0; k < 3; ++k) v->n[k] = dims[k];
+  v->origin = ld3(origin);
+  v->spacing = ld3(spacing);
+  v->wlo = v->origin;
+  v->whi = add(v->origin, mul(v->spacing, mk((float)(dims[0] - 1), (float)(dims[1] - 1), (float)(dims[2] - 1))));
+  v->cpm = cpm;
+  for (int k = 0; k < 3; ++k) v->m[k] = (dims[k] - 1 + cpm - 1) / cpm;
+  v->mcOrigin = v->wlo;
+  v->mcSpacing = mul(mk((float)cpm, (float)cpm, (float)cpm), v->spacing);
+  const size_t nm = (size_t)v->m[0] * v->m[1] * v->m[2];
+  v->rlo.assign(nm, kInf);
+  v->rhi.assign(nm, -kInf);
+  v->maj.assign(nm, 0.f);
+  // StructuredData.cu:30-61: min/max over the (cpm+1)^3 voxels of each MC
+  for (int mz = 0; mz < v->m[2]; ++mz)
+    for (int my = 0; my < v->m[1]; ++my)
+      for (int mx = 0; mx < v->m[0]; ++mx) {
+        float lo = kInf, hi = -kInf;
+        for (int iz = 0; iz <= cpm; ++iz)
+          for (int iy = 0; iy <= cpm; ++iy)
+            for (int ix = 0; ix <= cpm; ++ix) {
+              const int x = mx * cpm + ix, y = my * cpm + iy, z = mz * cpm + iz;
+              if (x >= dims[0] || y >= dims[1] || z >= dims[2]) continue;
+              const float f = scalars[(size_t)x + (size_t)dims[0] * ((size_t)y + (size_t)dims[1] * z)];
+              lo = std::min(lo, f);
+              hi = std::max(hi, f);
+            }
+        const size_t i = (size_t)mx + (size_t)v->m[0] * ((size_t)my + (size_t)v->m[1] * mz);
+        v->rlo[i] = lo;
+        v->rhi[i] = hi;
+      }
+  return v;
+}
+void ref_volume_destroy(ref_volume *v) { delete v; }
+void ref_volume_set_tf(ref_volume *v, const float *values, int numValues, float lo, float hi, float base) {
+  v->tf.assign(values, values + 4 * numValues);
+  v->numTF = numValues;
+  v->dlo = lo;
+  v->dhi = hi;
+  v->base = base;
+  for (size_t i = 0; i < v->maj.size(); ++i) v->maj[i] = tfmaj(v, v->rlo[i], v->rhi[i]);
+}
+void ref_volume_mc_dims(const ref_volume *v, int *dims) {
+  for (int k = 0; k < 3; ++k) dims[k] = v->m[k];
+}
+void ref_volume_get_mcs(const ref_volume *v, float *ranges, float *maj) {
+  for (size_t i = 0; i < v->maj.size(); ++i) {
+    if (ranges) { ranges[2 * i] = v->rlo[i]; ranges[2 * i + 1] = v->rhi[i]; }
+    if (maj) maj[i] = v->maj[i];
+  }
+}
+float ref_volume_sample(const ref_volume *v, const float *P) { return vsample(v, ld3(P)); }
+void ref_tf_map(const ref_volume *v, float s, float *o) { tfmap(v, s, o); }
+float ref_tf_majorant(const ref_volume *v, float lo, float hi) { return tfmaj(v, lo, hi); }
+
+void ref_volume_sample_rays(const ref_volume *v, const float *rays, const uint64_t *seeds, float *out, int n,
+                            int exactLog, int threads, long long *samples) {
+  std::atomic<long long> ns(0);
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * i;
+    Rng seed;
+    seed.s = seeds[i];
+    long long cnt = 0;
+    const VHit h = vol_isect(v, ld3(r), fix_dir(ld3(r + 4)), r[3], r[7], r[7], seed, exactLog != 0, cnt);
+    ns += cnt;
+    float *o = out + 4 * i;
+    if (h.hit) { o[0] = h.t; o[1] = h.rgb[0]; o[2] = h.rgb[1]; o[3] = h.rgb[2]; }
+    else { o[0] = kInf; o[1] = o[2] = o[3] = 0.f; }
+  });
+  if (samples) *samples = ns.load();
+}
+
+int ref_dda_cells(const float *org, const float *dir, float tMax, const int *g, int *cells, float *ts, int maxCells) {
+  int n = 0;
+  dda3(ld3(org), ld3(dir), tMax, g, [&](const int *c, float c0, float c1) -> bool {
+    if (n < maxCells) {
+      cells[3 * n] = c[0]; cells[3 * n + 1] = c[1]; cells[3 * n + 2] = c[2];
+      ts[2 * n] = c0; ts[2 * n + 1] = c1;
+    }
+    ++n;
+    return n < 100000;
+  });
+  return n;
+}
+
+long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                            int y1, int threads, long long *samples) {
+  const Cam c = make_cam(cam);
+  const int W = p->width;
+  std::atomic<long long> nrays(0), ns(0);
+  parallel_for((y1 - y0) * W, default_threads(threads), [&](int li) {
+    const int ix = li % W, iy = y0 + li / W, pix = ix + iy * W;
+    float acc[4] = {0, 0, 0, 0};
+    long long nr = 0, cnt = 0;
+    for (int pp = 0; pp < p->pathsPerPixel; ++pp) {
+      const int a = p->accumID + pp;
+      PRay ray = cam_ray(c, p, ix, iy, a);
+      const float cmax = 10.f * (1 + a);
+      V3 sum = mk(0, 0, 0);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const V3 d = fix_dir(ray.dir);
+        const VHit vh = vol_isect(v, ray.org, d, 0.f, ray.tMax, ray.tMax, ray.rng, false, cnt);
+        ++nr;
+        HitInfo hi;
+        hi.hit = vh.hit;
+        if (vh.hit) {
+          hi.P = add(ray.org, scl(vh.t, d));
+          hi.N = mk(0, 0, 0);
+          hi.t = vh.t;
+          hi.type = 2;
+          hi.albedo = scl(.7f, mk(vh.rgb[0], vh.rgb[1], vh.rgb[2]));
+        }
+        if (gen == 0) alpha = vh.hit ? 1.f : ray.missA;
+        PRay sh;
+        add_frag(sum, bounce(p, ray, hi, sh, gen, 10, 1), cmax);
+        if (sh.tMax > 0.f) {
+          const VHit sv = vol_isect(v, sh.org, fix_dir(sh.dir), 0.f, sh.tMax, sh.tMax, sh.rng, false, cnt);
+          ++nr;
+          HitInfo hs;
+          hs.hit = sv.hit;
+          PRay dummy;
+          add_frag(sum, bounce(p, sh, hs, dummy, gen + 1, 10, 1), cmax);
+        }
+      }
+      acc[0] += sum.x; acc[1] += sum.y; acc[2] += sum.z; acc[3] += alpha;
+    }
+    float *o = accum + 4 * pix;
+    if (p->accumID == 0) { o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3]; }
+    else { o[0] += acc[0]; o[1] += acc[1]; o[2] += acc[2]; o[3] += acc[3]; }
+    nrays += nr;
+    ns += cnt;
+  });
+  if (samples) *samples = ns.load();
+  return nrays.load();
+}
+
+}  // extern "C"

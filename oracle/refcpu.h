@@ -1,0 +1,113 @@
+/* refcpu.h -- CPU oracle for the barney-amd ray-tracing core.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain C++ restatement of barney's
+ * embree-backend semantics for the hot path (the reference's embree, OWL and
+ * cuBQL dependencies are not available here, so nothing of the reference is
+ * linked). Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it, as the checker; the product never calls it.
+ *
+ * Each function cites the reference file:line it follows (paths relative to
+ * the reference repository root).
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- RNG (barney/common/barney-common.h:65-111) --- */
+uint64_t ref_rng_seed(uint64_t state, uint32_t a, uint32_t b);
+/* advances 'state' n times, writing each 32-bit output */
+void ref_rng_sequence(uint64_t *state, uint32_t *out, int n);
+/* Random2(a,b) then n floats */
+void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n);
+uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c);
+float ref_fast_log(float f);
+
+/* --- geometry input (host arrays) --- */
+typedef struct {
+  int kind;               /* 0 triangles, 1 spheres, 2 cylinders */
+  int numPrims;
+  const float *vertices;  /* float3 */
+  const int *indices;     /* int3 / int2 */
+  const float *radii;     /* or NULL */
+  float defaultRadius;
+  const float *colors;    /* per-prim float3 or NULL */
+  float color[3];         /* matte color */
+  int userID;
+} ref_geom;
+
+typedef struct ref_scene_s ref_scene;
+
+/* Builds the oracle's scene: a binned-SAH binary BVH over all geoms (one
+ * identity instance, geomID = index in the array). */
+ref_scene *ref_scene_create(const ref_geom *geoms, int numGeoms);
+void ref_scene_destroy(ref_scene *s);
+
+typedef struct {
+  float t, u, v;
+  int primID, geomID, instID;
+  float N[3];
+  int instIndex;
+} ref_hit;
+
+/* closest hit for each ray (rays: 8 floats org,tmin,dir,tmax);
+ * bruteForce != 0 tests every primitive. nodeVisits/primTests may be NULL. */
+void ref_trace(const ref_scene *s, const float *rays, ref_hit *hits, int n, int bruteForce,
+               int threads, long long *primTests);
+
+/* FP64 triangle intersection (no tie rule): returns t or -1 */
+double ref_triangle_t_f64(const double *org, const double *dir, const double *v0, const double *v1,
+                          const double *v2);
+
+/* --- camera + surface render (generateRays.cu, shadeRays.cu bounce) --- */
+typedef struct {
+  float position[3], direction[3], up[3];
+  float fovy;
+} ref_camera;
+
+typedef struct {
+  int width, height, accumID, pathsPerPixel;
+  float ambientRadiance;
+  float bgColor[4];
+  int maxPathDepth, maxDiffuseBounces;
+} ref_render_params;
+
+/* primary ray for pixel (ix,iy): out = org[3], dir[3], seed (as 2 floats bits) */
+void ref_camera_ray(const ref_camera *cam, const ref_render_params *p, int ix, int iy, float *org,
+                    float *dir, uint64_t *seed);
+
+/* renders rows [y0, y1) of the frame into accum (float4 per pixel, full
+ * frame layout); returns rays traced. */
+long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const ref_render_params *p,
+                             float *accum, int *ids, int y0, int y1, int threads);
+
+/* --- structured volume (barney/volume) --- */
+typedef struct ref_volume_s ref_volume;
+ref_volume *ref_volume_create(const float *scalars, const int *dims, const float *origin,
+                              const float *spacing, int cellsPerMC);
+void ref_volume_destroy(ref_volume *v);
+void ref_volume_set_tf(ref_volume *v, const float *values, int numValues, float lo, float hi,
+                       float baseDensity);
+void ref_volume_mc_dims(const ref_volume *v, int *dims);
+void ref_volume_get_mcs(const ref_volume *v, float *ranges, float *majorants);
+float ref_volume_sample(const ref_volume *v, const float *P); /* trilinear, NaN outside */
+void ref_tf_map(const ref_volume *v, float s, float *out4);
+float ref_tf_majorant(const ref_volume *v, float lo, float hi);
+/* first accepted Woodcock sample per ray: out = t (inf if none), r, g, b */
+void ref_volume_sample_rays(const ref_volume *v, const float *rays, const uint64_t *seeds, float *out,
+                            int n, int exactLog, int threads, long long *samples);
+/* DDA cell enumeration for one ray (grid space): writes up to maxCells
+ * (x,y,z,t0bits,t1bits) records; returns the number of cells visited. */
+int ref_dda_cells(const float *org, const float *dir, float tMax, const int *gridSize, int *cells,
+                  float *ts, int maxCells);
+long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p,
+                            float *accum, int y0, int y1, int threads, long long *samples);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

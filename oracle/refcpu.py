@@ -1,0 +1,235 @@
+"""ctypes binding of the CPU oracle (oracle/librefcpu.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "librefcpu.so")
+
+
+def build():
+    r = subprocess.run(["make", "-C", HERE, "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB
+
+
+class ref_geom(C.Structure):
+    _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
+                ("radii", C.c_void_p), ("defaultRadius", C.c_float), ("colors", C.c_void_p),
+                ("color", C.c_float * 3), ("userID", C.c_int)]
+
+
+class ref_hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("primID", C.c_int), ("geomID", C.c_int),
+                ("instID", C.c_int), ("N", C.c_float * 3), ("instIndex", C.c_int)]
+
+
+class ref_camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float)]
+
+
+class ref_render_params(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int), ("pathsPerPixel", C.c_int),
+                ("ambientRadiance", C.c_float), ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
+                ("maxDiffuseBounces", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    P = C.c_void_p
+    sig = {
+        "ref_rng_seed": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+        "ref_rng_sequence": (None, [P, P, C.c_int]),
+        "ref_random2_floats": (None, [C.c_uint32, C.c_uint32, P, C.c_int]),
+        "ref_hash3": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "ref_fast_log": (C.c_float, [C.c_float]),
+        "ref_scene_create": (P, [P, C.c_int]),
+        "ref_scene_destroy": (None, [P]),
+        "ref_trace": (None, [P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_triangle_t_f64": (C.c_double, [P, P, P, P, P]),
+        "ref_camera_ray": (None, [P, P, C.c_int, C.c_int, P, P, P]),
+        "ref_render_surface": (C.c_longlong, [P, P, P, P, P, C.c_int, C.c_int, C.c_int]),
+        "ref_volume_create": (P, [P, P, P, P, C.c_int]),
+        "ref_volume_destroy": (None, [P]),
+        "ref_volume_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "ref_volume_mc_dims": (None, [P, P]),
+        "ref_volume_get_mcs": (None, [P, P, P]),
+        "ref_volume_sample": (C.c_float, [P, P]),
+        "ref_tf_map": (None, [P, C.c_float, P]),
+        "ref_tf_majorant": (C.c_float, [P, C.c_float, C.c_float]),
+        "ref_volume_sample_rays": (None, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_dda_cells": (C.c_int, [P, P, C.c_float, P, P, P, C.c_int]),
+        "ref_render_volume": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Scene:
+    """Oracle scene over host geometry (one identity instance)."""
+
+    def __init__(self, geoms):
+        """geoms: list of dicts kind, vertices, indices, radii, colors,
+        default_radius, color, user_id."""
+        L = load()
+        self._keep = []
+        arr = (ref_geom * max(len(geoms), 1))()
+        for k, g in enumerate(geoms):
+            v = np.ascontiguousarray(g["vertices"], np.float32)
+            i = None if g.get("indices") is None else np.ascontiguousarray(g["indices"], np.int32)
+            r = None if g.get("radii") is None else np.ascontiguousarray(g["radii"], np.float32)
+            c = None if g.get("colors") is None else np.ascontiguousarray(g["colors"], np.float32)
+            self._keep += [v, i, r, c]
+            kind = g["kind"]
+            n = len(v.reshape(-1, 3)) if kind == 1 else len(i.reshape(-1, 3 if kind == 0 else 2))
+            e = arr[k]
+            e.kind, e.numPrims = kind, n
+            e.vertices, e.indices, e.radii, e.colors = _ptr(v), _ptr(i), _ptr(r), _ptr(c)
+            e.defaultRadius = float(g.get("default_radius", 1.0))
+            e.color[:] = [float(x) for x in g.get("color", (0.8, 0.8, 0.8))]
+            e.userID = int(g.get("user_id", 0))
+        self._arr = arr
+        self.handle = L.ref_scene_create(arr, len(geoms))
+
+    def trace(self, rays, brute=False, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = (ref_hit * max(n, 1))()
+        tests = C.c_longlong(0)
+        load().ref_trace(self.handle, rays.ctypes.data, hits, n, int(brute), threads, C.byref(tests))
+        raw = np.frombuffer(hits, dtype=np.float32, count=10 * n).reshape(n, 10)
+        ints = raw.view(np.int32)
+        return {"t": raw[:, 0].copy(), "u": raw[:, 1].copy(), "v": raw[:, 2].copy(), "primID": ints[:, 3].copy(),
+                "geomID": ints[:, 4].copy(), "instID": ints[:, 5].copy(), "N": raw[:, 6:9].copy(),
+                "instIndex": ints[:, 9].copy(), "primTests": tests.value}
+
+    def render(self, cam, params, y0=0, y1=None, threads=0, with_ids=False, accum=None):
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        ids = np.zeros((params.height, params.width, 4), np.int32) if with_ids else None
+        n = load().ref_render_surface(self.handle, C.byref(cam), C.byref(params), acc.ctypes.data, _ptr(ids), y0,
+                                      y1, threads)
+        return acc, ids, n
+
+    def __del__(self):
+        try:
+            load().ref_scene_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Volume:
+    def __init__(self, scalars, origin=(0, 0, 0), spacing=(1, 1, 1), cells_per_mc=8):
+        scalars = np.ascontiguousarray(scalars, np.float32)
+        nz, ny, nx = scalars.shape
+        dims = np.array([nx, ny, nz], np.int32)
+        o = np.array(origin, np.float32)
+        s = np.array(spacing, np.float32)
+        self.handle = load().ref_volume_create(scalars.ctypes.data, dims.ctypes.data, o.ctypes.data,
+                                               s.ctypes.data, int(cells_per_mc))
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
+        load().ref_volume_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]), float(domain[1]),
+                                 float(base_density))
+
+    def mc_dims(self):
+        d = np.zeros(3, np.int32)
+        load().ref_volume_mc_dims(self.handle, d.ctypes.data)
+        return tuple(int(x) for x in d)
+
+    def macrocells(self):
+        mx, my, mz = self.mc_dims()
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        load().ref_volume_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data)
+        return ranges, maj
+
+    def sample(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        return load().ref_volume_sample(self.handle, P.ctypes.data)
+
+    def tf_map(self, s):
+        o = np.zeros(4, np.float32)
+        load().ref_tf_map(self.handle, float(s), o.ctypes.data)
+        return o
+
+    def tf_majorant(self, lo, hi):
+        return load().ref_tf_majorant(self.handle, float(lo), float(hi))
+
+    def sample_rays(self, rays, seeds, exact_log=False, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        out = np.zeros((len(rays), 4), np.float32)
+        ns = C.c_longlong(0)
+        load().ref_volume_sample_rays(self.handle, rays.ctypes.data, seeds.ctypes.data, out.ctypes.data, len(rays),
+                                      int(exact_log), threads, C.byref(ns))
+        return out, ns.value
+
+    def render(self, cam, params, y0=0, y1=None, threads=0, accum=None):
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        ns = C.c_longlong(0)
+        n = load().ref_render_volume(self.handle, C.byref(cam), C.byref(params), acc.ctypes.data, y0, y1, threads,
+                                     C.byref(ns))
+        return acc, n, ns.value
+
+    def __del__(self):
+        try:
+            load().ref_volume_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def camera(position, direction, up=(0, 1, 0), fovy=60.0):
+    c = ref_camera()
+    c.position[:] = [float(x) for x in position]
+    c.direction[:] = [float(x) for x in direction]
+    c.up[:] = [float(x) for x in up]
+    c.fovy = float(fovy)
+    return c
+
+
+def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
+                  max_diffuse=1):
+    p = ref_render_params()
+    p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
+    p.ambientRadiance = float(ambient)
+    p.bgColor[:] = [float(x) for x in bg]
+    p.maxPathDepth, p.maxDiffuseBounces = max_depth, max_diffuse
+    return p
+
+
+def dda_cells(org, dir, tmax, grid, max_cells=4096):
+    org = np.ascontiguousarray(org, np.float32)
+    dir = np.ascontiguousarray(dir, np.float32)
+    g = np.ascontiguousarray(grid, np.int32)
+    cells = np.zeros((max_cells, 3), np.int32)
+    ts = np.zeros((max_cells, 2), np.float32)
+    n = load().ref_dda_cells(org.ctypes.data, dir.ctypes.data, float(tmax), g.ctypes.data, cells.ctypes.data,
+                             ts.ctypes.data, max_cells)
+    return cells[:min(n, max_cells)], ts[:min(n, max_cells)], n
