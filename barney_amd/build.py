@@ -20,6 +20,7 @@ SOURCES = [
     "trace.hip",
     "render_surface.hip",
     "volume.hip",
+    "ea.hip",
 ]
 
 # -packed-fp32-ops: packed FP32 forms are kept out of the generated code.
@@ -27,6 +28,9 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
+    # every fp32 expression rounds as written (bit-comparable with the
+    # oracle); hot loops use explicit fmaf where fusion is wanted
+    "-ffp-contract=off",
 ]
 
 

@@ -1,0 +1,171 @@
+"""Benchmark: structured-volume delta tracking on one MI355X.
+
+Workload (the volume configuration the headline metric is quoted on): a
+512^3 fp32 field (32 seeded Gaussian blobs + 4-octave value noise), 256-entry
+RGBA ramp transfer function, 16^3-voxel macrocells, delta tracking with
+single scattering, 1920x1080, 16 paths per pixel. One step = one full
+frame (all 16 paths) through bnamd_volume_render.
+
+Prints ONE JSON line (value = volume Msamples/s), with the roofline of the
+dominant kernel, the CPU oracle timed on a bounded sample of the same frame,
+and the surface megakernel's Mrays/s on the 1,024 tessellated spheres scene
+as an extra figure.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Mrays/s + volume Msamples/s on 1 MI355X; adjoint ms/iter vs forward"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
+    from barney_amd import scenes
+    t0 = time.time()
+    w2 = scenes.workload2(n=n)
+    log(f"field {n}^3 generated in {time.time() - t0:.1f}s")
+    vol = api.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
+    vol.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
+    W, H = w2["size"]
+    spp = w2["spp"]
+    cam = api.camera(**w2["camera"])
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    for k in range(warmup):
+        vol.render(cam, api.render_params(W, H, spp=spp, accum_id=0), accum=acc)
+    api.capi.check(api.capi.load().bnamd_synchronize())
+    samples = dda = rays = 0
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o = vol.render(cam, api.render_params(W, H, spp=spp, accum_id=0), accum=acc)
+        samples += o["stats"]["samples"]
+        dda += o["stats"]["ddaSteps"]
+        rays += o["stats"]["raysTraced"]
+        kernel_ms += o["stats"]["ms"]
+        log(f"volume step {k}: {o['stats']['ms']:.1f} ms, {o['stats']['samples'] / 1e6:.1f} M samples")
+    api.capi.check(api.capi.load().bnamd_synchronize())
+    wall = time.perf_counter() - t0
+    res = {"samples": samples, "ddaSteps": dda, "rays": rays, "wall_s": wall, "kernel_ms": kernel_ms,
+           "steps": steps, "image_mean": float(acc.numpy()[..., :3].mean() / spp)}
+    if cpu:
+        res["cpu"] = volume_cpu_baseline(w2, cpu_seconds)
+    return res
+
+
+def volume_cpu_baseline(w2, seconds):
+    """The oracle on host cores over a band of rows of the same frame."""
+    from oracle import refcpu
+    import ctypes as C
+    threads = os.cpu_count() or 1
+    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+    rv = refcpu.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
+    rv.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
+    W, H = w2["size"]
+    spp = w2["spp"]
+    cam = refcpu.camera(**w2["camera"])
+    p = refcpu.render_params(W, H, spp=spp)
+    acc = np.zeros((H, W, 4), np.float32)
+    mid = H // 2
+    # calibrate on one row, then run a band sized to ~'seconds'
+    t0 = time.perf_counter()
+    _, _, s1 = rv.render(cam, p, mid, mid + 1, threads, accum=acc)
+    dt = time.perf_counter() - t0
+    rows = int(max(1, min(H // 2, seconds / max(dt, 1e-3))))
+    y0 = max(0, mid - rows // 2)
+    t0 = time.perf_counter()
+    _, nrays, ns = rv.render(cam, p, y0, y0 + rows, threads, accum=acc)
+    dt = time.perf_counter() - t0
+    return {"value": ns / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {y0}..{y0 + rows - 1} of the 1920x1080x16spp frame ({ns} samples, {dt:.1f}s)"}
+
+
+def surface_bench(api, steps=3, warmup=1):
+    from barney_amd import scenes
+    w1 = scenes.workload1()
+    geoms, mats = [], []
+    for k, (v, t, c) in enumerate(w1["geoms"]):
+        geoms.append(api.Geom(api.GEOM_TRIANGLES, v, t, material=k, user_id=k))
+        mats.append(api.matte(c))
+    world = api.World([geoms], materials=mats)
+    info = world.info()
+    W, H = w1["size"]
+    cam = api.camera(**w1["camera"])
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    for _ in range(warmup):
+        api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
+    rays = 0
+    ms = 0.0
+    for _ in range(steps):
+        o = api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
+        rays += o["stats"]["raysTraced"]
+        ms += o["stats"]["ms"]
+    return {"Mrays_per_s": rays / (ms * 1e-3) / 1e6, "rays_per_frame": rays // steps,
+            "ms_per_frame": ms / steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
+            "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surface", action="store_true")
+    ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by image tiles / views, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from barney_amd import api
+    api.init(0)
+    vb = volume_bench(api, args.steps, args.warmup, n=args.n, cpu=not args.no_cpu)
+    per_launch_samples = vb["samples"] / vb["steps"]
+    per_launch_dda = vb["ddaSteps"] / vb["steps"]
+    kernel_s = vb["kernel_ms"] * 1e-3 / vb["steps"]
+    alg_bytes = 32.0 * per_launch_samples + 4.0 * per_launch_dda
+    achieved = alg_bytes / kernel_s / 1e9
+    value = vb["samples"] / vb["wall_s"] / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(vb["wall_s"] * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "structured volume 512^3 fp32, 32 Gaussian blobs + 4-octave value noise, "
+                               "256-entry RGBA TF, 16^3-voxel macrocells, delta tracking + single scattering, "
+                               "1920x1080, 16 spp",
+                   "width": 1920, "height": 1080, "spp": 16, "volume": args.n, "cells_per_mc": 16},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_render_volume", "avg_launch_ms": round(kernel_s * 1e3, 3),
+                     "alg_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": vb.get("cpu"),
+        "extra": {"samples_per_step": int(per_launch_samples), "dda_steps_per_step": int(per_launch_dda),
+                  "rays_per_step": vb["rays"] // vb["steps"], "image_mean": vb["image_mean"]},
+    }
+    if not args.no_surface:
+        out["extra"]["surface_workload1"] = surface_bench(api)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

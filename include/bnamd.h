@@ -213,6 +213,51 @@ int bnamd_volume_sample_rays(const bnamd_volume *vol, const bnamd_ray *rays,
                              const uint64_t *seeds, float *out, int numRays, int exactLog,
                              void *stream);
 
+/* ------------------------------------------------------------------------
+ * Emission-absorption volume rendering and its adjoint (differentiable
+ * form; not in the reference). Fixed-step march over a structured density
+ * grid sampled like StructuredDataSampler (barney/volume/StructuredData.h:
+ * 104-115) and mapped through a TransferFunction (TransferFunction.h:67-86):
+ *   C = sum_i T_i a_i c_i,  a_i = 1 - exp(-s_i dt),  T_{i+1} = T_i (1 - a_i)
+ * with (c_i, s_i) = TF(density(x_i)), x_i = o + (t0 + (i + 1/2) dt) d over
+ * the ray's overlap [t0, t1] with the grid box, i < floor((t1 - t0) / dt).
+ * The forward pass stores transmittance every 'checkpointEvery' samples;
+ * the adjoint replays each segment from its checkpoint in reverse order.
+ * --------------------------------------------------------------------- */
+typedef struct {
+  const float *density; /* device float[dims.x*dims.y*dims.z], x fastest */
+  int dims[3];
+  float gridOrigin[3];
+  float gridSpacing[3];
+  const float *tf;      /* device float4[numTF] (rgb, density) */
+  int numTF;
+  float domainLo, domainHi;
+  float baseDensity;
+  float step;           /* march step in world units */
+} bnamd_ea_volume;
+
+typedef struct {
+  int width, height, numViews;
+  const bnamd_camera *cameras; /* host array [numViews] */
+  int checkpointEvery;         /* samples per checkpoint segment (e.g. 16) */
+} bnamd_ea_views;
+
+typedef struct {
+  long long samples;
+  float ms;
+} bnamd_ea_stats;
+
+/* checkpoints per ray the forward pass writes (buffer = rays * count floats) */
+int bnamd_ea_checkpoint_count(const bnamd_ea_volume *vol, const bnamd_ea_views *views, int *count);
+/* image: device float3[numViews*height*width]; checkpoints: device or NULL */
+int bnamd_ea_forward(const bnamd_ea_volume *vol, const bnamd_ea_views *views, float *image,
+                     float *checkpoints, void *stream, bnamd_ea_stats *stats);
+/* gradImage: device float3 per pixel (dL/dC); gradDensity (dims) and
+ * gradTF (float4[numTF]) are accumulated into (caller zeroes them). */
+int bnamd_ea_backward(const bnamd_ea_volume *vol, const bnamd_ea_views *views, const float *gradImage,
+                      const float *checkpoints, float *gradDensity, float *gradTF, void *stream,
+                      bnamd_ea_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
