@@ -73,6 +73,11 @@ SIGNATURES = {
     "bnamd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "bnamd_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "bnamd_synchronize": (C.c_int, []),
+    "bnamd_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "bnamd_ea_checkpoint_count": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    "bnamd_ea_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_ea_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
     "bnamd_world_build": (C.c_int, [C.POINTER(bnamd_group), C.c_int, C.POINTER(bnamd_instance), C.c_int,
                                     C.POINTER(bnamd_material), C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "bnamd_world_destroy": (C.c_int, [C.c_void_p]),

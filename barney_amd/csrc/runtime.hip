@@ -64,3 +64,24 @@ extern "C" int bnamd_memset(void *dst, int value, size_t bytes) {
 extern "C" int bnamd_synchronize(void) {
   return guarded([&] { BN_HIP(hipDeviceSynchronize()); });
 }
+
+namespace {
+// Plain streaming copy (one float4 per thread per iteration); used to
+// calibrate HBM byte counters against a kernel of known traffic.
+__global__ void k_copy(const float4 *__restrict src, float4 *__restrict dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int bnamd_copy_kernel(void *dst, const void *src, size_t bytes, void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(bytes % 16 == 0, "bnamd_copy_kernel: size must be a multiple of 16");
+    BN_REQUIRE(dst && src, "bnamd_copy_kernel: NULL pointer");
+    const size_t n = bytes / 16;
+    hipStream_t s = (hipStream_t)stream;
+    k_copy<<<2048, 256, 0, s>>>((const float4 *)src, (float4 *)dst, n);
+    BN_HIP(hipGetLastError());
+    if (!stream) BN_HIP(hipStreamSynchronize(s));
+  });
+}

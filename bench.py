@@ -81,7 +81,7 @@ def volume_cpu_baseline(w2, seconds):
     t0 = time.perf_counter()
     _, _, s1 = rv.render(cam, p, mid, mid + 1, threads, accum=acc)
     dt = time.perf_counter() - t0
-    rows = int(max(1, min(H // 2, seconds / max(dt, 1e-3))))
+    rows = int(max(1, min(H, seconds / max(dt, 1e-3))))
     y0 = max(0, mid - rows // 2)
     t0 = time.perf_counter()
     _, nrays, ns = rv.render(cam, p, y0, y0 + rows, threads, accum=acc)
@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surface", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -130,6 +132,12 @@ def main():
         sys.exit(2)
     from barney_amd import api
     api.init(0)
+    if args.copy_calib:
+        src = api.DeviceArray.empty((1 << 28,), np.float32).zero()
+        dst = api.DeviceArray.empty((1 << 28,), np.float32)
+        for _ in range(3):
+            api.capi.check(api.capi.load().bnamd_copy_kernel(dst.addr, src.addr, 1 << 30, None))
+        log("copy calibration: 3 x 1 GiB copies done")
     vb = volume_bench(api, args.steps, args.warmup, n=args.n, cpu=not args.no_cpu)
     per_launch_samples = vb["samples"] / vb["steps"]
     per_launch_dda = vb["ddaSteps"] / vb["steps"]

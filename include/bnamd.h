@@ -38,6 +38,9 @@ int bnamd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int bnamd_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int bnamd_memset(void *dst, int value, size_t bytes);
 int bnamd_synchronize(void);
+/* Streaming device copy kernel (bytes % 16 == 0); a traffic reference for
+ * hardware byte counters. No counterpart in the reference. */
+int bnamd_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------------------
  * Geometry and acceleration structures.
