@@ -215,13 +215,22 @@ BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmi
   const int dx = ddir.x > 0.f ? 1 : -1, dy = ddir.y > 0.f ? 1 : -1, dz = ddir.z > 0.f ? 1 : -1;
   int cx = (int)f_cell.x, cy = (int)f_cell.y, cz = (int)f_cell.z;
   float next_cell_begin = 0.f;
+  // The next cell of the walk is known before the current cell's Woodcock
+  // loop runs, so its majorant is fetched one step ahead to overlap the load
+  // latency with the sampling of the current cell.
+  float majorant = v.majorants[cx + v.mx * (cy + v.my * cz)];
   while (true) {
     const float t_closest = reduce_min(t_next);
     const float cell_t0 = ray_t0 + next_cell_begin;
     const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
     cnt.ddaSteps++;
+    int ncx = cx, ncy = cy, ncz = cz;
+    if (t_next.x == t_closest) ncx += dx;
+    if (t_next.y == t_closest) ncy += dy;
+    if (t_next.z == t_closest) ncz += dz;
+    const bool inside = ncx >= 0 && ncx < v.mx && ncy >= 0 && ncy < v.my && ncz >= 0 && ncz < v.mz;
+    const float nextMajorant = inside ? v.majorants[ncx + v.mx * (ncy + v.my * ncz)] : 0.f;
     // ---- lambda body (MCAccelerator.h:619-705) ----
-    const float majorant = v.majorants[cx + v.mx * (cy + v.my * cz)];
     if (majorant != 0.f) {
       const float lower = cell_t0;
       const float upper = fminf(cell_t1, rayTMax);
@@ -244,6 +253,7 @@ BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmi
       }
     }
     // ---- DDA step ----
+    majorant = nextMajorant;
     next_cell_begin = t_closest;
     if (t_next.x == t_closest) {
       t_next.x += t_step.x;

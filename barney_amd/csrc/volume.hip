@@ -61,21 +61,32 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long
   if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
 }
 
+// One thread per (pixel, path): the L = min(64, pow2 >= spp) paths of a pixel
+// run in adjacent lanes (so a wavefront marches near-identical rays through
+// the same macrocells and voxels), each lane looping over paths sub,
+// sub+L, ...; the per-pixel sum is a butterfly over those L lanes.
+template <int L>
 __global__ void __launch_bounds__(256) k_render_volume(VolumeDD v, CameraDD cam, RenderDD rp, int spp,
                                                        float4 *__restrict accum,
                                                        unsigned long long *__restrict counters) {
-  const int tilesX = (rp.width + 15) >> 4, tilesY = (rp.height + 15) >> 4;
+  constexpr int PW = 64 / L;            // pixels per wavefront
+  constexpr int PB = 4 * PW;            // pixels per workgroup
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  const int tilesX = (rp.width + TW - 1) / TW, tilesY = (rp.height + TH - 1) / TH;
   const int tile = xcd_remap_v(blockIdx.x, tilesX * tilesY);
   if (tile < 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ix = (tile % tilesX) * 16 + (wave & 1) * 8 + (lane & 7);
-  const int iy = (tile / tilesX) * 16 + (wave >> 1) * 8 + (lane >> 3);
+  const int sub = lane & (L - 1);
+  const int q = wave * PW + lane / L;
+  const int ix = (tile % tilesX) * TW + (q % TW);
+  const int iy = (tile / tilesX) * TH + (q / TW);
   VolCounters cnt{0, 0};
   unsigned long long nrays = 0;
-  if (ix < rp.width && iy < rp.height) {
-    const int pix = ix + iy * rp.width;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int p = 0; p < spp; ++p) {
+  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
+  const bool active = ix < rp.width && iy < rp.height;
+  if (active) {
+    for (int p = sub; p < spp; p += L) {
       RenderDD rpp = rp;
       rpp.accumID = rp.accumID + p;
       PathRay ray = camera_ray(cam, rpp, ix, iy);
@@ -114,22 +125,63 @@ __global__ void __launch_bounds__(256) k_render_volume(VolumeDD v, CameraDD cam,
           if (frag.z > 0.f) sum.z += frag.z;
         }
       }
-      acc.x += sum.x;
-      acc.y += sum.y;
-      acc.z += sum.z;
-      acc.w += alpha;
+      sr += sum.x;
+      sg += sum.y;
+      sb += sum.z;
+      sa += alpha;
     }
+  }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    sg += __shfl_xor(sg, off);
+    sb += __shfl_xor(sb, off);
+    sa += __shfl_xor(sa, off);
+  }
+  if (active && sub == 0) {
+    const int pix = ix + iy * rp.width;
     if (rp.accumID == 0) {
-      accum[pix] = acc;
+      accum[pix] = make_float4(sr, sg, sb, sa);
     } else {
       float4 o = accum[pix];
-      o.x += acc.x; o.y += acc.y; o.z += acc.z; o.w += acc.w;
+      o.x += sr; o.y += sg; o.z += sb; o.w += sa;
       accum[pix] = o;
     }
   }
   wave_add_u64(cnt.samples, counters + 0);
   wave_add_u64(cnt.ddaSteps, counters + 1);
   wave_add_u64(nrays, counters + 2);
+}
+
+template <int L>
+void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
+                          unsigned long long *counters, hipStream_t s) {
+  constexpr int PB = 4 * (64 / L);
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
+  k_render_volume<L><<<tiles, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
+}
+
+void render_volume_dispatch(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
+                            unsigned long long *counters, hipStream_t s) {
+  // Lanes per pixel: a few lanes per pixel, each looping over several of
+  // its paths, balance coherence (near-identical rays in a wavefront)
+  // against per-lane work variance; 4 measured best for 16 spp.
+  // BNAMD_VOLUME_LANES overrides (1/2/4/8) for tuning.
+  int lanes = 4;
+  if (const char *e = getenv("BNAMD_VOLUME_LANES")) lanes = atoi(e);
+  if (lanes <= 1) { launch_render_volume<1>(v, cam, rp, spp, accum, counters, s); return; }
+  if (lanes == 2) { launch_render_volume<2>(v, cam, rp, spp, accum, counters, s); return; }
+  if (lanes == 4) { launch_render_volume<4>(v, cam, rp, spp, accum, counters, s); return; }
+  if (lanes == 8) { launch_render_volume<8>(v, cam, rp, spp, accum, counters, s); return; }
+  if (spp >= 64) launch_render_volume<64>(v, cam, rp, spp, accum, counters, s);
+  else if (spp > 16) launch_render_volume<32>(v, cam, rp, spp, accum, counters, s);
+  else if (spp > 8) launch_render_volume<16>(v, cam, rp, spp, accum, counters, s);
+  else if (spp > 4) launch_render_volume<8>(v, cam, rp, spp, accum, counters, s);
+  else if (spp > 2) launch_render_volume<4>(v, cam, rp, spp, accum, counters, s);
+  else if (spp > 1) launch_render_volume<2>(v, cam, rp, spp, accum, counters, s);
+  else launch_render_volume<1>(v, cam, rp, spp, accum, counters, s);
 }
 
 __global__ void k_volume_rays(VolumeDD v, const bnamd_ray *__restrict rays, const uint64_t *__restrict seeds,
@@ -248,13 +300,11 @@ extern "C" int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *
     BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
-    const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
     hipEvent_t e0, e1;
     BN_HIP(hipEventCreate(&e0));
     BN_HIP(hipEventCreate(&e1));
     BN_HIP(hipEventRecord(e0, s));
-    k_render_volume<<<tiles, 256, 0, s>>>(vol->v.dd, cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                          counters.ptr);
+    render_volume_dispatch(vol->v.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, counters.ptr, s);
     BN_HIP(hipGetLastError());
     BN_HIP(hipEventRecord(e1, s));
     BN_HIP(hipEventSynchronize(e1));
