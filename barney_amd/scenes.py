@@ -126,3 +126,42 @@ def workload2(n=512, seed=0x5EED0001):
     cam = dict(position=pos, direction=(c - pos[0], c - pos[1], c - pos[2]), up=(0.0, 1.0, 0.0), fovy=40.0)
     return dict(field=field, tf=ramp_tf(), base_density=0.05, cells_per_mc=16, camera=cam, size=(1920, 1080),
                 spp=16)
+
+
+# ---------------------------------------------------------------------------
+# Workload 4: differentiable emission-absorption volume
+# ---------------------------------------------------------------------------
+
+def random_tf(seed, n=256, knots=9):
+    """Smooth random RGBA transfer function (piecewise-linear through
+    'knots' random control points), opacity in [0, 0.6]."""
+    x = np.linspace(0.0, 1.0, n)
+    kx = np.linspace(0.0, 1.0, knots)
+    vals = uniform(seed, 7, 4 * knots).reshape(knots, 4)
+    vals[:, 3] *= 0.6
+    vals[0, 3] = 0.0
+    return np.stack([np.interp(x, kx, vals[:, k]) for k in range(4)], -1).astype(np.float32)
+
+
+def orbit_cameras(center, radius, count, fovy):
+    """'count' views on a Fibonacci sphere around 'center', looking at it."""
+    cams = []
+    ga = np.pi * (3.0 - np.sqrt(5.0))
+    for k in range(count):
+        y = 1.0 - 2.0 * (k + 0.5) / count
+        r = np.sqrt(max(0.0, 1.0 - y * y))
+        a = ga * k
+        d = np.array([np.cos(a) * r, y, np.sin(a) * r])
+        pos = np.asarray(center) + radius * d
+        up = (0.0, 1.0, 0.0) if abs(y) < 0.99 else (1.0, 0.0, 0.0)
+        cams.append(dict(position=tuple(pos), direction=tuple(np.asarray(center) - pos), up=up, fovy=fovy))
+    return cams
+
+
+def workload4(n=256, views=64, size=512, seed=0x5EED0003):
+    dens = structured_field(n=n, blobs=32, sigma=(10.0 * n / 512 * 2, 60.0 * n / 512 * 2), octaves=0, seed=seed)
+    c = (n - 1) / 2.0
+    diag = (n - 1) * np.sqrt(3.0)
+    cams = orbit_cameras((c, c, c), 1.5 * diag, views, fovy=40.0)
+    return dict(density=dens, tf_target=random_tf(seed + 1), tf_init=random_tf(seed + 2), cameras=cams,
+                size=(size, size), step=0.5, base_density=0.05)

@@ -115,6 +115,40 @@ def surface_bench(api, steps=3, warmup=1):
             "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
 
 
+def ea_bench(steps=3):
+    """Workload 4: 256^3 density, 64 orbit views at 512^2, L2 loss against
+    images of a second transfer function; forward and checkpointed adjoint."""
+    import torch
+    from barney_amd import ea, scenes
+    w4 = scenes.workload4()
+    n = w4["density"].shape[0]
+    W, H = w4["size"]
+    setup = ea.EASetup((n, n, n), w4["cameras"], W, H, step=w4["step"], base_density=w4["base_density"],
+                       checkpoint_every=16)
+    dens = torch.tensor(w4["density"], device="cuda", requires_grad=True)
+    tf = torch.tensor(w4["tf_init"], device="cuda", requires_grad=True)
+    with torch.no_grad():
+        target = ea.ea_render(dens.detach(), torch.tensor(w4["tf_target"], device="cuda"), setup)
+    fwd, bwd = [], []
+    for k in range(steps + 1):
+        dens.grad = None
+        tf.grad = None
+        img = ea.ea_render(dens, tf, setup)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        if k > 0:
+            fwd.append(setup.last_forward["ms"])
+            bwd.append(setup.last_backward["ms"])
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    samples = setup.last_forward["samples"]
+    return {"forward_ms": f, "adjoint_ms": b, "adjoint_over_forward": b / f, "samples_per_iter": samples,
+            "forward_Msamples_per_s": samples / (f * 1e-3) / 1e6,
+            "forward_GBps_alg": 32.0 * samples / (f * 1e-3) / 1e9,
+            "adjoint_GBps_alg": 96.0 * samples / (b * 1e-3) / 1e9,
+            "loss": float(loss), "grad_tf_norm": float(tf.grad.norm()), "grad_density_norm": float(dens.grad.norm())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +156,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surface", action="store_true")
+    ap.add_argument("--no-ea", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
@@ -172,6 +207,8 @@ def main():
     }
     if not args.no_surface:
         out["extra"]["surface_workload1"] = surface_bench(api)
+    if not args.no_ea:
+        out["extra"]["ea_workload4"] = ea_bench()
     print(json.dumps(out), flush=True)
 
 
