@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbnamd.so")
+# BNAMD_LIBRARY selects another build of the same ABI (e.g. a debug build)
+LIB_PATH = os.environ.get("BNAMD_LIBRARY", os.path.join(HERE, "libbnamd.so"))
 
 
 class bnamd_geom(C.Structure):

@@ -196,22 +196,109 @@ __global__ void __launch_bounds__(256) k_ea_forward(EADD e, float *__restrict im
   if ((threadIdx.x & 63) == 0) atomicAdd(counter, nsamp);
 }
 
-// Adds a corner-weighted gradient for one trilinear cell.
+// Per-workgroup LDS cache of density gradients: 64-bit entries (value bits
+// << 32 | voxel index), updated with compare-and-swap. A colliding voxel
+// evicts the resident one to global memory; the cache is drained after each
+// tile. Rays of one tile touch overlapping voxels, so most contributions are
+// merged in LDS before reaching the device atomics.
+constexpr int kCacheSlots = 2048;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+__device__ __forceinline__ void grad_add(unsigned long long *cache, float *__restrict gd, uint32_t v, float g) {
+  const uint32_t slot = (v * 2654435761u) >> 21;  // 11-bit hash
+  unsigned long long old = cache[slot];
+  for (int tries = 0; tries < 8; ++tries) {
+    const uint32_t k = (uint32_t)old;
+    const float val = __uint_as_float((uint32_t)(old >> 32));
+    const float nval = (k == v) ? val + g : g;
+    const unsigned long long nw = ((unsigned long long)__float_as_uint(nval) << 32) | v;
+    const unsigned long long prev = atomicCAS(&cache[slot], old, nw);
+    if (prev == old) {
+      if (k != v && k != kEmpty && val != 0.f) atomicAdd(gd + k, val);
+      return;
+    }
+    old = prev;
+  }
+  atomicAdd(gd + v, g);
+}
+
+__device__ __forceinline__ void cache_drain(unsigned long long *cache, float *__restrict gd) {
+  for (int i = threadIdx.x; i < kCacheSlots; i += blockDim.x) {
+    const unsigned long long e = cache[i];
+    const uint32_t k = (uint32_t)e;
+    const float val = __uint_as_float((uint32_t)(e >> 32));
+    if (k != kEmpty && val != 0.f) atomicAdd(gd + k, val);
+    cache[i] = (unsigned long long)kEmpty;
+  }
+}
+
+// Density-gradient accumulator for the 8 corner voxels of the current
+// trilinear cell. When the walk moves to the next cell, corners shared with
+// it are carried over (summed into the new cell's slots) and only the others
+// are flushed with device atomics, so each voxel is written about once per
+// ray instead of once per cell.
 struct CellAcc {
-  size_t base;
-  int dx, dy, dz;
+  uint32_t v[8];  // absolute voxel index of each corner
   float g[8];
   bool live;
 };
 
-__device__ __forceinline__ void cell_flush(CellAcc &c, float *__restrict gd, size_t sy, size_t sz) {
+__device__ __forceinline__ void cell_corners(const Tri &s, size_t sy, size_t sz, uint32_t *v) {
+  const uint32_t b = (uint32_t)s.base, ox = (uint32_t)s.dx, oy = (uint32_t)(s.dy * sy),
+                 oz = (uint32_t)(s.dz * sz);
+  v[0] = b; v[1] = b + ox; v[2] = b + oy; v[3] = b + ox + oy;
+  v[4] = b + oz; v[5] = b + ox + oz; v[6] = b + oy + oz; v[7] = b + ox + oy + oz;
+}
+
+__device__ __forceinline__ void cell_flush(CellAcc &c, unsigned long long *cache, float *__restrict gd) {
   if (!c.live) return;
-  const size_t ox = c.dx, oy = c.dy * sy, oz = c.dz * sz;
-  const size_t off[8] = {0, ox, oy, ox + oy, oz, ox + oz, oy + oz, ox + oy + oz};
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    if (c.g[k] != 0.f) atomicAdd(gd + c.base + off[k], c.g[k]);
+    if (c.g[k] != 0.f) grad_add(cache, gd, c.v[k], c.g[k]);
   c.live = false;
+}
+
+// Switches the accumulator to the cell with corners nv, carrying shared
+// corners over and flushing the rest.
+__device__ __forceinline__ void cell_move(CellAcc &c, const uint32_t *nv, unsigned long long *cache,
+                                          float *__restrict gd) {
+  float ng[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ng[j] = 0.f;
+  if (c.live) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      bool kept = false;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool m = !kept && nv[j] == c.v[k];
+        ng[j] += m ? c.g[k] : 0.f;
+        kept = kept || m;
+      }
+      if (!kept && c.g[k] != 0.f) grad_add(cache, gd, c.v[k], c.g[k]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    c.v[j] = nv[j];
+    c.g[j] = ng[j];
+  }
+  c.live = true;
+}
+
+// Transfer-function gradient of the current TF bin, merged in registers
+// while consecutive samples fall into the same bin.
+struct TfAcc {
+  int idx;
+  float g[8];
+};
+
+__device__ __forceinline__ void tf_flush(TfAcc &a, float *gtf) {
+  if (a.idx < 0) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (a.g[q] != 0.f) atomicAdd(&gtf[4 * a.idx + q], a.g[q]);
+  a.idx = -1;
 }
 
 template <int K>
@@ -221,99 +308,98 @@ __global__ void __launch_bounds__(256) k_ea_backward(EADD e, const float *__rest
                                                      unsigned long long *__restrict counter) {
   __shared__ float4 lut[256];
   __shared__ float gtf[256 * 4];
+  __shared__ unsigned long long cache[kCacheSlots];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) {
     if (i < e.tf.numValues) lut[i] = e.tf.values[i];
     gtf[4 * i + 0] = gtf[4 * i + 1] = gtf[4 * i + 2] = gtf[4 * i + 3] = 0.f;
   }
+  for (int i = threadIdx.x; i < kCacheSlots; i += blockDim.x) cache[i] = (unsigned long long)kEmpty;
   __syncthreads();
   const size_t sy = (size_t)e.vol.nx, sz = (size_t)e.vol.nx * e.vol.ny;
   const int numTiles = e.numViews * ((e.W + kTile - 1) / kTile) * ((e.H + kTile - 1) / kTile);
   unsigned long long nsamp = 0;
   for (int kk = 0;; ++kk) {
-    const int tile = xcd_tile(blockIdx.x, kk, numTiles);
+    const int tile = xcd_tile(blockIdx.x, kk, numTiles);  // uniform over the workgroup
     if (tile < 0) break;
+    do {  // per-ray work; 'break' skips to the tile's drain barrier
     int view, ix, iy;
-    if (!ea_tile(e, tile, view, ix, iy)) continue;
+    if (!ea_tile(e, tile, view, ix, iy)) break;
     const long long ray = ((long long)view * e.H + iy) * e.W + ix;
     vec3f org, dir;
     float t0;
     int n;
-    if (!ea_setup(e, ray, ix, iy, view, org, dir, t0, n)) continue;
+    if (!ea_setup(e, ray, ix, iy, view, org, dir, t0, n)) break;
     const float gr = gimg[3 * ray + 0], gg = gimg[3 * ray + 1], gb = gimg[3 * ray + 2];
-    if (gr == 0.f && gg == 0.f && gb == 0.f) continue;
+    if (gr == 0.f && gg == 0.f && gb == 0.f) break;
     nsamp += (unsigned long long)n;
     float Sr = 0.f, Sg = 0.f, Sb = 0.f;  // suffix radiance behind the current sample
     CellAcc cell;
     cell.live = false;
     const int nseg = (n + K - 1) / K;
+    TfAcc tfa;
+    tfa.idx = -1;
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
       const int i0 = sgi * K;
-      float Ts[K], As[K], Cr[K], Cg[K], Cb[K], Rho[K];
-      bool Vs[K];
+      // re-march the segment from its checkpoint; keep T and the density
+      // of each sample (NaN marks samples outside the grid)
+      float Ts[K], Rho[K];
       float T = ckpt[(long long)sgi * e.numRays + ray];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        Vs[k] = false;
         Ts[k] = T;
-        As[k] = 0.f;
-        Cr[k] = Cg[k] = Cb[k] = Rho[k] = 0.f;
+        Rho[k] = __builtin_nanf("");
         if (i0 + k < n) {
           const float t = t0 + ((float)(i0 + k) + 0.5f) * e.step;
           const Tri s = tri_sample(e.vol, org + t * dir);
           if (s.valid) {
             const TFLook m = tf_look(e.tf, lut, s.value);
             const float alpha = 1.f - expf(-m.s * e.step);
-            Vs[k] = true;
-            As[k] = alpha;
-            Cr[k] = m.r;
-            Cg[k] = m.g;
-            Cb[k] = m.b;
             Rho[k] = s.value;
             T *= (1.f - alpha);
           }
         }
       }
+      // replay in reverse
 #pragma unroll
       for (int k = K - 1; k >= 0; --k) {
-        if (!Vs[k]) continue;
-        const float Ti = Ts[k], a = As[k];
+        const float rho = Rho[k];
+        if (rho != rho) continue;
+        const TFLook m = tf_look(e.tf, lut, rho);
+        const float a = 1.f - expf(-m.s * e.step);
+        const float Ti = Ts[k];
         const float Tn = Ti * (1.f - a);
         const float w = Ti * a;
-        // dL/dc and dL/dsigma
         const float dcr = gr * w, dcg = gg * w, dcb = gb * w;
-        const float dsig = e.step * (Tn * (gr * Cr[k] + gg * Cg[k] + gb * Cb[k]) - (gr * Sr + gg * Sg + gb * Sb));
-        Sr += w * Cr[k];
-        Sg += w * Cg[k];
-        Sb += w * Cb[k];
-        // route through the TF lookup and the trilinear sample
-        const float t = t0 + ((float)(i0 + k) + 0.5f) * e.step;
-        const vec3f P = org + t * dir;
-        const Tri s = tri_locate(e.vol, P);
-        const TFLook m = tf_look(e.tf, lut, Rho[k]);
+        const float dsig = e.step * (Tn * (gr * m.r + gg * m.g + gb * m.b) - (gr * Sr + gg * Sg + gb * Sb));
+        Sr += w * m.r;
+        Sg += w * m.g;
+        Sb += w * m.b;
         const float f = m.f;
-        const int idx = m.idx;
         const float dsw = dsig * e.tf.baseDensity;  // d/d(w entries)
-        atomicAdd(&gtf[4 * idx + 0], dcr * (1.f - f));
-        atomicAdd(&gtf[4 * idx + 1], dcg * (1.f - f));
-        atomicAdd(&gtf[4 * idx + 2], dcb * (1.f - f));
-        atomicAdd(&gtf[4 * idx + 3], dsw * (1.f - f));
-        atomicAdd(&gtf[4 * idx + 4], dcr * f);
-        atomicAdd(&gtf[4 * idx + 5], dcg * f);
-        atomicAdd(&gtf[4 * idx + 6], dcb * f);
-        atomicAdd(&gtf[4 * idx + 7], dsw * f);
+        if (m.idx != tfa.idx) {
+          tf_flush(tfa, gtf);
+          tfa.idx = m.idx;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) tfa.g[q] = 0.f;
+        }
+        tfa.g[0] += dcr * (1.f - f);
+        tfa.g[1] += dcg * (1.f - f);
+        tfa.g[2] += dcb * (1.f - f);
+        tfa.g[3] += dsw * (1.f - f);
+        tfa.g[4] += dcr * f;
+        tfa.g[5] += dcg * f;
+        tfa.g[6] += dcb * f;
+        tfa.g[7] += dsw * f;
         if (m.dfds != 0.f) {
-          const float4 va = lut[idx], vb = lut[idx + 1];
+          const float4 va = lut[m.idx], vb = lut[m.idx + 1];
           const float drho = m.dfds * (dcr * (vb.x - va.x) + dcg * (vb.y - va.y) + dcb * (vb.z - va.z) +
                                        dsw * (vb.w - va.w));
           if (drho != 0.f) {
-            if (!cell.live || cell.base != s.base || cell.dx != s.dx || cell.dy != s.dy || cell.dz != s.dz) {
-              cell_flush(cell, gdens, sy, sz);
-              cell.base = s.base;
-              cell.dx = s.dx; cell.dy = s.dy; cell.dz = s.dz;
-#pragma unroll
-              for (int q = 0; q < 8; ++q) cell.g[q] = 0.f;
-              cell.live = true;
-            }
+            const float t = t0 + ((float)(i0 + k) + 0.5f) * e.step;
+            const Tri s = tri_locate(e.vol, org + t * dir);
+            uint32_t nv[8];
+            cell_corners(s, sy, sz, nv);
+            if (!cell.live || nv[0] != cell.v[0] || nv[7] != cell.v[7]) cell_move(cell, nv, cache, gdens);
             const float ax = 1.f - s.fx, ay = 1.f - s.fy, az = 1.f - s.fz;
             cell.g[0] += drho * (ax * ay * az);
             cell.g[1] += drho * (s.fx * ay * az);
@@ -327,7 +413,12 @@ __global__ void __launch_bounds__(256) k_ea_backward(EADD e, const float *__rest
         }
       }
     }
-    cell_flush(cell, gdens, sy, sz);
+    tf_flush(tfa, gtf);
+    cell_flush(cell, cache, gdens);
+    } while (0);
+    __syncthreads();
+    cache_drain(cache, gdens);
+    __syncthreads();
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) tfPartial[(size_t)blockIdx.x * 1024 + i] = gtf[i];
@@ -355,6 +446,8 @@ EADD make_ea(const bnamd_ea_volume *v, const bnamd_ea_views *views) {
   BN_REQUIRE(v->density && v->tf, "bnamd_ea: density and tf are required");
   BN_REQUIRE(v->dims[0] >= 2 && v->dims[1] >= 2 && v->dims[2] >= 2, "bnamd_ea: dims must be >= 2");
   BN_REQUIRE(v->numTF >= 2 && v->numTF <= 256, "bnamd_ea: numTF must be in [2, 256]");
+  BN_REQUIRE((double)v->dims[0] * v->dims[1] * v->dims[2] < 4294967296.0,
+             "bnamd_ea: grids are limited to 2^32 voxels (32-bit voxel indices)");
   BN_REQUIRE(v->domainHi > v->domainLo, "bnamd_ea: empty TF domain");
   BN_REQUIRE(v->step > 0.f, "bnamd_ea: step must be positive");
   BN_REQUIRE(views->width > 0 && views->height > 0 && views->numViews > 0 && views->cameras,

@@ -5,6 +5,9 @@ bnamd_ea_backward (include/bnamd.h): the forward stores sparse transmittance
 checkpoints, the backward replays each checkpoint segment in reverse and
 returns gradients for the density grid and the transfer function. Tensors
 must live on the GPU; the kernels run on torch's current stream.
+
+Import torch (or this module) before the first barney_amd.api call in a
+process: torch has to initialise the HIP runtime that both then share.
 """
 import ctypes as C
 

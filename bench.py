@@ -124,7 +124,7 @@ def ea_bench(steps=3):
     n = w4["density"].shape[0]
     W, H = w4["size"]
     setup = ea.EASetup((n, n, n), w4["cameras"], W, H, step=w4["step"], base_density=w4["base_density"],
-                       checkpoint_every=16)
+                       checkpoint_every=int(os.environ.get("BNAMD_EA_CHECKPOINT", "8")))
     dens = torch.tensor(w4["density"], device="cuda", requires_grad=True)
     tf = torch.tensor(w4["tf_init"], device="cuda", requires_grad=True)
     with torch.no_grad():
@@ -165,6 +165,11 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by image tiles / views, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if not args.no_ea:
+        # torch must bring up the HIP runtime before libbnamd does when both
+        # share the process
+        import torch  # noqa: F401
+        torch.cuda.init()
     from barney_amd import api
     api.init(0)
     if args.copy_calib:
