@@ -213,6 +213,55 @@ def render_surface(world, cam, params, with_ids=False, accum=None):
     return out
 
 
+BATCH_MEGAKERNEL, BATCH_WAVEFRONT = 0, 1
+
+
+class Views:
+    """Several cameras of (possibly) different resolutions, rendered in one
+    batch; pixels of view v are accum[offsets[v]:offsets[v+1]] (row-major)."""
+
+    def __init__(self, cams, widths, heights):
+        n = len(cams)
+        self.cams = (capi.bnamd_camera * n)(*cams)
+        self.widths = (C.c_int * n)(*[int(w) for w in widths])
+        self.heights = (C.c_int * n)(*[int(h) for h in heights])
+        self.c = capi.bnamd_views()
+        self.c.numViews = n
+        self.c.cameras = C.cast(self.cams, C.POINTER(capi.bnamd_camera))
+        self.c.widths = C.cast(self.widths, C.POINTER(C.c_int))
+        self.c.heights = C.cast(self.heights, C.POINTER(C.c_int))
+        sizes = [int(w) * int(h) for w, h in zip(widths, heights)]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.num_pixels = int(self.offsets[-1])
+
+    def image(self, accum_np, v):
+        w, h = self.widths[v], self.heights[v]
+        return accum_np[self.offsets[v]:self.offsets[v + 1]].reshape(h, w, 4)
+
+
+def render_batch(world, views, params, mode=BATCH_WAVEFRONT, accum=None):
+    acc = accum if accum is not None else DeviceArray.empty((views.num_pixels, 4), np.float32)
+    st = capi.bnamd_batch_stats()
+    check(capi.load().bnamd_render_batch(world.handle, C.byref(views.c), C.byref(params), acc.addr, int(mode), None,
+                                         C.byref(st)))
+    return {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays,
+                                    "shadowRays": st.shadowRays, "ms": st.ms, "graphReplays": st.graphReplays}}
+
+
+def batch_autotune(world, views, params, path=None):
+    check(capi.load().bnamd_batch_autotune(world.handle, C.byref(views.c), C.byref(params),
+                                           None if path is None else path.encode()))
+
+
+def pbr(color, metallic=0.0, roughness=0.5, transmission=0.0, ior=1.45):
+    m = capi.bnamd_material()
+    m.type = MAT_PBR
+    m.color[:] = [float(c) for c in color]
+    m.opacity = 1.0
+    m.metallic, m.roughness, m.transmission, m.ior, m.specular = metallic, roughness, transmission, ior, 0.5
+    return m
+
+
 class Volume:
     """Structured scalar field + macrocell grid + transfer function."""
 

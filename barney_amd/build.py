@@ -21,6 +21,7 @@ SOURCES = [
     "render_surface.hip",
     "volume.hip",
     "ea.hip",
+    "render_batch.hip",
 ]
 
 # -packed-fp32-ops: packed FP32 forms are kept out of the generated code.

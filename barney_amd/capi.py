@@ -63,6 +63,16 @@ class bnamd_volume_stats(C.Structure):
                 ("ms", C.c_float)]
 
 
+class bnamd_views(C.Structure):
+    _fields_ = [("numViews", C.c_int), ("cameras", C.POINTER(bnamd_camera)), ("widths", C.POINTER(C.c_int)),
+                ("heights", C.POINTER(C.c_int))]
+
+
+class bnamd_batch_stats(C.Structure):
+    _fields_ = [("raysTraced", C.c_longlong), ("primaryRays", C.c_longlong), ("shadowRays", C.c_longlong),
+                ("ms", C.c_float), ("graphReplays", C.c_int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "bnamd_init": (C.c_int, [C.c_int]),
@@ -95,6 +105,10 @@ SIGNATURES = {
                                       C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
     "bnamd_volume_sample_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                            C.c_void_p]),
+    "bnamd_render_batch": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
+                                     C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
+    "bnamd_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
+                                       C.c_char_p]),
 }
 
 _lib = None

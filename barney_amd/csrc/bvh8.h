@@ -11,6 +11,11 @@
 namespace bnamd {
 
 enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2 };
+// c.w of a PrimRec: type in bits 0-7, flags above.
+// PRIM_SHADOW_TRANSPARENT: a triangle whose material is Glass; barney's
+// triangle anyHit ignores it for shadow rays (Glass::getOpacity returns 0
+// for shadow rays, Triangles.dev.cu:125-133).
+constexpr uint32_t PRIM_SHADOW_TRANSPARENT = 1u << 8;
 
 // One node = one 128-byte line. Child boxes are quantized to 16 bits per
 // plane relative to the node origin with a power-of-two scale per axis.
@@ -225,7 +230,7 @@ BN_D void test_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, ve
   const float4 a = prims[idx].a;
   const float4 b = prims[idx].b;
   const float4 c = prims[idx].c;
-  const uint32_t type = __float_as_uint(c.w);
+  const uint32_t type = __float_as_uint(c.w) & 0xffu;
   const uint32_t flat = __float_as_uint(a.w);
   const vec3f p0 = {a.x, a.y, a.z};
   if (type == PRIM_TRIANGLE) {
@@ -235,6 +240,20 @@ BN_D void test_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, ve
   } else {
     isect_cylinder(org, dir, tmin, tmax, p0, {b.x, b.y, b.z}, c.x, flat, best);
   }
+}
+
+// Occlusion test of one primitive for shadow rays: any accepted hit in the
+// ray's range (shadow-transparent primitives skipped).
+BN_D bool occludes_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, vec3f dir, float tmin,
+                        float tmax) {
+  const float4 c = prims[idx].c;
+  const uint32_t bits = __float_as_uint(c.w);
+  if (bits & PRIM_SHADOW_TRANSPARENT) return false;
+  HitRec h;
+  h.t = tmax;
+  h.flat = 0xffffffffu;
+  test_prim(prims, idx, org, dir, tmin, tmax, h);
+  return h.flat != 0xffffffffu;
 }
 
 // Stack entry: node group = {childBase, (hitbits | imask << 8)}.
@@ -249,9 +268,12 @@ struct TraverseCtx {
 // Closest-hit traversal of the whole world BVH8. Returns false if the stack
 // would have overflowed (the builder reports depth, so this does not happen
 // when the kernel is launched with that depth).
-BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims,
-                           vec3f org, vec3f dir, float tmin, float tmax, HitRec &best,
-                           const TraverseCtx &ctx) {
+// kAny: stop at the first occluding primitive (shadow rays; returns true
+// when occluded). Otherwise: closest hit into 'best' (returns false only on
+// stack overflow).
+template <bool kAny>
+BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
+                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx) {
   best.t = tmax;
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
@@ -271,7 +293,7 @@ BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__rest
       const uint32_t imask = grpBits >> 8;
       const uint32_t nodeIdx = grpBase + __builtin_popcount(imask & ((1u << slot) - 1u));
       if ((grpBits & 0xffu) != 0) {
-        if (sp >= ctx.maxDepth) return false;
+        if (sp >= ctx.maxDepth) return kAny;
         ctx.stack[sp * ctx.stride] = make_uint2(grpBase, grpBits);
         ++sp;
       }
@@ -327,7 +349,11 @@ BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__rest
     while (primMask) {
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
-      test_prim(prims, primBase + i, org, dir, tmin, tmax, best);
+      if (kAny) {
+        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) return true;
+      } else {
+        test_prim(prims, primBase + i, org, dir, tmin, tmax, best);
+      }
     }
     if ((grpBits & 0xffu) == 0) {
       if (sp == 0) break;
@@ -337,7 +363,18 @@ BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__rest
       grpBits = e.y;
     }
   }
-  return true;
+  return !kAny;
+}
+
+BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
+                           float tmin, float tmax, HitRec &best, const TraverseCtx &ctx) {
+  return traverse<false>(nodes, prims, org, dir, tmin, tmax, best, ctx);
+}
+
+BN_D bool traverse_occluded(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org,
+                            vec3f dir, float tmin, float tmax, const TraverseCtx &ctx) {
+  HitRec h;
+  return traverse<true>(nodes, prims, org, dir, tmin, tmax, h, ctx);
 }
 
 

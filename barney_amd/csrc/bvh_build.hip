@@ -24,8 +24,8 @@ struct BinNode {
   int right;
 };
 
-__global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, uint32_t numPrims,
-                         PrimRec *__restrict recs, float4 *__restrict boxes) {
+__global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, const MaterialDev *__restrict mats,
+                         uint32_t numPrims, PrimRec *__restrict recs, float4 *__restrict boxes) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= numPrims) return;
   const int g = find_geom(geoms, numGeoms, i);
@@ -41,7 +41,9 @@ __global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, uint32_t
     const vec3f e1 = v1 - v0, e2 = v2 - v0;
     r.a = make_float4(v0.x, v0.y, v0.z, __uint_as_float(i));
     r.b = make_float4(e1.x, e1.y, e1.z, 0.f);
-    r.c = make_float4(e2.x, e2.y, e2.z, __uint_as_float(PRIM_TRIANGLE));
+    const MaterialDev &m = mats[gd.materialID];
+    const bool glass = m.type == BNAMD_MAT_PBR && m.ior != 1.f && m.transmission >= 1e-3f;
+    r.c = make_float4(e2.x, e2.y, e2.z, __uint_as_float(PRIM_TRIANGLE | (glass ? PRIM_SHADOW_TRANSPARENT : 0u)));
     lo = vmin(v0, vmin(v1, v2));
     hi = vmax(v0, vmax(v1, v2));
   } else if (gd.kind == BNAMD_GEOM_SPHERES) {
@@ -434,7 +436,7 @@ void validate_bvh(const World &w) {
     uint32_t flat, type;
     memcpy(&flat, &prims[p].a.w, 4);
     memcpy(&type, &prims[p].c.w, 4);
-    BN_REQUIRE(flat < w.numPrims && type <= 2u, "BVH8 check: corrupt primitive record");
+    BN_REQUIRE(flat < w.numPrims && (type & 0xffu) <= 2u, "BVH8 check: corrupt primitive record");
   }
 }
 
@@ -457,7 +459,7 @@ void build_world_bvh(World &w, hipStream_t stream) {
   }
   DevBuf<PrimRec> recs(n);
   DevBuf<float4> boxes(2ull * n);
-  k_gather<<<div_up(n, 256), 256, 0, stream>>>(w.geoms.ptr, numGeoms, n, recs.ptr, boxes.ptr);
+  k_gather<<<div_up(n, 256), 256, 0, stream>>>(w.geoms.ptr, numGeoms, w.materials.ptr, n, recs.ptr, boxes.ptr);
   BN_HIP(hipGetLastError());
 
   DevBuf<uint32_t> bounds(12);

@@ -1,0 +1,734 @@
+// Batched multi-view path tracing: a per-pixel megakernel and a wavefront
+// renderer over the same per-path computation (camera_ray + traverse +
+// bounce, see shade.h), so both produce the same per-pixel values up to the
+// order of floating-point accumulation.
+//
+// Wavefront organisation, per bounce d (all views and paths at once):
+//   trace   persistent workgroups pull 64-ray chunks from an atomic counter
+//           and write one 16-byte hit record per ray;
+//   sort    a counting sort of ray indices by BSDF type (miss, Lambert,
+//           NVisii, Glass) so each wavefront of the shade kernel runs one
+//           material model;
+//   shade   barney's bounce(): fragments are added to the pixel with float
+//           atomics, surviving paths are appended (compacted) to the next
+//           queue, shadow rays to the shadow queue;
+//   shadow  persistent occlusion queries; unoccluded contributions added.
+// The whole loop, including counter resets (fill kernels, not memsets), is
+// captured once into a hipGraph and replayed per batch of paths.
+#include "shade.h"
+#include <chrono>
+#include <cstdio>
+#include <string>
+
+namespace bnamd {
+
+size_t trace_lds_bytes(int depth, int block);
+const World &world_of(const bnamd_world *w);
+
+namespace {
+
+constexpr int kMaxViews = 4096;
+
+struct ViewsDD {
+  int numViews;
+  const int *pixOffset;   // [numViews + 1]
+  const int *tileOffset;  // [numViews + 1] (megakernel tiles)
+  const int2 *size;       // [numViews]
+  const CameraDD *cams;   // [numViews]
+};
+
+__device__ __forceinline__ int find_view(const int *off, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ ShadeHit shade_hit(const WorldDev &w, vec3f org, vec3f dir, float tmax, const HitRec &h) {
+  ShadeHit sh;
+  if (h.flat == 0xffffffffu) {
+    sh.hit = false;
+    return sh;
+  }
+  const SurfaceHit s = surface_epilogue(w, org, dir, 0.f, tmax, h);
+  const GeomDev &g = w.geoms[s.geom];
+  const MaterialDev &m = w.materials[g.materialID];
+  vec3f color = {m.color[0], m.color[1], m.color[2]};
+  if (g.colors) color = load3(g.colors + 3 * s.primID);
+  sh.hit = true;
+  sh.P = s.P;
+  sh.N = s.Ng;
+  sh.t = h.t;
+  sh.bsdf = material_bsdf(m, color);
+  return sh;
+}
+
+__device__ __forceinline__ void add_frag(vec3f &sum, vec3f frag, float clampMax) {
+  frag = vmin(frag, clampMax);
+  if (frag.x > 0.f) sum.x += frag.x;
+  if (frag.y > 0.f) sum.y += frag.y;
+  if (frag.z > 0.f) sum.z += frag.z;
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long v, unsigned long long *dst) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// ===========================================================================
+// Megakernel: L lanes per pixel (each looping over its share of the paths),
+// 4 wavefronts per workgroup covering a TW x TH pixel tile of one view.
+// ===========================================================================
+template <int L>
+__global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, RenderDD rp, int spp,
+                                                    float4 *__restrict accum,
+                                                    unsigned long long *__restrict counters) {
+  extern __shared__ uint2 lds_stack[];
+  constexpr int PW = 64 / L, PB = 4 * PW;
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  // XCD-aware: each XCD walks a contiguous range of tiles
+  const int numTiles = vd.tileOffset[vd.numViews];
+  const int per = (numTiles + 7) >> 3;
+  const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if ((blockIdx.x >> 3) >= per || tile >= numTiles) return;
+  const int view = find_view(vd.tileOffset, vd.numViews, tile);
+  const int t = tile - vd.tileOffset[view];
+  const int2 sz = vd.size[view];
+  const int tilesX = (sz.x + TW - 1) / TW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane & (L - 1);
+  const int q = wave * PW + lane / L;
+  const int ix = (t % tilesX) * TW + (q % TW);
+  const int iy = (t / tilesX) * TH + (q / TW);
+  const bool active = ix < sz.x && iy < sz.y;
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
+  unsigned long long nrays = 0, nshadow = 0;
+  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
+  const CameraDD cam = vd.cams[view];
+  if (active) {
+    for (int p = sub; p < spp; p += L) {
+      RenderDD rpp = rp;
+      rpp.width = sz.x;
+      rpp.height = sz.y;
+      rpp.accumID = rp.accumID + p;
+      PathRay ray = camera_ray(cam, rpp, ix, iy);
+      const float clampMax = 10.f * (1 + rpp.accumID);
+      vec3f sum = vec3f(0.f);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const vec3f dir = fix_dir(ray.dir);
+        HitRec h;
+        traverse_closest(w.nodes, w.prims, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        ++nrays;
+        const ShadeHit sh = shade_hit(w, ray.org, dir, ray.tMax, h);
+        if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
+        PathRay shadow;
+        add_frag(sum, bounce(rpp, ray, sh, shadow, gen), clampMax);
+        if (shadow.tMax > 0.f) {
+          ShadeHit shs;
+          shs.hit = traverse_occluded(w.nodes, w.prims, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, ctx);
+          ++nshadow;
+          PathRay dummy;
+          add_frag(sum, bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
+        }
+      }
+      sr += sum.x;
+      sg += sum.y;
+      sb += sum.z;
+      sa += alpha;
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    sg += __shfl_xor(sg, off);
+    sb += __shfl_xor(sb, off);
+    sa += __shfl_xor(sa, off);
+  }
+  if (active && sub == 0) {
+    const int pix = vd.pixOffset[view] + ix + iy * sz.x;
+    if (rp.accumID == 0) {
+      accum[pix] = make_float4(sr, sg, sb, sa);
+    } else {
+      float4 o = accum[pix];
+      o.x += sr; o.y += sg; o.z += sb; o.w += sa;
+      accum[pix] = o;
+    }
+  }
+  wave_add(nrays, counters + 0);
+  wave_add(nshadow, counters + 1);
+}
+
+// ===========================================================================
+// Wavefront
+// ===========================================================================
+struct PathRec {
+  float4 a;  // org, tMax
+  float4 b;  // dir, mis
+  float4 c;  // throughput, path id (uint bits)
+  uint4 d;   // rng lo, rng hi, flags (numDiffuse | medium << 8), 0
+};
+struct ShadowRec {
+  float4 a;  // org, tMax
+  float4 b;  // dir, path id
+  float4 c;  // contribution (mis * throughput), 0
+};
+
+// device counters (one slot per bounce depth)
+struct WfCounters {
+  unsigned int count[32];     // paths in the queue of depth d
+  unsigned int shadow[32];    // shadow rays spawned at depth d
+  unsigned int work[32];      // persistent trace work counter at depth d
+  unsigned int swork[32];     // persistent shadow work counter at depth d
+  unsigned int bins[32][4];   // sort histogram
+  unsigned int cursor[32][4]; // sort scatter cursors
+  unsigned long long rays, shadowRays;
+};
+
+struct WfDD {
+  WorldDev w;
+  ViewsDD vd;
+  RenderDD rp;
+  int numPixels;  // all views
+  int sppBatch;   // paths per pixel in one batch
+  int accum0;     // accumID of the first path of the batch
+  PathRec *q[2];
+  HitRec *hits;
+  unsigned int *keys;
+  unsigned int *perm;
+  ShadowRec *sq;
+  float4 *accum;
+  WfCounters *cnt;
+  int chunk;      // rays per persistent work fetch (per wavefront)
+};
+
+// path id = s * numPixels + global pixel
+__device__ __forceinline__ void path_pixel(const WfDD &e, unsigned pid, int &s, int &view, int &ix, int &iy,
+                                           int &pix) {
+  s = (int)(pid / (unsigned)e.numPixels);
+  pix = (int)(pid % (unsigned)e.numPixels);
+  view = find_view(e.vd.pixOffset, e.vd.numViews, pix);
+  const int local = pix - e.vd.pixOffset[view];
+  const int w = e.vd.size[view].x;
+  ix = local % w;
+  iy = local / w;
+}
+
+__global__ void k_wf_reset(WfCounters *c) {
+  unsigned int *p = (unsigned int *)c;
+  const int n = (int)(sizeof(WfCounters) / sizeof(unsigned int));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_wf_generate(WfDD e) {
+  const unsigned total = (unsigned)e.numPixels * (unsigned)e.sppBatch;
+  const unsigned pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= total) return;
+  int s, view, ix, iy, pix;
+  path_pixel(e, pid, s, view, ix, iy, pix);
+  RenderDD rpp = e.rp;
+  rpp.width = e.vd.size[view].x;
+  rpp.height = e.vd.size[view].y;
+  rpp.accumID = e.accum0 + s;
+  const PathRay r = camera_ray(e.vd.cams[view], rpp, ix, iy);
+  PathRec rec;
+  rec.a = make_float4(r.org.x, r.org.y, r.org.z, r.tMax);
+  rec.b = make_float4(r.dir.x, r.dir.y, r.dir.z, r.mis);
+  rec.c = make_float4(r.tp.x, r.tp.y, r.tp.z, __uint_as_float(pid));
+  rec.d = make_uint4((uint32_t)r.rng.state, (uint32_t)(r.rng.state >> 32), 0u, 0u);
+  e.q[0][pid] = rec;
+  if (pid == 0) e.cnt->count[0] = total;
+}
+
+// Persistent trace: each wavefront fetches 'chunk' rays at a time.
+__global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
+  extern __shared__ uint2 lds_stack[];
+  const unsigned n = e.cnt->count[depth];
+  const PathRec *__restrict q = e.q[depth & 1];
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
+  const int lane = threadIdx.x & 63;
+  unsigned long long nr = 0;
+  while (true) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&e.cnt->work[depth], (unsigned)e.chunk);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    for (unsigned i = base + lane; i < base + e.chunk && i < n; i += 64) {
+      const float4 a = q[i].a, b = q[i].b;
+      const vec3f org = {a.x, a.y, a.z};
+      const vec3f dir = fix_dir({b.x, b.y, b.z});
+      HitRec h;
+      traverse_closest(e.w.nodes, e.w.prims, org, dir, 0.f, a.w, h, ctx);
+      e.hits[i] = h;
+      ++nr;
+    }
+  }
+  wave_add(nr, &e.cnt->rays);
+}
+
+__device__ __forceinline__ unsigned bsdf_key(const WfDD &e, const HitRec &h) {
+  if (h.flat == 0xffffffffu) return 0u;
+  const int g = find_geom(e.w.geoms, e.w.numGeoms, h.flat);
+  const MaterialDev &m = e.w.materials[e.w.geoms[g].materialID];
+  if (m.type != BNAMD_MAT_PBR) return 1u;
+  return (m.ior != 1.f && m.transmission >= 1e-3f) ? 3u : 2u;
+}
+
+__global__ void k_wf_sort_count(WfDD e, int depth) {
+  __shared__ unsigned int hist[4];
+  if (threadIdx.x < 4) hist[threadIdx.x] = 0u;
+  __syncthreads();
+  const unsigned n = e.cnt->count[depth];
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned k = bsdf_key(e, e.hits[i]);
+    e.keys[i] = k;
+    atomicAdd(&hist[k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && hist[threadIdx.x]) atomicAdd(&e.cnt->bins[depth][threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void k_wf_sort_scatter(WfDD e, int depth) {
+  const unsigned n = e.cnt->count[depth];
+  __shared__ unsigned int start[4];
+  if (threadIdx.x == 0) {
+    unsigned acc = 0;
+    for (int k = 0; k < 4; ++k) {
+      start[k] = acc;
+      acc += e.cnt->bins[depth][k];
+    }
+  }
+  __syncthreads();
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned k = e.keys[i];
+    const unsigned pos = start[k] + atomicAdd(&e.cnt->cursor[depth][k], 1u);
+    e.perm[pos] = i;
+  }
+}
+
+__device__ __forceinline__ void accum_add(float4 *accum, int pix, vec3f f, float clampMax) {
+  f = vmin(f, clampMax);
+  float *p = (float *)(accum + pix);
+  if (f.x > 0.f) atomicAdd(p + 0, f.x);
+  if (f.y > 0.f) atomicAdd(p + 1, f.y);
+  if (f.z > 0.f) atomicAdd(p + 2, f.z);
+}
+
+__global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
+  const unsigned n = e.cnt->count[depth];
+  const PathRec *__restrict qin = e.q[depth & 1];
+  PathRec *__restrict qout = e.q[(depth + 1) & 1];
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned r = e.perm[i];
+    const PathRec rec = qin[r];
+    const HitRec h = e.hits[r];
+    const unsigned pid = __float_as_uint(rec.c.w);
+    int s, view, ix, iy, pix;
+    path_pixel(e, pid, s, view, ix, iy, pix);
+    RenderDD rpp = e.rp;
+    rpp.width = e.vd.size[view].x;
+    rpp.height = e.vd.size[view].y;
+    rpp.accumID = e.accum0 + s;
+    PathRay ray;
+    ray.org = {rec.a.x, rec.a.y, rec.a.z};
+    ray.tMax = rec.a.w;
+    ray.dir = {rec.b.x, rec.b.y, rec.b.z};
+    ray.mis = rec.b.w;
+    ray.tp = {rec.c.x, rec.c.y, rec.c.z};
+    ray.rng.state = (uint64_t)rec.d.x | ((uint64_t)rec.d.y << 32);
+    ray.numDiffuse = (int)(rec.d.z & 0xffu);
+    ray.isInMedium = (rec.d.z >> 8) & 1u;
+    ray.isShadow = false;
+    ray.missColor = vec3f(0.f);
+    ray.missAlpha = 0.f;
+    if (depth == 0) {
+      // generateRays' background for primary rays (generateRays.cu:199-210)
+      const float t = (iy + .5f) / float(rpp.height);
+      if (rpp.bg[3] >= 0.f) {
+        ray.missColor = {rpp.bg[0], rpp.bg[1], rpp.bg[2]};
+        ray.missAlpha = rpp.bg[3];
+      } else {
+        ray.missColor = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
+        ray.missAlpha = (1.0f - t) * 1.f + t * 1.f;
+      }
+    }
+    const vec3f dir = fix_dir(ray.dir);
+    const ShadeHit sh = shade_hit(e.w, ray.org, dir, ray.tMax, h);
+    const float clampMax = 10.f * (1 + rpp.accumID);
+    if (depth == 0) {
+      const float alpha = sh.hit ? 1.f : ray.missAlpha;
+      if (alpha > 0.f) atomicAdd(((float *)(e.accum + pix)) + 3, alpha);
+    }
+    PathRay shadow;
+    const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
+    accum_add(e.accum, pix, frag, clampMax);
+    if (shadow.tMax > 0.f) {
+      const unsigned si = atomicAdd(&e.cnt->shadow[depth], 1u);
+      ShadowRec srec;
+      const vec3f c = shadow.mis * shadow.tp;  // fragment if unoccluded
+      srec.a = make_float4(shadow.org.x, shadow.org.y, shadow.org.z, shadow.tMax);
+      srec.b = make_float4(shadow.dir.x, shadow.dir.y, shadow.dir.z, __uint_as_float(pid));
+      srec.c = make_float4(c.x, c.y, c.z, 0.f);
+      e.sq[si] = srec;
+    }
+    if (ray.tMax > 0.f) {
+      const unsigned o = atomicAdd(&e.cnt->count[depth + 1], 1u);
+      PathRec out;
+      out.a = make_float4(ray.org.x, ray.org.y, ray.org.z, ray.tMax);
+      out.b = make_float4(ray.dir.x, ray.dir.y, ray.dir.z, ray.mis);
+      out.c = make_float4(ray.tp.x, ray.tp.y, ray.tp.z, __uint_as_float(pid));
+      out.d = make_uint4((uint32_t)ray.rng.state, (uint32_t)(ray.rng.state >> 32),
+                         (uint32_t)ray.numDiffuse | ((ray.isInMedium ? 1u : 0u) << 8), 0u);
+      qout[o] = out;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
+  extern __shared__ uint2 lds_stack[];
+  const unsigned n = e.cnt->shadow[depth];
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
+  const int lane = threadIdx.x & 63;
+  unsigned long long nr = 0;
+  while (true) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&e.cnt->swork[depth], (unsigned)e.chunk);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    for (unsigned i = base + lane; i < base + e.chunk && i < n; i += 64) {
+      const ShadowRec s = e.sq[i];
+      const vec3f org = {s.a.x, s.a.y, s.a.z};
+      const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
+      const bool occluded = traverse_occluded(e.w.nodes, e.w.prims, org, dir, 0.f, s.a.w, ctx);
+      ++nr;
+      if (!occluded) {
+        const unsigned pid = __float_as_uint(s.b.w);
+        const int sidx = (int)(pid / (unsigned)e.numPixels);
+        const int pix = (int)(pid % (unsigned)e.numPixels);
+        accum_add(e.accum, pix, {s.c.x, s.c.y, s.c.z}, 10.f * (1 + e.accum0 + sidx));
+      }
+    }
+  }
+  wave_add(nr, &e.cnt->shadowRays);
+}
+
+__global__ void k_fill_f4(float4 *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct TuneChoice {
+  int traceBlocksPerCU = 2;  // persistent trace/shadow grid
+  int chunk = 64;            // rays per work fetch
+  int sppBatch = 0;          // 0 = all paths of the frame in one batch
+};
+
+struct BatchState {
+  DevBuf<int> pixOffset, tileOffset;
+  DevBuf<int2> sizes;
+  DevBuf<CameraDD> cams;
+  DevBuf<PathRec> q0, q1;
+  DevBuf<HitRec> hits;
+  DevBuf<unsigned> keys, perm;
+  DevBuf<ShadowRec> sq;
+  DevBuf<WfCounters> cnt;
+  DevBuf<unsigned long long> megaCnt;
+  TuneChoice tune;
+  bool tuned = false;
+  // graph cache
+  hipGraphExec_t exec = nullptr;
+  std::string graphKey;
+  ~BatchState() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+BatchState &state() {
+  static BatchState s;
+  return s;
+}
+
+int num_cus() {
+  static int n = 0;
+  if (!n) {
+    int dev = 0;
+    BN_HIP(hipGetDevice(&dev));
+    BN_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return n;
+}
+
+struct HostViews {
+  std::vector<int> pixOffset, tileOffset;
+  std::vector<int2> sizes;
+  std::vector<CameraDD> cams;
+  int numPixels = 0;
+};
+
+HostViews make_views(const bnamd_views *v, int TW, int TH) {
+  BN_REQUIRE(v && v->numViews > 0 && v->numViews <= kMaxViews && v->cameras && v->widths && v->heights,
+             "bnamd_render_batch: bad views");
+  HostViews h;
+  h.pixOffset.push_back(0);
+  h.tileOffset.push_back(0);
+  long long px = 0, tl = 0;
+  for (int i = 0; i < v->numViews; ++i) {
+    const int w = v->widths[i], hh = v->heights[i];
+    BN_REQUIRE(w > 0 && hh > 0, "bnamd_render_batch: empty view");
+    px += (long long)w * hh;
+    tl += (long long)div_up(w, TW) * div_up(hh, TH);
+    BN_REQUIRE(px < (1ll << 31), "bnamd_render_batch: too many pixels");
+    h.pixOffset.push_back((int)px);
+    h.tileOffset.push_back((int)tl);
+    h.sizes.push_back(make_int2(w, hh));
+    h.cams.push_back(make_camera(v->cameras[i]));
+  }
+  h.numPixels = (int)px;
+  return h;
+}
+
+ViewsDD upload_views(BatchState &st, const HostViews &h, hipStream_t s) {
+  st.pixOffset.upload(h.pixOffset.data(), h.pixOffset.size(), s);
+  st.tileOffset.upload(h.tileOffset.data(), h.tileOffset.size(), s);
+  st.sizes.upload(h.sizes.data(), h.sizes.size(), s);
+  st.cams.upload(h.cams.data(), h.cams.size(), s);
+  ViewsDD vd;
+  vd.numViews = (int)h.sizes.size();
+  vd.pixOffset = st.pixOffset.ptr;
+  vd.tileOffset = st.tileOffset.ptr;
+  vd.size = st.sizes.ptr;
+  vd.cams = st.cams.ptr;
+  return vd;
+}
+
+template <int L>
+void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, int spp, float4 *accum,
+                 unsigned long long *cnt, hipStream_t s) {
+  constexpr int PB = 4 * (64 / L);
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  BatchState &st = state();
+  const HostViews hv = make_views(views, TW, TH);
+  const ViewsDD vd = upload_views(st, hv, s);
+  const int tiles = hv.tileOffset.back();
+  const int grid = ((tiles + 7) / 8) * 8;
+  const size_t lds = trace_lds_bytes(w.depth, 256);
+  k_batch_mega<L><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
+  BN_HIP(hipGetLastError());
+}
+
+void render_mega(const World &w, const bnamd_views *views, const RenderDD &rp, int spp, float4 *accum,
+                 unsigned long long *cnt, hipStream_t s) {
+  if (spp >= 8) launch_mega<8>(w, views, rp, spp, accum, cnt, s);
+  else if (spp >= 4) launch_mega<4>(w, views, rp, spp, accum, cnt, s);
+  else if (spp >= 2) launch_mega<2>(w, views, rp, spp, accum, cnt, s);
+  else launch_mega<1>(w, views, rp, spp, accum, cnt, s);
+}
+
+// Records the kernels of one batch (generate + maxDepth+1 bounces) on s.
+void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds, hipStream_t s) {
+  const unsigned total = (unsigned)e.numPixels * (unsigned)e.sppBatch;
+  k_wf_reset<<<1, 256, 0, s>>>(e.cnt);
+  k_wf_generate<<<div_up(total, 256), 256, 0, s>>>(e);
+  const int gridStream = std::min(div_up(total, 256), 8 * num_cus());
+  for (int d = 0; d <= maxDepth; ++d) {
+    k_wf_trace<<<gridPersist, 256, lds, s>>>(e, d);
+    k_wf_sort_count<<<gridStream, 256, 0, s>>>(e, d);
+    k_wf_sort_scatter<<<gridStream, 256, 0, s>>>(e, d);
+    k_wf_shade<<<gridStream, 256, 0, s>>>(e, d);
+    k_wf_shadow<<<gridPersist, 256, lds, s>>>(e, d);
+  }
+}
+
+float render_wavefront(const World &w, const bnamd_views *views, const RenderDD &rp0, int spp, float4 *accum,
+                       hipStream_t s, bnamd_batch_stats *stats, const TuneChoice &tune, bool useGraph) {
+  BatchState &st = state();
+  const HostViews hv = make_views(views, 16, 16);
+  const ViewsDD vd = upload_views(st, hv, s);
+  const int maxDepth = std::min(rp0.maxPathDepth, 30);
+  int sppBatch = tune.sppBatch > 0 ? std::min(tune.sppBatch, spp) : spp;
+  // keep the queues within a memory budget (~24 GB)
+  const size_t perPath = 2 * sizeof(PathRec) + sizeof(ShadowRec) + sizeof(HitRec) + 2 * sizeof(unsigned);
+  while (sppBatch > 1 && (size_t)hv.numPixels * sppBatch * perPath > (24ull << 30)) --sppBatch;
+  const size_t cap = (size_t)hv.numPixels * sppBatch;
+  BN_REQUIRE(cap < (1ull << 32), "bnamd_render_batch: too many paths per batch");
+  st.q0.ensure(cap);
+  st.q1.ensure(cap);
+  st.hits.ensure(cap);
+  st.keys.ensure(cap);
+  st.perm.ensure(cap);
+  st.sq.ensure(cap);
+  st.cnt.ensure(1);
+  const size_t lds = trace_lds_bytes(w.depth, 256);
+  BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_batch: BVH too deep for the LDS stack");
+  const int gridPersist = std::max(1, tune.traceBlocksPerCU) * num_cus();
+  if (rp0.accumID == 0)
+    k_fill_f4<<<std::min(div_up(hv.numPixels, 256), 4096), 256, 0, s>>>(accum, (size_t)hv.numPixels);
+  hipEvent_t e0, e1;
+  BN_HIP(hipEventCreate(&e0));
+  BN_HIP(hipEventCreate(&e1));
+  BN_HIP(hipEventRecord(e0, s));
+  unsigned long long rays = 0, shadows = 0;
+  int replays = 0;
+  for (int p0 = 0; p0 < spp; p0 += sppBatch) {
+    WfDD e;
+    e.w = w.dd();
+    e.vd = vd;
+    e.rp = rp0;
+    e.numPixels = hv.numPixels;
+    e.sppBatch = std::min(sppBatch, spp - p0);
+    e.accum0 = rp0.accumID + p0;
+    e.q[0] = st.q0.ptr;
+    e.q[1] = st.q1.ptr;
+    e.hits = st.hits.ptr;
+    e.keys = st.keys.ptr;
+    e.perm = st.perm.ptr;
+    e.sq = st.sq.ptr;
+    e.accum = accum;
+    e.cnt = st.cnt.ptr;
+    e.chunk = tune.chunk;
+    if (useGraph) {
+      // kernel arguments (accum0, buffers) are baked into the graph, so the
+      // cache key covers them
+      char key[512];
+      snprintf(key, sizeof key, "%p %p %d %d %d %d %d %d %p %d %d %zu", (void *)e.w.nodes, (void *)accum,
+               e.numPixels, e.sppBatch, e.accum0, maxDepth, gridPersist, e.chunk, (void *)st.cams.ptr,
+               rp0.rrDepth, rp0.maxDiffuseBounces, st.q0.count);
+      if (!st.exec || st.graphKey != key) {
+        if (st.exec) {
+          (void)hipGraphExecDestroy(st.exec);
+          st.exec = nullptr;
+        }
+        hipStream_t cs;
+        BN_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t g;
+        BN_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        enqueue_wavefront(e, maxDepth, gridPersist, lds, cs);
+        BN_HIP(hipStreamEndCapture(cs, &g));
+        BN_HIP(hipGraphInstantiate(&st.exec, g, nullptr, nullptr, 0));
+        BN_HIP(hipGraphDestroy(g));
+        BN_HIP(hipStreamDestroy(cs));
+        st.graphKey = key;
+      }
+      BN_HIP(hipGraphLaunch(st.exec, s));
+      ++replays;
+    } else {
+      enqueue_wavefront(e, maxDepth, gridPersist, lds, s);
+    }
+    WfCounters hc;
+    BN_HIP(hipMemcpyAsync(&hc, st.cnt.ptr, sizeof(hc), hipMemcpyDeviceToHost, s));
+    BN_HIP(hipStreamSynchronize(s));
+    rays += hc.rays;
+    shadows += hc.shadowRays;
+  }
+  BN_HIP(hipEventRecord(e1, s));
+  BN_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    stats->raysTraced = (long long)(rays + shadows);
+    stats->primaryRays = (long long)hv.numPixels * spp;
+    stats->shadowRays = (long long)shadows;
+    stats->ms = ms;
+    stats->graphReplays = replays;
+  }
+  return ms;
+}
+
+}  // namespace
+}  // namespace bnamd
+
+using namespace bnamd;
+
+extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views, const bnamd_render_params *params,
+                                  float *accum, int mode, void *stream, bnamd_batch_stats *stats) {
+  return guarded([&] {
+    BN_REQUIRE(world && views && params && accum, "bnamd_render_batch: NULL argument");
+    BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_render_batch: pathsPerPixel must be >= 1");
+    BN_REQUIRE(mode == BNAMD_BATCH_MEGAKERNEL || mode == BNAMD_BATCH_WAVEFRONT, "bnamd_render_batch: bad mode");
+    const World &w = world_of(world);
+    hipStream_t s = (hipStream_t)stream;
+    RenderDD rp = make_render_dd(*params);
+    if (mode == BNAMD_BATCH_MEGAKERNEL) {
+      BatchState &st = state();
+      st.megaCnt.ensure(2);
+      BN_HIP(hipMemsetAsync(st.megaCnt.ptr, 0, 16, s));
+      hipEvent_t e0, e1;
+      BN_HIP(hipEventCreate(&e0));
+      BN_HIP(hipEventCreate(&e1));
+      BN_HIP(hipEventRecord(e0, s));
+      render_mega(w, views, rp, params->pathsPerPixel, (float4 *)accum, st.megaCnt.ptr, s);
+      BN_HIP(hipEventRecord(e1, s));
+      BN_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      unsigned long long h[2];
+      BN_HIP(hipMemcpy(h, st.megaCnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
+      if (stats) {
+        long long px = 0;
+        for (int i = 0; i < views->numViews; ++i) px += (long long)views->widths[i] * views->heights[i];
+        stats->raysTraced = (long long)(h[0] + h[1]);
+        stats->primaryRays = px * params->pathsPerPixel;
+        stats->shadowRays = (long long)h[1];
+        stats->ms = ms;
+        stats->graphReplays = 0;
+      }
+    } else {
+      BatchState &st = state();
+      const char *ng = getenv("BNAMD_NO_GRAPH");
+      render_wavefront(w, views, rp, params->pathsPerPixel, (float4 *)accum, s, stats, st.tune,
+                       !(ng && ng[0] == '1'));
+    }
+  });
+}
+
+extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views *views,
+                                    const bnamd_render_params *params, const char *path) {
+  return guarded([&] {
+    BN_REQUIRE(world && views && params, "bnamd_batch_autotune: NULL argument");
+    const World &w = world_of(world);
+    BatchState &st = state();
+    RenderDD rp = make_render_dd(*params);
+    rp.accumID = 0;
+    long long px = 0;
+    for (int i = 0; i < views->numViews; ++i) px += (long long)views->widths[i] * views->heights[i];
+    DevBuf<float4> accum((size_t)px);
+    const int blocks[] = {1, 2, 4, 8};
+    const int chunks[] = {64, 128, 256};
+    TuneChoice best;
+    float bestMs = 1e30f;
+    for (int b : blocks)
+      for (int c : chunks) {
+        TuneChoice t;
+        t.traceBlocksPerCU = b;
+        t.chunk = c;
+        t.sppBatch = params->pathsPerPixel;
+        // one spp per candidate keeps tuning short
+        float ms = render_wavefront(w, views, rp, 1, accum.ptr, 0, nullptr, t, false);
+        if (ms < bestMs) {
+          bestMs = ms;
+          best = t;
+        }
+      }
+    best.sppBatch = 0;
+    st.tune = best;
+    st.tuned = true;
+    if (path) {
+      FILE *f = fopen(path, "w");
+      BN_REQUIRE(f, std::string("bnamd_batch_autotune: cannot write ") + path);
+      fprintf(f, "{\"traceBlocksPerCU\": %d, \"chunk\": %d, \"bestMs1spp\": %.4f}\n", best.traceBlocksPerCU,
+              best.chunk, bestMs);
+      fclose(f);
+    }
+  });
+}

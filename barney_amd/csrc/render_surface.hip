@@ -38,7 +38,7 @@ __device__ __forceinline__ ShadeHit shade_surface(const WorldDev &w, vec3f org, 
   sh.P = s.P;
   sh.N = s.Ng;
   sh.t = h.t;
-  sh.bsdf = matte_bsdf(color);
+  sh.bsdf = material_bsdf(m, color);
   return sh;
 }
 
@@ -99,11 +99,9 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
         add_fragment(sum, frag, clampMax);
         if (shadow.tMax > 0.f) {
           const vec3f sdir = fix_dir(shadow.dir);
-          HitRec hs;
-          traverse_closest(w.nodes, w.prims, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx);
-          ++nrays;
           ShadeHit shs;
-          shs.hit = hs.flat != 0xffffffffu;
+          shs.hit = traverse_occluded(w.nodes, w.prims, shadow.org, sdir, 0.f, shadow.tMax, ctx);
+          ++nrays;
           PathRay dummy;
           frag = bounce(rpp, shadow, shs, dummy, gen + 1);
           add_fragment(sum, frag, clampMax);

@@ -8,6 +8,7 @@
 //   materials        material/AnariMatte.h:51-63
 //   accumulation     shadeRays.cu:970-1038
 #pragma once
+#include <hip/hip_fp16.h>
 #include "world.h"
 
 namespace bnamd {
@@ -53,11 +54,16 @@ inline RenderDD make_render_dd(const bnamd_render_params &p) {
   return r;
 }
 
-enum BsdfType : int { BSDF_NONE = 0, BSDF_LAMBERT = 1, BSDF_PHASE = 2 };
+enum BsdfType : int { BSDF_NONE = 0, BSDF_LAMBERT = 1, BSDF_PHASE = 2, BSDF_NVISII = 3, BSDF_GLASS = 4 };
 
+// Packed BSDF (barney/packedBSDF/PackedBSDF.h:20-138): Lambertian albedo,
+// phase albedo, NVisii base color + parameters, or Glass ior.
 struct Bsdf {
   int type;
-  vec3f albedo;
+  vec3f albedo;     // Lambert/Phase albedo, NVisii base color
+  float metallic;   // NVisii
+  float roughness;  // NVisii
+  float ior;        // NVisii / Glass
 };
 
 // One ray of a path: Ray + PathState of barney/render/Ray.h:18-99 (the
@@ -70,6 +76,7 @@ struct PathRay {
   float mis;
   int numDiffuse;
   bool isShadow;
+  bool isInMedium;
   vec3f missColor;
   float missAlpha;
 };
@@ -109,6 +116,7 @@ BN_HD PathRay camera_ray(const CameraDD &cam, const RenderDD &rp, int ix, int iy
   ray.mis = 0.f;
   ray.numDiffuse = 0;
   ray.isShadow = false;
+  ray.isInMedium = false;
   const float t = (iy + .5f) / float(rp.height);
   if (rp.bg[3] >= 0.f) {
     ray.missColor = {rp.bg[0], rp.bg[1], rp.bg[2]};
@@ -137,14 +145,199 @@ struct EvalRes {
   float pdf;
 };
 
-// Lambertian::eval / Phase::eval (single-scatter phase is isotropic 1/4pi)
-BN_HD EvalRes bsdf_eval(const Bsdf &b, vec3f Ns, vec3f wi) {
+BN_HD bool is_inf(float f) { return f == BN_INF || f == -BN_INF; }
+
+// ---------------------------------------------------------------------------
+// NVisii (Disney-style) BSDF, restating barney/packedBSDF/NVisii.h for the
+// parameters AnariPBR::createBSDF sets (AnariPBR.h:68-94): anisotropy,
+// sheen, clearcoat, flatness and specular transmission at their defaults
+// (0), whose lobes then contribute exactly zero.
+// ---------------------------------------------------------------------------
+namespace nvisii {
+#define BN_NV_MIN_ALPHA .002f
+#define BN_NV_SMALL_EPS 2e-10f
+BN_HD float sqr(float f) { return f * f; }
+BN_HD float lerp_r(float a, float b, float t) { return (1.f - t) * a + t * b; }
+BN_HD vec3f lerp_r(vec3f a, vec3f b, vec3f t) { return (vec3f(1.f) - t) * a + t * b; }
+BN_HD float saturate(float f) { return fmaxf(0.f, fminf(f, 1.f)); }
+BN_HD float schlick_weight(float c) { return powf(saturate(1.f - c), 5.f); }
+BN_HD bool same_hemisphere(vec3f wo, vec3f wi, vec3f n) { return dot(wo, n) * dot(wi, n) > 0.f; }
+// NVisii.h:138-145 (no alpha^2 numerator, single power in the denominator)
+BN_HD float gtr_2(float cos_theta_h, float alpha) {
+  const float a2 = alpha * alpha;
+  const float den1 = 1.f + (a2 - 1.f) * cos_theta_h * cos_theta_h;
+  const float den2 = fmaxf(den1, BN_NV_SMALL_EPS);
+  return (float)M_1_PI * 1.f / den2;
+}
+BN_HD float smith_ggx(float n_dot_o, float alpha_g) {
+  const float a = alpha_g * alpha_g;
+  const float b = n_dot_o * n_dot_o;
+  return 1.f / (n_dot_o + sqrtf(a + b - a * b));
+}
+BN_HD float luminance(vec3f c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
+BN_HD vec3f reflect(vec3f v, vec3f n) { return v - (2.f * dot(v, n)) * n; }
+
+struct Mat {
+  vec3f base_color, subsurface_color;
+  float metallic, specular, roughness, specular_tint;
+};
+
+BN_HD Mat unpack(const Bsdf &b) {
+  Mat m;
+  m.base_color = b.albedo;
+  m.subsurface_color = vec3f(.8f);
+  m.metallic = b.metallic;
+  m.specular = .5f;
+  m.roughness = b.roughness;
+  m.specular_tint = 0.f;
+  return m;
+}
+
+// disney_brdf (NVisii.h:600-660) for the active lobes
+BN_HD vec3f brdf(const Mat &mat, vec3f n, vec3f w_o, vec3f w_i, vec3f w_h) {
+#pragma clang fp contract(off)
+  const float n_dot_o = fabsf(dot(w_o, n));
+  const float n_dot_i = fabsf(dot(w_i, n));
+  const float i_dot_h = dot(w_i, w_h);
+  // disney_diffuse (no 1/pi, as in NVisii.h:338-352)
+  const float fd90 = 0.5f + 1.0f * mat.roughness * i_dot_h * i_dot_h;
+  const float fi = schlick_weight(n_dot_i), fo = schlick_weight(n_dot_o);
+  const float diffuse = lerp_r(1.f, fd90, fi) * lerp_r(1.f, fd90, fo);
+  // disney_microfacet_isotropic (NVisii.h:430-455)
+  const float lum = luminance(mat.base_color);
+  const vec3f tint = lum > 0.f ? mat.base_color * (1.f / lum) : vec3f(1.f);
+  const vec3f spec = lerp_r(mat.specular * 0.08f * lerp_r(vec3f(1.f), tint, vec3f(mat.specular_tint)),
+                            mat.base_color, vec3f(mat.metallic));
+  const float alpha = fmaxf(BN_NV_MIN_ALPHA, mat.roughness * mat.roughness);
+  const float d = gtr_2(fabsf(dot(n, w_h)), alpha);
+  const vec3f f = lerp_r(spec, vec3f(1.f),
+                         vec3f(schlick_weight(fabsf(dot(w_i, w_h))) * lerp_r(.5f, 1.f, fmaxf(mat.metallic, alpha))));
+  const float g = smith_ggx(fabsf(dot(n, w_i)), alpha) * smith_ggx(fabsf(dot(n, w_o)), alpha);
+  const vec3f gloss = d * f * g;
+  const vec3f flat = diffuse * mat.base_color;  // flatness 0
+  return flat * (1.f - mat.metallic) * (1.f - 0.f) + gloss;
+}
+
+// disney_pdf (NVisii.h:700-790), IMPORTANCE_SAMPLE_BRDF weights (unnormalized)
+BN_HD float pdf(const Mat &mat, vec3f n, vec3f w_o, vec3f w_i, vec3f w_h) {
+#pragma clang fp contract(off)
+  const float diffuse_weight = (1.f - mat.metallic) * (1.f - 0.f);
+  const float glossy_weight = 1.f + mat.metallic + 0.f + mat.specular + mat.roughness;
+  const float alpha = fmaxf(BN_NV_MIN_ALPHA, mat.roughness * mat.roughness);
+  const float d_i = dot(w_i, n);
+  const float diffuse = d_i > 0.f ? d_i * (float)M_1_PI : 0.f;
+  float microfacet = 0.f;
+  if (same_hemisphere(w_o, w_i, n)) {
+    const float c = fabsf(dot(n, w_h));
+    microfacet = gtr_2(c, alpha) * c / (4.f * fabsf(dot(w_o, w_h)));
+  }
+  return diffuse_weight * diffuse + glossy_weight * microfacet;
+}
+
+// concentric disk mapping (NVisii.h:62-82)
+BN_HD vec3f cos_sample_hemisphere(float ux, float uy) {
+#pragma clang fp contract(off)
+  const float sx = 2.f * ux - 1.f, sy = 2.f * uy - 1.f;
+  float radius = 0.f, theta = 0.f;
+  if (!(sx == 0.f && sy == 0.f)) {
+    if (fabsf(sx) > fabsf(sy)) {
+      radius = sx;
+      theta = (float)M_PI / 4.f * (sy / sx);
+    } else {
+      radius = sy;
+      theta = (float)M_PI / 2.f - (float)M_PI / 4.f * (sx / sy);
+    }
+  }
+  const float dx = radius * cosf(theta), dy = radius * sinf(theta);
+  return {dx, dy, sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy))};
+}
+}  // namespace nvisii
+
+// RobustDielectric::sample of barney/packedBSDF/Glass.h (ospray-derived)
+struct GlassSample {
+  vec3f wi;
+  bool transmit;
+};
+
+BN_HD GlassSample glass_sample(float iorEta, vec3f wo, vec3f Ns, vec3f Ng, Random2 &random) {
+#pragma clang fp contract(off)
+  const float eta = 1.f / iorEta;
+  const vec3f incoming = -wo;
+  vec3f sn = Ns, gn = Ng;
+  const float glass_epsilon = 1.e-6f;
+  const bool canReflect = (eta < 1.f - glass_epsilon || eta > 1.f + glass_epsilon);
+  if (dot(incoming, sn) > 0.f) sn = -sn;
+  if (dot(incoming, gn) > 0.f) gn = -gn;
+  // reflectionDirection
+  const vec3f par = dot(incoming, sn) * sn;
+  const vec3f dirR = (incoming - par) - par;
+  // refractionDirection (the head-on branch can never trigger: its test
+  // reads 1-1e-6 < c < 1-1e-6)
+  vec3f dirT;
+  float fresnelR;
+  bool mustReflect = false;
+  {
+    const float cos1 = dot(incoming, sn);
+    float cos1_abs = cos1 < 0.f ? -cos1 : cos1;
+    const vec3f dpar = cos1 * sn;
+    const vec3f dperp = incoming - dpar;
+    const vec3f b = normalize(dperp);
+    const float sin1 = dot(dperp, b);
+    const float sin2 = sin1 / eta;
+    if (sin2 > 1.f) {
+      dirT = vec3f(0.f);
+      fresnelR = 1.f;
+      mustReflect = true;
+    } else {
+      const float cos2 = sqrtf(fminf(fmaxf(1.f - sin2 * sin2, 0.f), 1.f));
+      const vec3f N2 = normalize(dpar);
+      dirT = N2 * cos2 + b * sin2;
+      const float c1 = cos1_abs, c2 = cos2;
+      const float r1 = (eta * c1 - c2) / (eta * c1 + c2);
+      const float r2 = (c1 - eta * c2) / (c1 + eta * c2);
+      fresnelR = 0.5f * (r1 * r1 + r2 * r2);
+    }
+    fresnelR = fminf(fmaxf(fresnelR, 0.f), 1.f);
+  }
+  GlassSample res;
+  if ((random() < fresnelR && canReflect) || mustReflect) {
+    res.wi = dirR;
+    res.transmit = false;
+    const float gc = dot(res.wi, gn);
+    if (gc >= 0.f) {
+      if (gc < glass_epsilon) res.wi = normalize(res.wi + glass_epsilon * gn);
+    } else {
+      res.wi = res.wi - 2.f * gn * gc;
+    }
+  } else {
+    res.wi = dirT;
+    res.transmit = true;
+    const float gc = dot(res.wi, gn);
+    if (gc <= 0.f) {
+      if (gc > -glass_epsilon) res.wi = normalize(res.wi - glass_epsilon * gn);
+    } else {
+      res.transmit = false;  // counted as a reflection
+    }
+  }
+  return res;
+}
+
+// BSDF::eval (dg.Ns = dg.Ng = Ng, the unflipped geometric normal)
+BN_HD EvalRes bsdf_eval(const Bsdf &b, vec3f Ns, vec3f wi, vec3f wo) {
 #pragma clang fp contract(off)
   EvalRes r;
   if (b.type == BSDF_LAMBERT) {
     const float cosThetaI = fmaxf(dot(wi, Ns), 0.f);
     r.pdf = cosThetaI * BN_ONE_OVER_PI;
     r.value = b.albedo * BN_ONE_OVER_PI * cosThetaI;
+  } else if (b.type == BSDF_NVISII) {
+    const nvisii::Mat m = nvisii::unpack(b);
+    const vec3f w_h = normalize(wi + wo);
+    r.value = nvisii::brdf(m, Ns, wo, wi, w_h);
+    r.pdf = nvisii::pdf(m, Ns, wo, wi, w_h);
+  } else if (b.type == BSDF_GLASS) {
+    r.value = vec3f(0.f);
+    r.pdf = 0.f;
   } else {
     const float density = BN_ONE_OVER_FOUR_PI;
     r.value = density * b.albedo;
@@ -153,7 +346,7 @@ BN_HD EvalRes bsdf_eval(const Bsdf &b, vec3f Ns, vec3f wi) {
   return r;
 }
 
-BN_HD float bsdf_pdf(const Bsdf &b, vec3f Ng, vec3f wi) {
+BN_HD float bsdf_pdf(const Bsdf &b, vec3f Ng, vec3f wi, vec3f wo) {
 #pragma clang fp contract(off)
   if (b.type == BSDF_LAMBERT) {
     vec3f N = Ng;
@@ -161,18 +354,29 @@ BN_HD float bsdf_pdf(const Bsdf &b, vec3f Ng, vec3f wi) {
     const float cosThetaI = fmaxf(dot(wi, N), 0.f);
     return cosThetaI * BN_ONE_OVER_PI;
   }
+  if (b.type == BSDF_NVISII) {
+    const nvisii::Mat m = nvisii::unpack(b);
+    return nvisii::pdf(m, Ng, wo, wi, normalize(wi + wo));
+  }
+  if (b.type == BSDF_GLASS) return 0.f;
   return BN_ONE_OVER_FOUR_PI;
 }
+
+enum ScatterType : int { SCATTER_DIFFUSE = 0, SCATTER_GLOSSY = 1, SCATTER_SPECULAR = 2, SCATTER_VOLUME = 3 };
 
 struct ScatterRes {
   vec3f f_r, dir;
   float pdf;
-  bool diffuse;  // DIFFUSE vs VOLUME
+  int type;
+  float offsetDirection;
+  bool changedMedium;
 };
 
-BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, Random2 &rng) {
+BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedium, Random2 &rng) {
 #pragma clang fp contract(off)
   ScatterRes s;
+  s.offsetDirection = 1.f;
+  s.changedMedium = false;
   if (b.type == BSDF_LAMBERT) {
     vec3f N = Ng;
     if (dot(wo, N) < 0.f) N = -N;
@@ -188,7 +392,58 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, Random2 &rng) {
     s.dir = xfmVector(make_frame(N), local);
     s.pdf = local.z * BN_ONE_OVER_PI;
     s.f_r = b.albedo;
-    s.diffuse = true;
+    s.type = SCATTER_DIFFUSE;
+  } else if (b.type == BSDF_NVISII) {
+    // NVisii::scatter + sample_disney_brdf (NVisii.h:1092-1163, 820-975)
+    const nvisii::Mat m = nvisii::unpack(b);
+    const vec3f n = Ng;
+    const vec3f v_y = normalize(cross(Ng, wo));
+    const vec3f v_x = normalize(cross(Ng, v_y));
+    float dw = (1.f - m.metallic) * (1.f - 0.f);
+    float gw = 1.f + m.metallic + 0.f + m.specular + m.roughness;
+    const float sum = dw + gw;
+    const float scale = 1.f / sum;
+    dw *= scale;
+    gw *= scale;
+    const float type_rng = rng();
+    const bool diffuse = type_rng < dw;
+    const float ux = rng();
+    const float uy = rng();
+    vec3f w_i;
+    s.type = diffuse ? SCATTER_DIFFUSE : SCATTER_GLOSSY;
+    if (diffuse) {
+      const vec3f h = normalize(nvisii::cos_sample_hemisphere(ux, uy));
+      w_i = h.x * v_x + h.y * v_y + h.z * n;
+    } else {
+      const float alpha = fmaxf(BN_NV_MIN_ALPHA, m.roughness * m.roughness);
+      const float phi_h = 2.f * (float)M_PI * ux;
+      const float c2 = (1.f - uy) / (1.f + (alpha * alpha - 1.f) * uy);
+      const float cos_h = sqrtf(c2);
+      const float sin_h = 1.f - c2;  // as in NVisii.h:188 (no square root)
+      const vec3f hd = normalize(vec3f(sin_h * cosf(phi_h), sin_h * sinf(phi_h), cos_h));
+      const vec3f w_h = hd.x * v_x + hd.y * v_y + hd.z * n;
+      w_i = nvisii::reflect(-wo, w_h);
+      if (!nvisii::same_hemisphere(wo, w_i, n)) {
+        s.pdf = 0.f;
+        s.f_r = vec3f(0.f);
+        s.dir = vec3f(0.f);
+        return s;
+      }
+    }
+    const vec3f w_h = normalize(w_i + wo);
+    s.pdf = nvisii::pdf(m, n, wo, w_i, w_h);
+    s.f_r = nvisii::brdf(m, n, wo, w_i, w_h);
+    s.dir = normalize(w_i);
+  } else if (b.type == BSDF_GLASS) {
+    // Glass::scatter (Glass.h:219-241)
+    const float eta = (!insideMedium) ? 1.f / b.ior : b.ior;
+    const GlassSample g = glass_sample(eta, wo, Ng, Ng, rng);
+    s.f_r = vec3f(1.f);
+    s.dir = g.wi;
+    s.pdf = BN_INF;
+    s.offsetDirection = g.transmit ? -1.f : 1.f;
+    s.changedMedium = g.transmit;
+    s.type = SCATTER_SPECULAR;
   } else {
     const float r1 = rng();
     const float r2 = rng();
@@ -198,12 +453,10 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, Random2 &rng) {
     s.pdf = BN_ONE_OVER_FOUR_PI;
     s.f_r = b.albedo * BN_ONE_OVER_FOUR_PI;
     s.dir = {x, y, z};
-    s.diffuse = false;
+    s.type = SCATTER_VOLUME;
   }
   return s;
 }
-
-BN_HD bool is_inf(float f) { return f == BN_INF || f == -BN_INF; }
 
 // shadeRays.cu:476-878 for env/ambient lighting. Returns the fragment this
 // ray adds to its pixel. On return ray.tMax > 0 means the path continues with
@@ -266,7 +519,7 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
     }
   }
   if (haveLight) {
-    EvalRes f_r = bsdf_eval(bsdf, Ng, lsDir);
+    EvalRes f_r = bsdf_eval(bsdf, Ng, lsDir, wo);
     if (!(f_r.pdf > 0.f) || reduce_max(f_r.value) < 1e-4f) {
       shadow.tMax = -1.f;
     } else {
@@ -282,9 +535,10 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
       ray.rng.next_hash((uint64_t)float_as_uint(h.t));
       shadow.mis = 1.f;
       shadow.numDiffuse = ray.numDiffuse;
+      shadow.isInMedium = ray.isInMedium;
       if (lightNeedsMIS) {
         const float pl = BN_ONE_OVER_FOUR_PI;  // EnvMapLight::DD::pdf without a texture
-        const float pb = bsdf_pdf(bsdf, Ng, lsDir);
+        const float pb = bsdf_pdf(bsdf, Ng, lsDir, wo);
         shadow.mis = pl / (pl + pb + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tMax = -1.f;
       }
@@ -296,25 +550,28 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
     return fragment;
   }
   ray.tMax = BN_INF;
-  const ScatterRes sr = bsdf_scatter(bsdf, Ng, wo, random);
+  const ScatterRes sr = bsdf_scatter(bsdf, Ng, wo, ray.isInMedium, random);
   if (!(sr.pdf > 0.f) || sr.pdf <= 1e-6f) return fragment;  // shadeRays.cu:782-783
-  if (!sr.diffuse) {  // VOLUME scatter in SCI_VIS_MODE: volumes shadow, no bounce
+  if (sr.type == SCATTER_VOLUME) {  // SCI_VIS_MODE: volumes shadow, no bounce
     ray.tMax = -1.f;
     return fragment;
   }
-  if (ray.numDiffuse >= rp.maxDiffuseBounces) {
-    ray.tMax = -1.f;
-    return fragment;
+  if (sr.type == SCATTER_DIFFUSE) {
+    if (ray.numDiffuse >= rp.maxDiffuseBounces) {
+      ray.tMax = -1.f;
+      return fragment;
+    }
+    ray.numDiffuse = ray.numDiffuse + 1;
   }
-  ray.numDiffuse = ray.numDiffuse + 1;
-  ray.org = P + 1.f * offsetEps * ffOffset;
+  ray.org = P + sr.offsetDirection * offsetEps * ffOffset;
   ray.dir = normalize(sr.dir);
   vec3f scatterFactor = sr.f_r * (isVolumeHit ? 1.f : fabsf(dot(Ng, normalize(ray.dir)))) /
                         (is_inf(sr.pdf) ? 1.f : (sr.pdf + 1e-10f));
   scatterFactor = vmin(scatterFactor, 2.5f);
   ray.tp = ray.tp * scatterFactor;
+  if (sr.changedMedium) ray.isInMedium = !ray.isInMedium;
   if (lightNeedsMIS && !is_inf(sr.pdf)) {
-    const float pb = bsdf_pdf(bsdf, Ng, ray.dir);
+    const float pb = bsdf_pdf(bsdf, Ng, ray.dir, wo);
     const float pl = BN_ONE_OVER_FOUR_PI;
     ray.mis = pb / (pb + pl);
   } else {
@@ -332,12 +589,43 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   return fragment;
 }
 
+// DeviceMaterial::createBSDF (material/DeviceMaterial.h:47-62): matte ->
+// Lambertian; PBR -> Glass when ior != 1 and transmission >= 1e-3
+// (AnariPBR.h:75-80), else NVisii with roughness clamped to [.001, .9]. The
+// NVisii parameters are halfs in barney (NVisii.h:1062-1083); the material
+// table holds them already rounded to half.
+BN_HD Bsdf material_bsdf(const MaterialDev &m, vec3f color) {
+#pragma clang fp contract(off)
+  Bsdf b;
+  if (m.type == BNAMD_MAT_PBR) {
+    if (m.ior != 1.f && m.transmission >= 1e-3f) {
+      b.type = BSDF_GLASS;
+      b.albedo = vec3f(1.f);
+      b.ior = m.ior;
+      b.metallic = b.roughness = 0.f;
+      return b;
+    }
+    b.type = BSDF_NVISII;
+    b.albedo = {__half2float(__float2half(color.x)), __half2float(__float2half(color.y)),
+                __half2float(__float2half(color.z))};
+    b.metallic = m.metallic;
+    b.roughness = m.roughness;
+    b.ior = m.ior;
+    return b;
+  }
+  b.type = BSDF_LAMBERT;
+  b.albedo = .85f * color * BN_ONE_OVER_PI;
+  b.metallic = b.roughness = b.ior = 0.f;
+  return b;
+}
+
 // AnariMatte::createBSDF: albedo = 0.85 * color / pi (AnariMatte.h:56-62)
 BN_HD Bsdf matte_bsdf(vec3f color) {
 #pragma clang fp contract(off)
   Bsdf b;
   b.type = BSDF_LAMBERT;
   b.albedo = .85f * color * BN_ONE_OVER_PI;
+  b.metallic = b.roughness = b.ior = 0.f;
   return b;
 }
 
@@ -347,6 +635,7 @@ BN_HD Bsdf phase_bsdf(vec3f color) {
   Bsdf b;
   b.type = BSDF_PHASE;
   b.albedo = .7f * color;
+  b.metallic = b.roughness = b.ior = 0.f;
   return b;
 }
 

@@ -1,7 +1,9 @@
 // World construction (rtc::Device::createInstanceGroup + buildAccel) and the
 // batch closest-hit trace entry point (rtc::TraceInterface::traceRay).
 #include "world.h"
+#include <hip/hip_fp16.h>
 #include <cmath>
+#include <algorithm>
 
 namespace bnamd {
 
@@ -44,6 +46,8 @@ __global__ void __launch_bounds__(256) k_trace(WorldDev w, const bnamd_ray *__re
   }
   hits[i] = out;
 }
+
+float round_half(float f) { return __half2float(__float2half(f)); }
 
 bool is_identity(const float *x) {
   static const float id[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
@@ -123,6 +127,13 @@ extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const
           md.color[0] = s.color[0]; md.color[1] = s.color[1]; md.color[2] = s.color[2];
           md.opacity = s.opacity; md.metallic = s.metallic; md.roughness = s.roughness;
           md.transmission = s.transmission; md.ior = s.ior; md.specular = s.specular;
+          if (s.type == BNAMD_MAT_PBR && !(s.ior != 1.f && s.transmission >= 1e-3f)) {
+            // NVisii stores its parameters as halfs (NVisii.h:1062-1083);
+            // roughness is clamped to [.001, .9] (AnariPBR.h:86)
+            md.metallic = round_half(s.metallic);
+            md.roughness = round_half(std::min(std::max(s.roughness, .001f), .9f));
+            md.ior = round_half(s.ior);
+          }
         } else {
           md.type = BNAMD_MAT_MATTE;
           md.color[0] = md.color[1] = md.color[2] = 0.8f;

@@ -165,3 +165,144 @@ def workload4(n=256, views=64, size=512, seed=0x5EED0003):
     cams = orbit_cameras((c, c, c), 1.5 * diag, views, fovy=40.0)
     return dict(density=dens, tf_target=random_tf(seed + 1), tf_init=random_tf(seed + 2), cameras=cams,
                 size=(size, size), step=0.5, base_density=0.05)
+
+
+# ---------------------------------------------------------------------------
+# Workload 3: batched multi-view mixed scene
+# ---------------------------------------------------------------------------
+
+def value_noise_2d(n, lattice, seed, stream):
+    """Bilinear value noise on an n x n grid over a (lattice+1)^2 lattice."""
+    lat = uniform(seed, stream, (lattice + 1) ** 2).reshape(lattice + 1, lattice + 1)
+    x = np.arange(n, dtype=np.float64) * (lattice / max(n - 1, 1))
+    i0 = np.minimum(np.floor(x).astype(np.int64), lattice - 1)
+    f = x - i0
+    M = np.zeros((n, lattice + 1))
+    M[np.arange(n), i0] = 1.0 - f
+    M[np.arange(n), i0 + 1] += f
+    return M @ lat @ M.T
+
+
+def heightfield(n=2236, height=120.0, seed=0x5EED0002):
+    """n x n jittered vertex grid (x,z in [0, n-1], +-0.25 cell jitter),
+    heights from 4-octave value noise; 2 (n-1)^2 triangles facing +y."""
+    h = np.zeros((n, n))
+    amp = 1.0
+    for o in range(4):
+        h += amp * value_noise_2d(n, 4 << o, seed, 20 + o)
+        amp *= 0.5
+    h = (h - h.min()) / max(h.max() - h.min(), 1e-12) * height
+    jx = uniform(seed, 30, n * n, -0.25, 0.25).reshape(n, n)
+    jz = uniform(seed, 31, n * n, -0.25, 0.25).reshape(n, n)
+    zz, xx = np.meshgrid(np.arange(n, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
+    verts = np.stack([xx + jx, h, zz + jz], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    v00 = (i * n + j).ravel()
+    v01 = v00 + 1
+    v10 = v00 + n
+    v11 = v10 + 1
+    # winding so that cross(v1-v0, v2-v0) points up (+y)
+    t0 = np.stack([v00, v10, v11], -1)
+    t1 = np.stack([v00, v11, v01], -1)
+    tris = np.stack([t0, t1], 1).reshape(-1, 3).astype(np.int32)
+    return verts, tris, h
+
+
+def split_material(n, seed, stream, fractions=(0.6, 0.3, 0.1)):
+    """Random assignment of n items to len(fractions) classes."""
+    u = uniform(seed, stream, n)
+    edges = np.cumsum(fractions)
+    return np.searchsorted(edges, u, side="right").clip(0, len(fractions) - 1)
+
+
+def workload3(grid=2236, spheres=200_000, cylinders=100_000, views=64, seed=0x5EED0002):
+    """Mixed scene: heightfield triangles + spheres + cylinders, materials
+    diffuse/glossy/dielectric 60/30/10, 64 views of uneven size."""
+    verts, tris, h = heightfield(grid, seed=seed)
+    ext = float(grid - 1)
+    hmax = float(h.max())
+    # spheres and cylinders scattered over the terrain, above the surface
+    sx = uniform(seed, 40, spheres, 0.0, ext)
+    sz = uniform(seed, 41, spheres, 0.0, ext)
+    gi = np.clip(sz.astype(np.int64), 0, grid - 1) * grid + np.clip(sx.astype(np.int64), 0, grid - 1)
+    sr = uniform(seed, 42, spheres, 0.5, 3.0)
+    sy = h.ravel()[gi] + sr + uniform(seed, 43, spheres, 0.0, 20.0)
+    centers = np.stack([sx, sy, sz], -1).astype(np.float32)
+    cx = uniform(seed, 50, cylinders, 0.0, ext)
+    cz = uniform(seed, 51, cylinders, 0.0, ext)
+    ci = np.clip(cz.astype(np.int64), 0, grid - 1) * grid + np.clip(cx.astype(np.int64), 0, grid - 1)
+    cy = h.ravel()[ci] + uniform(seed, 52, cylinders, 0.0, 10.0)
+    a = np.stack([cx, cy, cz], -1)
+    d = uniform(seed, 53, 3 * cylinders, -1.0, 1.0).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True) + 1e-9
+    length = uniform(seed, 54, cylinders, 2.0, 10.0)
+    b = a + d * length[:, None]
+    cverts = np.concatenate([a, b]).astype(np.float32)
+    cidx = np.stack([np.arange(cylinders), np.arange(cylinders) + cylinders], 1).astype(np.int32)
+    crad = uniform(seed, 55, cylinders, 0.1, 0.5).astype(np.float32)
+    # materials: 0 diffuse, 1 glossy, 2 dielectric
+    mats = []
+    for k in range(4):
+        col = uniform(seed, 60 + k, 3, 0.2, 0.9)
+        mats.append(dict(type="matte", color=tuple(col)))
+    for k in range(4):
+        col = uniform(seed, 70 + k, 3, 0.2, 0.9)
+        rough = float(uniform(seed, 80 + k, 1, 0.05, 0.5)[0])
+        mats.append(dict(type="pbr", color=tuple(col), roughness=rough, metallic=0.0))
+    mats.append(dict(type="glass", ior=1.5))
+    def pick(cls, n, stream):
+        sub = uniform(seed, stream, n)
+        out = np.empty(n, np.int64)
+        out[cls == 0] = (sub[cls == 0] * 4).astype(np.int64)
+        out[cls == 1] = 4 + (sub[cls == 1] * 4).astype(np.int64)
+        out[cls == 2] = 8
+        return out
+    geoms = []
+    tcls = split_material(len(tris), seed, 90)
+    tmat = pick(tcls, len(tris), 91)
+    for m in range(len(mats)):
+        sel = tmat == m
+        if sel.any():
+            geoms.append(dict(kind=0, vertices=verts, indices=tris[sel], material=m))
+    scls = split_material(spheres, seed, 92)
+    smat = pick(scls, spheres, 93)
+    for m in range(len(mats)):
+        sel = smat == m
+        if sel.any():
+            geoms.append(dict(kind=1, vertices=centers[sel], radii=sr[sel].astype(np.float32), material=m))
+    ccls = split_material(cylinders, seed, 94)
+    cmat = pick(ccls, cylinders, 95)
+    for m in range(len(mats)):
+        sel = cmat == m
+        if sel.any():
+            idx = np.nonzero(sel)[0]
+            geoms.append(dict(kind=2, vertices=cverts, indices=cidx[idx], radii=crad[idx], material=m))
+    # 64 views of uneven resolution looking down at the terrain
+    cams, widths, heights = [], [], []
+    wv = (256 + np.floor(uniform(seed, 100, views) * 769)).astype(int)
+    hv = (256 + np.floor(uniform(seed, 101, views) * 769)).astype(int)
+    px = uniform(seed, 102, views, 0.1 * ext, 0.9 * ext)
+    pz = uniform(seed, 103, views, 0.1 * ext, 0.9 * ext)
+    py = hmax + uniform(seed, 104, views, 40.0, 300.0)
+    tx = px + uniform(seed, 105, views, -300.0, 300.0)
+    tz = pz + uniform(seed, 106, views, -300.0, 300.0)
+    for k in range(views):
+        pos = (float(px[k]), float(py[k]), float(pz[k]))
+        tgt = (float(tx[k]), float(0.5 * hmax), float(tz[k]))
+        cams.append(dict(position=pos, direction=tuple(tgt[i] - pos[i] for i in range(3)), fovy=50.0))
+        widths.append(int(wv[k]))
+        heights.append(int(hv[k]))
+    return dict(geoms=geoms, materials=mats, cameras=cams, widths=widths, heights=heights, spp=4, max_depth=8,
+                rr_depth=2, max_diffuse=8)
+
+
+def product_materials(api, mats):
+    out = []
+    for m in mats:
+        if m["type"] == "matte":
+            out.append(api.matte(m["color"]))
+        elif m["type"] == "pbr":
+            out.append(api.pbr(m["color"], metallic=m.get("metallic", 0.0), roughness=m["roughness"]))
+        else:
+            out.append(api.pbr((1.0, 1.0, 1.0), transmission=1.0, ior=m["ior"]))
+    return out

@@ -115,6 +115,43 @@ def surface_bench(api, steps=3, warmup=1):
             "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
 
 
+def batch_bench(api, steps=2):
+    """Workload 3: 10M-triangle heightfield + 200k spheres + 100k cylinders,
+    60/30/10 diffuse/glossy/dielectric, depth <= 8 with Russian roulette,
+    64 views of uneven size, 4 paths per pixel: megakernel vs wavefront."""
+    from barney_amd import scenes
+    t0 = time.time()
+    w3 = scenes.workload3()
+    geoms = [api.Geom(g["kind"], g["vertices"], g.get("indices"), g.get("radii"), material=g["material"])
+             for g in w3["geoms"]]
+    world = api.World([geoms], materials=scenes.product_materials(api, w3["materials"]))
+    info = world.info()
+    log(f"workload 3 built in {time.time() - t0:.1f}s (BVH {info['buildMs']:.0f} ms, {info['numPrims']} prims)")
+    views = api.Views([api.camera(**c) for c in w3["cameras"]], w3["widths"], w3["heights"])
+    p = api.render_params(0, 0, spp=w3["spp"], ambient=1.0, bg=(-1, -1, -1, -1), max_depth=w3["max_depth"],
+                          max_diffuse=w3["max_diffuse"], russian_roulette=w3["rr_depth"])
+    acc = api.DeviceArray.empty((views.num_pixels, 4), np.float32)
+    res = {}
+    api.batch_autotune(world, views, p)
+    for name, mode in (("megakernel", api.BATCH_MEGAKERNEL), ("wavefront", api.BATCH_WAVEFRONT)):
+        api.render_batch(world, views, p, mode, accum=acc)  # warm-up (graph capture)
+        ms, rays = [], 0
+        for _ in range(steps):
+            o = api.render_batch(world, views, p, mode, accum=acc)
+            ms.append(o["stats"]["ms"])
+            rays = o["stats"]["raysTraced"]
+        img = acc.numpy()
+        res[name] = {"ms_per_frame": float(np.median(ms)), "rays_per_frame": int(rays),
+                     "Mrays_per_s": rays / (np.median(ms) * 1e-3) / 1e6, "image": img}
+        log(f"workload 3 {name}: {np.median(ms):.1f} ms, {rays / 1e6:.1f} M rays")
+    a, b = res["megakernel"].pop("image"), res["wavefront"].pop("image")
+    res["max_abs_diff_wavefront_vs_megakernel"] = float(np.abs(a - b).max())
+    res["wavefront_speedup"] = res["wavefront"]["Mrays_per_s"] / res["megakernel"]["Mrays_per_s"]
+    res["pixels"] = views.num_pixels
+    res["bvh_build_ms"] = info["buildMs"]
+    return res
+
+
 def ea_bench(steps=3):
     """Workload 4: 256^3 density, 64 orbit views at 512^2, L2 loss against
     images of a second transfer function; forward and checkpointed adjoint."""
@@ -157,6 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surface", action="store_true")
     ap.add_argument("--no-ea", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
@@ -212,6 +250,8 @@ def main():
     }
     if not args.no_surface:
         out["extra"]["surface_workload1"] = surface_bench(api)
+    if not args.no_batch:
+        out["extra"]["batch_workload3"] = batch_bench(api)
     if not args.no_ea:
         out["extra"]["ea_workload4"] = ea_bench()
     print(json.dumps(out), flush=True)

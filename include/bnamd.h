@@ -261,6 +261,51 @@ int bnamd_ea_backward(const bnamd_ea_volume *vol, const bnamd_ea_views *views, c
                       const float *checkpoints, float *gradDensity, float *gradTF, void *stream,
                       bnamd_ea_stats *stats);
 
+/* ------------------------------------------------------------------------
+ * Batched multi-view path tracing (not in the reference: barney renders one
+ * camera per frame). All views are rendered in one launch sequence; view v's
+ * pixels occupy accum[offset_v .. offset_v + w_v*h_v) with offsets the
+ * prefix sum of w*h in view order. The per-path computation is barney's
+ * generateRays + trace + bounce() (shadeRays.cu:476-878); shadow rays use an
+ * occlusion query, which decides the same hit/miss as barney's closest-hit
+ * shadow trace.
+ *   mode BNAMD_BATCH_MEGAKERNEL: one kernel, a few lanes per pixel.
+ *   mode BNAMD_BATCH_WAVEFRONT:  per-bounce queues (trace, sort by BSDF,
+ *        shade with compaction, shadow), persistent trace kernels, the bounce
+ *        loop captured once in a hipGraph and replayed; queue and launch
+ *        sizes from the autotuner (bnamd_batch_autotune) when one ran.
+ * --------------------------------------------------------------------- */
+#define BNAMD_BATCH_MEGAKERNEL 0
+#define BNAMD_BATCH_WAVEFRONT 1
+
+typedef struct {
+  int numViews;
+  const bnamd_camera *cameras; /* host [numViews] */
+  const int *widths;           /* host [numViews] */
+  const int *heights;          /* host [numViews] */
+} bnamd_views;
+
+typedef struct {
+  long long raysTraced;   /* primary + bounce + shadow rays */
+  long long primaryRays;
+  long long shadowRays;
+  float ms;
+  int graphReplays;       /* wavefront: graph launches used */
+} bnamd_batch_stats;
+
+/* params->width/height are ignored (per-view sizes); pathsPerPixel paths
+ * per pixel starting at params->accumID. accum is overwritten when
+ * params->accumID == 0, accumulated into otherwise. */
+int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views,
+                       const bnamd_render_params *params, float *accum, int mode,
+                       void *stream, bnamd_batch_stats *stats);
+
+/* Times candidate wavefront launch configurations on this world and views
+ * and keeps the fastest for later bnamd_render_batch calls; if path is
+ * non-NULL the choice is also written there as JSON. */
+int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views *views,
+                         const bnamd_render_params *params, const char *path);
+
 #ifdef __cplusplus
 }
 #endif
