@@ -72,6 +72,20 @@ __device__ __forceinline__ void add_frag(vec3f &sum, vec3f frag, float clampMax)
   if (frag.z > 0.f) sum.z += frag.z;
 }
 
+// Wavefront-aggregated append: one atomic per wavefront on 'ctr', each
+// active lane with 'pred' gets base + its rank among those lanes. Must be
+// called by all active lanes of the wavefront together.
+__device__ __forceinline__ unsigned wave_append(unsigned *ctr, bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return 0u;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0u;
+  if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
 __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long long *dst) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
@@ -304,7 +318,12 @@ __global__ void k_wf_sort_scatter(WfDD e, int depth) {
   __syncthreads();
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const unsigned k = e.keys[i];
-    const unsigned pos = start[k] + atomicAdd(&e.cnt->cursor[depth][k], 1u);
+    unsigned pos = 0u;
+#pragma unroll
+    for (unsigned b = 0; b < 4; ++b) {
+      const unsigned p = wave_append(&e.cnt->cursor[depth][b], k == b);
+      if (k == b) pos = start[b] + p;
+    }
     e.perm[pos] = i;
   }
 }
@@ -365,8 +384,11 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
     PathRay shadow;
     const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
     accum_add(e.accum, pix, frag, clampMax);
-    if (shadow.tMax > 0.f) {
-      const unsigned si = atomicAdd(&e.cnt->shadow[depth], 1u);
+    const bool wantShadow = shadow.tMax > 0.f;
+    const bool wantCont = ray.tMax > 0.f;
+    const unsigned si = wave_append(&e.cnt->shadow[depth], wantShadow);
+    const unsigned o = wave_append(&e.cnt->count[depth + 1], wantCont);
+    if (wantShadow) {
       ShadowRec srec;
       const vec3f c = shadow.mis * shadow.tp;  // fragment if unoccluded
       srec.a = make_float4(shadow.org.x, shadow.org.y, shadow.org.z, shadow.tMax);
@@ -374,8 +396,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
       srec.c = make_float4(c.x, c.y, c.z, 0.f);
       e.sq[si] = srec;
     }
-    if (ray.tMax > 0.f) {
-      const unsigned o = atomicAdd(&e.cnt->count[depth + 1], 1u);
+    if (wantCont) {
       PathRec out;
       out.a = make_float4(ray.org.x, ray.org.y, ray.org.z, ray.tMax);
       out.b = make_float4(ray.dir.x, ray.dir.y, ray.dir.z, ray.mis);
