@@ -21,7 +21,9 @@ constexpr uint32_t PRIM_SHADOW_TRANSPARENT = 1u << 8;
 // plane relative to the node origin with a power-of-two scale per axis.
 //   n0: ox oy oz | ex ey ez imask
 //   n1: childBase | primBase | meta[0..3] | meta[4..7]
-//   n2..n7: qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8] qhi.z[8] (u16)
+//   n2..n4: slots 0-3, n5..n7: slots 4-7, each as
+//           qlo.x[4] qlo.y[4] qlo.z[4] qhi.x[4] qhi.y[4] qhi.z[4] (u16)
+// so the traversal tests one half-node (48 bytes) at a time.
 // meta[slot]: leaf child -> (count << 5) | offset into the node's prim block;
 // internal child -> imask bit set, node = childBase + rank among internal
 // slots; empty -> 0 with an inverted (empty) box.
@@ -31,7 +33,9 @@ struct alignas(128) BNode8 {
   uint32_t childBase;
   uint32_t primBase;
   uint8_t meta[8];
-  uint16_t q[6][8];
+  uint16_t q[2][6][4];  // [slot >> 2][plane][slot & 3]
+  BN_HD uint16_t &plane(int k, int slot) { return q[slot >> 2][k][slot & 3]; }
+  BN_HD uint16_t plane(int k, int slot) const { return q[slot >> 2][k][slot & 3]; }
 };
 static_assert(sizeof(BNode8) == 128, "BNode8 must be one 128B line");
 
@@ -278,7 +282,6 @@ BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict pri
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
   const vec3f inv = rcp(dir);
-  const vec3f oi = org * inv;
   const uint32_t oct = (dir.x < 0.f ? 1u : 0u) | (dir.y < 0.f ? 2u : 0u) | (dir.z < 0.f ? 4u : 0u);
 
   uint32_t grpBase = 0;
@@ -300,44 +303,54 @@ BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict pri
       const uint4 *np = (const uint4 *)(nodes + nodeIdx);
       const uint4 n0 = np[0];
       const uint4 n1 = np[1];
-      const uint4 qx0 = np[2], qy0 = np[3], qz0 = np[4];
-      const uint4 qx1 = np[5], qy1 = np[6], qz1 = np[7];
       const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
       const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
       const uint32_t nimask = n0.w >> 24;
       const float sx = __uint_as_float(ex << 23) * inv.x;
       const float sy = __uint_as_float(ey << 23) * inv.y;
       const float sz = __uint_as_float(ez << 23) * inv.z;
-      const float bx = ox * inv.x - oi.x;
-      const float by = oy * inv.y - oi.y;
-      const float bz = oz * inv.z - oi.z;
+      // slab offsets from (node origin - ray origin): the rounding error
+      // stays relative to the node's extent, which the one-quantum slack of
+      // the quantized boxes covers; ox * inv - org * inv would cancel
+      // catastrophically far from the world origin
+      const float bx = (ox - org.x) * inv.x;
+      const float by = (oy - org.y) * inv.y;
+      const float bz = (oz - org.z) * inv.z;
       const float tcap = best.t;
       uint32_t hitInner = 0, hitPrims = 0;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t wlx = (&qx0.x)[c >> 1], wly = (&qy0.x)[c >> 1], wlz = (&qz0.x)[c >> 1];
-        const uint32_t whx = (&qx1.x)[c >> 1], why = (&qy1.x)[c >> 1], whz = (&qz1.x)[c >> 1];
-        const int sh = (c & 1) * 16;
-        const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
-                    lz = (float)((wlz >> sh) & 0xffffu);
-        const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
-                    hz = (float)((whz >> sh) & 0xffffu);
-        const float tlx = fmaf(lx, sx, bx), thx = fmaf(hx, sx, bx);
-        const float tly = fmaf(ly, sy, by), thy = fmaf(hy, sy, by);
-        const float tlz = fmaf(lz, sz, bz), thz = fmaf(hz, sz, bz);
-        float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), tmin));
-        float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fmaxf(tlz, thz));
-        // conservative widening against rounding in the quantized slab test
-        tn = tn - fabsf(tn) * 2e-6f;
-        tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
-        const bool hit = (tn <= tf) && (lx <= hx);
-        if (hit) {
-          const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
-          if (nimask & (1u << c)) {
-            hitInner |= 1u << (c ^ oct);
-          } else {
-            const uint32_t cnt = meta >> 5, off = meta & 31u;
-            hitPrims |= ((1u << cnt) - 1u) << off;
+      for (int h = 0; h < 2; ++h) {
+        const uint4 A = np[2 + 3 * h], B = np[3 + 3 * h], C = np[4 + 3 * h];
+        // u32 word pairs per plane: lo.x A.xy, lo.y A.zw, lo.z B.xy,
+        // hi.x B.zw, hi.y C.xy, hi.z C.zw
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = 4 * h + j;
+          const uint32_t wlx = (j < 2) ? A.x : A.y, wly = (j < 2) ? A.z : A.w;
+          const uint32_t wlz = (j < 2) ? B.x : B.y, whx = (j < 2) ? B.z : B.w;
+          const uint32_t why = (j < 2) ? C.x : C.y, whz = (j < 2) ? C.z : C.w;
+          const int sh = (j & 1) * 16;
+          const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
+                      lz = (float)((wlz >> sh) & 0xffffu);
+          const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
+                      hz = (float)((whz >> sh) & 0xffffu);
+          const float tlx = fmaf(lx, sx, bx), thx = fmaf(hx, sx, bx);
+          const float tly = fmaf(ly, sy, by), thy = fmaf(hy, sy, by);
+          const float tlz = fmaf(lz, sz, bz), thz = fmaf(hz, sz, bz);
+          float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), tmin));
+          float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fmaxf(tlz, thz));
+          // conservative widening against rounding in the quantized slab test
+          tn = tn - fabsf(tn) * 2e-6f;
+          tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
+          const bool hit = (tn <= tf) && (lx <= hx);
+          if (hit) {
+            const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
+            if (nimask & (1u << c)) {
+              hitInner |= 1u << (c ^ oct);
+            } else {
+              const uint32_t cnt = meta >> 5, off = meta & 31u;
+              hitPrims |= ((1u << cnt) - 1u) << off;
+            }
           }
         }
       }

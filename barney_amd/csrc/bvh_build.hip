@@ -354,7 +354,7 @@ __global__ void k_collapse(const CollapseTask *__restrict tasks, uint32_t numTas
     const int i = childOfSlot[s];
     node.meta[s] = 0;
     if (i < 0) {
-      for (int k = 0; k < 3; ++k) { node.q[k][s] = 0xffff; node.q[3 + k][s] = 0; }
+      for (int k = 0; k < 3; ++k) { node.plane(k, s) = 0xffff; node.plane(3 + k, s) = 0; }
       continue;
     }
     const double o[3] = {nlo[0], nlo[1], nlo[2]};
@@ -363,8 +363,8 @@ __global__ void k_collapse(const CollapseTask *__restrict tasks, uint32_t numTas
       double qh = ceil(((double)chi[i][k] - o[k]) / scale[k]);
       ql = fmax(0.0, ql - 1.0);
       qh = fmin(65535.0, qh + 1.0);
-      node.q[k][s] = (uint16_t)ql;
-      node.q[3 + k][s] = (uint16_t)qh;
+      node.plane(k, s) = (uint16_t)ql;
+      node.plane(3 + k, s) = (uint16_t)qh;
     }
     const int cnt = child_count(ch[i], range);
     if (cnt > kLeafMax) {
@@ -394,14 +394,52 @@ __global__ void k_empty_root(BNode8 *out) {
   node.primBase = 0;
   for (int s = 0; s < 8; ++s) {
     node.meta[s] = 0;
-    for (int k = 0; k < 3; ++k) { node.q[k][s] = 0xffff; node.q[3 + k][s] = 0; }
+    for (int k = 0; k < 3; ++k) { node.plane(k, s) = 0xffff; node.plane(3 + k, s) = 0; }
   }
   out[0] = node;
 }
 
-// Host-side structural check of a finished BVH8: every child / primitive
-// reference in range, every primitive referenced exactly once. Guards the
-// traversal kernels against out-of-bounds fetches; disable with
+// Decoded (float) box of one slot of a node.
+void slot_box(const BNode8 &nd, int s, float lo[3], float hi[3]) {
+  const float o[3] = {nd.ox, nd.oy, nd.oz};
+  const int e[3] = {nd.ex, nd.ey, nd.ez};
+  for (int k = 0; k < 3; ++k) {
+    const float sc = ldexpf(1.f, e[k] - 127);
+    lo[k] = o[k] + (float)nd.plane(k, s) * sc;
+    hi[k] = o[k] + (float)nd.plane(3 + k, s) * sc;
+  }
+}
+
+void prim_box(const PrimRec &r, uint32_t type, float lo[3], float hi[3]) {
+  const float a[3] = {r.a.x, r.a.y, r.a.z}, b[3] = {r.b.x, r.b.y, r.b.z}, c[3] = {r.c.x, r.c.y, r.c.z};
+  for (int k = 0; k < 3; ++k) {
+    if (type == PRIM_TRIANGLE) {
+      lo[k] = std::min(a[k], std::min(a[k] + b[k], a[k] + c[k]));
+      hi[k] = std::max(a[k], std::max(a[k] + b[k], a[k] + c[k]));
+    } else if (type == PRIM_SPHERE) {
+      lo[k] = a[k] - r.b.x;
+      hi[k] = a[k] + r.b.x;
+    } else {
+      lo[k] = std::min(a[k], b[k]) - r.c.x;
+      hi[k] = std::max(a[k], b[k]) + r.c.x;
+    }
+  }
+}
+
+// inner box within outer box up to float rounding of the decode
+bool box_inside(const float ilo[3], const float ihi[3], const float olo[3], const float ohi[3]) {
+  for (int k = 0; k < 3; ++k) {
+    const float tol = 1e-6f * (1.f + std::max(fabsf(olo[k]), fabsf(ohi[k])));
+    if (ilo[k] < olo[k] - tol || ihi[k] > ohi[k] + tol) return false;
+  }
+  return true;
+}
+
+// Host-side check of a finished BVH8: every child / primitive reference in
+// range, every primitive referenced exactly once, and every decoded child
+// box containing what it bounds (the primitives of a leaf slot, all slots of
+// an inner child node). Guards the traversal kernels against out-of-bounds
+// fetches and the quantization against dropping geometry; disable with
 // BNAMD_SKIP_BVH_CHECK=1 for very large scenes once trusted.
 void validate_bvh(const World &w) {
   const char *skip = getenv("BNAMD_SKIP_BVH_CHECK");
@@ -416,7 +454,8 @@ void validate_bvh(const World &w) {
     int rank = 0;
     for (int s = 0; s < 8; ++s) {
       if (nd.imask & (1u << s)) {
-        BN_REQUIRE(nd.childBase + rank < w.numNodes && nd.childBase + rank > i, "BVH8 check: bad child index");
+        const uint32_t ci = nd.childBase + rank;
+        BN_REQUIRE(ci < w.numNodes && ci > i, "BVH8 check: bad child index");
         ++rank;
       } else {
         const int cnt = nd.meta[s] >> 5, off = nd.meta[s] & 31;
@@ -430,6 +469,57 @@ void validate_bvh(const World &w) {
     }
     BN_REQUIRE(nd.ex >= 1 && nd.ey >= 1 && nd.ez >= 1 && nd.ex <= 254 && nd.ey <= 254 && nd.ez <= 254,
                "BVH8 check: bad quantization exponent");
+  }
+  // Bottom-up (children have larger indices than their parent): the union
+  // of the primitive boxes below each node must lie inside the slot box
+  // that leads to it.
+  std::vector<float> content(6 * (size_t)w.numNodes);
+  for (int64_t i = (int64_t)w.numNodes - 1; i >= 0; --i) {
+    const BNode8 &nd = nodes[i];
+    float *u = &content[6 * i];
+    for (int k = 0; k < 3; ++k) {
+      u[k] = INFINITY;
+      u[3 + k] = -INFINITY;
+    }
+    int rank = 0;
+    for (int s = 0; s < 8; ++s) {
+      float blo[3], bhi[3], lo[3], hi[3];
+      slot_box(nd, s, blo, bhi);
+      bool any = false;
+      if (nd.imask & (1u << s)) {
+        const float *c = &content[6 * (size_t)(nd.childBase + rank)];
+        ++rank;
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = c[k];
+          hi[k] = c[3 + k];
+        }
+        any = true;
+      } else {
+        const int cnt = nd.meta[s] >> 5, off = nd.meta[s] & 31;
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = INFINITY;
+          hi[k] = -INFINITY;
+        }
+        for (int j = 0; j < cnt; ++j) {
+          const PrimRec &r = prims[nd.primBase + off + j];
+          uint32_t type;
+          memcpy(&type, &r.c.w, 4);
+          float plo[3], phi[3];
+          prim_box(r, type & 0xffu, plo, phi);
+          for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], plo[k]);
+            hi[k] = std::max(hi[k], phi[k]);
+          }
+          any = true;
+        }
+      }
+      if (!any) continue;
+      BN_REQUIRE(box_inside(lo, hi, blo, bhi), "BVH8 check: geometry outside the slot box that leads to it");
+      for (int k = 0; k < 3; ++k) {
+        u[k] = std::min(u[k], lo[k]);
+        u[3 + k] = std::max(u[3 + k], hi[k]);
+      }
+    }
   }
   for (uint32_t p = 0; p < w.numPrims; ++p) {
     BN_REQUIRE(seen[p], "BVH8 check: unreferenced primitive");

@@ -4,14 +4,18 @@
 // order of floating-point accumulation.
 //
 // Wavefront organisation, per bounce d (all views and paths at once):
-//   trace   persistent workgroups pull 64-ray chunks from an atomic counter
-//           and write one 16-byte hit record per ray;
-//   sort    a counting sort of ray indices by BSDF type (miss, Lambert,
-//           NVisii, Glass) so each wavefront of the shade kernel runs one
-//           material model;
+//   trace   a resident grid (sized from the occupancy of the traversal
+//           kernel) walks the queue in 64-ray slices per wavefront and
+//           writes one 16-byte hit record per ray;
+//   sort    a stable counting sort of ray indices by BSDF type (miss,
+//           Lambert, NVisii, Glass) so each wavefront of the shade kernel
+//           runs one material model: per-block key histograms, one scan,
+//           then a scatter that ranks keys with wavefront ballots -- no
+//           global atomics, so the permutation is deterministic;
 //   shade   barney's bounce(): fragments are added to the pixel with float
 //           atomics, surviving paths are appended (compacted) to the next
-//           queue, shadow rays to the shadow queue;
+//           queue, shadow rays to the shadow queue, with one counter atomic
+//           per workgroup;
 //   shadow  persistent occlusion queries; unoccluded contributions added.
 // The whole loop, including counter resets (fill kernels, not memsets), is
 // captured once into a hipGraph and replayed per batch of paths.
@@ -84,6 +88,38 @@ __device__ __forceinline__ unsigned wave_append(unsigned *ctr, bool pred) {
   if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(mask));
   base = __shfl(base, leader);
   return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Workgroup-aggregated append of two predicates to two counters: one atomic
+// per counter per 256-thread workgroup. Every thread of the workgroup must
+// call it (with pred = false where it has nothing to append). Within the
+// workgroup, positions follow thread order.
+__device__ __forceinline__ void block_append2(unsigned *ctrA, bool predA, unsigned &posA, unsigned *ctrB,
+                                              bool predB, unsigned &posB) {
+  __shared__ unsigned cntA[4], cntB[4], baseA, baseB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long mA = __ballot(predA), mB = __ballot(predB);
+  if (lane == 0) {
+    cntA[wave] = (unsigned)__popcll(mA);
+    cntB[wave] = (unsigned)__popcll(mB);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned tA = cntA[0] + cntA[1] + cntA[2] + cntA[3];
+    const unsigned tB = cntB[0] + cntB[1] + cntB[2] + cntB[3];
+    baseA = tA ? atomicAdd(ctrA, tA) : 0u;
+    baseB = tB ? atomicAdd(ctrB, tB) : 0u;
+  }
+  __syncthreads();
+  unsigned oA = baseA, oB = baseB;
+  for (int w = 0; w < wave; ++w) {
+    oA += cntA[w];
+    oB += cntB[w];
+  }
+  posA = oA + (unsigned)__popcll(mA & lt);
+  posB = oB + (unsigned)__popcll(mB & lt);
+  __syncthreads();  // counts are rewritten by the next call
 }
 
 __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long long *dst) {
@@ -202,6 +238,10 @@ struct WfCounters {
   unsigned long long rays, shadowRays;
 };
 
+// Counting sort geometry: kSortBlocks workgroups, each owning one contiguous
+// segment of the queue (a multiple of 256 rays).
+constexpr int kSortBlocks = 1024;
+
 struct WfDD {
   WorldDev w;
   ViewsDD vd;
@@ -213,11 +253,41 @@ struct WfDD {
   HitRec *hits;
   unsigned int *keys;
   unsigned int *perm;
+  unsigned int *blockHist;  // [kSortBlocks][4] key counts, then offsets
   ShadowRec *sq;
   float4 *accum;
   WfCounters *cnt;
-  int chunk;      // rays per persistent work fetch (per wavefront)
+  int chunk;      // 0: static 64-ray slices per wavefront; else rays per
+                  // dynamic work fetch (one atomic per fetch)
 };
+
+__device__ __forceinline__ unsigned sort_segment(unsigned n) {
+  const unsigned per = (n + kSortBlocks - 1) / kSortBlocks;
+  return (per + 255u) & ~255u;
+}
+
+// Static slices: wavefront g of G handles rays [(g + k G) 64, +64).
+// Dynamic: wavefronts fetch 'chunk' rays at a time from 'work'.
+template <typename F>
+__device__ __forceinline__ void for_each_ray(unsigned n, int chunk, unsigned *work, F &&f) {
+  const int lane = threadIdx.x & 63;
+  if (chunk == 0) {
+    const unsigned G = gridDim.x * (blockDim.x >> 6);
+    const unsigned g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (unsigned base = g * 64u; base < n; base += G * 64u) {
+      const unsigned i = base + lane;
+      if (i < n) f(i);
+    }
+    return;
+  }
+  while (true) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(work, (unsigned)chunk);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    for (unsigned i = base + lane; i < base + chunk && i < n; i += 64) f(i);
+  }
+}
 
 // path id = s * numPixels + global pixel
 __device__ __forceinline__ void path_pixel(const WfDD &e, unsigned pid, int &s, int &view, int &ix, int &iy,
@@ -257,29 +327,21 @@ __global__ void k_wf_generate(WfDD e) {
   if (pid == 0) e.cnt->count[0] = total;
 }
 
-// Persistent trace: each wavefront fetches 'chunk' rays at a time.
 __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->count[depth];
   const PathRec *__restrict q = e.q[depth & 1];
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
-  const int lane = threadIdx.x & 63;
   unsigned long long nr = 0;
-  while (true) {
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(&e.cnt->work[depth], (unsigned)e.chunk);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    for (unsigned i = base + lane; i < base + e.chunk && i < n; i += 64) {
-      const float4 a = q[i].a, b = q[i].b;
-      const vec3f org = {a.x, a.y, a.z};
-      const vec3f dir = fix_dir({b.x, b.y, b.z});
-      HitRec h;
-      traverse_closest(e.w.nodes, e.w.prims, org, dir, 0.f, a.w, h, ctx);
-      e.hits[i] = h;
-      ++nr;
-    }
-  }
+  for_each_ray(n, e.chunk, &e.cnt->work[depth], [&](unsigned i) {
+    const float4 a = q[i].a, b = q[i].b;
+    const vec3f org = {a.x, a.y, a.z};
+    const vec3f dir = fix_dir({b.x, b.y, b.z});
+    HitRec h;
+    traverse_closest(e.w.nodes, e.w.prims, org, dir, 0.f, a.w, h, ctx);
+    e.hits[i] = h;
+    ++nr;
+  });
   wave_add(nr, &e.cnt->rays);
 }
 
@@ -291,40 +353,101 @@ __device__ __forceinline__ unsigned bsdf_key(const WfDD &e, const HitRec &h) {
   return (m.ior != 1.f && m.transmission >= 1e-3f) ? 3u : 2u;
 }
 
-__global__ void k_wf_sort_count(WfDD e, int depth) {
+// Pass 1: keys and per-block key counts over the block's segment.
+__global__ void __launch_bounds__(256) k_wf_sort_count(WfDD e, int depth) {
   __shared__ unsigned int hist[4];
   if (threadIdx.x < 4) hist[threadIdx.x] = 0u;
   __syncthreads();
   const unsigned n = e.cnt->count[depth];
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const unsigned seg = sort_segment(n);
+  const unsigned lo = blockIdx.x * seg, hi = min(n, lo + seg);
+  for (unsigned i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     const unsigned k = bsdf_key(e, e.hits[i]);
     e.keys[i] = k;
     atomicAdd(&hist[k], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < 4 && hist[threadIdx.x]) atomicAdd(&e.cnt->bins[depth][threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x < 4) e.blockHist[blockIdx.x * 4 + threadIdx.x] = hist[threadIdx.x];
 }
 
-__global__ void k_wf_sort_scatter(WfDD e, int depth) {
-  const unsigned n = e.cnt->count[depth];
-  __shared__ unsigned int start[4];
-  if (threadIdx.x == 0) {
+// Pass 2 (one workgroup of kSortBlocks threads): blockHist becomes the
+// first output position of each (block, key), key-major.
+__global__ void __launch_bounds__(kSortBlocks) k_wf_sort_scan(WfDD e) {
+  __shared__ unsigned waveTot[kSortBlocks / 64][4];
+  __shared__ unsigned keyStart[4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  uint4 c = *(const uint4 *)(e.blockHist + 4 * t);
+  unsigned v[4] = {c.x, c.y, c.z, c.w};
+  unsigned incl[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    unsigned x = v[k];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    incl[k] = x;
+    if (lane == 63) waveTot[wave][k] = x;
+  }
+  __syncthreads();
+  if (t < 4) {
+    unsigned acc = 0;
+    for (int w = 0; w < kSortBlocks / 64; ++w) {
+      const unsigned x = waveTot[w][t];
+      waveTot[w][t] = acc;  // exclusive prefix over wavefronts
+      acc += x;
+    }
+    keyStart[t] = acc;  // total of key t
+  }
+  __syncthreads();
+  if (t == 0) {
     unsigned acc = 0;
     for (int k = 0; k < 4; ++k) {
-      start[k] = acc;
-      acc += e.cnt->bins[depth][k];
+      const unsigned x = keyStart[k];
+      keyStart[k] = acc;
+      acc += x;
     }
   }
   __syncthreads();
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const unsigned k = e.keys[i];
-    unsigned pos = 0u;
+  uint4 o;
+  o.x = keyStart[0] + waveTot[wave][0] + incl[0] - v[0];
+  o.y = keyStart[1] + waveTot[wave][1] + incl[1] - v[1];
+  o.z = keyStart[2] + waveTot[wave][2] + incl[2] - v[2];
+  o.w = keyStart[3] + waveTot[wave][3] + incl[3] - v[3];
+  *(uint4 *)(e.blockHist + 4 * t) = o;
+}
+
+// Pass 3: stable scatter of ray indices; ranks within a 256-ray slice come
+// from wavefront ballots, so equal keys keep their queue order.
+__global__ void __launch_bounds__(256) k_wf_sort_scatter(WfDD e, int depth) {
+  __shared__ unsigned waveCnt[4][4];
+  const unsigned n = e.cnt->count[depth];
+  const unsigned seg = sort_segment(n);
+  const unsigned lo = blockIdx.x * seg, hi = min(n, lo + seg);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const uint4 o = *(const uint4 *)(e.blockHist + 4 * blockIdx.x);
+  unsigned run[4] = {o.x, o.y, o.z, o.w};
+  for (unsigned base = lo; base < hi; base += 256u) {
+    const unsigned i = base + threadIdx.x;
+    const unsigned k = i < hi ? e.keys[i] : 4u;
+    unsigned long long mine = 0ull;
 #pragma unroll
     for (unsigned b = 0; b < 4; ++b) {
-      const unsigned p = wave_append(&e.cnt->cursor[depth][b], k == b);
-      if (k == b) pos = start[b] + p;
+      const unsigned long long m = __ballot(k == b);
+      if (k == b) mine = m;
+      if (lane == 0) waveCnt[wave][b] = (unsigned)__popcll(m);
     }
-    e.perm[pos] = i;
+    __syncthreads();
+    if (k < 4u) {
+      unsigned pos = run[k] + (unsigned)__popcll(mine & lt);
+      for (int w = 0; w < wave; ++w) pos += waveCnt[w][k];
+      e.perm[pos] = i;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) run[b] += waveCnt[0][b] + waveCnt[1][b] + waveCnt[2][b] + waveCnt[3][b];
+    __syncthreads();
   }
 }
 
@@ -340,54 +463,59 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   const unsigned n = e.cnt->count[depth];
   const PathRec *__restrict qin = e.q[depth & 1];
   PathRec *__restrict qout = e.q[(depth + 1) & 1];
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const unsigned r = e.perm[i];
-    const PathRec rec = qin[r];
-    const HitRec h = e.hits[r];
-    const unsigned pid = __float_as_uint(rec.c.w);
-    int s, view, ix, iy, pix;
-    path_pixel(e, pid, s, view, ix, iy, pix);
-    RenderDD rpp = e.rp;
-    rpp.width = e.vd.size[view].x;
-    rpp.height = e.vd.size[view].y;
-    rpp.accumID = e.accum0 + s;
-    PathRay ray;
-    ray.org = {rec.a.x, rec.a.y, rec.a.z};
-    ray.tMax = rec.a.w;
-    ray.dir = {rec.b.x, rec.b.y, rec.b.z};
-    ray.mis = rec.b.w;
-    ray.tp = {rec.c.x, rec.c.y, rec.c.z};
-    ray.rng.state = (uint64_t)rec.d.x | ((uint64_t)rec.d.y << 32);
-    ray.numDiffuse = (int)(rec.d.z & 0xffu);
-    ray.isInMedium = (rec.d.z >> 8) & 1u;
-    ray.isShadow = false;
-    ray.missColor = vec3f(0.f);
-    ray.missAlpha = 0.f;
-    if (depth == 0) {
-      // generateRays' background for primary rays (generateRays.cu:199-210)
-      const float t = (iy + .5f) / float(rpp.height);
-      if (rpp.bg[3] >= 0.f) {
-        ray.missColor = {rpp.bg[0], rpp.bg[1], rpp.bg[2]};
-        ray.missAlpha = rpp.bg[3];
-      } else {
-        ray.missColor = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
-        ray.missAlpha = (1.0f - t) * 1.f + t * 1.f;
+  // workgroup-uniform trip count: block_append2 needs every thread
+  for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const unsigned i = base + threadIdx.x;
+    bool wantShadow = false, wantCont = false;
+    PathRay ray, shadow;
+    unsigned pid = 0;
+    if (i < n) {
+      const unsigned r = e.perm[i];
+      const PathRec rec = qin[r];
+      const HitRec h = e.hits[r];
+      pid = __float_as_uint(rec.c.w);
+      int s, view, ix, iy, pix;
+      path_pixel(e, pid, s, view, ix, iy, pix);
+      RenderDD rpp = e.rp;
+      rpp.width = e.vd.size[view].x;
+      rpp.height = e.vd.size[view].y;
+      rpp.accumID = e.accum0 + s;
+      ray.org = {rec.a.x, rec.a.y, rec.a.z};
+      ray.tMax = rec.a.w;
+      ray.dir = {rec.b.x, rec.b.y, rec.b.z};
+      ray.mis = rec.b.w;
+      ray.tp = {rec.c.x, rec.c.y, rec.c.z};
+      ray.rng.state = (uint64_t)rec.d.x | ((uint64_t)rec.d.y << 32);
+      ray.numDiffuse = (int)(rec.d.z & 0xffu);
+      ray.isInMedium = (rec.d.z >> 8) & 1u;
+      ray.isShadow = false;
+      ray.missColor = vec3f(0.f);
+      ray.missAlpha = 0.f;
+      if (depth == 0) {
+        // generateRays' background for primary rays (generateRays.cu:199-210)
+        const float t = (iy + .5f) / float(rpp.height);
+        if (rpp.bg[3] >= 0.f) {
+          ray.missColor = {rpp.bg[0], rpp.bg[1], rpp.bg[2]};
+          ray.missAlpha = rpp.bg[3];
+        } else {
+          ray.missColor = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
+          ray.missAlpha = (1.0f - t) * 1.f + t * 1.f;
+        }
       }
+      const vec3f dir = fix_dir(ray.dir);
+      const ShadeHit sh = shade_hit(e.w, ray.org, dir, ray.tMax, h);
+      const float clampMax = 10.f * (1 + rpp.accumID);
+      if (depth == 0) {
+        const float alpha = sh.hit ? 1.f : ray.missAlpha;
+        if (alpha > 0.f) atomicAdd(((float *)(e.accum + pix)) + 3, alpha);
+      }
+      const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
+      accum_add(e.accum, pix, frag, clampMax);
+      wantShadow = shadow.tMax > 0.f;
+      wantCont = ray.tMax > 0.f;
     }
-    const vec3f dir = fix_dir(ray.dir);
-    const ShadeHit sh = shade_hit(e.w, ray.org, dir, ray.tMax, h);
-    const float clampMax = 10.f * (1 + rpp.accumID);
-    if (depth == 0) {
-      const float alpha = sh.hit ? 1.f : ray.missAlpha;
-      if (alpha > 0.f) atomicAdd(((float *)(e.accum + pix)) + 3, alpha);
-    }
-    PathRay shadow;
-    const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
-    accum_add(e.accum, pix, frag, clampMax);
-    const bool wantShadow = shadow.tMax > 0.f;
-    const bool wantCont = ray.tMax > 0.f;
-    const unsigned si = wave_append(&e.cnt->shadow[depth], wantShadow);
-    const unsigned o = wave_append(&e.cnt->count[depth + 1], wantCont);
+    unsigned si, o;
+    block_append2(&e.cnt->shadow[depth], wantShadow, si, &e.cnt->count[depth + 1], wantCont, o);
     if (wantShadow) {
       ShadowRec srec;
       const vec3f c = shadow.mis * shadow.tp;  // fragment if unoccluded
@@ -412,27 +540,20 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->shadow[depth];
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
-  const int lane = threadIdx.x & 63;
   unsigned long long nr = 0;
-  while (true) {
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(&e.cnt->swork[depth], (unsigned)e.chunk);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    for (unsigned i = base + lane; i < base + e.chunk && i < n; i += 64) {
-      const ShadowRec s = e.sq[i];
-      const vec3f org = {s.a.x, s.a.y, s.a.z};
-      const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
-      const bool occluded = traverse_occluded(e.w.nodes, e.w.prims, org, dir, 0.f, s.a.w, ctx);
-      ++nr;
-      if (!occluded) {
-        const unsigned pid = __float_as_uint(s.b.w);
-        const int sidx = (int)(pid / (unsigned)e.numPixels);
-        const int pix = (int)(pid % (unsigned)e.numPixels);
-        accum_add(e.accum, pix, {s.c.x, s.c.y, s.c.z}, 10.f * (1 + e.accum0 + sidx));
-      }
+  for_each_ray(n, e.chunk, &e.cnt->swork[depth], [&](unsigned i) {
+    const ShadowRec s = e.sq[i];
+    const vec3f org = {s.a.x, s.a.y, s.a.z};
+    const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
+    const bool occluded = traverse_occluded(e.w.nodes, e.w.prims, org, dir, 0.f, s.a.w, ctx);
+    ++nr;
+    if (!occluded) {
+      const unsigned pid = __float_as_uint(s.b.w);
+      const int sidx = (int)(pid / (unsigned)e.numPixels);
+      const int pix = (int)(pid % (unsigned)e.numPixels);
+      accum_add(e.accum, pix, {s.c.x, s.c.y, s.c.z}, 10.f * (1 + e.accum0 + sidx));
     }
-  }
+  });
   wave_add(nr, &e.cnt->shadowRays);
 }
 
@@ -445,8 +566,8 @@ __global__ void k_fill_f4(float4 *p, size_t n) {
 // host side
 // ---------------------------------------------------------------------------
 struct TuneChoice {
-  int traceBlocksPerCU = 2;  // persistent trace/shadow grid
-  int chunk = 64;            // rays per work fetch
+  int traceBlocksPerCU = 0;  // trace/shadow workgroups per CU; 0 = as many as are resident
+  int chunk = 0;             // 0 = static 64-ray slices, else rays per dynamic fetch
   int sppBatch = 0;          // 0 = all paths of the frame in one batch
 };
 
@@ -456,7 +577,7 @@ struct BatchState {
   DevBuf<CameraDD> cams;
   DevBuf<PathRec> q0, q1;
   DevBuf<HitRec> hits;
-  DevBuf<unsigned> keys, perm;
+  DevBuf<unsigned> keys, perm, blockHist;
   DevBuf<ShadowRec> sq;
   DevBuf<WfCounters> cnt;
   DevBuf<unsigned long long> megaCnt;
@@ -483,6 +604,18 @@ int num_cus() {
     BN_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
   }
   return n;
+}
+
+// Trace/shadow grid: blocksPerCU workgroups per CU, capped at what the
+// occupancy of the two traversal kernels allows (static slices assume every
+// workgroup is resident).
+int trace_grid(int blocksPerCU, size_t lds) {
+  int occT = 0, occS = 0;
+  BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace, 256, lds));
+  BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow, 256, lds));
+  int occ = std::max(1, std::min(occT, occS));
+  if (blocksPerCU > 0) occ = std::min(occ, blocksPerCU);
+  return occ * num_cus();
 }
 
 struct HostViews {
@@ -560,8 +693,9 @@ void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds,
   const int gridStream = std::min(div_up(total, 256), 8 * num_cus());
   for (int d = 0; d <= maxDepth; ++d) {
     k_wf_trace<<<gridPersist, 256, lds, s>>>(e, d);
-    k_wf_sort_count<<<gridStream, 256, 0, s>>>(e, d);
-    k_wf_sort_scatter<<<gridStream, 256, 0, s>>>(e, d);
+    k_wf_sort_count<<<kSortBlocks, 256, 0, s>>>(e, d);
+    k_wf_sort_scan<<<1, kSortBlocks, 0, s>>>(e);
+    k_wf_sort_scatter<<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_shade<<<gridStream, 256, 0, s>>>(e, d);
     k_wf_shadow<<<gridPersist, 256, lds, s>>>(e, d);
   }
@@ -584,11 +718,12 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
   st.hits.ensure(cap);
   st.keys.ensure(cap);
   st.perm.ensure(cap);
+  st.blockHist.ensure(4 * kSortBlocks);
   st.sq.ensure(cap);
   st.cnt.ensure(1);
   const size_t lds = trace_lds_bytes(w.depth, 256);
   BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_batch: BVH too deep for the LDS stack");
-  const int gridPersist = std::max(1, tune.traceBlocksPerCU) * num_cus();
+  const int gridPersist = trace_grid(tune.traceBlocksPerCU, lds);
   if (rp0.accumID == 0)
     k_fill_f4<<<std::min(div_up(hv.numPixels, 256), 4096), 256, 0, s>>>(accum, (size_t)hv.numPixels);
   hipEvent_t e0, e1;
@@ -610,6 +745,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     e.hits = st.hits.ptr;
     e.keys = st.keys.ptr;
     e.perm = st.perm.ptr;
+    e.blockHist = st.blockHist.ptr;
     e.sq = st.sq.ptr;
     e.accum = accum;
     e.cnt = st.cnt.ptr;
@@ -724,8 +860,8 @@ extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views 
     long long px = 0;
     for (int i = 0; i < views->numViews; ++i) px += (long long)views->widths[i] * views->heights[i];
     DevBuf<float4> accum((size_t)px);
-    const int blocks[] = {1, 2, 4, 8};
-    const int chunks[] = {64, 128, 256};
+    const int blocks[] = {2, 3, 4, 0};
+    const int chunks[] = {0, 256, 1024};
     TuneChoice best;
     float bestMs = 1e30f;
     for (int b : blocks)
