@@ -176,20 +176,42 @@ BN_HD float smith_ggx(float n_dot_o, float alpha_g) {
 }
 BN_HD float luminance(vec3f c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
 BN_HD vec3f reflect(vec3f v, vec3f n) { return v - (2.f * dot(v, n)) * n; }
+// render/DG.h:121-128: the scalar eta*d is added to the vector sqrtf(k)*n
+// there (vec3f(eta*d) + sqrtf(k)*n); kept as written
+BN_HD vec3f refract(vec3f v, vec3f n, float eta) {
+#pragma clang fp contract(off)
+  const float d = dot(n, v);
+  const float k = 1.f - eta * eta * (1.f - d * d);
+  if (!(k >= 0.f)) return vec3f(0.f);
+  return eta * v - (vec3f(eta * d) + sqrtf(k) * n);
+}
+// sample_gtr_2_h (NVisii.h:186-194); sin_theta_h = 1 - cos^2 as written
+BN_HD vec3f sample_gtr_2_h(vec3f n, vec3f v_x, vec3f v_y, float alpha, float ux, float uy) {
+#pragma clang fp contract(off)
+  const float phi_h = 2.f * (float)M_PI * ux;
+  const float c2 = (1.f - uy) / (1.f + (alpha * alpha - 1.f) * uy);
+  const float cos_h = sqrtf(c2);
+  const float sin_h = 1.f - c2;
+  const vec3f hd = normalize(vec3f(sin_h * cosf(phi_h), sin_h * sinf(phi_h), cos_h));
+  return hd.x * v_x + hd.y * v_y + hd.z * n;
+}
 
 struct Mat {
-  vec3f base_color, subsurface_color;
-  float metallic, specular, roughness, specular_tint;
+  vec3f base_color;
+  float metallic, specular, roughness, specular_tint, ior, transmission_roughness;
 };
 
+// NVisii::setDefaults (NVisii.h:1040-1060, stored as halfs) with the
+// AnariPBR overrides; specular .5 and transmission roughness .04 as halfs
 BN_HD Mat unpack(const Bsdf &b) {
   Mat m;
   m.base_color = b.albedo;
-  m.subsurface_color = vec3f(.8f);
   m.metallic = b.metallic;
   m.specular = .5f;
   m.roughness = b.roughness;
   m.specular_tint = 0.f;
+  m.ior = b.ior;
+  m.transmission_roughness = 0.0400390625f;  // half(.04)
   return m;
 }
 
@@ -405,28 +427,41 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedi
     const float scale = 1.f / sum;
     dw *= scale;
     gw *= scale;
+    // lobe selection as written (:853-870); with clear coat weight 0 the
+    // clear-coat branch cannot be taken, and the transmission branch only
+    // when the normalized weights round to a sum below type_rng
     const float type_rng = rng();
-    const bool diffuse = type_rng < dw;
+    const int lobe = type_rng < dw ? 0 : ((type_rng - dw) < gw ? 1 : 3);
     const float ux = rng();
     const float uy = rng();
     vec3f w_i;
-    s.type = diffuse ? SCATTER_DIFFUSE : SCATTER_GLOSSY;
-    if (diffuse) {
+    s.type = lobe == 0 ? SCATTER_DIFFUSE : SCATTER_GLOSSY;
+    if (lobe == 0) {
       const vec3f h = normalize(nvisii::cos_sample_hemisphere(ux, uy));
       w_i = h.x * v_x + h.y * v_y + h.z * n;
-    } else {
+    } else if (lobe == 1) {
       const float alpha = fmaxf(BN_NV_MIN_ALPHA, m.roughness * m.roughness);
-      const float phi_h = 2.f * (float)M_PI * ux;
-      const float c2 = (1.f - uy) / (1.f + (alpha * alpha - 1.f) * uy);
-      const float cos_h = sqrtf(c2);
-      const float sin_h = 1.f - c2;  // as in NVisii.h:188 (no square root)
-      const vec3f hd = normalize(vec3f(sin_h * cosf(phi_h), sin_h * sinf(phi_h), cos_h));
-      const vec3f w_h = hd.x * v_x + hd.y * v_y + hd.z * n;
+      const vec3f w_h = nvisii::sample_gtr_2_h(n, v_x, v_y, alpha, ux, uy);
       w_i = nvisii::reflect(-wo, w_h);
       if (!nvisii::same_hemisphere(wo, w_i, n)) {
         s.pdf = 0.f;
         s.f_r = vec3f(0.f);
         s.dir = vec3f(0.f);
+        return s;
+      }
+    } else {
+      // DISNEY_TRANSMISSION_BRDF sampling (:955-973)
+      const float alpha = fmaxf(BN_NV_MIN_ALPHA, m.transmission_roughness * m.transmission_roughness);
+      const vec3f w_h = nvisii::sample_gtr_2_h(n, v_x, v_y, alpha, ux, uy);
+      const bool entering = dot(wo, w_h) > 0.f;
+      const float eta_i = entering ? 1.f : m.ior, eta_o = entering ? m.ior : 1.f;
+      const vec3f hn = entering ? w_h : -w_h;
+      w_i = nvisii::refract(-wo, hn, eta_i / eta_o);
+      if (w_i.x == 0.f && w_i.y == 0.f && w_i.z == 0.f) {
+        w_i = nvisii::reflect(-wo, hn);
+        s.pdf = 1.f;
+        s.f_r = vec3f(1.f);
+        s.dir = normalize(w_i);
         return s;
       }
     }

@@ -283,9 +283,12 @@ def workload3(grid=2236, spheres=200_000, cylinders=100_000, views=64, seed=0x5E
     hv = (256 + np.floor(uniform(seed, 101, views) * 769)).astype(int)
     px = uniform(seed, 102, views, 0.1 * ext, 0.9 * ext)
     pz = uniform(seed, 103, views, 0.1 * ext, 0.9 * ext)
-    py = hmax + uniform(seed, 104, views, 40.0, 300.0)
-    tx = px + uniform(seed, 105, views, -300.0, 300.0)
-    tz = pz + uniform(seed, 106, views, -300.0, 300.0)
+    # camera heights and look-at offsets scale with the terrain (1 at the
+    # full 2236^2 grid, so reduced test scenes still look at geometry)
+    sc = ext / 2235.0
+    py = hmax + uniform(seed, 104, views, 40.0, 300.0) * sc
+    tx = px + uniform(seed, 105, views, -300.0, 300.0) * sc
+    tz = pz + uniform(seed, 106, views, -300.0, 300.0) * sc
     for k in range(views):
         pos = (float(px[k]), float(py[k]), float(pz[k]))
         tgt = (float(tx[k]), float(0.5 * hmax), float(tz[k]))

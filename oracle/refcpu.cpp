@@ -124,6 +124,9 @@ inline bool better(float t, uint32_t key, const Best &b) { return t < b.t || (t 
 
 struct ref_scene_s {
   std::vector<ref_geom> geoms;
+  // triangles whose material is Glass: Glass::getOpacity returns 0 for
+  // shadow rays and the triangle anyHit ignores them (Triangles.dev.cu:119-133)
+  std::vector<uint8_t> shadowTransparent;
   std::vector<Prim> prims;  // reordered into BVH order
   struct Node {
     float lo[3], hi[3];
@@ -271,11 +274,16 @@ inline bool box_hit(const float *lo, const float *hi, const double *o, const dou
   return t0 <= t1 * (1.0 + 1e-9) + 1e-9;
 }
 
-Best closest(const ref_scene *sc, V3 org, V3 dir, float tmin, float tmax, bool brute, long long *tests) {
+Best closest(const ref_scene *sc, V3 org, V3 dir, float tmin, float tmax, bool brute, long long *tests,
+             bool shadowRay = false) {
   Best b;
   b.t = tmax; b.u = b.v = 0.f; b.key = 0xffffffffu; b.geom = -1; b.prim = -1;
+  auto test = [&](const Prim &p) {
+    if (shadowRay && sc->shadowTransparent[p.geom]) return;
+    test_prim(sc, p, org, dir, tmin, tmax, b);
+  };
   if (brute) {
-    for (const Prim &p : sc->prims) test_prim(sc, p, org, dir, tmin, tmax, b);
+    for (const Prim &p : sc->prims) test(p);
     if (tests) *tests += (long long)sc->prims.size();
     return b;
   }
@@ -289,7 +297,7 @@ Best closest(const ref_scene *sc, V3 org, V3 dir, float tmin, float tmax, bool b
     const auto &n = sc->nodes[stack[--sp]];
     if (!box_hit(n.lo, n.hi, o, id, tmin, b.t)) continue;
     if (n.left < 0) {
-      for (int i = 0; i < n.count; ++i) test_prim(sc, sc->prims[n.first + i], org, dir, tmin, tmax, b);
+      for (int i = 0; i < n.count; ++i) test(sc->prims[n.first + i]);
       if (tests) *tests += n.count;
     } else {
       stack[sp++] = n.left + 1;
@@ -450,6 +458,7 @@ struct PRay {
   float mis;
   int nDiff;
   bool shadow;
+  bool inMedium;
   V3 miss;
   float missA;
 };
@@ -472,6 +481,7 @@ PRay cam_ray(const Cam &c, const ref_render_params *p, int ix, int iy, int accum
   r.mis = 0.f;
   r.nDiff = 0;
   r.shadow = false;
+  r.inMedium = false;
   const float t = (iy + .5f) / float(p->height);
   if (p->bgColor[3] >= 0.f) {
     r.miss = ld3(p->bgColor);
@@ -483,15 +493,7 @@ PRay cam_ray(const Cam &c, const ref_render_params *p, int ix, int iy, int accum
   return r;
 }
 
-// ---- shading: shadeRays.cu bounce() with the env light -------------------
-struct HitInfo {
-  bool hit;
-  V3 P, N;
-  float t;
-  int type;  // 1 lambert, 2 phase
-  V3 albedo;
-};
-
+// ---- materials --------------------------------------------------------------
 V3 frame_xfm(V3 N, V3 l) {  // published embree/ospray frame(N)
   const V3 dx0 = mk(0.f, N.z, -N.y), dx1 = mk(-N.z, 0.f, N.x);
   const V3 dx = nrm(dot(dx0, dx0) > dot(dx1, dx1) ? dx0 : dx1);
@@ -499,10 +501,421 @@ V3 frame_xfm(V3 N, V3 l) {  // published embree/ospray frame(N)
   return add(add(scl(l.x, dx), scl(l.y, dy)), scl(l.z, N));
 }
 
+// float -> binary16 -> float with round-to-nearest-even: barney keeps the
+// NVisii parameters as halfs (NVisii.h:1062-1083).
+float half_round(float x) {
+  if (!std::isfinite(x) || x == 0.f) return x;
+  const float ax = std::fabs(x);
+  if (ax >= 65520.f) return std::copysign(kInf, x);
+  int e;
+  std::frexp(ax, &e);                      // ax in [2^(e-1), 2^e)
+  const float q = std::ldexp(1.f, std::max(e - 11, -24));  // 11 significant bits; subnormal step 2^-24
+  return std::copysign(std::nearbyint(ax / q) * q, x);
+}
+
+enum { M_LAMBERT = 1, M_PHASE = 2, M_NVISII = 3, M_GLASS = 4 };
+
+// what Ray::setHit leaves for bounce(): point, geometric normal, BSDF
+struct HitInfo {
+  bool hit;
+  V3 P, N;
+  float t;
+  int type;
+  V3 albedo;       // Lambert/phase albedo, NVisii base color
+  float metallic, roughness, ior;
+};
+
+// DeviceMaterial::createBSDF for one geometry's material at one primitive:
+// AnariMatte.h:51-63 (albedo .85 color / pi), AnariPBR.h:68-94 (Glass or NVisii).
+void material_hit(const ref_geom &g, V3 col, HitInfo &h) {
+  if (g.matType == 1) {
+    if (g.ior != 1.f && g.transmission >= 1e-3f) {
+      h.type = M_GLASS;
+      h.albedo = mk(1, 1, 1);
+      h.ior = g.ior;
+      h.metallic = h.roughness = 0.f;
+      return;
+    }
+    h.type = M_NVISII;
+    h.albedo = mk(half_round(col.x), half_round(col.y), half_round(col.z));
+    h.metallic = half_round(g.metallic);
+    h.roughness = half_round(std::fmin(std::fmax(g.roughness, .001f), 1.f - .1f));
+    h.ior = half_round(g.ior);
+    return;
+  }
+  h.type = M_LAMBERT;
+  h.albedo = mk(.85f * col.x * (1.f / kPi), .85f * col.y * (1.f / kPi), .85f * col.z * (1.f / kPi));
+  h.metallic = h.roughness = h.ior = 0.f;
+}
+
+// ---- NVisii (barney/packedBSDF/NVisii.h), all lobes as written there ------
+namespace nv {
+const float kMinAlpha = .002f, kSmallEps = 2e-10f;
+const float k1Pi = (float)M_1_PI;
+struct Mat {
+  V3 base, subsurf;
+  float metallic, specular, roughness, specTint, aniso, sheen, sheenTint, clearcoat, clearcoatGloss, ior,
+      specTrans, transRough, flatness;
+};
+// NVisii::setDefaults (NVisii.h:1040-1060) stored as halfs, then the
+// AnariPBR overrides (base color, metallic, roughness, ior)
+Mat unpack(const HitInfo &h) {
+  Mat m;
+  m.base = h.albedo;
+  const float ss = half_round(.8f);
+  m.subsurf = mk(ss, ss, ss);
+  m.metallic = h.metallic;
+  m.specular = half_round(.5f);
+  m.roughness = h.roughness;
+  m.specTint = 0.f;
+  m.aniso = 0.f;
+  m.sheen = 0.f;
+  m.sheenTint = half_round(.5f);
+  m.clearcoat = 0.f;
+  m.clearcoatGloss = half_round(1.f - .03f * .03f);
+  m.ior = h.ior;
+  m.specTrans = 0.f;
+  m.transRough = half_round(.04f);
+  m.flatness = 0.f;
+  return m;
+}
+float saturate(float f) { return std::fmax(0.f, std::fmin(f, 1.f)); }
+float lerp_r(float a, float b, float t) { return (1.f - t) * a + t * b; }
+V3 lerp_r(V3 a, V3 b, float t) { return add(scl(1.f - t, a), scl(t, b)); }
+float schlick(float c) { return std::pow(saturate(1.f - c), 5.f); }
+bool same_hemi(V3 wo, V3 wi, V3 n) { return dot(wo, n) * dot(wi, n) > 0.f; }
+float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }  // DG.h:109
+V3 reflect(V3 v, V3 n) { return sub(v, scl(2.f * dot(v, n), n)); }                    // DG.h:115
+// DG.h:121-128; '(eta*d + sqrtf(k)*n)' adds a scalar to a vector there,
+// i.e. vec3f(eta*d) + sqrtf(k)*n, which is kept
+V3 refract(V3 v, V3 n, float eta) {
+  const float d = dot(n, v);
+  const float k = 1.f - eta * eta * (1.f - d * d);
+  if (!(k >= 0.f)) return mk(0, 0, 0);
+  const float ed = eta * d;
+  return sub(scl(eta, v), add(mk(ed, ed, ed), scl(std::sqrt(k), n)));
+}
+float gtr_1(float c, float a) {  // :120-128
+  if (a >= 1.f) return k1Pi;
+  const float a2 = a * a;
+  return k1Pi * (a2 - 1.f) / (std::log(a2) * (1.f + (a2 - 1.f) * c * c));
+}
+float gtr_2(float c, float a) {  // :132-141 (no a^2 numerator)
+  const float a2 = a * a;
+  const float den1 = 1.f + (a2 - 1.f) * c * c;
+  const float den2 = std::fmax(den1, kSmallEps);
+  return k1Pi * 1.f / den2;
+}
+float smith(float n_dot_o, float a_g) {  // :150-154
+  const float a = a_g * a_g, b = n_dot_o * n_dot_o;
+  return 1.f / (n_dot_o + std::sqrt(a + b - a * b));
+}
+float lambertian_pdf(V3 wi, V3 n) {  // :204-210
+  const float d = dot(wi, n);
+  return d > 0.f ? d * k1Pi : 0.f;
+}
+float gtr_1_pdf(V3 wo, V3 wi, V3 wh, V3 n, float a) {  // :213-220
+  if (!same_hemi(wo, wi, n)) return 0.f;
+  const float c = dot(n, wh);
+  return gtr_1(c, a) * c / (4.f * dot(wo, wh));
+}
+float gtr_2_pdf(V3 wo, V3 wi, V3 wh, V3 n, float a) {  // :223-230
+  if (!same_hemi(wo, wi, n)) return 0.f;
+  const float c = std::fabs(dot(n, wh));
+  return gtr_2(c, a) * c / (4.f * std::fabs(dot(wo, wh)));
+}
+// cos_sample_hemisphere (:73-91): concentric disk
+V3 cos_hemi(float ux, float uy) {
+  const float sx = 2.f * ux - 1.f, sy = 2.f * uy - 1.f;
+  float radius = 0.f, theta = 0.f;
+  if (!(sx == 0.f && sy == 0.f)) {
+    if (std::fabs(sx) > std::fabs(sy)) {
+      radius = sx;
+      theta = (float)M_PI / 4.f * (sy / sx);
+    } else {
+      radius = sy;
+      theta = (float)M_PI / 2.f - (float)M_PI / 4.f * (sx / sy);
+    }
+  }
+  const float dx = radius * std::cos(theta), dy = radius * std::sin(theta);
+  return mk(dx, dy, std::sqrt(std::fmax(0.f, 1.f - dx * dx - dy * dy)));
+}
+V3 on_frame(V3 h, V3 vx, V3 vy, V3 n) { return add(add(scl(h.x, vx), scl(h.y, vy)), scl(h.z, n)); }
+// sample_gtr_2_h (:186-194); sin_theta_h = 1 - cos^2 as written
+V3 sample_gtr_2_h(V3 n, V3 vx, V3 vy, float a, float sx, float sy) {
+  const float phi = 2.f * (float)M_PI * sx;
+  const float c2 = (1.f - sy) / (1.f + (a * a - 1.f) * sy);
+  const float ch = std::sqrt(c2), sh = 1.f - c2;
+  const V3 hd = nrm(mk(sh * std::cos(phi), sh * std::sin(phi), ch));
+  return on_frame(hd, vx, vy, n);
+}
+// disney_brdf (:655-700) with the isotropic microfacet lobe (:480-503)
+V3 brdf(const Mat &m, V3 n, V3 wo, V3 wi, V3 wh) {
+  const float n_dot_o = std::fabs(dot(wo, n)), n_dot_i = std::fabs(dot(wi, n));
+  const float i_dot_h = dot(wi, wh);
+  // clear coat (:605-613)
+  const float cca = lerp_r(0.1f, kMinAlpha, m.clearcoatGloss);
+  const float coat = m.clearcoat * gtr_1(std::fabs(dot(n, wh)), cca) * lerp_r(0.04f, 1.f, schlick(dot(wi, n))) *
+                     (smith(std::fabs(dot(n, wi)), 0.25f) * smith(std::fabs(dot(n, wo)), 0.25f));
+  // sheen (:617-624)
+  const float lum = luminance(m.base);
+  const V3 tint = lum > 0.f ? scl(1.f / lum, m.base) : mk(1, 1, 1);
+  const V3 sheen = scl(schlick(dot(wi, n)) * m.sheen, lerp_r(mk(1, 1, 1), tint, m.sheenTint));
+  // diffuse (:338-352, no 1/pi)
+  const float fd90 = 0.5f + 1.0f * m.roughness * i_dot_h * i_dot_h;
+  const float diff = lerp_r(1.f, fd90, schlick(n_dot_i)) * lerp_r(1.f, fd90, schlick(n_dot_o));
+  // subsurface (:355-367)
+  const float FL = schlick(n_dot_i), FV = schlick(n_dot_o);
+  const float Fss90 = i_dot_h * i_dot_h * m.roughness;
+  const float Fss = lerp_r(1.0f, Fss90, FL) * lerp_r(1.0f, Fss90, FV);
+  const float ssv = k1Pi * (1.25f * (Fss * (1.f / (n_dot_i + n_dot_o) - .5f) + .5f));
+  // microfacet (:480-503)
+  const V3 spec = lerp_r(scl(m.specular * 0.08f, lerp_r(mk(1, 1, 1), tint, m.specTint)), m.base, m.metallic);
+  const float alpha = std::fmax(kMinAlpha, m.roughness * m.roughness);
+  const float d = gtr_2(std::fabs(dot(n, wh)), alpha);
+  const float fw = schlick(std::fabs(dot(wi, wh))) * lerp_r(.5f, 1.f, std::fmax(m.metallic, alpha));
+  const V3 f = lerp_r(spec, mk(1, 1, 1), fw);
+  const float g = smith(std::fabs(dot(n, wi)), alpha) * smith(std::fabs(dot(n, wo)), alpha);
+  const V3 gloss = scl(g, scl(d, f));
+  const V3 flat = lerp_r(scl(diff, m.base), scl(ssv, m.subsurf), m.flatness);
+  V3 r = scl((1.f - m.specTrans), scl((1.f - m.metallic), flat));
+  r = add(r, sheen);
+  r = add(r, mk(coat, coat, coat));
+  return add(r, gloss);
+}
+// disney_pdf (:717-810), IMPORTANCE_SAMPLE_BRDF weights, unnormalized
+float pdf(const Mat &m, V3 n, V3 wo, V3 wi, V3 wh) {
+  const float dw = (1.f - m.metallic) * (1.f - m.specTrans);
+  const float gw = 1.f + m.metallic + m.sheen + m.specular + m.roughness;
+  const float cw = m.clearcoat, tw = 0.f;
+  if (dw + gw + cw + tw == 0.f) return 0.f;
+  const float alpha = std::fmax(kMinAlpha, m.roughness * m.roughness);
+  const float cca = lerp_r(0.1f, kMinAlpha, m.clearcoatGloss);
+  const float diffuse = lambertian_pdf(wi, n);
+  const float cc = gtr_1_pdf(wo, wi, wh, n, cca);
+  const float mf = gtr_2_pdf(wo, wi, wh, n, alpha);
+  return dw * diffuse + gw * mf + cw * cc + tw * 0.f;
+}
+}  // namespace nv
+
+// ---- Glass: RobustDielectric::sample (barney/packedBSDF/Glass.h:85-190) -----
+V3 glass_sample(float eta, V3 wo, V3 Ns, V3 Ng, Rng &rnd, bool &transmit) {
+  const V3 incoming = neg(wo);
+  const float geps = 1.e-6f;
+  const bool canReflect = (eta < 1.f - geps || eta > 1.f + geps);
+  if (dot(incoming, Ns) > 0.f) Ns = neg(Ns);
+  if (dot(incoming, Ng) > 0.f) Ng = neg(Ng);
+  // reflectionDirection (:72-80)
+  const V3 par = scl(dot(incoming, Ns), Ns);
+  const V3 dirR = sub(sub(incoming, par), par);
+  // refractionDirection (:20-68)
+  V3 dirT;
+  float fres;
+  bool mustReflect = false;
+  {
+    const float cos1 = dot(incoming, Ns);
+    float c1a = cos1;
+    if (c1a < 0.0) c1a = -c1a;
+    const bool headOn = (1.f - 1e-6f < c1a && c1a < 1.f - 1e-6f);
+    if (headOn) {
+      const float q = (eta - 1.f) / (eta + 1.f);
+      fres = q * q;
+      dirT = incoming;
+    } else {
+      const V3 dpar = scl(cos1, Ns);
+      const V3 dperp = sub(incoming, dpar);
+      const V3 b = nrm(dperp);
+      const float sin1 = dot(dperp, b);
+      const float sin2 = sin1 / eta;
+      if (sin2 > 1.f) {
+        dirT = mk(0, 0, 0);
+        fres = 1.f;
+        mustReflect = true;
+      } else {
+        const float cos2 = std::sqrt(std::fmin(std::fmax(1.f - sin2 * sin2, 0.f), 1.f));
+        const V3 N2 = nrm(dpar);
+        dirT = add(scl(cos2, N2), scl(sin2, b));
+        const float r1 = (eta * c1a - cos2) / (eta * c1a + cos2);
+        const float r2 = (c1a - eta * cos2) / (c1a + eta * cos2);
+        fres = 0.5f * (r1 * r1 + r2 * r2);
+      }
+    }
+    fres = std::fmin(std::fmax(fres, 0.f), 1.f);
+  }
+  V3 wi;
+  if ((rnd.uni() < fres && canReflect) || mustReflect) {
+    wi = dirR;
+    transmit = false;
+    const float gc = dot(wi, Ng);
+    if (gc >= 0.f) {
+      if (gc < geps) wi = nrm(add(wi, scl(geps, Ng)));
+    } else {
+      wi = sub(wi, scl(gc, scl(2.f, Ng)));
+    }
+  } else {
+    wi = dirT;
+    transmit = true;
+    const float gc = dot(wi, Ng);
+    if (gc <= 0.f) {
+      if (gc > -geps) wi = nrm(sub(wi, scl(geps, Ng)));
+    } else {
+      transmit = false;
+    }
+  }
+  return wi;
+}
+
+enum { S_DIFFUSE = 0, S_GLOSSY = 1, S_SPECULAR = 2, S_VOLUME = 3 };
+
+// BSDF::eval / pdf / scatter dispatch (PackedBSDF.h), dg.Ns = dg.Ng = Ng
+float bsdf_eval(const HitInfo &h, V3 Ng, V3 wo, V3 wi, V3 &val) {
+  if (h.type == M_LAMBERT) {  // Lambertian.h:55-64
+    const float c = std::fmax(dot(wi, Ng), 0.f);
+    val = mk(h.albedo.x * (1.f / kPi) * c, h.albedo.y * (1.f / kPi) * c, h.albedo.z * (1.f / kPi) * c);
+    return c * (1.f / kPi);
+  }
+  if (h.type == M_NVISII) {  // NVisii.h:1167-1195
+    const nv::Mat m = nv::unpack(h);
+    const V3 wh = nrm(add(wi, wo));
+    val = nv::brdf(m, Ng, wo, wi, wh);
+    return nv::pdf(m, Ng, wo, wi, wh);
+  }
+  if (h.type == M_GLASS) {
+    val = mk(0, 0, 0);
+    return 0.f;
+  }
+  const float d = 1.f / (4.f * kPi);  // Phase.h:120-141
+  val = scl(d, h.albedo);
+  return d;
+}
+
+float bsdf_pdf(const HitInfo &h, V3 Ng, V3 wo, V3 wi) {
+  if (h.type == M_LAMBERT) {
+    V3 N = Ng;
+    if (dot(wi, N) < 0.f) N = neg(N);
+    return std::fmax(dot(wi, N), 0.f) * (1.f / kPi);
+  }
+  if (h.type == M_NVISII) {
+    const nv::Mat m = nv::unpack(h);
+    return nv::pdf(m, Ng, wo, wi, nrm(add(wi, wo)));
+  }
+  if (h.type == M_GLASS) return 0.f;
+  return 1.f / (4.f * kPi);
+}
+
+struct Scat {
+  V3 fr, dir;
+  float pdf;
+  int type;
+  float offDir;
+  bool medium;
+};
+
+Scat bsdf_scatter(const HitInfo &h, V3 Ng, V3 wo, bool inMedium, Rng &rnd) {
+  Scat s;
+  s.offDir = 1.f;
+  s.medium = false;
+  if (h.type == M_LAMBERT) {  // Lambertian.h:66-87, DG.h:160-166
+    V3 N = Ng;
+    if (dot(wo, N) < 0.f) N = neg(N);
+    const float sx = rnd.uni(), sy = rnd.uni();
+    const float phi = (2.f * kPi) * sx;
+    const float ct = std::sqrt(sy), st = std::sqrt(1.0f - sy);
+    const V3 l = mk(std::cos(phi) * st, std::sin(phi) * st, ct);
+    s.dir = frame_xfm(N, l);
+    s.pdf = l.z * (1.f / kPi);
+    s.fr = h.albedo;
+    s.type = S_DIFFUSE;
+  } else if (h.type == M_NVISII) {  // NVisii.h:1092-1163 + sample_disney_brdf :820-975
+    const nv::Mat m = nv::unpack(h);
+    const V3 n = Ng;
+    const V3 vy = nrm(cross(Ng, wo)), vx = nrm(cross(Ng, vy));
+    float dw = (1.f - m.metallic) * (1.f - m.specTrans);
+    float gw = 1.f + m.metallic + m.sheen + m.specular + m.roughness;
+    float cw = m.clearcoat, tw = 0.f;
+    const float sum = dw + gw + cw + tw;
+    if (sum == 0.f) {
+      s.pdf = 0.f;
+      s.fr = s.dir = mk(0, 0, 0);
+      s.type = S_GLOSSY;
+      return s;
+    }
+    const float sc = 1.f / sum;
+    dw *= sc; gw *= sc; cw *= sc; tw *= sc;
+    const float tr = rnd.uni();
+    int lobe;
+    if (tr < dw) lobe = 0;
+    else if ((tr - dw) < gw) lobe = 1;
+    else if ((tr - dw - gw) < cw) lobe = 2;
+    else lobe = 3;
+    const float ux = rnd.uni(), uy = rnd.uni();
+    V3 wi;
+    s.type = lobe == 0 ? S_DIFFUSE : S_GLOSSY;
+    if (lobe == 0) {
+      wi = nv::on_frame(nrm(nv::cos_hemi(ux, uy)), vx, vy, n);
+    } else if (lobe == 1) {
+      const float alpha = std::fmax(nv::kMinAlpha, m.roughness * m.roughness);
+      const V3 wh = nv::sample_gtr_2_h(n, vx, vy, alpha, ux, uy);
+      wi = nv::reflect(neg(wo), wh);
+      if (!nv::same_hemi(wo, wi, n)) {
+        s.pdf = 0.f;
+        s.fr = s.dir = mk(0, 0, 0);
+        return s;
+      }
+    } else if (lobe == 2) {
+      // clear coat lobe: weight 0 here (clearcoat = 0), never selected
+      s.pdf = 0.f;
+      s.fr = s.dir = mk(0, 0, 0);
+      return s;
+    } else {
+      // transmission lobe: reachable only when the normalized weights
+      // round to a sum below the drawn number
+      const float alpha = std::fmax(nv::kMinAlpha, m.transRough * m.transRough);
+      const V3 wh = nv::sample_gtr_2_h(n, vx, vy, alpha, ux, uy);
+      const bool entering = dot(wo, wh) > 0.f;
+      const float eta_i = entering ? 1.f : m.ior, eta_o = entering ? m.ior : 1.f;
+      const V3 hn = entering ? wh : neg(wh);
+      wi = nv::refract(neg(wo), hn, eta_i / eta_o);
+      if (wi.x == 0.f && wi.y == 0.f && wi.z == 0.f) {
+        wi = nv::reflect(neg(wo), hn);
+        s.pdf = 1.f;
+        s.fr = mk(1, 1, 1);
+        s.dir = nrm(wi);
+        return s;
+      }
+    }
+    const V3 wh = nrm(add(wi, wo));
+    s.pdf = nv::pdf(m, n, wo, wi, wh);
+    s.fr = nv::brdf(m, n, wo, wi, wh);
+    s.dir = nrm(wi);
+  } else if (h.type == M_GLASS) {  // Glass.h:219-241
+    const float eta = !inMedium ? 1.f / h.ior : h.ior;
+    bool tr = false;
+    s.dir = glass_sample(1.f / eta, wo, Ng, Ng, rnd, tr);
+    s.fr = mk(1, 1, 1);
+    s.pdf = kInf;
+    s.offDir = tr ? -1.f : 1.f;
+    s.medium = tr;
+    s.type = S_SPECULAR;
+  } else {  // Phase.h:143-172
+    const float r1 = rnd.uni(), r2 = rnd.uni();
+    const float x = std::cos(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
+    const float y = std::sin(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
+    const float z = (1.f - 2.f * r2);
+    s.pdf = 1.f / (4.f * kPi);
+    s.fr = scl(s.pdf, h.albedo);
+    s.dir = mk(x, y, z);
+    s.type = S_VOLUME;
+  }
+  return s;
+}
+
+// shadeRays.cu:476-878 with only the environment light present
 V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int depth, int maxDepth, int maxDiff) {
   sh.tMax = -1.f;
   V3 frag = mk(0, 0, 0);
   const V3 tp0 = ray.tp;
+  const float amb = p->ambientRadiance;
   if (ray.shadow) {
     if (!h.hit) frag = scl(ray.mis, ray.tp);
     ray.tMax = -1.f;
@@ -513,8 +926,7 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
   if (!vol) Ng = nrm(Ng);
   const V3 Ngff = dot(ray.dir, Ng) < 0.f ? Ng : neg(Ng);
   if (!h.hit) {
-    frag = depth == 0 ? ray.miss : scl(ray.mis, mul(ray.tp, mk(p->ambientRadiance, p->ambientRadiance, p->ambientRadiance)));
-    if (depth != 0) frag = scl(ray.mis, mul(ray.tp, mk(p->ambientRadiance, p->ambientRadiance, p->ambientRadiance)));
+    frag = depth == 0 ? ray.miss : scl(ray.mis, mul(ray.tp, mk(amb, amb, amb)));
     ray.tMax = -1.f;
     return frag;
   }
@@ -535,11 +947,11 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
   V3 lrad;
   float lpdf;
   if (!eq(Ngff, mk(0, 0, 0))) {
-    lrad = mk(2.f * kPi * p->ambientRadiance, 2.f * kPi * p->ambientRadiance, 2.f * kPi * p->ambientRadiance);
+    lrad = mk(2.f * kPi * amb, 2.f * kPi * amb, 2.f * kPi * amb);
     if (dot(ld, Ngff) < 0.f) ld = neg(ld);
     lpdf = 1.f / (2.f * kPi);
   } else {
-    const float r4 = 4.f * kPi * p->ambientRadiance;
+    const float r4 = 4.f * kPi * amb;
     lrad = mk(r4, r4, r4);
     lpdf = 1.f / (4.f * kPi);
   }
@@ -555,28 +967,9 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
       have = !(lpdf <= 0.f) && !std::isnan(lpdf);
     }
   }
-  auto eval = [&](V3 wi, V3 &val) -> float {
-    if (h.type == 1) {
-      const float c = std::fmax(dot(wi, Ng), 0.f);
-      val = scl(c, scl(1.f / kPi, h.albedo));
-      val = mk(h.albedo.x * (1.f / kPi) * c, h.albedo.y * (1.f / kPi) * c, h.albedo.z * (1.f / kPi) * c);
-      return c * (1.f / kPi);
-    }
-    const float d = 1.f / (4.f * kPi);
-    val = scl(d, h.albedo);
-    return d;
-  };
-  auto pdf = [&](V3 wi) -> float {
-    if (h.type == 1) {
-      V3 N = Ng;
-      if (dot(wi, N) < 0.f) N = neg(N);
-      return std::fmax(dot(wi, N), 0.f) * (1.f / kPi);
-    }
-    return 1.f / (4.f * kPi);
-  };
   if (have) {
     V3 fv;
-    const float fp = eval(ld, fv);
+    const float fp = bsdf_eval(h, Ng, wo, ld, fv);
     if (!(fp > 0.f) || maxc(fv) < 1e-4f) {
       sh.tMax = -1.f;
     } else {
@@ -593,9 +986,10 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
       ray.rng.mix((uint64_t)fbits(h.t));
       sh.mis = 1.f;
       sh.nDiff = ray.nDiff;
+      sh.inMedium = ray.inMedium;
       if (needMIS) {
         const float pl = 1.f / (4.f * kPi);
-        const float pb = pdf(ld);
+        const float pb = bsdf_pdf(h, Ng, wo, ld);
         sh.mis = pl / (pl + pb + 1e-10f);
         if (sh.mis < 1e-5f) sh.tMax = -1.f;
       }
@@ -606,50 +1000,40 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
     return frag;
   }
   ray.tMax = kInf;
-  V3 sdir, fr;
-  float spdf;
-  bool diffuse;
-  if (h.type == 1) {
-    V3 N = Ng;
-    if (dot(wo, N) < 0.f) N = neg(N);
-    const float sx = rnd.uni(), sy = rnd.uni();
-    const float phi = (2.f * kPi) * sx;
-    const float ct = std::sqrt(sy), st = std::sqrt(1.0f - sy);
-    const V3 l = mk(std::cos(phi) * st, std::sin(phi) * st, ct);
-    sdir = frame_xfm(N, l);
-    spdf = l.z * (1.f / kPi);
-    fr = h.albedo;
-    diffuse = true;
-  } else {
-    (void)rnd.uni();
-    (void)rnd.uni();
-    spdf = 1.f / (4.f * kPi);
-    fr = scl(spdf, h.albedo);
-    sdir = mk(0, 0, 1);
-    diffuse = false;
-  }
-  if (!(spdf > 0.f) || spdf <= 1e-6f) return frag;
-  if (!diffuse) {
+  const Scat sr = bsdf_scatter(h, Ng, wo, ray.inMedium, rnd);
+  if (!(sr.pdf > 0.f) || sr.pdf <= 1e-6f) return frag;
+  if (sr.type == S_VOLUME) {  // SCI_VIS_MODE
     ray.tMax = -1.f;
     return frag;
   }
-  if (ray.nDiff >= maxDiff) {
-    ray.tMax = -1.f;
-    return frag;
+  if (sr.type == S_DIFFUSE) {
+    if (ray.nDiff >= maxDiff) {
+      ray.tMax = -1.f;
+      return frag;
+    }
+    ray.nDiff += 1;
   }
-  ray.nDiff += 1;
-  ray.org = add(P, scl(1.f * eps, off));
-  ray.dir = nrm(sdir);
+  ray.org = add(P, scl(sr.offDir * eps, off));
+  ray.dir = nrm(sr.dir);
   const float cosf_ = vol ? 1.f : std::fabs(dot(Ng, nrm(ray.dir)));
-  const float den = std::isinf(spdf) ? 1.f : (spdf + 1e-10f);
-  V3 sf = mk(fr.x * cosf_ / den, fr.y * cosf_ / den, fr.z * cosf_ / den);
+  const float den = std::isinf(sr.pdf) ? 1.f : (sr.pdf + 1e-10f);
+  V3 sf = mk(sr.fr.x * cosf_ / den, sr.fr.y * cosf_ / den, sr.fr.z * cosf_ / den);
   sf = vmins(sf, 2.5f);
   ray.tp = mul(ray.tp, sf);
-  if (needMIS && !std::isinf(spdf)) {
-    const float pb = pdf(ray.dir), pl = 1.f / (4.f * kPi);
+  if (sr.medium) ray.inMedium = !ray.inMedium;
+  if (needMIS && !std::isinf(sr.pdf)) {
+    const float pb = bsdf_pdf(h, Ng, wo, ray.dir), pl = 1.f / (4.f * kPi);
     ray.mis = pb / (pb + pl);
   } else {
     ray.mis = 1.f;
+  }
+  if (p->russianRoulette > 0 && depth + 1 >= p->russianRoulette) {
+    const float q = std::fmin(maxc(ray.tp), 0.95f);
+    if (rnd.uni() >= q) {
+      ray.tMax = -1.f;
+      return frag;
+    }
+    ray.tp = scl(1.f / q, ray.tp);
   }
   return frag;
 }
@@ -687,6 +1071,41 @@ void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n) {
 }
 uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c) { return fnv3(a, b, c); }
 
+float ref_half_round(float x) { return half_round(x); }
+
+// BSDF probes for the oracle's own tests: the material of 'g' at 'color',
+// dg.Ng = dg.Ns = Ng. Sample i draws from Rng::seed(i, seed).
+void ref_bsdf_sample(const ref_geom *g, const float *color, const float *Ng, const float *wo, int inMedium,
+                     uint32_t seed, int n, float *out) {
+  HitInfo h;
+  h.hit = true;
+  material_hit(*g, ld3(color), h);
+  for (int i = 0; i < n; ++i) {
+    Rng r;
+    r.seed((uint32_t)i, seed);
+    const Scat sc = bsdf_scatter(h, ld3(Ng), ld3(wo), inMedium != 0, r);
+    float *o = out + 10 * i;
+    o[0] = sc.dir.x; o[1] = sc.dir.y; o[2] = sc.dir.z;
+    o[3] = sc.fr.x; o[4] = sc.fr.y; o[5] = sc.fr.z;
+    o[6] = sc.pdf;
+    o[7] = (float)sc.type;
+    o[8] = sc.offDir;
+    o[9] = sc.medium ? 1.f : 0.f;
+  }
+}
+
+// out: eval value[3], eval pdf, BSDF::pdf
+void ref_bsdf_eval(const ref_geom *g, const float *color, const float *Ng, const float *wo, const float *wi,
+                   float *out) {
+  HitInfo h;
+  h.hit = true;
+  material_hit(*g, ld3(color), h);
+  V3 v;
+  out[3] = bsdf_eval(h, ld3(Ng), ld3(wo), ld3(wi), v);
+  out[0] = v.x; out[1] = v.y; out[2] = v.z;
+  out[4] = bsdf_pdf(h, ld3(Ng), ld3(wo), ld3(wi));
+}
+
 // Volume.h:174-184
 float ref_fast_log(float f) {
   f = (f - 1.f) / (f + 1.f);
@@ -702,6 +1121,9 @@ float ref_fast_log(float f) {
 ref_scene *ref_scene_create(const ref_geom *geoms, int numGeoms) {
   auto *s = new ref_scene_s;
   s->geoms.assign(geoms, geoms + numGeoms);
+  for (int g = 0; g < numGeoms; ++g)
+    s->shadowTransparent.push_back(geoms[g].kind == 0 && geoms[g].matType == 1 && geoms[g].ior != 1.f &&
+                                   geoms[g].transmission >= 1e-3f);
   uint32_t key = 0;
   for (int g = 0; g < numGeoms; ++g) {
     const ref_geom &gd = geoms[g];
@@ -814,11 +1236,9 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
         if (hi.hit) {
           hit_point_normal(s, b, ray.org, d, 0.f, ray.tMax, hi.P, hi.N);
           hi.t = b.t;
-          hi.type = 1;
           const ref_geom &g = s->geoms[b.geom];
           const V3 col = g.colors ? ld3(g.colors + 3 * b.prim) : ld3(g.color);
-          hi.albedo = scl(1.f / kPi, scl(.85f, col));
-          hi.albedo = mk(.85f * col.x * (1.f / kPi), .85f * col.y * (1.f / kPi), .85f * col.z * (1.f / kPi));
+          material_hit(g, col, hi);
         }
         if (gen == 0) {
           alpha = hi.hit ? 1.f : ray.missA;
@@ -840,7 +1260,7 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
         PRay sh;
         add_frag(sum, bounce(p, ray, hi, sh, gen, maxDepth, maxDiff), cmax);
         if (sh.tMax > 0.f) {
-          const Best bs = closest(s, sh.org, fix_dir(sh.dir), 0.f, sh.tMax, false, nullptr);
+          const Best bs = closest(s, sh.org, fix_dir(sh.dir), 0.f, sh.tMax, false, nullptr, true);
           ++nr;
           HitInfo hs;
           hs.hit = bs.geom >= 0;

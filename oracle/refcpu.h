@@ -36,8 +36,12 @@ typedef struct {
   const float *radii;     /* or NULL */
   float defaultRadius;
   const float *colors;    /* per-prim float3 or NULL */
-  float color[3];         /* matte color */
+  float color[3];         /* material color (matte color / PBR base color) */
   int userID;
+  /* material: 0 AnariMatte, 1 AnariPBR (Glass when ior != 1 and
+   * transmission >= 1e-3, else NVisii; AnariPBR.h:68-94) */
+  int matType;
+  float metallic, roughness, transmission, ior;
 } ref_geom;
 
 typedef struct ref_scene_s ref_scene;
@@ -63,6 +67,19 @@ void ref_trace(const ref_scene *s, const float *rays, ref_hit *hits, int n, int 
 double ref_triangle_t_f64(const double *org, const double *dir, const double *v0, const double *v1,
                           const double *v2);
 
+/* float -> IEEE binary16 -> float, round to nearest even */
+float ref_half_round(float x);
+
+/* BSDF probes (materials as in ref_geom; color = material color). Sample
+ * i uses RNG seed(i, seed); out per sample: dir[3], f_r[3], pdf, scatter
+ * type (0 diffuse, 1 glossy, 2 specular, 3 volume), offset direction,
+ * changedMedium. */
+void ref_bsdf_sample(const ref_geom *mat, const float *color, const float *Ng, const float *wo, int inMedium,
+                     uint32_t seed, int n, float *out);
+/* out: eval value[3], eval pdf, pdf() */
+void ref_bsdf_eval(const ref_geom *mat, const float *color, const float *Ng, const float *wo, const float *wi,
+                   float *out);
+
 /* --- camera + surface render (generateRays.cu, shadeRays.cu bounce) --- */
 typedef struct {
   float position[3], direction[3], up[3];
@@ -74,6 +91,9 @@ typedef struct {
   float ambientRadiance;
   float bgColor[4];
   int maxPathDepth, maxDiffuseBounces;
+  /* Russian roulette from this path depth on (0 = off; not in barney):
+   * survive with p = min(max throughput, 0.95), throughput /= p */
+  int russianRoulette;
 } ref_render_params;
 
 /* primary ray for pixel (ix,iy): out = org[3], dir[3], seed (as 2 floats bits) */

@@ -23,7 +23,8 @@ def build():
 class ref_geom(C.Structure):
     _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
                 ("radii", C.c_void_p), ("defaultRadius", C.c_float), ("colors", C.c_void_p),
-                ("color", C.c_float * 3), ("userID", C.c_int)]
+                ("color", C.c_float * 3), ("userID", C.c_int), ("matType", C.c_int), ("metallic", C.c_float),
+                ("roughness", C.c_float), ("transmission", C.c_float), ("ior", C.c_float)]
 
 
 class ref_hit(C.Structure):
@@ -39,7 +40,7 @@ class ref_camera(C.Structure):
 class ref_render_params(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int), ("pathsPerPixel", C.c_int),
                 ("ambientRadiance", C.c_float), ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
-                ("maxDiffuseBounces", C.c_int)]
+                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int)]
 
 
 _lib = None
@@ -59,6 +60,9 @@ def load():
         "ref_random2_floats": (None, [C.c_uint32, C.c_uint32, P, C.c_int]),
         "ref_hash3": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "ref_fast_log": (C.c_float, [C.c_float]),
+        "ref_half_round": (C.c_float, [C.c_float]),
+        "ref_bsdf_sample": (None, [P, P, P, P, C.c_int, C.c_uint32, C.c_int, P]),
+        "ref_bsdf_eval": (None, [P, P, P, P, P, P]),
         "ref_scene_create": (P, [P, C.c_int]),
         "ref_scene_destroy": (None, [P]),
         "ref_trace": (None, [P, P, P, C.c_int, C.c_int, C.c_int, P]),
@@ -94,7 +98,8 @@ class Scene:
 
     def __init__(self, geoms):
         """geoms: list of dicts kind, vertices, indices, radii, colors,
-        default_radius, color, user_id."""
+        default_radius, color, user_id and optionally material = dict(type
+        "matte" | "pbr", metallic, roughness, transmission, ior)."""
         L = load()
         self._keep = []
         arr = (ref_geom * max(len(geoms), 1))()
@@ -112,6 +117,14 @@ class Scene:
             e.defaultRadius = float(g.get("default_radius", 1.0))
             e.color[:] = [float(x) for x in g.get("color", (0.8, 0.8, 0.8))]
             e.userID = int(g.get("user_id", 0))
+            m = g.get("material") or {}
+            e.matType = 1 if m.get("type", "matte") == "pbr" else 0
+            e.metallic = float(m.get("metallic", 0.0))
+            e.roughness = float(m.get("roughness", 0.5))
+            e.transmission = float(m.get("transmission", 0.0))
+            e.ior = float(m.get("ior", 1.45))
+            if "color" in m:
+                e.color[:] = [float(x) for x in m["color"]]
         self._arr = arr
         self.handle = L.ref_scene_create(arr, len(geoms))
 
@@ -215,13 +228,50 @@ def camera(position, direction, up=(0, 1, 0), fovy=60.0):
 
 
 def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
-                  max_diffuse=1):
+                  max_diffuse=1, russian_roulette=0):
     p = ref_render_params()
     p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
     p.ambientRadiance = float(ambient)
     p.bgColor[:] = [float(x) for x in bg]
     p.maxPathDepth, p.maxDiffuseBounces = max_depth, max_diffuse
+    p.russianRoulette = russian_roulette
     return p
+
+
+def half_round(x):
+    return load().ref_half_round(float(x))
+
+
+def _material(m):
+    g = ref_geom()
+    g.matType = 1 if m.get("type", "matte") == "pbr" else 0
+    g.metallic = float(m.get("metallic", 0.0))
+    g.roughness = float(m.get("roughness", 0.5))
+    g.transmission = float(m.get("transmission", 0.0))
+    g.ior = float(m.get("ior", 1.45))
+    return g
+
+
+def bsdf_sample(material, color, Ng, wo, n, in_medium=False, seed=1):
+    """n scatter samples: dict dir, f, pdf, type, offset, medium."""
+    g = _material(material)
+    col = np.ascontiguousarray(color, np.float32)
+    N = np.ascontiguousarray(Ng, np.float32)
+    w = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros((n, 10), np.float32)
+    load().ref_bsdf_sample(C.byref(g), col.ctypes.data, N.ctypes.data, w.ctypes.data, int(in_medium), seed, n,
+                           out.ctypes.data)
+    return {"dir": out[:, 0:3], "f": out[:, 3:6], "pdf": out[:, 6], "type": out[:, 7].astype(int),
+            "offset": out[:, 8], "medium": out[:, 9] > 0}
+
+
+def bsdf_eval(material, color, Ng, wo, wi):
+    """value[3], eval pdf, pdf() for one direction pair."""
+    g = _material(material)
+    a = [np.ascontiguousarray(x, np.float32) for x in (color, Ng, wo, wi)]
+    out = np.zeros(5, np.float32)
+    load().ref_bsdf_eval(C.byref(g), *[x.ctypes.data for x in a], out.ctypes.data)
+    return out[:3].copy(), float(out[3]), float(out[4])
 
 
 def dda_cells(org, dir, tmax, grid, max_cells=4096):
