@@ -324,3 +324,80 @@ class Volume:
             self.destroy()
         except Exception:
             pass
+
+
+class UMesh:
+    """Unstructured tetrahedral volume (per-vertex scalars): point-location
+    BVH, macrocell grid, transfer function; volume and iso-surface render."""
+
+    def __init__(self, vertices, scalars, tets):
+        lib = init()
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        sc = np.ascontiguousarray(scalars, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        if len(sc) != len(v):
+            raise ValueError("one scalar per vertex expected")
+        dv, ds, dt = DeviceArray(v), DeviceArray(sc), DeviceArray(t)
+        d = capi.bnamd_umesh_desc()
+        d.vertices, d.scalars, d.numVertices = dv.addr, ds.addr, len(v)
+        d.tets, d.numTets = dt.addr, len(t)
+        self.handle = C.c_void_p()
+        check(lib.bnamd_umesh_create(C.byref(d), None, C.byref(self.handle)))
+        dv.free(); ds.free(); dt.free()
+
+    def info(self):
+        i = capi.bnamd_umesh_info()
+        check(capi.load().bnamd_umesh_info_get(self.handle, C.byref(i)))
+        return {"numTets": i.numTets, "numNodes": i.numNodes, "bvhDepth": i.bvhDepth, "mcDims": tuple(i.mcDims),
+                "lower": list(i.worldLower), "upper": list(i.worldUpper), "buildMs": i.buildMs, "mcMs": i.mcMs}
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 4)
+        check(capi.load().bnamd_umesh_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]),
+                                             float(domain[1]), float(base_density)))
+
+    def macrocells(self):
+        mx, my, mz = self.info()["mcDims"]
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        check(capi.load().bnamd_umesh_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data))
+        return ranges, maj
+
+    def sample(self, points):
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        dp = DeviceArray(p)
+        do = DeviceArray.empty((len(p),), np.float32)
+        check(capi.load().bnamd_umesh_sample_points(self.handle, dp.addr, do.addr, len(p), None))
+        return do.numpy()
+
+    def iso_rays(self, rays, iso):
+        """rays: [n, 8] (org, tmin, dir, tmax) -> [n, 4] (t or inf, normal)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        dr = DeviceArray(r)
+        do = DeviceArray.empty((len(r), 4), np.float32)
+        check(capi.load().bnamd_umesh_iso_rays(self.handle, dr.addr, float(iso), do.addr, len(r), None))
+        return do.numpy()
+
+    def render(self, cam, params, volume=True, iso=None, iso_color=(0.8, 0.8, 0.8), accum=None):
+        acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+        o = capi.bnamd_umesh_render_opts()
+        o.volume = int(bool(volume))
+        o.iso = int(iso is not None)
+        o.isoValue = float(iso if iso is not None else 0.0)
+        o.isoColor[:] = [float(c) for c in iso_color]
+        st = capi.bnamd_volume_stats()
+        check(capi.load().bnamd_umesh_render(self.handle, C.byref(cam), C.byref(params), C.byref(o), acc.addr, None,
+                                             C.byref(st)))
+        return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
+                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            capi.load().bnamd_umesh_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -22,6 +22,7 @@ SOURCES = [
     "volume.hip",
     "ea.hip",
     "render_batch.hip",
+    "umesh.hip",
 ]
 
 # -packed-fp32-ops: packed FP32 forms are kept out of the generated code.

@@ -63,6 +63,21 @@ class bnamd_volume_stats(C.Structure):
                 ("ms", C.c_float)]
 
 
+class bnamd_umesh_desc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("scalars", C.c_void_p), ("numVertices", C.c_int), ("tets", C.c_void_p),
+                ("numTets", C.c_int)]
+
+
+class bnamd_umesh_info(C.Structure):
+    _fields_ = [("numTets", C.c_int), ("numNodes", C.c_int), ("bvhDepth", C.c_int), ("mcDims", C.c_int * 3),
+                ("worldLower", C.c_float * 3), ("worldUpper", C.c_float * 3), ("buildMs", C.c_float),
+                ("mcMs", C.c_float)]
+
+
+class bnamd_umesh_render_opts(C.Structure):
+    _fields_ = [("volume", C.c_int), ("iso", C.c_int), ("isoValue", C.c_float), ("isoColor", C.c_float * 3)]
+
+
 class bnamd_views(C.Structure):
     _fields_ = [("numViews", C.c_int), ("cameras", C.POINTER(bnamd_camera)), ("widths", C.POINTER(C.c_int)),
                 ("heights", C.POINTER(C.c_int))]
@@ -105,6 +120,16 @@ SIGNATURES = {
                                       C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
     "bnamd_volume_sample_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                            C.c_void_p]),
+    "bnamd_umesh_create": (C.c_int, [C.POINTER(bnamd_umesh_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bnamd_umesh_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_umesh_info_get": (C.c_int, [C.c_void_p, C.POINTER(bnamd_umesh_info)]),
+    "bnamd_umesh_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "bnamd_umesh_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_umesh_sample_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_umesh_iso_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_umesh_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                     C.POINTER(bnamd_umesh_render_opts), C.c_void_p, C.c_void_p,
+                                     C.POINTER(bnamd_volume_stats)]),
     "bnamd_render_batch": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
                                      C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
     "bnamd_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),

@@ -10,7 +10,7 @@
 
 namespace bnamd {
 
-enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2 };
+enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2, PRIM_TET = 3 };
 // c.w of a PrimRec: type in bits 0-7, flags above.
 // PRIM_SHADOW_TRANSPARENT: a triangle whose material is Glass; barney's
 // triangle anyHit ignores it for shadow rays (Glass::getOpacity returns 0
@@ -43,6 +43,8 @@ static_assert(sizeof(BNode8) == 128, "BNode8 must be one 128B line");
 //   triangle: a = v0 | flat,  b = e1 = v1-v0 | -, c = e2 = v2-v0 | type
 //   sphere:   a = center | flat, b = (radius,0,0) | -, c = 0 | type
 //   cylinder: a = v0 | flat, b = v1 | -, c = (radius,0,0) | type
+//   tet:      a = box lo | flat, b = box hi | -, c = 0 | type (point-location
+//             BVHs only; the tets themselves are packed separately)
 // 'flat' is the primitive's index in instance-major, geom-major, prim-minor
 // order; it doubles as the deterministic tie-break key at equal t.
 struct PrimRec {

@@ -54,6 +54,17 @@ __global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, const Ma
     r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(PRIM_SPHERE));
     lo = c - vec3f(rad);
     hi = c + vec3f(rad);
+  } else if (gd.kind == GEOM_TETS) {
+    const int4 ix = ((const int4 *)gd.indices)[primID];
+    const float4 *v4 = (const float4 *)gd.vertices;
+    const float4 p0 = v4[ix.x], p1 = v4[ix.y], p2 = v4[ix.z], p3 = v4[ix.w];
+    lo = {fminf(fminf(p0.x, p1.x), fminf(p2.x, p3.x)), fminf(fminf(p0.y, p1.y), fminf(p2.y, p3.y)),
+          fminf(fminf(p0.z, p1.z), fminf(p2.z, p3.z))};
+    hi = {fmaxf(fmaxf(p0.x, p1.x), fmaxf(p2.x, p3.x)), fmaxf(fmaxf(p0.y, p1.y), fmaxf(p2.y, p3.y)),
+          fmaxf(fmaxf(p0.z, p1.z), fmaxf(p2.z, p3.z))};
+    r.a = make_float4(lo.x, lo.y, lo.z, __uint_as_float(i));
+    r.b = make_float4(hi.x, hi.y, hi.z, 0.f);
+    r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(PRIM_TET));
   } else {
     const int *ix = gd.indices + 2 * primID;
     const vec3f a = load3(gd.vertices + 3 * ix[0]);
@@ -419,6 +430,9 @@ void prim_box(const PrimRec &r, uint32_t type, float lo[3], float hi[3]) {
     } else if (type == PRIM_SPHERE) {
       lo[k] = a[k] - r.b.x;
       hi[k] = a[k] + r.b.x;
+    } else if (type == PRIM_TET) {
+      lo[k] = a[k];
+      hi[k] = b[k];
     } else {
       lo[k] = std::min(a[k], b[k]) - r.c.x;
       hi[k] = std::max(a[k], b[k]) + r.c.x;
@@ -526,7 +540,7 @@ void validate_bvh(const World &w) {
     uint32_t flat, type;
     memcpy(&flat, &prims[p].a.w, 4);
     memcpy(&type, &prims[p].c.w, 4);
-    BN_REQUIRE(flat < w.numPrims && (type & 0xffu) <= 2u, "BVH8 check: corrupt primitive record");
+    BN_REQUIRE(flat < w.numPrims && (type & 0xffu) <= 3u, "BVH8 check: corrupt primitive record");
   }
 }
 

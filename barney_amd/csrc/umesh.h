@@ -1,0 +1,133 @@
+// Unstructured tetrahedral volume: point location over a BVH8 of the tets
+// and barycentric interpolation of per-vertex scalars, restating
+//   UMeshCuBQLSampler::DD::sample   barney/umesh/mc/UMeshCuBQLSampler.h:51-67
+//     (box query, first element whose inside test passes)
+//   UMeshField::DD::tetScalar       barney/umesh/common/UMeshField.h:390-434
+//   evalToImplicitPlane             barney/umesh/common/UMeshField.h:346-352
+// The macrocell grid and its majorants follow UMeshField::buildInitialMacroCells
+// (UMeshField.cu:173-200) with rasterTet<5> (:39-106) and rasterBox
+// (barney/volume/MCGrid.cuh:92-122).
+#pragma once
+#include "bvh8.h"
+#include "volume.h"
+
+namespace bnamd {
+
+// Tets are stored in BVH leaf order, one 64-byte record per tet: the four
+// vertices as (x, y, z, scalar), so a leaf test reads one contiguous line.
+struct UMeshDD {
+  const BNode8 *nodes;
+  const float4 *tets;
+  int depth;
+  vec3f worldLo, worldHi;  // bounds of the vertices
+  const float *majorants;
+  const float2 *ranges;
+  int mx, my, mz;
+  vec3f mcOrigin, mcSpacing;
+  TFDD tf;
+};
+
+// tetScalar: inside if all four implicit-plane distances are >= 0 (tets
+// are positively oriented), value = sum t_i s_i / sum t_i.
+BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
+#pragma clang fp contract(off)
+  const float4 a4 = t[0], b4 = t[1], c4 = t[2], d4 = t[3];
+  const vec3f v0 = {a4.x, a4.y, a4.z}, v1 = {b4.x, b4.y, b4.z}, v2 = {c4.x, c4.y, c4.z}, v3 = {d4.x, d4.y, d4.z};
+  const float t3 = dot(P - v0, cross(v1 - v0, v2 - v0));
+  if (t3 < 0.f) return false;
+  const float t2 = dot(P - v0, cross(v3 - v0, v1 - v0));
+  if (t2 < 0.f) return false;
+  const float t1 = dot(P - v0, cross(v2 - v0, v3 - v0));
+  if (t1 < 0.f) return false;
+  const float t0 = dot(P - v1, cross(v3 - v1, v2 - v1));
+  if (t0 < 0.f) return false;
+  const float scale = 1.f / (t0 + t1 + t2 + t3);
+  val = scale * (t0 * a4.w + t1 * b4.w + t2 * c4.w + t3 * d4.w);
+  return true;
+}
+
+// First tet (in BVH traversal order) containing P; NaN when none does.
+// Child boxes are tested in the node's quantized frame: (P - origin) * 2^-e
+// against the 16-bit planes, which carry one quantum of slack on each side,
+// so the test is conservative.
+BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
+  uint32_t grpBase = 0;
+  uint32_t grpBits = 1u | (1u << 8);  // root = internal slot 0
+  uint32_t primBase = 0, primMask = 0;
+  int sp = 0;
+  while (true) {
+    if ((grpBits & 0xffu) != 0) {
+      const uint32_t k = __builtin_ctz(grpBits);
+      grpBits &= ~(1u << k);
+      const uint32_t imask = grpBits >> 8;
+      const uint32_t nodeIdx = grpBase + __builtin_popcount(imask & ((1u << k) - 1u));
+      if ((grpBits & 0xffu) != 0) {
+        if (sp >= ctx.maxDepth) return __builtin_nanf("");
+        ctx.stack[sp * ctx.stride] = make_uint2(grpBase, grpBits);
+        ++sp;
+      }
+      const uint4 *np = (const uint4 *)(u.nodes + nodeIdx);
+      const uint4 n0 = np[0];
+      const uint4 n1 = np[1];
+      const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
+      const uint32_t nimask = n0.w >> 24;
+      // 2^(127 - e): exact scale into the quantized frame
+      const float px = (P.x - __uint_as_float(n0.x)) * __uint_as_float((254u - ex) << 23);
+      const float py = (P.y - __uint_as_float(n0.y)) * __uint_as_float((254u - ey) << 23);
+      const float pz = (P.z - __uint_as_float(n0.z)) * __uint_as_float((254u - ez) << 23);
+      uint32_t hitInner = 0, hitPrims = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint4 A = np[2 + 3 * h], B = np[3 + 3 * h], C = np[4 + 3 * h];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = 4 * h + j;
+          const uint32_t wlx = (j < 2) ? A.x : A.y, wly = (j < 2) ? A.z : A.w;
+          const uint32_t wlz = (j < 2) ? B.x : B.y, whx = (j < 2) ? B.z : B.w;
+          const uint32_t why = (j < 2) ? C.x : C.y, whz = (j < 2) ? C.z : C.w;
+          const int sh = (j & 1) * 16;
+          const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
+                      lz = (float)((wlz >> sh) & 0xffffu);
+          const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
+                      hz = (float)((whz >> sh) & 0xffffu);
+          const bool in = px >= lx && px <= hx && py >= ly && py <= hy && pz >= lz && pz <= hz;
+          if (in) {
+            if (nimask & (1u << c)) {
+              hitInner |= 1u << c;
+            } else {
+              const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
+              const uint32_t cnt = meta >> 5, off = meta & 31u;
+              hitPrims |= ((1u << cnt) - 1u) << off;
+            }
+          }
+        }
+      }
+      grpBase = n1.x;
+      grpBits = hitInner | (nimask << 8);
+      primBase = n1.y;
+      primMask = hitPrims;
+    }
+    while (primMask) {
+      const uint32_t i = __builtin_ctz(primMask);
+      primMask &= primMask - 1u;
+      float val;
+      if (tet_scalar(u.tets + 4ull * (primBase + i), P, val)) return val;
+    }
+    if ((grpBits & 0xffu) == 0) {
+      if (sp == 0) break;
+      --sp;
+      const uint2 e = ctx.stack[sp * ctx.stride];
+      grpBase = e.x;
+      grpBits = e.y;
+    }
+  }
+  return __builtin_nanf("");
+}
+
+BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
+  const float f = umesh_locate(u, P, ctx);
+  if (f != f) return VolSample{0.f, 0.f, 0.f, 0.f};
+  return tf_map(u.tf, f);
+}
+
+}  // namespace bnamd

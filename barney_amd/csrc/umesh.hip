@@ -1,0 +1,542 @@
+// Unstructured tetrahedral volume object: BVH8 over the tets (the world
+// builder with the tet primitive kind), tets packed in leaf order, the
+// macrocell grid of UMeshField::buildInitialMacroCells (UMeshField.cu:173-200)
+// rasterized per tet with rasterTet<5> (:39-106, MCGrid.cuh:92-122), majorants
+// (mapMCs, MCGrid.cu:94-116), and the volume + iso-surface renderer.
+#include "shade.h"
+#include "umesh.h"
+#include <chrono>
+#include <cmath>
+
+namespace bnamd {
+
+size_t trace_lds_bytes(int depth, int block);
+
+struct UMesh {
+  World w;                 // point-location BVH (prims freed after packing)
+  DevBuf<float4> tets;     // 4 per tet, leaf order
+  DevBuf<uint2> rangesOrd; // ordered-uint min/max during rasterization
+  DevBuf<float2> ranges;
+  DevBuf<float> majorants;
+  DevBuf<float4> tf;
+  UMeshDD dd{};
+  int numTets = 0;
+  bool haveTF = false;
+  float buildMs = 0.f, mcMs = 0.f;
+};
+
+namespace {
+
+__global__ void k_pack_verts(const float *__restrict v3, const float *__restrict s, int n, float4 *__restrict out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = make_float4(v3[3 * i + 0], v3[3 * i + 1], v3[3 * i + 2], s[i]);
+}
+
+// tets in BVH leaf order: position p holds the tet whose flat index the
+// builder stored in prims[p]
+__global__ void k_pack_tets(const PrimRec *__restrict prims, uint32_t n, const int4 *__restrict tets,
+                            const float4 *__restrict v4, float4 *__restrict out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t t = __float_as_uint(prims[p].a.w);
+  const int4 ix = tets[t];
+  out[4ull * p + 0] = v4[ix.x];
+  out[4ull * p + 1] = v4[ix.y];
+  out[4ull * p + 2] = v4[ix.z];
+  out[4ull * p + 3] = v4[ix.w];
+}
+
+__device__ __forceinline__ uint32_t ord_of(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float float_of_ord(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_clear_ranges(uint2 *__restrict r, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    r[i] = make_uint2(ord_of(BN_INF), ord_of(-BN_INF));
+}
+
+struct MCRaster {
+  uint2 *ranges;
+  int mx, my, mz;
+  vec3f origin, rcpSpacing;
+};
+
+struct V4 {
+  float x, y, z, w;
+};
+__device__ __forceinline__ V4 mid4(const V4 &a, const V4 &b) {
+#pragma clang fp contract(off)
+  return V4{0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.5f * (a.w + b.w)};
+}
+__device__ __forceinline__ float len3(const V4 &a, const V4 &b) {
+#pragma clang fp contract(off)
+  const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// rasterBox(grid, box4f) (MCGrid.cuh:92-122): boxes flat in any dimension
+// are skipped; cell range = [min w, max w] via atomic min/max
+__device__ void raster_box(const MCRaster &g, const V4 &a, const V4 &b, const V4 &c, const V4 &d) {
+#pragma clang fp contract(off)
+  const float lx = fminf(fminf(a.x, b.x), fminf(c.x, d.x)), hx = fmaxf(fmaxf(a.x, b.x), fmaxf(c.x, d.x));
+  const float ly = fminf(fminf(a.y, b.y), fminf(c.y, d.y)), hy = fmaxf(fmaxf(a.y, b.y), fmaxf(c.y, d.y));
+  const float lz = fminf(fminf(a.z, b.z), fminf(c.z, d.z)), hz = fmaxf(fmaxf(a.z, b.z), fmaxf(c.z, d.z));
+  const float lw = fminf(fminf(a.w, b.w), fminf(c.w, d.w)), hw = fmaxf(fmaxf(a.w, b.w), fmaxf(c.w, d.w));
+  if (lx >= hx || ly >= hy || lz >= hz) return;
+  int x0 = (int)((lx - g.origin.x) * g.rcpSpacing.x), x1 = (int)((hx - g.origin.x) * g.rcpSpacing.x);
+  int y0 = (int)((ly - g.origin.y) * g.rcpSpacing.y), y1 = (int)((hy - g.origin.y) * g.rcpSpacing.y);
+  int z0 = (int)((lz - g.origin.z) * g.rcpSpacing.z), z1 = (int)((hz - g.origin.z) * g.rcpSpacing.z);
+  x0 = min(max(x0, 0), g.mx - 1); x1 = min(max(x1, 0), g.mx - 1);
+  y0 = min(max(y0, 0), g.my - 1); y1 = min(max(y1, 0), g.my - 1);
+  z0 = min(max(z0, 0), g.mz - 1); z1 = min(max(z1, 0), g.mz - 1);
+  const uint32_t olo = ord_of(lw), ohi = ord_of(hw);
+  for (int iz = z0; iz <= z1; ++iz)
+    for (int iy = y0; iy <= y1; ++iy)
+      for (int ix = x0; ix <= x1; ++ix) {
+        uint2 *cell = g.ranges + ix + (size_t)iy * g.mx + (size_t)iz * g.mx * g.my;
+        atomicMin(&cell->x, olo);
+        atomicMax(&cell->y, ohi);
+      }
+}
+
+// rasterTet<D> (UMeshField.cu:39-106): split at the midpoint of the longest
+// edge, D levels deep, then raster the 2^D sub-tet boxes
+template <int D>
+__device__ void raster_tet(const MCRaster &g, const V4 &a, const V4 &b, const V4 &c, const V4 &d) {
+  if constexpr (D == 0) {
+    raster_box(g, a, b, c, d);
+  } else {
+    const float lab = len3(a, b), lac = len3(a, c), lad = len3(a, d);
+    const float lbc = len3(b, c), lbd = len3(b, d), lcd = len3(c, d);
+    const float maxLen = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(lab, lac), lad), lbc), lbd), lcd);
+    V4 oa, ob, oc, od0, od1;
+    if (lab >= maxLen) { oa = mid4(a, b); ob = c; oc = d; od0 = a; od1 = b; }
+    else if (lac >= maxLen) { oa = mid4(a, c); ob = b; oc = d; od0 = a; od1 = c; }
+    else if (lad >= maxLen) { oa = mid4(a, d); ob = b; oc = c; od0 = a; od1 = d; }
+    else if (lbc >= maxLen) { oa = mid4(b, c); ob = a; oc = d; od0 = b; od1 = c; }
+    else if (lbd >= maxLen) { oa = mid4(b, d); ob = a; oc = c; od0 = b; od1 = d; }
+    else { oa = mid4(c, d); ob = a; oc = b; od0 = c; od1 = d; }
+    raster_tet<D - 1>(g, oa, ob, oc, od0);
+    raster_tet<D - 1>(g, oa, ob, oc, od1);
+  }
+}
+
+__global__ void k_raster_tets(MCRaster g, const int4 *__restrict tets, const float4 *__restrict v4, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int4 ix = tets[t];
+  const float4 p0 = v4[ix.x], p1 = v4[ix.y], p2 = v4[ix.z], p3 = v4[ix.w];
+  raster_tet<5>(g, V4{p0.x, p0.y, p0.z, p0.w}, V4{p1.x, p1.y, p1.z, p1.w}, V4{p2.x, p2.y, p2.z, p2.w},
+                V4{p3.x, p3.y, p3.z, p3.w});
+}
+
+__global__ void k_finish_ranges(const uint2 *__restrict in, float2 *__restrict out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = make_float2(float_of_ord(in[i].x), float_of_ord(in[i].y));
+}
+
+__global__ void k_umesh_majorants(TFDD tf, const float2 *__restrict ranges, float *__restrict maj, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    maj[i] = tf_majorant(tf, ranges[i].x, ranges[i].y);
+}
+
+__global__ void __launch_bounds__(256) k_umesh_sample(UMeshDD u, const float *__restrict pts, float *__restrict out,
+                                                      int n) {
+  extern __shared__ uint2 lds_stack[];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
+  out[i] = umesh_locate(u, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, ctx);
+}
+
+__global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_ray *__restrict rays, float iso,
+                                                        float4 *__restrict out, int n) {
+  extern __shared__ uint2 lds_stack[];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
+  const bnamd_ray r = rays[i];
+  RNGSeed seed;
+  const IsoHit h = iso_intersect(
+      u, [&](vec3f P) { return umesh_locate(u, P, ctx); }, iso, {r.org[0], r.org[1], r.org[2]},
+      fix_dir({r.dir[0], r.dir[1], r.dir[2]}), r.tmin, r.tmax, r.tmax, seed, bn_hash3(0u, 1u, 0u));
+  out[i] = h.hit ? make_float4(h.t, h.N.x, h.N.y, h.N.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int per = (nb + 7) >> 3;
+  const int t = (bid & 7) * per + (bid >> 3);
+  return t < nb ? t : -1;
+}
+
+__device__ __forceinline__ void wave_sum_u64(unsigned long long v, unsigned long long *dst) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
+}
+
+struct UMeshOpts {
+  int volume, iso;
+  float isoValue;
+  vec3f isoColor;
+};
+
+// Closest of the volume (geometry 0) and iso-surface (geometry 1) hits for
+// one ray; each intersection program advances the ray's seed once.
+__device__ ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
+                                RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt) {
+  ShadeHit sh;
+  sh.hit = false;
+  float tClosest = tMax;
+  if (o.volume) {
+    const VolHit vh = volume_intersect_g(
+        u, [&](vec3f P) { return umesh_sample_and_map(u, P, ctx); }, org, dir, 0.f, tMax, tMax, seed, false,
+        cnt, bn_hash3(0u, 0u, 0u));
+    if (vh.hit) {
+      sh.hit = true;
+      sh.P = org + vh.t * dir;
+      sh.N = vec3f(0.f);
+      sh.t = vh.t;
+      sh.bsdf = phase_bsdf(vh.rgb);
+      tClosest = vh.t;
+    }
+  }
+  if (o.iso) {
+    const IsoHit ih = iso_intersect(
+        u,
+        [&](vec3f P) {
+          cnt.samples++;
+          return umesh_locate(u, P, ctx);
+        },
+        o.isoValue, org, dir, 0.f, tMax, tClosest, seed, bn_hash3(0u, 1u, 0u));
+    if (ih.hit) {
+      sh.hit = true;
+      sh.P = ih.P;
+      sh.N = ih.N;
+      sh.t = ih.t;
+      sh.bsdf = matte_bsdf(o.isoColor);
+    }
+  }
+  return sh;
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp,
+                                                      float4 *__restrict accum,
+                                                      unsigned long long *__restrict counters) {
+  extern __shared__ uint2 lds_stack[];
+  constexpr int PW = 64 / L;
+  constexpr int PB = 4 * PW;
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  const int tilesX = (rp.width + TW - 1) / TW, tilesY = (rp.height + TH - 1) / TH;
+  const int tile = xcd_tile(blockIdx.x, tilesX * tilesY);
+  if (tile < 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane & (L - 1);
+  const int q = wave * PW + lane / L;
+  const int ix = (tile % tilesX) * TW + (q % TW);
+  const int iy = (tile / tilesX) * TH + (q / TW);
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
+  VolCounters cnt{0, 0};
+  unsigned long long nrays = 0;
+  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
+  const bool active = ix < rp.width && iy < rp.height;
+  if (active) {
+    for (int p = sub; p < spp; p += L) {
+      RenderDD rpp = rp;
+      rpp.accumID = rp.accumID + p;
+      PathRay ray = camera_ray(cam, rpp, ix, iy);
+      const float clampMax = 10.f * (1 + rpp.accumID);
+      vec3f sum = vec3f(0.f);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const vec3f dir = fix_dir(ray.dir);
+        const ShadeHit sh = umesh_trace(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt);
+        ++nrays;
+        if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
+        PathRay shadow;
+        vec3f frag = vmin(bounce(rpp, ray, sh, shadow, gen), clampMax);
+        if (frag.x > 0.f) sum.x += frag.x;
+        if (frag.y > 0.f) sum.y += frag.y;
+        if (frag.z > 0.f) sum.z += frag.z;
+        if (shadow.tMax > 0.f) {
+          const ShadeHit shs = umesh_trace(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt);
+          ++nrays;
+          PathRay dummy;
+          frag = vmin(bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          if (frag.x > 0.f) sum.x += frag.x;
+          if (frag.y > 0.f) sum.y += frag.y;
+          if (frag.z > 0.f) sum.z += frag.z;
+        }
+      }
+      sr += sum.x;
+      sg += sum.y;
+      sb += sum.z;
+      sa += alpha;
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    sg += __shfl_xor(sg, off);
+    sb += __shfl_xor(sb, off);
+    sa += __shfl_xor(sa, off);
+  }
+  if (active && sub == 0) {
+    const int pix = ix + iy * rp.width;
+    if (rp.accumID == 0) {
+      accum[pix] = make_float4(sr, sg, sb, sa);
+    } else {
+      float4 a = accum[pix];
+      a.x += sr; a.y += sg; a.z += sb; a.w += sa;
+      accum[pix] = a;
+    }
+  }
+  wave_sum_u64(cnt.samples, counters + 0);
+  wave_sum_u64(cnt.ddaSteps, counters + 1);
+  wave_sum_u64(nrays, counters + 2);
+}
+
+template <int L>
+void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &cam, const RenderDD &rp, int spp,
+                         float4 *accum, unsigned long long *counters, size_t lds, hipStream_t s) {
+  constexpr int PB = 4 * (64 / L);
+  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int TH = PB / TW;
+  const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
+  const int grid = ((tiles + 7) / 8) * 8;
+  k_render_umesh<L><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+}
+
+void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi, float base) {
+  m.tf.upload((const float4 *)values, numValues, 0);
+  m.dd.tf.values = m.tf.ptr;
+  m.dd.tf.numValues = numValues;
+  m.dd.tf.domLo = lo;
+  m.dd.tf.domHi = hi;
+  m.dd.tf.baseDensity = base;
+  const size_t ncell = (size_t)m.dd.mx * m.dd.my * m.dd.mz;
+  k_umesh_majorants<<<(int)std::min<size_t>(div_up(ncell, 256), 8192), 256>>>(m.dd.tf, m.ranges.ptr,
+                                                                              m.majorants.ptr, ncell);
+  BN_HIP(hipGetLastError());
+  BN_HIP(hipDeviceSynchronize());
+  m.haveTF = true;
+}
+
+}  // namespace
+}  // namespace bnamd
+
+using namespace bnamd;
+
+struct bnamd_umesh_s {
+  UMesh m;
+};
+
+extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bnamd_umesh **out) {
+  return guarded([&] {
+    BN_REQUIRE(desc && out, "bnamd_umesh_create: NULL argument");
+    BN_REQUIRE(desc->numVertices > 0 && desc->numTets > 0, "bnamd_umesh_create: empty mesh");
+    BN_REQUIRE(desc->vertices && desc->scalars && desc->tets, "bnamd_umesh_create: NULL array");
+    auto *um = new bnamd_umesh_s;
+    try {
+      UMesh &m = um->m;
+      hipStream_t s = (hipStream_t)stream;
+      const auto t0 = std::chrono::steady_clock::now();
+      m.numTets = desc->numTets;
+      DevBuf<float4> v4(desc->numVertices);
+      k_pack_verts<<<div_up(desc->numVertices, 256), 256, 0, s>>>(desc->vertices, desc->scalars, desc->numVertices,
+                                                                  v4.ptr);
+      BN_HIP(hipGetLastError());
+      GeomDev g{};
+      g.kind = GEOM_TETS;
+      g.numPrims = desc->numTets;
+      g.primOffset = 0;
+      g.vertices = (const float *)v4.ptr;
+      g.indices = desc->tets;
+      m.w.hgeoms.push_back(g);
+      m.w.numPrims = (uint32_t)desc->numTets;
+      MaterialDev md{};
+      m.w.materials.upload(&md, 1, s);
+      build_world_bvh(m.w, s);
+      m.tets.alloc(4ull * desc->numTets);
+      k_pack_tets<<<div_up(desc->numTets, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)desc->numTets,
+                                                            (const int4 *)desc->tets, v4.ptr, m.tets.ptr);
+      BN_HIP(hipGetLastError());
+      BN_HIP(hipStreamSynchronize(s));
+      m.w.prims.release();
+      const auto t1 = std::chrono::steady_clock::now();
+      m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
+      // macrocell grid (UMeshField.cu:173-200)
+      const vec3f lo = {m.w.lo[0], m.w.lo[1], m.w.lo[2]}, hi = {m.w.hi[0], m.w.hi[1], m.w.hi[2]};
+      const vec3f size = hi - lo;
+      const float maxWidth = reduce_max(size);
+      const int MC_GRID_SIZE = 200 + int(sqrtf(desc->numTets / 1000.f));
+      const float f = (MC_GRID_SIZE - 1) / maxWidth;
+      m.dd.mx = 1 + (int)(size.x * f);
+      m.dd.my = 1 + (int)(size.y * f);
+      m.dd.mz = 1 + (int)(size.z * f);
+      m.dd.mcOrigin = lo;
+      m.dd.mcSpacing = size * rcp(vec3f((float)m.dd.mx, (float)m.dd.my, (float)m.dd.mz));
+      m.dd.worldLo = lo;
+      m.dd.worldHi = hi;
+      const size_t ncell = (size_t)m.dd.mx * m.dd.my * m.dd.mz;
+      m.rangesOrd.alloc(ncell);
+      m.ranges.alloc(ncell);
+      m.majorants.alloc(ncell);
+      const int gs = (int)std::min<size_t>(div_up(ncell, 256), 8192);
+      k_clear_ranges<<<gs, 256, 0, s>>>(m.rangesOrd.ptr, ncell);
+      MCRaster r;
+      r.ranges = m.rangesOrd.ptr;
+      r.mx = m.dd.mx; r.my = m.dd.my; r.mz = m.dd.mz;
+      r.origin = m.dd.mcOrigin;
+      r.rcpSpacing = rcp(m.dd.mcSpacing);
+      k_raster_tets<<<div_up(desc->numTets, 128), 128, 0, s>>>(r, (const int4 *)desc->tets, v4.ptr, desc->numTets);
+      BN_HIP(hipGetLastError());
+      k_finish_ranges<<<gs, 256, 0, s>>>(m.rangesOrd.ptr, m.ranges.ptr, ncell);
+      BN_HIP(hipGetLastError());
+      BN_HIP(hipStreamSynchronize(s));
+      m.rangesOrd.release();
+      m.mcMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t1).count();
+      m.dd.nodes = m.w.nodes.ptr;
+      m.dd.tets = m.tets.ptr;
+      m.dd.depth = m.w.depth;
+      m.dd.ranges = m.ranges.ptr;
+      m.dd.majorants = m.majorants.ptr;
+      // default TF: constant white, density 1 (replaced by bnamd_umesh_set_tf)
+      const float def[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+      upload_tf(m, def, 2, 0.f, 1.f, 1.f);
+      m.haveTF = false;
+    } catch (...) {
+      delete um;
+      throw;
+    }
+    *out = um;
+  });
+}
+
+extern "C" int bnamd_umesh_destroy(bnamd_umesh *mesh) {
+  return guarded([&] { delete mesh; });
+}
+
+extern "C" int bnamd_umesh_info_get(const bnamd_umesh *mesh, bnamd_umesh_info *info) {
+  return guarded([&] {
+    BN_REQUIRE(mesh && info, "bnamd_umesh_info_get: NULL argument");
+    const UMesh &m = mesh->m;
+    info->numTets = m.numTets;
+    info->numNodes = (int)m.w.numNodes;
+    info->bvhDepth = m.w.depth;
+    info->mcDims[0] = m.dd.mx; info->mcDims[1] = m.dd.my; info->mcDims[2] = m.dd.mz;
+    for (int k = 0; k < 3; ++k) {
+      info->worldLower[k] = m.w.lo[k];
+      info->worldUpper[k] = m.w.hi[k];
+    }
+    info->buildMs = m.buildMs;
+    info->mcMs = m.mcMs;
+  });
+}
+
+extern "C" int bnamd_umesh_set_tf(bnamd_umesh *mesh, const float *values, int numValues, float domainLo,
+                                  float domainHi, float baseDensity) {
+  return guarded([&] {
+    BN_REQUIRE(mesh && values, "bnamd_umesh_set_tf: NULL argument");
+    BN_REQUIRE(numValues >= 2, "bnamd_umesh_set_tf: need at least 2 values");
+    BN_REQUIRE(domainHi > domainLo, "bnamd_umesh_set_tf: empty domain");
+    upload_tf(mesh->m, values, numValues, domainLo, domainHi, baseDensity);
+  });
+}
+
+extern "C" int bnamd_umesh_get_mcs(const bnamd_umesh *mesh, float *ranges, float *majorants) {
+  return guarded([&] {
+    BN_REQUIRE(mesh, "bnamd_umesh_get_mcs: NULL mesh");
+    const UMesh &m = mesh->m;
+    const size_t n = (size_t)m.dd.mx * m.dd.my * m.dd.mz;
+    if (ranges) BN_HIP(hipMemcpy(ranges, m.ranges.ptr, n * sizeof(float2), hipMemcpyDeviceToHost));
+    if (majorants) BN_HIP(hipMemcpy(majorants, m.majorants.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int bnamd_umesh_sample_points(const bnamd_umesh *mesh, const float *points, float *out, int numPoints,
+                                         void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(mesh, "bnamd_umesh_sample_points: NULL mesh");
+    BN_REQUIRE(numPoints >= 0, "bnamd_umesh_sample_points: negative count");
+    if (numPoints == 0) return;
+    BN_REQUIRE(points && out, "bnamd_umesh_sample_points: NULL buffer");
+    const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
+    BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_sample_points: BVH too deep for the LDS stack");
+    hipStream_t s = (hipStream_t)stream;
+    k_umesh_sample<<<div_up(numPoints, 256), 256, lds, s>>>(mesh->m.dd, points, out, numPoints);
+    BN_HIP(hipGetLastError());
+    if (!stream) BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int bnamd_umesh_iso_rays(const bnamd_umesh *mesh, const bnamd_ray *rays, float isoValue, float *out,
+                                    int numRays, void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(mesh, "bnamd_umesh_iso_rays: NULL mesh");
+    BN_REQUIRE(numRays >= 0, "bnamd_umesh_iso_rays: negative count");
+    if (numRays == 0) return;
+    BN_REQUIRE(rays && out, "bnamd_umesh_iso_rays: NULL buffer");
+    const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
+    BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_iso_rays: BVH too deep for the LDS stack");
+    hipStream_t s = (hipStream_t)stream;
+    k_umesh_iso_rays<<<div_up(numRays, 256), 256, lds, s>>>(mesh->m.dd, rays, isoValue, (float4 *)out, numRays);
+    BN_HIP(hipGetLastError());
+    if (!stream) BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *camera,
+                                  const bnamd_render_params *params, const bnamd_umesh_render_opts *opts,
+                                  float *accum, void *stream, bnamd_volume_stats *stats) {
+  return guarded([&] {
+    BN_REQUIRE(mesh && camera && params && opts && accum, "bnamd_umesh_render: NULL argument");
+    BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
+               "bnamd_umesh_render: bad frame size");
+    BN_REQUIRE(!opts->volume || mesh->m.haveTF, "bnamd_umesh_render: volume rendering needs a transfer function");
+    const CameraDD cam = make_camera(*camera);
+    RenderDD rp = make_render_dd(*params);
+    UMeshOpts o;
+    o.volume = opts->volume;
+    o.iso = opts->iso;
+    o.isoValue = opts->isoValue;
+    o.isoColor = {opts->isoColor[0], opts->isoColor[1], opts->isoColor[2]};
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf<unsigned long long> cnt(3);
+    BN_HIP(hipMemsetAsync(cnt.ptr, 0, 3 * sizeof(unsigned long long), s));
+    const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
+    BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_render: BVH too deep for the LDS stack");
+    hipEvent_t e0, e1;
+    BN_HIP(hipEventCreate(&e0));
+    BN_HIP(hipEventCreate(&e1));
+    BN_HIP(hipEventRecord(e0, s));
+    const int spp = params->pathsPerPixel;
+    int lanes = spp >= 4 ? 4 : (spp >= 2 ? 2 : 1);
+    if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
+    if (lanes >= 8) launch_render_umesh<8>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else if (lanes >= 4) launch_render_umesh<4>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else if (lanes >= 2) launch_render_umesh<2>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else launch_render_umesh<1>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    BN_HIP(hipGetLastError());
+    BN_HIP(hipEventRecord(e1, s));
+    BN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long h[3];
+    BN_HIP(hipMemcpy(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->samples = (long long)h[0];
+      stats->ddaSteps = (long long)h[1];
+      stats->raysTraced = (long long)h[2];
+      stats->ms = ms;
+    }
+  });
+}

@@ -161,8 +161,13 @@ BN_HD int vfloor_i(float f) { return (int)floorf(f); }
 // MCVolumeAccel::isProg for one ray segment [tmin, tmax]: box test, DDA over
 // macrocells, Woodcock tracking in each cell with a non-zero majorant. The
 // ray's RNG seed is advanced as Random2(seed, hash(inst, geom, 0)) does.
-BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmin, float tmax,
-                              float rayTMax, RNGSeed &seed, bool exactLog, VolCounters &cnt) {
+// V supplies the macrocell grid (worldLo/Hi, mcOrigin, mcSpacing,
+// majorants, mx/my/mz); sample(P) returns the TF-mapped field sample (the
+// structured sampler, or the tet-mesh point location).
+template <class V, class S>
+BN_HD VolHit volume_intersect_g(const V &v, const S &sample, vec3f org, vec3f dir, float tmin, float tmax,
+                                float rayTMax, RNGSeed &seed, bool exactLog, VolCounters &cnt,
+                                uint64_t geomHash) {
 #pragma clang fp contract(off)
   VolHit res;
   res.hit = false;
@@ -178,7 +183,7 @@ BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmi
   }
   const vec3f dorg = (org - v.mcOrigin) * rcp(v.mcSpacing);
   const vec3f ddir = dir * rcp(v.mcSpacing);
-  Random2 rng(seed, bn_hash3(0u, 0u, 0u));
+  Random2 rng(seed, geomHash);
 
   // dda::dda3(dorg, ddir, t1, mcDims, lambda)
   const float tMax = t1;
@@ -242,7 +247,7 @@ BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmi
         t += dt;
         if (t >= upper) break;
         const vec3f P = org + t * dir;
-        const VolSample s = sample_and_map(v, P);
+        const VolSample s = sample(P);
         cnt.samples++;
         if (s.w >= rng() * majorant) {
           res.hit = true;
@@ -271,6 +276,167 @@ BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmi
       if (cz == stopz) return res;
     }
   }
+}
+
+
+BN_HD VolHit volume_intersect(const VolumeDD &v, vec3f org, vec3f dir, float tmin, float tmax, float rayTMax,
+                              RNGSeed &seed, bool exactLog, VolCounters &cnt) {
+  return volume_intersect_g(
+      v, [&](vec3f P) { return sample_and_map(v, P); }, org, dir, tmin, tmax, rayTMax, seed, exactLog, cnt,
+      bn_hash3(0u, 0u, 0u));
+}
+
+// dda::dda3 (barney/volume/DDA.h:65-206) over an mx x my x mz grid in grid
+// space: cb(cx, cy, cz, t0, t1) per visited cell, false stops the walk.
+template <class F>
+BN_HD void dda3(vec3f dorg, vec3f ddir, float tMax, int mx, int my, int mz, F &&cb) {
+#pragma clang fp contract(off)
+  const vec3f f_size = {(float)mx, (float)my, (float)mz};
+  const vec3f rcp_dir = rcp(ddir);
+  vec3f t_lo = (vec3f(0.f) - dorg) * rcp(ddir);
+  vec3f t_hi = (f_size - dorg) * rcp(ddir);
+  vec3f t_nr = vmin(t_lo, t_hi);
+  vec3f t_fr = vmax(t_lo, t_hi);
+  if (ddir.x == 0.f) {
+    if (dorg.x < 0.f || dorg.x > f_size.x) return;
+    t_nr.x = -BN_INF; t_fr.x = BN_INF;
+  }
+  if (ddir.y == 0.f) {
+    if (dorg.y < 0.f || dorg.y > f_size.y) return;
+    t_nr.y = -BN_INF; t_fr.y = BN_INF;
+  }
+  if (ddir.z == 0.f) {
+    if (dorg.z < 0.f || dorg.z > f_size.z) return;
+    t_nr.z = -BN_INF; t_fr.z = BN_INF;
+  }
+  const float ray_t0 = fmaxf(0.f, reduce_max(t_nr));
+  const float ray_t1 = fminf(tMax, reduce_min(t_fr));
+  if (ray_t0 > ray_t1) return;
+  const vec3f oiv = dorg + ray_t0 * ddir;
+  const vec3f f_cell = vmax(vec3f(0.f), vmin(f_size - vec3f(1.f), vec3f(floorf(oiv.x), floorf(oiv.y), floorf(oiv.z))));
+  const vec3f f_cell_end = {ddir.x > 0.f ? f_cell.x + 1.f : f_cell.x, ddir.y > 0.f ? f_cell.y + 1.f : f_cell.y,
+                            ddir.z > 0.f ? f_cell.z + 1.f : f_cell.z};
+  const vec3f t_step = vabs(rcp_dir);
+  vec3f t_next = {(ddir.x == 0.f) ? BN_INF : (fabsf(f_cell_end.x - oiv.x) * t_step.x),
+                  (ddir.y == 0.f) ? BN_INF : (fabsf(f_cell_end.y - oiv.y) * t_step.y),
+                  (ddir.z == 0.f) ? BN_INF : (fabsf(f_cell_end.z - oiv.z) * t_step.z)};
+  const int stopx = ddir.x > 0.f ? mx : -1, stopy = ddir.y > 0.f ? my : -1, stopz = ddir.z > 0.f ? mz : -1;
+  const int dx = ddir.x > 0.f ? 1 : -1, dy = ddir.y > 0.f ? 1 : -1, dz = ddir.z > 0.f ? 1 : -1;
+  int cx = (int)f_cell.x, cy = (int)f_cell.y, cz = (int)f_cell.z;
+  float next_cell_begin = 0.f;
+  while (true) {
+    const float t_closest = reduce_min(t_next);
+    const float cell_t0 = ray_t0 + next_cell_begin;
+    const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
+    if (!cb(cx, cy, cz, cell_t0, cell_t1)) return;
+    next_cell_begin = t_closest;
+    if (t_next.x == t_closest) {
+      t_next.x += t_step.x;
+      cx += dx;
+      if (cx == stopx) return;
+    }
+    if (t_next.y == t_closest) {
+      t_next.y += t_step.y;
+      cy += dy;
+      if (cy == stopy) return;
+    }
+    if (t_next.z == t_closest) {
+      t_next.z += t_step.z;
+      cz += dz;
+      if (cz == stopz) return;
+    }
+  }
+}
+
+struct IsoHit {
+  bool hit;
+  float t;
+  vec3f P, N;
+};
+
+// MCIsoSurfaceAccel::isProg (barney/volume/MCAccelerator.h:338-567): DDA over
+// the macrocells whose value range contains the iso value, 10 sub-steps per
+// cell with a linear root, then a central-difference gradient normal.
+// V supplies the grid and ranges (float2 lo/hi per macrocell); field(P)
+// returns the raw scalar (NaN outside). The ray's RNG seed is advanced as
+// the Random construction there does.
+template <class V, class F>
+BN_HD IsoHit iso_intersect(const V &v, const F &field, float isoValue, vec3f org, vec3f dir, float tmin,
+                           float tmax, float rayTMax, RNGSeed &seed, uint64_t geomHash) {
+#pragma clang fp contract(off)
+  IsoHit res;
+  res.hit = false;
+  float t0 = tmin, t1 = tmax;
+  {
+    const vec3f t_lo = (v.worldLo - org) * rcp(dir);
+    const vec3f t_hi = (v.worldHi - org) * rcp(dir);
+    const vec3f t_nr = vmin(t_lo, t_hi);
+    const vec3f t_fr = vmax(t_lo, t_hi);
+    t0 = fmaxf(t0, reduce_max(t_nr));
+    t1 = fminf(t1, reduce_min(t_fr));
+    if (!(t0 <= t1)) return res;
+  }
+  const vec3f dorg = (org - v.mcOrigin) * rcp(v.mcSpacing);
+  const vec3f ddir = dir * rcp(v.mcSpacing);
+  Random2 rng(seed, geomHash);
+  (void)rng;  // opacity < 1 is not supported: the stream is only advanced
+  float tHit = rayTMax;
+  dda3(dorg, ddir, t1, v.mx, v.my, v.mz, [&](int cx, int cy, int cz, float ct0, float ct1) -> bool {
+    const float _t0 = ct0, _t1 = ct1;
+    float rlo = ct0, rhi = fminf(ct1, rayTMax);
+    if (rlo >= rhi) return true;
+    const float2 vr = v.ranges[cx + v.mx * (cy + v.my * cz)];
+    float vlo = vr.x, vhi = vr.y;
+    if (!(isoValue >= vlo && isoValue <= vhi)) return true;
+    float ff0 = 0.f, ff1;
+    float tt1 = ct0;
+    ff1 = field(org + tt1 * dir);
+    const int numSteps = 10;
+    for (int i = 1; i <= numSteps; ++i) {
+      const float tt0 = tt1;
+      ff0 = ff1;
+      tt1 = lerp_l(i / float(numSteps), _t0, _t1);
+      ff1 = field(org + tt1 * dir);
+      if (ff0 != ff0 || ff1 != ff1) continue;
+      vlo = fminf(ff0, ff1);
+      vhi = fmaxf(ff0, ff1);
+      rlo = tt0;
+      rhi = tt1;
+      if (isoValue >= vlo && isoValue <= vhi) {
+        float t = (isoValue - ff0) / (ff1 - ff0);
+        t = lerp_l(t, rlo, rhi);
+        tHit = fminf(tHit, t);
+        if (tHit < rayTMax) return false;
+      }
+    }
+    return true;
+  });
+  if (tHit >= rayTMax) return res;
+  const vec3f osP = org + tHit * dir;
+  const vec3f size = v.worldHi - v.worldLo;
+  const float delta = length(size) * .1f / float(v.mx + v.my + v.mz);
+  const float fP = field(osP);
+  float fPx0 = field(osP + vec3f(-delta, 0.f, 0.f));
+  float fPx1 = field(osP + vec3f(+delta, 0.f, 0.f));
+  float fPy0 = field(osP + vec3f(0.f, -delta, 0.f));
+  float fPy1 = field(osP + vec3f(0.f, +delta, 0.f));
+  float fPz0 = field(osP + vec3f(0.f, 0.f, -delta));
+  float fPz1 = field(osP + vec3f(0.f, 0.f, +delta));
+  float dx = 2.f, dy = 2.f, dz = 2.f;
+  if (fPx0 != fPx0) { dx -= 1.f; fPx0 = fP; }
+  if (fPx1 != fPx1) { dx -= 1.f; fPx1 = fP; }
+  if (fPy0 != fPy0) { dy -= 1.f; fPy0 = fP; }
+  if (fPy1 != fPy1) { dy -= 1.f; fPy1 = fP; }
+  if (fPz0 != fPz0) { dz -= 1.f; fPz0 = fP; }
+  if (fPz1 != fPz1) { dz -= 1.f; fPz1 = fP; }
+  vec3f N = {dx == 0.f ? 0.f : (fPx1 - fPx0) / dx, dy == 0.f ? 0.f : (fPy1 - fPy0) / dy,
+             dz == 0.f ? 0.f : (fPz1 - fPz0) / dz};
+  if (N == vec3f(0.f)) N = -normalize(dir);
+  res.hit = true;
+  res.t = tHit;
+  res.P = osP;
+  res.N = normalize(N);
+  return res;
 }
 
 }  // namespace bnamd

@@ -7,6 +7,10 @@
 
 namespace bnamd {
 
+// Internal geometry kind for point-location BVHs over tetrahedra:
+// vertices = float4 (x, y, z, scalar), indices = int4 per tet.
+constexpr int GEOM_TETS = 3;
+
 // Device-side geometry table entry (one per instanced geom).
 struct GeomDev {
   int kind;

@@ -309,3 +309,60 @@ def product_materials(api, mats):
         else:
             out.append(api.pbr((1.0, 1.0, 1.0), transmission=1.0, ior=m["ior"]))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Workload 5: unstructured tetrahedral mesh
+# ---------------------------------------------------------------------------
+
+# corner c of a hex = (c & 1, (c >> 1) & 1, (c >> 2) & 1); two mirror-image
+# 5-tet splits, alternated by cell parity so shared faces use the same
+# diagonal (a conforming mesh)
+_TETS_EVEN = ((0, 1, 2, 4), (3, 1, 7, 2), (5, 1, 4, 7), (6, 2, 7, 4), (1, 2, 4, 7))
+_TETS_ODD = ((1, 0, 5, 3), (2, 0, 3, 6), (4, 0, 6, 5), (7, 3, 5, 6), (0, 3, 6, 5))
+
+
+def tet_grid(n=128, jitter=0.2, seed=0x5EED0004, blobs=16):
+    """(n+1)^3 vertex grid, jittered by +-jitter cells, each hex split into 5
+    positively oriented tets (det(v1-v0, v2-v0, v3-v0) > 0). Per-vertex
+    scalars: sum of Gaussian blobs normalized to [0, 1].
+    Returns vertices (float32 [V,3]), scalars (float32 [V]), tets (int32 [T,4])."""
+    m = n + 1
+    zz, yy, xx = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    P = np.stack([xx, yy, zz], -1).reshape(-1, 3).astype(np.float64)
+    P += uniform(seed, 1, 3 * m ** 3, -jitter, jitter).reshape(-1, 3)
+    centers = uniform(seed, 2, 3 * blobs, 0.0, float(n)).reshape(-1, 3)
+    sig = uniform(seed, 3, blobs, 0.08 * n, 0.25 * n)
+    amp = uniform(seed, 4, blobs, 0.5, 1.0)
+    s = np.zeros(len(P))
+    for b in range(blobs):
+        d2 = ((P - centers[b]) ** 2).sum(1)
+        s += amp[b] * np.exp(-d2 / (2.0 * sig[b] ** 2))
+    s = (s - s.min()) / max(s.max() - s.min(), 1e-30)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    base = (k * m + j) * m + i
+    corner = [(c & 1) + ((c >> 1) & 1) * m + ((c >> 2) & 1) * m * m for c in range(8)]
+    parity = ((i + j + k) & 1).ravel()
+    base = base.ravel()
+    tets = np.empty((n ** 3, 5, 4), np.int64)
+    for t in range(5):
+        for q in range(4):
+            even = base + corner[_TETS_EVEN[t][q]]
+            odd = base + corner[_TETS_ODD[t][q]]
+            tets[:, t, q] = np.where(parity == 0, even, odd)
+    tets = tets.reshape(-1, 4)
+    # orient: swap the last two vertices where the triple product is negative
+    a, b, c, d = (P[tets[:, q]] for q in range(4))
+    det = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)
+    neg = det < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    return P.astype(np.float32), s.astype(np.float32), tets.astype(np.int32)
+
+
+def workload5(n=128, seed=0x5EED0004):
+    verts, scal, tets = tet_grid(n, seed=seed)
+    c = n / 2.0
+    pos = (c - 0.9 * n, c + 0.6 * n, c - 1.5 * n)
+    cam = dict(position=pos, direction=(c - pos[0], c - pos[1], c - pos[2]), up=(0.0, 1.0, 0.0), fovy=45.0)
+    return dict(vertices=verts, scalars=scal, tets=tets, tf=ramp_tf(), base_density=0.1, iso=0.5,
+                iso_color=(0.9, 0.75, 0.4), camera=cam, size=(1024, 1024), spp=8)

@@ -217,6 +217,67 @@ int bnamd_volume_sample_rays(const bnamd_volume *vol, const bnamd_ray *rays,
                              void *stream);
 
 /* ------------------------------------------------------------------------
+ * Unstructured tetrahedral volumes (barney/umesh): UMeshField with per-vertex
+ * scalars, UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,
+ * UMeshField.h:390-434), the macrocell grid of buildInitialMacroCells
+ * (UMeshField.cu:39-200), MCVolumeAccel<UMeshCuBQLSampler> delta tracking and
+ * MCIsoSurfaceAccel<UMeshCuBQLSampler> iso-surfaces (MCAccelerator.h:338-708).
+ * --------------------------------------------------------------------- */
+typedef struct {
+  const float *vertices; /* device float3 [numVertices] */
+  const float *scalars;  /* device float [numVertices] (per-vertex data) */
+  int numVertices;
+  const int *tets;       /* device int4 [numTets]; positively oriented
+                            (det(v1-v0, v2-v0, v3-v0) > 0), as tetScalar's
+                            inside test requires */
+  int numTets;
+} bnamd_umesh_desc;
+
+typedef struct bnamd_umesh_s bnamd_umesh;
+
+typedef struct {
+  int numTets, numNodes, bvhDepth;
+  int mcDims[3];
+  float worldLower[3], worldUpper[3];
+  float buildMs;   /* BVH build + tet packing */
+  float mcMs;      /* macrocell rasterization */
+} bnamd_umesh_info;
+
+int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bnamd_umesh **out);
+int bnamd_umesh_destroy(bnamd_umesh *mesh);
+int bnamd_umesh_info_get(const bnamd_umesh *mesh, bnamd_umesh_info *info);
+/* values: host float4[numValues]; recomputes the majorants */
+int bnamd_umesh_set_tf(bnamd_umesh *mesh, const float *values, int numValues, float domainLo,
+                       float domainHi, float baseDensity);
+/* host copies of the macrocell ranges (float2 per cell) and majorants */
+int bnamd_umesh_get_mcs(const bnamd_umesh *mesh, float *ranges, float *majorants);
+/* point queries: out[i] = interpolated scalar at points[i] (device float3),
+ * NaN outside the mesh */
+int bnamd_umesh_sample_points(const bnamd_umesh *mesh, const float *points, float *out, int numPoints,
+                              void *stream);
+
+/* iso-surface query per ray: out[i] = {t, N.x, N.y, N.z} of the first
+ * crossing of isoValue in [tmin, tmax] (t = +inf if none); N is the
+ * normalized central-difference gradient (MCAccelerator.h:477-507) */
+int bnamd_umesh_iso_rays(const bnamd_umesh *mesh, const bnamd_ray *rays, float isoValue, float *out,
+                         int numRays, void *stream);
+
+typedef struct {
+  int volume;        /* delta-tracked volume on/off */
+  int iso;           /* iso-surface on/off */
+  float isoValue;
+  float isoColor[3]; /* matte (AnariMatte) color of the iso-surface */
+} bnamd_umesh_render_opts;
+
+/* Volume and/or iso-surface render with the env light (same bounce as the
+ * surface path; volume scattering ends the path as in SCI_VIS_MODE). Both
+ * are intersected per ray, the volume first. stats->samples counts field
+ * evaluations (Woodcock samples and iso sub-steps). */
+int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *camera,
+                       const bnamd_render_params *params, const bnamd_umesh_render_opts *opts,
+                       float *accum, void *stream, bnamd_volume_stats *stats);
+
+/* ------------------------------------------------------------------------
  * Emission-absorption volume rendering and its adjoint (differentiable
  * form; not in the reference). Fixed-step march over a structured density
  * grid sampled like StructuredDataSampler (barney/volume/StructuredData.h:

@@ -1409,9 +1409,11 @@ struct VHit {
   float rgb[3];
 };
 
-// MCAccelerator.h:569-708 + Volume.h:188-223
-VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed, bool exactLog,
-               long long &samples) {
+// MCAccelerator.h:569-708 + Volume.h:188-223. 'v' supplies the macrocell
+// grid and transfer function, field(P) the raw scalar (NaN outside).
+template <class Field>
+VHit vol_isect_g(const ref_volume_s *v, const Field &field, V3 org, V3 dir, float tmin, float tmax, float rayTMax,
+                 Rng &seed, bool exactLog, long long &samples, uint64_t geomHash) {
   VHit res;
   res.hit = false;
   float t0 = tmin, t1 = tmax;
@@ -1423,7 +1425,7 @@ VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, fl
   }
   const V3 dorg = mul(sub(org, v->mcOrigin), inv(v->mcSpacing));
   const V3 ddir = mul(dir, inv(v->mcSpacing));
-  Rng rng = child(seed, fnv3(0, 0, 0));
+  Rng rng = child(seed, geomHash);
   dda3(dorg, ddir, t1, v->m, [&](const int *c, float c0, float c1) -> bool {
     const float maj = v->maj[(size_t)c[0] + (size_t)v->m[0] * ((size_t)c[1] + (size_t)v->m[1] * c[2])];
     if (maj == 0.f) return true;
@@ -1435,7 +1437,7 @@ VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, fl
       t += dt;
       if (t >= upper) return true;
       const V3 P = add(org, scl(t, dir));
-      const float f = vsample(v, P);
+      const float f = field(P);
       float s[4] = {0, 0, 0, 0};
       if (!std::isnan(f)) tfmap(v, f, s);
       ++samples;
@@ -1448,6 +1450,12 @@ VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, fl
     }
   });
   return res;
+}
+
+VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed, bool exactLog,
+               long long &samples) {
+  return vol_isect_g(v, [&](V3 P) { return vsample(v, P); }, org, dir, tmin, tmax, rayTMax, seed, exactLog, samples,
+                     fnv3(0, 0, 0));
 }
 
 }  // namespace
@@ -1582,6 +1590,401 @@ long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const re
           hs.hit = sv.hit;
           PRay dummy;
           add_frag(sum, bounce(p, sh, hs, dummy, gen + 1, 10, 1), cmax);
+        }
+      }
+      acc[0] += sum.x; acc[1] += sum.y; acc[2] += sum.z; acc[3] += alpha;
+    }
+    float *o = accum + 4 * pix;
+    if (p->accumID == 0) { o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3]; }
+    else { o[0] += acc[0]; o[1] += acc[1]; o[2] += acc[2]; o[3] += acc[3]; }
+    nrays += nr;
+    ns += cnt;
+  });
+  if (samples) *samples = ns.load();
+  return nrays.load();
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Unstructured tetrahedral volumes (barney/umesh)
+// ============================================================================
+struct ref_umesh_s {
+  std::vector<float> v4;  // x, y, z, scalar per vertex
+  std::vector<int> tets;  // 4 vertex indices per tet
+  ref_scene_s bvh;        // binned-SAH BVH over the tet boxes (prim = tet)
+  ref_volume_s grid;      // macrocell grid + transfer function
+};
+
+namespace {
+
+// UMeshField.h:346-352 evalToImplicitPlane + :390-434 tetScalar
+float implicit_plane(V3 P, V3 a, V3 b, V3 c) { return dot(sub(P, a), cross(sub(b, a), sub(c, a))); }
+
+bool tet_value(const ref_umesh_s *u, int t, V3 P, float &val) {
+  const int *ix = &u->tets[4 * (size_t)t];
+  const float *a = &u->v4[4 * (size_t)ix[0]], *b = &u->v4[4 * (size_t)ix[1]];
+  const float *c = &u->v4[4 * (size_t)ix[2]], *d = &u->v4[4 * (size_t)ix[3]];
+  const V3 v0 = ld3(a), v1 = ld3(b), v2 = ld3(c), v3 = ld3(d);
+  const float t3 = implicit_plane(P, v0, v1, v2);
+  if (t3 < 0.f) return false;
+  const float t2 = implicit_plane(P, v0, v3, v1);
+  if (t2 < 0.f) return false;
+  const float t1 = implicit_plane(P, v0, v2, v3);
+  if (t1 < 0.f) return false;
+  const float t0 = implicit_plane(P, v1, v3, v2);
+  if (t0 < 0.f) return false;
+  const float scale = 1.f / (t0 + t1 + t2 + t3);
+  val = scale * (t0 * a[3] + t1 * b[3] + t2 * c[3] + t3 * d[3]);
+  return true;
+}
+
+// UMeshCuBQLSampler.h:51-67: box query with the degenerate box {P, P}; the
+// first element whose inside test passes gives the value, NaN if none
+float um_sample(const ref_umesh_s *u, V3 P) {
+  const auto &nodes = u->bvh.nodes;
+  if (nodes.empty()) return NAN;
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  const float p[3] = {P.x, P.y, P.z};
+  while (sp) {
+    const auto &n = nodes[stack[--sp]];
+    bool in = true;
+    for (int k = 0; k < 3; ++k) in = in && p[k] >= n.lo[k] && p[k] <= n.hi[k];
+    if (!in) continue;
+    if (n.left < 0) {
+      for (int i = 0; i < n.count; ++i) {
+        float v;
+        if (tet_value(u, u->bvh.prims[n.first + i].prim, P, v)) return v;
+      }
+    } else {
+      stack[sp++] = n.left + 1;
+      stack[sp++] = n.left;
+    }
+  }
+  return NAN;
+}
+
+// rasterBox(grid, box4f) (MCGrid.cuh:92-122) and rasterTet<D>
+// (UMeshField.cu:39-106)
+struct P4 {
+  float x, y, z, w;
+};
+P4 mid(const P4 &a, const P4 &b) { return P4{0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.5f * (a.w + b.w)}; }
+float edge_len(const P4 &a, const P4 &b) {
+  const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+void raster_box4(ref_volume_s *g, const P4 *q) {
+  float lo[4], hi[4];
+  for (int k = 0; k < 4; ++k) {
+    lo[k] = kInf;
+    hi[k] = -kInf;
+  }
+  for (int i = 0; i < 4; ++i) {
+    const float c[4] = {q[i].x, q[i].y, q[i].z, q[i].w};
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = std::fmin(lo[k], c[k]);
+      hi[k] = std::fmax(hi[k], c[k]);
+    }
+  }
+  if (lo[0] >= hi[0] || lo[1] >= hi[1] || lo[2] >= hi[2]) return;
+  const float org[3] = {g->mcOrigin.x, g->mcOrigin.y, g->mcOrigin.z};
+  const float sp[3] = {g->mcSpacing.x, g->mcSpacing.y, g->mcSpacing.z};
+  int a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = std::min(std::max((int)((lo[k] - org[k]) * (1.f / sp[k])), 0), g->m[k] - 1);
+    b[k] = std::min(std::max((int)((hi[k] - org[k]) * (1.f / sp[k])), 0), g->m[k] - 1);
+  }
+  for (int z = a[2]; z <= b[2]; ++z)
+    for (int y = a[1]; y <= b[1]; ++y)
+      for (int x = a[0]; x <= b[0]; ++x) {
+        const size_t id = (size_t)x + (size_t)g->m[0] * ((size_t)y + (size_t)g->m[1] * z);
+        g->rlo[id] = std::fmin(g->rlo[id], lo[3]);
+        g->rhi[id] = std::fmax(g->rhi[id], hi[3]);
+      }
+}
+
+void raster_tet(ref_volume_s *g, int depth, P4 a, P4 b, P4 c, P4 d) {
+  if (depth == 0) {
+    const P4 q[4] = {a, b, c, d};
+    raster_box4(g, q);
+    return;
+  }
+  const float lab = edge_len(a, b), lac = edge_len(a, c), lad = edge_len(a, d);
+  const float lbc = edge_len(b, c), lbd = edge_len(b, d), lcd = edge_len(c, d);
+  const float mx = std::fmax(std::fmax(std::fmax(std::fmax(std::fmax(lab, lac), lad), lbc), lbd), lcd);
+  P4 oa, ob, oc, od0, od1;
+  if (lab >= mx) { oa = mid(a, b); ob = c; oc = d; od0 = a; od1 = b; }
+  else if (lac >= mx) { oa = mid(a, c); ob = b; oc = d; od0 = a; od1 = c; }
+  else if (lad >= mx) { oa = mid(a, d); ob = b; oc = c; od0 = a; od1 = d; }
+  else if (lbc >= mx) { oa = mid(b, c); ob = a; oc = d; od0 = b; od1 = c; }
+  else if (lbd >= mx) { oa = mid(b, d); ob = a; oc = c; od0 = b; od1 = d; }
+  else { oa = mid(c, d); ob = a; oc = b; od0 = c; od1 = d; }
+  raster_tet(g, depth - 1, oa, ob, oc, od0);
+  raster_tet(g, depth - 1, oa, ob, oc, od1);
+}
+
+struct IHit {
+  bool hit;
+  float t;
+  V3 P, N;
+};
+
+// MCIsoSurfaceAccel::isProg (MCAccelerator.h:338-567)
+template <class Field>
+IHit iso_isect(const ref_volume_s *g, const Field &field, float iso, V3 org, V3 dir, float tmin, float tmax,
+               float rayTMax, Rng &seed, uint64_t geomHash) {
+  IHit res;
+  res.hit = false;
+  float t0 = tmin, t1 = tmax;
+  {
+    const V3 a = mul(sub(g->wlo, org), inv(dir)), b = mul(sub(g->whi, org), inv(dir));
+    t0 = std::fmax(t0, maxc(vmin(a, b)));
+    t1 = std::fmin(t1, minc(vmax(a, b)));
+    if (!(t0 <= t1)) return res;
+  }
+  const V3 dorg = mul(sub(org, g->mcOrigin), inv(g->mcSpacing));
+  const V3 ddir = mul(dir, inv(g->mcSpacing));
+  Rng rng = child(seed, geomHash);
+  (void)rng;
+  float tHit = rayTMax;
+  dda3(dorg, ddir, t1, g->m, [&](const int *c, float c0, float c1) -> bool {
+    float rlo = c0, rhi = std::fmin(c1, rayTMax);
+    if (rlo >= rhi) return true;
+    const size_t id = (size_t)c[0] + (size_t)g->m[0] * ((size_t)c[1] + (size_t)g->m[1] * c[2]);
+    float vlo = g->rlo[id], vhi = g->rhi[id];
+    if (!(iso >= vlo && iso <= vhi)) return true;
+    float ff0 = 0.f, ff1 = field(add(org, scl(c0, dir)));
+    float tt1 = c0;
+    for (int i = 1; i <= 10; ++i) {
+      const float tt0 = tt1;
+      ff0 = ff1;
+      tt1 = lerpl(i / 10.f, c0, c1);
+      ff1 = field(add(org, scl(tt1, dir)));
+      if (std::isnan(ff0) || std::isnan(ff1)) continue;
+      vlo = std::fmin(ff0, ff1);
+      vhi = std::fmax(ff0, ff1);
+      rlo = tt0;
+      rhi = tt1;
+      if (iso >= vlo && iso <= vhi) {
+        float t = (iso - ff0) / (ff1 - ff0);
+        t = lerpl(t, rlo, rhi);
+        tHit = std::fmin(tHit, t);
+        if (tHit < rayTMax) return false;
+      }
+    }
+    return true;
+  });
+  if (tHit >= rayTMax) return res;
+  const V3 P = add(org, scl(tHit, dir));
+  const V3 size = sub(g->whi, g->wlo);
+  const float delta = len(size) * .1f / float(g->m[0] + g->m[1] + g->m[2]);
+  const float fP = field(P);
+  float f[6] = {field(add(P, mk(-delta, 0.f, 0.f))), field(add(P, mk(+delta, 0.f, 0.f))),
+                field(add(P, mk(0.f, -delta, 0.f))), field(add(P, mk(0.f, +delta, 0.f))),
+                field(add(P, mk(0.f, 0.f, -delta))), field(add(P, mk(0.f, 0.f, +delta)))};
+  float dd[3] = {2.f, 2.f, 2.f};
+  for (int k = 0; k < 6; ++k)
+    if (std::isnan(f[k])) {
+      dd[k / 2] -= 1.f;
+      f[k] = fP;
+    }
+  V3 N = mk(dd[0] == 0.f ? 0.f : (f[1] - f[0]) / dd[0], dd[1] == 0.f ? 0.f : (f[3] - f[2]) / dd[1],
+            dd[2] == 0.f ? 0.f : (f[5] - f[4]) / dd[2]);
+  if (eq(N, mk(0, 0, 0))) N = neg(nrm(dir));
+  res.hit = true;
+  res.t = tHit;
+  res.P = P;
+  res.N = nrm(N);
+  return res;
+}
+
+// volume (geometry 0) first, then the iso-surface (geometry 1); each
+// intersection advances the ray seed once
+HitInfo um_trace(const ref_umesh_s *u, bool vol, bool iso, float isoValue, V3 isoColor, V3 org, V3 dir, float tMax,
+                 Rng &seed, long long &samples) {
+  HitInfo h;
+  h.hit = false;
+  float tc = tMax;
+  auto field = [&](V3 P) { return um_sample(u, P); };
+  if (vol) {
+    const VHit vh = vol_isect_g(&u->grid, field, org, dir, 0.f, tMax, tMax, seed, false, samples, fnv3(0, 0, 0));
+    if (vh.hit) {
+      h.hit = true;
+      h.P = add(org, scl(vh.t, dir));
+      h.N = mk(0, 0, 0);
+      h.t = vh.t;
+      h.type = M_PHASE;
+      h.albedo = scl(.7f, mk(vh.rgb[0], vh.rgb[1], vh.rgb[2]));
+      tc = vh.t;
+    }
+  }
+  if (iso) {
+    auto counted = [&](V3 P) {
+      ++samples;
+      return um_sample(u, P);
+    };
+    const IHit ih = iso_isect(&u->grid, counted, isoValue, org, dir, 0.f, tMax, tc, seed, fnv3(0, 1, 0));
+    if (ih.hit) {
+      h.hit = true;
+      h.P = ih.P;
+      h.N = ih.N;
+      h.t = ih.t;
+      h.type = M_LAMBERT;
+      h.albedo = mk(.85f * isoColor.x * (1.f / kPi), .85f * isoColor.y * (1.f / kPi), .85f * isoColor.z * (1.f / kPi));
+      h.metallic = h.roughness = h.ior = 0.f;
+    }
+  }
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int numVertices, const int *tets, int numTets) {
+  auto *u = new ref_umesh_s;
+  u->v4.resize(4 * (size_t)numVertices);
+  for (int i = 0; i < numVertices; ++i) {
+    for (int k = 0; k < 3; ++k) u->v4[4 * (size_t)i + k] = vertices[3 * (size_t)i + k];
+    u->v4[4 * (size_t)i + 3] = scalars[i];
+  }
+  u->tets.assign(tets, tets + 4 * (size_t)numTets);
+  // element boxes (UMeshField cellBounds) and world bounds
+  V3 wlo = mk(kInf, kInf, kInf), whi = mk(-kInf, -kInf, -kInf);
+  for (int t = 0; t < numTets; ++t) {
+    Prim pr;
+    pr.geom = 0;
+    pr.prim = t;
+    pr.key = (uint32_t)t;
+    for (int k = 0; k < 3; ++k) {
+      pr.lo[k] = kInf;
+      pr.hi[k] = -kInf;
+    }
+    for (int q = 0; q < 4; ++q) {
+      const float *p = &u->v4[4 * (size_t)tets[4 * (size_t)t + q]];
+      for (int k = 0; k < 3; ++k) {
+        pr.lo[k] = std::fmin(pr.lo[k], p[k]);
+        pr.hi[k] = std::fmax(pr.hi[k], p[k]);
+      }
+    }
+    wlo = vmin(wlo, mk(pr.lo[0], pr.lo[1], pr.lo[2]));
+    whi = vmax(whi, mk(pr.hi[0], pr.hi[1], pr.hi[2]));
+    u->bvh.prims.push_back(pr);
+  }
+  build_bvh(&u->bvh);
+  // buildInitialMacroCells (UMeshField.cu:173-200)
+  ref_volume_s &g = u->grid;
+  g.wlo = wlo;
+  g.whi = whi;
+  const V3 size = sub(whi, wlo);
+  const float maxWidth = maxc(size);
+  const int MC = 200 + int(std::sqrt(numTets / 1000.f));
+  const float f = (MC - 1) / maxWidth;
+  g.m[0] = 1 + (int)(size.x * f);
+  g.m[1] = 1 + (int)(size.y * f);
+  g.m[2] = 1 + (int)(size.z * f);
+  g.mcOrigin = wlo;
+  g.mcSpacing = mul(size, inv(mk((float)g.m[0], (float)g.m[1], (float)g.m[2])));
+  const size_t nc = (size_t)g.m[0] * g.m[1] * g.m[2];
+  g.rlo.assign(nc, kInf);
+  g.rhi.assign(nc, -kInf);
+  g.maj.assign(nc, 0.f);
+  for (int t = 0; t < numTets; ++t) {
+    P4 p[4];
+    for (int q = 0; q < 4; ++q) {
+      const float *v = &u->v4[4 * (size_t)tets[4 * (size_t)t + q]];
+      p[q] = P4{v[0], v[1], v[2], v[3]};
+    }
+    raster_tet(&g, 5, p[0], p[1], p[2], p[3]);
+  }
+  const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  g.tf.assign(one, one + 8);
+  g.numTF = 2;
+  return u;
+}
+
+void ref_umesh_destroy(ref_umesh *u) { delete u; }
+
+void ref_umesh_set_tf(ref_umesh *u, const float *values, int numValues, float lo, float hi, float base) {
+  ref_volume_s &g = u->grid;
+  g.tf.assign(values, values + 4 * (size_t)numValues);
+  g.numTF = numValues;
+  g.dlo = lo;
+  g.dhi = hi;
+  g.base = base;
+  for (size_t i = 0; i < g.maj.size(); ++i) g.maj[i] = tfmaj(&g, g.rlo[i], g.rhi[i]);
+}
+
+void ref_umesh_mc_dims(const ref_umesh *u, int *dims) {
+  for (int k = 0; k < 3; ++k) dims[k] = u->grid.m[k];
+}
+
+void ref_umesh_get_mcs(const ref_umesh *u, float *ranges, float *maj) {
+  const ref_volume_s &g = u->grid;
+  for (size_t i = 0; i < g.maj.size(); ++i) {
+    if (ranges) {
+      ranges[2 * i] = g.rlo[i];
+      ranges[2 * i + 1] = g.rhi[i];
+    }
+    if (maj) maj[i] = g.maj[i];
+  }
+}
+
+void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, float *out, int n, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * (size_t)i;
+    Rng seed;
+    auto field = [&](V3 P) { return um_sample(u, P); };
+    const V3 org = ld3(r), dir = fix_dir(ld3(r + 4));
+    const IHit h = iso_isect(&u->grid, field, isoValue, org, dir, r[3], r[7], r[7], seed, fnv3(0, 1, 0));
+    float *o = out + 4 * (size_t)i;
+    o[0] = h.hit ? h.t : kInf;
+    o[1] = h.hit ? h.N.x : 0.f;
+    o[2] = h.hit ? h.N.y : 0.f;
+    o[3] = h.hit ? h.N.z : 0.f;
+  });
+}
+
+void ref_umesh_sample(const ref_umesh *u, const float *points, float *out, int n, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) { out[i] = um_sample(u, ld3(points + 3 * (size_t)i)); });
+}
+
+long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_render_params *p, int volume, int iso,
+                           float isoValue, const float *isoColor, float *accum, int y0, int y1, int threads,
+                           long long *samples) {
+  const Cam c = make_cam(cam);
+  const int W = p->width;
+  const V3 icol = ld3(isoColor);
+  const int maxDepth = p->maxPathDepth > 0 ? p->maxPathDepth : 10;
+  const int maxDiff = p->maxDiffuseBounces >= 0 ? p->maxDiffuseBounces : 1;
+  std::atomic<long long> nrays(0), ns(0);
+  parallel_for((y1 - y0) * W, default_threads(threads), [&](int li) {
+    const int ix = li % W, iy = y0 + li / W, pix = ix + iy * W;
+    float acc[4] = {0, 0, 0, 0};
+    long long nr = 0, cnt = 0;
+    for (int pp = 0; pp < p->pathsPerPixel; ++pp) {
+      const int a = p->accumID + pp;
+      PRay ray = cam_ray(c, p, ix, iy, a);
+      const float cmax = 10.f * (1 + a);
+      V3 sum = mk(0, 0, 0);
+      float alpha = 0.f;
+      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+        const V3 d = fix_dir(ray.dir);
+        const HitInfo hi = um_trace(u, volume != 0, iso != 0, isoValue, icol, ray.org, d, ray.tMax, ray.rng, cnt);
+        ++nr;
+        if (gen == 0) alpha = hi.hit ? 1.f : ray.missA;
+        PRay sh;
+        add_frag(sum, bounce(p, ray, hi, sh, gen, maxDepth, maxDiff), cmax);
+        if (sh.tMax > 0.f) {
+          const HitInfo hs = um_trace(u, volume != 0, iso != 0, isoValue, icol, sh.org, fix_dir(sh.dir), sh.tMax,
+                                      sh.rng, cnt);
+          ++nr;
+          PRay dummy;
+          add_frag(sum, bounce(p, sh, hs, dummy, gen + 1, maxDepth, maxDiff), cmax);
         }
       }
       acc[0] += sum.x; acc[1] += sum.y; acc[2] += sum.z; acc[3] += alpha;

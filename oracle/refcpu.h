@@ -127,6 +127,25 @@ int ref_dda_cells(const float *org, const float *dir, float tMax, const int *gri
 long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p,
                             float *accum, int y0, int y1, int threads, long long *samples);
 
+/* --- unstructured tetrahedral volume (barney/umesh) --- */
+typedef struct ref_umesh_s ref_umesh;
+/* vertices float3, per-vertex scalars, tets int4 (positively oriented) */
+ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int numVertices, const int *tets,
+                            int numTets);
+void ref_umesh_destroy(ref_umesh *u);
+void ref_umesh_set_tf(ref_umesh *u, const float *values, int numValues, float lo, float hi, float baseDensity);
+void ref_umesh_mc_dims(const ref_umesh *u, int *dims);
+void ref_umesh_get_mcs(const ref_umesh *u, float *ranges, float *majorants);
+/* point location + barycentric interpolation; NaN outside */
+void ref_umesh_sample(const ref_umesh *u, const float *points, float *out, int n, int threads);
+/* iso-surface hit per ray (8 floats org,tmin,dir,tmax): out = t (inf if
+ * none), normal[3] */
+void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, float *out, int n, int threads);
+/* volume (delta tracking) and/or iso-surface render, rows [y0, y1) */
+long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_render_params *p, int volume,
+                           int iso, float isoValue, const float *isoColor, float *accum, int y0, int y1,
+                           int threads, long long *samples);
+
 #ifdef __cplusplus
 }
 #endif

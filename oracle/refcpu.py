@@ -80,6 +80,15 @@ def load():
         "ref_volume_sample_rays": (None, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_dda_cells": (C.c_int, [P, P, C.c_float, P, P, P, C.c_int]),
         "ref_render_volume": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_umesh_create": (P, [P, P, C.c_int, P, C.c_int]),
+        "ref_umesh_destroy": (None, [P]),
+        "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "ref_umesh_mc_dims": (None, [P, P]),
+        "ref_umesh_get_mcs": (None, [P, P, P]),
+        "ref_umesh_sample": (None, [P, P, P, C.c_int, C.c_int]),
+        "ref_umesh_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
+        "ref_umesh_render": (C.c_longlong, [P, P, P, C.c_int, C.c_int, C.c_float, P, P, C.c_int, C.c_int, C.c_int,
+                                            P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -214,6 +223,63 @@ class Volume:
     def __del__(self):
         try:
             load().ref_volume_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class UMesh:
+    """Oracle tetrahedral mesh: binned-SAH BVH over tet boxes, macrocell grid
+    by rasterTet<5>, transfer function, point sampling and render."""
+
+    def __init__(self, vertices, scalars, tets):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        sc = np.ascontiguousarray(scalars, np.float32).reshape(-1)
+        t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        self.handle = load().ref_umesh_create(v.ctypes.data, sc.ctypes.data, len(v), t.ctypes.data, len(t))
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
+        load().ref_umesh_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]), float(domain[1]),
+                                float(base_density))
+
+    def mc_dims(self):
+        d = np.zeros(3, np.int32)
+        load().ref_umesh_mc_dims(self.handle, d.ctypes.data)
+        return tuple(int(x) for x in d)
+
+    def macrocells(self):
+        mx, my, mz = self.mc_dims()
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        load().ref_umesh_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data)
+        return ranges, maj
+
+    def sample(self, points, threads=0):
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(len(p), np.float32)
+        load().ref_umesh_sample(self.handle, p.ctypes.data, out.ctypes.data, len(p), threads)
+        return out
+
+    def iso_rays(self, rays, iso, threads=0):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty((len(r), 4), np.float32)
+        load().ref_umesh_iso_rays(self.handle, r.ctypes.data, float(iso), out.ctypes.data, len(r), threads)
+        return out
+
+    def render(self, cam, params, volume=True, iso=None, iso_color=(0.8, 0.8, 0.8), y0=0, y1=None, threads=0,
+               accum=None):
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        col = np.ascontiguousarray(iso_color, np.float32)
+        ns = C.c_longlong(0)
+        n = load().ref_umesh_render(self.handle, C.byref(cam), C.byref(params), int(bool(volume)),
+                                    int(iso is not None), float(iso if iso is not None else 0.0), col.ctypes.data,
+                                    acc.ctypes.data, y0, y1, threads, C.byref(ns))
+        return acc, n, ns.value
+
+    def __del__(self):
+        try:
+            load().ref_umesh_destroy(self.handle)
         except Exception:
             pass
 
