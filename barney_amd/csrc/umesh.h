@@ -50,7 +50,15 @@ BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
 // Child boxes are tested in the node's quantized frame: (P - origin) * 2^-e
 // against the 16-bit planes, which carry one quantum of slack on each side,
 // so the test is conservative.
-BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
+// 'hint' (a leaf position or ~0u) is tested first and updated to the tet
+// found: consecutive samples along a ray mostly fall into the same tet. The
+// value differs from the traversal-order answer only for points on a face
+// shared by two tets, where both interpolate the same plane.
+BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
+  if (hint != ~0u) {
+    float val;
+    if (tet_scalar(u.tets + 4ull * hint, P, val)) return val;
+  }
   uint32_t grpBase = 0;
   uint32_t grpBits = 1u | (1u << 8);  // root = internal slot 0
   uint32_t primBase = 0, primMask = 0;
@@ -111,7 +119,10 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
       float val;
-      if (tet_scalar(u.tets + 4ull * (primBase + i), P, val)) return val;
+      if (tet_scalar(u.tets + 4ull * (primBase + i), P, val)) {
+        hint = primBase + i;
+        return val;
+      }
     }
     if ((grpBits & 0xffu) == 0) {
       if (sp == 0) break;
@@ -124,8 +135,8 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
   return __builtin_nanf("");
 }
 
-BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx) {
-  const float f = umesh_locate(u, P, ctx);
+BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
+  const float f = umesh_locate(u, P, ctx, hint);
   if (f != f) return VolSample{0.f, 0.f, 0.f, 0.f};
   return tf_map(u.tf, f);
 }

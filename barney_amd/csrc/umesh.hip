@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(256) k_umesh_sample(UMeshDD u, const float *__
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
-  out[i] = umesh_locate(u, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, ctx);
+  uint32_t hint = ~0u;
+  out[i] = umesh_locate(u, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, ctx, hint);
 }
 
 __global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_ray *__restrict rays, float iso,
@@ -162,8 +163,9 @@ __global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_r
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
   const bnamd_ray r = rays[i];
   RNGSeed seed;
+  uint32_t hint = ~0u;
   const IsoHit h = iso_intersect(
-      u, [&](vec3f P) { return umesh_locate(u, P, ctx); }, iso, {r.org[0], r.org[1], r.org[2]},
+      u, [&](vec3f P) { return umesh_locate(u, P, ctx, hint); }, iso, {r.org[0], r.org[1], r.org[2]},
       fix_dir({r.dir[0], r.dir[1], r.dir[2]}), r.tmin, r.tmax, r.tmax, seed, bn_hash3(0u, 1u, 0u));
   out[i] = h.hit ? make_float4(h.t, h.N.x, h.N.y, h.N.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
 }
@@ -187,14 +189,15 @@ struct UMeshOpts {
 
 // Closest of the volume (geometry 0) and iso-surface (geometry 1) hits for
 // one ray; each intersection program advances the ray's seed once.
+template <bool kVol, bool kIso>
 __device__ ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
-                                RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt) {
+                                RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt, uint32_t &hint) {
   ShadeHit sh;
   sh.hit = false;
   float tClosest = tMax;
-  if (o.volume) {
+  if (kVol) {
     const VolHit vh = volume_intersect_g(
-        u, [&](vec3f P) { return umesh_sample_and_map(u, P, ctx); }, org, dir, 0.f, tMax, tMax, seed, false,
+        u, [&](vec3f P) { return umesh_sample_and_map(u, P, ctx, hint); }, org, dir, 0.f, tMax, tMax, seed, false,
         cnt, bn_hash3(0u, 0u, 0u));
     if (vh.hit) {
       sh.hit = true;
@@ -205,12 +208,12 @@ __device__ ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org,
       tClosest = vh.t;
     }
   }
-  if (o.iso) {
+  if (kIso) {
     const IsoHit ih = iso_intersect(
         u,
         [&](vec3f P) {
           cnt.samples++;
-          return umesh_locate(u, P, ctx);
+          return umesh_locate(u, P, ctx, hint);
         },
         o.isoValue, org, dir, 0.f, tMax, tClosest, seed, bn_hash3(0u, 1u, 0u));
     if (ih.hit) {
@@ -224,7 +227,7 @@ __device__ ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org,
   return sh;
 }
 
-template <int L>
+template <int L, bool kVol, bool kIso>
 __global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp,
                                                       float4 *__restrict accum,
                                                       unsigned long long *__restrict counters) {
@@ -244,6 +247,7 @@ __global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, Ca
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
   VolCounters cnt{0, 0};
   unsigned long long nrays = 0;
+  uint32_t hint = ~0u;
   float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
   const bool active = ix < rp.width && iy < rp.height;
   if (active) {
@@ -256,7 +260,7 @@ __global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, Ca
       float alpha = 0.f;
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
-        const ShadeHit sh = umesh_trace(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt);
+        const ShadeHit sh = umesh_trace<kVol, kIso>(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt, hint);
         ++nrays;
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
         PathRay shadow;
@@ -265,7 +269,8 @@ __global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, Ca
         if (frag.y > 0.f) sum.y += frag.y;
         if (frag.z > 0.f) sum.z += frag.z;
         if (shadow.tMax > 0.f) {
-          const ShadeHit shs = umesh_trace(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt);
+          const ShadeHit shs =
+              umesh_trace<kVol, kIso>(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt, hint);
           ++nrays;
           PathRay dummy;
           frag = vmin(bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
@@ -310,7 +315,14 @@ void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &c
   constexpr int TH = PB / TW;
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   const int grid = ((tiles + 7) / 8) * 8;
-  k_render_umesh<L><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+  if (o.volume && o.iso)
+    k_render_umesh<L, true, true><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+  else if (o.volume)
+    k_render_umesh<L, true, false><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+  else if (o.iso)
+    k_render_umesh<L, false, true><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+  else
+    k_render_umesh<L, false, false><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
 }
 
 void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi, float base) {

@@ -383,28 +383,26 @@ BN_HD IsoHit iso_intersect(const V &v, const F &field, float isoValue, vec3f org
   float tHit = rayTMax;
   dda3(dorg, ddir, t1, v.mx, v.my, v.mz, [&](int cx, int cy, int cz, float ct0, float ct1) -> bool {
     const float _t0 = ct0, _t1 = ct1;
-    float rlo = ct0, rhi = fminf(ct1, rayTMax);
-    if (rlo >= rhi) return true;
+    if (ct0 >= fminf(ct1, rayTMax)) return true;
     const float2 vr = v.ranges[cx + v.mx * (cy + v.my * cz)];
-    float vlo = vr.x, vhi = vr.y;
-    if (!(isoValue >= vlo && isoValue <= vhi)) return true;
-    float ff0 = 0.f, ff1;
-    float tt1 = ct0;
-    ff1 = field(org + tt1 * dir);
+    if (!(isoValue >= vr.x && isoValue <= vr.y)) return true;
+    // sub-step i samples tt_i = lerp(i / 10, t0, t1) (tt_0 = t0); one
+    // field() call site so the point location is inlined once
+    float ff0 = 0.f, ff1 = 0.f, tt0 = 0.f, tt1 = 0.f;
     const int numSteps = 10;
-    for (int i = 1; i <= numSteps; ++i) {
-      const float tt0 = tt1;
+    for (int i = 0; i <= numSteps; ++i) {
+      const float tt = (i == 0) ? ct0 : lerp_l(i / float(numSteps), _t0, _t1);
+      const float f = field(org + tt * dir);
+      tt0 = tt1;
       ff0 = ff1;
-      tt1 = lerp_l(i / float(numSteps), _t0, _t1);
-      ff1 = field(org + tt1 * dir);
+      tt1 = tt;
+      ff1 = f;
+      if (i == 0) continue;
       if (ff0 != ff0 || ff1 != ff1) continue;
-      vlo = fminf(ff0, ff1);
-      vhi = fmaxf(ff0, ff1);
-      rlo = tt0;
-      rhi = tt1;
+      const float vlo = fminf(ff0, ff1), vhi = fmaxf(ff0, ff1);
       if (isoValue >= vlo && isoValue <= vhi) {
         float t = (isoValue - ff0) / (ff1 - ff0);
-        t = lerp_l(t, rlo, rhi);
+        t = lerp_l(t, tt0, tt1);
         tHit = fminf(tHit, t);
         if (tHit < rayTMax) return false;
       }
@@ -415,22 +413,34 @@ BN_HD IsoHit iso_intersect(const V &v, const F &field, float isoValue, vec3f org
   const vec3f osP = org + tHit * dir;
   const vec3f size = v.worldHi - v.worldLo;
   const float delta = length(size) * .1f / float(v.mx + v.my + v.mz);
-  const float fP = field(osP);
-  float fPx0 = field(osP + vec3f(-delta, 0.f, 0.f));
-  float fPx1 = field(osP + vec3f(+delta, 0.f, 0.f));
-  float fPy0 = field(osP + vec3f(0.f, -delta, 0.f));
-  float fPy1 = field(osP + vec3f(0.f, +delta, 0.f));
-  float fPz0 = field(osP + vec3f(0.f, 0.f, -delta));
-  float fPz1 = field(osP + vec3f(0.f, 0.f, +delta));
-  float dx = 2.f, dy = 2.f, dz = 2.f;
-  if (fPx0 != fPx0) { dx -= 1.f; fPx0 = fP; }
-  if (fPx1 != fPx1) { dx -= 1.f; fPx1 = fP; }
-  if (fPy0 != fPy0) { dy -= 1.f; fPy0 = fP; }
-  if (fPy1 != fPy1) { dy -= 1.f; fPy1 = fP; }
-  if (fPz0 != fPz0) { dz -= 1.f; fPz0 = fP; }
-  if (fPz1 != fPz1) { dz -= 1.f; fPz1 = fP; }
-  vec3f N = {dx == 0.f ? 0.f : (fPx1 - fPx0) / dx, dy == 0.f ? 0.f : (fPy1 - fPy0) / dy,
-             dz == 0.f ? 0.f : (fPz1 - fPz0) / dz};
+  // central differences: samples k = 0 (center), then -x, +x, -y, +y, -z, +z
+  float fP = 0.f, lo = 0.f, cnt = 0.f;
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll 1
+  for (int k = 0; k < 7; ++k) {
+    const int axis = (k - 1) >> 1;
+    const float off = (k & 1) ? -delta : delta;
+    const vec3f Q = {osP.x + (axis == 0 ? off : 0.f), osP.y + (axis == 1 ? off : 0.f),
+                     osP.z + (axis == 2 ? off : 0.f)};
+    float f = field(k == 0 ? osP : Q);
+    if (k == 0) {
+      fP = f;
+      continue;
+    }
+    const bool miss = f != f;
+    if (miss) f = fP;
+    if (k & 1) {  // -delta sample of this axis
+      lo = f;
+      cnt = miss ? 1.f : 2.f;
+    } else {      // +delta sample: (f(+) - f(-)) / number of valid samples
+      const float dd = cnt - (miss ? 1.f : 0.f);
+      const float gv = dd == 0.f ? 0.f : (f - lo) / dd;
+      if (axis == 0) gx = gv;
+      else if (axis == 1) gy = gv;
+      else gz = gv;
+    }
+  }
+  vec3f N = {gx, gy, gz};
   if (N == vec3f(0.f)) N = -normalize(dir);
   res.hit = true;
   res.t = tHit;
