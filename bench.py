@@ -7,9 +7,12 @@ single scattering, 1920x1080, 16 paths per pixel. One step = one full
 frame (all 16 paths) through bnamd_volume_render.
 
 Prints ONE JSON line (value = volume Msamples/s), with the roofline of the
-dominant kernel, the CPU oracle timed on a bounded sample of the same frame,
-and the surface megakernel's Mrays/s on the 1,024 tessellated spheres scene
-as an extra figure.
+dominant kernel (its HBM traffic from the committed rocprofv3 byte-counter
+summary), the CPU oracle timed on a bounded sample of the same frame, and
+the other workloads as extra figures: surface megakernel Mrays/s (1,024
+tessellated spheres), the multi-view megakernel vs wavefront renderer
+(heightfield + spheres + cylinders, 64 views), the tetrahedral volume with
+its iso-surface, and the emission-absorption forward/adjoint pass.
 """
 import argparse
 import json
@@ -152,6 +155,51 @@ def batch_bench(api, steps=2):
     return res
 
 
+def umesh_bench(api, steps=2):
+    """Workload 5: 128^3 jittered hex grid split into 5 tets per hex (10.5M
+    tets), per-vertex Gaussian-blob scalars; BVH point location with
+    barycentric sampling, rasterTet macrocell majorants, delta tracking plus
+    an iso-surface at 0.5; 1024x1024, 8 paths per pixel."""
+    from barney_amd import scenes
+    t0 = time.time()
+    w5 = scenes.workload5()
+    m = api.UMesh(w5["vertices"], w5["scalars"], w5["tets"])
+    m.set_tf(w5["tf"], (0.0, 1.0), w5["base_density"])
+    info = m.info()
+    log(f"workload 5 built in {time.time() - t0:.1f}s (BVH {info['buildMs']:.0f} ms, MC {info['mcMs']:.0f} ms)")
+    W, H = w5["size"]
+    cam = api.camera(**w5["camera"])
+    p = api.render_params(W, H, spp=w5["spp"], ambient=1.0, bg=(-1, -1, -1, -1))
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    m.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], accum=acc)
+    ms, st = [], None
+    for _ in range(steps):
+        o = m.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], accum=acc)
+        st = o["stats"]
+        ms.append(st["ms"])
+    t = float(np.median(ms))
+    log(f"workload 5: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M field samples")
+    return {"ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
+            "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"]),
+            "Mrays_per_s": st["raysTraced"] / (t * 1e-3) / 1e6, "dda_steps_per_frame": int(st["ddaSteps"]),
+            "tets": info["numTets"], "bvh_nodes": info["numNodes"], "bvh_depth": info["bvhDepth"],
+            "mc_dims": list(info["mcDims"]), "bvh_build_ms": info["buildMs"], "mc_build_ms": info["mcMs"],
+            "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"])}
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of 'kernel' from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), each counter
+    scaled by what it reads for a copy kernel of known size."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return int((d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024)
+
+
 def ea_bench(steps=3):
     """Workload 4: 256^3 density, 64 orbit views at 512^2, L2 loss against
     images of a second transfer function; forward and checkpointed adjoint."""
@@ -195,6 +243,7 @@ def main():
     ap.add_argument("--no-surface", action="store_true")
     ap.add_argument("--no-ea", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-umesh", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
@@ -241,7 +290,7 @@ def main():
                                "1920x1080, 16 spp",
                    "width": 1920, "height": 1080, "spp": 16, "volume": args.n, "cells_per_mc": 16},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": profile_traffic("k_render_volume"),
                      "kernel": "k_render_volume", "avg_launch_ms": round(kernel_s * 1e3, 3),
                      "alg_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": vb.get("cpu"),
@@ -252,6 +301,8 @@ def main():
         out["extra"]["surface_workload1"] = surface_bench(api)
     if not args.no_batch:
         out["extra"]["batch_workload3"] = batch_bench(api)
+    if not args.no_umesh:
+        out["extra"]["umesh_workload5"] = umesh_bench(api)
     if not args.no_ea:
         out["extra"]["ea_workload4"] = ea_bench()
     print(json.dumps(out), flush=True)
