@@ -1,4 +1,6 @@
-"""Builds libbnamd.so (the HIP kernels + C ABI) for gfx950, in-tree.
+"""Builds libbnamd.so (the HIP kernels + C ABI) for gfx950, in-tree, and the
+C++ host layer over it (librtcore_amd.so, plain g++) with its self-test
+program (tests/cpp/rtcore_amd_selftest).
 
 Each .hip translation unit is compiled separately (in parallel) and linked
 with hipcc into barney_amd/libbnamd.so. Usage: python -m barney_amd.build
@@ -13,6 +15,11 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libbnamd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+RTC_SRC = os.path.join(HERE, "rtcore_amd", "rtcore_amd.cpp")
+RTC_LIB = os.path.join(HERE, "librtcore_amd.so")
+SELFTEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_selftest.cpp")
+SELFTEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_selftest")
 
 SOURCES = [
     "runtime.hip",
@@ -57,7 +64,29 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host_layer(verbose)
     return LIB
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{r.stderr}")
+
+
+def build_host_layer(verbose=False):
+    """The rtc::amd C++ layer is plain host C++ over the C ABI."""
+    common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC"]
+    tmp = RTC_LIB + ".tmp"
+    _run([CXX, *common, "-shared", RTC_SRC, "-o", tmp, "-L" + HERE, "-lbnamd", "-Wl,-rpath,$ORIGIN"],
+         "g++ (rtcore_amd)")
+    os.replace(tmp, RTC_LIB)
+    tmp = SELFTEST + ".tmp"
+    _run([CXX, *common, SELFTEST_SRC, "-o", tmp, "-L" + HERE, "-lrtcore_amd", "-lbnamd",
+          "-Wl,-rpath,$ORIGIN/../../barney_amd"], "g++ (selftest)")
+    os.replace(tmp, SELFTEST)
+    if verbose:
+        print(f"built {RTC_LIB} and {SELFTEST}")
 
 
 if __name__ == "__main__":

@@ -99,6 +99,12 @@ SIGNATURES = {
     "bnamd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "bnamd_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "bnamd_synchronize": (C.c_int, []),
+    "bnamd_memcpy_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "bnamd_memset_async": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]),
+    "bnamd_host_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "bnamd_host_free": (C.c_int, [C.c_void_p]),
+    "bnamd_get_device": (C.c_int, [C.POINTER(C.c_int)]),
+    "bnamd_set_device": (C.c_int, [C.c_int]),
     "bnamd_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "bnamd_ea_checkpoint_count": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
     "bnamd_ea_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -64,6 +64,37 @@ extern "C" int bnamd_memset(void *dst, int value, size_t bytes) {
 extern "C" int bnamd_synchronize(void) {
   return guarded([&] { BN_HIP(hipDeviceSynchronize()); });
 }
+extern "C" int bnamd_memcpy_async(void *dst, const void *src, size_t bytes, void *stream) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream));
+  });
+}
+extern "C" int bnamd_memset_async(void *dst, int value, size_t bytes, void *stream) {
+  return guarded([&] {
+    if (bytes) BN_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+  });
+}
+extern "C" int bnamd_host_malloc(void **ptr, size_t bytes) {
+  return guarded([&] {
+    BN_REQUIRE(ptr, "bnamd_host_malloc: NULL out pointer");
+    *ptr = nullptr;
+    if (bytes) BN_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  });
+}
+extern "C" int bnamd_host_free(void *ptr) {
+  return guarded([&] {
+    if (ptr) BN_HIP(hipHostFree(ptr));
+  });
+}
+extern "C" int bnamd_get_device(int *device) {
+  return guarded([&] {
+    BN_REQUIRE(device, "bnamd_get_device: NULL out pointer");
+    BN_HIP(hipGetDevice(device));
+  });
+}
+extern "C" int bnamd_set_device(int device) {
+  return guarded([&] { BN_HIP(hipSetDevice(device)); });
+}
 
 namespace {
 // Plain streaming copy (one float4 per thread per iteration); used to

@@ -1,0 +1,236 @@
+// rtc::amd host layer over the libbnamd C ABI (see rtcore_amd.h).
+#include "rtcore_amd.h"
+
+#include <cstring>
+
+namespace rtc {
+namespace amd {
+
+void check(int status, const char *what) {
+  if (status != BNAMD_OK) throw std::runtime_error(std::string(what) + ": " + bnamd_last_error());
+}
+
+// ---------------------------------------------------------------- Buffer
+Buffer::Buffer(Device *device, size_t numBytes, const void *initValues) : device(device) {
+  resize(numBytes);
+  if (initValues && numBytes) upload(initValues, numBytes);
+}
+
+Buffer::~Buffer() {
+  if (d_data) (void)bnamd_device_free(d_data);
+}
+
+void Buffer::upload(const void *data, size_t bytes, size_t offset) {
+  if (offset + bytes > numBytes) throw std::runtime_error("rtc::amd::Buffer::upload: out of range");
+  check(bnamd_memcpy_h2d((uint8_t *)d_data + offset, data, bytes), "rtc::amd::Buffer::upload");
+}
+
+void Buffer::resize(size_t newNumBytes) {
+  if (d_data) check(bnamd_device_free(d_data), "rtc::amd::Buffer::resize");
+  d_data = nullptr;
+  numBytes = newNumBytes;
+  if (newNumBytes) check(bnamd_device_malloc(&d_data, newNumBytes), "rtc::amd::Buffer::resize");
+}
+
+// ----------------------------------------------------------- TextureData
+TextureData::TextureData(Device *device, vec3i dims, DataType format, const void *texels)
+    : device(device), dims(dims) {
+  if (format != DataType::FLOAT) throw std::runtime_error("rtc::amd::TextureData: only FLOAT texels");
+  const size_t bytes = sizeof(float) * (size_t)dims.x * dims.y * dims.z;
+  void *p = nullptr;
+  check(bnamd_device_malloc(&p, bytes), "rtc::amd::TextureData");
+  d_texels = (float *)p;
+  if (texels) check(bnamd_memcpy_h2d(d_texels, texels, bytes), "rtc::amd::TextureData");
+}
+
+TextureData::~TextureData() {
+  if (d_texels) (void)bnamd_device_free(d_texels);
+}
+
+// -------------------------------------------------------- GeomType / Geom
+GeomType::GeomType(Device *device, GeomKind kind, const char *name) : device(device), kind(kind), name(name) {}
+
+Geom *GeomType::createGeom() { return new Geom(this); }
+
+Geom::Geom(GeomType *gt) : gt(gt) {}
+
+void Geom::setVertices(Buffer *v, int n) {
+  vertices = v;
+  numVertices = n;
+  if (gt->kind == GeomKind::Spheres) primCount = n;
+}
+
+void Geom::setIndices(Buffer *i, int n) {
+  if (gt->kind == GeomKind::Spheres) throw std::runtime_error("rtc::amd::Geom::setIndices: spheres take no indices");
+  indices = i;
+  numIndices = n;
+  primCount = n;
+}
+
+void Geom::setPrimCount(int n) {
+  if (gt->kind == GeomKind::Triangles)
+    throw std::runtime_error("rtc::amd::Geom::setPrimCount: triangle geoms count their indices");
+  primCount = n;
+}
+
+void Geom::setRadii(Buffer *r, float defRadius) {
+  radii = r;
+  defaultRadius = defRadius;
+}
+
+void Geom::setColors(Buffer *c) { colors = c; }
+
+void Geom::setMaterial(int mat, int user) {
+  materialID = mat;
+  userID = user;
+}
+
+bnamd_geom Geom::desc() const {
+  bnamd_geom g{};
+  g.kind = (int)gt->kind;
+  g.numPrims = primCount;
+  g.numVertices = numVertices;
+  g.materialID = materialID;
+  g.vertices = vertices ? (const float *)vertices->getDD() : nullptr;
+  g.indices = indices ? (const int *)indices->getDD() : nullptr;
+  g.radii = radii ? (const float *)radii->getDD() : nullptr;
+  g.colors = colors ? (const float *)colors->getDD() : nullptr;
+  g.defaultRadius = defaultRadius;
+  g.userID = userID;
+  return g;
+}
+
+// ---------------------------------------------------------- InstanceGroup
+InstanceGroup::InstanceGroup(Device *device, const std::vector<Group *> &groups, const std::vector<int> &instIDs,
+                             const std::vector<affine3f> &xfms)
+    : Group(device), groups(groups), instIDs(instIDs), xfms(xfms) {
+  if (!instIDs.empty() && instIDs.size() != groups.size())
+    throw std::runtime_error("rtc::amd::InstanceGroup: one instance ID per group expected");
+  if (xfms.size() != groups.size())
+    throw std::runtime_error("rtc::amd::InstanceGroup: one transform per group expected");
+}
+
+InstanceGroup::~InstanceGroup() {
+  if (world) (void)bnamd_world_destroy(world);
+}
+
+void InstanceGroup::setTransforms(const std::vector<affine3f> &newXfms) {
+  if (newXfms.size() != groups.size())
+    throw std::runtime_error("rtc::amd::InstanceGroup::setTransforms: one transform per instance expected");
+  xfms = newXfms;
+}
+
+void InstanceGroup::buildAccel() {
+  // each instance references its own group; geometry lists per group
+  std::vector<std::vector<bnamd_geom>> geoms(groups.size());
+  std::vector<bnamd_group> gs(groups.size());
+  std::vector<bnamd_instance> inst(groups.size());
+  for (size_t i = 0; i < groups.size(); ++i) {
+    auto *gg = dynamic_cast<GeomGroup *>(groups[i]);
+    if (!gg) throw std::runtime_error("rtc::amd::InstanceGroup::buildAccel: nested instance groups");
+    for (Geom *g : gg->geoms) geoms[i].push_back(g->desc());
+    gs[i].geoms = geoms[i].data();
+    gs[i].numGeoms = (int)geoms[i].size();
+    inst[i].group = (int)i;
+    inst[i].instID = instIDs.empty() ? (int)i : instIDs[i];
+    const affine3f &x = xfms[i];
+    const float m[12] = {x.vx.x, x.vx.y, x.vx.z, x.vy.x, x.vy.y, x.vy.z,
+                         x.vz.x, x.vz.y, x.vz.z, x.p.x,  x.p.y,  x.p.z};
+    std::memcpy(inst[i].xfm, m, sizeof(m));
+  }
+  bnamd_world *w = nullptr;
+  check(bnamd_world_build(gs.data(), (int)gs.size(), inst.data(), (int)inst.size(),
+                          materials.empty() ? nullptr : materials.data(), (int)materials.size(), nullptr, &w),
+        "rtc::amd::InstanceGroup::buildAccel");
+  if (world) (void)bnamd_world_destroy(world);
+  world = w;
+}
+
+void InstanceGroup::traceRays(const bnamd_ray *d_rays, bnamd_hit *d_hits, int numRays) {
+  if (!world) throw std::runtime_error("rtc::amd::InstanceGroup::traceRays: buildAccel() first");
+  check(bnamd_trace(world, d_rays, d_hits, numRays, nullptr), "rtc::amd::InstanceGroup::traceRays");
+}
+
+// ------------------------------------------------------------------ Device
+Device::Device(int physicalGPU) : physicalID(physicalGPU) { check(bnamd_init(physicalGPU), "rtc::amd::Device"); }
+
+Device::~Device() = default;
+
+void Device::copyAsync(void *dst, const void *src, size_t n) {
+  check(bnamd_memcpy_async(dst, src, n, nullptr), "rtc::amd::Device::copyAsync");
+}
+
+void *Device::allocHost(size_t n) {
+  void *p = nullptr;
+  check(bnamd_host_malloc(&p, n), "rtc::amd::Device::allocHost");
+  return p;
+}
+
+void Device::freeHost(void *mem) { check(bnamd_host_free(mem), "rtc::amd::Device::freeHost"); }
+
+void Device::memsetAsync(void *mem, int value, size_t size) {
+  check(bnamd_memset_async(mem, value, size, nullptr), "rtc::amd::Device::memsetAsync");
+}
+
+void *Device::allocMem(size_t n) {
+  void *p = nullptr;
+  check(bnamd_device_malloc(&p, n), "rtc::amd::Device::allocMem");
+  return p;
+}
+
+void Device::freeMem(void *mem) { check(bnamd_device_free(mem), "rtc::amd::Device::freeMem"); }
+
+void Device::sync() { check(bnamd_synchronize(), "rtc::amd::Device::sync"); }
+
+int Device::setActive() const {
+  int old = 0;
+  check(bnamd_get_device(&old), "rtc::amd::Device::setActive");
+  check(bnamd_set_device(physicalID), "rtc::amd::Device::setActive");
+  return old;
+}
+
+void Device::restoreActive(int oldActive) const {
+  check(bnamd_set_device(oldActive), "rtc::amd::Device::restoreActive");
+}
+
+TextureData *Device::createTextureData(vec3i dims, DataType format, const void *texels) {
+  return new TextureData(this, dims, format, texels);
+}
+
+void Device::freeTextureData(TextureData *t) { delete t; }
+
+Buffer *Device::createBuffer(size_t numBytes, const void *initValues) { return new Buffer(this, numBytes, initValues); }
+
+void Device::freeBuffer(Buffer *b) { delete b; }
+
+void Device::freeGeomType(GeomType *gt) { delete gt; }
+
+void Device::freeGeom(Geom *g) { delete g; }
+
+Group *Device::createTrianglesGroup(const std::vector<Geom *> &geoms) {
+  for (Geom *g : geoms)
+    if (g->gt->kind != GeomKind::Triangles)
+      throw std::runtime_error("rtc::amd::Device::createTrianglesGroup: non-triangle geom");
+  return new TrianglesGeomGroup(this, geoms);
+}
+
+Group *Device::createUserGeomsGroup(const std::vector<Geom *> &geoms) {
+  for (Geom *g : geoms)
+    if (g->gt->kind == GeomKind::Triangles)
+      throw std::runtime_error("rtc::amd::Device::createUserGeomsGroup: triangle geom in a user group");
+  return new UserGeomGroup(this, geoms);
+}
+
+Group *Device::createInstanceGroup(const std::vector<Group *> &groups, const std::vector<int> &instIDs,
+                                   const std::vector<affine3f> &xfms) {
+  return new InstanceGroup(this, groups, instIDs, xfms);
+}
+
+void Device::freeGroup(Group *g) { delete g; }
+
+GeomType *createGeomType_Triangles(Device *device) { return new GeomType(device, GeomKind::Triangles, "Triangles"); }
+GeomType *createGeomType_Spheres(Device *device) { return new GeomType(device, GeomKind::Spheres, "Spheres"); }
+GeomType *createGeomType_Cylinders(Device *device) { return new GeomType(device, GeomKind::Cylinders, "Cylinders"); }
+
+}  // namespace amd
+}  // namespace rtc

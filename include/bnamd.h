@@ -38,6 +38,16 @@ int bnamd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int bnamd_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int bnamd_memset(void *dst, int value, size_t bytes);
 int bnamd_synchronize(void);
+/* copyAsync / memsetAsync on a stream (NULL = default); copies may be
+ * host<->device or device<->device (direction inferred) */
+int bnamd_memcpy_async(void *dst, const void *src, size_t bytes, void *stream);
+int bnamd_memset_async(void *dst, int value, size_t bytes, void *stream);
+/* pinned host memory (allocHost / freeHost) */
+int bnamd_host_malloc(void **ptr, size_t bytes);
+int bnamd_host_free(void *ptr);
+/* active device (setActive / restoreActive) */
+int bnamd_get_device(int *device);
+int bnamd_set_device(int device);
 /* Streaming device copy kernel (bytes % 16 == 0); a traffic reference for
  * hardware byte counters. No counterpart in the reference. */
 int bnamd_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);

@@ -1,0 +1,164 @@
+// Conformance check of the rtc::amd host layer (built with g++ against
+// librtcore_amd.so / libbnamd.so): device memory and pinned host memory,
+// the active-device guard, Triangles / Spheres / Cylinders geometry types,
+// groups, an instance group, closest-hit tracing with analytic answers,
+// and error reporting as std::runtime_error.
+// Exit codes: 0 = all checks passed, 1 = a check failed, 3 = no usable GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../barney_amd/rtcore_amd/rtcore_amd.h"
+
+using namespace rtc::amd;
+
+static int failures = 0;
+#define CHECK(cond)                                                   \
+  do {                                                                \
+    if (!(cond)) {                                                    \
+      std::printf("CHECK FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+      ++failures;                                                     \
+    }                                                                 \
+  } while (0)
+
+static bool near(float a, float b, float rel) { return std::fabs(a - b) <= rel * std::fmax(1.f, std::fabs(b)); }
+
+int main() {
+  Device *dev = nullptr;
+  try {
+    dev = new Device(0);
+  } catch (const std::runtime_error &e) {
+    std::printf("NO_GPU: %s\n", e.what());
+    return 3;
+  }
+  std::printf("%s\n", dev->toString().c_str());
+
+  // memory: pinned host <-> device round trip, memset, active device
+  {
+    const size_t n = 1024;
+    int *h = (int *)dev->allocHost(n * sizeof(int));
+    for (size_t i = 0; i < n; ++i) h[i] = (int)(i * 7);
+    void *d = dev->allocMem(n * sizeof(int));
+    dev->copy(d, h, n * sizeof(int));
+    std::memset(h, 0, n * sizeof(int));
+    dev->copy(h, d, n * sizeof(int));
+    bool ok = true;
+    for (size_t i = 0; i < n; ++i) ok = ok && h[i] == (int)(i * 7);
+    CHECK(ok);
+    dev->memsetAsync(d, 0, n * sizeof(int));
+    dev->sync();
+    dev->copy(h, d, n * sizeof(int));
+    ok = true;
+    for (size_t i = 0; i < n; ++i) ok = ok && h[i] == 0;
+    CHECK(ok);
+    dev->freeMem(d);
+    dev->freeHost(h);
+    const int old = dev->setActive();
+    dev->restoreActive(old);
+  }
+
+  // geometry: a quad (2 triangles) in z = 5 over [-1,1]^2, a sphere of
+  // radius 1 at (3,0,5), a cylinder of radius 0.5 from (-3,-1,5) to (-3,1,5)
+  const float quadV[] = {-1, -1, 5, 1, -1, 5, 1, 1, 5, -1, 1, 5};
+  const int quadI[] = {0, 1, 2, 0, 2, 3};
+  const float sphV[] = {3, 0, 5};
+  const float sphR[] = {1.f};
+  const float cylV[] = {-3, -1, 5, -3, 1, 5};
+  const int cylI[] = {0, 1};
+  const float cylR[] = {0.5f};
+  GeomType *triT = createGeomType_Triangles(dev);
+  GeomType *sphT = createGeomType_Spheres(dev);
+  GeomType *cylT = createGeomType_Cylinders(dev);
+  Buffer *qv = dev->createBuffer(sizeof(quadV), quadV), *qi = dev->createBuffer(sizeof(quadI), quadI);
+  Buffer *sv = dev->createBuffer(sizeof(sphV), sphV), *sr = dev->createBuffer(sizeof(sphR), sphR);
+  Buffer *cv = dev->createBuffer(sizeof(cylV), cylV), *ci = dev->createBuffer(sizeof(cylI), cylI);
+  Buffer *cr = dev->createBuffer(sizeof(cylR), cylR);
+  Geom *quad = triT->createGeom();
+  quad->setVertices(qv, 4);
+  quad->setIndices(qi, 2);
+  Geom *sph = sphT->createGeom();
+  sph->setVertices(sv, 1);
+  sph->setRadii(sr, 1.f);
+  Geom *cyl = cylT->createGeom();
+  cyl->setVertices(cv, 2);
+  cyl->setIndices(ci, 1);
+  cyl->setRadii(cr, 0.5f);
+  Group *tg = dev->createTrianglesGroup({quad});
+  Group *ug = dev->createUserGeomsGroup({sph, cyl});
+  // mixing the kinds the reference keeps apart is rejected
+  bool threw = false;
+  try {
+    delete dev->createTrianglesGroup({sph});
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  CHECK(threw);
+  auto *ig = (InstanceGroup *)dev->createInstanceGroup({tg, ug}, {7, 8}, {affine3f(), affine3f()});
+  ig->buildAccel();
+
+  // rays from the origin; expected closest hits
+  struct Want {
+    float dir[3];
+    float t;
+    int instID, geomID, primID;
+  };
+  const float d2s = std::sqrt(3.f * 3.f + 5.f * 5.f);
+  const float d2c = std::sqrt(3.f * 3.f + 5.f * 5.f);
+  std::vector<Want> want = {
+      {{0.f, 0.f, 1.f}, 5.f, 7, 0, -2},                        // quad (either triangle of the diagonal)
+      {{0.5f, 0.25f, 5.f}, std::sqrt(0.25f + 0.0625f + 25.f), 7, 0, 0},
+      {{3.f / d2s, 0.f, 5.f / d2s}, d2s - 1.f, 8, 0, 0},      // sphere, front point
+      {{-3.f / d2c, 0.f, 5.f / d2c}, d2c - 0.5f, 8, 1, 0},    // cylinder side
+      {{0.f, 0.f, -1.f}, INFINITY, -1, -1, -1},                // miss
+  };
+  std::vector<bnamd_ray> rays(want.size());
+  for (size_t i = 0; i < want.size(); ++i) {
+    bnamd_ray &r = rays[i];
+    r.org[0] = r.org[1] = r.org[2] = 0.f;
+    const float *d = want[i].dir;
+    const float l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; ++k) r.dir[k] = d[k] / l;
+    r.tmin = 0.f;
+    r.tmax = 1e30f;
+  }
+  void *dRays = dev->allocMem(rays.size() * sizeof(bnamd_ray));
+  void *dHits = dev->allocMem(rays.size() * sizeof(bnamd_hit));
+  dev->copy(dRays, rays.data(), rays.size() * sizeof(bnamd_ray));
+  ig->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, (int)rays.size());
+  std::vector<bnamd_hit> hits(rays.size());
+  dev->copy(hits.data(), dHits, hits.size() * sizeof(bnamd_hit));
+  for (size_t i = 0; i < want.size(); ++i) {
+    const bnamd_hit &h = hits[i];
+    const Want &w = want[i];
+    std::printf("ray %zu: t=%g inst=%d geom=%d prim=%d\n", i, h.t, h.instID, h.geomID, h.primID);
+    CHECK(h.instID == w.instID && h.geomID == w.geomID);
+    if (w.primID != -2) CHECK(h.primID == w.primID);
+    if (std::isinf(w.t)) CHECK(h.primID == -1);
+    else CHECK(near(h.t, w.t, 1e-5f));
+  }
+  // a non-identity transform is rejected with a message, not silently wrong
+  affine3f moved;
+  moved.p = {0.f, 0.f, 1.f};
+  ig->setTransforms({moved, affine3f()});
+  threw = false;
+  try {
+    ig->buildAccel();
+  } catch (const std::runtime_error &e) {
+    threw = std::strstr(e.what(), "transform") != nullptr;
+  }
+  CHECK(threw);
+
+  dev->freeMem(dRays);
+  dev->freeMem(dHits);
+  dev->freeGroup(ig);
+  dev->freeGroup(tg);
+  dev->freeGroup(ug);
+  for (Geom *g : {quad, sph, cyl}) dev->freeGeom(g);
+  for (Buffer *b : {qv, qi, sv, sr, cv, ci, cr}) dev->freeBuffer(b);
+  for (GeomType *t : {triT, sphT, cylT}) dev->freeGeomType(t);
+  delete dev;
+  std::printf(failures ? "FAILED %d\n" : "OK\n", failures);
+  return failures ? 1 : 0;
+}
