@@ -10,7 +10,7 @@
 
 namespace bnamd {
 
-enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2, PRIM_TET = 3 };
+enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2, PRIM_TET = 3, PRIM_INSTANCE = 4 };
 // c.w of a PrimRec: type in bits 0-7, flags above.
 // PRIM_SHADOW_TRANSPARENT: a triangle whose material is Glass; barney's
 // triangle anyHit ignores it for shadow rays (Glass::getOpacity returns 0
@@ -45,6 +45,8 @@ static_assert(sizeof(BNode8) == 128, "BNode8 must be one 128B line");
 //   cylinder: a = v0 | flat, b = v1 | -, c = (radius,0,0) | type
 //   tet:      a = box lo | flat, b = box hi | -, c = 0 | type (point-location
 //             BVHs only; the tets themselves are packed separately)
+//   instance: a = world box lo | instance index, b = world box hi | -,
+//             c = 0 | type (top level of a two-level world)
 // 'flat' is the primitive's index in instance-major, geom-major, prim-minor
 // order; it doubles as the deterministic tie-break key at equal t.
 struct PrimRec {
@@ -231,13 +233,15 @@ BN_HD vec3f fix_dir(vec3f d) {
 }
 
 
+// keyBase: high bits of the tie-break key (the instance index in a
+// two-level world, 0 otherwise), OR-ed onto the record's flat index.
 BN_D void test_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, vec3f dir,
-                    float tmin, float tmax, HitRec &best) {
+                    float tmin, float tmax, HitRec &best, uint32_t keyBase = 0u) {
   const float4 a = prims[idx].a;
   const float4 b = prims[idx].b;
   const float4 c = prims[idx].c;
   const uint32_t type = __float_as_uint(c.w) & 0xffu;
-  const uint32_t flat = __float_as_uint(a.w);
+  const uint32_t flat = keyBase | __float_as_uint(a.w);
   const vec3f p0 = {a.x, a.y, a.z};
   if (type == PRIM_TRIANGLE) {
     isect_triangle(org, dir, tmax, p0, {b.x, b.y, b.z}, {c.x, c.y, c.z}, flat, best);
@@ -271,21 +275,86 @@ struct TraverseCtx {
   int maxDepth;
 };
 
-// Closest-hit traversal of the whole world BVH8. Returns false if the stack
-// would have overflowed (the builder reports depth, so this does not happen
-// when the kernel is launched with that depth).
-// kAny: stop at the first occluding primitive (shadow rays; returns true
-// when occluded). Otherwise: closest hit into 'best' (returns false only on
-// stack overflow).
-template <bool kAny>
-BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
-                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx) {
-  best.t = tmax;
-  best.flat = 0xffffffffu;
-  best.u = best.v = 0.f;
-  const vec3f inv = rcp(dir);
-  const uint32_t oct = (dir.x < 0.f ? 1u : 0u) | (dir.y < 0.f ? 2u : 0u) | (dir.z < 0.f ? 4u : 0u);
+// One node visit: slab tests of the 8 quantized child boxes against the
+// ray (inv = 1/dir, octant oct), clipped to [tmin, tcap]. Returns the node
+// group of hit internal children (bits k = slot ^ oct, imask above) and the
+// bit mask of primitives in hit leaf children.
+BN_D void bvh8_visit(const BNode8 *__restrict nodes, uint32_t nodeIdx, vec3f org, vec3f inv, uint32_t oct,
+                     float tmin, float tcap, uint32_t &grpBase, uint32_t &grpBits, uint32_t &primBase,
+                     uint32_t &primMask) {
+  const uint4 *np = (const uint4 *)(nodes + nodeIdx);
+  const uint4 n0 = np[0];
+  const uint4 n1 = np[1];
+  const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
+  const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
+  const uint32_t nimask = n0.w >> 24;
+  const float sx = __uint_as_float(ex << 23) * inv.x;
+  const float sy = __uint_as_float(ey << 23) * inv.y;
+  const float sz = __uint_as_float(ez << 23) * inv.z;
+  // slab offsets from (node origin - ray origin): the rounding error
+  // stays relative to the node's extent, which the one-quantum slack of
+  // the quantized boxes covers; ox * inv - org * inv would cancel
+  // catastrophically far from the world origin
+  const float bx = (ox - org.x) * inv.x;
+  const float by = (oy - org.y) * inv.y;
+  const float bz = (oz - org.z) * inv.z;
+  uint32_t hitInner = 0, hitPrims = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint4 A = np[2 + 3 * h], B = np[3 + 3 * h], C = np[4 + 3 * h];
+    // u32 word pairs per plane: lo.x A.xy, lo.y A.zw, lo.z B.xy,
+    // hi.x B.zw, hi.y C.xy, hi.z C.zw
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = 4 * h + j;
+      const uint32_t wlx = (j < 2) ? A.x : A.y, wly = (j < 2) ? A.z : A.w;
+      const uint32_t wlz = (j < 2) ? B.x : B.y, whx = (j < 2) ? B.z : B.w;
+      const uint32_t why = (j < 2) ? C.x : C.y, whz = (j < 2) ? C.z : C.w;
+      const int sh = (j & 1) * 16;
+      const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
+                  lz = (float)((wlz >> sh) & 0xffffu);
+      const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
+                  hz = (float)((whz >> sh) & 0xffffu);
+      const float tlx = fmaf(lx, sx, bx), thx = fmaf(hx, sx, bx);
+      const float tly = fmaf(ly, sy, by), thy = fmaf(hy, sy, by);
+      const float tlz = fmaf(lz, sz, bz), thz = fmaf(hz, sz, bz);
+      float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), tmin));
+      float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fmaxf(tlz, thz));
+      // conservative widening against rounding in the quantized slab test
+      tn = tn - fabsf(tn) * 2e-6f;
+      tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
+      const bool hit = (tn <= tf) && (lx <= hx);
+      if (hit) {
+        const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
+        if (nimask & (1u << c)) {
+          hitInner |= 1u << (c ^ oct);
+        } else {
+          const uint32_t cnt = meta >> 5, off = meta & 31u;
+          hitPrims |= ((1u << cnt) - 1u) << off;
+        }
+      }
+    }
+  }
+  grpBase = n1.x;
+  grpBits = hitInner | (nimask << 8);
+  primBase = n1.y;
+  primMask = hitPrims;
+}
 
+BN_D uint32_t ray_octant(vec3f dir) {
+  return (dir.x < 0.f ? 1u : 0u) | (dir.y < 0.f ? 2u : 0u) | (dir.z < 0.f ? 4u : 0u);
+}
+
+// Traversal of one BVH8 continuing from 'best' (closest hit so far, keys
+// with prefix keyBase). kAny: stop at the first occluding primitive (shadow
+// rays; returns true when occluded). Otherwise: closest hit into 'best'
+// (returns false only when the stack would overflow, which the builder's
+// reported depth rules out).
+template <bool kAny>
+BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
+                        float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, uint32_t keyBase) {
+  const vec3f inv = rcp(dir);
+  const uint32_t oct = ray_octant(dir);
   uint32_t grpBase = 0;
   uint32_t grpBits = (1u << oct) | (1u << 8);  // root = slot 0 (k = 0 ^ oct)
   uint32_t primBase = 0, primMask = 0;
@@ -302,64 +371,7 @@ BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict pri
         ctx.stack[sp * ctx.stride] = make_uint2(grpBase, grpBits);
         ++sp;
       }
-      const uint4 *np = (const uint4 *)(nodes + nodeIdx);
-      const uint4 n0 = np[0];
-      const uint4 n1 = np[1];
-      const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
-      const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
-      const uint32_t nimask = n0.w >> 24;
-      const float sx = __uint_as_float(ex << 23) * inv.x;
-      const float sy = __uint_as_float(ey << 23) * inv.y;
-      const float sz = __uint_as_float(ez << 23) * inv.z;
-      // slab offsets from (node origin - ray origin): the rounding error
-      // stays relative to the node's extent, which the one-quantum slack of
-      // the quantized boxes covers; ox * inv - org * inv would cancel
-      // catastrophically far from the world origin
-      const float bx = (ox - org.x) * inv.x;
-      const float by = (oy - org.y) * inv.y;
-      const float bz = (oz - org.z) * inv.z;
-      const float tcap = best.t;
-      uint32_t hitInner = 0, hitPrims = 0;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint4 A = np[2 + 3 * h], B = np[3 + 3 * h], C = np[4 + 3 * h];
-        // u32 word pairs per plane: lo.x A.xy, lo.y A.zw, lo.z B.xy,
-        // hi.x B.zw, hi.y C.xy, hi.z C.zw
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int c = 4 * h + j;
-          const uint32_t wlx = (j < 2) ? A.x : A.y, wly = (j < 2) ? A.z : A.w;
-          const uint32_t wlz = (j < 2) ? B.x : B.y, whx = (j < 2) ? B.z : B.w;
-          const uint32_t why = (j < 2) ? C.x : C.y, whz = (j < 2) ? C.z : C.w;
-          const int sh = (j & 1) * 16;
-          const float lx = (float)((wlx >> sh) & 0xffffu), ly = (float)((wly >> sh) & 0xffffu),
-                      lz = (float)((wlz >> sh) & 0xffffu);
-          const float hx = (float)((whx >> sh) & 0xffffu), hy = (float)((why >> sh) & 0xffffu),
-                      hz = (float)((whz >> sh) & 0xffffu);
-          const float tlx = fmaf(lx, sx, bx), thx = fmaf(hx, sx, bx);
-          const float tly = fmaf(ly, sy, by), thy = fmaf(hy, sy, by);
-          const float tlz = fmaf(lz, sz, bz), thz = fmaf(hz, sz, bz);
-          float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), tmin));
-          float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fmaxf(tlz, thz));
-          // conservative widening against rounding in the quantized slab test
-          tn = tn - fabsf(tn) * 2e-6f;
-          tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
-          const bool hit = (tn <= tf) && (lx <= hx);
-          if (hit) {
-            const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
-            if (nimask & (1u << c)) {
-              hitInner |= 1u << (c ^ oct);
-            } else {
-              const uint32_t cnt = meta >> 5, off = meta & 31u;
-              hitPrims |= ((1u << cnt) - 1u) << off;
-            }
-          }
-        }
-      }
-      grpBase = n1.x;
-      grpBits = hitInner | (nimask << 8);
-      primBase = n1.y;
-      primMask = hitPrims;
+      bvh8_visit(nodes, nodeIdx, org, inv, oct, tmin, best.t, grpBase, grpBits, primBase, primMask);
     }
     while (primMask) {
       const uint32_t i = __builtin_ctz(primMask);
@@ -367,7 +379,7 @@ BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict pri
       if (kAny) {
         if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) return true;
       } else {
-        test_prim(prims, primBase + i, org, dir, tmin, tmax, best);
+        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, keyBase);
       }
     }
     if ((grpBits & 0xffu) == 0) {
@@ -379,6 +391,16 @@ BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict pri
     }
   }
   return !kAny;
+}
+
+// Closest-hit (or occlusion) traversal of one BVH8 from scratch.
+template <bool kAny>
+BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
+                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx) {
+  best.t = tmax;
+  best.flat = 0xffffffffu;
+  best.u = best.v = 0.f;
+  return traverse_from<kAny>(nodes, prims, org, dir, tmin, tmax, best, ctx, 0u);
 }
 
 BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,

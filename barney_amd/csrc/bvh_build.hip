@@ -54,6 +54,14 @@ __global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, const Ma
     r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(PRIM_SPHERE));
     lo = c - vec3f(rad);
     hi = c + vec3f(rad);
+  } else if (gd.kind == GEOM_INSTANCES) {
+    const float4 *bx = (const float4 *)gd.vertices;
+    const float4 l = bx[2 * primID], h = bx[2 * primID + 1];
+    lo = {l.x, l.y, l.z};
+    hi = {h.x, h.y, h.z};
+    r.a = make_float4(lo.x, lo.y, lo.z, __uint_as_float(i));
+    r.b = make_float4(hi.x, hi.y, hi.z, 0.f);
+    r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(PRIM_INSTANCE));
   } else if (gd.kind == GEOM_TETS) {
     const int4 ix = ((const int4 *)gd.indices)[primID];
     const float4 *v4 = (const float4 *)gd.vertices;
@@ -430,7 +438,7 @@ void prim_box(const PrimRec &r, uint32_t type, float lo[3], float hi[3]) {
     } else if (type == PRIM_SPHERE) {
       lo[k] = a[k] - r.b.x;
       hi[k] = a[k] + r.b.x;
-    } else if (type == PRIM_TET) {
+    } else if (type == PRIM_TET || type == PRIM_INSTANCE) {
       lo[k] = a[k];
       hi[k] = b[k];
     } else {
@@ -540,7 +548,7 @@ void validate_bvh(const World &w) {
     uint32_t flat, type;
     memcpy(&flat, &prims[p].a.w, 4);
     memcpy(&type, &prims[p].c.w, 4);
-    BN_REQUIRE(flat < w.numPrims && (type & 0xffu) <= 3u, "BVH8 check: corrupt primitive record");
+    BN_REQUIRE(flat < w.numPrims && (type & 0xffu) <= 4u, "BVH8 check: corrupt primitive record");
   }
 }
 

@@ -50,14 +50,15 @@ __device__ __forceinline__ int find_view(const int *off, int n, int g) {
   return lo;
 }
 
+template <bool TL>
 __device__ __forceinline__ ShadeHit shade_hit(const WorldDev &w, vec3f org, vec3f dir, float tmax, const HitRec &h) {
   ShadeHit sh;
   if (h.flat == 0xffffffffu) {
     sh.hit = false;
     return sh;
   }
-  const SurfaceHit s = surface_epilogue(w, org, dir, 0.f, tmax, h);
-  const GeomDev &g = w.geoms[s.geom];
+  const SurfaceHit s = surface_epilogue<TL>(w, org, dir, 0.f, tmax, h);
+  const GeomDev &g = *s.g;
   const MaterialDev &m = w.materials[g.materialID];
   vec3f color = {m.color[0], m.color[1], m.color[2]};
   if (g.colors) color = load3(g.colors + 3 * s.primID);
@@ -131,7 +132,7 @@ __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long lon
 // Megakernel: L lanes per pixel (each looping over its share of the paths),
 // 4 wavefronts per workgroup covering a TW x TH pixel tile of one view.
 // ===========================================================================
-template <int L>
+template <int L, bool TL>
 __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, RenderDD rp, int spp,
                                                     float4 *__restrict accum,
                                                     unsigned long long *__restrict counters) {
@@ -171,15 +172,16 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        traverse_closest(w.nodes, w.prims, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx);
         ++nrays;
-        const ShadeHit sh = shade_hit(w, ray.org, dir, ray.tMax, h);
+        const ShadeHit sh = shade_hit<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
         PathRay shadow;
         add_frag(sum, bounce(rpp, ray, sh, shadow, gen), clampMax);
         if (shadow.tMax > 0.f) {
           ShadeHit shs;
-          shs.hit = traverse_occluded(w.nodes, w.prims, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, ctx);
+          HitRec hs;
+          shs.hit = world_traverse<TL, true>(w, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, hs, ctx);
           ++nshadow;
           PathRay dummy;
           add_frag(sum, bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
@@ -327,6 +329,7 @@ __global__ void k_wf_generate(WfDD e) {
   if (pid == 0) e.cnt->count[0] = total;
 }
 
+template <bool TL>
 __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->count[depth];
@@ -338,22 +341,23 @@ __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
     const vec3f org = {a.x, a.y, a.z};
     const vec3f dir = fix_dir({b.x, b.y, b.z});
     HitRec h;
-    traverse_closest(e.w.nodes, e.w.prims, org, dir, 0.f, a.w, h, ctx);
+    world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx);
     e.hits[i] = h;
     ++nr;
   });
   wave_add(nr, &e.cnt->rays);
 }
 
+template <bool TL>
 __device__ __forceinline__ unsigned bsdf_key(const WfDD &e, const HitRec &h) {
   if (h.flat == 0xffffffffu) return 0u;
-  const int g = find_geom(e.w.geoms, e.w.numGeoms, h.flat);
-  const MaterialDev &m = e.w.materials[e.w.geoms[g].materialID];
+  const MaterialDev &m = e.w.materials[hit_geom<TL>(e.w, h.flat).materialID];
   if (m.type != BNAMD_MAT_PBR) return 1u;
   return (m.ior != 1.f && m.transmission >= 1e-3f) ? 3u : 2u;
 }
 
 // Pass 1: keys and per-block key counts over the block's segment.
+template <bool TL>
 __global__ void __launch_bounds__(256) k_wf_sort_count(WfDD e, int depth) {
   __shared__ unsigned int hist[4];
   if (threadIdx.x < 4) hist[threadIdx.x] = 0u;
@@ -362,7 +366,7 @@ __global__ void __launch_bounds__(256) k_wf_sort_count(WfDD e, int depth) {
   const unsigned seg = sort_segment(n);
   const unsigned lo = blockIdx.x * seg, hi = min(n, lo + seg);
   for (unsigned i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const unsigned k = bsdf_key(e, e.hits[i]);
+    const unsigned k = bsdf_key<TL>(e, e.hits[i]);
     e.keys[i] = k;
     atomicAdd(&hist[k], 1u);
   }
@@ -459,6 +463,7 @@ __device__ __forceinline__ void accum_add(float4 *accum, int pix, vec3f f, float
   if (f.z > 0.f) atomicAdd(p + 2, f.z);
 }
 
+template <bool TL>
 __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   const unsigned n = e.cnt->count[depth];
   const PathRec *__restrict qin = e.q[depth & 1];
@@ -503,7 +508,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
         }
       }
       const vec3f dir = fix_dir(ray.dir);
-      const ShadeHit sh = shade_hit(e.w, ray.org, dir, ray.tMax, h);
+      const ShadeHit sh = shade_hit<TL>(e.w, ray.org, dir, ray.tMax, h);
       const float clampMax = 10.f * (1 + rpp.accumID);
       if (depth == 0) {
         const float alpha = sh.hit ? 1.f : ray.missAlpha;
@@ -536,6 +541,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   }
 }
 
+template <bool TL>
 __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->shadow[depth];
@@ -545,7 +551,8 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
     const ShadowRec s = e.sq[i];
     const vec3f org = {s.a.x, s.a.y, s.a.z};
     const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
-    const bool occluded = traverse_occluded(e.w.nodes, e.w.prims, org, dir, 0.f, s.a.w, ctx);
+    HitRec hs;
+    const bool occluded = world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx);
     ++nr;
     if (!occluded) {
       const unsigned pid = __float_as_uint(s.b.w);
@@ -609,10 +616,15 @@ int num_cus() {
 // Trace/shadow grid: blocksPerCU workgroups per CU, capped at what the
 // occupancy of the two traversal kernels allows (static slices assume every
 // workgroup is resident).
-int trace_grid(int blocksPerCU, size_t lds) {
+int trace_grid(int blocksPerCU, size_t lds, bool twoLevel) {
   int occT = 0, occS = 0;
-  BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace, 256, lds));
-  BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow, 256, lds));
+  if (twoLevel) {
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<true>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<true>, 256, lds));
+  } else {
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<false>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<false>, 256, lds));
+  }
   int occ = std::max(1, std::min(occT, occS));
   if (blocksPerCU > 0) occ = std::min(occ, blocksPerCU);
   return occ * num_cus();
@@ -673,7 +685,10 @@ void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
   const int tiles = hv.tileOffset.back();
   const int grid = ((tiles + 7) / 8) * 8;
   const size_t lds = trace_lds_bytes(w.depth, 256);
-  k_batch_mega<L><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
+  if (w.twoLevel)
+    k_batch_mega<L, true><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
+  else
+    k_batch_mega<L, false><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
   BN_HIP(hipGetLastError());
 }
 
@@ -686,19 +701,25 @@ void render_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
 }
 
 // Records the kernels of one batch (generate + maxDepth+1 bounces) on s.
-void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds, hipStream_t s) {
+template <bool TL>
+void enqueue_wavefront_t(const WfDD &e, int maxDepth, int gridPersist, size_t lds, hipStream_t s) {
   const unsigned total = (unsigned)e.numPixels * (unsigned)e.sppBatch;
   k_wf_reset<<<1, 256, 0, s>>>(e.cnt);
   k_wf_generate<<<div_up(total, 256), 256, 0, s>>>(e);
   const int gridStream = std::min(div_up(total, 256), 8 * num_cus());
   for (int d = 0; d <= maxDepth; ++d) {
-    k_wf_trace<<<gridPersist, 256, lds, s>>>(e, d);
-    k_wf_sort_count<<<kSortBlocks, 256, 0, s>>>(e, d);
+    k_wf_trace<TL><<<gridPersist, 256, lds, s>>>(e, d);
+    k_wf_sort_count<TL><<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_sort_scan<<<1, kSortBlocks, 0, s>>>(e);
     k_wf_sort_scatter<<<kSortBlocks, 256, 0, s>>>(e, d);
-    k_wf_shade<<<gridStream, 256, 0, s>>>(e, d);
-    k_wf_shadow<<<gridPersist, 256, lds, s>>>(e, d);
+    k_wf_shade<TL><<<gridStream, 256, 0, s>>>(e, d);
+    k_wf_shadow<TL><<<gridPersist, 256, lds, s>>>(e, d);
   }
+}
+
+void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds, bool twoLevel, hipStream_t s) {
+  if (twoLevel) enqueue_wavefront_t<true>(e, maxDepth, gridPersist, lds, s);
+  else enqueue_wavefront_t<false>(e, maxDepth, gridPersist, lds, s);
 }
 
 float render_wavefront(const World &w, const bnamd_views *views, const RenderDD &rp0, int spp, float4 *accum,
@@ -723,7 +744,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
   st.cnt.ensure(1);
   const size_t lds = trace_lds_bytes(w.depth, 256);
   BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_batch: BVH too deep for the LDS stack");
-  const int gridPersist = trace_grid(tune.traceBlocksPerCU, lds);
+  const int gridPersist = trace_grid(tune.traceBlocksPerCU, lds, w.twoLevel);
   if (rp0.accumID == 0)
     k_fill_f4<<<std::min(div_up(hv.numPixels, 256), 4096), 256, 0, s>>>(accum, (size_t)hv.numPixels);
   hipEvent_t e0, e1;
@@ -766,7 +787,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
         BN_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t g;
         BN_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        enqueue_wavefront(e, maxDepth, gridPersist, lds, cs);
+        enqueue_wavefront(e, maxDepth, gridPersist, lds, w.twoLevel, cs);
         BN_HIP(hipStreamEndCapture(cs, &g));
         BN_HIP(hipGraphInstantiate(&st.exec, g, nullptr, nullptr, 0));
         BN_HIP(hipGraphDestroy(g));
@@ -776,7 +797,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
       BN_HIP(hipGraphLaunch(st.exec, s));
       ++replays;
     } else {
-      enqueue_wavefront(e, maxDepth, gridPersist, lds, s);
+      enqueue_wavefront(e, maxDepth, gridPersist, lds, w.twoLevel, s);
     }
     WfCounters hc;
     BN_HIP(hipMemcpyAsync(&hc, st.cnt.ptr, sizeof(hc), hipMemcpyDeviceToHost, s));

@@ -22,6 +22,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return t < nb ? t : -1;
 }
 
+template <bool TL>
 __device__ __forceinline__ ShadeHit shade_surface(const WorldDev &w, vec3f org, vec3f dir, float tmax,
                                                   const HitRec &h) {
   ShadeHit sh;
@@ -29,8 +30,8 @@ __device__ __forceinline__ ShadeHit shade_surface(const WorldDev &w, vec3f org, 
     sh.hit = false;
     return sh;
   }
-  const SurfaceHit s = surface_epilogue(w, org, dir, 0.f, tmax, h);
-  const GeomDev &g = w.geoms[s.geom];
+  const SurfaceHit s = surface_epilogue<TL>(w, org, dir, 0.f, tmax, h);
+  const GeomDev &g = *s.g;
   const MaterialDev &m = w.materials[g.materialID];
   vec3f color = {m.color[0], m.color[1], m.color[2]};
   if (g.colors) color = load3(g.colors + 3 * s.primID);
@@ -50,6 +51,7 @@ __device__ __forceinline__ void add_fragment(vec3f &sum, vec3f frag, float clamp
   if (frag.z > 0.f) sum.z += frag.z;
 }
 
+template <bool TL>
 __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam, RenderDD rp, int spp,
                                                         float4 *__restrict accum, int4 *__restrict ids,
                                                         unsigned long long *__restrict counters) {
@@ -77,18 +79,18 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        traverse_closest(w.nodes, w.prims, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx);
         ++nrays;
-        const ShadeHit sh = shade_surface(w, ray.org, dir, ray.tMax, h);
+        const ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) {
           alpha = sh.hit ? 1.f : ray.missAlpha;
           if (ids && p == 0 && accum0 == 0) {
             int4 rec = make_int4(-1, -1, -1, __float_as_int(BN_INF));
             if (sh.hit) {
-              const SurfaceHit s = surface_epilogue(w, ray.org, dir, 0.f, ray.tMax, h);
-              const GeomDev &g = w.geoms[s.geom];
+              const SurfaceHit s = surface_epilogue<TL>(w, ray.org, dir, 0.f, ray.tMax, h);
+              const GeomDev &g = *s.g;
               // Spheres.dev.cu:192-195 writes 13+rank as the instance ID.
-              rec = make_int4(s.primID, g.kind == BNAMD_GEOM_SPHERES ? 13 : g.instID, g.userID,
+              rec = make_int4(s.primID, g.kind == BNAMD_GEOM_SPHERES ? 13 : s.instID, g.userID,
                               __float_as_int(h.t));
             }
             ids[pix] = rec;
@@ -100,7 +102,8 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
         if (shadow.tMax > 0.f) {
           const vec3f sdir = fix_dir(shadow.dir);
           ShadeHit shs;
-          shs.hit = traverse_occluded(w.nodes, w.prims, shadow.org, sdir, 0.f, shadow.tMax, ctx);
+          HitRec hs;
+          shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx);
           ++nrays;
           PathRay dummy;
           frag = bounce(rpp, shadow, shs, dummy, gen + 1);
@@ -154,8 +157,12 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     BN_HIP(hipEventCreate(&e0));
     BN_HIP(hipEventCreate(&e1));
     BN_HIP(hipEventRecord(e0, s));
-    k_render_surface<<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                               (int4 *)ids, counter.ptr);
+    if (w.twoLevel)
+      k_render_surface<true><<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+                                                       (int4 *)ids, counter.ptr);
+    else
+      k_render_surface<false><<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+                                                        (int4 *)ids, counter.ptr);
     BN_HIP(hipGetLastError());
     BN_HIP(hipEventRecord(e1, s));
     BN_HIP(hipEventSynchronize(e1));

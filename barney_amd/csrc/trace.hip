@@ -9,6 +9,7 @@ namespace bnamd {
 
 namespace {
 
+template <bool TL>
 __global__ void __launch_bounds__(256) k_trace(WorldDev w, const bnamd_ray *__restrict rays,
                                                bnamd_hit *__restrict hits, int n) {
   extern __shared__ uint2 lds_stack[];
@@ -20,7 +21,7 @@ __global__ void __launch_bounds__(256) k_trace(WorldDev w, const bnamd_ray *__re
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
   HitRec h;
   bnamd_hit out;
-  if (!traverse_closest(w.nodes, w.prims, org, dir, r.tmin, r.tmax, h, ctx)) {
+  if (!world_traverse<TL, false>(w, org, dir, r.tmin, r.tmax, h, ctx)) {
     h.flat = 0xffffffffu;
     h.t = __builtin_nanf("");
   }
@@ -34,12 +35,11 @@ __global__ void __launch_bounds__(256) k_trace(WorldDev w, const bnamd_ray *__re
     out.instIndex = -1;
     out.N[0] = out.N[1] = out.N[2] = 0.f;
   } else {
-    const SurfaceHit s = surface_epilogue(w, org, dir, r.tmin, r.tmax, h);
-    const GeomDev &g = w.geoms[s.geom];
+    const SurfaceHit s = surface_epilogue<TL>(w, org, dir, r.tmin, r.tmax, h);
     out.primID = s.primID;
-    out.geomID = g.geomID;
-    out.instID = g.instID;
-    out.instIndex = g.instIndex;
+    out.geomID = s.g->geomID;
+    out.instID = s.instID;
+    out.instIndex = s.instIndex;
     out.N[0] = s.Ng.x;
     out.N[1] = s.Ng.y;
     out.N[2] = s.Ng.z;
@@ -68,82 +68,225 @@ struct bnamd_world_s {
   World w;
 };
 
+namespace bnamd {
+namespace {
+
+void check_geom(const bnamd_geom &src, int numMaterials) {
+  BN_REQUIRE(src.kind >= 0 && src.kind <= 2, "bnamd_world_build: unknown geom kind");
+  BN_REQUIRE(src.numPrims >= 0, "bnamd_world_build: negative prim count");
+  BN_REQUIRE(src.numPrims == 0 || src.vertices, "bnamd_world_build: geom without vertices");
+  BN_REQUIRE(src.kind == BNAMD_GEOM_SPHERES || src.numPrims == 0 || src.indices,
+             "bnamd_world_build: geom without indices");
+  BN_REQUIRE(src.kind != BNAMD_GEOM_CYLINDERS || src.numPrims == 0 || src.radii,
+             "bnamd_world_build: cylinders need per-primitive radii");
+  BN_REQUIRE(src.materialID >= 0 && src.materialID < std::max(numMaterials, 1),
+             "bnamd_world_build: materialID out of range");
+}
+
+GeomDev make_geom(const bnamd_geom &src, int geomID, int instIndex, int instID, uint64_t offset) {
+  GeomDev g{};
+  g.kind = src.kind;
+  g.numPrims = src.numPrims;
+  g.primOffset = (uint32_t)offset;
+  g.geomID = geomID;
+  g.instIndex = instIndex;
+  g.instID = instID;
+  g.materialID = src.materialID;
+  g.userID = src.userID;
+  g.vertices = src.vertices;
+  g.indices = src.indices;
+  g.radii = src.radii;
+  g.colors = src.colors;
+  g.defaultRadius = src.defaultRadius;
+  return g;
+}
+
+std::vector<MaterialDev> make_materials(const bnamd_material *materials, int numMaterials) {
+  std::vector<MaterialDev> mats(std::max(numMaterials, 1));
+  for (int m = 0; m < (int)mats.size(); ++m) {
+    MaterialDev md{};
+    if (m < numMaterials) {
+      const bnamd_material &s = materials[m];
+      md.type = s.type;
+      md.color[0] = s.color[0]; md.color[1] = s.color[1]; md.color[2] = s.color[2];
+      md.opacity = s.opacity; md.metallic = s.metallic; md.roughness = s.roughness;
+      md.transmission = s.transmission; md.ior = s.ior; md.specular = s.specular;
+      if (s.type == BNAMD_MAT_PBR && !(s.ior != 1.f && s.transmission >= 1e-3f)) {
+        // NVisii stores its parameters as halfs (NVisii.h:1062-1083);
+        // roughness is clamped to [.001, .9] (AnariPBR.h:86)
+        md.metallic = round_half(s.metallic);
+        md.roughness = round_half(std::min(std::max(s.roughness, .001f), .9f));
+        md.ior = round_half(s.ior);
+      }
+    } else {
+      md.type = BNAMD_MAT_MATTE;
+      md.color[0] = md.color[1] = md.color[2] = 0.8f;
+      md.opacity = 1.f;
+    }
+    mats[m] = md;
+  }
+  return mats;
+}
+
+// rcp(affine3f) in double, rounded once to float: linear part inverted by
+// the adjugate, p' = -(L^-1 p). m = columns vx, vy, vz, then p.
+void invert_affine(const float *m, float *out) {
+  const double a = m[0], b = m[3], c = m[6];  // row 0: vx.x vy.x vz.x
+  const double d = m[1], e = m[4], f = m[7];
+  const double g = m[2], h = m[5], i = m[8];
+  const double A = e * i - f * h, B = -(d * i - f * g), Cc = d * h - e * g;
+  const double det = a * A + b * B + c * Cc;
+  BN_REQUIRE(det != 0.0 && std::isfinite(det), "bnamd_world_build: singular instance transform");
+  const double r = 1.0 / det;
+  // inverse rows
+  const double inv[3][3] = {{A * r, -(b * i - c * h) * r, (b * f - c * e) * r},
+                            {B * r, (a * i - c * g) * r, -(a * f - c * d) * r},
+                            {Cc * r, -(a * h - b * g) * r, (a * e - b * d) * r}};
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row) out[3 * col + row] = (float)inv[row][col];
+  for (int row = 0; row < 3; ++row)
+    out[9 + row] = (float)-(inv[row][0] * m[9] + inv[row][1] * m[10] + inv[row][2] * m[11]);
+}
+
+// world box of an object box under o2w, rounded outward
+void xfm_box(const float *m, const float *lo, const float *hi, float4 &blo, float4 &bhi) {
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; ++c) {
+    const double p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+    for (int k = 0; k < 3; ++k) {
+      const double v = m[k] * p[0] + m[3 + k] * p[1] + m[6 + k] * p[2] + m[9 + k];
+      mn[k] = std::min(mn[k], v);
+      mx[k] = std::max(mx[k], v);
+    }
+  }
+  float l[3], u[3];
+  for (int k = 0; k < 3; ++k) {
+    l[k] = (float)mn[k];
+    if ((double)l[k] > mn[k]) l[k] = std::nextafter(l[k], -INFINITY);
+    u[k] = (float)mx[k];
+    if ((double)u[k] < mx[k]) u[k] = std::nextafter(u[k], INFINITY);
+  }
+  blo = make_float4(l[0], l[1], l[2], 0.f);
+  bhi = make_float4(u[0], u[1], u[2], 0.f);
+}
+
+// Two levels: one BVH8 per referenced group in object space, and a BVH8
+// over the instances' world boxes (InstanceGroup::buildAccel,
+// rtcore/cuda/cudaGroup.cu:144-258).
+void build_two_level(World &w, const bnamd_group *groups, int numGroups, const bnamd_instance *instances,
+                     int numInstances, const std::vector<MaterialDev> &mats, int numMaterials, hipStream_t s) {
+  w.twoLevel = true;
+  std::vector<int> blasOf(numGroups, -1);
+  uint32_t maxPrims = 1;
+  int maxDepth = 1;
+  uint64_t total = 0, nodes = 0;
+  for (int ii = 0; ii < numInstances; ++ii) {
+    const int gidx = instances[ii].group;
+    if (blasOf[gidx] < 0) {
+      auto b = std::make_unique<World>();
+      const bnamd_group &grp = groups[gidx];
+      uint64_t offset = 0;
+      for (int gi = 0; gi < grp.numGeoms; ++gi) {
+        const bnamd_geom &src = grp.geoms[gi];
+        check_geom(src, numMaterials);
+        if (src.numPrims == 0) continue;
+        b->hgeoms.push_back(make_geom(src, gi, -1, 0, offset));
+        offset += (uint64_t)src.numPrims;
+      }
+      BN_REQUIRE(offset < 0xfffffff0ull, "bnamd_world_build: too many primitives in a group");
+      b->numPrims = (uint32_t)offset;
+      b->materials.upload(mats.data(), mats.size(), s);
+      build_world_bvh(*b, s);
+      maxPrims = std::max(maxPrims, b->numPrims);
+      maxDepth = std::max(maxDepth, b->depth);
+      blasOf[gidx] = (int)w.blas.size();
+      w.blas.push_back(std::move(b));
+    }
+    const World &b = *w.blas[blasOf[gidx]];
+    total += b.numPrims;
+    nodes += b.numNodes;
+  }
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < (uint64_t)maxPrims) ++bits;
+  BN_REQUIRE(bits < 32 && (uint64_t)numInstances < (1ull << (32 - bits)),
+             "bnamd_world_build: instances x primitives per group exceed the 32-bit hit key");
+  w.localBits = bits;
+  std::vector<InstDev> insts(numInstances);
+  std::vector<float4> boxes(2ull * numInstances);
+  for (int ii = 0; ii < numInstances; ++ii) {
+    const bnamd_instance &in = instances[ii];
+    const World &b = *w.blas[blasOf[in.group]];
+    InstDev &I = insts[ii];
+    for (int k = 0; k < 12; ++k) I.o2w[k] = in.xfm[k];
+    invert_affine(in.xfm, I.w2o);
+    I.nodes = b.nodes.ptr;
+    I.prims = b.prims.ptr;
+    I.geoms = b.geoms.ptr;
+    I.numGeoms = (int)b.hgeoms.size();
+    I.instID = in.instID;
+    const float zero[3] = {0.f, 0.f, 0.f};
+    xfm_box(in.xfm, b.numPrims ? b.lo : zero, b.numPrims ? b.hi : zero, boxes[2 * ii], boxes[2 * ii + 1]);
+  }
+  w.insts.upload(insts.data(), insts.size(), s);
+  w.instBoxes.upload(boxes.data(), boxes.size(), s);
+  GeomDev top{};
+  top.kind = GEOM_INSTANCES;
+  top.numPrims = numInstances;
+  top.primOffset = 0;
+  top.vertices = (const float *)w.instBoxes.ptr;
+  w.hgeoms.push_back(top);
+  w.numPrims = (uint32_t)numInstances;
+  w.materials.upload(mats.data(), mats.size(), s);
+  build_world_bvh(w, s);
+  w.tlasDepth = w.depth;
+  w.depth = w.tlasDepth + maxDepth + 1;
+  w.totalPrims = total;
+  w.totalNodes = nodes + w.numNodes;
+}
+
+}  // namespace
+}  // namespace bnamd
+
 extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const bnamd_instance *instances,
                                  int numInstances, const bnamd_material *materials, int numMaterials,
                                  void *stream, bnamd_world **out) {
   return guarded([&] {
     BN_REQUIRE(out, "bnamd_world_build: out is NULL");
     BN_REQUIRE(numGroups >= 0 && numInstances >= 0, "bnamd_world_build: negative counts");
+    BN_REQUIRE(numInstances == 0 || (groups && instances), "bnamd_world_build: NULL group or instance array");
     auto *wo = new bnamd_world_s;
     try {
       World &w = wo->w;
-      uint64_t offset = 0;
-      for (int ii = 0; ii < numInstances; ++ii) {
-        const bnamd_instance &inst = instances[ii];
-        BN_REQUIRE(inst.group >= 0 && inst.group < numGroups, "bnamd_world_build: bad group index");
-        // Instances are flattened into one BVH8; this is exact only for
-        // identity transforms (object space == world space).
-        BN_REQUIRE(is_identity(inst.xfm),
-                   "bnamd_world_build: only identity instance transforms are supported");
-        const bnamd_group &grp = groups[inst.group];
-        for (int gi = 0; gi < grp.numGeoms; ++gi) {
-          const bnamd_geom &src = grp.geoms[gi];
-          BN_REQUIRE(src.kind >= 0 && src.kind <= 2, "bnamd_world_build: unknown geom kind");
-          BN_REQUIRE(src.numPrims >= 0, "bnamd_world_build: negative prim count");
-          BN_REQUIRE(src.numPrims == 0 || src.vertices, "bnamd_world_build: geom without vertices");
-          BN_REQUIRE(src.kind == BNAMD_GEOM_SPHERES || src.numPrims == 0 || src.indices,
-                     "bnamd_world_build: geom without indices");
-          BN_REQUIRE(src.kind != BNAMD_GEOM_CYLINDERS || src.numPrims == 0 || src.radii,
-                     "bnamd_world_build: cylinders need per-primitive radii");
-          BN_REQUIRE(src.materialID >= 0 && src.materialID < std::max(numMaterials, 1),
-                     "bnamd_world_build: materialID out of range");
-          if (src.numPrims == 0) continue;
-          GeomDev g{};
-          g.kind = src.kind;
-          g.numPrims = src.numPrims;
-          g.primOffset = (uint32_t)offset;
-          g.geomID = gi;
-          g.instIndex = ii;
-          g.instID = inst.instID;
-          g.materialID = src.materialID;
-          g.userID = src.userID;
-          g.vertices = src.vertices;
-          g.indices = src.indices;
-          g.radii = src.radii;
-          g.colors = src.colors;
-          g.defaultRadius = src.defaultRadius;
-          w.hgeoms.push_back(g);
-          offset += (uint64_t)src.numPrims;
-        }
-      }
-      BN_REQUIRE(offset < 0xfffffff0ull, "bnamd_world_build: too many primitives");
-      w.numPrims = (uint32_t)offset;
-      std::vector<MaterialDev> mats(std::max(numMaterials, 1));
-      for (int m = 0; m < (int)mats.size(); ++m) {
-        MaterialDev md{};
-        if (m < numMaterials) {
-          const bnamd_material &s = materials[m];
-          md.type = s.type;
-          md.color[0] = s.color[0]; md.color[1] = s.color[1]; md.color[2] = s.color[2];
-          md.opacity = s.opacity; md.metallic = s.metallic; md.roughness = s.roughness;
-          md.transmission = s.transmission; md.ior = s.ior; md.specular = s.specular;
-          if (s.type == BNAMD_MAT_PBR && !(s.ior != 1.f && s.transmission >= 1e-3f)) {
-            // NVisii stores its parameters as halfs (NVisii.h:1062-1083);
-            // roughness is clamped to [.001, .9] (AnariPBR.h:86)
-            md.metallic = round_half(s.metallic);
-            md.roughness = round_half(std::min(std::max(s.roughness, .001f), .9f));
-            md.ior = round_half(s.ior);
-          }
-        } else {
-          md.type = BNAMD_MAT_MATTE;
-          md.color[0] = md.color[1] = md.color[2] = 0.8f;
-          md.opacity = 1.f;
-        }
-        mats[m] = md;
-      }
       hipStream_t s = (hipStream_t)stream;
-      w.materials.upload(mats.data(), mats.size(), s);
-      build_world_bvh(w, s);
+      bool allIdentity = true;
+      for (int ii = 0; ii < numInstances; ++ii) {
+        BN_REQUIRE(instances[ii].group >= 0 && instances[ii].group < numGroups, "bnamd_world_build: bad group index");
+        allIdentity = allIdentity && is_identity(instances[ii].xfm);
+      }
+      const std::vector<MaterialDev> mats = make_materials(materials, numMaterials);
+      if (!allIdentity) {
+        build_two_level(w, groups, numGroups, instances, numInstances, mats, numMaterials, s);
+      } else {
+        // identity instances: one flattened BVH8 over all primitives
+        uint64_t offset = 0;
+        for (int ii = 0; ii < numInstances; ++ii) {
+          const bnamd_instance &inst = instances[ii];
+          const bnamd_group &grp = groups[inst.group];
+          for (int gi = 0; gi < grp.numGeoms; ++gi) {
+            const bnamd_geom &src = grp.geoms[gi];
+            check_geom(src, numMaterials);
+            if (src.numPrims == 0) continue;
+            w.hgeoms.push_back(make_geom(src, gi, ii, inst.instID, offset));
+            offset += (uint64_t)src.numPrims;
+          }
+        }
+        BN_REQUIRE(offset < 0xfffffff0ull, "bnamd_world_build: too many primitives");
+        w.numPrims = (uint32_t)offset;
+        w.materials.upload(mats.data(), mats.size(), s);
+        build_world_bvh(w, s);
+        w.totalPrims = w.numPrims;
+        w.totalNodes = w.numNodes;
+      }
       BN_HIP(hipStreamSynchronize(s));
     } catch (...) {
       delete wo;
@@ -160,8 +303,8 @@ extern "C" int bnamd_world_destroy(bnamd_world *world) {
 extern "C" int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info) {
   return guarded([&] {
     BN_REQUIRE(world && info, "bnamd_world_info_get: NULL argument");
-    info->numPrims = world->w.numPrims;
-    info->numNodes = world->w.numNodes;
+    info->numPrims = (long long)world->w.totalPrims;
+    info->numNodes = (long long)world->w.totalNodes;
     info->depth = world->w.depth;
     info->buildMs = world->w.buildMs;
     for (int k = 0; k < 3; ++k) {
@@ -182,7 +325,10 @@ extern "C" int bnamd_trace(const bnamd_world *world, const bnamd_ray *rays, bnam
     const size_t lds = trace_lds_bytes(world->w.depth, block);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_trace: BVH too deep for the LDS stack");
     hipStream_t s = (hipStream_t)stream;
-    k_trace<<<div_up(numRays, block), block, lds, s>>>(world->w.dd(), rays, hits, numRays);
+    if (world->w.twoLevel)
+      k_trace<true><<<div_up(numRays, block), block, lds, s>>>(world->w.dd(), rays, hits, numRays);
+    else
+      k_trace<false><<<div_up(numRays, block), block, lds, s>>>(world->w.dd(), rays, hits, numRays);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
   });

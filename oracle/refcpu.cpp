@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -1997,6 +1998,117 @@ long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_
   });
   if (samples) *samples = ns.load();
   return nrays.load();
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Instanced worlds (rtcore/embree/Group.cpp:181-217 InstanceGroup): each
+// instance = a scene + affine transform; rays go to object space through
+// rcp(xfm) (computed in double, rounded once), hits come back through xfm.
+// ============================================================================
+struct ref_world_s {
+  std::vector<const ref_scene *> groups;
+  std::vector<int> instGroup, instID;
+  std::vector<std::array<float, 12>> o2w, w2o;
+};
+
+namespace {
+
+V3 xpoint(const float *m, V3 p) {
+  return mk(m[0] * p.x + m[3] * p.y + m[6] * p.z + m[9], m[1] * p.x + m[4] * p.y + m[7] * p.z + m[10],
+            m[2] * p.x + m[5] * p.y + m[8] * p.z + m[11]);
+}
+V3 xvector(const float *m, V3 v) {
+  return mk(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+            m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+
+// inverse of the affine map (columns vx, vy, vz, p): cofactor matrix over
+// the determinant in double, p' = -(L^-1 p)
+void invert12(const float *m, float *out) {
+  double L[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) L[r][c] = m[3 * c + r];
+  double C[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      C[r][c] = L[r1][c1] * L[r2][c2] - L[r1][c2] * L[r2][c1];
+    }
+  const double det = L[0][0] * C[0][0] + L[0][1] * C[0][1] + L[0][2] * C[0][2];
+  double inv[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) inv[r][c] = C[c][r] / det;
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) out[3 * c + r] = (float)inv[r][c];
+  for (int r = 0; r < 3; ++r) out[9 + r] = (float)-(inv[r][0] * m[9] + inv[r][1] * m[10] + inv[r][2] * m[11]);
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_world *ref_world_create(const ref_scene *const *groups, int numGroups, const int *instGroup, const int *instIDs,
+                            const float *xfms, int numInstances) {
+  auto *w = new ref_world_s;
+  w->groups.assign(groups, groups + numGroups);
+  for (int i = 0; i < numInstances; ++i) {
+    w->instGroup.push_back(instGroup[i]);
+    w->instID.push_back(instIDs[i]);
+    std::array<float, 12> a, b;
+    for (int k = 0; k < 12; ++k) a[k] = xfms[12 * (size_t)i + k];
+    invert12(a.data(), b.data());
+    w->o2w.push_back(a);
+    w->w2o.push_back(b);
+  }
+  return w;
+}
+
+void ref_world_destroy(ref_world *w) { delete w; }
+
+void ref_world_trace(const ref_world *w, const float *rays, ref_hit *hits, int n, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * (size_t)i;
+    const V3 org = ld3(r), dir = fix_dir(ld3(r + 4));
+    Best best;
+    best.t = r[7];
+    best.u = best.v = 0.f;
+    best.key = 0xffffffffu;
+    best.geom = best.prim = -1;
+    int bestInst = -1;
+    V3 bo = org, bd = dir;
+    for (size_t ii = 0; ii < w->instGroup.size(); ++ii) {
+      const ref_scene *sc = w->groups[w->instGroup[ii]];
+      const V3 o = xpoint(w->w2o[ii].data(), org);
+      const V3 d = fix_dir(xvector(w->w2o[ii].data(), dir));
+      const Best b = closest(sc, o, d, r[3], r[7], false, nullptr);
+      // closest over instances; ties to the smaller (instance, key)
+      if (b.geom >= 0 && (bestInst < 0 || b.t < best.t)) {
+        best = b;
+        bestInst = (int)ii;
+        bo = o;
+        bd = d;
+      }
+    }
+    ref_hit &h = hits[i];
+    h.t = best.t;
+    h.u = best.u;
+    h.v = best.v;
+    if (bestInst < 0) {
+      h.primID = h.geomID = h.instID = h.instIndex = -1;
+      h.N[0] = h.N[1] = h.N[2] = 0.f;
+    } else {
+      V3 P, N;
+      hit_point_normal(w->groups[w->instGroup[bestInst]], best, bo, bd, r[3], r[7], P, N);
+      N = xvector(w->o2w[bestInst].data(), N);
+      h.primID = best.prim;
+      h.geomID = best.geom;
+      h.instID = w->instID[bestInst];
+      h.instIndex = bestInst;
+      h.N[0] = N.x; h.N[1] = N.y; h.N[2] = N.z;
+    }
+  });
 }
 
 }  // extern "C"

@@ -146,6 +146,15 @@ long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_
                            int iso, float isoValue, const float *isoColor, float *accum, int y0, int y1,
                            int threads, long long *samples);
 
+/* --- instanced worlds: scenes with affine transforms (columns vx, vy, vz,
+ * p; object to world), closest hit over all instances, ties to the
+ * smallest (instance, geom, prim) --- */
+typedef struct ref_world_s ref_world;
+ref_world *ref_world_create(const ref_scene *const *groups, int numGroups, const int *instGroup,
+                            const int *instIDs, const float *xfms, int numInstances);
+void ref_world_destroy(ref_world *w);
+void ref_world_trace(const ref_world *w, const float *rays, ref_hit *hits, int n, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,9 @@ def load():
         "ref_volume_sample_rays": (None, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_dda_cells": (C.c_int, [P, P, C.c_float, P, P, P, C.c_int]),
         "ref_render_volume": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_world_create": (P, [P, C.c_int, P, P, P, C.c_int]),
+        "ref_world_destroy": (None, [P]),
+        "ref_world_trace": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_umesh_create": (P, [P, P, C.c_int, P, C.c_int]),
         "ref_umesh_destroy": (None, [P]),
         "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -160,6 +163,39 @@ class Scene:
     def __del__(self):
         try:
             load().ref_scene_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class World:
+    """Instanced oracle world: scenes placed by affine transforms (12 floats,
+    columns vx, vy, vz, p)."""
+
+    def __init__(self, scenes, instances):
+        """scenes: list of Scene; instances: list of (scene index, instID,
+        xfm[12])."""
+        self._scenes = scenes
+        g = (C.c_void_p * max(len(scenes), 1))(*[sc.handle for sc in scenes])
+        ig = np.array([i[0] for i in instances], np.int32)
+        ids = np.array([i[1] for i in instances], np.int32)
+        x = np.ascontiguousarray(np.array([i[2] for i in instances], np.float32).reshape(-1, 12))
+        self.handle = load().ref_world_create(g, len(scenes), ig.ctypes.data, ids.ctypes.data, x.ctypes.data,
+                                              len(instances))
+
+    def trace(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = (ref_hit * max(n, 1))()
+        load().ref_world_trace(self.handle, rays.ctypes.data, hits, n, threads)
+        raw = np.frombuffer(hits, dtype=np.float32, count=10 * n).reshape(n, 10)
+        ints = raw.view(np.int32)
+        return {"t": raw[:, 0].copy(), "u": raw[:, 1].copy(), "v": raw[:, 2].copy(), "primID": ints[:, 3].copy(),
+                "geomID": ints[:, 4].copy(), "instID": ints[:, 5].copy(), "N": raw[:, 6:9].copy(),
+                "instIndex": ints[:, 9].copy()}
+
+    def __del__(self):
+        try:
+            load().ref_world_destroy(self.handle)
         except Exception:
             pass
 

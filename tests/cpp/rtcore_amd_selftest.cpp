@@ -138,15 +138,31 @@ int main() {
     if (std::isinf(w.t)) CHECK(h.primID == -1);
     else CHECK(near(h.t, w.t, 1e-5f));
   }
-  // a non-identity transform is rejected with a message, not silently wrong
+  // instance transforms: the quad group moved by +1 in z and scaled by 2 in
+  // x/y (two-level world); the head-on ray now meets it at t = 6
   affine3f moved;
+  moved.vx = {2.f, 0.f, 0.f};
+  moved.vy = {0.f, 2.f, 0.f};
   moved.p = {0.f, 0.f, 1.f};
   ig->setTransforms({moved, affine3f()});
+  ig->buildAccel();
+  ig->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, (int)rays.size());
+  dev->copy(hits.data(), dHits, hits.size() * sizeof(bnamd_hit));
+  std::printf("moved: t=%g inst=%d prim=%d N=(%g %g %g)\n", hits[0].t, hits[0].instID, hits[0].primID,
+              hits[0].N[0], hits[0].N[1], hits[0].N[2]);
+  CHECK(hits[0].instID == 7 && near(hits[0].t, 6.f, 1e-6f));
+  CHECK(std::fabs(hits[0].N[2]) > 0.99f);
+  // the sphere behind it is unaffected
+  CHECK(hits[2].instID == 8 && near(hits[2].t, d2s - 1.f, 1e-5f));
+  // a singular transform is rejected with a message
+  affine3f flat;
+  flat.vz = {0.f, 0.f, 0.f};
+  ig->setTransforms({flat, affine3f()});
   threw = false;
   try {
     ig->buildAccel();
   } catch (const std::runtime_error &e) {
-    threw = std::strstr(e.what(), "transform") != nullptr;
+    threw = std::strstr(e.what(), "singular") != nullptr;
   }
   CHECK(threw);
 
