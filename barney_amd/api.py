@@ -401,3 +401,57 @@ class UMesh:
             self.destroy()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ frame-buffer readback
+FB_FLOAT4, FB_UFIXED8_RGBA, FB_UFIXED8_RGBA_SRGB = 0, 1, 2
+TILE_SIZE = 32
+
+
+def fb_tile_descs(width, height, rank=0, size=1):
+    """Device int32 [numActive, 2]: lower corners of the 32x32 tiles that
+    device `rank` of `size` owns (global tiles rank, rank+size, ...)."""
+    lib = init()
+    n = C.c_int()
+    check(lib.bnamd_fb_tile_descs(width, height, rank, size, None, C.byref(n), None))
+    d = DeviceArray.empty((n.value, 2), np.int32)
+    check(lib.bnamd_fb_tile_descs(width, height, rank, size, d.ptr, C.byref(n), None))
+    return d
+
+
+def _color_out(width, height, fmt, out):
+    if out is not None:
+        return out
+    if fmt == FB_FLOAT4:
+        return DeviceArray.empty((height, width, 4), np.float32)
+    return DeviceArray.empty((height, width), np.uint32)
+
+
+def fb_linearize(tiles, descs, width, height, accum_scale, fmt, normal=None, out=None):
+    """tiles: DeviceArray of accumulation tiles (float accum[1024][4] then
+    float normal[1024][3] per tile), descs from fb_tile_descs. Returns the
+    linear color DeviceArray ([H, W, 4] float32 or [H, W] uint32); normal,
+    if given, is a DeviceArray [H, W, 3] that receives the normals."""
+    o = _color_out(width, height, fmt, out)
+    check(init().bnamd_fb_linearize(tiles.ptr, descs.ptr, descs.shape[0], width, height, float(accum_scale), fmt,
+                                    o.ptr, None if normal is None else normal.ptr, None))
+    return o
+
+
+def fb_linearize_aux(aux, descs, width, height, out=None):
+    o = out if out is not None else DeviceArray.empty((height, width), np.uint32)
+    check(init().bnamd_fb_linearize_aux(aux.ptr, descs.ptr, descs.shape[0], width, height, o.ptr, None))
+    return o
+
+
+def fb_resolve(accum, width, height, accum_scale, fmt, out=None):
+    """Linear float4 accumulation (as the renderers write it) -> color."""
+    o = _color_out(width, height, fmt, out)
+    check(init().bnamd_fb_resolve(accum.ptr, width, height, float(accum_scale), fmt, o.ptr, None))
+    return o
+
+
+def fb_to_fixed8(rgba, width, height, srgb, out=None):
+    o = out if out is not None else DeviceArray.empty((height, width), np.uint32)
+    check(init().bnamd_fb_to_fixed8(rgba.ptr, width, height, int(bool(srgb)), o.ptr, None))
+    return o

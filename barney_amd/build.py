@@ -30,6 +30,7 @@ SOURCES = [
     "ea.hip",
     "render_batch.hip",
     "umesh.hip",
+    "fb.hip",
 ]
 
 # -packed-fp32-ops: packed FP32 forms are kept out of the generated code.

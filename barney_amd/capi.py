@@ -140,6 +140,14 @@ SIGNATURES = {
                                      C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
     "bnamd_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
                                        C.c_char_p]),
+    "bnamd_fb_tile_descs": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int),
+                                      C.c_void_p]),
+    "bnamd_fb_linearize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_fb_linearize_aux": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_void_p]),
+    "bnamd_fb_resolve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p, C.c_void_p]),
+    "bnamd_fb_to_fixed8": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

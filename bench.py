@@ -12,7 +12,8 @@ summary), the CPU oracle timed on a bounded sample of the same frame, and
 the other workloads as extra figures: surface megakernel Mrays/s (1,024
 tessellated spheres), the multi-view megakernel vs wavefront renderer
 (heightfield + spheres + cylinders, 64 views), the tetrahedral volume with
-its iso-surface, and the emission-absorption forward/adjoint pass.
+its iso-surface, frame readback (tiled accumulation to linear float4 /
+sRGB RGBA8), and the emission-absorption forward/adjoint pass.
 """
 import argparse
 import json
@@ -187,6 +188,57 @@ def umesh_bench(api, steps=2):
             "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"])}
 
 
+def fb_bench(api, reps=20):
+    """Frame readback of a 3840x2160 frame (8160 tiles of 32x32): tiled
+    accumulation -> linear float4 + normals, -> sRGB RGBA8, and the resolve of
+    a linear accumulation buffer to float4. Timed with HIP events on one
+    stream around `reps` launches; algorithmic bytes per pixel: 56 (float4 +
+    normal in and out), 20 (float4 in, RGBA8 out), 32 (float4 in and out)."""
+    import ctypes as C
+    import torch
+    lib = api.capi.load()
+    W, H = 3840, 2160
+    descs = api.fb_tile_descs(W, H)
+    n = descs.shape[0]
+    rng = np.random.default_rng(7)
+    tiles = api.DeviceArray(rng.random((n, 1024 * 7), dtype=np.float32) * 4.0)
+    accum = api.DeviceArray(rng.random((H, W, 4), dtype=np.float32) * 4.0)
+    color = api.DeviceArray.empty((H, W, 4), np.float32)
+    normal = api.DeviceArray.empty((H, W, 3), np.float32)
+    rgba8 = api.DeviceArray.empty((H, W), np.uint32)
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        s.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    px = W * H
+    legs = {
+        "linearize_float4_normal": (lambda: lib.bnamd_fb_linearize(tiles.ptr, descs.ptr, n, W, H, 0.25, 0, color.ptr,
+                                                                    normal.ptr, sp), 56),
+        "linearize_srgb8": (lambda: lib.bnamd_fb_linearize(tiles.ptr, descs.ptr, n, W, H, 0.25, 2, rgba8.ptr, None,
+                                                           sp), 20),
+        "resolve_float4": (lambda: lib.bnamd_fb_resolve(accum.ptr, W, H, 0.25, 0, color.ptr, sp), 32),
+        "resolve_srgb8": (lambda: lib.bnamd_fb_resolve(accum.ptr, W, H, 0.25, 2, rgba8.ptr, sp), 20),
+    }
+    res = {"width": W, "height": H, "tiles": n}
+    for name, (fn, bpp) in legs.items():
+        ms = timed(fn)
+        gbps = bpp * px / (ms * 1e-3) / 1e9
+        res[name] = {"us": round(ms * 1e3, 2), "GBps_alg": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 3)}
+        log(f"readback {name}: {ms * 1e3:.1f} us, {gbps:.0f} GB/s")
+    api.capi.check(lib.bnamd_synchronize())
+    return res
+
+
 def profile_traffic(kernel):
     """HBM bytes per launch of 'kernel' from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), each counter
@@ -244,6 +296,7 @@ def main():
     ap.add_argument("--no-ea", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-umesh", action="store_true")
+    ap.add_argument("--no-fb", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
@@ -252,7 +305,7 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by image tiles / views, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    if not args.no_ea:
+    if not (args.no_ea and args.no_fb):
         # torch must bring up the HIP runtime before libbnamd does when both
         # share the process
         import torch  # noqa: F401
@@ -303,6 +356,8 @@ def main():
         out["extra"]["batch_workload3"] = batch_bench(api)
     if not args.no_umesh:
         out["extra"]["umesh_workload5"] = umesh_bench(api)
+    if not args.no_fb:
+        out["extra"]["readback_3840x2160"] = fb_bench(api)
     if not args.no_ea:
         out["extra"]["ea_workload4"] = ea_bench()
     print(json.dumps(out), flush=True)

@@ -377,6 +377,67 @@ int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views,
 int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views *views,
                          const bnamd_render_params *params, const char *path);
 
+/* ------------------------------------------------------------------------
+ * Frame-buffer tiles and readback.
+ * The accumulation tiles, tile descriptors and aux tiles have the layout of
+ * barney's TiledFB (fb/TiledFB.h:16-36): 32x32-pixel tiles, pixel i of a
+ * tile at (lower.x + i % 32, lower.y + i / 32).  Device g of G owns global
+ * tiles g, g+G, g+2G, ... in row-major tile order.
+ * Color formats: BNAMD_FB_FLOAT4 writes float4 per pixel; the UFIXED8
+ * formats write one uint32 per pixel, R in the low byte, each channel
+ * min(255, max(0, int(c * 256))), the _SRGB one after the sRGB transfer
+ * function on R, G and B (alpha stays linear) (common/math.h:52-66,
+ * 94-116).  Pixels outside width x height are not written.
+ * --------------------------------------------------------------------- */
+#define BNAMD_TILE_SIZE 32
+#define BNAMD_PIXELS_PER_TILE (BNAMD_TILE_SIZE * BNAMD_TILE_SIZE)
+#define BNAMD_FB_FLOAT4 0
+#define BNAMD_FB_UFIXED8_RGBA 1
+#define BNAMD_FB_UFIXED8_RGBA_SRGB 2
+
+typedef struct {
+  int lower[2]; /* pixel coordinates of the tile's lower-left corner */
+} bnamd_tile_desc;
+
+typedef struct {
+  float accum[BNAMD_PIXELS_PER_TILE][4];  /* RGBA sums over accumID frames */
+  float normal[BNAMD_PIXELS_PER_TILE][3];
+} bnamd_accum_tile;
+
+/* Number of tiles device globalRank of globalSize owns for a width x height
+ * frame, and (if descs != NULL, a device array of that many) their
+ * descriptors.  Replaces TiledFB::resize's tile count and
+ * setTileCoordsKernel (fb/TiledFB.cu:226-264). */
+int bnamd_fb_tile_descs(int width, int height, int globalRank, int globalSize,
+                        bnamd_tile_desc *descs, int *numActiveTiles, void *stream);
+
+/* Writes numTiles accumulation tiles into a linear width x height frame:
+ * color = accum * accumScale (barney passes 1/accumID) in the given format,
+ * and the normal unchanged into outNormal (float3 per pixel) when it is not
+ * NULL.  Replaces TiledFB::linearizeColorAndNormal and its kernel
+ * (fb/TiledFB.cu:13-83). */
+int bnamd_fb_linearize(const bnamd_accum_tile *tiles, const bnamd_tile_desc *descs, int numTiles,
+                       int width, int height, float accumScale, int format, void *outColor,
+                       float *outNormal, void *stream);
+
+/* Writes numTiles aux tiles (uint32 or float per pixel: depth, primID,
+ * instID, objID) into a linear width x height array.  Replaces
+ * TiledFB::linearizeAuxTiles (fb/TiledFB.cu:88-130). */
+int bnamd_fb_linearize_aux(const uint32_t *auxTiles, const bnamd_tile_desc *descs, int numTiles,
+                           int width, int height, uint32_t *out, void *stream);
+
+/* The same color conversion for the linear float4 accumulation buffer that
+ * bnamd_render_surface, bnamd_volume_render, bnamd_umesh_render and
+ * bnamd_render_batch write (one device owns every pixel). */
+int bnamd_fb_resolve(const float *accum, int width, int height, float accumScale, int format,
+                     void *out, void *stream);
+
+/* float4 -> RGBA8 after clamping every channel to [0,1], with the sRGB
+ * transfer on R, G, B when srgb != 0.  Replaces the LinearToFixed8 compute
+ * kernel of the denoised read path (fb/FrameBuffer.cu:105-128, 268-285). */
+int bnamd_fb_to_fixed8(const float *in, int width, int height, int srgb, uint32_t *out,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2112,3 +2112,124 @@ void ref_world_trace(const ref_world *w, const float *rays, ref_hit *hits, int n
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Frame-buffer tiles and readback (barney/fb/TiledFB.cu, FrameBuffer.cu,
+// barney/common/math.h). Tiles are 32x32 pixels; pixel i of a tile sits at
+// (lower.x + i % 32, lower.y + i / 32).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kFbTile = 32;
+constexpr int kFbTilePixels = kFbTile * kFbTile;
+
+// make_8bit (barney/common/math.h:94-97): min(255, max(0, int(f * 256)));
+// int() as the GPU converts (saturating, NaN -> 0).
+uint32_t fb_8bit(float f) {
+  const float s = f * 256.f;
+  if (s >= 256.f) return 255u;
+  if (s > 0.f) return (uint32_t)(int)s;
+  return 0u;
+}
+
+// linear_to_srgb (barney/common/math.h:52-58); the multiply-add fused as
+// nvcc contracts it by default.
+float fb_srgb(float x) {
+  if (x <= 0.0031308f) return 12.92f * x;
+  return std::fma(1.055f, powf(x, 1.f / 2.4f), -0.055f);
+}
+
+// saturate (barney/common/math.h:39-46), fminf/fmaxf semantics.
+float fb_saturate(float f) { return std::fmax(0.f, std::fmin(f, 1.f)); }
+
+// make_rgba(vec4f) (barney/common/math.h:109-116), sRGB on R, G, B only
+// (math.h:63-66).
+uint32_t fb_rgba(const float *c, bool srgb) {
+  float r = c[0], g = c[1], b = c[2];
+  if (srgb) {
+    r = fb_srgb(r);
+    g = fb_srgb(g);
+    b = fb_srgb(b);
+  }
+  return fb_8bit(r) | (fb_8bit(g) << 8) | (fb_8bit(b) << 16) | (fb_8bit(c[3]) << 24);
+}
+
+void fb_store(void *out, size_t idx, const float *c, int format) {
+  if (format == 0)
+    std::memcpy((float *)out + 4 * idx, c, 16);
+  else
+    ((uint32_t *)out)[idx] = fb_rgba(c, format == 2);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t ref_fb_make_8bit(float f) { return fb_8bit(f); }
+float ref_fb_linear_to_srgb(float x) { return fb_srgb(x); }
+
+// TiledFB::resize (barney/fb/TiledFB.cu:243-252) and setTileCoordsKernel
+// (TiledFB.cu:226-238).
+int ref_fb_tile_descs(int width, int height, int rank, int size, int *lower) {
+  const int tx = (width + kFbTile - 1) / kFbTile, ty = (height + kFbTile - 1) / kFbTile;
+  const int total = tx * ty;
+  const int numActive = (std::max(0, total - rank) + size - 1) / size;
+  if (lower)
+    for (int t = 0; t < numActive; ++t) {
+      const int id = t * size + rank;
+      lower[2 * t + 0] = (id % tx) * kFbTile;
+      lower[2 * t + 1] = (id / tx) * kFbTile;
+    }
+  return numActive;
+}
+
+// linearizeColorAndNormalKernel (barney/fb/TiledFB.cu:13-48); a tile is
+// float accum[1024][4] followed by float normal[1024][3].
+void ref_fb_linearize(const float *tiles, const int *lower, int numTiles, int width, int height, float scale,
+                      int format, void *out, float *outNormal) {
+  for (int t = 0; t < numTiles; ++t) {
+    const float *acc = tiles + (size_t)t * kFbTilePixels * 7;
+    const float *nrm = acc + kFbTilePixels * 4;
+    for (int sub = 0; sub < kFbTilePixels; ++sub) {
+      const int ix = lower[2 * t] + sub % kFbTile, iy = lower[2 * t + 1] + sub / kFbTile;
+      if (ix >= width || iy >= height) continue;
+      const size_t idx = (size_t)ix + (size_t)width * iy;
+      const float c[4] = {acc[4 * sub] * scale, acc[4 * sub + 1] * scale, acc[4 * sub + 2] * scale,
+                          acc[4 * sub + 3] * scale};
+      fb_store(out, idx, c, format);
+      if (outNormal) std::memcpy(outNormal + 3 * idx, nrm + 3 * sub, 12);
+    }
+  }
+}
+
+// linearizeAuxTilesKernel (barney/fb/TiledFB.cu:88-105).
+void ref_fb_linearize_aux(const uint32_t *aux, const int *lower, int numTiles, int width, int height,
+                          uint32_t *out) {
+  for (int t = 0; t < numTiles; ++t)
+    for (int sub = 0; sub < kFbTilePixels; ++sub) {
+      const int ix = lower[2 * t] + sub % kFbTile, iy = lower[2 * t + 1] + sub / kFbTile;
+      if (ix >= width || iy >= height) continue;
+      out[(size_t)ix + (size_t)width * iy] = aux[(size_t)t * kFbTilePixels + sub];
+    }
+}
+
+// Linear accumulation -> output format, scaled (the color step of
+// linearizeColorAndNormalKernel on an untiled frame).
+void ref_fb_resolve(const float *accum, size_t n, float scale, int format, void *out) {
+  for (size_t i = 0; i < n; ++i) {
+    const float c[4] = {accum[4 * i] * scale, accum[4 * i + 1] * scale, accum[4 * i + 2] * scale,
+                        accum[4 * i + 3] * scale};
+    fb_store(out, i, c, format);
+  }
+}
+
+// LinearToFixed8::run (barney/fb/FrameBuffer.cu:116-128).
+void ref_fb_to_fixed8(const float *in, size_t n, int srgb, uint32_t *out) {
+  for (size_t i = 0; i < n; ++i) {
+    const float c[4] = {fb_saturate(in[4 * i]), fb_saturate(in[4 * i + 1]), fb_saturate(in[4 * i + 2]),
+                        fb_saturate(in[4 * i + 3])};
+    out[i] = fb_rgba(c, srgb != 0);
+  }
+}
+
+}  // extern "C"

@@ -155,6 +155,20 @@ ref_world *ref_world_create(const ref_scene *const *groups, int numGroups, const
 void ref_world_destroy(ref_world *w);
 void ref_world_trace(const ref_world *w, const float *rays, ref_hit *hits, int n, int threads);
 
+/* --- frame-buffer tiles and readback (barney/fb/TiledFB.cu:13-130,
+ * 226-264, FrameBuffer.cu:105-128, common/math.h:39-116). A tile is
+ * float accum[1024][4] then float normal[1024][3]; lower = int2 per tile.
+ * format 0 float4, 1 RGBA8, 2 RGBA8 with sRGB on R, G, B. --- */
+uint32_t ref_fb_make_8bit(float f);
+float ref_fb_linear_to_srgb(float x);
+int ref_fb_tile_descs(int width, int height, int rank, int size, int *lower);
+void ref_fb_linearize(const float *tiles, const int *lower, int numTiles, int width, int height, float scale,
+                      int format, void *out, float *outNormal);
+void ref_fb_linearize_aux(const uint32_t *aux, const int *lower, int numTiles, int width, int height,
+                          uint32_t *out);
+void ref_fb_resolve(const float *accum, size_t n, float scale, int format, void *out);
+void ref_fb_to_fixed8(const float *in, size_t n, int srgb, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,13 @@ def load():
         "ref_umesh_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
         "ref_umesh_render": (C.c_longlong, [P, P, P, C.c_int, C.c_int, C.c_float, P, P, C.c_int, C.c_int, C.c_int,
                                             P]),
+        "ref_fb_make_8bit": (C.c_uint32, [C.c_float]),
+        "ref_fb_linear_to_srgb": (C.c_float, [C.c_float]),
+        "ref_fb_tile_descs": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P]),
+        "ref_fb_linearize": (None, [P, P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, P, P]),
+        "ref_fb_linearize_aux": (None, [P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_fb_resolve": (None, [P, C.c_size_t, C.c_float, C.c_int, P]),
+        "ref_fb_to_fixed8": (None, [P, C.c_size_t, C.c_int, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -385,3 +392,69 @@ def dda_cells(org, dir, tmax, grid, max_cells=4096):
     n = load().ref_dda_cells(org.ctypes.data, dir.ctypes.data, float(tmax), g.ctypes.data, cells.ctypes.data,
                              ts.ctypes.data, max_cells)
     return cells[:min(n, max_cells)], ts[:min(n, max_cells)], n
+
+
+# ---------------------------------------------------------------- frame buffer
+FB_FLOAT4, FB_RGBA8, FB_SRGB8 = 0, 1, 2
+TILE = 32
+
+
+def fb_make_8bit(f):
+    return int(load().ref_fb_make_8bit(float(f)))
+
+
+def fb_linear_to_srgb(x):
+    return float(load().ref_fb_linear_to_srgb(float(x)))
+
+
+def fb_tile_descs(width, height, rank=0, size=1):
+    """int32 [numActive, 2] lower corners of the tiles device `rank` of
+    `size` owns."""
+    L = load()
+    n = L.ref_fb_tile_descs(width, height, rank, size, None)
+    out = np.zeros((max(n, 1), 2), np.int32)
+    L.ref_fb_tile_descs(width, height, rank, size, out.ctypes.data)
+    return out[:n]
+
+
+def _out(width, height, fmt, fill):
+    if fmt == FB_FLOAT4:
+        return np.full((height, width, 4), fill, np.float32)
+    return np.full((height, width), fill, np.uint32)
+
+
+def fb_linearize(tiles, lower, width, height, scale, fmt, with_normal=False, fill=0):
+    """tiles: float32 [numTiles, 1024*7] (accum float4 x 1024, normal float3
+    x 1024); returns color [H, W, 4] float32 or [H, W] uint32 (and normal
+    [H, W, 3]); pixels no tile covers keep `fill`."""
+    tiles = np.ascontiguousarray(tiles, np.float32).reshape(-1, TILE * TILE * 7)
+    lower = np.ascontiguousarray(lower, np.int32).reshape(-1, 2)
+    out = _out(width, height, fmt, fill)
+    nrm = np.full((height, width, 3), fill, np.float32) if with_normal else None
+    load().ref_fb_linearize(tiles.ctypes.data, lower.ctypes.data, len(lower), width, height, float(scale), fmt,
+                            out.ctypes.data, _ptr(nrm))
+    return (out, nrm) if with_normal else out
+
+
+def fb_linearize_aux(aux, lower, width, height, fill=0):
+    aux = np.ascontiguousarray(aux).view(np.uint32).reshape(-1, TILE * TILE)
+    lower = np.ascontiguousarray(lower, np.int32).reshape(-1, 2)
+    out = np.full((height, width), fill, np.uint32)
+    load().ref_fb_linearize_aux(aux.ctypes.data, lower.ctypes.data, len(lower), width, height, out.ctypes.data)
+    return out
+
+
+def fb_resolve(accum, scale, fmt):
+    """accum float32 [..., 4] -> [..., 4] float32 or [...] uint32."""
+    a = np.ascontiguousarray(accum, np.float32)
+    shape = a.shape[:-1]
+    out = np.zeros(shape + (4,), np.float32) if fmt == FB_FLOAT4 else np.zeros(shape, np.uint32)
+    load().ref_fb_resolve(a.ctypes.data, a.size // 4, float(scale), fmt, out.ctypes.data)
+    return out
+
+
+def fb_to_fixed8(rgba, srgb):
+    a = np.ascontiguousarray(rgba, np.float32)
+    out = np.zeros(a.shape[:-1], np.uint32)
+    load().ref_fb_to_fixed8(a.ctypes.data, a.size // 4, int(bool(srgb)), out.ctypes.data)
+    return out
