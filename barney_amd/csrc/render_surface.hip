@@ -150,6 +150,9 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     DevBuf<unsigned long long> counter(1);
     BN_HIP(hipMemsetAsync(counter.ptr, 0, sizeof(unsigned long long), s));
     const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
+    // xcd_remap hands XCD x the tiles [x*per, (x+1)*per): the grid must hold
+    // 8*per workgroups (a multiple of 8) to cover every tile
+    const int grid = ((tiles + 7) / 8) * 8;
     const int block = 256;
     const size_t lds = trace_lds_bytes(w.depth, block);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_surface: BVH too deep for the LDS stack");
@@ -158,10 +161,10 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     BN_HIP(hipEventCreate(&e1));
     BN_HIP(hipEventRecord(e0, s));
     if (w.twoLevel)
-      k_render_surface<true><<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+      k_render_surface<true><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
                                                        (int4 *)ids, counter.ptr);
     else
-      k_render_surface<false><<<tiles, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+      k_render_surface<false><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
                                                         (int4 *)ids, counter.ptr);
     BN_HIP(hipGetLastError());
     BN_HIP(hipEventRecord(e1, s));

@@ -160,7 +160,10 @@ void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD
   constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
-  k_render_volume<L><<<tiles, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
+  // xcd_remap_v hands XCD x the tiles [x*per, (x+1)*per): the grid must
+  // hold 8*per workgroups (a multiple of 8) to cover every tile
+  const int grid = ((tiles + 7) / 8) * 8;
+  k_render_volume<L><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
 }
 
 void render_volume_dispatch(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,

@@ -82,3 +82,20 @@ def test_volume_render_matches_oracle(gpu):
     np.testing.assert_allclose(gacc, racc, rtol=1e-5, atol=1e-6)
     assert gout["stats"]["samples"] == nsamp
     assert gout["stats"]["raysTraced"] == nrays
+
+
+@pytest.mark.parametrize("W,H,spp", [(50, 30, 4), (13, 7, 1), (9, 70, 16)])
+def test_volume_render_every_pixel_any_frame_size(gpu, W, H, spp):
+    """Tile counts that are not a multiple of the 8 XCDs: every pixel is
+    written (the output starts as NaN) and equals the oracle."""
+    f = _field(24)
+    gv, rv = _pair(gpu, f, base=0.3)
+    c = np.array([1.0, -2.0, 0.5]) + np.array([0.5, 0.5, 0.75]) * 11.5
+    pos = c + np.array([-15.0, 8.0, -18.0])
+    acc = gpu.DeviceArray(np.full((H, W, 4), np.nan, np.float32))
+    gout = gv.render(gpu.camera(pos, c - pos, fovy=40), gpu.render_params(W, H, spp=spp), accum=acc)
+    racc, nrays, nsamp = rv.render(refcpu.camera(pos, c - pos, fovy=40), refcpu.render_params(W, H, spp=spp))
+    gacc = gout["accum"].numpy()
+    assert np.isfinite(gacc).all()
+    np.testing.assert_allclose(gacc, racc, rtol=1e-5, atol=1e-6)
+    assert gout["stats"]["samples"] == nsamp
