@@ -123,6 +123,7 @@ struct RNGSeed {
 };
 
 struct Random2 : RNGSeed {
+  BN_HD Random2() = default;
   BN_HD Random2(uint32_t a, uint32_t b) { seed(a, b); }
   // Random2(RNGSeed&, b): copies the parent's state, advances the parent,
   // then mixes in b (290374 when b == 0).

@@ -496,17 +496,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
       ray.isShadow = false;
       ray.missColor = vec3f(0.f);
       ray.missAlpha = 0.f;
-      if (depth == 0) {
-        // generateRays' background for primary rays (generateRays.cu:199-210)
-        const float t = (iy + .5f) / float(rpp.height);
-        if (rpp.bg[3] >= 0.f) {
-          ray.missColor = {rpp.bg[0], rpp.bg[1], rpp.bg[2]};
-          ray.missAlpha = rpp.bg[3];
-        } else {
-          ray.missColor = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
-          ray.missAlpha = (1.0f - t) * 1.f + t * 1.f;
-        }
-      }
+      if (depth == 0) miss_color(rpp, iy, ray.missColor, ray.missAlpha);  // primary rays' background
       const vec3f dir = fix_dir(ray.dir);
       const ShadeHit sh = shade_hit<TL>(e.w, ray.org, dir, ray.tMax, h);
       const float clampMax = 10.f * (1 + rpp.accumID);

@@ -94,6 +94,20 @@ struct Fragment {
   vec3f rgb;
 };
 
+// What a primary ray of row iy shows when it escapes (generateRays.cu:199-210):
+// the background color, or (bg alpha < 0) the default vertical gradient.
+BN_HD void miss_color(const RenderDD &rp, int iy, vec3f &color, float &alpha) {
+#pragma clang fp contract(off)
+  const float t = (iy + .5f) / float(rp.height);
+  if (rp.bg[3] >= 0.f) {
+    color = {rp.bg[0], rp.bg[1], rp.bg[2]};
+    alpha = rp.bg[3];
+  } else {
+    color = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
+    alpha = (1.0f - t) * 1.f + t * 1.f;
+  }
+}
+
 // generateRays.cu:73-129, 199-210 for a perspective camera without depth of
 // field: pixel jitter 0.5 on the first frame, else from Random(ix+W*a, iy+H*a).
 BN_HD PathRay camera_ray(const CameraDD &cam, const RenderDD &rp, int ix, int iy) {
@@ -117,14 +131,7 @@ BN_HD PathRay camera_ray(const CameraDD &cam, const RenderDD &rp, int ix, int iy
   ray.numDiffuse = 0;
   ray.isShadow = false;
   ray.isInMedium = false;
-  const float t = (iy + .5f) / float(rp.height);
-  if (rp.bg[3] >= 0.f) {
-    ray.missColor = {rp.bg[0], rp.bg[1], rp.bg[2]};
-    ray.missAlpha = rp.bg[3];
-  } else {
-    ray.missColor = (1.0f - t) * vec3f(0.9f, 0.9f, 0.9f) + t * vec3f(0.15f, 0.25f, .8f);
-    ray.missAlpha = (1.0f - t) * 1.f + t * 1.f;
-  }
+  miss_color(rp, iy, ray.missColor, ray.missAlpha);
   return ray;
 }
 

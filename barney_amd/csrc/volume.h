@@ -158,6 +158,183 @@ struct VolHit {
 
 BN_HD int vfloor_i(float f) { return (int)floorf(f); }
 
+// MCVolumeAccel::isProg (MCAccelerator.h:569-708) as a resumable walk: the
+// box test, dda::dda3 over the macrocells (DDA.h:65-206) and
+// Woodcock::sampleRange (Volume.h:188-223) in every cell with a non-zero
+// majorant, with the walk's state held here so that one call is one unit
+// of work: advance() enters or leaves a macrocell or draws a collision
+// distance; sample() evaluates the field at a pending collision. A kernel
+// can then keep rays at different stages in one loop and choose, per
+// wavefront iteration, which kind of work to run for the lanes that need it,
+// instead of nesting a per-ray DDA loop around a per-cell Woodcock loop. The arithmetic, the order of the RNG draws and the
+// counters are those of the nested form, operation for operation.
+struct DeltaTrack {
+  enum : int { DONE = 0, ENTER = 1, WOODCOCK = 2, ADVANCE = 3, SAMPLE = 4 };
+  vec3f org, dir;
+  float rayTMax;
+  Random2 rng;
+  vec3f t_next, t_step;
+  int cx, cy, cz;
+  uint32_t neg;           // bit k: the DDA steps down along axis k
+  float ray_t0, tMax;     // DDA origin offset, clipped segment end
+  float next_cell_begin;
+  float majorant, nextMajorant;
+  float t, upper;         // Woodcock position and end of the current cell
+  int mode;
+  bool hit;
+  vec3f rgb;
+
+  BN_HD bool done() const { return mode == DONE; }
+
+  // Ray set-up up to the first macrocell. Like isProg, the ray's seed is
+  // advanced (Random2(seed, geomHash)) only when the box test passes.
+  template <class V>
+  BN_HD void init(const V &v, vec3f o, vec3f d, float tmin, float tmax, float rtmax, RNGSeed &seed,
+                  uint64_t geomHash) {
+#pragma clang fp contract(off)
+    org = o;
+    dir = d;
+    rayTMax = rtmax;
+    hit = false;
+    mode = DONE;
+    float t0 = tmin, t1 = tmax;
+    {
+      const vec3f t_lo = (v.worldLo - org) * rcp(dir);
+      const vec3f t_hi = (v.worldHi - org) * rcp(dir);
+      const vec3f t_nr = vmin(t_lo, t_hi);
+      const vec3f t_fr = vmax(t_lo, t_hi);
+      t0 = fmaxf(t0, reduce_max(t_nr));
+      t1 = fminf(t1, reduce_min(t_fr));
+      if (!(t0 <= t1)) return;
+    }
+    const vec3f dorg = (org - v.mcOrigin) * rcp(v.mcSpacing);
+    const vec3f ddir = dir * rcp(v.mcSpacing);
+    rng = Random2(seed, geomHash);
+    // dda::dda3(dorg, ddir, t1, mcDims, lambda)
+    tMax = t1;
+    const vec3f f_size = {(float)v.mx, (float)v.my, (float)v.mz};
+    const vec3f rcp_dir = rcp(ddir);
+    vec3f t_lo = (vec3f(0.f) - dorg) * rcp(ddir);
+    vec3f t_hi = (f_size - dorg) * rcp(ddir);
+    vec3f t_nr = vmin(t_lo, t_hi);
+    vec3f t_fr = vmax(t_lo, t_hi);
+    if (ddir.x == 0.f) {
+      if (dorg.x < 0.f || dorg.x > f_size.x) return;
+      t_nr.x = -BN_INF; t_fr.x = BN_INF;
+    }
+    if (ddir.y == 0.f) {
+      if (dorg.y < 0.f || dorg.y > f_size.y) return;
+      t_nr.y = -BN_INF; t_fr.y = BN_INF;
+    }
+    if (ddir.z == 0.f) {
+      if (dorg.z < 0.f || dorg.z > f_size.z) return;
+      t_nr.z = -BN_INF; t_fr.z = BN_INF;
+    }
+    ray_t0 = fmaxf(0.f, reduce_max(t_nr));
+    const float ray_t1 = fminf(tMax, reduce_min(t_fr));
+    if (ray_t0 > ray_t1) return;
+    const vec3f oiv = dorg + ray_t0 * ddir;
+    const vec3f f_cell =
+        vmax(vec3f(0.f), vmin(f_size - vec3f(1.f), vec3f(floorf(oiv.x), floorf(oiv.y), floorf(oiv.z))));
+    const vec3f f_cell_end = {ddir.x > 0.f ? f_cell.x + 1.f : f_cell.x, ddir.y > 0.f ? f_cell.y + 1.f : f_cell.y,
+                              ddir.z > 0.f ? f_cell.z + 1.f : f_cell.z};
+    t_step = vabs(rcp_dir);
+    t_next = {(ddir.x == 0.f) ? BN_INF : (fabsf(f_cell_end.x - oiv.x) * t_step.x),
+              (ddir.y == 0.f) ? BN_INF : (fabsf(f_cell_end.y - oiv.y) * t_step.y),
+              (ddir.z == 0.f) ? BN_INF : (fabsf(f_cell_end.z - oiv.z) * t_step.z)};
+    neg = (ddir.x > 0.f ? 0u : 1u) | (ddir.y > 0.f ? 0u : 2u) | (ddir.z > 0.f ? 0u : 4u);
+    cx = (int)f_cell.x;
+    cy = (int)f_cell.y;
+    cz = (int)f_cell.z;
+    next_cell_begin = 0.f;
+    majorant = v.majorants[cx + v.mx * (cy + v.my * cz)];
+    mode = ENTER;
+  }
+
+  // One cheap unit of work: a collision distance drawn in the current cell
+  // (a point inside the cell leaves the walk in SAMPLE, waiting for the
+  // field), or a macrocell step of the DDA (leaving the cell, then entering
+  // the next one, with its majorant fetched one cell ahead).
+  // kExactLog: draw with logf instead of barney's fastLog.
+  template <bool kExactLog, class V>
+  BN_HD void advance(const V &v, VolCounters &cnt) {
+#pragma clang fp contract(off)
+    if (mode == WOODCOCK) {
+      const float r = rng();
+      const float dt = kExactLog ? -logf(1.f - r) / majorant : -fast_log(1.f - r) / majorant;
+      t += dt;
+      if (!(t >= upper)) {
+        mode = SAMPLE;
+        return;
+      }
+      mode = ADVANCE;
+    }
+    if (mode == ADVANCE) {
+      const float t_closest = reduce_min(t_next);
+      majorant = nextMajorant;
+      next_cell_begin = t_closest;
+      if (t_next.x == t_closest) {
+        t_next.x += t_step.x;
+        cx += (neg & 1u) ? -1 : 1;
+        if (cx == ((neg & 1u) ? -1 : v.mx)) { mode = DONE; return; }
+      }
+      if (t_next.y == t_closest) {
+        t_next.y += t_step.y;
+        cy += (neg & 2u) ? -1 : 1;
+        if (cy == ((neg & 2u) ? -1 : v.my)) { mode = DONE; return; }
+      }
+      if (t_next.z == t_closest) {
+        t_next.z += t_step.z;
+        cz += (neg & 4u) ? -1 : 1;
+        if (cz == ((neg & 4u) ? -1 : v.mz)) { mode = DONE; return; }
+      }
+    }
+    // ENTER: the cell's segment, and the next cell's majorant fetched ahead
+    const float t_closest = reduce_min(t_next);
+    const float cell_t0 = ray_t0 + next_cell_begin;
+    const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
+    cnt.ddaSteps++;
+    int ncx = cx, ncy = cy, ncz = cz;
+    if (t_next.x == t_closest) ncx += (neg & 1u) ? -1 : 1;
+    if (t_next.y == t_closest) ncy += (neg & 2u) ? -1 : 1;
+    if (t_next.z == t_closest) ncz += (neg & 4u) ? -1 : 1;
+    const bool inside = ncx >= 0 && ncx < v.mx && ncy >= 0 && ncy < v.my && ncz >= 0 && ncz < v.mz;
+    nextMajorant = inside ? v.majorants[ncx + v.mx * (ncy + v.my * ncz)] : 0.f;
+    if (majorant != 0.f) {
+      t = cell_t0;
+      upper = fminf(cell_t1, rayTMax);
+      mode = WOODCOCK;
+    } else {
+      mode = ADVANCE;
+    }
+  }
+
+  // The pending collision (mode SAMPLE): S maps P to the TF-mapped sample;
+  // accepted with probability sample.w / majorant.
+  template <class S>
+  BN_HD void sample(const S &sampler, VolCounters &cnt) {
+#pragma clang fp contract(off)
+    const vec3f P = org + t * dir;
+    const VolSample s = sampler(P);
+    cnt.samples++;
+    if (s.w >= rng() * majorant) {
+      hit = true;
+      rgb = {s.r, s.g, s.b};
+      mode = DONE;
+    } else {
+      mode = WOODCOCK;
+    }
+  }
+
+  BN_HD VolHit result() const {
+    VolHit r;
+    r.hit = hit;
+    r.t = t;
+    r.rgb = rgb;
+    return r;
+  }
+};
+
 // MCVolumeAccel::isProg for one ray segment [tmin, tmax]: box test, DDA over
 // macrocells, Woodcock tracking in each cell with a non-zero majorant. The
 // ray's RNG seed is advanced as Random2(seed, hash(inst, geom, 0)) does.
@@ -168,114 +345,20 @@ template <class V, class S>
 BN_HD VolHit volume_intersect_g(const V &v, const S &sample, vec3f org, vec3f dir, float tmin, float tmax,
                                 float rayTMax, RNGSeed &seed, bool exactLog, VolCounters &cnt,
                                 uint64_t geomHash) {
-#pragma clang fp contract(off)
-  VolHit res;
-  res.hit = false;
-  float t0 = tmin, t1 = tmax;
-  {
-    const vec3f t_lo = (v.worldLo - org) * rcp(dir);
-    const vec3f t_hi = (v.worldHi - org) * rcp(dir);
-    const vec3f t_nr = vmin(t_lo, t_hi);
-    const vec3f t_fr = vmax(t_lo, t_hi);
-    t0 = fmaxf(t0, reduce_max(t_nr));
-    t1 = fminf(t1, reduce_min(t_fr));
-    if (!(t0 <= t1)) return res;
-  }
-  const vec3f dorg = (org - v.mcOrigin) * rcp(v.mcSpacing);
-  const vec3f ddir = dir * rcp(v.mcSpacing);
-  Random2 rng(seed, geomHash);
-
-  // dda::dda3(dorg, ddir, t1, mcDims, lambda)
-  const float tMax = t1;
-  const vec3f f_size = {(float)v.mx, (float)v.my, (float)v.mz};
-  const vec3f rcp_dir = rcp(ddir);
-  vec3f t_lo = (vec3f(0.f) - dorg) * rcp(ddir);
-  vec3f t_hi = (f_size - dorg) * rcp(ddir);
-  vec3f t_nr = vmin(t_lo, t_hi);
-  vec3f t_fr = vmax(t_lo, t_hi);
-  if (ddir.x == 0.f) {
-    if (dorg.x < 0.f || dorg.x > f_size.x) return res;
-    t_nr.x = -BN_INF; t_fr.x = BN_INF;
-  }
-  if (ddir.y == 0.f) {
-    if (dorg.y < 0.f || dorg.y > f_size.y) return res;
-    t_nr.y = -BN_INF; t_fr.y = BN_INF;
-  }
-  if (ddir.z == 0.f) {
-    if (dorg.z < 0.f || dorg.z > f_size.z) return res;
-    t_nr.z = -BN_INF; t_fr.z = BN_INF;
-  }
-  const float ray_t0 = fmaxf(0.f, reduce_max(t_nr));
-  const float ray_t1 = fminf(tMax, reduce_min(t_fr));
-  if (ray_t0 > ray_t1) return res;
-  const vec3f oiv = dorg + ray_t0 * ddir;
-  const vec3f f_cell = vmax(vec3f(0.f), vmin(f_size - vec3f(1.f), vec3f(floorf(oiv.x), floorf(oiv.y), floorf(oiv.z))));
-  const vec3f f_cell_end = {ddir.x > 0.f ? f_cell.x + 1.f : f_cell.x, ddir.y > 0.f ? f_cell.y + 1.f : f_cell.y,
-                            ddir.z > 0.f ? f_cell.z + 1.f : f_cell.z};
-  const vec3f t_step = vabs(rcp_dir);
-  vec3f t_next = {(ddir.x == 0.f) ? BN_INF : (fabsf(f_cell_end.x - oiv.x) * t_step.x),
-                  (ddir.y == 0.f) ? BN_INF : (fabsf(f_cell_end.y - oiv.y) * t_step.y),
-                  (ddir.z == 0.f) ? BN_INF : (fabsf(f_cell_end.z - oiv.z) * t_step.z)};
-  const int stopx = ddir.x > 0.f ? v.mx : -1, stopy = ddir.y > 0.f ? v.my : -1, stopz = ddir.z > 0.f ? v.mz : -1;
-  const int dx = ddir.x > 0.f ? 1 : -1, dy = ddir.y > 0.f ? 1 : -1, dz = ddir.z > 0.f ? 1 : -1;
-  int cx = (int)f_cell.x, cy = (int)f_cell.y, cz = (int)f_cell.z;
-  float next_cell_begin = 0.f;
-  // The next cell of the walk is known before the current cell's Woodcock
-  // loop runs, so its majorant is fetched one step ahead to overlap the load
-  // latency with the sampling of the current cell.
-  float majorant = v.majorants[cx + v.mx * (cy + v.my * cz)];
-  while (true) {
-    const float t_closest = reduce_min(t_next);
-    const float cell_t0 = ray_t0 + next_cell_begin;
-    const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
-    cnt.ddaSteps++;
-    int ncx = cx, ncy = cy, ncz = cz;
-    if (t_next.x == t_closest) ncx += dx;
-    if (t_next.y == t_closest) ncy += dy;
-    if (t_next.z == t_closest) ncz += dz;
-    const bool inside = ncx >= 0 && ncx < v.mx && ncy >= 0 && ncy < v.my && ncz >= 0 && ncz < v.mz;
-    const float nextMajorant = inside ? v.majorants[ncx + v.mx * (ncy + v.my * ncz)] : 0.f;
-    // ---- lambda body (MCAccelerator.h:619-705) ----
-    if (majorant != 0.f) {
-      const float lower = cell_t0;
-      const float upper = fminf(cell_t1, rayTMax);
-      // Woodcock::sampleRange (Volume.h:188-223)
-      float t = lower;
-      while (true) {
-        const float r = rng();
-        const float dt = exactLog ? -logf(1.f - r) / majorant : -fast_log(1.f - r) / majorant;
-        t += dt;
-        if (t >= upper) break;
-        const vec3f P = org + t * dir;
-        const VolSample s = sample(P);
-        cnt.samples++;
-        if (s.w >= rng() * majorant) {
-          res.hit = true;
-          res.t = t;
-          res.rgb = {s.r, s.g, s.b};
-          return res;
-        }
-      }
+  DeltaTrack tr;
+  tr.init(v, org, dir, tmin, tmax, rayTMax, seed, geomHash);
+  for (;;) {
+    // cheap steps until the walk needs the field (the lanes of a wavefront
+    // then sample together) or ends
+    if (exactLog) {
+      while (tr.mode != DeltaTrack::DONE && tr.mode != DeltaTrack::SAMPLE) tr.advance<true>(v, cnt);
+    } else {
+      while (tr.mode != DeltaTrack::DONE && tr.mode != DeltaTrack::SAMPLE) tr.advance<false>(v, cnt);
     }
-    // ---- DDA step ----
-    majorant = nextMajorant;
-    next_cell_begin = t_closest;
-    if (t_next.x == t_closest) {
-      t_next.x += t_step.x;
-      cx += dx;
-      if (cx == stopx) return res;
-    }
-    if (t_next.y == t_closest) {
-      t_next.y += t_step.y;
-      cy += dy;
-      if (cy == stopy) return res;
-    }
-    if (t_next.z == t_closest) {
-      t_next.z += t_step.z;
-      cz += dz;
-      if (cz == stopz) return res;
-    }
+    if (tr.done()) break;
+    tr.sample(sample, cnt);
   }
+  return tr.result();
 }
 
 

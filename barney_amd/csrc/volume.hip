@@ -56,134 +56,273 @@ __device__ __forceinline__ int xcd_remap_v(int bid, int nb) {
   return t < nb ? t : -1;
 }
 
+// Per-lane stage of the volume path tracer (k_render_volume).
+enum VolStage : int {
+  VS_DEAD = 0,   // the wavefront's pool of paths is used up
+  VS_GEN = 1,    // needs a path from the pool (and its camera ray)
+  VS_INIT = 2,   // a ray waits in the walk's org/dir/rayTMax for set-up
+  VS_MARCH = 3,  // the walk is running (or has ended: see DeltaTrack::mode)
+};
+
+// Paths a wavefront owns at a time (one LDS result slot each).
+constexpr int kPoolPaths = 256;
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long long *dst) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
 }
 
-// One thread per (pixel, path): the L = min(64, pow2 >= spp) paths of a pixel
-// run in adjacent lanes (so a wavefront marches near-identical rays through
-// the same macrocells and voxels), each lane looping over paths sub,
-// sub+L, ...; the per-pixel sum is a butterfly over those L lanes.
-template <int L>
-__global__ void __launch_bounds__(256) k_render_volume(VolumeDD v, CameraDD cam, RenderDD rp, int spp,
+// Each wavefront owns PW pixels of a workgroup tile and the PW * spp paths
+// of those pixels as a pool (in units of at most kPoolPaths paths when spp
+// is large): a lane takes the next path of the pool whenever its previous
+// one has ended, so lanes whose rays leave the volume at once keep working
+// while neighbours walk long macrocell paths. A path's sum (rgb, alpha) goes
+// to an LDS slot; when the pool is used up, each pixel adds its paths in path
+// order, as barney accumulates them.
+//
+// A path is a sequence of small kinds of work: camera ray, walk set-up
+// (DeltaTrack::init), cheap walk steps (macrocell DDA step or collision
+// distance), field samples at collisions, shading of a collision (barney's
+// bounce(): NEE shadow ray toward the environment) and shading of an escaped
+// ray (background, or the unoccluded shadow contribution). Every wavefront
+// iteration runs ONE kind, the one most lanes are waiting for, for all of
+// them, so no iteration pays for code its lanes do not need. Per path the
+// rays, RNG draws and fragment order are those of barney's generate /
+// trace / shade loop.
+//
+// Occupancy: the kernel is issue-bound, and three wavefronts per SIMD
+// (<= 168 VGPRs) measured 8.8 ms per 1920x1080x16spp frame against 12.4 ms
+// at the two that the unconstrained allocation (178 VGPRs) leaves; the cap
+// costs a few dwords of spill in the shading code.
+template <int PW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_render_volume(VolumeDD v, CameraDD cam, RenderDD rp, int spp,
                                                        float4 *__restrict accum,
                                                        unsigned long long *__restrict counters) {
-  constexpr int PW = 64 / L;            // pixels per wavefront
-  constexpr int PB = 4 * PW;            // pixels per workgroup
-  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int PB = 4 * PW;  // pixels per workgroup
+  constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
+  constexpr int kUnit = kPoolPaths / PW;  // paths per pixel in one pool
+  __shared__ float4 pathSum[4][kPoolPaths];
   const int tilesX = (rp.width + TW - 1) / TW, tilesY = (rp.height + TH - 1) / TH;
   const int tile = xcd_remap_v(blockIdx.x, tilesX * tilesY);
   if (tile < 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int sub = lane & (L - 1);
-  const int q = wave * PW + lane / L;
-  const int ix = (tile % tilesX) * TW + (q % TW);
-  const int iy = (tile / tilesX) * TH + (q / TW);
+  const int tx0 = (tile % tilesX) * TW, ty0 = (tile / tilesX) * TH;
+  float4 *const slots = pathSum[wave];
+  const unsigned long long lanesBelow = (1ull << lane) - 1ull;
   VolCounters cnt{0, 0};
-  unsigned long long nrays = 0;
-  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
-  const bool active = ix < rp.width && iy < rp.height;
-  if (active) {
-    for (int p = sub; p < spp; p += L) {
-      RenderDD rpp = rp;
-      rpp.accumID = rp.accumID + p;
-      PathRay ray = camera_ray(cam, rpp, ix, iy);
-      const float clampMax = 10.f * (1 + rpp.accumID);
-      vec3f sum = vec3f(0.f);
-      float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
-        const vec3f dir = fix_dir(ray.dir);
-        const VolHit vh = volume_intersect(v, ray.org, dir, 0.f, ray.tMax, ray.tMax, ray.rng, false, cnt);
-        ++nrays;
-        ShadeHit sh;
-        sh.hit = vh.hit;
-        if (vh.hit) {
-          sh.P = ray.org + vh.t * dir;
-          sh.N = vec3f(0.f);
-          sh.t = vh.t;
-          sh.bsdf = phase_bsdf(vh.rgb);
+  unsigned nrays = 0;
+  const auto sampler = [&](vec3f P) { return sample_and_map(v, P); };
+  const uint64_t geomHash = bn_hash3(0u, 0u, 0u);
+  float4 pixSum = make_float4(0.f, 0.f, 0.f, 0.f);  // lane j's pixel, across pools (PW == 1)
+
+  for (int s0 = 0; s0 < spp; s0 += kUnit) {
+    const int nPaths = PW * min(kUnit, spp - s0);
+    int next = 0;                 // the pool's next path (wavefront-uniform)
+    int k = 0;                    // this lane's path: pixel k % PW, sample s0 + k / PW
+    int stage = VS_GEN;
+    bool inShadow = false;        // the walk holds the path's shadow ray
+    int gen = 0;
+    RenderDD rpp = rp;
+    PathRay ray;                  // the path's current (non-shadow) ray
+    vec3f shadowTp = vec3f(0.f);  // contribution of the shadow ray if unoccluded
+    float shadowMis = 0.f;
+    vec3f sum = vec3f(0.f);
+    float alpha = 0.f;
+    DeltaTrack tr;
+    tr.mode = DeltaTrack::DONE;
+
+    for (;;) {
+      const bool march = stage == VS_MARCH;
+      const bool wCheap = march && tr.mode != DeltaTrack::DONE && tr.mode != DeltaTrack::SAMPLE;
+      const bool wSample = march && tr.mode == DeltaTrack::SAMPLE;
+      const bool wHit = march && tr.mode == DeltaTrack::DONE && tr.hit && !inShadow;
+      const bool wMiss = march && tr.mode == DeltaTrack::DONE && !(tr.hit && !inShadow);
+      const bool wGen = stage == VS_GEN, wInit = stage == VS_INIT;
+      const unsigned long long genMask = __ballot(wGen);
+      if (__ballot(stage != VS_DEAD) == 0ull) break;
+      // the kind of work most lanes wait for (counts from ballots are
+      // wavefront-uniform, so the branch below is too)
+      int best = 0, most = (int)__popcll(__ballot(wCheap));
+      const int waiting[5] = {(int)__popcll(__ballot(wSample)), (int)__popcll(genMask),
+                              (int)__popcll(__ballot(wInit)), (int)__popcll(__ballot(wHit)),
+                              (int)__popcll(__ballot(wMiss))};
+#pragma unroll
+      for (int w = 0; w < 5; ++w)
+        if (waiting[w] > most) {
+          most = waiting[w];
+          best = w + 1;
         }
-        if (gen == 0) alpha = vh.hit ? 1.f : ray.missAlpha;
-        PathRay shadow;
-        vec3f frag = vmin(bounce(rpp, ray, sh, shadow, gen), clampMax);
-        if (frag.x > 0.f) sum.x += frag.x;
-        if (frag.y > 0.f) sum.y += frag.y;
-        if (frag.z > 0.f) sum.z += frag.z;
-        if (shadow.tMax > 0.f) {
-          const vec3f sdir = fix_dir(shadow.dir);
-          const VolHit sv =
-              volume_intersect(v, shadow.org, sdir, 0.f, shadow.tMax, shadow.tMax, shadow.rng, false, cnt);
+      bool pathDone = false;
+      if (best == 0) {
+        if (wCheap) tr.advance<false>(v, cnt);
+      } else if (best == 1) {
+        if (wSample) tr.sample(sampler, cnt);
+      } else if (best == 2) {
+        if (wGen) {  // take the next path of the pool; generateRays for it
+          k = next + (int)__popcll(genMask & lanesBelow);
+          const int j = k % PW, q = wave * PW + j;
+          const int ix = tx0 + q % TW, iy = ty0 + q / TW;
+          if (k >= nPaths) {
+            stage = VS_DEAD;
+          } else if (ix >= rp.width || iy >= rp.height) {
+            slots[k] = make_float4(0.f, 0.f, 0.f, 0.f);  // outside the frame: take another path
+          } else {
+            rpp.accumID = rp.accumID + s0 + k / PW;
+            ray = camera_ray(cam, rpp, ix, iy);
+            sum = vec3f(0.f);
+            alpha = 0.f;
+            gen = 0;
+            inShadow = false;
+            tr.org = ray.org;
+            tr.dir = fix_dir(ray.dir);
+            tr.rayTMax = ray.tMax;
+            stage = VS_INIT;
+          }
+        }
+        next += (int)__popcll(genMask);
+      } else if (best == 3) {
+        if (wInit) {  // a shadow ray's seed waits in tr.rng; a path ray's is ray.rng
+          RNGSeed seed;
+          seed.state = inShadow ? tr.rng.state : ray.rng.state;
+          tr.init(v, tr.org, tr.dir, 0.f, tr.rayTMax, tr.rayTMax, seed, geomHash);
+          if (!inShadow) ray.rng.state = seed.state;
+          stage = VS_MARCH;
+        }
+      } else if (best == 4) {
+        if (wHit) {  // a collision of the path ray: shade, spawn the shadow ray
           ++nrays;
-          ShadeHit shs;
-          shs.hit = sv.hit;
-          PathRay dummy;
-          frag = vmin(bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          ShadeHit sh;
+          sh.hit = true;
+          sh.t = tr.t;
+          sh.P = tr.org + tr.t * tr.dir;
+          sh.N = vec3f(0.f);
+          sh.bsdf = phase_bsdf(tr.rgb);
+          if (gen == 0) alpha = 1.f;
+          PathRay sray;
+          const vec3f frag = vmin(bounce(rpp, ray, sh, sray, gen), 10.f * (1 + rpp.accumID));
           if (frag.x > 0.f) sum.x += frag.x;
           if (frag.y > 0.f) sum.y += frag.y;
           if (frag.z > 0.f) sum.z += frag.z;
+          if (sray.tMax > 0.f) {
+            shadowTp = sray.tp;
+            shadowMis = sray.mis;
+            tr.org = sray.org;
+            tr.dir = fix_dir(sray.dir);
+            tr.rayTMax = sray.tMax;
+            tr.rng.state = sray.rng.state;
+            inShadow = true;
+            stage = VS_INIT;
+          } else if (ray.tMax > 0.f) {
+            ++gen;
+            tr.org = ray.org;
+            tr.dir = fix_dir(ray.dir);
+            tr.rayTMax = ray.tMax;
+            stage = VS_INIT;
+          } else {
+            pathDone = true;
+          }
+        }
+      } else {
+        if (wMiss) {  // an escaped path ray, or a finished shadow ray
+          ++nrays;
+          vec3f frag;
+          if (inShadow) {
+            // bounce() of a shadow ray: its contribution when unoccluded
+            frag = tr.hit ? vec3f(0.f) : shadowMis * shadowTp;
+            inShadow = false;
+          } else {
+            ShadeHit sh;
+            sh.hit = false;
+            // camera_ray's background, recomputed here rather than carried
+            // through the walk
+            const int q = wave * PW + k % PW;
+            miss_color(rpp, ty0 + q / TW, ray.missColor, ray.missAlpha);
+            if (gen == 0) alpha = ray.missAlpha;
+            PathRay sray;
+            frag = bounce(rpp, ray, sh, sray, gen);
+          }
+          frag = vmin(frag, 10.f * (1 + rpp.accumID));
+          if (frag.x > 0.f) sum.x += frag.x;
+          if (frag.y > 0.f) sum.y += frag.y;
+          if (frag.z > 0.f) sum.z += frag.z;
+          if (ray.tMax > 0.f) {
+            ++gen;
+            tr.org = ray.org;
+            tr.dir = fix_dir(ray.dir);
+            tr.rayTMax = ray.tMax;
+            stage = VS_INIT;
+          } else {
+            pathDone = true;
+          }
         }
       }
-      sr += sum.x;
-      sg += sum.y;
-      sb += sum.z;
-      sa += alpha;
+      if (pathDone) {
+        slots[k] = make_float4(sum.x, sum.y, sum.z, alpha);
+        stage = VS_GEN;
+      }
     }
-  }
-#pragma unroll
-  for (int off = L / 2; off > 0; off >>= 1) {
-    sr += __shfl_xor(sr, off);
-    sg += __shfl_xor(sg, off);
-    sb += __shfl_xor(sb, off);
-    sa += __shfl_xor(sa, off);
-  }
-  if (active && sub == 0) {
-    const int pix = ix + iy * rp.width;
-    if (rp.accumID == 0) {
-      accum[pix] = make_float4(sr, sg, sb, sa);
-    } else {
-      float4 o = accum[pix];
-      o.x += sr; o.y += sg; o.z += sb; o.w += sa;
-      accum[pix] = o;
+    // every lane's slot writes are done: make them visible across the
+    // wavefront, then each pixel adds its paths in path order
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int ns = nPaths / PW;
+    for (int j = lane; j < PW; j += 64) {
+      float4 a = (s0 == 0) ? make_float4(0.f, 0.f, 0.f, 0.f) : pixSum;
+      for (int si = 0; si < ns; ++si) {
+        const float4 r = slots[si * PW + j];
+        a.x += r.x;
+        a.y += r.y;
+        a.z += r.z;
+        a.w += r.w;
+      }
+      pixSum = a;
+      const int q = wave * PW + j;
+      const int ix = tx0 + q % TW, iy = ty0 + q / TW;
+      if (s0 + kUnit >= spp && ix < rp.width && iy < rp.height) {
+        const int pix = ix + iy * rp.width;
+        if (rp.accumID == 0) {
+          accum[pix] = a;
+        } else {
+          float4 o = accum[pix];
+          o.x += a.x; o.y += a.y; o.z += a.z; o.w += a.w;
+          accum[pix] = o;
+        }
+      }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // slots are reused by the next pool
   }
   wave_add_u64(cnt.samples, counters + 0);
   wave_add_u64(cnt.ddaSteps, counters + 1);
-  wave_add_u64(nrays, counters + 2);
+  wave_add_u64((unsigned long long)nrays, counters + 2);
 }
 
-template <int L>
+template <int PW>
 void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                           unsigned long long *counters, hipStream_t s) {
-  constexpr int PB = 4 * (64 / L);
-  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
+  constexpr int PB = 4 * PW;
+  constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   // xcd_remap_v hands XCD x the tiles [x*per, (x+1)*per): the grid must
   // hold 8*per workgroups (a multiple of 8) to cover every tile
   const int grid = ((tiles + 7) / 8) * 8;
-  k_render_volume<L><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
+  k_render_volume<PW><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
 }
 
+// Pixels per wavefront: as many as keep the pool at kPoolPaths paths (the
+// pixels' paths all in one pool), at least one.
 void render_volume_dispatch(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                             unsigned long long *counters, hipStream_t s) {
-  // Lanes per pixel: a few lanes per pixel, each looping over several of
-  // its paths, balance coherence (near-identical rays in a wavefront)
-  // against per-lane work variance; 4 measured best for 16 spp.
-  // BNAMD_VOLUME_LANES overrides (1/2/4/8) for tuning.
-  int lanes = 4;
-  if (const char *e = getenv("BNAMD_VOLUME_LANES")) lanes = atoi(e);
-  if (lanes <= 1) { launch_render_volume<1>(v, cam, rp, spp, accum, counters, s); return; }
-  if (lanes == 2) { launch_render_volume<2>(v, cam, rp, spp, accum, counters, s); return; }
-  if (lanes == 4) { launch_render_volume<4>(v, cam, rp, spp, accum, counters, s); return; }
-  if (lanes == 8) { launch_render_volume<8>(v, cam, rp, spp, accum, counters, s); return; }
-  if (spp >= 64) launch_render_volume<64>(v, cam, rp, spp, accum, counters, s);
-  else if (spp > 16) launch_render_volume<32>(v, cam, rp, spp, accum, counters, s);
-  else if (spp > 8) launch_render_volume<16>(v, cam, rp, spp, accum, counters, s);
-  else if (spp > 4) launch_render_volume<8>(v, cam, rp, spp, accum, counters, s);
-  else if (spp > 2) launch_render_volume<4>(v, cam, rp, spp, accum, counters, s);
-  else if (spp > 1) launch_render_volume<2>(v, cam, rp, spp, accum, counters, s);
+  if (spp <= 1) launch_render_volume<256>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 2) launch_render_volume<128>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 4) launch_render_volume<64>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 8) launch_render_volume<32>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 16) launch_render_volume<16>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 32) launch_render_volume<8>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 64) launch_render_volume<4>(v, cam, rp, spp, accum, counters, s);
+  else if (spp <= 128) launch_render_volume<2>(v, cam, rp, spp, accum, counters, s);
   else launch_render_volume<1>(v, cam, rp, spp, accum, counters, s);
 }
 
