@@ -306,6 +306,14 @@ class Volume:
         check(capi.load().bnamd_volume_sample_rays(self.handle, dr.addr, ds.addr, do.addr, n, int(exact_log), None))
         return do.numpy()
 
+    def iso_rays(self, rays, iso):
+        """rays: [n, 8] (org, tmin, dir, tmax) -> [n, 4] (t or inf, normal)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        dr = DeviceArray(r)
+        do = DeviceArray.empty((len(r), 4), np.float32)
+        check(capi.load().bnamd_volume_iso_rays(self.handle, dr.addr, float(iso), do.addr, len(r), None))
+        return do.numpy()
+
     def render(self, cam, params, accum=None):
         acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
         st = capi.bnamd_volume_stats()

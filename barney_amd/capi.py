@@ -132,6 +132,7 @@ SIGNATURES = {
     "bnamd_umesh_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
     "bnamd_umesh_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bnamd_umesh_sample_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_volume_iso_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_umesh_iso_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_umesh_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
                                      C.POINTER(bnamd_umesh_render_opts), C.c_void_p, C.c_void_p,

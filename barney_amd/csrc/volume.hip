@@ -340,6 +340,27 @@ __global__ void k_volume_rays(VolumeDD v, const bnamd_ray *__restrict rays, cons
   out[i] = h.hit ? make_float4(h.t, h.rgb.x, h.rgb.y, h.rgb.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
 }
 
+// The macrocell value ranges of a structured volume in the form
+// iso_intersect walks (MCIsoSurfaceAccel over StructuredData).
+struct StructuredIsoGrid {
+  vec3f worldLo, worldHi, mcOrigin, mcSpacing;
+  int mx, my, mz;
+  const float2 *ranges;
+};
+
+__global__ void __launch_bounds__(256) k_volume_iso_rays(VolumeDD v, StructuredIsoGrid g,
+                                                         const bnamd_ray *__restrict rays, float iso,
+                                                         float4 *__restrict out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bnamd_ray r = rays[i];
+  RNGSeed seed;
+  const IsoHit h = iso_intersect(
+      g, [&](vec3f P) { return sample_field(v, P); }, iso, {r.org[0], r.org[1], r.org[2]},
+      fix_dir({r.dir[0], r.dir[1], r.dir[2]}), r.tmin, r.tmax, r.tmax, seed, bn_hash3(0u, 0u, 0u));
+  out[i] = h.hit ? make_float4(h.t, h.N.x, h.N.y, h.N.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
+}
+
 }  // namespace
 }  // namespace bnamd
 
@@ -474,6 +495,20 @@ extern "C" int bnamd_volume_sample_rays(const bnamd_volume *vol, const bnamd_ray
     hipStream_t s = (hipStream_t)stream;
     k_volume_rays<<<div_up(numRays, 256), 256, 0, s>>>(vol->v.dd, rays, seeds, (float4 *)out, numRays,
                                                        exactLog);
+    BN_HIP(hipGetLastError());
+    if (!stream) BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int bnamd_volume_iso_rays(const bnamd_volume *vol, const bnamd_ray *rays, float isoValue, float *out,
+                                     int numRays, void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(vol && rays && out, "bnamd_volume_iso_rays: NULL argument");
+    if (numRays <= 0) return;
+    const VolumeDD &d = vol->v.dd;
+    const StructuredIsoGrid g{d.worldLo, d.worldHi, d.mcOrigin, d.mcSpacing, d.mx, d.my, d.mz, vol->v.ranges.ptr};
+    hipStream_t s = (hipStream_t)stream;
+    k_volume_iso_rays<<<div_up(numRays, 256), 256, 0, s>>>(d, g, rays, isoValue, (float4 *)out, numRays);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
   });

@@ -226,6 +226,15 @@ int bnamd_volume_sample_rays(const bnamd_volume *vol, const bnamd_ray *rays,
                              const uint64_t *seeds, float *out, int numRays, int exactLog,
                              void *stream);
 
+/* Iso-surface query per ray on the structured field (MCIsoSurfaceAccel over
+ * StructuredDataSampler, MCAccelerator.h:338-567): out[i] = {t, N.x, N.y,
+ * N.z} of the first crossing of isoValue in [tmin, tmax] found by the
+ * macrocell walk with 10 linear sub-steps per cell (t = +inf if none); N is
+ * the normalized central-difference gradient. Needs no
+ * transfer function. */
+int bnamd_volume_iso_rays(const bnamd_volume *vol, const bnamd_ray *rays, float isoValue, float *out,
+                          int numRays, void *stream);
+
 /* ------------------------------------------------------------------------
  * Unstructured tetrahedral volumes (barney/umesh): UMeshField with per-vertex
  * scalars, UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,

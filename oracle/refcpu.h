@@ -141,6 +141,9 @@ void ref_umesh_sample(const ref_umesh *u, const float *points, float *out, int n
 /* iso-surface hit per ray (8 floats org,tmin,dir,tmax): out = t (inf if
  * none), normal[3] */
 void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, float *out, int n, int threads);
+/* iso-surface hit per ray on a structured volume (same layout as
+ * ref_umesh_iso_rays) */
+void ref_volume_iso_rays(const ref_volume *v, const float *rays, float isoValue, float *out, int n, int threads);
 /* volume (delta tracking) and/or iso-surface render, rows [y0, y1) */
 long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_render_params *p, int volume,
                            int iso, float isoValue, const float *isoColor, float *accum, int y0, int y1,

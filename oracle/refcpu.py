@@ -78,6 +78,7 @@ def load():
         "ref_tf_map": (None, [P, C.c_float, P]),
         "ref_tf_majorant": (C.c_float, [P, C.c_float, C.c_float]),
         "ref_volume_sample_rays": (None, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_volume_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
         "ref_dda_cells": (C.c_int, [P, P, C.c_float, P, P, P, C.c_int]),
         "ref_render_volume": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_world_create": (P, [P, C.c_int, P, P, P, C.c_int]),
@@ -254,6 +255,13 @@ class Volume:
         load().ref_volume_sample_rays(self.handle, rays.ctypes.data, seeds.ctypes.data, out.ctypes.data, len(rays),
                                       int(exact_log), threads, C.byref(ns))
         return out, ns.value
+
+    def iso_rays(self, rays, iso, threads=0):
+        """rays: [n, 8] (org, tmin, dir, tmax) -> [n, 4] (t or inf, normal)."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((len(r), 4), np.float32)
+        load().ref_volume_iso_rays(self.handle, r.ctypes.data, float(iso), out.ctypes.data, len(r), threads)
+        return out
 
     def render(self, cam, params, y0=0, y1=None, threads=0, accum=None):
         y1 = params.height if y1 is None else y1

@@ -99,3 +99,24 @@ def test_volume_render_every_pixel_any_frame_size(gpu, W, H, spp):
     assert np.isfinite(gacc).all()
     np.testing.assert_allclose(gacc, racc, rtol=1e-5, atol=1e-6)
     assert gout["stats"]["samples"] == nsamp
+
+
+@pytest.mark.parametrize("cpm", [8, 5])
+def test_iso_rays_bit_exact(gpu, cpm):
+    """Structured iso-surface walk: hit t and normal equal the oracle's
+    bit for bit (same macrocell ranges, sub-steps and gradient arithmetic)."""
+    f = _field(40)
+    gv, rv = _pair(gpu, f, cpm=cpm)
+    lo = np.array([1.0, -2.0, 0.5])
+    hi = lo + np.array([0.5, 0.5, 0.75]) * 39
+    rays = random_rays(20000, lo, hi, seed=23)
+    for iso in (0.3, 0.55, 2.0):
+        g = gv.iso_rays(rays, iso)
+        r = rv.iso_rays(rays, iso)
+        hit = np.isfinite(r[:, 0])
+        assert np.array_equal(np.isfinite(g[:, 0]), hit)
+        if iso < 1.0:
+            assert hit.sum() > 1000
+        else:
+            assert hit.sum() == 0
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
