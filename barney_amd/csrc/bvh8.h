@@ -393,6 +393,81 @@ BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restric
   return !kAny;
 }
 
+// The loop of traverse_from as a resumable walk: step() is one iteration
+// (a node visit and the primitives of its hit leaves), so that a kernel can
+// hand a lane its next ray as soon as the lane's walk is over while the rest
+// of the wavefront keeps walking. Closest-hit acceptance (better()) is a
+// total order on (t, flat index), so the hit found does not depend on the
+// order in which rays or nodes are visited.
+template <bool kAny>
+struct BvhWalk {
+  vec3f org, dir, inv;
+  uint32_t oct;
+  float tmin, tmax;
+  HitRec best;
+  uint32_t grpBase, grpBits, primBase, primMask;
+  int sp;
+  bool occluded;  // kAny: an occluder was found (or the stack overflowed)
+
+  BN_D void init(vec3f o, vec3f d, float t0, float t1) {
+    org = o;
+    dir = d;
+    inv = rcp(d);
+    oct = ray_octant(d);
+    tmin = t0;
+    tmax = t1;
+    best.t = t1;
+    best.flat = 0xffffffffu;
+    best.u = best.v = 0.f;
+    grpBase = 0;
+    grpBits = (1u << oct) | (1u << 8);  // root = slot 0 (k = 0 ^ oct)
+    primBase = 0;
+    primMask = 0;
+    sp = 0;
+    occluded = false;
+  }
+
+  // One iteration; true when the walk is over.
+  BN_D bool step(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, const TraverseCtx &ctx) {
+    if ((grpBits & 0xffu) != 0) {
+      const uint32_t k = __builtin_ctz(grpBits);
+      grpBits &= ~(1u << k);
+      const uint32_t slot = k ^ oct;
+      const uint32_t imask = grpBits >> 8;
+      const uint32_t nodeIdx = grpBase + __builtin_popcount(imask & ((1u << slot) - 1u));
+      if ((grpBits & 0xffu) != 0) {
+        if (sp >= ctx.maxDepth) {
+          occluded = kAny;
+          return true;
+        }
+        ctx.stack[sp * ctx.stride] = make_uint2(grpBase, grpBits);
+        ++sp;
+      }
+      bvh8_visit(nodes, nodeIdx, org, inv, oct, tmin, best.t, grpBase, grpBits, primBase, primMask);
+    }
+    while (primMask) {
+      const uint32_t i = __builtin_ctz(primMask);
+      primMask &= primMask - 1u;
+      if (kAny) {
+        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) {
+          occluded = true;
+          return true;
+        }
+      } else {
+        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, 0u);
+      }
+    }
+    if ((grpBits & 0xffu) == 0) {
+      if (sp == 0) return true;
+      --sp;
+      const uint2 e = ctx.stack[sp * ctx.stride];
+      grpBase = e.x;
+      grpBits = e.y;
+    }
+    return false;
+  }
+};
+
 // Closest-hit (or occlusion) traversal of one BVH8 from scratch.
 template <bool kAny>
 BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,

@@ -554,6 +554,91 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
   wave_add(nr, &e.cnt->shadowRays);
 }
 
+// Trace (kAny = false: closest hit into e.hits) or shadow (kAny = true:
+// unoccluded contributions into accum) pass over a flattened world with
+// per-lane ray replacement: a resident grid of wavefronts walks the queue;
+// each wavefront takes kWalkChunk rays at a time from the queue's work
+// counter and hands them to its lanes one by one, giving a lane its next ray
+// as soon as its walk is over (once at least kRefillLanes lanes wait, or no
+// lane walks), so that a wavefront does not idle behind its longest ray.
+constexpr unsigned kWalkChunk = 256;
+constexpr int kRefillLanes = 16;
+
+template <bool kAny>
+__global__ void __launch_bounds__(256) k_wf_walk(WfDD e, int depth) {
+  extern __shared__ uint2 lds_stack[];
+  const unsigned n = kAny ? e.cnt->shadow[depth] : e.cnt->count[depth];
+  unsigned *work = kAny ? &e.cnt->swork[depth] : &e.cnt->work[depth];
+  const PathRec *__restrict q = e.q[depth & 1];
+  const TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  BvhWalk<kAny> wk;
+  unsigned rid = ~0u;             // this lane's ray (~0u: none)
+  bool dry = false;               // the queue is used up (wavefront-uniform)
+  unsigned cNext = 0, cEnd = 0;   // rest of the wavefront's chunk
+  unsigned long long nr = 0;
+  for (;;) {
+    const unsigned long long need = __ballot(rid == ~0u);
+    const int nNeed = (int)__popcll(need);
+    if (!dry && (nNeed >= kRefillLanes || nNeed == 64)) {
+      const int rank = (int)__popcll(need & below);
+      unsigned mine = ~0u;
+      int given = 0;
+      while (given < nNeed) {
+        if (cNext == cEnd) {
+          unsigned base = 0;
+          if (lane == 0) base = atomicAdd(work, kWalkChunk);
+          base = __shfl(base, 0);
+          if (base >= n) {
+            dry = true;
+            break;
+          }
+          cNext = base;
+          cEnd = min(base + kWalkChunk, n);
+        }
+        const int take = min(nNeed - given, (int)(cEnd - cNext));
+        if (rid == ~0u && rank >= given && rank < given + take) mine = cNext + (unsigned)(rank - given);
+        cNext += (unsigned)take;
+        given += take;
+      }
+      if (mine != ~0u) {
+        rid = mine;
+        float4 a, b;
+        if (kAny) {
+          a = e.sq[rid].a;
+          b = e.sq[rid].b;
+        } else {
+          a = q[rid].a;
+          b = q[rid].b;
+        }
+        wk.init({a.x, a.y, a.z}, fix_dir({b.x, b.y, b.z}), 0.f, a.w);
+      }
+    }
+    if (__ballot(rid != ~0u) == 0ull) {
+      if (dry) break;
+      continue;
+    }
+    if (rid != ~0u && wk.step(e.w.nodes, e.w.prims, ctx)) {
+      ++nr;
+      if (kAny) {
+        if (!wk.occluded) {
+          const ShadowRec &sr = e.sq[rid];
+          const float4 c = sr.c;
+          const unsigned pid = __float_as_uint(sr.b.w);
+          const int sidx = (int)(pid / (unsigned)e.numPixels);
+          const int pix = (int)(pid % (unsigned)e.numPixels);
+          accum_add(e.accum, pix, {c.x, c.y, c.z}, 10.f * (1 + e.accum0 + sidx));
+        }
+      } else {
+        e.hits[rid] = wk.best;
+      }
+      rid = ~0u;
+    }
+  }
+  wave_add(nr, kAny ? &e.cnt->shadowRays : &e.cnt->rays);
+}
+
 __global__ void k_fill_f4(float4 *p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -612,8 +697,8 @@ int trace_grid(int blocksPerCU, size_t lds, bool twoLevel) {
     BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<true>, 256, lds));
     BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<true>, 256, lds));
   } else {
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<false>, 256, lds));
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<false>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_walk<false>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_walk<true>, 256, lds));
   }
   int occ = std::max(1, std::min(occT, occS));
   if (blocksPerCU > 0) occ = std::min(occ, blocksPerCU);
@@ -698,12 +783,14 @@ void enqueue_wavefront_t(const WfDD &e, int maxDepth, int gridPersist, size_t ld
   k_wf_generate<<<div_up(total, 256), 256, 0, s>>>(e);
   const int gridStream = std::min(div_up(total, 256), 8 * num_cus());
   for (int d = 0; d <= maxDepth; ++d) {
-    k_wf_trace<TL><<<gridPersist, 256, lds, s>>>(e, d);
+    if (TL) k_wf_trace<TL><<<gridPersist, 256, lds, s>>>(e, d);
+    else k_wf_walk<false><<<gridPersist, 256, lds, s>>>(e, d);
     k_wf_sort_count<TL><<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_sort_scan<<<1, kSortBlocks, 0, s>>>(e);
     k_wf_sort_scatter<<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_shade<TL><<<gridStream, 256, 0, s>>>(e, d);
-    k_wf_shadow<TL><<<gridPersist, 256, lds, s>>>(e, d);
+    if (TL) k_wf_shadow<TL><<<gridPersist, 256, lds, s>>>(e, d);
+    else k_wf_walk<true><<<gridPersist, 256, lds, s>>>(e, d);
   }
 }
 
