@@ -67,34 +67,106 @@ def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
     return res
 
 
-def volume_cpu_baseline(w2, seconds):
-    """The oracle on host cores over a band of rows of the same frame."""
-    from oracle import refcpu
-    import ctypes as C
+def cpu_threads():
     threads = os.cpu_count() or 1
-    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-    rv = refcpu.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
-    rv.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
-    W, H = w2["size"]
-    spp = w2["spp"]
-    cam = refcpu.camera(**w2["camera"])
-    p = refcpu.render_params(W, H, spp=spp)
-    acc = np.zeros((H, W, 4), np.float32)
+    return min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+
+
+def band_timer(render_rows, H, seconds, unit, what):
+    """Times the oracle over a band of rows around the frame's middle, sized
+    from a one-row calibration to take about 'seconds'. render_rows(y0, y1)
+    returns the count of work units (rays or samples) it did."""
     mid = H // 2
-    # calibrate on one row, then run a band sized to ~'seconds'
     t0 = time.perf_counter()
-    _, _, s1 = rv.render(cam, p, mid, mid + 1, threads, accum=acc)
+    render_rows(mid, mid + 1)
     dt = time.perf_counter() - t0
     rows = int(max(1, min(H, seconds / max(dt, 1e-3))))
     y0 = max(0, mid - rows // 2)
+    units, reps = 0, 0
     t0 = time.perf_counter()
-    _, nrays, ns = rv.render(cam, p, y0, y0 + rows, threads, accum=acc)
-    dt = time.perf_counter() - t0
-    return {"value": ns / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"rows {y0}..{y0 + rows - 1} of the 1920x1080x16spp frame ({ns} samples, {dt:.1f}s)"}
+    while True:  # the band again until the time is used (fast workloads)
+        units += render_rows(y0, y0 + rows)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * seconds or reps >= 1000:
+            break
+    return {"value": units / dt / 1e6, "unit": unit, "cores": cpu_threads(), "kind": "port",
+            "sample": f"rows {y0}..{y0 + rows - 1} of {what}, {reps}x ({units} {unit.split('/')[0][1:]}, "
+                      f"{dt:.1f}s)"}
 
 
-def surface_bench(api, steps=3, warmup=1):
+def volume_cpu_baseline(w2, seconds):
+    """The oracle on host cores over a band of rows of the same frame."""
+    from oracle import refcpu
+    rv = refcpu.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
+    rv.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
+    W, H = w2["size"]
+    cam = refcpu.camera(**w2["camera"])
+    p = refcpu.render_params(W, H, spp=w2["spp"])
+    acc = np.zeros((H, W, 4), np.float32)
+    return band_timer(lambda y0, y1: rv.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Msamples/s", f"the {W}x{H}x{w2['spp']}spp frame")
+
+
+def surface_cpu_baseline(w1, seconds):
+    """Workload 1 in the oracle (its own binned-SAH BVH over the same
+    triangles): rays traced per second over a band of rows."""
+    from oracle import refcpu
+    ref = refcpu.Scene([dict(kind=0, vertices=v, indices=t, color=c, user_id=k)
+                        for k, (v, t, c) in enumerate(w1["geoms"])])
+    W, H = w1["size"]
+    cam = refcpu.camera(**w1["camera"])
+    p = refcpu.render_params(W, H, spp=1)
+    acc = np.zeros((H, W, 4), np.float32)
+    return band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Mrays/s", f"the {W}x{H} frame, 1 spp")
+
+
+def _oracle_material(m):
+    if m["type"] == "matte":
+        return dict(type="matte", color=m["color"])
+    if m["type"] == "pbr":
+        return dict(type="pbr", color=m["color"], metallic=m.get("metallic", 0.0), roughness=m["roughness"],
+                    transmission=0.0, ior=1.45)
+    return dict(type="pbr", color=(1.0, 1.0, 1.0), metallic=0.0, roughness=0.5, transmission=1.0, ior=m["ior"])
+
+
+def batch_cpu_baseline(w3, seconds):
+    """Workload 3 in the oracle: the first view's rows, 4 paths per pixel,
+    same materials, depth and Russian roulette; all rays traced per second."""
+    from oracle import refcpu
+    t0 = time.time()
+    ref = refcpu.Scene([dict(kind=g["kind"], vertices=g["vertices"], indices=g.get("indices"),
+                             radii=g.get("radii"), material=_oracle_material(w3["materials"][g["material"]]))
+                        for g in w3["geoms"]])
+    log(f"workload 3 oracle scene built in {time.time() - t0:.1f}s")
+    W, H = w3["widths"][0], w3["heights"][0]
+    cam = refcpu.camera(**w3["cameras"][0])
+    p = refcpu.render_params(W, H, spp=w3["spp"], ambient=1.0, bg=(-1, -1, -1, -1), max_depth=w3["max_depth"],
+                             max_diffuse=w3["max_diffuse"], russian_roulette=w3["rr_depth"])
+    acc = np.zeros((H, W, 4), np.float32)
+    return band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Mrays/s", f"view 0 ({W}x{H}, {w3['spp']} spp)")
+
+
+def umesh_cpu_baseline(w5, seconds):
+    """Workload 5 in the oracle (its own tet BVH and rasterTet macrocells):
+    field samples per second over a band of rows, volume + iso-surface."""
+    from oracle import refcpu
+    t0 = time.time()
+    u = refcpu.UMesh(w5["vertices"], w5["scalars"], w5["tets"])
+    u.set_tf(w5["tf"], (0.0, 1.0), w5["base_density"])
+    log(f"workload 5 oracle mesh built in {time.time() - t0:.1f}s")
+    W, H = w5["size"]
+    cam = refcpu.camera(**w5["camera"])
+    p = refcpu.render_params(W, H, spp=w5["spp"], ambient=1.0, bg=(-1, -1, -1, -1))
+    acc = np.zeros((H, W, 4), np.float32)
+    return band_timer(lambda y0, y1: u.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], y0=y0,
+                                              y1=y1, threads=cpu_threads(), accum=acc)[2], H, seconds,
+                      "Msamples/s", f"the {W}x{H}x{w5['spp']}spp frame")
+
+
+def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
     from barney_amd import scenes
     w1 = scenes.workload1()
     geoms, mats = [], []
@@ -114,12 +186,15 @@ def surface_bench(api, steps=3, warmup=1):
         o = api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
         rays += o["stats"]["raysTraced"]
         ms += o["stats"]["ms"]
-    return {"Mrays_per_s": rays / (ms * 1e-3) / 1e6, "rays_per_frame": rays // steps,
-            "ms_per_frame": ms / steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
-            "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
+    res = {"Mrays_per_s": rays / (ms * 1e-3) / 1e6, "rays_per_frame": rays // steps,
+           "ms_per_frame": ms / steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
+           "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
+    if cpu:
+        res["cpu_baseline"] = surface_cpu_baseline(w1, cpu_seconds)
+    return res
 
 
-def batch_bench(api, steps=2):
+def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     """Workload 3: 10M-triangle heightfield + 200k spheres + 100k cylinders,
     60/30/10 diffuse/glossy/dielectric, depth <= 8 with Russian roulette,
     64 views of uneven size, 4 paths per pixel: megakernel vs wavefront."""
@@ -153,10 +228,12 @@ def batch_bench(api, steps=2):
     res["wavefront_speedup"] = res["wavefront"]["Mrays_per_s"] / res["megakernel"]["Mrays_per_s"]
     res["pixels"] = views.num_pixels
     res["bvh_build_ms"] = info["buildMs"]
+    if cpu:
+        res["cpu_baseline"] = batch_cpu_baseline(w3, cpu_seconds)
     return res
 
 
-def umesh_bench(api, steps=2):
+def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     """Workload 5: 128^3 jittered hex grid split into 5 tets per hex (10.5M
     tets), per-vertex Gaussian-blob scalars; BVH point location with
     barycentric sampling, rasterTet macrocell majorants, delta tracking plus
@@ -185,7 +262,8 @@ def umesh_bench(api, steps=2):
             "Mrays_per_s": st["raysTraced"] / (t * 1e-3) / 1e6, "dda_steps_per_frame": int(st["ddaSteps"]),
             "tets": info["numTets"], "bvh_nodes": info["numNodes"], "bvh_depth": info["bvhDepth"],
             "mc_dims": list(info["mcDims"]), "bvh_build_ms": info["buildMs"], "mc_build_ms": info["mcMs"],
-            "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"])}
+            "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"]),
+            "cpu_baseline": umesh_cpu_baseline(w5, cpu_seconds) if cpu else None}
 
 
 def fb_bench(api, reps=20):
@@ -351,11 +429,11 @@ def main():
                   "rays_per_step": vb["rays"] // vb["steps"], "image_mean": vb["image_mean"]},
     }
     if not args.no_surface:
-        out["extra"]["surface_workload1"] = surface_bench(api)
+        out["extra"]["surface_workload1"] = surface_bench(api, cpu=not args.no_cpu)
     if not args.no_batch:
-        out["extra"]["batch_workload3"] = batch_bench(api)
+        out["extra"]["batch_workload3"] = batch_bench(api, cpu=not args.no_cpu)
     if not args.no_umesh:
-        out["extra"]["umesh_workload5"] = umesh_bench(api)
+        out["extra"]["umesh_workload5"] = umesh_bench(api, cpu=not args.no_cpu)
     if not args.no_fb:
         out["extra"]["readback_3840x2160"] = fb_bench(api)
     if not args.no_ea:

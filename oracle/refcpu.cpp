@@ -1678,6 +1678,36 @@ float edge_len(const P4 &a, const P4 &b) {
   return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// min / max into a cell shared between threads (compare-and-swap on the
+// float's bits); fmin / fmax are order-independent, so the grid does not
+// depend on the thread schedule.
+inline void atomic_fmin(float *p, float v) {
+  uint32_t *u = reinterpret_cast<uint32_t *>(p);
+  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
+  for (;;) {
+    float of;
+    std::memcpy(&of, &old, 4);
+    const float nv = std::fmin(of, v);
+    if (nv == of) return;
+    uint32_t nu;
+    std::memcpy(&nu, &nv, 4);
+    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+  }
+}
+inline void atomic_fmax(float *p, float v) {
+  uint32_t *u = reinterpret_cast<uint32_t *>(p);
+  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
+  for (;;) {
+    float of;
+    std::memcpy(&of, &old, 4);
+    const float nv = std::fmax(of, v);
+    if (nv == of) return;
+    uint32_t nu;
+    std::memcpy(&nu, &nv, 4);
+    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+  }
+}
+
 void raster_box4(ref_volume_s *g, const P4 *q) {
   float lo[4], hi[4];
   for (int k = 0; k < 4; ++k) {
@@ -1703,8 +1733,8 @@ void raster_box4(ref_volume_s *g, const P4 *q) {
     for (int y = a[1]; y <= b[1]; ++y)
       for (int x = a[0]; x <= b[0]; ++x) {
         const size_t id = (size_t)x + (size_t)g->m[0] * ((size_t)y + (size_t)g->m[1] * z);
-        g->rlo[id] = std::fmin(g->rlo[id], lo[3]);
-        g->rhi[id] = std::fmax(g->rhi[id], hi[3]);
+        atomic_fmin(&g->rlo[id], lo[3]);
+        atomic_fmax(&g->rhi[id], hi[3]);
       }
 }
 
@@ -1894,14 +1924,14 @@ ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int num
   g.rlo.assign(nc, kInf);
   g.rhi.assign(nc, -kInf);
   g.maj.assign(nc, 0.f);
-  for (int t = 0; t < numTets; ++t) {
+  parallel_for(numTets, default_threads(0), [&](int t) {
     P4 p[4];
     for (int q = 0; q < 4; ++q) {
       const float *v = &u->v4[4 * (size_t)tets[4 * (size_t)t + q]];
       p[q] = P4{v[0], v[1], v[2], v[3]};
     }
     raster_tet(&g, 5, p[0], p[1], p[2], p[3]);
-  }
+  });
   const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
   g.tf.assign(one, one + 8);
   g.numTF = 2;
