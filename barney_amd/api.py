@@ -206,7 +206,8 @@ def render_surface(world, cam, params, with_ids=False, accum=None):
     st = capi.bnamd_render_stats()
     check(capi.load().bnamd_render_surface(world.handle, C.byref(cam), C.byref(params), acc.addr,
                                            ids.addr if ids else None, None, C.byref(st)))
-    out = {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays, "ms": st.ms},
+    out = {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays, "ms": st.ms,
+                                   "nodeVisits": st.nodeVisits, "primBytes": st.primBytes},
            "pixels": n}
     if ids is not None:
         out["ids"] = ids.numpy()
@@ -245,7 +246,8 @@ def render_batch(world, views, params, mode=BATCH_WAVEFRONT, accum=None):
     check(capi.load().bnamd_render_batch(world.handle, C.byref(views.c), C.byref(params), acc.addr, int(mode), None,
                                          C.byref(st)))
     return {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays,
-                                    "shadowRays": st.shadowRays, "ms": st.ms, "graphReplays": st.graphReplays}}
+                                    "shadowRays": st.shadowRays, "ms": st.ms, "graphReplays": st.graphReplays,
+                                    "nodeVisits": st.nodeVisits, "primBytes": st.primBytes}}
 
 
 def batch_autotune(world, views, params, path=None):

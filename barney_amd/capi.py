@@ -50,7 +50,8 @@ class bnamd_render_params(C.Structure):
 
 
 class bnamd_render_stats(C.Structure):
-    _fields_ = [("raysTraced", C.c_longlong), ("primaryRays", C.c_longlong), ("ms", C.c_float)]
+    _fields_ = [("raysTraced", C.c_longlong), ("primaryRays", C.c_longlong), ("ms", C.c_float),
+                ("nodeVisits", C.c_longlong), ("primBytes", C.c_longlong)]
 
 
 class bnamd_structured_desc(C.Structure):
@@ -85,7 +86,8 @@ class bnamd_views(C.Structure):
 
 class bnamd_batch_stats(C.Structure):
     _fields_ = [("raysTraced", C.c_longlong), ("primaryRays", C.c_longlong), ("shadowRays", C.c_longlong),
-                ("ms", C.c_float), ("graphReplays", C.c_int)]
+                ("ms", C.c_float), ("graphReplays", C.c_int), ("nodeVisits", C.c_longlong),
+                ("primBytes", C.c_longlong)]
 
 
 # name -> (restype, argtypes)

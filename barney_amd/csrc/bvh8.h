@@ -266,6 +266,19 @@ BN_D bool occludes_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org
   return h.flat != 0xffffffffu;
 }
 
+// Work counters of a traversal, for the algorithmic-bytes figure of a
+// launch: node visits (one 128-byte node each) and primitive tests weighted
+// by the bytes a test needs (triangle 36: three vertices; sphere 16: center
+// and radius; cylinder 28: two end points and radius).
+struct TraceCount {
+  uint32_t nodes = 0;
+  uint32_t primBytes = 0;
+  BN_D void prim(const PrimRec *__restrict prims, uint32_t idx) {
+    const uint32_t type = __float_as_uint(prims[idx].c.w) & 0xffu;
+    primBytes += type == PRIM_TRIANGLE ? 36u : (type == PRIM_SPHERE ? 16u : 28u);
+  }
+};
+
 // Stack entry: node group = {childBase, (hitbits | imask << 8)}.
 // The per-lane stack lives in LDS as stack[depth][lanes] so consecutive
 // lanes hit consecutive banks.
@@ -352,7 +365,8 @@ BN_D uint32_t ray_octant(vec3f dir) {
 // reported depth rules out).
 template <bool kAny>
 BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
-                        float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, uint32_t keyBase) {
+                        float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, uint32_t keyBase,
+                        TraceCount *tc = nullptr) {
   const vec3f inv = rcp(dir);
   const uint32_t oct = ray_octant(dir);
   uint32_t grpBase = 0;
@@ -372,10 +386,12 @@ BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restric
         ++sp;
       }
       bvh8_visit(nodes, nodeIdx, org, inv, oct, tmin, best.t, grpBase, grpBits, primBase, primMask);
+      if (tc) ++tc->nodes;
     }
     while (primMask) {
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
+      if (tc) tc->prim(prims, primBase + i);
       if (kAny) {
         if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) return true;
       } else {
@@ -408,6 +424,7 @@ struct BvhWalk {
   uint32_t grpBase, grpBits, primBase, primMask;
   int sp;
   bool occluded;  // kAny: an occluder was found (or the stack overflowed)
+  TraceCount count;  // over every ray this walk has taken (init keeps it)
 
   BN_D void init(vec3f o, vec3f d, float t0, float t1) {
     org = o;
@@ -444,10 +461,12 @@ struct BvhWalk {
         ++sp;
       }
       bvh8_visit(nodes, nodeIdx, org, inv, oct, tmin, best.t, grpBase, grpBits, primBase, primMask);
+      ++count.nodes;
     }
     while (primMask) {
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
+      count.prim(prims, primBase + i);
       if (kAny) {
         if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) {
           occluded = true;
@@ -471,11 +490,11 @@ struct BvhWalk {
 // Closest-hit (or occlusion) traversal of one BVH8 from scratch.
 template <bool kAny>
 BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
-                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx) {
+                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, TraceCount *tc = nullptr) {
   best.t = tmax;
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
-  return traverse_from<kAny>(nodes, prims, org, dir, tmin, tmax, best, ctx, 0u);
+  return traverse_from<kAny>(nodes, prims, org, dir, tmin, tmax, best, ctx, 0u, tc);
 }
 
 BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,

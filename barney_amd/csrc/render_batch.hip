@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
   const bool active = ix < sz.x && iy < sz.y;
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
   unsigned long long nrays = 0, nshadow = 0;
+  TraceCount tc;
   float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
   const CameraDD cam = vd.cams[view];
   if (active) {
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
         const ShadeHit sh = shade_hit<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
         if (shadow.tMax > 0.f) {
           ShadeHit shs;
           HitRec hs;
-          shs.hit = world_traverse<TL, true>(w, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, hs, ctx);
+          shs.hit = world_traverse<TL, true>(w, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, hs, ctx, &tc);
           ++nshadow;
           PathRay dummy;
           add_frag(sum, bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
@@ -212,6 +213,8 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
   }
   wave_add(nrays, counters + 0);
   wave_add(nshadow, counters + 1);
+  wave_add(tc.nodes, counters + 2);
+  wave_add(tc.primBytes, counters + 3);
 }
 
 // ===========================================================================
@@ -238,6 +241,7 @@ struct WfCounters {
   unsigned int bins[32][4];   // sort histogram
   unsigned int cursor[32][4]; // sort scatter cursors
   unsigned long long rays, shadowRays;
+  unsigned long long nodeVisits, primBytes;  // traversal work (TraceCount)
 };
 
 // Counting sort geometry: kSortBlocks workgroups, each owning one contiguous
@@ -336,16 +340,19 @@ __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
   const PathRec *__restrict q = e.q[depth & 1];
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
   unsigned long long nr = 0;
+  TraceCount tc;
   for_each_ray(n, e.chunk, &e.cnt->work[depth], [&](unsigned i) {
     const float4 a = q[i].a, b = q[i].b;
     const vec3f org = {a.x, a.y, a.z};
     const vec3f dir = fix_dir({b.x, b.y, b.z});
     HitRec h;
-    world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx);
+    world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx, &tc);
     e.hits[i] = h;
     ++nr;
   });
   wave_add(nr, &e.cnt->rays);
+  wave_add(tc.nodes, &e.cnt->nodeVisits);
+  wave_add(tc.primBytes, &e.cnt->primBytes);
 }
 
 template <bool TL>
@@ -537,12 +544,13 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
   const unsigned n = e.cnt->shadow[depth];
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
   unsigned long long nr = 0;
+  TraceCount tc;
   for_each_ray(n, e.chunk, &e.cnt->swork[depth], [&](unsigned i) {
     const ShadowRec s = e.sq[i];
     const vec3f org = {s.a.x, s.a.y, s.a.z};
     const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
     HitRec hs;
-    const bool occluded = world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx);
+    const bool occluded = world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx, &tc);
     ++nr;
     if (!occluded) {
       const unsigned pid = __float_as_uint(s.b.w);
@@ -552,6 +560,8 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
     }
   });
   wave_add(nr, &e.cnt->shadowRays);
+  wave_add(tc.nodes, &e.cnt->nodeVisits);
+  wave_add(tc.primBytes, &e.cnt->primBytes);
 }
 
 // Trace (kAny = false: closest hit into e.hits) or shadow (kAny = true:
@@ -561,7 +571,7 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
 // counter and hands them to its lanes one by one, giving a lane its next ray
 // as soon as its walk is over (once at least kRefillLanes lanes wait, or no
 // lane walks), so that a wavefront does not idle behind its longest ray.
-constexpr unsigned kWalkChunk = 256;
+constexpr unsigned kWalkChunk = 128;
 constexpr int kRefillLanes = 16;
 
 template <bool kAny>
@@ -637,6 +647,8 @@ __global__ void __launch_bounds__(256) k_wf_walk(WfDD e, int depth) {
     }
   }
   wave_add(nr, kAny ? &e.cnt->shadowRays : &e.cnt->rays);
+  wave_add(wk.count.nodes, &e.cnt->nodeVisits);
+  wave_add(wk.count.primBytes, &e.cnt->primBytes);
 }
 
 __global__ void k_fill_f4(float4 *p, size_t n) {
@@ -828,7 +840,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
   BN_HIP(hipEventCreate(&e0));
   BN_HIP(hipEventCreate(&e1));
   BN_HIP(hipEventRecord(e0, s));
-  unsigned long long rays = 0, shadows = 0;
+  unsigned long long rays = 0, shadows = 0, nodeVisits = 0, primBytes = 0;
   int replays = 0;
   for (int p0 = 0; p0 < spp; p0 += sppBatch) {
     WfDD e;
@@ -881,6 +893,8 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     BN_HIP(hipStreamSynchronize(s));
     rays += hc.rays;
     shadows += hc.shadowRays;
+    nodeVisits += hc.nodeVisits;
+    primBytes += hc.primBytes;
   }
   BN_HIP(hipEventRecord(e1, s));
   BN_HIP(hipEventSynchronize(e1));
@@ -894,6 +908,8 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     stats->shadowRays = (long long)shadows;
     stats->ms = ms;
     stats->graphReplays = replays;
+    stats->nodeVisits = (long long)nodeVisits;
+    stats->primBytes = (long long)primBytes;
   }
   return ms;
 }
@@ -914,8 +930,8 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
     RenderDD rp = make_render_dd(*params);
     if (mode == BNAMD_BATCH_MEGAKERNEL) {
       BatchState &st = state();
-      st.megaCnt.ensure(2);
-      BN_HIP(hipMemsetAsync(st.megaCnt.ptr, 0, 16, s));
+      st.megaCnt.ensure(4);
+      BN_HIP(hipMemsetAsync(st.megaCnt.ptr, 0, 4 * sizeof(unsigned long long), s));
       hipEvent_t e0, e1;
       BN_HIP(hipEventCreate(&e0));
       BN_HIP(hipEventCreate(&e1));
@@ -927,7 +943,7 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
       BN_HIP(hipEventElapsedTime(&ms, e0, e1));
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
-      unsigned long long h[2];
+      unsigned long long h[4];
       BN_HIP(hipMemcpy(h, st.megaCnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
       if (stats) {
         long long px = 0;
@@ -937,6 +953,8 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
         stats->shadowRays = (long long)h[1];
         stats->ms = ms;
         stats->graphReplays = 0;
+        stats->nodeVisits = (long long)h[2];
+        stats->primBytes = (long long)h[3];
       }
     } else {
       BatchState &st = state();

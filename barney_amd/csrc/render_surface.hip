@@ -65,6 +65,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
   const bool active = ix < rp.width && iy < rp.height;
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
   unsigned long long nrays = 0;
+  TraceCount tc;
   if (active) {
     const int pix = ix + iy * rp.width;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx);
+        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
         const ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) {
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
           const vec3f sdir = fix_dir(shadow.dir);
           ShadeHit shs;
           HitRec hs;
-          shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx);
+          shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
           ++nrays;
           PathRay dummy;
           frag = bounce(rpp, shadow, shs, dummy, gen + 1);
@@ -123,9 +124,18 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
       accum[pix] = o;
     }
   }
-  // one atomic per wavefront for the ray counter
-  for (int off = 32; off > 0; off >>= 1) nrays += __shfl_xor(nrays, off);
-  if (lane == 0) atomicAdd(counters, nrays);
+  // one atomic per wavefront per counter
+  unsigned long long nv = tc.nodes, pb = tc.primBytes;
+  for (int off = 32; off > 0; off >>= 1) {
+    nrays += __shfl_xor(nrays, off);
+    nv += __shfl_xor(nv, off);
+    pb += __shfl_xor(pb, off);
+  }
+  if (lane == 0) {
+    atomicAdd(counters + 0, nrays);
+    atomicAdd(counters + 1, nv);
+    atomicAdd(counters + 2, pb);
+  }
 }
 
 }  // namespace
@@ -147,8 +157,8 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counter(1);
-    BN_HIP(hipMemsetAsync(counter.ptr, 0, sizeof(unsigned long long), s));
+    DevBuf<unsigned long long> counter(3);
+    BN_HIP(hipMemsetAsync(counter.ptr, 0, 3 * sizeof(unsigned long long), s));
     const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
     // xcd_remap hands XCD x the tiles [x*per, (x+1)*per): the grid must hold
     // 8*per workgroups (a multiple of 8) to cover every tile
@@ -173,10 +183,12 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     BN_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned long long n = 0;
-    BN_HIP(hipMemcpy(&n, counter.ptr, sizeof(n), hipMemcpyDeviceToHost));
+    unsigned long long n[3] = {0, 0, 0};
+    BN_HIP(hipMemcpy(n, counter.ptr, sizeof(n), hipMemcpyDeviceToHost));
     if (stats) {
-      stats->raysTraced = (long long)n;
+      stats->raysTraced = (long long)n[0];
+      stats->nodeVisits = (long long)n[1];
+      stats->primBytes = (long long)n[2];
       stats->primaryRays = (long long)params->width * params->height * params->pathsPerPixel;
       stats->ms = ms;
     }

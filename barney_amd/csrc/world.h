@@ -223,8 +223,8 @@ BN_D const GeomDev &hit_geom(const WorldDev &w, uint32_t flat) {
 // from the best hit so far, with tie keys prefixed by the instance index.
 template <bool TL, bool kAny>
 BN_D bool world_traverse(const WorldDev &w, vec3f org, vec3f dir, float tmin, float tmax, HitRec &best,
-                         const TraverseCtx &ctx) {
-  if (!TL) return traverse<kAny>(w.nodes, w.prims, org, dir, tmin, tmax, best, ctx);
+                         const TraverseCtx &ctx, TraceCount *tc = nullptr) {
+  if (!TL) return traverse<kAny>(w.nodes, w.prims, org, dir, tmin, tmax, best, ctx, tc);
   best.t = tmax;
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
@@ -249,6 +249,7 @@ BN_D bool world_traverse(const WorldDev &w, vec3f org, vec3f dir, float tmin, fl
         ++sp;
       }
       bvh8_visit(w.nodes, nodeIdx, org, inv, oct, tmin, best.t, grpBase, grpBits, primBase, primMask);
+      if (tc) ++tc->nodes;
     }
     while (primMask) {
       const uint32_t i = __builtin_ctz(primMask);
@@ -261,9 +262,9 @@ BN_D bool world_traverse(const WorldDev &w, vec3f org, vec3f dir, float tmin, fl
         HitRec tmp;
         tmp.t = tmax;
         tmp.flat = 0xffffffffu;
-        if (traverse_from<true>(I.nodes, I.prims, o, d, tmin, tmax, tmp, sub, 0u)) return true;
+        if (traverse_from<true>(I.nodes, I.prims, o, d, tmin, tmax, tmp, sub, 0u, tc)) return true;
       } else {
-        traverse_from<false>(I.nodes, I.prims, o, d, tmin, tmax, best, sub, inst << w.localBits);
+        traverse_from<false>(I.nodes, I.prims, o, d, tmin, tmax, best, sub, inst << w.localBits, tc);
       }
     }
     if ((grpBits & 0xffu) == 0) {

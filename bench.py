@@ -27,6 +27,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+RAY_IO_BYTES = 48  # per traced ray: 32 B in (org, dir, tmin, tmax) + 16 B hit out
+
+
+def surface_roofline(rays, node_visits, prim_bytes, ms):
+    """Algorithmic bytes of surface rays: 48 B ray I/O + 128 B per BVH8 node
+    visit + the bytes of each primitive test (36 triangle, 16 sphere, 28
+    cylinder), counted by the kernels; as GB/s and fraction of HBM peak."""
+    b = RAY_IO_BYTES * rays + 128 * node_visits + prim_bytes
+    gbps = b / (ms * 1e-3) / 1e9
+    return {"alg_bytes": int(b), "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+            "node_visits_per_ray": round(node_visits / max(rays, 1), 2),
+            "prim_bytes_per_ray": round(prim_bytes / max(rays, 1), 1)}
 METRIC = "Mrays/s + volume Msamples/s on 1 MI355X; adjoint ms/iter vs forward"
 
 
@@ -180,15 +192,18 @@ def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
     acc = api.DeviceArray.empty((H, W, 4), np.float32)
     for _ in range(warmup):
         api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
-    rays = 0
+    rays = nodes = pbytes = 0
     ms = 0.0
     for _ in range(steps):
         o = api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
         rays += o["stats"]["raysTraced"]
+        nodes += o["stats"]["nodeVisits"]
+        pbytes += o["stats"]["primBytes"]
         ms += o["stats"]["ms"]
     res = {"Mrays_per_s": rays / (ms * 1e-3) / 1e6, "rays_per_frame": rays // steps,
            "ms_per_frame": ms / steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
-           "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"]}
+           "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"],
+           "roofline": surface_roofline(rays, nodes, pbytes, ms)}
     if cpu:
         res["cpu_baseline"] = surface_cpu_baseline(w1, cpu_seconds)
     return res
@@ -219,9 +234,11 @@ def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
             o = api.render_batch(world, views, p, mode, accum=acc)
             ms.append(o["stats"]["ms"])
             rays = o["stats"]["raysTraced"]
+            nodes, pbytes = o["stats"]["nodeVisits"], o["stats"]["primBytes"]
         img = acc.numpy()
         res[name] = {"ms_per_frame": float(np.median(ms)), "rays_per_frame": int(rays),
-                     "Mrays_per_s": rays / (np.median(ms) * 1e-3) / 1e6, "image": img}
+                     "Mrays_per_s": rays / (np.median(ms) * 1e-3) / 1e6, "image": img,
+                     "roofline": surface_roofline(rays, nodes, pbytes, float(np.median(ms)))}
         log(f"workload 3 {name}: {np.median(ms):.1f} ms, {rays / 1e6:.1f} M rays")
     a, b = res["megakernel"].pop("image"), res["wavefront"].pop("image")
     res["max_abs_diff_wavefront_vs_megakernel"] = float(np.abs(a - b).max())

@@ -175,6 +175,9 @@ typedef struct {
   long long raysTraced; /* all rays traced: primary, shadow and bounce */
   long long primaryRays;
   float ms;
+  long long nodeVisits; /* BVH8 nodes visited (128 B each) */
+  long long primBytes;  /* primitive tests x bytes a test reads (triangle
+                         * 36, sphere 16, cylinder 28) */
 } bnamd_render_stats;
 
 /* accum: device float4[width*height] (RGBA accumulation, as the accum
@@ -371,6 +374,8 @@ typedef struct {
   long long shadowRays;
   float ms;
   int graphReplays;       /* wavefront: graph launches used */
+  long long nodeVisits;   /* as bnamd_render_stats */
+  long long primBytes;
 } bnamd_batch_stats;
 
 /* params->width/height are ignored (per-view sizes); pathsPerPixel paths
