@@ -337,15 +337,13 @@ BN_D void bvh8_visit(const BNode8 *__restrict nodes, uint32_t nodeIdx, vec3f org
       tn = tn - fabsf(tn) * 2e-6f;
       tf = fminf(tf + fabsf(tf) * 2e-6f, tcap);
       const bool hit = (tn <= tf) && (lx <= hx);
-      if (hit) {
-        const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
-        if (nimask & (1u << c)) {
-          hitInner |= 1u << (c ^ oct);
-        } else {
-          const uint32_t cnt = meta >> 5, off = meta & 31u;
-          hitPrims |= ((1u << cnt) - 1u) << off;
-        }
-      }
+      // bookkeeping with selects, not branches: a hit internal child sets
+      // its group bit, a hit leaf child its primitives' bits
+      const uint32_t meta = (c < 4) ? ((n1.z >> (8 * c)) & 0xffu) : ((n1.w >> (8 * (c - 4))) & 0xffu);
+      const bool inner = (nimask >> c) & 1u;
+      const uint32_t leafBits = ((1u << (meta >> 5)) - 1u) << (meta & 31u);
+      hitInner |= (hit && inner) ? (1u << (c ^ oct)) : 0u;
+      hitPrims |= (hit && !inner) ? leafBits : 0u;
     }
   }
   grpBase = n1.x;

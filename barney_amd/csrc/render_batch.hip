@@ -263,8 +263,9 @@ struct WfDD {
   ShadowRec *sq;
   float4 *accum;
   WfCounters *cnt;
-  int chunk;      // 0: static 64-ray slices per wavefront; else rays per
-                  // dynamic work fetch (one atomic per fetch)
+  int chunk;      // rays per dynamic work fetch (one atomic per fetch);
+                  // 0: kWalkChunk in k_wf_walk, static 64-ray slices in the
+                  // two-level passes
 };
 
 __device__ __forceinline__ unsigned sort_segment(unsigned n) {
@@ -567,8 +568,8 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
 // Trace (kAny = false: closest hit into e.hits) or shadow (kAny = true:
 // unoccluded contributions into accum) pass over a flattened world with
 // per-lane ray replacement: a resident grid of wavefronts walks the queue;
-// each wavefront takes kWalkChunk rays at a time from the queue's work
-// counter and hands them to its lanes one by one, giving a lane its next ray
+// each wavefront takes e.chunk rays (kWalkChunk when 0) at a time from the
+// queue's work counter and hands them to its lanes one by one, giving a lane its next ray
 // as soon as its walk is over (once at least kRefillLanes lanes wait, or no
 // lane walks), so that a wavefront does not idle behind its longest ray.
 constexpr unsigned kWalkChunk = 128;
@@ -583,6 +584,7 @@ __global__ void __launch_bounds__(256) k_wf_walk(WfDD e, int depth) {
   const TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned chunk = e.chunk > 0 ? (unsigned)e.chunk : kWalkChunk;
   BvhWalk<kAny> wk;
   unsigned rid = ~0u;             // this lane's ray (~0u: none)
   bool dry = false;               // the queue is used up (wavefront-uniform)
@@ -598,14 +600,14 @@ __global__ void __launch_bounds__(256) k_wf_walk(WfDD e, int depth) {
       while (given < nNeed) {
         if (cNext == cEnd) {
           unsigned base = 0;
-          if (lane == 0) base = atomicAdd(work, kWalkChunk);
+          if (lane == 0) base = atomicAdd(work, chunk);
           base = __shfl(base, 0);
           if (base >= n) {
             dry = true;
             break;
           }
           cNext = base;
-          cEnd = min(base + kWalkChunk, n);
+          cEnd = min(base + chunk, n);
         }
         const int take = min(nNeed - given, (int)(cEnd - cNext));
         if (rid == ~0u && rank >= given && rank < given + take) mine = cNext + (unsigned)(rank - given);
@@ -977,17 +979,23 @@ extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views 
     for (int i = 0; i < views->numViews; ++i) px += (long long)views->widths[i] * views->heights[i];
     DevBuf<float4> accum((size_t)px);
     const int blocks[] = {2, 3, 4, 0};
-    const int chunks[] = {0, 256, 1024};
+    // chunk: rays per work fetch (flattened worlds: the walk passes' fetch,
+    // 0 = 128; two-level worlds: 0 = static 64-ray slices)
+    const int chunksFlat[] = {0, 64, 256};
+    const int chunksTwoLevel[] = {0, 256, 1024};
+    const int *chunks = w.twoLevel ? chunksTwoLevel : chunksFlat;
     TuneChoice best;
     float bestMs = 1e30f;
     for (int b : blocks)
-      for (int c : chunks) {
+      for (int ci = 0; ci < 3; ++ci) {
         TuneChoice t;
         t.traceBlocksPerCU = b;
-        t.chunk = c;
+        t.chunk = chunks[ci];
         t.sppBatch = params->pathsPerPixel;
-        // one spp per candidate keeps tuning short
+        // one spp per candidate keeps tuning short; the better of two runs
+        // keeps one noisy run from deciding
         float ms = render_wavefront(w, views, rp, 1, accum.ptr, 0, nullptr, t, false);
+        ms = std::min(ms, render_wavefront(w, views, rp, 1, accum.ptr, 0, nullptr, t, false));
         if (ms < bestMs) {
           bestMs = ms;
           best = t;

@@ -226,7 +226,11 @@ def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
                           max_diffuse=w3["max_diffuse"], russian_roulette=w3["rr_depth"])
     acc = api.DeviceArray.empty((views.num_pixels, 4), np.float32)
     res = {}
-    api.batch_autotune(world, views, p)
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        api.batch_autotune(world, views, p, os.path.join(td, "tune.json"))
+        with open(os.path.join(td, "tune.json")) as f:
+            res["autotune"] = json.load(f)
     for name, mode in (("megakernel", api.BATCH_MEGAKERNEL), ("wavefront", api.BATCH_WAVEFRONT)):
         api.render_batch(world, views, p, mode, accum=acc)  # warm-up (graph capture)
         ms, rays = [], 0
