@@ -64,21 +64,26 @@ enum VolStage : int {
   VS_MARCH = 3,  // the walk is running (or has ended: see DeltaTrack::mode)
 };
 
-// Paths a wavefront owns at a time (one LDS result slot each).
-constexpr int kPoolPaths = 256;
+// Paths a workgroup owns at a time (one LDS result slot each).
+constexpr int kPoolPaths = 2048;
+// Weights of the waiting-lane counts when a wavefront picks its next kind of
+// work: cheap step, field sample, camera ray, walk set-up, collision
+// shading, escape shading.
+constexpr int kPhaseWeight[6] = {1, 1, 1, 1, 1, 1};
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long long *dst) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
 }
 
-// Each wavefront owns PW pixels of a workgroup tile and the PW * spp paths
-// of those pixels as a pool (in units of at most kPoolPaths paths when spp
-// is large): a lane takes the next path of the pool whenever its previous
-// one has ended, so lanes whose rays leave the volume at once keep working
-// while neighbours walk long macrocell paths. A path's sum (rgb, alpha) goes
-// to an LDS slot; when the pool is used up, each pixel adds its paths in path
-// order, as barney accumulates them.
+// Each workgroup owns PB pixels (a tile) and the PB * spp paths of those
+// pixels as a pool (in units of at most kPoolPaths paths when spp is large):
+// its four wavefronts take consecutive paths from the pool through an LDS
+// counter whenever lanes' previous paths have ended, so lanes whose rays leave
+// the volume at once keep working while others walk long macrocell paths,
+// and no wavefront idles while its neighbours still have paths. A path's sum
+// (rgb, alpha) goes to an LDS slot; when the pool is used up, each pixel adds
+// its paths in path order, as barney accumulates them.
 //
 // A path is a sequence of small kinds of work: camera ray, walk set-up
 // (DeltaTrack::init), cheap walk steps (macrocell DDA step or collision
@@ -94,32 +99,33 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long
 // (<= 168 VGPRs) measured 8.8 ms per 1920x1080x16spp frame against 12.4 ms
 // at the two that the unconstrained allocation (178 VGPRs) leaves; the cap
 // costs a few dwords of spill in the shading code.
-template <int PW>
+template <int PB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_render_volume(VolumeDD v, CameraDD cam, RenderDD rp, int spp,
                                                        float4 *__restrict accum,
                                                        unsigned long long *__restrict counters) {
-  constexpr int PB = 4 * PW;  // pixels per workgroup
   constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
-  constexpr int kUnit = kPoolPaths / PW;  // paths per pixel in one pool
-  __shared__ float4 pathSum[4][kPoolPaths];
+  static_assert(PB >= 1 && PB <= kPoolPaths && PB <= 1024, "a pool holds at least one path per pixel");
+  constexpr int kUnit = kPoolPaths / PB;  // paths per pixel in one pool
+  __shared__ float4 slots[kPoolPaths];
+  __shared__ int poolNext;
   const int tilesX = (rp.width + TW - 1) / TW, tilesY = (rp.height + TH - 1) / TH;
   const int tile = xcd_remap_v(blockIdx.x, tilesX * tilesY);
   if (tile < 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int tx0 = (tile % tilesX) * TW, ty0 = (tile / tilesX) * TH;
-  float4 *const slots = pathSum[wave];
   const unsigned long long lanesBelow = (1ull << lane) - 1ull;
   VolCounters cnt{0, 0};
   unsigned nrays = 0;
   const auto sampler = [&](vec3f P) { return sample_and_map(v, P); };
   const uint64_t geomHash = bn_hash3(0u, 0u, 0u);
-  float4 pixSum = make_float4(0.f, 0.f, 0.f, 0.f);  // lane j's pixel, across pools (PW == 1)
+  float4 pixSum = make_float4(0.f, 0.f, 0.f, 0.f);  // thread j's pixel, across pools (PB <= 256)
 
   for (int s0 = 0; s0 < spp; s0 += kUnit) {
-    const int nPaths = PW * min(kUnit, spp - s0);
-    int next = 0;                 // the pool's next path (wavefront-uniform)
-    int k = 0;                    // this lane's path: pixel k % PW, sample s0 + k / PW
+    const int nPaths = PB * min(kUnit, spp - s0);
+    if (threadIdx.x == 0) poolNext = 0;
+    __syncthreads();
+    int k = 0;                    // this lane's path: pixel k % PB, sample s0 + k / PB
     int stage = VS_GEN;
     bool inShadow = false;        // the walk holds the path's shadow ray
     int gen = 0;
@@ -143,10 +149,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (__ballot(stage != VS_DEAD) == 0ull) break;
       // the kind of work most lanes wait for (counts from ballots are
       // wavefront-uniform, so the branch below is too)
-      int best = 0, most = (int)__popcll(__ballot(wCheap));
-      const int waiting[5] = {(int)__popcll(__ballot(wSample)), (int)__popcll(genMask),
-                              (int)__popcll(__ballot(wInit)), (int)__popcll(__ballot(wHit)),
-                              (int)__popcll(__ballot(wMiss))};
+      int best = 0, most = (int)__popcll(__ballot(wCheap)) * kPhaseWeight[0];
+      const int waiting[5] = {(int)__popcll(__ballot(wSample)) * kPhaseWeight[1],
+                              (int)__popcll(genMask) * kPhaseWeight[2],
+                              (int)__popcll(__ballot(wInit)) * kPhaseWeight[3],
+                              (int)__popcll(__ballot(wHit)) * kPhaseWeight[4],
+                              (int)__popcll(__ballot(wMiss)) * kPhaseWeight[5]};
 #pragma unroll
       for (int w = 0; w < 5; ++w)
         if (waiting[w] > most) {
@@ -159,16 +167,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       } else if (best == 1) {
         if (wSample) tr.sample(sampler, cnt);
       } else if (best == 2) {
+        // the workgroup's pool hands out consecutive paths to this wavefront
+        const int leader = __ffsll((long long)genMask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&poolNext, (int)__popcll(genMask));
+        base = __shfl(base, leader);
         if (wGen) {  // take the next path of the pool; generateRays for it
-          k = next + (int)__popcll(genMask & lanesBelow);
-          const int j = k % PW, q = wave * PW + j;
-          const int ix = tx0 + q % TW, iy = ty0 + q / TW;
+          k = base + (int)__popcll(genMask & lanesBelow);
+          const int j = k % PB;
+          const int ix = tx0 + j % TW, iy = ty0 + j / TW;
           if (k >= nPaths) {
             stage = VS_DEAD;
           } else if (ix >= rp.width || iy >= rp.height) {
             slots[k] = make_float4(0.f, 0.f, 0.f, 0.f);  // outside the frame: take another path
           } else {
-            rpp.accumID = rp.accumID + s0 + k / PW;
+            rpp.accumID = rp.accumID + s0 + k / PB;
             ray = camera_ray(cam, rpp, ix, iy);
             sum = vec3f(0.f);
             alpha = 0.f;
@@ -180,7 +193,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             stage = VS_INIT;
           }
         }
-        next += (int)__popcll(genMask);
       } else if (best == 3) {
         if (wInit) {  // a shadow ray's seed waits in tr.rng; a path ray's is ray.rng
           RNGSeed seed;
@@ -236,8 +248,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             sh.hit = false;
             // camera_ray's background, recomputed here rather than carried
             // through the walk
-            const int q = wave * PW + k % PW;
-            miss_color(rpp, ty0 + q / TW, ray.missColor, ray.missAlpha);
+            miss_color(rpp, ty0 + (k % PB) / TW, ray.missColor, ray.missAlpha);
             if (gen == 0) alpha = ray.missAlpha;
             PathRay sray;
             frag = bounce(rpp, ray, sh, sray, gen);
@@ -262,23 +273,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         stage = VS_GEN;
       }
     }
-    // every lane's slot writes are done: make them visible across the
-    // wavefront, then each pixel adds its paths in path order
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const int ns = nPaths / PW;
-    for (int j = lane; j < PW; j += 64) {
+    // the workgroup's paths are done: each pixel adds its paths in path order
+    __syncthreads();
+    const int ns = nPaths / PB;
+    for (int j = threadIdx.x; j < PB; j += 256) {
       float4 a = (s0 == 0) ? make_float4(0.f, 0.f, 0.f, 0.f) : pixSum;
       for (int si = 0; si < ns; ++si) {
-        const float4 r = slots[si * PW + j];
+        const float4 r = slots[si * PB + j];
         a.x += r.x;
         a.y += r.y;
         a.z += r.z;
         a.w += r.w;
       }
       pixSum = a;
-      const int q = wave * PW + j;
-      const int ix = tx0 + q % TW, iy = ty0 + q / TW;
+      const int ix = tx0 + j % TW, iy = ty0 + j / TW;
       if (s0 + kUnit >= spp && ix < rp.width && iy < rp.height) {
         const int pix = ix + iy * rp.width;
         if (rp.accumID == 0) {
@@ -290,40 +298,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();  // slots are reused by the next pool
+    __syncthreads();  // slots and the pool counter are reused by the next pool
   }
   wave_add_u64(cnt.samples, counters + 0);
   wave_add_u64(cnt.ddaSteps, counters + 1);
   wave_add_u64((unsigned long long)nrays, counters + 2);
 }
 
-template <int PW>
+template <int PB>
 void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                           unsigned long long *counters, hipStream_t s) {
-  constexpr int PB = 4 * PW;
   constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   // xcd_remap_v hands XCD x the tiles [x*per, (x+1)*per): the grid must
   // hold 8*per workgroups (a multiple of 8) to cover every tile
   const int grid = ((tiles + 7) / 8) * 8;
-  k_render_volume<PW><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
+  k_render_volume<PB><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
 }
 
-// Pixels per wavefront: as many as keep the pool at kPoolPaths paths (the
-// pixels' paths all in one pool), at least one.
+// Pixels per workgroup: the largest power of two up to 1024 (a 32x32 tile)
+// whose paths fit the pool (all of a pixel's paths in one pool), at least one.
 void render_volume_dispatch(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                             unsigned long long *counters, hipStream_t s) {
-  if (spp <= 1) launch_render_volume<256>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 2) launch_render_volume<128>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 4) launch_render_volume<64>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 8) launch_render_volume<32>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 16) launch_render_volume<16>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 32) launch_render_volume<8>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 64) launch_render_volume<4>(v, cam, rp, spp, accum, counters, s);
-  else if (spp <= 128) launch_render_volume<2>(v, cam, rp, spp, accum, counters, s);
-  else launch_render_volume<1>(v, cam, rp, spp, accum, counters, s);
+  int pb = 1024;
+  while (pb > 1 && (long long)pb * spp > kPoolPaths) pb >>= 1;
+  switch (pb) {
+    case 1024: launch_render_volume<1024>(v, cam, rp, spp, accum, counters, s); break;
+    case 512: launch_render_volume<512>(v, cam, rp, spp, accum, counters, s); break;
+    case 256: launch_render_volume<256>(v, cam, rp, spp, accum, counters, s); break;
+    case 128: launch_render_volume<128>(v, cam, rp, spp, accum, counters, s); break;
+    case 64: launch_render_volume<64>(v, cam, rp, spp, accum, counters, s); break;
+    case 32: launch_render_volume<32>(v, cam, rp, spp, accum, counters, s); break;
+    case 16: launch_render_volume<16>(v, cam, rp, spp, accum, counters, s); break;
+    case 8: launch_render_volume<8>(v, cam, rp, spp, accum, counters, s); break;
+    case 4: launch_render_volume<4>(v, cam, rp, spp, accum, counters, s); break;
+    case 2: launch_render_volume<2>(v, cam, rp, spp, accum, counters, s); break;
+    default: launch_render_volume<1>(v, cam, rp, spp, accum, counters, s); break;
+  }
 }
 
 __global__ void k_volume_rays(VolumeDD v, const bnamd_ray *__restrict rays, const uint64_t *__restrict seeds,
