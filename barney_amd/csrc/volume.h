@@ -180,6 +180,7 @@ struct DeltaTrack {
   float next_cell_begin;
   float majorant, nextMajorant;
   float t, upper;         // Woodcock position and end of the current cell
+  float pendingR;         // the draw of a collision distance awaiting sample()
   int mode;
   bool hit;
   vec3f rgb;
@@ -251,19 +252,32 @@ struct DeltaTrack {
     mode = ENTER;
   }
 
-  // One cheap unit of work: a collision distance drawn in the current cell
-  // (a point inside the cell leaves the walk in SAMPLE, waiting for the
-  // field), or a macrocell step of the DDA (leaving the cell, then entering
-  // the next one, with its majorant fetched one cell ahead).
-  // kExactLog: draw with logf instead of barney's fastLog.
+  // One cheap unit of work: a collision distance drawn in the current cell,
+  // or a macrocell step of the DDA (leaving the cell, then entering the next
+  // one, with its majorant fetched one cell ahead). The distance's exact
+  // value (barney's fastLog and correctly rounded divisions) only matters
+  // when the collision falls inside the cell: a draw that certainly leaves
+  // the cell is decided with hardware reciprocals and a margin of 1e-5 of
+  // the distance plus 1e-6 of t (the approximation's error stays below an
+  // eighth of that margin), and t, which the next cell does not use, is not
+  // computed. Every other draw waits in SAMPLE, where sample() finishes it
+  // exactly. The RNG draws keep their order.
   template <bool kExactLog, class V>
   BN_HD void advance(const V &v, VolCounters &cnt) {
 #pragma clang fp contract(off)
     if (mode == WOODCOCK) {
       const float r = rng();
-      const float dt = kExactLog ? -logf(1.f - r) / majorant : -fast_log(1.f - r) / majorant;
-      t += dt;
-      if (!(t >= upper)) {
+      bool leaves = false;
+      if (!kExactLog) {
+        const float f = 1.f - r;
+        const float q = (f - 1.f) * __builtin_amdgcn_rcpf(f + 1.f);
+        const float q2 = q * q;
+        const float dtA = -2.f * (q * (1.f + q2 * (1.f / 3.f + q2 * (1.f / 5.f)))) * __builtin_amdgcn_rcpf(majorant);
+        const float tA = t + dtA;
+        leaves = tA - (1e-5f * dtA + 1e-6f * fabsf(tA)) >= upper;
+      }
+      if (!leaves) {
+        pendingR = r;
         mode = SAMPLE;
         return;
       }
@@ -309,11 +323,19 @@ struct DeltaTrack {
     }
   }
 
-  // The pending collision (mode SAMPLE): S maps P to the TF-mapped sample;
+  // The pending collision (mode SAMPLE): the exact distance of the draw
+  // advance() left pending (Woodcock::sampleRange, Volume.h:188-223); past
+  // the cell's end the walk moves on, else S maps P to the TF-mapped sample,
   // accepted with probability sample.w / majorant.
-  template <class S>
+  template <bool kExactLog, class S>
   BN_HD void sample(const S &sampler, VolCounters &cnt) {
 #pragma clang fp contract(off)
+    const float dt = kExactLog ? -logf(1.f - pendingR) / majorant : -fast_log(1.f - pendingR) / majorant;
+    t += dt;
+    if (t >= upper) {
+      mode = ADVANCE;
+      return;
+    }
     const vec3f P = org + t * dir;
     const VolSample s = sampler(P);
     cnt.samples++;
@@ -352,11 +374,13 @@ BN_HD VolHit volume_intersect_g(const V &v, const S &sample, vec3f org, vec3f di
     // then sample together) or ends
     if (exactLog) {
       while (tr.mode != DeltaTrack::DONE && tr.mode != DeltaTrack::SAMPLE) tr.advance<true>(v, cnt);
+      if (tr.done()) break;
+      tr.sample<true>(sample, cnt);
     } else {
       while (tr.mode != DeltaTrack::DONE && tr.mode != DeltaTrack::SAMPLE) tr.advance<false>(v, cnt);
+      if (tr.done()) break;
+      tr.sample<false>(sample, cnt);
     }
-    if (tr.done()) break;
-    tr.sample(sample, cnt);
   }
   return tr.result();
 }

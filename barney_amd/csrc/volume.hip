@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (best == 0) {
         if (wCheap) tr.advance<false>(v, cnt);
       } else if (best == 1) {
-        if (wSample) tr.sample(sampler, cnt);
+        if (wSample) tr.sample<false>(sampler, cnt);
       } else if (best == 2) {
         // the workgroup's pool hands out consecutive paths to this wavefront
         const int leader = __ffsll((long long)genMask) - 1;
