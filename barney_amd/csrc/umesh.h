@@ -18,6 +18,7 @@ namespace bnamd {
 struct UMeshDD {
   const BNode8 *nodes;
   const float4 *tets;
+  uint32_t numTets;
   int depth;
   vec3f worldLo, worldHi;  // bounds of the vertices
   const float *majorants;
@@ -51,13 +52,28 @@ BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
 // against the 16-bit planes, which carry one quantum of slack on each side,
 // so the test is conservative.
 // 'hint' (a leaf position or ~0u) is tested first and updated to the tet
-// found: consecutive samples along a ray mostly fall into the same tet. The
-// value differs from the traversal-order answer only for points on a face
-// shared by two tets, where both interpolate the same plane.
+// found: consecutive samples along a ray mostly fall into the same tet, and
+// otherwise often into one stored a few records away (tets are in BVH leaf
+// order, i.e. Morton order of their boxes), so kLocateWindow records on
+// either side are tested before a query from the root. The value differs
+// from the traversal-order answer only for points on a face shared by two
+// tets, where both interpolate the same plane.
+constexpr int kLocateWindow = 4;
 BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
   if (hint != ~0u) {
     float val;
     if (tet_scalar(u.tets + 4ull * hint, P, val)) return val;
+    for (int d = 1; d <= kLocateWindow; ++d) {
+      const uint32_t a = hint + (uint32_t)d, b = hint - (uint32_t)d;
+      if (a < u.numTets && tet_scalar(u.tets + 4ull * a, P, val)) {
+        hint = a;
+        return val;
+      }
+      if (hint >= (uint32_t)d && tet_scalar(u.tets + 4ull * b, P, val)) {
+        hint = b;
+        return val;
+      }
+    }
   }
   uint32_t grpBase = 0;
   uint32_t grpBits = 1u | (1u << 8);  // root = internal slot 0

@@ -190,7 +190,7 @@ struct UMeshOpts {
 // Closest of the volume (geometry 0) and iso-surface (geometry 1) hits for
 // one ray; each intersection program advances the ray's seed once.
 template <bool kVol, bool kIso>
-__device__ ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
+BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
                                 RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt, uint32_t &hint) {
   ShadeHit sh;
   sh.hit = false;
@@ -416,6 +416,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.mcMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t1).count();
       m.dd.nodes = m.w.nodes.ptr;
       m.dd.tets = m.tets.ptr;
+      m.dd.numTets = (uint32_t)desc->numTets;
       m.dd.depth = m.w.depth;
       m.dd.ranges = m.ranges.ptr;
       m.dd.majorants = m.majorants.ptr;
