@@ -84,10 +84,11 @@ def cpu_threads():
     return min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
 
 
-def band_timer(render_rows, H, seconds, unit, what):
+def band_timer(render_rows, H, seconds, unit, what, keep_rows=False):
     """Times the oracle over a band of rows around the frame's middle, sized
     from a one-row calibration to take about 'seconds'. render_rows(y0, y1)
-    returns the count of work units (rays or samples) it did."""
+    returns the count of work units (rays or samples) it did. keep_rows adds
+    the band's row range as "_rows" (for the caller to pop)."""
     mid = H // 2
     t0 = time.perf_counter()
     render_rows(mid, mid + 1)
@@ -102,9 +103,12 @@ def band_timer(render_rows, H, seconds, unit, what):
         dt = time.perf_counter() - t0
         if dt >= 0.5 * seconds or reps >= 1000:
             break
-    return {"value": units / dt / 1e6, "unit": unit, "cores": cpu_threads(), "kind": "port",
-            "sample": f"rows {y0}..{y0 + rows - 1} of {what}, {reps}x ({units} {unit.split('/')[0][1:]}, "
-                      f"{dt:.1f}s)"}
+    res = {"value": units / dt / 1e6, "unit": unit, "cores": cpu_threads(), "kind": "port",
+           "sample": f"rows {y0}..{y0 + rows - 1} of {what}, {reps}x ({units} {unit.split('/')[0][1:]}, "
+                     f"{dt:.1f}s)"}
+    if keep_rows:
+        res["_rows"] = (y0, y0 + rows)
+    return res
 
 
 def volume_cpu_baseline(w2, seconds):
@@ -145,7 +149,8 @@ def _oracle_material(m):
 
 def batch_cpu_baseline(w3, seconds):
     """Workload 3 in the oracle: the first view's rows, 4 paths per pixel,
-    same materials, depth and Russian roulette; all rays traced per second."""
+    same materials, depth and Russian roulette; all rays traced per second.
+    Returns the baseline and the oracle's image of those rows."""
     from oracle import refcpu
     t0 = time.time()
     ref = refcpu.Scene([dict(kind=g["kind"], vertices=g["vertices"], indices=g.get("indices"),
@@ -157,8 +162,22 @@ def batch_cpu_baseline(w3, seconds):
     p = refcpu.render_params(W, H, spp=w3["spp"], ambient=1.0, bg=(-1, -1, -1, -1), max_depth=w3["max_depth"],
                              max_diffuse=w3["max_diffuse"], russian_roulette=w3["rr_depth"])
     acc = np.zeros((H, W, 4), np.float32)
-    return band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
-                      "Mrays/s", f"view 0 ({W}x{H}, {w3['spp']} spp)")
+    base = band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Mrays/s", f"view 0 ({W}x{H}, {w3['spp']} spp)", keep_rows=True)
+    y0, y1 = base["_rows"]
+    return base, acc[y0:y1]
+
+
+def image_vs_oracle(g, r):
+    """Error of a band of a GPU image against the oracle's (RGB, the
+    accumulated 4-path sums): RMSE relative to the oracle's mean, and the
+    relative difference of the band means (the converged error: per-path RNG
+    streams re-seed from hit distances, so a last-ulp libm difference changes
+    the rest of a path, not its distribution)."""
+    g = g[..., :3].astype(np.float64)
+    r = r[..., :3].astype(np.float64)
+    m = r.mean()
+    return {"rel_rmse": float(np.sqrt(((g - r) ** 2).mean()) / m), "mean_rel_diff": float((g.mean() - m) / m)}
 
 
 def umesh_cpu_baseline(w5, seconds):
@@ -240,17 +259,23 @@ def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
             rays = o["stats"]["raysTraced"]
             nodes, pbytes = o["stats"]["nodeVisits"], o["stats"]["primBytes"]
         img = acc.numpy()
-        res[name] = {"ms_per_frame": float(np.median(ms)), "rays_per_frame": int(rays),
+        res[name] = {"view0": views.image(img, 0), "ms_per_frame": float(np.median(ms)), "rays_per_frame": int(rays),
                      "Mrays_per_s": rays / (np.median(ms) * 1e-3) / 1e6, "image": img,
                      "roofline": surface_roofline(rays, nodes, pbytes, float(np.median(ms)))}
         log(f"workload 3 {name}: {np.median(ms):.1f} ms, {rays / 1e6:.1f} M rays")
+    view0 = {k: res[k].pop("view0") for k in ("megakernel", "wavefront")}
     a, b = res["megakernel"].pop("image"), res["wavefront"].pop("image")
     res["max_abs_diff_wavefront_vs_megakernel"] = float(np.abs(a - b).max())
     res["wavefront_speedup"] = res["wavefront"]["Mrays_per_s"] / res["megakernel"]["Mrays_per_s"]
     res["pixels"] = views.num_pixels
     res["bvh_build_ms"] = info["buildMs"]
     if cpu:
-        res["cpu_baseline"] = batch_cpu_baseline(w3, cpu_seconds)
+        base, oimg = batch_cpu_baseline(w3, cpu_seconds)
+        y0, y1 = base.pop("_rows")
+        res["cpu_baseline"] = base
+        res["vs_oracle"] = {"rows": [y0, y1 - 1], **{k: image_vs_oracle(view0[k][y0:y1], oimg)
+                                                     for k in ("megakernel", "wavefront")}}
+        log(f"workload 3 vs oracle (view 0 rows {y0}..{y1 - 1}): {res['vs_oracle']}")
     return res
 
 
