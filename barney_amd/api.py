@@ -240,14 +240,21 @@ class Views:
         return accum_np[self.offsets[v]:self.offsets[v + 1]].reshape(h, w, 4)
 
 
-def render_batch(world, views, params, mode=BATCH_WAVEFRONT, accum=None):
+def render_batch(world, views, params, mode=BATCH_WAVEFRONT, accum=None, with_ids=False, ids=None):
+    """ids: a DeviceArray of int32 (num_pixels, 4) to write the ID channels
+    into (with_ids=True allocates one); out["ids"] is its host copy."""
     acc = accum if accum is not None else DeviceArray.empty((views.num_pixels, 4), np.float32)
+    if ids is None and with_ids:
+        ids = DeviceArray.empty((views.num_pixels, 4), np.int32)
     st = capi.bnamd_batch_stats()
-    check(capi.load().bnamd_render_batch(world.handle, C.byref(views.c), C.byref(params), acc.addr, int(mode), None,
-                                         C.byref(st)))
-    return {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays,
-                                    "shadowRays": st.shadowRays, "ms": st.ms, "graphReplays": st.graphReplays,
-                                    "nodeVisits": st.nodeVisits, "primBytes": st.primBytes}}
+    check(capi.load().bnamd_render_batch(world.handle, C.byref(views.c), C.byref(params), acc.addr,
+                                         ids.addr if ids is not None else None, int(mode), None, C.byref(st)))
+    out = {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays,
+                                   "shadowRays": st.shadowRays, "ms": st.ms, "graphReplays": st.graphReplays,
+                                   "nodeVisits": st.nodeVisits, "primBytes": st.primBytes}}
+    if ids is not None:
+        out["ids"] = ids.numpy()
+    return out
 
 
 def batch_autotune(world, views, params, path=None):

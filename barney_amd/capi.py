@@ -140,7 +140,7 @@ SIGNATURES = {
                                      C.POINTER(bnamd_umesh_render_opts), C.c_void_p, C.c_void_p,
                                      C.POINTER(bnamd_volume_stats)]),
     "bnamd_render_batch": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
-                                     C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
+                                     C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
     "bnamd_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
                                        C.c_char_p]),
     "bnamd_fb_tile_descs": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int),

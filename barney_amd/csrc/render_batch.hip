@@ -134,7 +134,7 @@ __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long lon
 // ===========================================================================
 template <int L, bool TL>
 __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, RenderDD rp, int spp,
-                                                    float4 *__restrict accum,
+                                                    float4 *__restrict accum, int4 *__restrict ids,
                                                     unsigned long long *__restrict counters) {
   extern __shared__ uint2 lds_stack[];
   constexpr int PW = 64 / L, PB = 4 * PW;
@@ -176,7 +176,11 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
         world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
         const ShadeHit sh = shade_hit<TL>(w, ray.org, dir, ray.tMax, h);
-        if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
+        if (gen == 0) {
+          alpha = sh.hit ? 1.f : ray.missAlpha;
+          if (ids && rpp.accumID == 0)
+            ids[vd.pixOffset[view] + ix + iy * sz.x] = hit_ids<TL>(w, ray.org, dir, ray.tMax, h);
+        }
         PathRay shadow;
         add_frag(sum, bounce(rpp, ray, sh, shadow, gen), clampMax);
         if (shadow.tMax > 0.f) {
@@ -262,6 +266,7 @@ struct WfDD {
   unsigned int *blockHist;  // [kSortBlocks][4] key counts, then offsets
   ShadowRec *sq;
   float4 *accum;
+  int4 *ids;      // ID buffer or null
   WfCounters *cnt;
   int chunk;      // rays per dynamic work fetch (one atomic per fetch);
                   // 0: kWalkChunk in k_wf_walk, static 64-ray slices in the
@@ -511,6 +516,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
       if (depth == 0) {
         const float alpha = sh.hit ? 1.f : ray.missAlpha;
         if (alpha > 0.f) atomicAdd(((float *)(e.accum + pix)) + 3, alpha);
+        if (e.ids && rpp.accumID == 0) e.ids[pix] = hit_ids<TL>(e.w, ray.org, dir, ray.tMax, h);
       }
       const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
       accum_add(e.accum, pix, frag, clampMax);
@@ -764,7 +770,7 @@ ViewsDD upload_views(BatchState &st, const HostViews &h, hipStream_t s) {
 
 template <int L>
 void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, int spp, float4 *accum,
-                 unsigned long long *cnt, hipStream_t s) {
+                 int4 *ids, unsigned long long *cnt, hipStream_t s) {
   constexpr int PB = 4 * (64 / L);
   constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
@@ -775,18 +781,18 @@ void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
   const int grid = ((tiles + 7) / 8) * 8;
   const size_t lds = trace_lds_bytes(w.depth, 256);
   if (w.twoLevel)
-    k_batch_mega<L, true><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
+    k_batch_mega<L, true><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt);
   else
-    k_batch_mega<L, false><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, cnt);
+    k_batch_mega<L, false><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt);
   BN_HIP(hipGetLastError());
 }
 
 void render_mega(const World &w, const bnamd_views *views, const RenderDD &rp, int spp, float4 *accum,
-                 unsigned long long *cnt, hipStream_t s) {
-  if (spp >= 8) launch_mega<8>(w, views, rp, spp, accum, cnt, s);
-  else if (spp >= 4) launch_mega<4>(w, views, rp, spp, accum, cnt, s);
-  else if (spp >= 2) launch_mega<2>(w, views, rp, spp, accum, cnt, s);
-  else launch_mega<1>(w, views, rp, spp, accum, cnt, s);
+                 int4 *ids, unsigned long long *cnt, hipStream_t s) {
+  if (spp >= 8) launch_mega<8>(w, views, rp, spp, accum, ids, cnt, s);
+  else if (spp >= 4) launch_mega<4>(w, views, rp, spp, accum, ids, cnt, s);
+  else if (spp >= 2) launch_mega<2>(w, views, rp, spp, accum, ids, cnt, s);
+  else launch_mega<1>(w, views, rp, spp, accum, ids, cnt, s);
 }
 
 // Records the kernels of one batch (generate + maxDepth+1 bounces) on s.
@@ -814,7 +820,8 @@ void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds,
 }
 
 float render_wavefront(const World &w, const bnamd_views *views, const RenderDD &rp0, int spp, float4 *accum,
-                       hipStream_t s, bnamd_batch_stats *stats, const TuneChoice &tune, bool useGraph) {
+                       int4 *ids, hipStream_t s, bnamd_batch_stats *stats, const TuneChoice &tune,
+                       bool useGraph) {
   BatchState &st = state();
   const HostViews hv = make_views(views, 16, 16);
   const ViewsDD vd = upload_views(st, hv, s);
@@ -860,15 +867,16 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     e.blockHist = st.blockHist.ptr;
     e.sq = st.sq.ptr;
     e.accum = accum;
+    e.ids = ids;
     e.cnt = st.cnt.ptr;
     e.chunk = tune.chunk;
     if (useGraph) {
       // kernel arguments (accum0, buffers) are baked into the graph, so the
       // cache key covers them
       char key[512];
-      snprintf(key, sizeof key, "%p %p %d %d %d %d %d %d %p %d %d %zu", (void *)e.w.nodes, (void *)accum,
-               e.numPixels, e.sppBatch, e.accum0, maxDepth, gridPersist, e.chunk, (void *)st.cams.ptr,
-               rp0.rrDepth, rp0.maxDiffuseBounces, st.q0.count);
+      snprintf(key, sizeof key, "%p %p %p %d %d %d %d %d %d %p %d %d %zu", (void *)e.w.nodes, (void *)accum,
+               (void *)ids, e.numPixels, e.sppBatch, e.accum0, maxDepth, gridPersist, e.chunk,
+               (void *)st.cams.ptr, rp0.rrDepth, rp0.maxDiffuseBounces, st.q0.count);
       if (!st.exec || st.graphKey != key) {
         if (st.exec) {
           (void)hipGraphExecDestroy(st.exec);
@@ -922,7 +930,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
 using namespace bnamd;
 
 extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views, const bnamd_render_params *params,
-                                  float *accum, int mode, void *stream, bnamd_batch_stats *stats) {
+                                  float *accum, int *ids, int mode, void *stream, bnamd_batch_stats *stats) {
   return guarded([&] {
     BN_REQUIRE(world && views && params && accum, "bnamd_render_batch: NULL argument");
     BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_render_batch: pathsPerPixel must be >= 1");
@@ -938,7 +946,7 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
       BN_HIP(hipEventCreate(&e0));
       BN_HIP(hipEventCreate(&e1));
       BN_HIP(hipEventRecord(e0, s));
-      render_mega(w, views, rp, params->pathsPerPixel, (float4 *)accum, st.megaCnt.ptr, s);
+      render_mega(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, st.megaCnt.ptr, s);
       BN_HIP(hipEventRecord(e1, s));
       BN_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -961,7 +969,7 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
     } else {
       BatchState &st = state();
       const char *ng = getenv("BNAMD_NO_GRAPH");
-      render_wavefront(w, views, rp, params->pathsPerPixel, (float4 *)accum, s, stats, st.tune,
+      render_wavefront(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, s, stats, st.tune,
                        !(ng && ng[0] == '1'));
     }
   });
@@ -994,8 +1002,8 @@ extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views 
         t.sppBatch = params->pathsPerPixel;
         // one spp per candidate keeps tuning short; the better of two runs
         // keeps one noisy run from deciding
-        float ms = render_wavefront(w, views, rp, 1, accum.ptr, 0, nullptr, t, false);
-        ms = std::min(ms, render_wavefront(w, views, rp, 1, accum.ptr, 0, nullptr, t, false));
+        float ms = render_wavefront(w, views, rp, 1, accum.ptr, nullptr, 0, nullptr, t, false);
+        ms = std::min(ms, render_wavefront(w, views, rp, 1, accum.ptr, nullptr, 0, nullptr, t, false));
         if (ms < bestMs) {
           bestMs = ms;
           best = t;

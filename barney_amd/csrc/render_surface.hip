@@ -85,17 +85,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
         const ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) {
           alpha = sh.hit ? 1.f : ray.missAlpha;
-          if (ids && p == 0 && accum0 == 0) {
-            int4 rec = make_int4(-1, -1, -1, __float_as_int(BN_INF));
-            if (sh.hit) {
-              const SurfaceHit s = surface_epilogue<TL>(w, ray.org, dir, 0.f, ray.tMax, h);
-              const GeomDev &g = *s.g;
-              // Spheres.dev.cu:192-195 writes 13+rank as the instance ID.
-              rec = make_int4(s.primID, g.kind == BNAMD_GEOM_SPHERES ? 13 : s.instID, g.userID,
-                              __float_as_int(h.t));
-            }
-            ids[pix] = rec;
-          }
+          if (ids && p == 0 && accum0 == 0) ids[pix] = hit_ids<TL>(w, ray.org, dir, ray.tMax, h);
         }
         PathRay shadow;
         vec3f frag = bounce(rpp, ray, sh, shadow, gen);

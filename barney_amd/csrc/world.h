@@ -207,6 +207,20 @@ BN_D SurfaceHit surface_epilogue(const WorldDev &w, vec3f org, vec3f dir, float 
   return s;
 }
 
+// The ID-buffer record of a primary hit, as the anyHit / intersect programs
+// write it for the first path of the first frame (Triangles.dev.cu:45-60;
+// Spheres.dev.cu:178-198, which writes 13 + rank as the instance ID;
+// Cylinders.dev.cu:244-255): {primID, instID, objID (the geometry's user
+// ID), depth as float bits} of the closest hit; a miss keeps
+// {-1, -1, -1, inf}.
+template <bool TL>
+BN_D int4 hit_ids(const WorldDev &w, vec3f org, vec3f dir, float tmax, const HitRec &h) {
+  if (h.flat == 0xffffffffu) return make_int4(-1, -1, -1, __float_as_int(BN_INF));
+  const SurfaceHit s = surface_epilogue<TL>(w, org, dir, 0.f, tmax, h);
+  const GeomDev &g = *s.g;
+  return make_int4(s.primID, g.kind == BNAMD_GEOM_SPHERES ? 13 : s.instID, g.userID, __float_as_int(h.t));
+}
+
 // Geometry record of a hit (material lookups).
 template <bool TL>
 BN_D const GeomDev &hit_geom(const WorldDev &w, uint32_t flat) {

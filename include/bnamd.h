@@ -380,9 +380,14 @@ typedef struct {
 
 /* params->width/height are ignored (per-view sizes); pathsPerPixel paths
  * per pixel starting at params->accumID. accum is overwritten when
- * params->accumID == 0, accumulated into otherwise. */
+ * params->accumID == 0, accumulated into otherwise. ids: NULL, or device
+ * int4 per pixel (same offsets as accum) that receives the ID record of
+ * bnamd_render_surface ({primID, instID, objID, depth bits} of the primary
+ * hit of the path with accumID 0, the hitIDs buffer barney's anyHit
+ * programs fill: Triangles.dev.cu:45-60, Spheres.dev.cu:178-198); left
+ * untouched when params->accumID != 0. */
 int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views,
-                       const bnamd_render_params *params, float *accum, int mode,
+                       const bnamd_render_params *params, float *accum, int *ids, int mode,
                        void *stream, bnamd_batch_stats *stats);
 
 /* Times candidate wavefront launch configurations on this world and views

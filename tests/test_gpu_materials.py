@@ -40,11 +40,11 @@ def _setup(gpu, mats_fn=None):
     from barney_amd import scenes
     w3 = scenes.workload3(grid=160, spheres=2000, cylinders=1000, views=3, seed=0x5EED0303)
     mats = w3["materials"] if mats_fn is None else mats_fn(w3["materials"])
-    geoms = [gpu.Geom(g["kind"], g["vertices"], g.get("indices"), g.get("radii"), material=g["material"])
-             for g in w3["geoms"]]
+    geoms = [gpu.Geom(g["kind"], g["vertices"], g.get("indices"), g.get("radii"), material=g["material"],
+                      user_id=100 + k) for k, g in enumerate(w3["geoms"])]
     world = gpu.World([geoms], materials=scenes.product_materials(gpu, mats))
     ogeoms = [dict(kind=g["kind"], vertices=g["vertices"], indices=g.get("indices"), radii=g.get("radii"),
-                   material=_oracle_material(mats[g["material"]])) for g in w3["geoms"]]
+                   material=_oracle_material(mats[g["material"]]), user_id=100 + k) for k, g in enumerate(w3["geoms"])]
     ref = refcpu.Scene(ogeoms)
     ws = [max(24, w // 10) for w in w3["widths"]]
     hs = [max(24, h // 10) for h in w3["heights"]]
@@ -121,3 +121,33 @@ def test_glass_paths_per_pixel(gpu, depth):
         print(f"view {v} {ws[v]}x{hs[v]} depth {depth} per-pixel close {close:.5f}", flush=True)
         assert close >= 0.999, (v, close)
     assert stats["raysTraced"] == rays
+
+
+@pytest.mark.parametrize("mode", ["megakernel", "wavefront"])
+def test_batch_ids_equal_oracle(gpu, mode):
+    """The ID channels of the batched renderers: each view's {primID, instID,
+    objID, depth} of the first path's primary hit equal the oracle's ID
+    buffer bit for bit (triangles, spheres with barney's 13 + rank instance
+    ID, cylinders, misses); a later frame (accumID > 0) leaves them alone."""
+    w3, world, ref, views, ws, hs = _setup(gpu)
+    m = gpu.BATCH_MEGAKERNEL if mode == "megakernel" else gpu.BATCH_WAVEFRONT
+    p = gpu.render_params(0, 0, spp=2, ambient=1.0, bg=(-1, -1, -1, -1), max_depth=1, max_diffuse=8,
+                          russian_roulette=0)
+    idbuf = gpu.DeviceArray.empty((views.num_pixels, 4), np.int32)
+    out = gpu.render_batch(world, views, p, m, ids=idbuf)
+    kinds = set()
+    for v, c in enumerate(w3["cameras"]):
+        rp = refcpu.render_params(ws[v], hs[v], spp=1, ambient=1.0, bg=(-1, -1, -1, -1), max_depth=1)
+        _, rids, _ = ref.render(refcpu.camera(c["position"], c["direction"], fovy=c["fovy"]), rp, with_ids=True)
+        g = views.image(out["ids"], v)
+        assert np.array_equal(g, rids), (mode, v, int((g != rids).any(-1).sum()))
+        kinds |= set(np.unique(rids[..., 2]).tolist())
+        assert (rids[..., 0] >= 0).mean() > 0.5
+    assert len(kinds - {-1}) >= 3  # heightfield, spheres and cylinders all seen
+    # a later frame does not touch the ID buffer
+    p2 = gpu.render_params(0, 0, spp=1, accum_id=2, ambient=1.0, bg=(-1, -1, -1, -1), max_depth=1,
+                           max_diffuse=8, russian_roulette=0)
+    out2 = gpu.render_batch(world, views, p2, m, accum=out["accum"], ids=idbuf)
+    assert out2["stats"]["raysTraced"] > 0
+    assert np.array_equal(out2["ids"], out["ids"])
+
