@@ -23,6 +23,7 @@ struct UMeshDD {
   vec3f worldLo, worldHi;  // bounds of the vertices
   const float *majorants;
   const float2 *ranges;
+  const uint32_t *isoMask;  // bit per macrocell: isoValue within its range (or null)
   int mx, my, mz;
   vec3f mcOrigin, mcSpacing;
   TFDD tf;
@@ -149,6 +150,17 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
     }
   }
   return __builtin_nanf("");
+}
+
+// The iso walk's range test through the per-render bit mask: one bit per
+// macrocell (3.4 MB for a 302^3 grid, L2-resident) instead of an 8-byte
+// range (220 MB), with the same comparison made when the mask was built.
+BN_HD bool iso_cell_active(const UMeshDD &u, int idx, float isoValue) {
+  if (!u.isoMask) {
+    const float2 vr = u.ranges[idx];
+    return isoValue >= vr.x && isoValue <= vr.y;
+  }
+  return (u.isoMask[(uint32_t)idx >> 5] >> ((uint32_t)idx & 31u)) & 1u;
 }
 
 BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {

@@ -19,6 +19,11 @@ struct UMesh {
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
   DevBuf<float4> tf;
+  // per-cell bits for isoMaskValue, rebuilt by render calls when the iso
+  // value changes (calls on one mesh are not concurrent, as in barney)
+  mutable DevBuf<uint32_t> isoMask;
+  mutable float isoMaskValue = 0.f;
+  mutable bool haveIsoMask = false;
   UMeshDD dd{};
   int numTets = 0;
   bool haveTF = false;
@@ -138,6 +143,22 @@ __global__ void k_raster_tets(MCRaster g, const int4 *__restrict tets, const flo
 __global__ void k_finish_ranges(const uint2 *__restrict in, float2 *__restrict out, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = make_float2(float_of_ord(in[i].x), float_of_ord(in[i].y));
+}
+
+// Bit c of the mask: isoValue lies within macrocell c's value range.
+__global__ void k_iso_mask(const float2 *__restrict ranges, size_t n, float isoValue, uint32_t *__restrict mask) {
+  const size_t nw = (n + 31) / 32;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < nw; w += (size_t)gridDim.x * blockDim.x) {
+    uint32_t bits = 0u;
+    for (int b = 0; b < 32; ++b) {
+      const size_t c = w * 32 + b;
+      if (c < n) {
+        const float2 vr = ranges[c];
+        if (isoValue >= vr.x && isoValue <= vr.y) bits |= 1u << b;
+      }
+    }
+    mask[w] = bits;
+  }
 }
 
 __global__ void k_umesh_majorants(TFDD tf, const float2 *__restrict ranges, float *__restrict maj, size_t n) {
@@ -340,6 +361,26 @@ void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi,
   m.haveTF = true;
 }
 
+// The mesh's device record for an iso-surface walk at isoValue: with the
+// macrocell bit mask of that value, rebuilt on s when the value changes.
+UMeshDD iso_dd(const UMesh &m, float isoValue, hipStream_t s) {
+  const size_t ncell = (size_t)m.dd.mx * m.dd.my * m.dd.mz;
+  const bool same = m.haveIsoMask && (m.isoMaskValue == isoValue ||
+                                      (m.isoMaskValue != m.isoMaskValue && isoValue != isoValue));
+  if (!same) {
+    m.isoMask.ensure((ncell + 31) / 32);
+    const size_t nw = (ncell + 31) / 32;
+    k_iso_mask<<<(int)std::min<size_t>(div_up(nw, 256), 4096), 256, 0, s>>>(m.ranges.ptr, ncell, isoValue,
+                                                                            m.isoMask.ptr);
+    BN_HIP(hipGetLastError());
+    m.isoMaskValue = isoValue;
+    m.haveIsoMask = true;
+  }
+  UMeshDD d = m.dd;
+  d.isoMask = m.isoMask.ptr;
+  return d;
+}
+
 }  // namespace
 }  // namespace bnamd
 
@@ -499,7 +540,8 @@ extern "C" int bnamd_umesh_iso_rays(const bnamd_umesh *mesh, const bnamd_ray *ra
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_iso_rays: BVH too deep for the LDS stack");
     hipStream_t s = (hipStream_t)stream;
-    k_umesh_iso_rays<<<div_up(numRays, 256), 256, lds, s>>>(mesh->m.dd, rays, isoValue, (float4 *)out, numRays);
+    k_umesh_iso_rays<<<div_up(numRays, 256), 256, lds, s>>>(iso_dd(mesh->m, isoValue, s), rays, isoValue,
+                                                            (float4 *)out, numRays);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
   });
@@ -529,13 +571,14 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     BN_HIP(hipEventCreate(&e0));
     BN_HIP(hipEventCreate(&e1));
     BN_HIP(hipEventRecord(e0, s));
+    const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
     const int spp = params->pathsPerPixel;
     int lanes = spp >= 4 ? 4 : (spp >= 2 ? 2 : 1);
     if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
-    if (lanes >= 8) launch_render_umesh<8>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else if (lanes >= 4) launch_render_umesh<4>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else if (lanes >= 2) launch_render_umesh<2>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else launch_render_umesh<1>(mesh->m.dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else launch_render_umesh<1>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     BN_HIP(hipGetLastError());
     BN_HIP(hipEventRecord(e1, s));
     BN_HIP(hipEventSynchronize(e1));

@@ -461,6 +461,15 @@ struct IsoHit {
   vec3f P, N;
 };
 
+// Whether macrocell idx's value range contains the iso value (the range test
+// of MCIsoSurfaceAccel::isProg). Grids with a precomputed per-cell bit for
+// the iso value overload it (found by argument-dependent lookup).
+template <class V>
+BN_HD bool iso_cell_active(const V &v, int idx, float isoValue) {
+  const float2 vr = v.ranges[idx];
+  return isoValue >= vr.x && isoValue <= vr.y;
+}
+
 // MCIsoSurfaceAccel::isProg (barney/volume/MCAccelerator.h:338-567): DDA over
 // the macrocells whose value range contains the iso value, 10 sub-steps per
 // cell with a linear root, then a central-difference gradient normal.
@@ -491,8 +500,7 @@ BN_HD IsoHit iso_intersect(const V &v, const F &field, float isoValue, vec3f org
   dda3(dorg, ddir, t1, v.mx, v.my, v.mz, [&](int cx, int cy, int cz, float ct0, float ct1) -> bool {
     const float _t0 = ct0, _t1 = ct1;
     if (ct0 >= fminf(ct1, rayTMax)) return true;
-    const float2 vr = v.ranges[cx + v.mx * (cy + v.my * cz)];
-    if (!(isoValue >= vr.x && isoValue <= vr.y)) return true;
+    if (!iso_cell_active(v, cx + v.mx * (cy + v.my * cz), isoValue)) return true;
     // sub-step i samples tt_i = lerp(i / 10, t0, t1) (tt_0 = t0); one
     // field() call site so the point location is inlined once
     float ff0 = 0.f, ff1 = 0.f, tt0 = 0.f, tt1 = 0.f;
