@@ -248,10 +248,17 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
   return sh;
 }
 
+// L lanes per pixel, each looping over its share of the pixel's paths.
+// The kernel waits on memory (point-location loads, 19 % VALU issue), so
+// it is built for six wavefronts per SIMD (<= 80 VGPRs, ~300 bytes of
+// spill per lane): 1024x1024x8 spp with volume + iso-surface takes 147 ms
+// at six wavefronts and 8 lanes per pixel, against 232 ms at the three
+// that the unconstrained allocation (168 VGPRs) leaves with 4 lanes; five
+// and eight wavefronts measured 156 and 159 ms.
 template <int L, bool kVol, bool kIso>
-__global__ void __launch_bounds__(256) k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp,
-                                                      float4 *__restrict accum,
-                                                      unsigned long long *__restrict counters) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
+               unsigned long long *__restrict counters) {
   extern __shared__ uint2 lds_stack[];
   constexpr int PW = 64 / L;
   constexpr int PB = 4 * PW;
@@ -573,7 +580,7 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     BN_HIP(hipEventRecord(e0, s));
     const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
     const int spp = params->pathsPerPixel;
-    int lanes = spp >= 4 ? 4 : (spp >= 2 ? 2 : 1);
+    int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
     if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
     if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);

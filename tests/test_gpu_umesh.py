@@ -98,16 +98,18 @@ def _compare(gpu, meshes, volume, iso, max_diffuse, W=96, H=72, spp=4):
     return ga, ra, close, out["stats"], nr, ns
 
 
-def test_volume_only_per_pixel(gpu, meshes):
-    ga, ra, close, st, nr, ns = _compare(gpu, meshes, True, None, 0)
+@pytest.mark.parametrize("spp", [4, 8])
+def test_volume_only_per_pixel(gpu, meshes, spp):
+    ga, ra, close, st, nr, ns = _compare(gpu, meshes, True, None, 0, spp=spp)
     print(f"volume close {close:.5f} samples gpu {st['samples']} oracle {ns}", flush=True)
     assert close >= 0.999
     assert abs(st["samples"] - ns) <= 1e-3 * ns and st["raysTraced"] == nr
     assert ra[..., 3].mean() > 0.2
 
 
-def test_iso_only_per_pixel(gpu, meshes):
-    ga, ra, close, st, nr, ns = _compare(gpu, meshes, False, 0.5, 0)
+@pytest.mark.parametrize("spp", [4, 8])
+def test_iso_only_per_pixel(gpu, meshes, spp):
+    ga, ra, close, st, nr, ns = _compare(gpu, meshes, False, 0.5, 0, spp=spp)
     print(f"iso close {close:.5f}", flush=True)
     assert close >= 0.999
     assert st["raysTraced"] == nr
