@@ -301,8 +301,12 @@ __device__ __forceinline__ void tf_flush(TfAcc &a, float *gtf) {
   a.idx = -1;
 }
 
+// Built for five wavefronts per SIMD (<= 96 VGPRs, ~100-180 bytes of spill
+// per lane): the replay waits on field loads and gradient atomics, and the
+// 64-view 512x512 adjoint takes 144 ms against 151 ms at the three or four
+// the unconstrained allocation (121-128 VGPRs) leaves; at six it is 201 ms.
 template <int K>
-__global__ void __launch_bounds__(256) k_ea_backward(EADD e, const float *__restrict gimg,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ea_backward(EADD e, const float *__restrict gimg,
                                                      const float *__restrict ckpt, float *__restrict gdens,
                                                      float *__restrict tfPartial,
                                                      unsigned long long *__restrict counter) {
