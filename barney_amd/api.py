@@ -344,23 +344,42 @@ class Volume:
 
 
 class UMesh:
-    """Unstructured tetrahedral volume (per-vertex scalars): point-location
-    BVH, macrocell grid, transfer function; volume and iso-surface render."""
+    """Unstructured volume (per-vertex scalars): point-location BVH,
+    macrocell grid, transfer function; volume and iso-surface render. Either
+    tets (int4, positively oriented) or general cells = (indices, offsets,
+    types) in UMeshField's layout (types: BNAMD_CELL_* / ANARI codes)."""
 
-    def __init__(self, vertices, scalars, tets):
+    def __init__(self, vertices, scalars, tets=None, cells=None):
         lib = init()
         v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         sc = np.ascontiguousarray(scalars, dtype=np.float32).reshape(-1)
-        t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
         if len(sc) != len(v):
             raise ValueError("one scalar per vertex expected")
-        dv, ds, dt = DeviceArray(v), DeviceArray(sc), DeviceArray(t)
+        dv, ds = DeviceArray(v), DeviceArray(sc)
         d = capi.bnamd_umesh_desc()
         d.vertices, d.scalars, d.numVertices = dv.addr, ds.addr, len(v)
-        d.tets, d.numTets = dt.addr, len(t)
+        bufs = [dv, ds]
+        if cells is None:
+            t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+            dt = DeviceArray(t)
+            bufs.append(dt)
+            d.tets, d.numTets = dt.addr, len(t)
+        else:
+            ix = np.ascontiguousarray(cells[0], dtype=np.int32).reshape(-1)
+            off = np.ascontiguousarray(cells[1], dtype=np.int32).reshape(-1)
+            ty = np.ascontiguousarray(cells[2], dtype=np.uint8).reshape(-1)
+            if len(off) != len(ty):
+                raise ValueError("one offset and one type per cell expected")
+            di, do, dty = DeviceArray(ix), DeviceArray(off), DeviceArray(ty)
+            bufs += [di, do, dty]
+            d.indices, d.numIndices = di.addr, len(ix)
+            d.cellOffsets, d.cellTypes, d.numCells = do.addr, dty.addr, len(off)
         self.handle = C.c_void_p()
-        check(lib.bnamd_umesh_create(C.byref(d), None, C.byref(self.handle)))
-        dv.free(); ds.free(); dt.free()
+        try:
+            check(lib.bnamd_umesh_create(C.byref(d), None, C.byref(self.handle)))
+        finally:
+            for b in bufs:
+                b.free()
 
     def info(self):
         i = capi.bnamd_umesh_info()

@@ -66,7 +66,8 @@ class bnamd_volume_stats(C.Structure):
 
 class bnamd_umesh_desc(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("scalars", C.c_void_p), ("numVertices", C.c_int), ("tets", C.c_void_p),
-                ("numTets", C.c_int)]
+                ("numTets", C.c_int), ("indices", C.c_void_p), ("numIndices", C.c_int), ("cellOffsets", C.c_void_p),
+                ("cellTypes", C.c_void_p), ("numCells", C.c_int)]
 
 
 class bnamd_umesh_info(C.Structure):

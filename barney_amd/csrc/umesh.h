@@ -17,8 +17,9 @@ namespace bnamd {
 // vertices as (x, y, z, scalar), so a leaf test reads one contiguous line.
 struct UMeshDD {
   const BNode8 *nodes;
-  const float4 *tets;
-  uint32_t numTets;
+  const float4 *tets;       // leaf order: 4 records per tet, or 8 per element of a mixed mesh
+  const uint8_t *elemTypes; // mixed meshes: ELEM_* per leaf position (null for tet meshes)
+  uint32_t numTets;         // elements
   int depth;
   vec3f worldLo, worldHi;  // bounds of the vertices
   const float *majorants;
@@ -48,7 +49,181 @@ BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
   return true;
 }
 
-// First tet (in BVH traversal order) containing P; NaN when none does.
+// ---------------------------------------------------------------------------
+// Pyramids, prisms (wedges) and hexahedra (UMeshField.h:436-501 pyr/prism/
+// hexScalar -> ElementIntersection.h intersectPyrEXT / intersectPrismEXT /
+// intersectHexEXT, adapted there from OpenVKL): Newton iteration for the
+// parametric coordinates (r, s, t) of P from (.5, .5, .5), at most 10 steps,
+// converged when every update is below 1e-4, diverged when a coordinate
+// leaves [-1e6, 1e6]; inside when r, s, t lie in [-1e-6, 1 + 1e-6] (prisms:
+// also r + s); the value interpolates the vertex scalars with the shape
+// functions of the last step's starting point, as the reference does.
+// Vertices follow VTK order; an element is 8 float4 records {x, y, z, s}.
+enum : uint8_t { ELEM_TET = 0, ELEM_PYR = 1, ELEM_PRISM = 2, ELEM_HEX = 3 };
+
+// shape functions sf[N] and derivatives d[3N] (r-, s-, t-derivatives)
+BN_HD void pyr_shape(const float *pc, float *sf, float *d) {
+#pragma clang fp contract(off)
+  const float rm = 1.f - pc[0], sm = 1.f - pc[1], tm = 1.f - pc[2];
+  sf[0] = rm * sm * tm;
+  sf[1] = pc[0] * sm * tm;
+  sf[2] = pc[0] * pc[1] * tm;
+  sf[3] = rm * pc[1] * tm;
+  sf[4] = pc[2];
+  d[0] = -(pc[1] - 1.f) * (pc[2] - 1.f);
+  d[1] = (pc[1] - 1.f) * (pc[2] - 1.f);
+  d[2] = pc[1] - pc[1] * pc[2];
+  d[3] = pc[1] * (pc[2] - 1.f);
+  d[4] = 0.f;
+  d[5] = -(pc[0] - 1.f) * (pc[2] - 1.f);
+  d[6] = pc[0] * (pc[2] - 1.f);
+  d[7] = pc[0] - pc[0] * pc[2];
+  d[8] = (pc[0] - 1.f) * (pc[2] - 1.f);
+  d[9] = 0.f;
+  d[10] = -(pc[0] - 1.f) * (pc[1] - 1.f);
+  d[11] = pc[0] * (pc[1] - 1.f);
+  d[12] = -pc[0] * pc[1];
+  d[13] = (pc[0] - 1.f) * pc[1];
+  d[14] = 1.f;
+}
+
+BN_HD void prism_shape(const float *pc, float *sf, float *d) {
+#pragma clang fp contract(off)
+  sf[0] = (1.f - pc[0] - pc[1]) * (1.f - pc[2]);
+  sf[1] = pc[0] * (1.f - pc[2]);
+  sf[2] = pc[1] * (1.f - pc[2]);
+  sf[3] = (1.f - pc[0] - pc[1]) * pc[2];
+  sf[4] = pc[0] * pc[2];
+  sf[5] = pc[1] * pc[2];
+  d[0] = -1.f + pc[2];
+  d[1] = 1.f - pc[2];
+  d[2] = 0.f;
+  d[3] = -pc[2];
+  d[4] = pc[2];
+  d[5] = 0.f;
+  d[6] = -1.f + pc[2];
+  d[7] = 0.f;
+  d[8] = 1.f - pc[2];
+  d[9] = -pc[2];
+  d[10] = 0.f;
+  d[11] = pc[2];
+  d[12] = -1.f + pc[0] + pc[1];
+  d[13] = -pc[0];
+  d[14] = -pc[1];
+  d[15] = 1.f - pc[0] - pc[1];
+  d[16] = pc[0];
+  d[17] = pc[1];
+}
+
+BN_HD void hex_shape(const float *pc, float *sf, float *d) {
+#pragma clang fp contract(off)
+  const float rm = 1.f - pc[0], sm = 1.f - pc[1], tm = 1.f - pc[2];
+  sf[0] = rm * sm * tm;
+  sf[1] = pc[0] * sm * tm;
+  sf[2] = pc[0] * pc[1] * tm;
+  sf[3] = rm * pc[1] * tm;
+  sf[4] = rm * sm * pc[2];
+  sf[5] = pc[0] * sm * pc[2];
+  sf[6] = pc[0] * pc[1] * pc[2];
+  sf[7] = rm * pc[1] * pc[2];
+  d[0] = -sm * tm;
+  d[1] = sm * tm;
+  d[2] = pc[1] * tm;
+  d[3] = -pc[1] * tm;
+  d[4] = -sm * pc[2];
+  d[5] = sm * pc[2];
+  d[6] = pc[1] * pc[2];
+  d[7] = -pc[1] * pc[2];
+  d[8] = -rm * tm;
+  d[9] = -pc[0] * tm;
+  d[10] = pc[0] * tm;
+  d[11] = rm * tm;
+  d[12] = -rm * pc[2];
+  d[13] = -pc[0] * pc[2];
+  d[14] = pc[0] * pc[2];
+  d[15] = rm * pc[2];
+  d[16] = -rm * sm;
+  d[17] = -pc[0] * sm;
+  d[18] = -pc[0] * pc[1];
+  d[19] = -rm * pc[1];
+  d[20] = rm * sm;
+  d[21] = pc[0] * sm;
+  d[22] = pc[0] * pc[1];
+  d[23] = rm * pc[1];
+}
+
+BN_HD float det3(vec3f x, vec3f y, vec3f z) { return dot(x, cross(y, z)); }
+
+template <int N, int KIND>
+BN_HD bool newton_scalar(const float4 *V, vec3f P, float &value) {
+#pragma clang fp contract(off)
+  float tol = 1e-6f;
+  if (KIND == ELEM_HEX) {  // 1e-10 x the diagonal of the vertices' box
+    vec3f lo = {V[0].x, V[0].y, V[0].z}, hi = lo;
+    for (int i = 1; i < N; ++i) {
+      const vec3f p = {V[i].x, V[i].y, V[i].z};
+      lo = vmin(lo, p);
+      hi = vmax(hi, p);
+    }
+    tol = 1e-10f * length(hi - lo);
+  }
+  float pc[3] = {.5f, .5f, .5f};
+  float w[N], d[3 * N];
+  bool converged = false;
+  for (int it = 0; !converged && it < 10; ++it) {
+    if (KIND == ELEM_PYR) pyr_shape(pc, w, d);
+    else if (KIND == ELEM_PRISM) prism_shape(pc, w, d);
+    else hex_shape(pc, w, d);
+    vec3f fcol = vec3f(0.f), rcol = vec3f(0.f), scol = vec3f(0.f), tcol = vec3f(0.f);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const vec3f pt = {V[i].x, V[i].y, V[i].z};
+      fcol = fcol + pt * w[i];
+      rcol = rcol + pt * d[i];
+      scol = scol + pt * d[i + N];
+      tcol = tcol + pt * d[i + 2 * N];
+    }
+    fcol = fcol - P;
+    const float D = det3(rcol, scol, tcol);
+    if (fabsf(D) < tol) return false;
+    const float d0 = det3(fcol, scol, tcol) / D;
+    const float d1 = det3(rcol, fcol, tcol) / D;
+    const float d2 = det3(rcol, scol, fcol) / D;
+    pc[0] = pc[0] - d0;
+    pc[1] = pc[1] - d1;
+    pc[2] = pc[2] - d2;
+    if (fabsf(d0) < 1e-4f && fabsf(d1) < 1e-4f && fabsf(d2) < 1e-4f)
+      converged = true;
+    else if (fabsf(pc[0]) > 1e6f || fabsf(pc[1]) > 1e6f || fabsf(pc[2]) > 1e6f)
+      return false;
+  }
+  if (!converged) return false;
+  const float lo = 0.f - 1e-6f, hi = 1.f + 1e-6f;
+  if (!(pc[0] >= lo && pc[0] <= hi && pc[1] >= lo && pc[1] <= hi && pc[2] >= lo && pc[2] <= hi)) return false;
+  if (KIND == ELEM_PRISM && !(pc[0] + pc[1] <= hi)) return false;
+  float val = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) val += w[i] * V[i].w;
+  value = val;
+  return true;
+}
+
+// eltScalar (UMeshField.h:362-388) over the packed leaf-order elements of a
+// mixed mesh (kMixed), or tetScalar over packed tets.
+template <bool kMixed>
+BN_D bool elem_scalar(const UMeshDD &u, uint32_t i, vec3f P, float &val) {
+  if (!kMixed) return tet_scalar(u.tets + 4ull * i, P, val);
+  const float4 *e = u.tets + 8ull * i;
+  switch (u.elemTypes[i]) {
+    case ELEM_TET: return tet_scalar(e, P, val);
+    case ELEM_PYR: return newton_scalar<5, ELEM_PYR>(e, P, val);
+    case ELEM_PRISM: return newton_scalar<6, ELEM_PRISM>(e, P, val);
+    default: return newton_scalar<8, ELEM_HEX>(e, P, val);
+  }
+}
+
+// First element (tet, or any element of a mixed mesh) in BVH traversal order
+// containing P; NaN when none does.
 // Child boxes are tested in the node's quantized frame: (P - origin) * 2^-e
 // against the 16-bit planes, which carry one quantum of slack on each side,
 // so the test is conservative.
@@ -60,17 +235,18 @@ BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
 // from the traversal-order answer only for points on a face shared by two
 // tets, where both interpolate the same plane.
 constexpr int kLocateWindow = 4;
+template <bool kMixed = false>
 BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
   if (hint != ~0u) {
     float val;
-    if (tet_scalar(u.tets + 4ull * hint, P, val)) return val;
+    if (elem_scalar<kMixed>(u, hint, P, val)) return val;
     for (int d = 1; d <= kLocateWindow; ++d) {
       const uint32_t a = hint + (uint32_t)d, b = hint - (uint32_t)d;
-      if (a < u.numTets && tet_scalar(u.tets + 4ull * a, P, val)) {
+      if (a < u.numTets && elem_scalar<kMixed>(u, a, P, val)) {
         hint = a;
         return val;
       }
-      if (hint >= (uint32_t)d && tet_scalar(u.tets + 4ull * b, P, val)) {
+      if (hint >= (uint32_t)d && elem_scalar<kMixed>(u, b, P, val)) {
         hint = b;
         return val;
       }
@@ -136,7 +312,7 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
       float val;
-      if (tet_scalar(u.tets + 4ull * (primBase + i), P, val)) {
+      if (elem_scalar<kMixed>(u, primBase + i, P, val)) {
         hint = primBase + i;
         return val;
       }
@@ -163,8 +339,9 @@ BN_HD bool iso_cell_active(const UMeshDD &u, int idx, float isoValue) {
   return (u.isoMask[(uint32_t)idx >> 5] >> ((uint32_t)idx & 31u)) & 1u;
 }
 
+template <bool kMixed = false>
 BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
-  const float f = umesh_locate(u, P, ctx, hint);
+  const float f = umesh_locate<kMixed>(u, P, ctx, hint);
   if (f != f) return VolSample{0.f, 0.f, 0.f, 0.f};
   return tf_map(u.tf, f);
 }

@@ -14,7 +14,8 @@ size_t trace_lds_bytes(int depth, int block);
 
 struct UMesh {
   World w;                 // point-location BVH (prims freed after packing)
-  DevBuf<float4> tets;     // 4 per tet, leaf order
+  DevBuf<float4> tets;     // leaf order: 4 per tet, or 8 per general element
+  DevBuf<uint8_t> types;   // general elements: ELEM_* per leaf position
   DevBuf<uint2> rangesOrd; // ordered-uint min/max during rasterization
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
@@ -50,6 +51,69 @@ __global__ void k_pack_tets(const PrimRec *__restrict prims, uint32_t n, const i
   out[4ull * p + 1] = v4[ix.y];
   out[4ull * p + 2] = v4[ix.z];
   out[4ull * p + 3] = v4[ix.w];
+}
+
+// General cells (UMeshField's layout): the element kind, its vertex count
+// and its box. Returns false for an unknown type or an index out of range.
+__device__ __forceinline__ int elem_kind(int type, int &nv) {
+  switch (type) {
+    case BNAMD_CELL_TET: case 0: nv = 4; return ELEM_TET;
+    case BNAMD_CELL_PYR: case 3: nv = 5; return ELEM_PYR;
+    case BNAMD_CELL_PRISM: case 2: nv = 6; return ELEM_PRISM;
+    case BNAMD_CELL_HEX: case 1: nv = 8; return ELEM_HEX;
+  }
+  nv = 0;
+  return -1;
+}
+
+struct CellsIn {
+  const int *indices;
+  const int *offsets;
+  const unsigned char *types;
+  int numIndices, numCells, numVertices;
+};
+
+// cellBounds (UMeshField.h:269-343): box of the element's vertices; kinds
+// into 'kinds'; 'bad' set when a cell cannot be read
+__global__ void k_elem_boxes(CellsIn c, const float4 *__restrict v4, float4 *__restrict boxes,
+                             uint8_t *__restrict kinds, int *__restrict bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.numCells) return;
+  int nv;
+  const int k = elem_kind(c.types[i], nv);
+  const int off = c.offsets[i];
+  bool ok = k >= 0 && off >= 0 && (long long)off + nv <= (long long)c.numIndices;
+  vec3f lo = vec3f(BN_INF), hi = vec3f(-BN_INF);
+  for (int q = 0; ok && q < nv; ++q) {
+    const int vi = c.indices[off + q];
+    if (vi < 0 || vi >= c.numVertices) {
+      ok = false;
+      break;
+    }
+    const float4 p = v4[vi];
+    lo = vmin(lo, vec3f{p.x, p.y, p.z});
+    hi = vmax(hi, vec3f{p.x, p.y, p.z});
+  }
+  if (!ok) {
+    atomicOr(bad, 1);
+    lo = hi = vec3f(0.f);
+  }
+  kinds[i] = (uint8_t)(ok ? k : 0);
+  boxes[2 * i] = make_float4(lo.x, lo.y, lo.z, 0.f);
+  boxes[2 * i + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+}
+
+// general elements in BVH leaf order: 8 records per element (unused ones 0)
+__global__ void k_pack_elems(const PrimRec *__restrict prims, uint32_t n, CellsIn c, const uint8_t *__restrict kinds,
+                             const float4 *__restrict v4, float4 *__restrict out, uint8_t *__restrict outKinds) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t e = __float_as_uint(prims[p].a.w);
+  const int k = kinds[e];
+  const int nv = k == ELEM_TET ? 4 : k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : 8;
+  const int off = c.offsets[e];
+  for (int q = 0; q < 8; ++q) out[8ull * p + q] = q < nv ? v4[c.indices[off + q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+  outKinds[p] = (uint8_t)k;
 }
 
 __device__ __forceinline__ uint32_t ord_of(float f) {
@@ -131,6 +195,51 @@ __device__ void raster_tet(const MCRaster &g, const V4 &a, const V4 &b, const V4
   }
 }
 
+// rasterBox(grid, worldBounds, box4f) (MCGrid.cuh:67-91), the variant
+// umeshRasterCells uses for cells other than tets: only an inverted box is
+// skipped, so flat boxes are rasterized
+__device__ void raster_box_cells(const MCRaster &g, const float4 *v, int nv) {
+#pragma clang fp contract(off)
+  float lx = v[0].x, ly = v[0].y, lz = v[0].z, lw = v[0].w, hx = lx, hy = ly, hz = lz, hw = lw;
+  for (int q = 1; q < nv; ++q) {
+    lx = fminf(lx, v[q].x); hx = fmaxf(hx, v[q].x);
+    ly = fminf(ly, v[q].y); hy = fmaxf(hy, v[q].y);
+    lz = fminf(lz, v[q].z); hz = fmaxf(hz, v[q].z);
+    lw = fminf(lw, v[q].w); hw = fmaxf(hw, v[q].w);
+  }
+  if (lx > hx || ly > hy || lz > hz) return;
+  int x0 = (int)((lx - g.origin.x) * g.rcpSpacing.x), x1 = (int)((hx - g.origin.x) * g.rcpSpacing.x);
+  int y0 = (int)((ly - g.origin.y) * g.rcpSpacing.y), y1 = (int)((hy - g.origin.y) * g.rcpSpacing.y);
+  int z0 = (int)((lz - g.origin.z) * g.rcpSpacing.z), z1 = (int)((hz - g.origin.z) * g.rcpSpacing.z);
+  x0 = min(max(x0, 0), g.mx - 1); x1 = min(max(x1, 0), g.mx - 1);
+  y0 = min(max(y0, 0), g.my - 1); y1 = min(max(y1, 0), g.my - 1);
+  z0 = min(max(z0, 0), g.mz - 1); z1 = min(max(z1, 0), g.mz - 1);
+  const uint32_t olo = ord_of(lw), ohi = ord_of(hw);
+  for (int iz = z0; iz <= z1; ++iz)
+    for (int iy = y0; iy <= y1; ++iy)
+      for (int ix = x0; ix <= x1; ++ix) {
+        uint2 *cell = g.ranges + ix + (size_t)iy * g.mx + (size_t)iz * g.mx * g.my;
+        atomicMin(&cell->x, olo);
+        atomicMax(&cell->y, ohi);
+      }
+}
+
+// umeshRasterCells (UMeshField.cu:110-160): tets by rasterTet<5>, other
+// cells by their box
+__global__ void k_raster_cells(MCRaster g, CellsIn c, const uint8_t *__restrict kinds, const float4 *__restrict v4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.numCells) return;
+  const int k = kinds[i];
+  const int nv = k == ELEM_TET ? 4 : k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : 8;
+  float4 v[8];
+  for (int q = 0; q < nv; ++q) v[q] = v4[c.indices[c.offsets[i] + q]];
+  if (k == ELEM_TET)
+    raster_tet<5>(g, V4{v[0].x, v[0].y, v[0].z, v[0].w}, V4{v[1].x, v[1].y, v[1].z, v[1].w},
+                  V4{v[2].x, v[2].y, v[2].z, v[2].w}, V4{v[3].x, v[3].y, v[3].z, v[3].w});
+  else
+    raster_box_cells(g, v, nv);
+}
+
 __global__ void k_raster_tets(MCRaster g, const int4 *__restrict tets, const float4 *__restrict v4, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -166,6 +275,7 @@ __global__ void k_umesh_majorants(TFDD tf, const float2 *__restrict ranges, floa
     maj[i] = tf_majorant(tf, ranges[i].x, ranges[i].y);
 }
 
+template <bool kMixed>
 __global__ void __launch_bounds__(256) k_umesh_sample(UMeshDD u, const float *__restrict pts, float *__restrict out,
                                                       int n) {
   extern __shared__ uint2 lds_stack[];
@@ -173,9 +283,10 @@ __global__ void __launch_bounds__(256) k_umesh_sample(UMeshDD u, const float *__
   if (i >= n) return;
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
   uint32_t hint = ~0u;
-  out[i] = umesh_locate(u, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, ctx, hint);
+  out[i] = umesh_locate<kMixed>(u, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, ctx, hint);
 }
 
+template <bool kMixed>
 __global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_ray *__restrict rays, float iso,
                                                         float4 *__restrict out, int n) {
   extern __shared__ uint2 lds_stack[];
@@ -186,7 +297,7 @@ __global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_r
   RNGSeed seed;
   uint32_t hint = ~0u;
   const IsoHit h = iso_intersect(
-      u, [&](vec3f P) { return umesh_locate(u, P, ctx, hint); }, iso, {r.org[0], r.org[1], r.org[2]},
+      u, [&](vec3f P) { return umesh_locate<kMixed>(u, P, ctx, hint); }, iso, {r.org[0], r.org[1], r.org[2]},
       fix_dir({r.dir[0], r.dir[1], r.dir[2]}), r.tmin, r.tmax, r.tmax, seed, bn_hash3(0u, 1u, 0u));
   out[i] = h.hit ? make_float4(h.t, h.N.x, h.N.y, h.N.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
 }
@@ -210,7 +321,7 @@ struct UMeshOpts {
 
 // Closest of the volume (geometry 0) and iso-surface (geometry 1) hits for
 // one ray; each intersection program advances the ray's seed once.
-template <bool kVol, bool kIso>
+template <bool kVol, bool kIso, bool kMixed>
 BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
                                 RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt, uint32_t &hint) {
   ShadeHit sh;
@@ -218,7 +329,7 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
   float tClosest = tMax;
   if (kVol) {
     const VolHit vh = volume_intersect_g(
-        u, [&](vec3f P) { return umesh_sample_and_map(u, P, ctx, hint); }, org, dir, 0.f, tMax, tMax, seed, false,
+        u, [&](vec3f P) { return umesh_sample_and_map<kMixed>(u, P, ctx, hint); }, org, dir, 0.f, tMax, tMax, seed, false,
         cnt, bn_hash3(0u, 0u, 0u));
     if (vh.hit) {
       sh.hit = true;
@@ -234,7 +345,7 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
         u,
         [&](vec3f P) {
           cnt.samples++;
-          return umesh_locate(u, P, ctx, hint);
+          return umesh_locate<kMixed>(u, P, ctx, hint);
         },
         o.isoValue, org, dir, 0.f, tMax, tClosest, seed, bn_hash3(0u, 1u, 0u));
     if (ih.hit) {
@@ -255,7 +366,7 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
 // at six wavefronts and 8 lanes per pixel, against 232 ms at the three
 // that the unconstrained allocation (168 VGPRs) leaves with 4 lanes; five
 // and eight wavefronts measured 156 and 159 ms.
-template <int L, bool kVol, bool kIso>
+template <int L, bool kVol, bool kIso, bool kMixed>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
                unsigned long long *__restrict counters) {
@@ -288,7 +399,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
       float alpha = 0.f;
       for (int gen = 0; ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
-        const ShadeHit sh = umesh_trace<kVol, kIso>(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt, hint);
+        const ShadeHit sh = umesh_trace<kVol, kIso, kMixed>(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt, hint);
         ++nrays;
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
         PathRay shadow;
@@ -298,7 +409,8 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
         if (frag.z > 0.f) sum.z += frag.z;
         if (shadow.tMax > 0.f) {
           const ShadeHit shs =
-              umesh_trace<kVol, kIso>(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt, hint);
+              umesh_trace<kVol, kIso, kMixed>(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt,
+                                              hint);
           ++nrays;
           PathRay dummy;
           frag = vmin(bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
@@ -335,7 +447,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
   wave_sum_u64(nrays, counters + 2);
 }
 
-template <int L>
+template <int L, bool kMixed = false>
 void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &cam, const RenderDD &rp, int spp,
                          float4 *accum, unsigned long long *counters, size_t lds, hipStream_t s) {
   constexpr int PB = 4 * (64 / L);
@@ -344,13 +456,13 @@ void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &c
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   const int grid = ((tiles + 7) / 8) * 8;
   if (o.volume && o.iso)
-    k_render_umesh<L, true, true><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    k_render_umesh<L, true, true, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
   else if (o.volume)
-    k_render_umesh<L, true, false><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    k_render_umesh<L, true, false, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
   else if (o.iso)
-    k_render_umesh<L, false, true><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    k_render_umesh<L, false, true, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
   else
-    k_render_umesh<L, false, false><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    k_render_umesh<L, false, false, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
 }
 
 void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi, float base) {
@@ -400,32 +512,62 @@ struct bnamd_umesh_s {
 extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bnamd_umesh **out) {
   return guarded([&] {
     BN_REQUIRE(desc && out, "bnamd_umesh_create: NULL argument");
-    BN_REQUIRE(desc->numVertices > 0 && desc->numTets > 0, "bnamd_umesh_create: empty mesh");
-    BN_REQUIRE(desc->vertices && desc->scalars && desc->tets, "bnamd_umesh_create: NULL array");
+    const bool cells = desc->tets == nullptr;
+    BN_REQUIRE(desc->numVertices > 0 && (cells ? desc->numCells > 0 && desc->numIndices > 0 : desc->numTets > 0),
+               "bnamd_umesh_create: empty mesh");
+    BN_REQUIRE(desc->vertices && desc->scalars && (!cells || (desc->indices && desc->cellOffsets && desc->cellTypes)),
+               "bnamd_umesh_create: NULL array");
     auto *um = new bnamd_umesh_s;
     try {
       UMesh &m = um->m;
       hipStream_t s = (hipStream_t)stream;
       const auto t0 = std::chrono::steady_clock::now();
-      m.numTets = desc->numTets;
+      const int numElems = cells ? desc->numCells : desc->numTets;
+      m.numTets = numElems;
       DevBuf<float4> v4(desc->numVertices);
       k_pack_verts<<<div_up(desc->numVertices, 256), 256, 0, s>>>(desc->vertices, desc->scalars, desc->numVertices,
                                                                   v4.ptr);
       BN_HIP(hipGetLastError());
+      const CellsIn c{desc->indices, desc->cellOffsets, desc->cellTypes, desc->numIndices, desc->numCells,
+                      desc->numVertices};
+      DevBuf<float4> boxes;
+      DevBuf<uint8_t> kinds;
       GeomDev g{};
-      g.kind = GEOM_TETS;
-      g.numPrims = desc->numTets;
       g.primOffset = 0;
-      g.vertices = (const float *)v4.ptr;
-      g.indices = desc->tets;
+      if (cells) {
+        boxes.alloc(2ull * numElems);
+        kinds.alloc(numElems);
+        DevBuf<int> bad(1);
+        BN_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), s));
+        k_elem_boxes<<<div_up(numElems, 256), 256, 0, s>>>(c, v4.ptr, boxes.ptr, kinds.ptr, bad.ptr);
+        BN_HIP(hipGetLastError());
+        int hbad = 0;
+        BN_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        BN_HIP(hipStreamSynchronize(s));
+        BN_REQUIRE(!hbad, "bnamd_umesh_create: a cell has an unknown type or an index out of range");
+        g.kind = GEOM_ELEM_BOXES;
+        g.vertices = (const float *)boxes.ptr;
+      } else {
+        g.kind = GEOM_TETS;
+        g.vertices = (const float *)v4.ptr;
+        g.indices = desc->tets;
+      }
+      g.numPrims = numElems;
       m.w.hgeoms.push_back(g);
-      m.w.numPrims = (uint32_t)desc->numTets;
+      m.w.numPrims = (uint32_t)numElems;
       MaterialDev md{};
       m.w.materials.upload(&md, 1, s);
       build_world_bvh(m.w, s);
-      m.tets.alloc(4ull * desc->numTets);
-      k_pack_tets<<<div_up(desc->numTets, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)desc->numTets,
-                                                            (const int4 *)desc->tets, v4.ptr, m.tets.ptr);
+      if (cells) {
+        m.tets.alloc(8ull * numElems);
+        m.types.alloc(numElems);
+        k_pack_elems<<<div_up(numElems, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)numElems, c, kinds.ptr, v4.ptr,
+                                                           m.tets.ptr, m.types.ptr);
+      } else {
+        m.tets.alloc(4ull * numElems);
+        k_pack_tets<<<div_up(numElems, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)numElems,
+                                                          (const int4 *)desc->tets, v4.ptr, m.tets.ptr);
+      }
       BN_HIP(hipGetLastError());
       BN_HIP(hipStreamSynchronize(s));
       m.w.prims.release();
@@ -435,7 +577,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       const vec3f lo = {m.w.lo[0], m.w.lo[1], m.w.lo[2]}, hi = {m.w.hi[0], m.w.hi[1], m.w.hi[2]};
       const vec3f size = hi - lo;
       const float maxWidth = reduce_max(size);
-      const int MC_GRID_SIZE = 200 + int(sqrtf(desc->numTets / 1000.f));
+      const int MC_GRID_SIZE = 200 + int(sqrtf(numElems / 1000.f));
       const float f = (MC_GRID_SIZE - 1) / maxWidth;
       m.dd.mx = 1 + (int)(size.x * f);
       m.dd.my = 1 + (int)(size.y * f);
@@ -455,7 +597,10 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       r.mx = m.dd.mx; r.my = m.dd.my; r.mz = m.dd.mz;
       r.origin = m.dd.mcOrigin;
       r.rcpSpacing = rcp(m.dd.mcSpacing);
-      k_raster_tets<<<div_up(desc->numTets, 128), 128, 0, s>>>(r, (const int4 *)desc->tets, v4.ptr, desc->numTets);
+      if (cells)
+        k_raster_cells<<<div_up(numElems, 128), 128, 0, s>>>(r, c, kinds.ptr, v4.ptr);
+      else
+        k_raster_tets<<<div_up(numElems, 128), 128, 0, s>>>(r, (const int4 *)desc->tets, v4.ptr, numElems);
       BN_HIP(hipGetLastError());
       k_finish_ranges<<<gs, 256, 0, s>>>(m.rangesOrd.ptr, m.ranges.ptr, ncell);
       BN_HIP(hipGetLastError());
@@ -464,7 +609,8 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.mcMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t1).count();
       m.dd.nodes = m.w.nodes.ptr;
       m.dd.tets = m.tets.ptr;
-      m.dd.numTets = (uint32_t)desc->numTets;
+      m.dd.elemTypes = cells ? m.types.ptr : nullptr;
+      m.dd.numTets = (uint32_t)numElems;
       m.dd.depth = m.w.depth;
       m.dd.ranges = m.ranges.ptr;
       m.dd.majorants = m.majorants.ptr;
@@ -531,7 +677,10 @@ extern "C" int bnamd_umesh_sample_points(const bnamd_umesh *mesh, const float *p
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_sample_points: BVH too deep for the LDS stack");
     hipStream_t s = (hipStream_t)stream;
-    k_umesh_sample<<<div_up(numPoints, 256), 256, lds, s>>>(mesh->m.dd, points, out, numPoints);
+    if (mesh->m.dd.elemTypes)
+      k_umesh_sample<true><<<div_up(numPoints, 256), 256, lds, s>>>(mesh->m.dd, points, out, numPoints);
+    else
+      k_umesh_sample<false><<<div_up(numPoints, 256), 256, lds, s>>>(mesh->m.dd, points, out, numPoints);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
   });
@@ -547,8 +696,11 @@ extern "C" int bnamd_umesh_iso_rays(const bnamd_umesh *mesh, const bnamd_ray *ra
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_iso_rays: BVH too deep for the LDS stack");
     hipStream_t s = (hipStream_t)stream;
-    k_umesh_iso_rays<<<div_up(numRays, 256), 256, lds, s>>>(iso_dd(mesh->m, isoValue, s), rays, isoValue,
-                                                            (float4 *)out, numRays);
+    const UMeshDD dd = iso_dd(mesh->m, isoValue, s);
+    if (dd.elemTypes)
+      k_umesh_iso_rays<true><<<div_up(numRays, 256), 256, lds, s>>>(dd, rays, isoValue, (float4 *)out, numRays);
+    else
+      k_umesh_iso_rays<false><<<div_up(numRays, 256), 256, lds, s>>>(dd, rays, isoValue, (float4 *)out, numRays);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
   });
@@ -582,7 +734,8 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     const int spp = params->pathsPerPixel;
     int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
     if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
-    if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    if (dd.elemTypes) launch_render_umesh<1, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else launch_render_umesh<1>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);

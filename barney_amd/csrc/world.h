@@ -14,6 +14,8 @@ constexpr int GEOM_TETS = 3;
 // Internal geometry kind for the top level of a two-level world:
 // vertices = float4 pairs (world box lo, hi) per instance.
 constexpr int GEOM_INSTANCES = 4;
+// element boxes of a general unstructured mesh ('vertices': lo, hi float4 pairs)
+constexpr int GEOM_ELEM_BOXES = 5;
 
 // Device-side geometry table entry (one per instanced geom).
 struct GeomDev {

@@ -366,3 +366,93 @@ def workload5(n=128, seed=0x5EED0004):
     cam = dict(position=pos, direction=(c - pos[0], c - pos[1], c - pos[2]), up=(0.0, 1.0, 0.0), fovy=45.0)
     return dict(vertices=verts, scalars=scal, tets=tets, tf=ramp_tf(), base_density=0.1, iso=0.5,
                 iso_color=(0.9, 0.75, 0.4), camera=cam, size=(1024, 1024), spp=8)
+
+
+# VTK hex corner order in the (x + 2y + 4z) corner numbering of tet_grid
+_HEX_VTK = (0, 1, 3, 2, 4, 5, 7, 6)
+# VTK hex faces (4 hex-local vertices each, cyclic)
+_HEX_FACES = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7))
+# 5-tet split of a VTK-ordered hex
+_HEX_TETS5 = ((0, 1, 3, 4), (1, 2, 3, 6), (1, 4, 5, 6), (3, 4, 6, 7), (1, 3, 4, 6))
+
+CELL_TET, CELL_HEX, CELL_PRISM, CELL_PYR = 10, 12, 13, 14
+
+
+def mixed_cells(n=10, seed=0x5EED0005, blobs=6):
+    """A conforming mixed mesh in UMeshField's cell layout: an n^3 lattice
+    of cells under a shear (an affine map, so every face stays planar); cell
+    (i, j, k) is, by (i + 2j + 3k) mod 4, one hex, two prisms (split through
+    the bottom / top diagonal), six pyramids on a center vertex, or five
+    positively oriented tets. Per-vertex scalars: Gaussian blobs in [0, 1].
+    Returns vertices (float32 [V,3]), scalars (float32 [V]) and
+    cells = (indices int32, offsets int32, types uint8)."""
+    m = n + 1
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 1.0, 0.15], [0.05, 0.0, 1.0]])
+    zz, yy, xx = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    G = np.stack([xx, yy, zz], -1).reshape(-1, 3).astype(np.float64)
+    ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    centers_g = np.stack([ci, cj, ck], -1) + 0.5
+    Pg = np.concatenate([G, centers_g])  # lattice points, then cell centers
+    P = Pg @ A.T
+    bc = uniform(seed, 2, 3 * blobs, 0.0, float(n)).reshape(-1, 3) @ A.T
+    sig = uniform(seed, 3, blobs, 0.15 * n, 0.4 * n)
+    amp = uniform(seed, 4, blobs, 0.5, 1.0)
+    s = np.zeros(len(P))
+    for b in range(blobs):
+        s += amp[b] * np.exp(-((P - bc[b]) ** 2).sum(1) / (2.0 * sig[b] ** 2))
+    s = (s - s.min()) / max(s.max() - s.min(), 1e-30)
+    corner = [(c & 1) + ((c >> 1) & 1) * m + ((c >> 2) & 1) * m * m for c in range(8)]
+    indices, offsets, types = [], [], []
+
+    def add(kind, vs):
+        offsets.append(len(indices))
+        types.append(kind)
+        indices.extend(int(v) for v in vs)
+
+    for c in range(n ** 3):
+        i, j, k = ci[c], cj[c], ck[c]
+        base = (k * m + j) * m + i
+        h = [base + corner[_HEX_VTK[q]] for q in range(8)]
+        kind = (i + 2 * j + 3 * k) % 4
+        if kind == 0:
+            add(CELL_HEX, h)
+        elif kind == 1:
+            add(CELL_PRISM, (h[0], h[1], h[2], h[4], h[5], h[6]))
+            add(CELL_PRISM, (h[0], h[2], h[3], h[4], h[6], h[7]))
+        elif kind == 2:
+            apex = m ** 3 + c
+            for f in _HEX_FACES:
+                add(CELL_PYR, [h[q] for q in f] + [apex])
+        else:
+            for t in _HEX_TETS5:
+                v = [h[q] for q in t]
+                a, b, cc, d = (P[x] for x in v)
+                if np.dot(np.cross(b - a, cc - a), d - a) < 0:
+                    v[2], v[3] = v[3], v[2]
+                add(CELL_TET, v)
+    return (P.astype(np.float32), s.astype(np.float32),
+            (np.array(indices, np.int32), np.array(offsets, np.int32), np.array(types, np.uint8)))
+
+
+def deformed_elements(seed=0x5EED0006, jitter=0.2, spacing=3.0):
+    """One pyramid, one prism and one hex, each a unit element with its
+    vertices moved by up to +-jitter (bilinear faces, non-affine maps), set
+    'spacing' apart along x so that no two overlap. Scalars: a smooth
+    nonlinear function of position. Same return layout as mixed_cells."""
+    unit = {CELL_PYR: [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0.5, 0.5, 1)],
+            CELL_PRISM: [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)],
+            CELL_HEX: [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]}
+    P, indices, offsets, types = [], [], [], []
+    for e, (kind, vs) in enumerate(unit.items()):
+        off = len(P)
+        jit = uniform(seed, 10 + e, 3 * len(vs), -jitter, jitter).reshape(-1, 3)
+        for q, v in enumerate(vs):
+            P.append(np.array(v, np.float64) + jit[q] + np.array([spacing * e, 0.0, 0.0]))
+        offsets.append(len(indices))
+        types.append(kind)
+        indices.extend(range(off, off + len(vs)))
+    P = np.array(P)
+    s = 0.5 + 0.3 * np.sin(1.3 * P[:, 0] + 0.4) * np.cos(0.9 * P[:, 1]) + 0.2 * P[:, 2] ** 2
+    return (P.astype(np.float32), s.astype(np.float32),
+            (np.array(indices, np.int32), np.array(offsets, np.int32), np.array(types, np.uint8)))

@@ -239,26 +239,42 @@ int bnamd_volume_iso_rays(const bnamd_volume *vol, const bnamd_ray *rays, float 
                           int numRays, void *stream);
 
 /* ------------------------------------------------------------------------
- * Unstructured tetrahedral volumes (barney/umesh): UMeshField with per-vertex
- * scalars, UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,
- * UMeshField.h:390-434), the macrocell grid of buildInitialMacroCells
- * (UMeshField.cu:39-200), MCVolumeAccel<UMeshCuBQLSampler> delta tracking and
+ * Unstructured volumes (barney/umesh): UMeshField with per-vertex scalars,
+ * UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,
+ * UMeshField.h:362-501 eltScalar for tets, pyramids, prisms and hexahedra),
+ * the macrocell grid of buildInitialMacroCells (UMeshField.cu:39-200),
+ * MCVolumeAccel<UMeshCuBQLSampler> delta tracking and
  * MCIsoSurfaceAccel<UMeshCuBQLSampler> iso-surfaces (MCAccelerator.h:338-708).
+ * A mesh is either all tets (tets != NULL) or general elements in
+ * UMeshField's layout (tets == NULL): cell c has type cellTypes[c] and its
+ * vertex indices at indices[cellOffsets[c] ...] in VTK order. Types are
+ * UMeshField's codes: VTK 10 tet, 12 hex, 13 prism (wedge), 14 pyramid, or
+ * ANARI 0 tet, 1 hex, 2 prism, 3 pyramid. Polyhedra are not supported.
  * --------------------------------------------------------------------- */
+#define BNAMD_CELL_TET 10
+#define BNAMD_CELL_HEX 12
+#define BNAMD_CELL_PRISM 13
+#define BNAMD_CELL_PYR 14
+
 typedef struct {
   const float *vertices; /* device float3 [numVertices] */
   const float *scalars;  /* device float [numVertices] (per-vertex data) */
   int numVertices;
   const int *tets;       /* device int4 [numTets]; positively oriented
                             (det(v1-v0, v2-v0, v3-v0) > 0), as tetScalar's
-                            inside test requires */
+                            inside test requires; NULL for general cells */
   int numTets;
+  const int *indices;            /* device [numIndices] (general cells) */
+  int numIndices;
+  const int *cellOffsets;        /* device [numCells] */
+  const unsigned char *cellTypes;/* device [numCells] */
+  int numCells;
 } bnamd_umesh_desc;
 
 typedef struct bnamd_umesh_s bnamd_umesh;
 
 typedef struct {
-  int numTets, numNodes, bvhDepth;
+  int numTets, numNodes, bvhDepth; /* numTets: elements (tets or cells) */
   int mcDims[3];
   float worldLower[3], worldUpper[3];
   float buildMs;   /* BVH build + tet packing */

@@ -1613,7 +1613,11 @@ long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const re
 struct ref_umesh_s {
   std::vector<float> v4;  // x, y, z, scalar per vertex
   std::vector<int> tets;  // 4 vertex indices per tet
-  ref_scene_s bvh;        // binned-SAH BVH over the tet boxes (prim = tet)
+  // general cells (UMeshField's layout) when 'cells' is set
+  bool cells = false;
+  std::vector<int> indices, offsets;
+  std::vector<int> kinds;  // 0 tet, 1 pyramid, 2 prism, 3 hex
+  ref_scene_s bvh;        // binned-SAH BVH over the element boxes (prim = element)
   ref_volume_s grid;      // macrocell grid + transfer function
 };
 
@@ -1622,8 +1626,7 @@ namespace {
 // UMeshField.h:346-352 evalToImplicitPlane + :390-434 tetScalar
 float implicit_plane(V3 P, V3 a, V3 b, V3 c) { return dot(sub(P, a), cross(sub(b, a), sub(c, a))); }
 
-bool tet_value(const ref_umesh_s *u, int t, V3 P, float &val) {
-  const int *ix = &u->tets[4 * (size_t)t];
+bool tet_value_ix(const ref_umesh_s *u, const int *ix, V3 P, float &val) {
   const float *a = &u->v4[4 * (size_t)ix[0]], *b = &u->v4[4 * (size_t)ix[1]];
   const float *c = &u->v4[4 * (size_t)ix[2]], *d = &u->v4[4 * (size_t)ix[3]];
   const V3 v0 = ld3(a), v1 = ld3(b), v2 = ld3(c), v3 = ld3(d);
@@ -1638,6 +1641,168 @@ bool tet_value(const ref_umesh_s *u, int t, V3 P, float &val) {
   const float scale = 1.f / (t0 + t1 + t2 + t3);
   val = scale * (t0 * a[3] + t1 * b[3] + t2 * c[3] + t3 * d[3]);
   return true;
+}
+
+bool tet_value(const ref_umesh_s *u, int t, V3 P, float &val) { return tet_value_ix(u, &u->tets[4 * (size_t)t], P, val); }
+
+// ElementIntersection.h (adapted there from OpenVKL): intersectPyrEXT,
+// intersectPrismEXT, intersectHexEXT. Newton iteration for the parametric
+// coordinates from (.5, .5, .5): at most 10 steps; converged when all three
+// updates are below 1e-4; diverged when a coordinate exceeds 1e6 in
+// magnitude; |det| below 1e-6 (hex: 1e-10 x the diagonal of the vertices'
+// box) rejects; inside within 1e-6 of [0, 1] (prism: also r + s); the value
+// uses the shape functions evaluated at the start of the last step.
+void pyr_shape(const float *pc, float *sf, float *d) {
+  const float rm = 1.f - pc[0], sm = 1.f - pc[1], tm = 1.f - pc[2];
+  sf[0] = rm * sm * tm;
+  sf[1] = pc[0] * sm * tm;
+  sf[2] = pc[0] * pc[1] * tm;
+  sf[3] = rm * pc[1] * tm;
+  sf[4] = pc[2];
+  d[0] = -(pc[1] - 1.f) * (pc[2] - 1.f);
+  d[1] = (pc[1] - 1.f) * (pc[2] - 1.f);
+  d[2] = pc[1] - pc[1] * pc[2];
+  d[3] = pc[1] * (pc[2] - 1.f);
+  d[4] = 0.f;
+  d[5] = -(pc[0] - 1.f) * (pc[2] - 1.f);
+  d[6] = pc[0] * (pc[2] - 1.f);
+  d[7] = pc[0] - pc[0] * pc[2];
+  d[8] = (pc[0] - 1.f) * (pc[2] - 1.f);
+  d[9] = 0.f;
+  d[10] = -(pc[0] - 1.f) * (pc[1] - 1.f);
+  d[11] = pc[0] * (pc[1] - 1.f);
+  d[12] = -pc[0] * pc[1];
+  d[13] = (pc[0] - 1.f) * pc[1];
+  d[14] = 1.f;
+}
+
+void prism_shape(const float *pc, float *sf, float *d) {
+  sf[0] = (1.f - pc[0] - pc[1]) * (1.f - pc[2]);
+  sf[1] = pc[0] * (1.f - pc[2]);
+  sf[2] = pc[1] * (1.f - pc[2]);
+  sf[3] = (1.f - pc[0] - pc[1]) * pc[2];
+  sf[4] = pc[0] * pc[2];
+  sf[5] = pc[1] * pc[2];
+  d[0] = -1.f + pc[2];
+  d[1] = 1.f - pc[2];
+  d[2] = 0.f;
+  d[3] = -pc[2];
+  d[4] = pc[2];
+  d[5] = 0.f;
+  d[6] = -1.f + pc[2];
+  d[7] = 0.f;
+  d[8] = 1.f - pc[2];
+  d[9] = -pc[2];
+  d[10] = 0.f;
+  d[11] = pc[2];
+  d[12] = -1.f + pc[0] + pc[1];
+  d[13] = -pc[0];
+  d[14] = -pc[1];
+  d[15] = 1.f - pc[0] - pc[1];
+  d[16] = pc[0];
+  d[17] = pc[1];
+}
+
+void hex_shape(const float *pc, float *sf, float *d) {
+  const float rm = 1.f - pc[0], sm = 1.f - pc[1], tm = 1.f - pc[2];
+  sf[0] = rm * sm * tm;
+  sf[1] = pc[0] * sm * tm;
+  sf[2] = pc[0] * pc[1] * tm;
+  sf[3] = rm * pc[1] * tm;
+  sf[4] = rm * sm * pc[2];
+  sf[5] = pc[0] * sm * pc[2];
+  sf[6] = pc[0] * pc[1] * pc[2];
+  sf[7] = rm * pc[1] * pc[2];
+  d[0] = -sm * tm;
+  d[1] = sm * tm;
+  d[2] = pc[1] * tm;
+  d[3] = -pc[1] * tm;
+  d[4] = -sm * pc[2];
+  d[5] = sm * pc[2];
+  d[6] = pc[1] * pc[2];
+  d[7] = -pc[1] * pc[2];
+  d[8] = -rm * tm;
+  d[9] = -pc[0] * tm;
+  d[10] = pc[0] * tm;
+  d[11] = rm * tm;
+  d[12] = -rm * pc[2];
+  d[13] = -pc[0] * pc[2];
+  d[14] = pc[0] * pc[2];
+  d[15] = rm * pc[2];
+  d[16] = -rm * sm;
+  d[17] = -pc[0] * sm;
+  d[18] = -pc[0] * pc[1];
+  d[19] = -rm * pc[1];
+  d[20] = rm * sm;
+  d[21] = pc[0] * sm;
+  d[22] = pc[0] * pc[1];
+  d[23] = rm * pc[1];
+}
+
+inline float det3(V3 x, V3 y, V3 z) { return dot(x, cross(y, z)); }
+
+// kind: 1 pyramid (5 vertices), 2 prism (6), 3 hex (8); V[i] -> x, y, z, s
+bool newton_value(int kind, const float *const *V, V3 P, float &value) {
+  const int N = kind == 1 ? 5 : kind == 2 ? 6 : 8;
+  float tol = 1e-6f;
+  if (kind == 3) {
+    V3 lo = ld3(V[0]), hi = lo;
+    for (int i = 1; i < N; ++i) {
+      lo = vmin(lo, ld3(V[i]));
+      hi = vmax(hi, ld3(V[i]));
+    }
+    const V3 e = sub(hi, lo);
+    tol = 1e-10f * std::sqrt(dot(e, e));
+  }
+  float pc[3] = {.5f, .5f, .5f};
+  float w[8], d[24];
+  bool converged = false;
+  for (int it = 0; !converged && it < 10; ++it) {
+    if (kind == 1) pyr_shape(pc, w, d);
+    else if (kind == 2) prism_shape(pc, w, d);
+    else hex_shape(pc, w, d);
+    V3 fcol = mk(0.f, 0.f, 0.f), rcol = fcol, scol = fcol, tcol = fcol;
+    for (int i = 0; i < N; ++i) {
+      const V3 pt = ld3(V[i]);
+      fcol = add(fcol, scl(w[i], pt));
+      rcol = add(rcol, scl(d[i], pt));
+      scol = add(scol, scl(d[i + N], pt));
+      tcol = add(tcol, scl(d[i + 2 * N], pt));
+    }
+    fcol = sub(fcol, P);
+    const float D = det3(rcol, scol, tcol);
+    if (std::fabs(D) < tol) return false;
+    const float d0 = det3(fcol, scol, tcol) / D;
+    const float d1 = det3(rcol, fcol, tcol) / D;
+    const float d2 = det3(rcol, scol, fcol) / D;
+    pc[0] = pc[0] - d0;
+    pc[1] = pc[1] - d1;
+    pc[2] = pc[2] - d2;
+    if (std::fabs(d0) < 1e-4f && std::fabs(d1) < 1e-4f && std::fabs(d2) < 1e-4f)
+      converged = true;
+    else if (std::fabs(pc[0]) > 1e6f || std::fabs(pc[1]) > 1e6f || std::fabs(pc[2]) > 1e6f)
+      return false;
+  }
+  if (!converged) return false;
+  const float lo = 0.f - 1e-6f, hi = 1.f + 1e-6f;
+  if (!(pc[0] >= lo && pc[0] <= hi && pc[1] >= lo && pc[1] <= hi && pc[2] >= lo && pc[2] <= hi)) return false;
+  if (kind == 2 && !(pc[0] + pc[1] <= hi)) return false;
+  float val = 0.f;
+  for (int i = 0; i < N; ++i) val += w[i] * V[i][3];
+  value = val;
+  return true;
+}
+
+// eltScalar (UMeshField.h:362-388)
+bool elem_value(const ref_umesh_s *u, int e, V3 P, float &val) {
+  if (!u->cells) return tet_value(u, e, P, val);
+  const int *ix = &u->indices[(size_t)u->offsets[e]];
+  const int k = u->kinds[e];
+  if (k == 0) return tet_value_ix(u, ix, P, val);
+  const float *V[8];
+  const int N = k == 1 ? 5 : k == 2 ? 6 : 8;
+  for (int i = 0; i < N; ++i) V[i] = &u->v4[4 * (size_t)ix[i]];
+  return newton_value(k, V, P, val);
 }
 
 // UMeshCuBQLSampler.h:51-67: box query with the degenerate box {P, P}; the
@@ -1657,7 +1822,7 @@ float um_sample(const ref_umesh_s *u, V3 P) {
     if (n.left < 0) {
       for (int i = 0; i < n.count; ++i) {
         float v;
-        if (tet_value(u, u->bvh.prims[n.first + i].prim, P, v)) return v;
+        if (elem_value(u, u->bvh.prims[n.first + i].prim, P, v)) return v;
       }
     } else {
       stack[sp++] = n.left + 1;
@@ -1876,27 +2041,73 @@ HitInfo um_trace(const ref_umesh_s *u, bool vol, bool iso, float isoValue, V3 is
 
 extern "C" {
 
-ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int numVertices, const int *tets, int numTets) {
-  auto *u = new ref_umesh_s;
-  u->v4.resize(4 * (size_t)numVertices);
-  for (int i = 0; i < numVertices; ++i) {
-    for (int k = 0; k < 3; ++k) u->v4[4 * (size_t)i + k] = vertices[3 * (size_t)i + k];
-    u->v4[4 * (size_t)i + 3] = scalars[i];
+}  // extern "C"
+
+namespace {
+
+// vertex indices and count of element e
+void um_elem(const ref_umesh_s *u, int e, const int *&ix, int &nv) {
+  if (!u->cells) {
+    ix = &u->tets[4 * (size_t)e];
+    nv = 4;
+    return;
   }
-  u->tets.assign(tets, tets + 4 * (size_t)numTets);
-  // element boxes (UMeshField cellBounds) and world bounds
+  ix = &u->indices[(size_t)u->offsets[e]];
+  const int k = u->kinds[e];
+  nv = k == 0 ? 4 : k == 1 ? 5 : k == 2 ? 6 : 8;
+}
+
+// rasterBox(grid, worldBounds, box4f) (MCGrid.cuh:67-91): cells other than
+// tets; only an inverted box is skipped
+void raster_box_cells(ref_volume_s *g, const int *ix, int nv, const ref_umesh_s *u) {
+  float lo[4], hi[4];
+  for (int k = 0; k < 4; ++k) {
+    lo[k] = kInf;
+    hi[k] = -kInf;
+  }
+  for (int i = 0; i < nv; ++i) {
+    const float *c = &u->v4[4 * (size_t)ix[i]];
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = std::fmin(lo[k], c[k]);
+      hi[k] = std::fmax(hi[k], c[k]);
+    }
+  }
+  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
+  const float org[3] = {g->mcOrigin.x, g->mcOrigin.y, g->mcOrigin.z};
+  const float sp[3] = {g->mcSpacing.x, g->mcSpacing.y, g->mcSpacing.z};
+  int a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = std::min(std::max((int)((lo[k] - org[k]) * (1.f / sp[k])), 0), g->m[k] - 1);
+    b[k] = std::min(std::max((int)((hi[k] - org[k]) * (1.f / sp[k])), 0), g->m[k] - 1);
+  }
+  for (int z = a[2]; z <= b[2]; ++z)
+    for (int y = a[1]; y <= b[1]; ++y)
+      for (int x = a[0]; x <= b[0]; ++x) {
+        const size_t id = (size_t)x + (size_t)g->m[0] * ((size_t)y + (size_t)g->m[1] * z);
+        atomic_fmin(&g->rlo[id], lo[3]);
+        atomic_fmax(&g->rhi[id], hi[3]);
+      }
+}
+
+// element boxes (UMeshField cellBounds), BVH, and the macrocell grid of
+// buildInitialMacroCells (UMeshField.cu:110-200): tets by rasterTet<5>,
+// other cells by their box
+ref_umesh_s *um_build(ref_umesh_s *u, int numElems) {
   V3 wlo = mk(kInf, kInf, kInf), whi = mk(-kInf, -kInf, -kInf);
-  for (int t = 0; t < numTets; ++t) {
+  for (int e = 0; e < numElems; ++e) {
     Prim pr;
     pr.geom = 0;
-    pr.prim = t;
-    pr.key = (uint32_t)t;
+    pr.prim = e;
+    pr.key = (uint32_t)e;
     for (int k = 0; k < 3; ++k) {
       pr.lo[k] = kInf;
       pr.hi[k] = -kInf;
     }
-    for (int q = 0; q < 4; ++q) {
-      const float *p = &u->v4[4 * (size_t)tets[4 * (size_t)t + q]];
+    const int *ix;
+    int nv;
+    um_elem(u, e, ix, nv);
+    for (int q = 0; q < nv; ++q) {
+      const float *p = &u->v4[4 * (size_t)ix[q]];
       for (int k = 0; k < 3; ++k) {
         pr.lo[k] = std::fmin(pr.lo[k], p[k]);
         pr.hi[k] = std::fmax(pr.hi[k], p[k]);
@@ -1907,13 +2118,12 @@ ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int num
     u->bvh.prims.push_back(pr);
   }
   build_bvh(&u->bvh);
-  // buildInitialMacroCells (UMeshField.cu:173-200)
   ref_volume_s &g = u->grid;
   g.wlo = wlo;
   g.whi = whi;
   const V3 size = sub(whi, wlo);
   const float maxWidth = maxc(size);
-  const int MC = 200 + int(std::sqrt(numTets / 1000.f));
+  const int MC = 200 + int(std::sqrt(numElems / 1000.f));
   const float f = (MC - 1) / maxWidth;
   g.m[0] = 1 + (int)(size.x * f);
   g.m[1] = 1 + (int)(size.y * f);
@@ -1924,18 +2134,71 @@ ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int num
   g.rlo.assign(nc, kInf);
   g.rhi.assign(nc, -kInf);
   g.maj.assign(nc, 0.f);
-  parallel_for(numTets, default_threads(0), [&](int t) {
-    P4 p[4];
-    for (int q = 0; q < 4; ++q) {
-      const float *v = &u->v4[4 * (size_t)tets[4 * (size_t)t + q]];
-      p[q] = P4{v[0], v[1], v[2], v[3]};
+  parallel_for(numElems, default_threads(0), [&](int e) {
+    const int *ix;
+    int nv;
+    um_elem(u, e, ix, nv);
+    if (nv == 4) {
+      P4 p[4];
+      for (int q = 0; q < 4; ++q) {
+        const float *v = &u->v4[4 * (size_t)ix[q]];
+        p[q] = P4{v[0], v[1], v[2], v[3]};
+      }
+      raster_tet(&g, 5, p[0], p[1], p[2], p[3]);
+    } else {
+      raster_box_cells(&g, ix, nv, u);
     }
-    raster_tet(&g, 5, p[0], p[1], p[2], p[3]);
   });
   const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
   g.tf.assign(one, one + 8);
   g.numTF = 2;
   return u;
+}
+
+void um_vertices(ref_umesh_s *u, const float *vertices, const float *scalars, int numVertices) {
+  u->v4.resize(4 * (size_t)numVertices);
+  for (int i = 0; i < numVertices; ++i) {
+    for (int k = 0; k < 3; ++k) u->v4[4 * (size_t)i + k] = vertices[3 * (size_t)i + k];
+    u->v4[4 * (size_t)i + 3] = scalars[i];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int numVertices, const int *tets, int numTets) {
+  auto *u = new ref_umesh_s;
+  um_vertices(u, vertices, scalars, numVertices);
+  u->tets.assign(tets, tets + 4 * (size_t)numTets);
+  return um_build(u, numTets);
+}
+
+ref_umesh *ref_umesh_create_cells(const float *vertices, const float *scalars, int numVertices, const int *indices,
+                                  int numIndices, const int *offsets, const unsigned char *types, int numCells) {
+  auto *u = new ref_umesh_s;
+  u->cells = true;
+  um_vertices(u, vertices, scalars, numVertices);
+  u->indices.assign(indices, indices + numIndices);
+  u->offsets.assign(offsets, offsets + numCells);
+  u->kinds.resize(numCells);
+  for (int c = 0; c < numCells; ++c) {
+    int k = -1, nv = 0;
+    switch (types[c]) {  // UMeshField's VTK and ANARI codes
+      case 10: case 0: k = 0; nv = 4; break;
+      case 14: case 3: k = 1; nv = 5; break;
+      case 13: case 2: k = 2; nv = 6; break;
+      case 12: case 1: k = 3; nv = 8; break;
+    }
+    bool ok = k >= 0 && offsets[c] >= 0 && (long long)offsets[c] + nv <= numIndices;
+    for (int q = 0; ok && q < nv; ++q) ok = indices[offsets[c] + q] >= 0 && indices[offsets[c] + q] < numVertices;
+    if (!ok) {
+      delete u;
+      return nullptr;
+    }
+    u->kinds[c] = k;
+  }
+  return um_build(u, numCells);
 }
 
 void ref_umesh_destroy(ref_umesh *u) { delete u; }

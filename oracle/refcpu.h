@@ -127,11 +127,16 @@ int ref_dda_cells(const float *org, const float *dir, float tMax, const int *gri
 long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p,
                             float *accum, int y0, int y1, int threads, long long *samples);
 
-/* --- unstructured tetrahedral volume (barney/umesh) --- */
+/* --- unstructured volume (barney/umesh) --- */
 typedef struct ref_umesh_s ref_umesh;
 /* vertices float3, per-vertex scalars, tets int4 (positively oriented) */
 ref_umesh *ref_umesh_create(const float *vertices, const float *scalars, int numVertices, const int *tets,
                             int numTets);
+/* general cells in UMeshField's layout: types VTK 10/12/13/14 or ANARI
+ * 0-3 (tet, hex, prism, pyramid), vertex indices at indices[offsets[c]..]
+ * in VTK order; NULL for an unknown type or an index out of range */
+ref_umesh *ref_umesh_create_cells(const float *vertices, const float *scalars, int numVertices, const int *indices,
+                                  int numIndices, const int *offsets, const unsigned char *types, int numCells);
 void ref_umesh_destroy(ref_umesh *u);
 void ref_umesh_set_tf(ref_umesh *u, const float *values, int numValues, float lo, float hi, float baseDensity);
 void ref_umesh_mc_dims(const ref_umesh *u, int *dims);

@@ -85,6 +85,7 @@ def load():
         "ref_world_destroy": (None, [P]),
         "ref_world_trace": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_umesh_create": (P, [P, P, C.c_int, P, C.c_int]),
+        "ref_umesh_create_cells": (P, [P, P, C.c_int, P, C.c_int, P, P, C.c_int]),
         "ref_umesh_destroy": (None, [P]),
         "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
         "ref_umesh_mc_dims": (None, [P, P]),
@@ -279,14 +280,25 @@ class Volume:
 
 
 class UMesh:
-    """Oracle tetrahedral mesh: binned-SAH BVH over tet boxes, macrocell grid
-    by rasterTet<5>, transfer function, point sampling and render."""
+    """Oracle unstructured mesh: binned-SAH BVH over element boxes, macrocell
+    grid (rasterTet<5> for tets, element boxes otherwise), transfer function,
+    point sampling and render. Either tets (int4, positively oriented) or
+    general cells: cells = (indices, offsets, types) in UMeshField's layout."""
 
-    def __init__(self, vertices, scalars, tets):
+    def __init__(self, vertices, scalars, tets=None, cells=None):
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         sc = np.ascontiguousarray(scalars, np.float32).reshape(-1)
-        t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
-        self.handle = load().ref_umesh_create(v.ctypes.data, sc.ctypes.data, len(v), t.ctypes.data, len(t))
+        if cells is None:
+            t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+            self.handle = load().ref_umesh_create(v.ctypes.data, sc.ctypes.data, len(v), t.ctypes.data, len(t))
+        else:
+            ix = np.ascontiguousarray(cells[0], np.int32).reshape(-1)
+            off = np.ascontiguousarray(cells[1], np.int32).reshape(-1)
+            ty = np.ascontiguousarray(cells[2], np.uint8).reshape(-1)
+            self.handle = load().ref_umesh_create_cells(v.ctypes.data, sc.ctypes.data, len(v), ix.ctypes.data,
+                                                        len(ix), off.ctypes.data, ty.ctypes.data, len(off))
+            if not self.handle:
+                raise ValueError("ref_umesh_create_cells: unknown cell type or index out of range")
 
     def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
         values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
