@@ -312,6 +312,50 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
             "cpu_baseline": umesh_cpu_baseline(w5, cpu_seconds) if cpu else None}
 
 
+def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
+    """General unstructured cells (not a BASELINE workload; the same path as
+    workload 5 with pyramids, prisms and hexahedra next to tets): the sheared
+    mixed mesh of scenes.mixed_cells on an n^3 lattice, delta tracking plus
+    an iso-surface at 0.5, 512x512, 4 paths per pixel."""
+    from barney_amd import scenes
+    from oracle import refcpu
+    t0 = time.time()
+    V, S, cells = scenes.mixed_cells(n)
+    m = api.UMesh(V, S, cells=cells)
+    tf = scenes.ramp_tf()
+    m.set_tf(tf, (0.0, 1.0), 0.3)
+    info = m.info()
+    log(f"mixed cells built in {time.time() - t0:.1f}s ({info['numTets']} cells)")
+    W, H, spp = 512, 512, 4
+    c = n / 2.0
+    pos = (c - 1.1 * n, c + 0.8 * n, c - 1.7 * n)
+    camargs = dict(position=pos, direction=(c - pos[0], c - pos[1], c - pos[2]), fovy=40.0)
+    p = api.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1))
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    cam = api.camera(**camargs)
+    m.render(cam, p, volume=True, iso=0.5, accum=acc)
+    ms, st = [], None
+    for _ in range(steps):
+        o = m.render(cam, p, volume=True, iso=0.5, accum=acc)
+        st = o["stats"]
+        ms.append(st["ms"])
+    t = float(np.median(ms))
+    log(f"mixed cells: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M field samples")
+    res = {"cells": info["numTets"], "types": {int(k): int(v) for k, v in zip(*np.unique(cells[2], return_counts=True))},
+           "ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
+           "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"])}
+    if cpu:
+        r = refcpu.UMesh(V, S, cells=cells)
+        r.set_tf(tf, (0.0, 1.0), 0.3)
+        rp = refcpu.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1))
+        rcam = refcpu.camera(**camargs)
+        racc = np.zeros((H, W, 4), np.float32)
+        res["cpu_baseline"] = band_timer(
+            lambda y0, y1: r.render(rcam, rp, volume=True, iso=0.5, y0=y0, y1=y1, threads=cpu_threads(),
+                                    accum=racc)[2], H, cpu_seconds, "Msamples/s", f"the {W}x{H}x{spp}spp frame")
+    return res
+
+
 def fb_bench(api, reps=20):
     """Frame readback of a 3840x2160 frame (8160 tiles of 32x32): tiled
     accumulation -> linear float4 + normals, -> sRGB RGBA8, and the resolve of
@@ -480,6 +524,7 @@ def main():
         out["extra"]["batch_workload3"] = batch_bench(api, cpu=not args.no_cpu)
     if not args.no_umesh:
         out["extra"]["umesh_workload5"] = umesh_bench(api, cpu=not args.no_cpu)
+        out["extra"]["umesh_mixed_cells"] = umesh_cells_bench(api, cpu=not args.no_cpu)
     if not args.no_fb:
         out["extra"]["readback_3840x2160"] = fb_bench(api)
     if not args.no_ea:
