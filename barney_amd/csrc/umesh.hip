@@ -734,7 +734,10 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     const int spp = params->pathsPerPixel;
     int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
     if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
-    if (dd.elemTypes) launch_render_umesh<1, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    if (dd.elemTypes) {  // general cells: 4 lanes per pixel from 4 spp, else 1
+      if (lanes >= 4) launch_render_umesh<4, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+      else launch_render_umesh<1, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
+    }
     else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
     else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);

@@ -82,10 +82,11 @@ def _cam(mod):
     return mod.camera(pos, (c - pos[0], c - pos[1], c - pos[2]), fovy=40.0)
 
 
-@pytest.mark.parametrize("volume,iso", [(True, None), (False, 0.5)])
-def test_mixed_renders_per_pixel(mixed, gpu, volume, iso):
+@pytest.mark.parametrize("volume,iso,spp", [(True, None, 4), (False, 0.5, 4), (True, 0.5, 1)])
+def test_mixed_renders_per_pixel(mixed, gpu, volume, iso, spp):
+    """4 spp: four lanes per pixel; 1 spp: one."""
     V, S, cells, g, r = mixed
-    W, H, spp = 80, 60, 4
+    W, H = 80, 60
     pg = gpu.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=0)
     pr = refcpu.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=0)
     col = (0.9, 0.7, 0.4)
