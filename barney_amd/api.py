@@ -343,6 +343,70 @@ class Volume:
             pass
 
 
+class Amr:
+    """Block-structured AMR volume (BlockStructuredField's layout): blocks
+    (origins, dims int3; levels; offsets of their first scalar), per-level
+    refinements, scalars; macrocell grid, transfer function, point samples
+    and the delta-tracked render."""
+
+    def __init__(self, origins, dims, levels, offsets, refinements, scalars):
+        lib = init()
+        self._o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        self._d = np.ascontiguousarray(dims, dtype=np.int32).reshape(-1, 3)
+        self._l = np.ascontiguousarray(levels, dtype=np.int32).reshape(-1)
+        self._off = np.ascontiguousarray(offsets, dtype=np.uint64).reshape(-1)
+        self._r = np.ascontiguousarray(refinements, dtype=np.int32).reshape(-1)
+        sc = np.ascontiguousarray(scalars, dtype=np.float32).reshape(-1)
+        self.scalars = DeviceArray(sc)  # the field keeps the caller's device scalars
+        d = capi.bnamd_amr_desc()
+        d.numBlocks = len(self._o)
+        d.origins, d.dims, d.levels = self._o.ctypes.data, self._d.ctypes.data, self._l.ctypes.data
+        d.offsets, d.numLevels, d.refinements = self._off.ctypes.data, len(self._r), self._r.ctypes.data
+        d.scalars, d.numScalars = self.scalars.addr, len(sc)
+        self.handle = C.c_void_p()
+        check(lib.bnamd_amr_create(C.byref(d), None, C.byref(self.handle)))
+
+    def info(self):
+        i = capi.bnamd_amr_info()
+        check(capi.load().bnamd_amr_info_get(self.handle, C.byref(i)))
+        return {"numBlocks": i.numBlocks, "mcDims": tuple(i.mcDims), "lookupDims": tuple(i.lookupDims),
+                "lower": list(i.worldLower), "upper": list(i.worldUpper), "buildMs": i.buildMs}
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 4)
+        check(capi.load().bnamd_amr_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]),
+                                           float(domain[1]), float(base_density)))
+
+    def macrocells(self):
+        mx, my, mz = self.info()["mcDims"]
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        check(capi.load().bnamd_amr_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data))
+        return ranges, maj
+
+    def sample(self, points):
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        dp = DeviceArray(p)
+        do = DeviceArray.empty((len(p),), np.float32)
+        check(capi.load().bnamd_amr_sample_points(self.handle, dp.addr, do.addr, len(p), None))
+        return do.numpy()
+
+    def render(self, cam, params, accum=None):
+        acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+        st = capi.bnamd_volume_stats()
+        check(capi.load().bnamd_amr_render(self.handle, C.byref(cam), C.byref(params), acc.addr, None, C.byref(st)))
+        return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
+                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value:
+                capi.load().bnamd_amr_destroy(self.handle)
+            self.handle = None
+        except Exception:
+            pass
+
+
 class UMesh:
     """Unstructured volume (per-vertex scalars): point-location BVH,
     macrocell grid, transfer function; volume and iso-surface render. Either

@@ -59,6 +59,17 @@ class bnamd_structured_desc(C.Structure):
                 ("gridSpacing", C.c_float * 3), ("cellsPerMC", C.c_int)]
 
 
+class bnamd_amr_desc(C.Structure):
+    _fields_ = [("numBlocks", C.c_int), ("origins", C.c_void_p), ("dims", C.c_void_p), ("levels", C.c_void_p),
+                ("offsets", C.c_void_p), ("numLevels", C.c_int), ("refinements", C.c_void_p),
+                ("scalars", C.c_void_p), ("numScalars", C.c_ulonglong)]
+
+
+class bnamd_amr_info(C.Structure):
+    _fields_ = [("numBlocks", C.c_int), ("mcDims", C.c_int * 3), ("lookupDims", C.c_int * 3),
+                ("worldLower", C.c_float * 3), ("worldUpper", C.c_float * 3), ("buildMs", C.c_float)]
+
+
 class bnamd_volume_stats(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("ddaSteps", C.c_longlong), ("raysTraced", C.c_longlong),
                 ("ms", C.c_float)]
@@ -127,6 +138,14 @@ SIGNATURES = {
     "bnamd_volume_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bnamd_volume_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
                                       C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
+    "bnamd_amr_create": (C.c_int, [C.POINTER(bnamd_amr_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bnamd_amr_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_amr_info_get": (C.c_int, [C.c_void_p, C.POINTER(bnamd_amr_info)]),
+    "bnamd_amr_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "bnamd_amr_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_amr_sample_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_amr_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                   C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
     "bnamd_volume_sample_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                            C.c_void_p]),
     "bnamd_umesh_create": (C.c_int, [C.POINTER(bnamd_umesh_desc), C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -4,6 +4,11 @@
 // SCI_VIS_MODE shade path (barney/kernels/shadeRays.cu:24-26, 787-790).
 #include "shade.h"
 #include "volume.h"
+#include "amr.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
 
 namespace bnamd {
 
@@ -43,7 +48,8 @@ __global__ void k_mc_ranges(VolumeDD v, int cpm, float2 *__restrict ranges) {
   ranges[tid] = make_float2(lo, hi);
 }
 
-__global__ void k_majorants(VolumeDD v, const float2 *__restrict ranges, float *__restrict maj) {
+template <class F>
+__global__ void k_majorants(F v, const float2 *__restrict ranges, float *__restrict maj) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= v.mx * v.my * v.mz) return;
   const float2 r = ranges[tid];
@@ -105,10 +111,14 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long
 // (<= 168 VGPRs) measured 8.8 ms per 1920x1080x16spp frame against 12.4 ms
 // at the two that the unconstrained allocation (178 VGPRs) leaves; the cap
 // costs a few dwords of spill in the shading code.
-template <int PB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_render_volume(VolumeDD v, CameraDD cam, RenderDD rp, int spp,
-                                                       float4 *__restrict accum,
-                                                       unsigned long long *__restrict counters) {
+//
+// F is the field: VolumeDD (structured data) or AmrDD (block-structured
+// AMR); both carry the macrocell grid and transfer function, and
+// sample_and_map(F, P) gives the mapped sample.
+template <int PB, class F>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
+                unsigned long long *__restrict counters) {
   constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
   static_assert(PB >= 1 && PB <= kPoolPaths && PB <= 1024, "a pool holds at least one path per pixel");
@@ -317,8 +327,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   wave_add_u64((unsigned long long)nrays, counters + 2);
 }
 
-template <int PB>
-void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
+template <int PB, class F>
+void launch_render_volume(const F &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                           unsigned long long *counters, hipStream_t s) {
   constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
   constexpr int TH = PB / TW;
@@ -326,12 +336,13 @@ void launch_render_volume(const VolumeDD &v, const CameraDD &cam, const RenderDD
   // xcd_remap_v hands XCD x the tiles [x*per, (x+1)*per): the grid must
   // hold 8*per workgroups (a multiple of 8) to cover every tile
   const int grid = ((tiles + 7) / 8) * 8;
-  k_render_volume<PB><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
+  k_render_volume<PB, F><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
 }
 
 // Pixels per workgroup: the largest power of two up to 1024 (a 32x32 tile)
 // whose paths fit the pool (all of a pixel's paths in one pool), at least one.
-void render_volume_dispatch(const VolumeDD &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
+template <class F>
+void render_volume_dispatch(const F &v, const CameraDD &cam, const RenderDD &rp, int spp, float4 *accum,
                             unsigned long long *counters, hipStream_t s) {
   int pb = 1024;
   while (pb > 1 && (long long)pb * spp > kPoolPaths) pb >>= 1;
@@ -535,5 +546,302 @@ extern "C" int bnamd_volume_iso_rays(const bnamd_volume *vol, const bnamd_ray *r
     k_volume_iso_rays<<<div_up(numRays, 256), 256, 0, s>>>(d, g, rays, isoValue, (float4 *)out, numRays);
     BN_HIP(hipGetLastError());
     if (!stream) BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ===========================================================================
+// Block-structured AMR (barney/amr/BlockStructuredField.{h,cu},
+// BlockStructuredCuBQLSampler.h): blocks of cells at levels; a level-L cell
+// is (int)powf(refinement[L], L) units wide.
+// ===========================================================================
+namespace bnamd {
+
+struct Amr {
+  AmrDD dd{};
+  int numBlocks = 0;
+  int lookup[3] = {0, 0, 0};
+  DevBuf<float4> dom;
+  DevBuf<int4> dims, origins;
+  DevBuf<unsigned long long> offsets;
+  DevBuf<uint32_t> cellStart, cellBlocks;
+  DevBuf<float2> ranges;
+  DevBuf<float> majorants;
+  DevBuf<float4> tf;
+  bool haveTF = false;
+  float buildMs = 0.f;
+};
+
+namespace {
+
+__device__ __forceinline__ uint32_t amr_ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void k_amr_clear(uint2 *__restrict r, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    r[i] = make_uint2(amr_ord(BN_INF), amr_ord(-BN_INF));
+}
+
+// BSField_rasterGrids (BlockStructuredField.cu:33-57): every cell's basis
+// support, cellBounds = [(origin + cell - 1/2) cs, + 2 cs], with its scalar,
+// through rasterBox (MCGrid.cuh:67-91). One workgroup per block.
+__global__ void k_amr_raster(AmrDD a, const int4 *__restrict origins, uint2 *__restrict ranges, int numBlocks,
+                             vec3f rcpSpacing) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x;
+  if (b >= numBlocks) return;
+  const float cs = a.dom[2 * b].w;
+  const int4 o = origins[b], d = a.dims[b];
+  const float *sc = a.scalars + a.offsets[b];
+  const int n = d.x * d.y * d.z;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    const int x = c % d.x, y = (c / d.x) % d.y, z = c / (d.x * d.y);
+    const uint32_t s = amr_ord(sc[c]);
+    const vec3f lo = {((float)(o.x + x) - .5f) * cs, ((float)(o.y + y) - .5f) * cs, ((float)(o.z + z) - .5f) * cs};
+    const vec3f hi = {lo.x + 2.f * cs, lo.y + 2.f * cs, lo.z + 2.f * cs};
+    const int x0 = min(max((int)((lo.x - a.mcOrigin.x) * rcpSpacing.x), 0), a.mx - 1);
+    const int y0 = min(max((int)((lo.y - a.mcOrigin.y) * rcpSpacing.y), 0), a.my - 1);
+    const int z0 = min(max((int)((lo.z - a.mcOrigin.z) * rcpSpacing.z), 0), a.mz - 1);
+    const int x1 = min(max((int)((hi.x - a.mcOrigin.x) * rcpSpacing.x), 0), a.mx - 1);
+    const int y1 = min(max((int)((hi.y - a.mcOrigin.y) * rcpSpacing.y), 0), a.my - 1);
+    const int z1 = min(max((int)((hi.z - a.mcOrigin.z) * rcpSpacing.z), 0), a.mz - 1);
+    for (int iz = z0; iz <= z1; ++iz)
+      for (int iy = y0; iy <= y1; ++iy)
+        for (int ix = x0; ix <= x1; ++ix) {
+          uint2 *cell = ranges + ix + (size_t)iy * a.mx + (size_t)iz * a.mx * a.my;
+          atomicMin(&cell->x, s);
+          atomicMax(&cell->y, s);
+        }
+  }
+}
+
+__global__ void k_amr_finish(const uint2 *__restrict in, float2 *__restrict out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const uint2 u = in[i];
+    auto f = [](uint32_t v) { return __uint_as_float((v & 0x80000000u) ? (v & 0x7fffffffu) : ~v); };
+    out[i] = make_float2(f(u.x), f(u.y));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_amr_sample(AmrDD a, const float *__restrict pts, float *__restrict out,
+                                                    int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = amr_sample(a, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+}
+
+}  // namespace
+}  // namespace bnamd
+
+struct bnamd_amr_s {
+  Amr a;
+};
+
+extern "C" int bnamd_amr_create(const bnamd_amr_desc *desc, void *stream, bnamd_amr **out) {
+  return guarded([&] {
+    BN_REQUIRE(desc && out, "bnamd_amr_create: NULL argument");
+    BN_REQUIRE(desc->numBlocks > 0 && desc->numLevels > 0, "bnamd_amr_create: no blocks or levels");
+    BN_REQUIRE(desc->origins && desc->dims && desc->levels && desc->offsets && desc->refinements && desc->scalars,
+               "bnamd_amr_create: NULL array");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = (hipStream_t)stream;
+    auto *ua = new bnamd_amr_s;
+    try {
+      Amr &A = ua->a;
+      const int nb = desc->numBlocks;
+      A.numBlocks = nb;
+      std::vector<float4> dom(2 * (size_t)nb);
+      std::vector<int4> dims(nb), origins(nb);
+      vec3f wlo = vec3f(BN_INF), whi = vec3f(-BN_INF);
+      for (int b = 0; b < nb; ++b) {
+        const int *o = desc->origins + 3 * (size_t)b, *d = desc->dims + 3 * (size_t)b;
+        const int L = desc->levels[b];
+        BN_REQUIRE(L >= 0 && L < desc->numLevels, "bnamd_amr_create: block level out of range");
+        BN_REQUIRE(d[0] > 0 && d[1] > 0 && d[2] > 0, "bnamd_amr_create: empty block");
+        BN_REQUIRE(desc->offsets[b] + (unsigned long long)d[0] * d[1] * d[2] <= desc->numScalars,
+                   "bnamd_amr_create: block scalars out of range");
+        // Block::getFrom / getDomain (BlockStructuredField.h:227-248)
+        const float cs = (float)(int)(powf((float)desc->refinements[L], (float)L));
+        const vec3f lo = {((float)o[0] - .5f) * cs, ((float)o[1] - .5f) * cs, ((float)o[2] - .5f) * cs};
+        const vec3f hi = {((float)(o[0] + d[0]) + .5f) * cs, ((float)(o[1] + d[1]) + .5f) * cs,
+                          ((float)(o[2] + d[2]) + .5f) * cs};
+        dom[2 * b] = make_float4(lo.x, lo.y, lo.z, cs);
+        dom[2 * b + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+        dims[b] = make_int4(d[0], d[1], d[2], 0);
+        origins[b] = make_int4(o[0], o[1], o[2], 0);
+        wlo = vmin(wlo, lo);
+        whi = vmax(whi, hi);
+      }
+      AmrDD &dd = A.dd;
+      dd.worldLo = wlo;
+      dd.worldHi = whi;
+      const vec3f size = whi - wlo;
+      const float maxWidth = reduce_max(size);
+      // buildMCs (BlockStructuredField.cu:82-105): 256 cells along the widest axis
+      {
+        const float f = (256 - 1) / maxWidth;
+        dd.mx = 1 + (int)(size.x * f);
+        dd.my = 1 + (int)(size.y * f);
+        dd.mz = 1 + (int)(size.z * f);
+        dd.mcOrigin = wlo;
+        dd.mcSpacing = size * rcp(vec3f((float)dd.mx, (float)dd.my, (float)dd.mz));
+      }
+      // block lookup grid (in place of the reference's BVH over block domains)
+      {
+        const int G = std::min(256, std::max(16, 4 * (int)std::ceil(std::cbrt((double)nb))));
+        const float f = (G - 1) / maxWidth;
+        dd.gx = 1 + (int)(size.x * f);
+        dd.gy = 1 + (int)(size.y * f);
+        dd.gz = 1 + (int)(size.z * f);
+        dd.gOrigin = wlo;
+        const vec3f gsp = size * rcp(vec3f((float)dd.gx, (float)dd.gy, (float)dd.gz));
+        dd.gRcp = rcp(gsp);
+        A.lookup[0] = dd.gx; A.lookup[1] = dd.gy; A.lookup[2] = dd.gz;
+        const size_t ng = (size_t)dd.gx * dd.gy * dd.gz;
+        std::vector<uint32_t> start(ng + 1, 0u);
+        std::vector<int> r(6 * (size_t)nb);
+        auto cell = [](float p, float o, float rc, int n) { return std::min(std::max((int)((p - o) * rc), 0), n - 1); };
+        for (int b = 0; b < nb; ++b) {
+          const float4 lo = dom[2 * b], hi = dom[2 * b + 1];
+          int *q = &r[6 * (size_t)b];
+          q[0] = cell(lo.x, wlo.x, dd.gRcp.x, dd.gx); q[3] = cell(hi.x, wlo.x, dd.gRcp.x, dd.gx);
+          q[1] = cell(lo.y, wlo.y, dd.gRcp.y, dd.gy); q[4] = cell(hi.y, wlo.y, dd.gRcp.y, dd.gy);
+          q[2] = cell(lo.z, wlo.z, dd.gRcp.z, dd.gz); q[5] = cell(hi.z, wlo.z, dd.gRcp.z, dd.gz);
+          for (int z = q[2]; z <= q[5]; ++z)
+            for (int y = q[1]; y <= q[4]; ++y)
+              for (int x = q[0]; x <= q[3]; ++x) ++start[1 + x + (size_t)dd.gx * (y + (size_t)dd.gy * z)];
+        }
+        for (size_t c = 0; c < ng; ++c) start[c + 1] += start[c];
+        std::vector<uint32_t> list(start[ng]), fill(start.begin(), start.end() - 1);
+        for (int b = 0; b < nb; ++b) {  // ascending block ids per cell
+          const int *q = &r[6 * (size_t)b];
+          for (int z = q[2]; z <= q[5]; ++z)
+            for (int y = q[1]; y <= q[4]; ++y)
+              for (int x = q[0]; x <= q[3]; ++x) list[fill[x + (size_t)dd.gx * (y + (size_t)dd.gy * z)]++] = b;
+        }
+        A.cellStart.upload(start.data(), start.size(), s);
+        A.cellBlocks.upload(list.data(), std::max<size_t>(list.size(), 1), s);
+      }
+      A.dom.upload(dom.data(), dom.size(), s);
+      A.dims.upload(dims.data(), dims.size(), s);
+      A.origins.upload(origins.data(), origins.size(), s);
+      A.offsets.upload(desc->offsets, nb, s);
+      dd.dom = A.dom.ptr;
+      dd.dims = A.dims.ptr;
+      dd.offsets = A.offsets.ptr;
+      dd.scalars = desc->scalars;
+      dd.cellStart = A.cellStart.ptr;
+      dd.cellBlocks = A.cellBlocks.ptr;
+      const size_t nmc = (size_t)dd.mx * dd.my * dd.mz;
+      BN_REQUIRE(nmc < (1ull << 31), "bnamd_amr_create: too many macrocells");
+      DevBuf<uint2> ord(nmc);
+      A.ranges.alloc(nmc);
+      A.majorants.alloc(nmc);
+      const int gs = (int)std::min<size_t>(div_up(nmc, 256), 8192);
+      k_amr_clear<<<gs, 256, 0, s>>>(ord.ptr, nmc);
+      k_amr_raster<<<nb, 256, 0, s>>>(dd, A.origins.ptr, ord.ptr, nb, rcp(dd.mcSpacing));
+      k_amr_finish<<<gs, 256, 0, s>>>(ord.ptr, A.ranges.ptr, nmc);
+      BN_HIP(hipGetLastError());
+      BN_HIP(hipMemsetAsync(A.majorants.ptr, 0, A.majorants.bytes(), s));
+      BN_HIP(hipStreamSynchronize(s));
+      dd.majorants = A.majorants.ptr;
+      dd.tf = TFDD{nullptr, 0, 0.f, 1.f, 1.f};
+      A.buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    } catch (...) {
+      delete ua;
+      throw;
+    }
+    *out = ua;
+  });
+}
+
+extern "C" int bnamd_amr_destroy(bnamd_amr *amr) {
+  return guarded([&] { delete amr; });
+}
+
+extern "C" int bnamd_amr_info_get(const bnamd_amr *amr, bnamd_amr_info *info) {
+  return guarded([&] {
+    BN_REQUIRE(amr && info, "bnamd_amr_info_get: NULL argument");
+    const Amr &A = amr->a;
+    info->numBlocks = A.numBlocks;
+    info->mcDims[0] = A.dd.mx; info->mcDims[1] = A.dd.my; info->mcDims[2] = A.dd.mz;
+    for (int k = 0; k < 3; ++k) info->lookupDims[k] = A.lookup[k];
+    info->worldLower[0] = A.dd.worldLo.x; info->worldLower[1] = A.dd.worldLo.y; info->worldLower[2] = A.dd.worldLo.z;
+    info->worldUpper[0] = A.dd.worldHi.x; info->worldUpper[1] = A.dd.worldHi.y; info->worldUpper[2] = A.dd.worldHi.z;
+    info->buildMs = A.buildMs;
+  });
+}
+
+extern "C" int bnamd_amr_set_tf(bnamd_amr *amr, const float *values, int numValues, float domainLo, float domainHi,
+                                float baseDensity) {
+  return guarded([&] {
+    BN_REQUIRE(amr && values, "bnamd_amr_set_tf: NULL argument");
+    BN_REQUIRE(numValues >= 2, "bnamd_amr_set_tf: need at least 2 values");
+    BN_REQUIRE(domainHi > domainLo, "bnamd_amr_set_tf: empty domain");
+    Amr &A = amr->a;
+    A.tf.upload((const float4 *)values, numValues);
+    A.dd.tf = TFDD{A.tf.ptr, numValues, domainLo, domainHi, baseDensity};
+    A.haveTF = true;
+    const int nmc = A.dd.mx * A.dd.my * A.dd.mz;
+    k_majorants<<<div_up(nmc, 256), 256>>>(A.dd, A.ranges.ptr, A.majorants.ptr);
+    BN_HIP(hipGetLastError());
+    BN_HIP(hipDeviceSynchronize());
+  });
+}
+
+extern "C" int bnamd_amr_get_mcs(const bnamd_amr *amr, float *ranges, float *majorants) {
+  return guarded([&] {
+    BN_REQUIRE(amr, "bnamd_amr_get_mcs: NULL argument");
+    const Amr &A = amr->a;
+    if (ranges) BN_HIP(hipMemcpy(ranges, A.ranges.ptr, A.ranges.bytes(), hipMemcpyDeviceToHost));
+    if (majorants) BN_HIP(hipMemcpy(majorants, A.majorants.ptr, A.majorants.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int bnamd_amr_sample_points(const bnamd_amr *amr, const float *points, float *out, int numPoints,
+                                       void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(amr && points && out, "bnamd_amr_sample_points: NULL argument");
+    if (numPoints <= 0) return;
+    hipStream_t s = (hipStream_t)stream;
+    k_amr_sample<<<div_up(numPoints, 256), 256, 0, s>>>(amr->a.dd, points, out, numPoints);
+    BN_HIP(hipGetLastError());
+    BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera, const bnamd_render_params *params,
+                                float *accum, void *stream, bnamd_volume_stats *stats) {
+  return guarded([&] {
+    BN_REQUIRE(amr && camera && params && accum, "bnamd_amr_render: NULL argument");
+    BN_REQUIRE(amr->a.haveTF, "bnamd_amr_render: no transfer function set");
+    BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
+               "bnamd_amr_render: bad frame size or pathsPerPixel");
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf<unsigned long long> counters(3);
+    BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
+    const CameraDD cam = make_camera(*camera);
+    const RenderDD rp = make_render_dd(*params);
+    hipEvent_t e0, e1;
+    BN_HIP(hipEventCreate(&e0));
+    BN_HIP(hipEventCreate(&e1));
+    BN_HIP(hipEventRecord(e0, s));
+    render_volume_dispatch(amr->a.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, counters.ptr, s);
+    BN_HIP(hipGetLastError());
+    BN_HIP(hipEventRecord(e1, s));
+    BN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long h[3];
+    BN_HIP(hipMemcpy(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->samples = (long long)h[0];
+      stats->ddaSteps = (long long)h[1];
+      stats->raysTraced = (long long)h[2];
+      stats->ms = ms;
+    }
   });
 }

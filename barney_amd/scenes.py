@@ -456,3 +456,44 @@ def deformed_elements(seed=0x5EED0006, jitter=0.2, spacing=3.0):
     s = 0.5 + 0.3 * np.sin(1.3 * P[:, 0] + 0.4) * np.cos(0.9 * P[:, 1]) + 0.2 * P[:, 2] ** 2
     return (P.astype(np.float32), s.astype(np.float32),
             (np.array(indices, np.int32), np.array(offsets, np.int32), np.array(types, np.uint8)))
+
+
+def amr_field(n=32, block=8, seed=0x5EED0007, blobs=8):
+    """A two-level block-structured AMR field (BlockStructuredField's
+    layout): level 1 (cells 2 units wide, refinement 2) covers [0, 2n)^3
+    with n^3 cells, level 0 (1 unit) covers the middle [n/2, 3n/2)^3 with
+    n^3 cells, both in block^3-cell blocks; every cell holds a smooth
+    Gaussian-blob field at its center, normalized to [0, 1].
+    Returns origins, dims (int32 [B,3]), levels (int32 [B]), offsets
+    (uint64 [B]), refinements (int32 [2]), scalars (float32)."""
+    assert n % block == 0 and (n // 2) % block == 0
+    nb = n // block
+    bc = uniform(seed, 1, 3 * blobs, 0.2 * n, 1.8 * n).reshape(-1, 3)
+    sig = uniform(seed, 2, blobs, 0.1 * n, 0.4 * n)
+    amp = uniform(seed, 3, blobs, 0.5, 1.0)
+
+    def field(P):
+        s = np.zeros(len(P))
+        for b in range(blobs):
+            s += amp[b] * np.exp(-((P - bc[b]) ** 2).sum(1) / (2.0 * sig[b] ** 2))
+        return s
+
+    origins, levels, vals = [], [], []
+    c = np.arange(block)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    local = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], -1)  # x fastest
+    for level, cs, base in ((1, 2.0, 0), (0, 1.0, n // 2)):
+        for k in range(nb):
+            for j in range(nb):
+                for i in range(nb):
+                    o = np.array([base + i * block, base + j * block, base + k * block])
+                    origins.append(o)
+                    levels.append(level)
+                    vals.append(field((o + local + 0.5) * cs))
+    s = np.concatenate(vals)
+    s = (s - s.min()) / max(s.max() - s.min(), 1e-30)
+    B = len(origins)
+    offsets = np.arange(B, dtype=np.uint64) * np.uint64(block ** 3)
+    return dict(origins=np.array(origins, np.int32), dims=np.full((B, 3), block, np.int32),
+                levels=np.array(levels, np.int32), offsets=offsets, refinements=np.array([2, 2], np.int32),
+                scalars=s.astype(np.float32), extent=2 * n)

@@ -239,6 +239,57 @@ int bnamd_volume_iso_rays(const bnamd_volume *vol, const bnamd_ray *rays, float 
                           int numRays, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Block-structured AMR volumes (barney/amr): BlockStructuredField with its
+ * blocks, levels and scalars (BlockStructuredField.h, .cu), sampled as
+ * BlockStructuredCuBQLSampler does (BlockStructuredCuBQLSampler.h:66-80:
+ * the tent basis functions of every block whose domain contains the point,
+ * normalized; NaN where none does), the 256-wide macrocell grid of
+ * BlockStructuredField::buildMCs, and MCVolumeAccel delta tracking with the
+ * structured-volume render path. Blocks are visited in ascending order (the
+ * reference: its BVH's order), which fixes the rounding where domains
+ * overlap. A level-L cell is (int)powf(refinements[L], L) units wide; block
+ * b covers cells origins[b] .. origins[b] + dims[b] - 1 of its level, its
+ * scalars at scalars[offsets[b] + x + dims.x * (y + dims.y * z)].
+ * --------------------------------------------------------------------- */
+typedef struct {
+  int numBlocks;
+  const int *origins;                /* host int3 [numBlocks] */
+  const int *dims;                   /* host int3 [numBlocks] */
+  const int *levels;                 /* host [numBlocks] */
+  const unsigned long long *offsets; /* host [numBlocks] */
+  int numLevels;
+  const int *refinements;            /* host [numLevels] */
+  const float *scalars;              /* device [numScalars] */
+  unsigned long long numScalars;
+} bnamd_amr_desc;
+
+typedef struct bnamd_amr_s bnamd_amr;
+
+typedef struct {
+  int numBlocks;
+  int mcDims[3];
+  int lookupDims[3]; /* the block lookup grid (ascending block lists) */
+  float worldLower[3], worldUpper[3];
+  float buildMs;
+} bnamd_amr_info;
+
+/* Replaces BlockStructuredField::setData/commit/buildMCs and the sampler's
+ * BVH build (BlockStructuredField.cu:82-105, 229-300). */
+int bnamd_amr_create(const bnamd_amr_desc *desc, void *stream, bnamd_amr **out);
+int bnamd_amr_destroy(bnamd_amr *amr);
+int bnamd_amr_info_get(const bnamd_amr *amr, bnamd_amr_info *info);
+/* values: host float4[numValues]; recomputes the majorants */
+int bnamd_amr_set_tf(bnamd_amr *amr, const float *values, int numValues, float domainLo, float domainHi,
+                     float baseDensity);
+/* host copies of the macrocell ranges (float2 per cell) and majorants */
+int bnamd_amr_get_mcs(const bnamd_amr *amr, float *ranges, float *majorants);
+/* out[i] = field value at points[i] (device float3), NaN outside */
+int bnamd_amr_sample_points(const bnamd_amr *amr, const float *points, float *out, int numPoints, void *stream);
+/* delta-tracked render, as bnamd_volume_render */
+int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera, const bnamd_render_params *params,
+                     float *accum, void *stream, bnamd_volume_stats *stats);
+
+/* ------------------------------------------------------------------------
  * Unstructured volumes (barney/umesh): UMeshField with per-vertex scalars,
  * UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,
  * UMeshField.h:362-501 eltScalar for tets, pyramids, prisms and hexahedra),

@@ -1302,6 +1302,37 @@ struct ref_volume_s {
 
 namespace {
 
+// min / max into a cell shared between threads (compare-and-swap on the
+// float's bits); fmin / fmax are order-independent, so the grid does not
+// depend on the thread schedule.
+inline void atomic_fmin(float *p, float v) {
+  uint32_t *u = reinterpret_cast<uint32_t *>(p);
+  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
+  for (;;) {
+    float of;
+    std::memcpy(&of, &old, 4);
+    const float nv = std::fmin(of, v);
+    if (nv == of) return;
+    uint32_t nu;
+    std::memcpy(&nu, &nv, 4);
+    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+  }
+}
+inline void atomic_fmax(float *p, float v) {
+  uint32_t *u = reinterpret_cast<uint32_t *>(p);
+  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
+  for (;;) {
+    float of;
+    std::memcpy(&of, &old, 4);
+    const float nv = std::fmax(of, v);
+    if (nv == of) return;
+    uint32_t nu;
+    std::memcpy(&nu, &nv, 4);
+    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+  }
+}
+
+
 // embree Texture.cpp:326-335
 void lcoords(int N, float tc, int &i0, int &i1, float &f) {
   if (tc - .5f <= 0.f) { i0 = i1 = 0; f = 0.f; return; }
@@ -1553,8 +1584,19 @@ int ref_dda_cells(const float *org, const float *dir, float tMax, const int *g, 
   return n;
 }
 
-long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
-                            int y1, int threads, long long *samples) {
+}  // extern "C"
+
+namespace {
+
+// The delta-tracking render of a field over v's macrocell grid and transfer
+// function (generate / trace / shade with the SCI_VIS_MODE bounce).
+template <class Field>
+long long render_volume_g(const ref_volume_s *v, const Field &field, const ref_camera *cam,
+                          const ref_render_params *p, float *accum, int y0, int y1, int threads, long long *samples) {
+  auto vol_isect = [&](const ref_volume_s *vv, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed,
+                       bool exactLog, long long &cnt) {
+    return vol_isect_g(vv, field, org, dir, tmin, tmax, rayTMax, seed, exactLog, cnt, fnv3(0, 0, 0));
+  };
   const Cam c = make_cam(cam);
   const int W = p->width;
   std::atomic<long long> nrays(0), ns(0);
@@ -1603,6 +1645,242 @@ long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const re
   });
   if (samples) *samples = ns.load();
   return nrays.load();
+}
+
+}  // namespace
+
+extern "C" {
+
+long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                            int y1, int threads, long long *samples) {
+  return render_volume_g(v, [&](V3 P) { return vsample(v, P); }, cam, p, accum, y0, y1, threads, samples);
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Block-structured AMR (barney/amr): BlockStructuredField.h:100-248,
+// BlockStructuredField.cu:33-105, BlockStructuredCuBQLSampler.h:66-80
+// ============================================================================
+struct ref_amr_s {
+  ref_volume_s grid;  // macrocell grid, world box and transfer function
+  std::vector<float> dom;  // per block: lower.xyz, cellSize, upper.xyz, 0
+  std::vector<int> dims, origins;
+  std::vector<unsigned long long> offsets;
+  std::vector<float> scalars;
+  int g[3];
+  V3 gOrigin, gRcp;
+  std::vector<uint32_t> start, list;  // ascending block ids per lookup cell
+};
+
+namespace {
+
+// addBasisFunctions (BlockStructuredField.h:100-200)
+void amr_add_block(const ref_amr_s *a, int b, V3 P, float &sumW, float &sumWV) {
+  const float *lo = &a->dom[8 * (size_t)b], *hi = lo + 4;
+  if (!(lo[0] <= P.x && lo[1] <= P.y && lo[2] <= P.z && P.x <= hi[0] && P.y <= hi[1] && P.z <= hi[2])) return;
+  const float cs = lo[3];
+  const int *d = &a->dims[3 * (size_t)b];
+  const float *s = &a->scalars[a->offsets[b]];
+  const float c0[3] = {lo[0] + cs, lo[1] + cs, lo[2] + cs};
+  const float lp[3] = {(P.x - c0[0]) / cs, (P.y - c0[1]) / cs, (P.z - c0[2]) / cs};
+  float fl[3], f[3], nf[3];
+  int l[3], h[3];
+  for (int k = 0; k < 3; ++k) {
+    fl[k] = std::floor(lp[k]);
+    l[k] = std::max(-1, (int)fl[k]);
+    h[k] = l[k] + 1;
+    f[k] = lp[k] - fl[k];
+    nf[k] = 1.f - f[k];
+  }
+  auto in = [&](int i, int k) { return i >= 0 && i < d[k]; };
+  auto add = [&](int x, int y, int z, float w) {
+    sumW += w;
+    sumWV += w * s[x + y * d[0] + z * d[0] * d[1]];
+  };
+  for (int zi = 0; zi < 2; ++zi) {
+    const int z = zi ? h[2] : l[2];
+    if (!in(z, 2)) continue;
+    const float wz = zi ? f[2] : nf[2];
+    for (int yi = 0; yi < 2; ++yi) {
+      const int y = yi ? h[1] : l[1];
+      if (!in(y, 1)) continue;
+      const float wy = yi ? f[1] : nf[1];
+      if (in(l[0], 0)) add(l[0], y, z, wz * wy * nf[0]);
+      if (in(h[0], 0)) add(h[0], y, z, wz * wy * f[0]);
+    }
+  }
+}
+
+// BlockStructuredCuBQLSampler::DD::sample with the blocks in ascending order
+float amr_sample(const ref_amr_s *a, V3 P) {
+  const ref_volume_s &g = a->grid;
+  if (!(g.wlo.x <= P.x && g.wlo.y <= P.y && g.wlo.z <= P.z && P.x <= g.whi.x && P.y <= g.whi.y && P.z <= g.whi.z))
+    return NAN;
+  const int cx = std::min(std::max((int)((P.x - a->gOrigin.x) * a->gRcp.x), 0), a->g[0] - 1);
+  const int cy = std::min(std::max((int)((P.y - a->gOrigin.y) * a->gRcp.y), 0), a->g[1] - 1);
+  const int cz = std::min(std::max((int)((P.z - a->gOrigin.z) * a->gRcp.z), 0), a->g[2] - 1);
+  const size_t c = (size_t)cx + (size_t)a->g[0] * ((size_t)cy + (size_t)a->g[1] * cz);
+  float sumW = 0.f, sumWV = 0.f;
+  for (uint32_t i = a->start[c]; i < a->start[c + 1]; ++i) amr_add_block(a, (int)a->list[i], P, sumW, sumWV);
+  return sumW == 0.f ? NAN : sumWV / sumW;
+}
+
+// the same without the lookup grid: every block, ascending
+float amr_sample_brute(const ref_amr_s *a, V3 P) {
+  float sumW = 0.f, sumWV = 0.f;
+  for (size_t b = 0; b < a->offsets.size(); ++b) amr_add_block(a, (int)b, P, sumW, sumWV);
+  return sumW == 0.f ? NAN : sumWV / sumW;
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_amr *ref_amr_create(const int *origins, const int *dims, const int *levels, const unsigned long long *offsets,
+                        int numBlocks, const int *refinements, int numLevels, const float *scalars,
+                        unsigned long long numScalars) {
+  auto *a = new ref_amr_s;
+  a->scalars.assign(scalars, scalars + numScalars);
+  a->offsets.assign(offsets, offsets + numBlocks);
+  a->dims.assign(dims, dims + 3 * (size_t)numBlocks);
+  a->origins.assign(origins, origins + 3 * (size_t)numBlocks);
+  a->dom.resize(8 * (size_t)numBlocks);
+  V3 wlo = mk(kInf, kInf, kInf), whi = mk(-kInf, -kInf, -kInf);
+  for (int b = 0; b < numBlocks; ++b) {
+    const int L = levels[b];
+    const int *o = origins + 3 * (size_t)b, *d = dims + 3 * (size_t)b;
+    if (L < 0 || L >= numLevels || d[0] <= 0 || d[1] <= 0 || d[2] <= 0 ||
+        offsets[b] + (unsigned long long)d[0] * d[1] * d[2] > numScalars) {
+      delete a;
+      return nullptr;
+    }
+    // Block::getFrom / getDomain (BlockStructuredField.h:227-248)
+    const float cs = (float)(int)(std::pow((float)refinements[L], (float)L));
+    float *q = &a->dom[8 * (size_t)b];
+    for (int k = 0; k < 3; ++k) {
+      q[k] = ((float)o[k] - .5f) * cs;
+      q[4 + k] = ((float)(o[k] + d[k]) + .5f) * cs;
+    }
+    q[3] = cs;
+    q[7] = 0.f;
+    wlo = vmin(wlo, mk(q[0], q[1], q[2]));
+    whi = vmax(whi, mk(q[4], q[5], q[6]));
+  }
+  ref_volume_s &g = a->grid;
+  g.wlo = wlo;
+  g.whi = whi;
+  const V3 size = sub(whi, wlo);
+  const float maxWidth = maxc(size);
+  {  // buildMCs (BlockStructuredField.cu:82-105)
+    const float f = (256 - 1) / maxWidth;
+    g.m[0] = 1 + (int)(size.x * f);
+    g.m[1] = 1 + (int)(size.y * f);
+    g.m[2] = 1 + (int)(size.z * f);
+    g.mcOrigin = wlo;
+    g.mcSpacing = mul(size, inv(mk((float)g.m[0], (float)g.m[1], (float)g.m[2])));
+  }
+  {  // block lookup grid
+    const int G = std::min(256, std::max(16, 4 * (int)std::ceil(std::cbrt((double)numBlocks))));
+    const float f = (G - 1) / maxWidth;
+    a->g[0] = 1 + (int)(size.x * f);
+    a->g[1] = 1 + (int)(size.y * f);
+    a->g[2] = 1 + (int)(size.z * f);
+    a->gOrigin = wlo;
+    a->gRcp = inv(mul(size, inv(mk((float)a->g[0], (float)a->g[1], (float)a->g[2]))));
+    const size_t ng = (size_t)a->g[0] * a->g[1] * a->g[2];
+    a->start.assign(ng + 1, 0u);
+    const float go[3] = {wlo.x, wlo.y, wlo.z}, gr[3] = {a->gRcp.x, a->gRcp.y, a->gRcp.z};
+    std::vector<int> r(6 * (size_t)numBlocks);
+    for (int b = 0; b < numBlocks; ++b)
+      for (int k = 0; k < 3; ++k) {
+        r[6 * (size_t)b + k] = std::min(std::max((int)((a->dom[8 * (size_t)b + k] - go[k]) * gr[k]), 0), a->g[k] - 1);
+        r[6 * (size_t)b + 3 + k] =
+            std::min(std::max((int)((a->dom[8 * (size_t)b + 4 + k] - go[k]) * gr[k]), 0), a->g[k] - 1);
+      }
+    auto each = [&](int b, auto &&fn) {
+      const int *q = &r[6 * (size_t)b];
+      for (int z = q[2]; z <= q[5]; ++z)
+        for (int y = q[1]; y <= q[4]; ++y)
+          for (int x = q[0]; x <= q[3]; ++x) fn((size_t)x + (size_t)a->g[0] * ((size_t)y + (size_t)a->g[1] * z));
+    };
+    for (int b = 0; b < numBlocks; ++b) each(b, [&](size_t c) { ++a->start[c + 1]; });
+    for (size_t c = 0; c < ng; ++c) a->start[c + 1] += a->start[c];
+    a->list.resize(a->start[ng]);
+    std::vector<uint32_t> fill(a->start.begin(), a->start.end() - 1);
+    for (int b = 0; b < numBlocks; ++b) each(b, [&](size_t c) { a->list[fill[c]++] = (uint32_t)b; });
+  }
+  // BSField_rasterGrids (BlockStructuredField.cu:33-57) through rasterBox
+  const size_t nc = (size_t)g.m[0] * g.m[1] * g.m[2];
+  g.rlo.assign(nc, kInf);
+  g.rhi.assign(nc, -kInf);
+  g.maj.assign(nc, 0.f);
+  const V3 rs = inv(g.mcSpacing);
+  const float mo[3] = {g.mcOrigin.x, g.mcOrigin.y, g.mcOrigin.z}, mr[3] = {rs.x, rs.y, rs.z};
+  parallel_for(numBlocks, default_threads(0), [&](int b) {
+    const float cs = a->dom[8 * (size_t)b + 3];
+    const int *o = &a->origins[3 * (size_t)b], *d = &a->dims[3 * (size_t)b];
+    const float *s = &a->scalars[a->offsets[b]];
+    for (int z = 0; z < d[2]; ++z)
+      for (int y = 0; y < d[1]; ++y)
+        for (int x = 0; x < d[0]; ++x) {
+          const int c[3] = {o[0] + x, o[1] + y, o[2] + z};
+          int lo[3], hi[3];
+          for (int k = 0; k < 3; ++k) {
+            const float l = ((float)c[k] - .5f) * cs, h = l + 2.f * cs;
+            lo[k] = std::min(std::max((int)((l - mo[k]) * mr[k]), 0), g.m[k] - 1);
+            hi[k] = std::min(std::max((int)((h - mo[k]) * mr[k]), 0), g.m[k] - 1);
+          }
+          const float v = s[x + y * d[0] + z * d[0] * d[1]];
+          for (int iz = lo[2]; iz <= hi[2]; ++iz)
+            for (int iy = lo[1]; iy <= hi[1]; ++iy)
+              for (int ix = lo[0]; ix <= hi[0]; ++ix) {
+                const size_t id = (size_t)ix + (size_t)g.m[0] * ((size_t)iy + (size_t)g.m[1] * iz);
+                atomic_fmin(&g.rlo[id], v);
+                atomic_fmax(&g.rhi[id], v);
+              }
+        }
+  });
+  const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  g.tf.assign(one, one + 8);
+  g.numTF = 2;
+  return a;
+}
+
+void ref_amr_destroy(ref_amr *a) { delete a; }
+
+void ref_amr_set_tf(ref_amr *a, const float *values, int numValues, float lo, float hi, float base) {
+  ref_volume_s &g = a->grid;
+  g.tf.assign(values, values + 4 * (size_t)numValues);
+  g.numTF = numValues;
+  g.dlo = lo;
+  g.dhi = hi;
+  g.base = base;
+  for (size_t i = 0; i < g.maj.size(); ++i) g.maj[i] = tfmaj(&g, g.rlo[i], g.rhi[i]);
+}
+
+void ref_amr_mc_dims(const ref_amr *a, int *dims) {
+  for (int k = 0; k < 3; ++k) dims[k] = a->grid.m[k];
+}
+
+void ref_amr_get_mcs(const ref_amr *a, float *ranges, float *maj) {
+  const ref_volume_s &g = a->grid;
+  for (size_t i = 0; i < g.maj.size(); ++i) {
+    if (ranges) { ranges[2 * i] = g.rlo[i]; ranges[2 * i + 1] = g.rhi[i]; }
+    if (maj) maj[i] = g.maj[i];
+  }
+}
+
+void ref_amr_sample(const ref_amr *a, const float *points, float *out, int n, int brute, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const V3 P = ld3(points + 3 * (size_t)i);
+    out[i] = brute ? amr_sample_brute(a, P) : amr_sample(a, P);
+  });
+}
+
+long long ref_amr_render(const ref_amr *a, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                         int y1, int threads, long long *samples) {
+  return render_volume_g(&a->grid, [&](V3 P) { return amr_sample(a, P); }, cam, p, accum, y0, y1, threads, samples);
 }
 
 }  // extern "C"
@@ -1843,35 +2121,6 @@ float edge_len(const P4 &a, const P4 &b) {
   return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// min / max into a cell shared between threads (compare-and-swap on the
-// float's bits); fmin / fmax are order-independent, so the grid does not
-// depend on the thread schedule.
-inline void atomic_fmin(float *p, float v) {
-  uint32_t *u = reinterpret_cast<uint32_t *>(p);
-  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
-  for (;;) {
-    float of;
-    std::memcpy(&of, &old, 4);
-    const float nv = std::fmin(of, v);
-    if (nv == of) return;
-    uint32_t nu;
-    std::memcpy(&nu, &nv, 4);
-    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
-  }
-}
-inline void atomic_fmax(float *p, float v) {
-  uint32_t *u = reinterpret_cast<uint32_t *>(p);
-  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED);
-  for (;;) {
-    float of;
-    std::memcpy(&of, &old, 4);
-    const float nv = std::fmax(of, v);
-    if (nv == of) return;
-    uint32_t nu;
-    std::memcpy(&nu, &nv, 4);
-    if (__atomic_compare_exchange_n(u, &old, nu, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
-  }
-}
 
 void raster_box4(ref_volume_s *g, const P4 *q) {
   float lo[4], hi[4];

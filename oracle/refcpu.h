@@ -127,6 +127,23 @@ int ref_dda_cells(const float *org, const float *dir, float tMax, const int *gri
 long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p,
                             float *accum, int y0, int y1, int threads, long long *samples);
 
+/* --- block-structured AMR (barney/amr) --- */
+typedef struct ref_amr_s ref_amr;
+/* origins/dims int3 per block, levels, offsets (first scalar of each block),
+ * refinements per level; NULL for an invalid block */
+ref_amr *ref_amr_create(const int *origins, const int *dims, const int *levels, const unsigned long long *offsets,
+                        int numBlocks, const int *refinements, int numLevels, const float *scalars,
+                        unsigned long long numScalars);
+void ref_amr_destroy(ref_amr *a);
+void ref_amr_set_tf(ref_amr *a, const float *values, int numValues, float lo, float hi, float baseDensity);
+void ref_amr_mc_dims(const ref_amr *a, int *dims);
+void ref_amr_get_mcs(const ref_amr *a, float *ranges, float *majorants);
+/* field samples (NaN outside); brute != 0 visits every block instead of
+ * the lookup grid's lists */
+void ref_amr_sample(const ref_amr *a, const float *points, float *out, int n, int brute, int threads);
+long long ref_amr_render(const ref_amr *a, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                         int y1, int threads, long long *samples);
+
 /* --- unstructured volume (barney/umesh) --- */
 typedef struct ref_umesh_s ref_umesh;
 /* vertices float3, per-vertex scalars, tets int4 (positively oriented) */

@@ -85,6 +85,13 @@ def load():
         "ref_world_destroy": (None, [P]),
         "ref_world_trace": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_umesh_create": (P, [P, P, C.c_int, P, C.c_int]),
+        "ref_amr_create": (P, [P, P, P, P, C.c_int, P, C.c_int, P, C.c_ulonglong]),
+        "ref_amr_destroy": (None, [P]),
+        "ref_amr_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "ref_amr_mc_dims": (None, [P, P]),
+        "ref_amr_get_mcs": (None, [P, P, P]),
+        "ref_amr_sample": (None, [P, P, P, C.c_int, C.c_int, C.c_int]),
+        "ref_amr_render": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_umesh_create_cells": (P, [P, P, C.c_int, P, C.c_int, P, P, C.c_int]),
         "ref_umesh_destroy": (None, [P]),
         "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -275,6 +282,62 @@ class Volume:
     def __del__(self):
         try:
             load().ref_volume_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Amr:
+    """Oracle block-structured AMR field: blocks (origins, dims int3; levels;
+    offsets of their first scalar), per-level refinements, scalars; the
+    macrocell grid, transfer function, point sampling and render."""
+
+    def __init__(self, origins, dims, levels, offsets, refinements, scalars):
+        self._o = np.ascontiguousarray(origins, np.int32).reshape(-1, 3)
+        self._d = np.ascontiguousarray(dims, np.int32).reshape(-1, 3)
+        self._l = np.ascontiguousarray(levels, np.int32).reshape(-1)
+        self._off = np.ascontiguousarray(offsets, np.uint64).reshape(-1)
+        self._r = np.ascontiguousarray(refinements, np.int32).reshape(-1)
+        self._s = np.ascontiguousarray(scalars, np.float32).reshape(-1)
+        self.handle = load().ref_amr_create(self._o.ctypes.data, self._d.ctypes.data, self._l.ctypes.data,
+                                            self._off.ctypes.data, len(self._o), self._r.ctypes.data, len(self._r),
+                                            self._s.ctypes.data, len(self._s))
+        if not self.handle:
+            raise ValueError("ref_amr_create: invalid block")
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
+        load().ref_amr_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]), float(domain[1]),
+                              float(base_density))
+
+    def mc_dims(self):
+        d = np.zeros(3, np.int32)
+        load().ref_amr_mc_dims(self.handle, d.ctypes.data)
+        return tuple(int(x) for x in d)
+
+    def macrocells(self):
+        mx, my, mz = self.mc_dims()
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        load().ref_amr_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data)
+        return ranges, maj
+
+    def sample(self, points, brute=False, threads=0):
+        P = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(len(P), np.float32)
+        load().ref_amr_sample(self.handle, P.ctypes.data, out.ctypes.data, len(P), int(bool(brute)), threads)
+        return out
+
+    def render(self, cam, params, y0=0, y1=None, threads=0, accum=None):
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        ns = C.c_longlong(0)
+        n = load().ref_amr_render(self.handle, C.byref(cam), C.byref(params), acc.ctypes.data, y0, y1, threads,
+                                  C.byref(ns))
+        return acc, n, ns.value
+
+    def __del__(self):
+        try:
+            load().ref_amr_destroy(self.handle)
         except Exception:
             pass
 
