@@ -356,6 +356,51 @@ def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
     return res
 
 
+def amr_bench(api, n=128, block=16, steps=2, cpu=True, cpu_seconds=8.0):
+    """Block-structured AMR (not a BASELINE workload; the structured-volume
+    path tracer with BlockStructuredField's sampler): scenes.amr_field with a
+    128^3-cell coarse level and a 128^3-cell fine level in 16^3 blocks
+    (1,024 blocks, 4.2 M cells), delta tracking, 1024x1024, 4 paths per
+    pixel."""
+    from barney_amd import scenes
+    from oracle import refcpu
+    t0 = time.time()
+    f = scenes.amr_field(n, block)
+    args = (f["origins"], f["dims"], f["levels"], f["offsets"], f["refinements"], f["scalars"])
+    a = api.Amr(*args)
+    tf = scenes.ramp_tf()
+    a.set_tf(tf, (0.0, 1.0), 0.2)
+    info = a.info()
+    log(f"amr built in {time.time() - t0:.1f}s ({info['numBlocks']} blocks, build {info['buildMs']:.0f} ms)")
+    W, H, spp = 1024, 1024, 4
+    c = f["extent"] / 2.0
+    pos = (c - 1.3 * f["extent"], c + 0.7 * f["extent"], c - 1.6 * f["extent"])
+    camargs = dict(position=pos, direction=(c - pos[0], c - pos[1], c - pos[2]), fovy=35.0)
+    p = api.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1))
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    cam = api.camera(**camargs)
+    a.render(cam, p, accum=acc)
+    ms, st = [], None
+    for _ in range(steps):
+        o = a.render(cam, p, accum=acc)
+        st = o["stats"]
+        ms.append(st["ms"])
+    t = float(np.median(ms))
+    log(f"amr: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M samples")
+    res = {"blocks": info["numBlocks"], "cells": int(len(f["scalars"])), "ms_per_frame": t,
+           "samples_per_frame": int(st["samples"]), "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6,
+           "dda_steps_per_frame": int(st["ddaSteps"]), "rays_per_frame": int(st["raysTraced"])}
+    if cpu:
+        r = refcpu.Amr(*args)
+        r.set_tf(tf, (0.0, 1.0), 0.2)
+        rp = refcpu.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1))
+        rcam = refcpu.camera(**camargs)
+        racc = np.zeros((H, W, 4), np.float32)
+        res["cpu_baseline"] = band_timer(lambda y0, y1: r.render(rcam, rp, y0, y1, cpu_threads(), accum=racc)[2], H,
+                                         cpu_seconds, "Msamples/s", f"the {W}x{H}x{spp}spp frame")
+    return res
+
+
 def fb_bench(api, reps=20):
     """Frame readback of a 3840x2160 frame (8160 tiles of 32x32): tiled
     accumulation -> linear float4 + normals, -> sRGB RGBA8, and the resolve of
@@ -525,6 +570,7 @@ def main():
     if not args.no_umesh:
         out["extra"]["umesh_workload5"] = umesh_bench(api, cpu=not args.no_cpu)
         out["extra"]["umesh_mixed_cells"] = umesh_cells_bench(api, cpu=not args.no_cpu)
+        out["extra"]["amr_two_levels"] = amr_bench(api, cpu=not args.no_cpu)
     if not args.no_fb:
         out["extra"]["readback_3840x2160"] = fb_bench(api)
     if not args.no_ea:
