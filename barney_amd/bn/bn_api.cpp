@@ -1,0 +1,430 @@
+// barney's bn* host API (barney/include/barney.h, barney/api/barney.cu) for
+// surface scenes, over the bnamd C ABI: objects hold staged parameters
+// (Object::set*, barney/Object.cpp), bnBuild turns the model's instances of
+// groups of geometries into one bnamd world (ModelSlot::build,
+// barney/ModelSlot.cpp:210-216), bnRender runs the fused surface path
+// tracer into the frame buffer's accumulation buffer (Context::render,
+// barney/Context.cpp:142-185) and bnFrameBufferRead resolves it
+// (FrameBuffer::read, barney/fb/FrameBuffer.cu:188-450).
+#include "bn_api.h"
+
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bnamd.h"
+
+namespace {
+
+void check(int rc, const char *what) {
+  if (rc != BNAMD_OK) throw std::runtime_error(std::string(what) + ": " + bnamd_last_error());
+}
+
+struct DevMem {
+  void *ptr = nullptr;
+  DevMem() = default;
+  explicit DevMem(size_t bytes) { check(bnamd_device_malloc(&ptr, bytes ? bytes : 1), "device allocation"); }
+  DevMem(const void *host, size_t bytes) : DevMem(bytes) {
+    if (bytes) check(bnamd_memcpy_h2d(ptr, host, bytes), "upload");
+  }
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem() {
+    if (ptr) (void)bnamd_device_free(ptr);
+  }
+};
+
+struct Context;
+
+struct Object : _BNObject {
+  explicit Object(Context *c, std::string t) : ctx(c), type(std::move(t)) {}
+  virtual ~Object() = default;
+  Context *ctx;
+  std::string type;
+  int appRefs = 1;
+  std::map<std::string, std::array<float, 4>> f;
+  std::map<std::string, int> i;
+  std::map<std::string, Object *> o;
+  float getf(const std::string &n, float def) const {
+    auto it = f.find(n);
+    return it == f.end() ? def : it->second[0];
+  }
+  std::array<float, 4> get4(const std::string &n, std::array<float, 4> def) const {
+    auto it = f.find(n);
+    return it == f.end() ? def : it->second;
+  }
+  int geti(const std::string &n, int def) const {
+    auto it = i.find(n);
+    return it == i.end() ? def : it->second;
+  }
+  Object *geto(const std::string &n) const {
+    auto it = o.find(n);
+    return it == o.end() ? nullptr : it->second;
+  }
+};
+
+struct Data : Object {
+  Data(Context *c, BNDataType t, size_t n, const void *items) : Object(c, "data"), dataType(t), count(n) {
+    const size_t sz = elem_size(t);
+    if (!sz) throw std::runtime_error("bnDataCreate: unsupported data type");
+    bytes.assign((const uint8_t *)items, (const uint8_t *)items + sz * n);
+  }
+  static size_t elem_size(BNDataType t) {
+    switch (t) {
+      case BN_INT32: case BN_FLOAT32: return 4;
+      case BN_INT32_VEC2: return 8;
+      case BN_INT32_VEC3: case BN_FLOAT32_VEC3: return 12;
+      case BN_FLOAT32_VEC4: return 16;
+      default: return 0;
+    }
+  }
+  BNDataType dataType;
+  size_t count;
+  std::vector<uint8_t> bytes;
+};
+
+struct Group : Object {
+  using Object::Object;
+  std::vector<Object *> geoms;
+};
+struct Model : Object {
+  using Object::Object;
+  std::vector<Group *> groups;
+  std::vector<BNTransform> xfms;
+  bnamd_world *world = nullptr;
+  std::vector<std::unique_ptr<DevMem>> mem;
+  ~Model() override {
+    if (world) (void)bnamd_world_destroy(world);
+  }
+};
+struct FrameBuffer : Object {
+  using Object::Object;
+  int w = 0, h = 0;
+  uint32_t channels = BN_FB_COLOR;
+  int paths = 0;  // paths accumulated per pixel
+  std::unique_ptr<DevMem> accum, ids;
+};
+
+struct Context {
+  std::vector<std::unique_ptr<Object>> objects;
+  template <class T>
+  T *add(T *p) {
+    objects.emplace_back(p);
+    return p;
+  }
+};
+
+Object *obj(BNObject h) {
+  if (!h) throw std::runtime_error("bn*: NULL object");
+  return static_cast<Object *>(h);
+}
+
+void slot0(int s) {
+  if (s != 0) throw std::runtime_error("bn*: this backend has one data slot (0)");
+}
+
+// AnariMatte / AnariPBR parameters (material/AnariMatte.cpp:29-90,
+// AnariPBR.cpp:36-80) to the ABI's material record
+bnamd_material make_material(const Object *m) {
+  bnamd_material r{};
+  r.opacity = 1.f;
+  r.ior = 1.45f;
+  r.specular = 0.5f;
+  if (!m || m->type == "matte" || m->type == "AnariMatte") {
+    const auto c = m ? m->get4("color", {.8f, .8f, .8f, 1.f}) : std::array<float, 4>{.8f, .8f, .8f, 1.f};
+    r.type = BNAMD_MAT_MATTE;
+    r.color[0] = c[0]; r.color[1] = c[1]; r.color[2] = c[2];
+    if (m) r.opacity = m->getf("opacity", 1.f);
+    return r;
+  }
+  if (m->type != "physicallyBased" && m->type != "AnariPBR") throw std::runtime_error("bnMaterialCreate: unknown type " + m->type);
+  const auto c = m->get4("baseColor", {1.f, 1.f, 1.f, 1.f});
+  r.type = BNAMD_MAT_PBR;
+  r.color[0] = c[0]; r.color[1] = c[1]; r.color[2] = c[2];
+  r.metallic = m->getf("metallic", 1.f);
+  r.roughness = m->getf("roughness", 1.f);
+  r.transmission = m->getf("transmission", 0.f);
+  r.ior = m->getf("ior", 1.5f);
+  r.opacity = m->getf("opacity", 1.f);
+  return r;
+}
+
+const Data *data_param(const Object *g, const char *name, BNDataType want, bool required) {
+  const Object *d = g->geto(name);
+  if (!d) {
+    if (required) throw std::runtime_error("bnBuild: " + g->type + " without '" + name + "'");
+    return nullptr;
+  }
+  const Data *dd = dynamic_cast<const Data *>(d);
+  if (!dd || dd->dataType != want) throw std::runtime_error("bnBuild: '" + std::string(name) + "' has the wrong type");
+  return dd;
+}
+
+}  // namespace
+
+BN_API BNContext bnContextCreate(const int *, int, const int *gpuIDs, int numGPUs) {
+  check(bnamd_init((gpuIDs && numGPUs > 0) ? gpuIDs[0] : 0), "bnContextCreate");
+  return reinterpret_cast<BNContext>(new Context);
+}
+
+BN_API void bnContextDestroy(BNContext context) { delete reinterpret_cast<Context *>(context); }
+
+BN_API void bnRelease(BNObject object) { --obj(object)->appRefs; }
+BN_API void bnAddReference(BNObject object) { ++obj(object)->appRefs; }
+
+BN_API BNCamera bnCameraCreate(BNContext context, const char *type) {
+  const std::string t = type ? type : "";
+  if (t != "perspective" && t != "pinhole") throw std::runtime_error("bnCameraCreate: unknown type " + t);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNCamera>(static_cast<_BNObject *>(c->add(new Object(c, "perspective"))));
+}
+
+BN_API BNFrameBuffer bnFrameBufferCreate(BNContext context, int) {
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNFrameBuffer>(static_cast<_BNObject *>(c->add(new FrameBuffer(c, "frameBuffer"))));
+}
+
+BN_API BNModel bnModelCreate(BNContext ctx) {
+  auto *c = reinterpret_cast<Context *>(ctx);
+  return static_cast<BNModel>(static_cast<_BNObject *>(c->add(new Model(c, "model"))));
+}
+
+BN_API BNRenderer bnRendererCreate(BNContext ctx, const char *type) {
+  const std::string t = type ? type : "default";
+  if (t != "default" && t != "pathTracer") throw std::runtime_error("bnRendererCreate: unknown type " + t);
+  auto *c = reinterpret_cast<Context *>(ctx);
+  return static_cast<BNRenderer>(static_cast<_BNObject *>(c->add(new Object(c, "renderer"))));
+}
+
+BN_API void bnCommit(BNObject target) { (void)obj(target); }  // parameters are read at build / render time
+
+BN_API void bnSetData(BNObject target, const char *paramName, BNData value) {
+  obj(target)->o[paramName] = value ? obj(value) : nullptr;
+}
+BN_API void bnSetObject(BNObject target, const char *paramName, const BNObject value) {
+  obj(target)->o[paramName] = value ? obj(value) : nullptr;
+}
+BN_API void bnSet1i(BNObject target, const char *paramName, int value) { obj(target)->i[paramName] = value; }
+BN_API void bnSet1f(BNObject target, const char *paramName, float value) {
+  obj(target)->f[paramName] = {value, 0.f, 0.f, 0.f};
+}
+BN_API void bnSet3f(BNObject target, const char *paramName, float x, float y, float z) {
+  obj(target)->f[paramName] = {x, y, z, 0.f};
+}
+BN_API void bnSet4f(BNObject target, const char *paramName, float x, float y, float z, float w) {
+  obj(target)->f[paramName] = {x, y, z, w};
+}
+
+BN_API BNData bnDataCreate(BNContext context, int whichSlot, BNDataType dataType, size_t numItems, const void *items) {
+  slot0(whichSlot);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNData>(static_cast<_BNObject *>(c->add(new Data(c, dataType, numItems, items))));
+}
+
+BN_API BNGeom bnGeometryCreate(BNContext context, int whichSlot, const char *type) {
+  slot0(whichSlot);
+  const std::string t = type ? type : "";
+  if (t != "triangles" && t != "spheres" && t != "cylinders") throw std::runtime_error("bnGeometryCreate: unknown type " + t);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNGeom>(static_cast<_BNObject *>(c->add(new Object(c, t))));
+}
+
+BN_API BNMaterial bnMaterialCreate(BNContext context, int whichSlot, const char *type) {
+  slot0(whichSlot);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNMaterial>(static_cast<_BNObject *>(c->add(new Object(c, type ? type : "matte"))));
+}
+
+BN_API BNGroup bnGroupCreate(BNContext context, int whichSlot, BNGeom *geoms, int numGeoms, BNVolume *, int numVolumes) {
+  slot0(whichSlot);
+  if (numVolumes > 0) throw std::runtime_error("bnGroupCreate: volumes go through bnamd_volume_* in this backend");
+  auto *c = reinterpret_cast<Context *>(context);
+  auto *g = c->add(new Group(c, "group"));
+  for (int k = 0; k < numGeoms; ++k) g->geoms.push_back(obj(geoms[k]));
+  return static_cast<BNGroup>(static_cast<_BNObject *>(g));
+}
+
+BN_API void bnGroupBuild(BNGroup group) { (void)obj(group); }  // built with the model (one world per model)
+
+BN_API void bnSetInstances(BNModel model, int whichSlot, BNGroup *groups, BNTransform *xfms, int numInstances) {
+  slot0(whichSlot);
+  auto *m = static_cast<Model *>(obj(model));
+  m->groups.clear();
+  m->xfms.clear();
+  for (int k = 0; k < numInstances; ++k) {
+    m->groups.push_back(static_cast<Group *>(obj(groups[k])));
+    m->xfms.push_back(xfms[k]);
+  }
+}
+
+BN_API void bnBuild(BNModel model, int whichDataSlot) {
+  slot0(whichDataSlot);
+  auto *m = static_cast<Model *>(obj(model));
+  if (m->world) {
+    check(bnamd_world_destroy(m->world), "bnBuild");
+    m->world = nullptr;
+  }
+  m->mem.clear();
+  std::vector<Group *> groups;  // distinct groups, in first-use order
+  std::vector<bnamd_instance> insts;
+  for (size_t k = 0; k < m->groups.size(); ++k) {
+    size_t gi = 0;
+    while (gi < groups.size() && groups[gi] != m->groups[k]) ++gi;
+    if (gi == groups.size()) groups.push_back(m->groups[k]);
+    bnamd_instance in{};
+    in.group = (int)gi;
+    in.instID = (int)k;
+    const BNTransform &x = m->xfms[k];
+    const float v[12] = {x.l.vx.x, x.l.vx.y, x.l.vx.z, x.l.vy.x, x.l.vy.y, x.l.vy.z,
+                         x.l.vz.x, x.l.vz.y, x.l.vz.z, x.p.x,    x.p.y,    x.p.z};
+    std::memcpy(in.xfm, v, sizeof v);
+    insts.push_back(in);
+  }
+  std::vector<bnamd_material> mats;
+  std::vector<const Object *> matObjs;
+  auto matIndex = [&](const Object *mo) {
+    for (size_t k = 0; k < matObjs.size(); ++k)
+      if (matObjs[k] == mo) return (int)k;
+    matObjs.push_back(mo);
+    mats.push_back(make_material(mo));
+    return (int)mats.size() - 1;
+  };
+  auto upload = [&](const Data *d) -> const void * {
+    if (!d) return nullptr;
+    m->mem.emplace_back(new DevMem(d->bytes.data(), d->bytes.size()));
+    return m->mem.back()->ptr;
+  };
+  std::vector<std::vector<bnamd_geom>> gg(groups.size());
+  std::vector<bnamd_group> bg(groups.size());
+  for (size_t k = 0; k < groups.size(); ++k) {
+    for (const Object *g : groups[k]->geoms) {
+      bnamd_geom e{};
+      e.materialID = matIndex(g->geto("material"));
+      e.userID = g->geti("userID", 0);
+      if (g->type == "triangles") {  // Triangles.cpp:29-40
+        const Data *v = data_param(g, "vertices", BN_FLOAT32_VEC3, true);
+        const Data *ix = data_param(g, "indices", BN_INT32_VEC3, true);
+        e.kind = BNAMD_GEOM_TRIANGLES;
+        e.numPrims = (int)ix->count;
+        e.numVertices = (int)v->count;
+        e.vertices = (const float *)upload(v);
+        e.indices = (const int *)upload(ix);
+      } else if (g->type == "spheres") {  // Spheres.cpp:48-70, default radius .1
+        const Data *v = data_param(g, "origins", BN_FLOAT32_VEC3, true);
+        const Data *r = data_param(g, "radii", BN_FLOAT32, false);
+        e.kind = BNAMD_GEOM_SPHERES;
+        e.numPrims = e.numVertices = (int)v->count;
+        e.vertices = (const float *)upload(v);
+        e.radii = (const float *)upload(r);
+        e.defaultRadius = g->getf("radius", .1f);
+      } else {  // cylinders: Cylinders.cpp:60-71
+        const Data *v = data_param(g, "vertices", BN_FLOAT32_VEC3, true);
+        const Data *ix = data_param(g, "indices", BN_INT32_VEC2, true);
+        const Data *r = data_param(g, "radii", BN_FLOAT32, false);
+        e.kind = BNAMD_GEOM_CYLINDERS;
+        e.numPrims = (int)ix->count;
+        e.numVertices = (int)v->count;
+        e.vertices = (const float *)upload(v);
+        e.indices = (const int *)upload(ix);
+        e.radii = (const float *)upload(r);
+        e.defaultRadius = g->getf("radius", .1f);
+      }
+      gg[k].push_back(e);
+    }
+    bg[k].geoms = gg[k].data();
+    bg[k].numGeoms = (int)gg[k].size();
+  }
+  if (mats.empty()) mats.push_back(make_material(nullptr));
+  check(bnamd_world_build(bg.data(), (int)bg.size(), insts.data(), (int)insts.size(), mats.data(), (int)mats.size(),
+                          nullptr, &m->world),
+        "bnBuild");
+}
+
+BN_API void bnAccumReset(BNFrameBuffer fb) { static_cast<FrameBuffer *>(obj(fb))->paths = 0; }
+
+BN_API void bnFrameBufferResize(BNFrameBuffer fbh, BNDataType, int sizeX, int sizeY, uint32_t requiredChannels) {
+  auto *fb = static_cast<FrameBuffer *>(obj(fbh));
+  if (sizeX <= 0 || sizeY <= 0) throw std::runtime_error("bnFrameBufferResize: bad size");
+  if (requiredChannels & BN_FB_NORMAL) throw std::runtime_error("bnFrameBufferResize: no normal channel in this backend");
+  fb->w = sizeX;
+  fb->h = sizeY;
+  fb->channels = requiredChannels | BN_FB_COLOR;
+  fb->paths = 0;
+  fb->accum.reset(new DevMem((size_t)sizeX * sizeY * 16));
+  const bool ids = requiredChannels & (BN_FB_DEPTH | BN_FB_PRIMID | BN_FB_INSTID | BN_FB_OBJID);
+  fb->ids.reset(ids ? new DevMem((size_t)sizeX * sizeY * 16) : nullptr);
+}
+
+BN_API void bnFrameBufferGetSize(BNFrameBuffer fbh, int *sizeX, int *sizeY) {
+  auto *fb = static_cast<FrameBuffer *>(obj(fbh));
+  if (sizeX) *sizeX = fb->w;
+  if (sizeY) *sizeY = fb->h;
+}
+
+BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFrameBuffer fbh) {
+  const Object *r = obj(renderer), *cam = obj(camera);
+  auto *m = static_cast<Model *>(obj(model));
+  auto *fb = static_cast<FrameBuffer *>(obj(fbh));
+  if (!m->world) throw std::runtime_error("bnRender: model not built (bnBuild)");
+  if (!fb->accum) throw std::runtime_error("bnRender: frame buffer not sized (bnFrameBufferResize)");
+  // PerspectiveCamera parameters (Camera.cu:30-80)
+  bnamd_camera c{};
+  const auto pos = cam->get4("position", {0.f, 0.f, 0.f, 0.f});
+  const auto dir = cam->get4("direction", {0.f, 0.f, 1.f, 0.f});
+  const auto up = cam->get4("up", {0.f, 1.f, 0.f, 0.f});
+  for (int k = 0; k < 3; ++k) {
+    c.position[k] = pos[k];
+    c.direction[k] = dir[k];
+    c.up[k] = up[k];
+  }
+  c.fovy = cam->getf("fovy", 60.f);
+  // Renderer parameters (render/Renderer.cpp:26-95; Renderer.h:55-57)
+  bnamd_render_params p{};
+  p.width = fb->w;
+  p.height = fb->h;
+  p.accumID = fb->paths;
+  p.pathsPerPixel = r->geti("pathsPerPixel", 1);
+  p.ambientRadiance = r->getf("ambientRadiance", 1.f);
+  const auto bg = r->get4("bgColor", {0.f, 0.f, 0.f, 1.f});
+  for (int k = 0; k < 4; ++k) p.bgColor[k] = bg[k];
+  p.maxPathDepth = 10;
+  p.maxDiffuseBounces = 1;
+  p.russianRoulette = 0;
+  int *ids = (fb->ids && fb->paths == 0) ? (int *)fb->ids->ptr : nullptr;
+  check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
+  fb->paths += p.pathsPerPixel;
+}
+
+BN_API void bnFrameBufferRead(BNFrameBuffer fbh, BNFrameBufferChannel channel, void *out, BNDataType format) {
+  auto *fb = static_cast<FrameBuffer *>(obj(fbh));
+  if (!fb->accum || fb->paths == 0) throw std::runtime_error("bnFrameBufferRead: nothing rendered");
+  const size_t n = (size_t)fb->w * fb->h;
+  if (channel == BN_FB_COLOR) {
+    int fmt;
+    size_t bpp;
+    switch (format) {
+      case BN_FLOAT32_VEC4: fmt = BNAMD_FB_FLOAT4; bpp = 16; break;
+      case BN_UFIXED8_RGBA: fmt = BNAMD_FB_UFIXED8_RGBA; bpp = 4; break;
+      case BN_UFIXED8_RGBA_SRGB: fmt = BNAMD_FB_UFIXED8_RGBA_SRGB; bpp = 4; break;
+      default: throw std::runtime_error("bnFrameBufferRead: unsupported color format");
+    }
+    DevMem tmp(n * bpp);
+    check(bnamd_fb_resolve((const float *)fb->accum->ptr, fb->w, fb->h, 1.f / (float)fb->paths, fmt, tmp.ptr, nullptr),
+          "bnFrameBufferRead");
+    check(bnamd_synchronize(), "bnFrameBufferRead");
+    check(bnamd_memcpy_d2h(out, tmp.ptr, n * bpp), "bnFrameBufferRead");
+    return;
+  }
+  if (!fb->ids || !(fb->channels & channel)) throw std::runtime_error("bnFrameBufferRead: channel not requested at resize");
+  std::vector<int32_t> rec(4 * n);
+  check(bnamd_memcpy_d2h(rec.data(), fb->ids->ptr, rec.size() * 4), "bnFrameBufferRead");
+  const int k = channel == BN_FB_DEPTH ? 3 : channel == BN_FB_PRIMID ? 0 : channel == BN_FB_INSTID ? 1 : 2;
+  if (channel == BN_FB_DEPTH && format != BN_FLOAT32) throw std::runtime_error("bnFrameBufferRead: depth is BN_FLOAT32");
+  if (channel != BN_FB_DEPTH && format != BN_INT32) throw std::runtime_error("bnFrameBufferRead: IDs are BN_INT32");
+  for (size_t q = 0; q < n; ++q) std::memcpy((char *)out + 4 * q, &rec[4 * q + k], 4);
+}

@@ -20,6 +20,10 @@ RTC_SRC = os.path.join(HERE, "rtcore_amd", "rtcore_amd.cpp")
 RTC_LIB = os.path.join(HERE, "librtcore_amd.so")
 SELFTEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_selftest.cpp")
 SELFTEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_selftest")
+BN_SRC = os.path.join(HERE, "bn", "bn_api.cpp")
+BN_LIB = os.path.join(HERE, "libbn_api.so")
+BN_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "bn_api_selftest.cpp")
+BN_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "bn_api_selftest")
 
 SOURCES = [
     "runtime.hip",
@@ -86,8 +90,18 @@ def build_host_layer(verbose=False):
     _run([CXX, *common, SELFTEST_SRC, "-o", tmp, "-L" + HERE, "-lrtcore_amd", "-lbnamd",
           "-Wl,-rpath,$ORIGIN/../../barney_amd"], "g++ (selftest)")
     os.replace(tmp, SELFTEST)
+    # the bn* API (barney.h subset) over the C ABI, and its self-test
+    tmp = BN_LIB + ".tmp"
+    inc = "-I" + os.path.join(os.path.dirname(HERE), "include")
+    _run([CXX, *common, inc, "-shared", BN_SRC, "-o", tmp, "-L" + HERE, "-lbnamd", "-Wl,-rpath,$ORIGIN"],
+         "g++ (bn_api)")
+    os.replace(tmp, BN_LIB)
+    tmp = BN_TEST + ".tmp"
+    _run([CXX, *common, inc, BN_TEST_SRC, "-o", tmp, "-L" + HERE, "-lbn_api", "-lbnamd",
+          "-Wl,-rpath,$ORIGIN/../../barney_amd"], "g++ (bn_api selftest)")
+    os.replace(tmp, BN_TEST)
     if verbose:
-        print(f"built {RTC_LIB} and {SELFTEST}")
+        print(f"built {RTC_LIB}, {SELFTEST}, {BN_LIB} and {BN_TEST}")
 
 
 if __name__ == "__main__":

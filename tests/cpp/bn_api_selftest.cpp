@@ -1,0 +1,161 @@
+// Renders one scene twice -- through the bn* API (libbn_api.so) and through
+// the bnamd C ABI directly -- and checks that colors, depth and IDs agree
+// bit for bit, over two progressive frames. Exit 3 without a GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "bn_api.h"
+#include "bnamd.h"
+
+static int fail(const char *what) {
+  std::printf("FAIL: %s\n", what);
+  return 1;
+}
+
+int main() {
+  BNContext ctx;
+  try {
+    ctx = bnContextCreate();
+  } catch (const std::exception &e) {
+    std::printf("NO_GPU: %s\n", e.what());
+    return 3;
+  }
+  const int W = 64, H = 48, SPP = 2;
+  // scene: two triangles (a floor) and three spheres of two materials
+  const float tv[] = {-4, 0, -4, 4, 0, -4, 4, 0, 4, -4, 0, 4};
+  const int ti[] = {0, 1, 2, 0, 2, 3};
+  const float so[] = {0, 1, 0, 1.5f, 0.6f, 0.5f, -1.2f, 0.5f, -0.3f};
+  const float sr[] = {1.f, 0.6f, 0.5f};
+  const float pos[3] = {0, 2.5f, -6}, dir[3] = {0, -0.35f, 1}, up[3] = {0, 1, 0};
+
+  BNMaterial floorMat = bnMaterialCreate(ctx, 0, "matte");
+  bnSet3f(floorMat, "color", .7f, .6f, .5f);
+  BNMaterial ball = bnMaterialCreate(ctx, 0, "physicallyBased");
+  bnSet3f(ball, "baseColor", .3f, .5f, .9f);
+  bnSet1f(ball, "metallic", 0.f);
+  bnSet1f(ball, "roughness", .4f);
+  BNGeom tri = bnGeometryCreate(ctx, 0, "triangles");
+  bnSetData(tri, "vertices", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 4, tv));
+  bnSetData(tri, "indices", bnDataCreate(ctx, 0, BN_INT32_VEC3, 2, ti));
+  bnSetObject(tri, "material", floorMat);
+  bnSet1i(tri, "userID", 7);
+  bnCommit(tri);
+  BNGeom sph = bnGeometryCreate(ctx, 0, "spheres");
+  bnSetData(sph, "origins", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 3, so));
+  bnSetData(sph, "radii", bnDataCreate(ctx, 0, BN_FLOAT32, 3, sr));
+  bnSetObject(sph, "material", ball);
+  bnSet1i(sph, "userID", 9);
+  bnCommit(sph);
+  BNGeom geoms[2] = {tri, sph};
+  BNGroup group = bnGroupCreate(ctx, 0, geoms, 2, nullptr, 0);
+  bnGroupBuild(group);
+  BNModel model = bnModelCreate(ctx);
+  BNTransform xfm = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {0, 0, 0}};
+  bnSetInstances(model, 0, &group, &xfm, 1);
+  bnBuild(model, 0);
+  BNCamera cam = bnCameraCreate(ctx, "perspective");
+  bnSet3f(cam, "position", pos[0], pos[1], pos[2]);
+  bnSet3f(cam, "direction", dir[0], dir[1], dir[2]);
+  bnSet3f(cam, "up", up[0], up[1], up[2]);
+  bnSet1f(cam, "fovy", 50.f);
+  bnCommit(cam);
+  BNRenderer rnd = bnRendererCreate(ctx, "default");
+  bnSet1i(rnd, "pathsPerPixel", SPP);
+  bnSet1f(rnd, "ambientRadiance", 0.9f);
+  bnSet4f(rnd, "bgColor", 0.1f, 0.2f, 0.3f, 1.f);
+  bnCommit(rnd);
+  BNFrameBuffer fb = bnFrameBufferCreate(ctx);
+  bnFrameBufferResize(fb, BN_FLOAT4, W, H, BN_FB_COLOR | BN_FB_DEPTH | BN_FB_PRIMID | BN_FB_INSTID | BN_FB_OBJID);
+  int sx = 0, sy = 0;
+  bnFrameBufferGetSize(fb, &sx, &sy);
+  if (sx != W || sy != H) return fail("frame buffer size");
+  bnRender(rnd, model, cam, fb);
+  bnRender(rnd, model, cam, fb);  // second progressive frame
+  std::vector<float> color(4 * W * H), depth(W * H);
+  std::vector<int> prim(W * H), inst(W * H), objid(W * H);
+  bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+  bnFrameBufferRead(fb, BN_FB_DEPTH, depth.data(), BN_FLOAT32);
+  bnFrameBufferRead(fb, BN_FB_PRIMID, prim.data(), BN_INT32);
+  bnFrameBufferRead(fb, BN_FB_INSTID, inst.data(), BN_INT32);
+  bnFrameBufferRead(fb, BN_FB_OBJID, objid.data(), BN_INT32);
+
+  // the same through the C ABI
+  void *dv, *di, *dso, *dsr, *acc, *ids, *out;
+  bnamd_device_malloc(&dv, sizeof tv);
+  bnamd_device_malloc(&di, sizeof ti);
+  bnamd_device_malloc(&dso, sizeof so);
+  bnamd_device_malloc(&dsr, sizeof sr);
+  bnamd_memcpy_h2d(dv, tv, sizeof tv);
+  bnamd_memcpy_h2d(di, ti, sizeof ti);
+  bnamd_memcpy_h2d(dso, so, sizeof so);
+  bnamd_memcpy_h2d(dsr, sr, sizeof sr);
+  bnamd_geom g[2] = {};
+  g[0].kind = BNAMD_GEOM_TRIANGLES; g[0].numPrims = 2; g[0].numVertices = 4; g[0].materialID = 0;
+  g[0].vertices = (const float *)dv; g[0].indices = (const int *)di; g[0].userID = 7;
+  g[1].kind = BNAMD_GEOM_SPHERES; g[1].numPrims = 3; g[1].numVertices = 3; g[1].materialID = 1;
+  g[1].vertices = (const float *)dso; g[1].radii = (const float *)dsr; g[1].defaultRadius = .1f; g[1].userID = 9;
+  bnamd_group grp = {g, 2};
+  bnamd_instance in = {0, 0, {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0}};
+  bnamd_material m[2] = {};
+  m[0].type = BNAMD_MAT_MATTE; m[0].color[0] = .7f; m[0].color[1] = .6f; m[0].color[2] = .5f; m[0].opacity = 1;
+  m[0].ior = 1.45f; m[0].specular = .5f;
+  m[1].type = BNAMD_MAT_PBR; m[1].color[0] = .3f; m[1].color[1] = .5f; m[1].color[2] = .9f; m[1].opacity = 1;
+  m[1].metallic = 0; m[1].roughness = .4f; m[1].transmission = 0; m[1].ior = 1.5f; m[1].specular = .5f;
+  bnamd_world *world = nullptr;
+  if (bnamd_world_build(&grp, 1, &in, 1, m, 2, nullptr, &world) != BNAMD_OK) return fail(bnamd_last_error());
+  bnamd_camera c = {{pos[0], pos[1], pos[2]}, {dir[0], dir[1], dir[2]}, {up[0], up[1], up[2]}, 50.f};
+  bnamd_render_params p = {W, H, 0, SPP, 0.9f, {0.1f, 0.2f, 0.3f, 1.f}, 10, 1, 0};
+  bnamd_device_malloc(&acc, 16 * W * H);
+  bnamd_device_malloc(&ids, 16 * W * H);
+  bnamd_device_malloc(&out, 16 * W * H);
+  if (bnamd_render_surface(world, &c, &p, (float *)acc, (int *)ids, nullptr, nullptr) != BNAMD_OK) return fail("render");
+  p.accumID = SPP;
+  if (bnamd_render_surface(world, &c, &p, (float *)acc, nullptr, nullptr, nullptr) != BNAMD_OK) return fail("render 2");
+  bnamd_fb_resolve((const float *)acc, W, H, 1.f / (2 * SPP), BNAMD_FB_FLOAT4, out, nullptr);
+  bnamd_synchronize();
+  std::vector<float> rc(4 * W * H);
+  std::vector<int> rid(4 * W * H);
+  bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+  bnamd_memcpy_d2h(rid.data(), ids, rid.size() * 4);
+  if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("color differs from the C ABI render");
+  int hits = 0;
+  for (int q = 0; q < W * H; ++q) {
+    float d;
+    std::memcpy(&d, &rid[4 * q + 3], 4);
+    if (prim[q] != rid[4 * q] || inst[q] != rid[4 * q + 1] || objid[q] != rid[4 * q + 2] ||
+        std::memcmp(&d, &depth[q], 4) != 0)
+      return fail("ID channels differ");
+    hits += prim[q] >= 0;
+  }
+  bool sawTri = false, sawSph = false;
+  for (int q = 0; q < W * H; ++q) {
+    sawTri |= objid[q] == 7;
+    sawSph |= objid[q] == 9;
+  }
+  if (!sawTri || !sawSph || hits < W * H / 4) return fail("scene not visible");
+  // bnAccumReset restarts accumulation: one frame equals the C ABI's first
+  bnAccumReset(fb);
+  bnRender(rnd, model, cam, fb);
+  bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+  p.accumID = 0;
+  bnamd_render_surface(world, &c, &p, (float *)acc, nullptr, nullptr, nullptr);
+  bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+  bnamd_synchronize();
+  bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+  if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("color after bnAccumReset");
+  // errors surface as exceptions
+  bool threw = false;
+  try {
+    bnGeometryCreate(ctx, 0, "capsules");
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  if (!threw) return fail("unknown geometry type accepted");
+  bnamd_world_destroy(world);
+  bnContextDestroy(ctx);
+  std::printf("hits %d of %d\nOK\n", hits, W * H);
+  return 0;
+}
