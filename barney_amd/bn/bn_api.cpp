@@ -88,18 +88,35 @@ struct Data : Object {
   std::vector<uint8_t> bytes;
 };
 
+struct TexData : Object {
+  TexData(Context *c, int w, int h, int d, const void *items) : Object(c, "textureData"), dims{w, h, d} {
+    texels.assign((const float *)items, (const float *)items + (size_t)w * h * d);
+  }
+  int dims[3];
+  std::vector<float> texels;
+};
+struct VolumeObj : Object {
+  VolumeObj(Context *c, Object *f) : Object(c, "volume"), field(f) {}
+  Object *field;
+  float domain[2] = {0.f, 1.f};
+  std::vector<float> xf = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 1.f, 1.f};
+  float densityAt1 = 1.f;
+};
 struct Group : Object {
   using Object::Object;
   std::vector<Object *> geoms;
+  std::vector<VolumeObj *> volumes;
 };
 struct Model : Object {
   using Object::Object;
   std::vector<Group *> groups;
   std::vector<BNTransform> xfms;
   bnamd_world *world = nullptr;
+  bnamd_volume *volume = nullptr;
   std::vector<std::unique_ptr<DevMem>> mem;
   ~Model() override {
     if (world) (void)bnamd_world_destroy(world);
+    if (volume) (void)bnamd_volume_destroy(volume);
   }
 };
 struct FrameBuffer : Object {
@@ -240,12 +257,47 @@ BN_API BNMaterial bnMaterialCreate(BNContext context, int whichSlot, const char 
   return static_cast<BNMaterial>(static_cast<_BNObject *>(c->add(new Object(c, type ? type : "matte"))));
 }
 
-BN_API BNGroup bnGroupCreate(BNContext context, int whichSlot, BNGeom *geoms, int numGeoms, BNVolume *, int numVolumes) {
+BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BNDataType texelFormat, int width,
+                                           int height, int depth, const void *items) {
   slot0(whichSlot);
-  if (numVolumes > 0) throw std::runtime_error("bnGroupCreate: volumes go through bnamd_volume_* in this backend");
+  if (texelFormat != BN_FLOAT32) throw std::runtime_error("bnTextureData3DCreate: only BN_FLOAT32 texels");
+  if (width < 2 || height < 2 || depth < 2 || !items) throw std::runtime_error("bnTextureData3DCreate: bad size");
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNTextureData>(static_cast<_BNObject *>(c->add(new TexData(c, width, height, depth, items))));
+}
+
+BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type) {
+  slot0(whichSlot);
+  const std::string t = type ? type : "";
+  if (t != "structured") throw std::runtime_error("bnScalarFieldCreate: unknown type " + t);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNScalarField>(static_cast<_BNObject *>(c->add(new Object(c, t))));
+}
+
+BN_API BNVolume bnVolumeCreate(BNContext context, int whichSlot, BNScalarField sf) {
+  slot0(whichSlot);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNVolume>(static_cast<_BNObject *>(c->add(new VolumeObj(c, obj(sf)))));
+}
+
+// Volume::setXF (barney/volume/Volume.cpp, TransferFunction.cpp)
+BN_API void bnVolumeSetXF(BNVolume volume, bn_float2 domain, const bn_float4 *colorMap, int numColorMapValues,
+                          float densityAt1) {
+  auto *v = static_cast<VolumeObj *>(obj(volume));
+  if (!colorMap || numColorMapValues < 2) throw std::runtime_error("bnVolumeSetXF: need at least 2 values");
+  v->domain[0] = domain.x;
+  v->domain[1] = domain.y;
+  v->xf.assign((const float *)colorMap, (const float *)colorMap + 4 * (size_t)numColorMapValues);
+  v->densityAt1 = densityAt1;
+}
+
+BN_API BNGroup bnGroupCreate(BNContext context, int whichSlot, BNGeom *geoms, int numGeoms, BNVolume *volumes,
+                             int numVolumes) {
+  slot0(whichSlot);
   auto *c = reinterpret_cast<Context *>(context);
   auto *g = c->add(new Group(c, "group"));
   for (int k = 0; k < numGeoms; ++k) g->geoms.push_back(obj(geoms[k]));
+  for (int k = 0; k < numVolumes; ++k) g->volumes.push_back(static_cast<VolumeObj *>(obj(volumes[k])));
   return static_cast<BNGroup>(static_cast<_BNObject *>(g));
 }
 
@@ -269,7 +321,43 @@ BN_API void bnBuild(BNModel model, int whichDataSlot) {
     check(bnamd_world_destroy(m->world), "bnBuild");
     m->world = nullptr;
   }
+  if (m->volume) {
+    check(bnamd_volume_destroy(m->volume), "bnBuild");
+    m->volume = nullptr;
+  }
   m->mem.clear();
+  // a volume model: one structured volume (StructuredData::commit and
+  // buildMCs with barney's 8 cells per macrocell, StructuredData.cu:25-95)
+  size_t nvol = 0, ngeom = 0;
+  const VolumeObj *vol = nullptr;
+  for (const Group *g : m->groups) {
+    nvol += g->volumes.size();
+    ngeom += g->geoms.size();
+    if (!g->volumes.empty()) vol = g->volumes[0];
+  }
+  if (nvol > 0) {
+    if (nvol != 1 || ngeom != 0 || m->groups.size() != 1)
+      throw std::runtime_error("bnBuild: a model holds either geometry or one volume in this backend");
+    const Object *f = vol->field;
+    const TexData *td = dynamic_cast<const TexData *>(f->geto("textureData"));
+    if (!td) throw std::runtime_error("bnBuild: structured field without 'textureData'");
+    m->mem.emplace_back(new DevMem(td->texels.data(), td->texels.size() * 4));
+    bnamd_structured_desc d{};
+    d.scalars = (const float *)m->mem.back()->ptr;
+    const auto o = f->get4("gridOrigin", {0.f, 0.f, 0.f, 0.f});
+    const auto sp = f->get4("gridSpacing", {1.f, 1.f, 1.f, 0.f});
+    for (int k = 0; k < 3; ++k) {
+      d.dims[k] = td->dims[k];
+      d.gridOrigin[k] = o[k];
+      d.gridSpacing[k] = sp[k];
+    }
+    d.cellsPerMC = 8;
+    check(bnamd_volume_create(&d, &m->volume), "bnBuild");
+    check(bnamd_volume_set_tf(m->volume, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
+                              vol->densityAt1),
+          "bnBuild");
+    return;
+  }
   std::vector<Group *> groups;  // distinct groups, in first-use order
   std::vector<bnamd_instance> insts;
   for (size_t k = 0; k < m->groups.size(); ++k) {
@@ -370,7 +458,7 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   const Object *r = obj(renderer), *cam = obj(camera);
   auto *m = static_cast<Model *>(obj(model));
   auto *fb = static_cast<FrameBuffer *>(obj(fbh));
-  if (!m->world) throw std::runtime_error("bnRender: model not built (bnBuild)");
+  if (!m->world && !m->volume) throw std::runtime_error("bnRender: model not built (bnBuild)");
   if (!fb->accum) throw std::runtime_error("bnRender: frame buffer not sized (bnFrameBufferResize)");
   // PerspectiveCamera parameters (Camera.cu:30-80)
   bnamd_camera c{};
@@ -396,7 +484,10 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   p.maxDiffuseBounces = 1;
   p.russianRoulette = 0;
   int *ids = (fb->ids && fb->paths == 0) ? (int *)fb->ids->ptr : nullptr;
-  check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
+  if (m->volume)
+    check(bnamd_volume_render(m->volume, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+  else
+    check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
   fb->paths += p.pathsPerPixel;
 }
 

@@ -1,12 +1,15 @@
 /* The bn* host API over this backend: the part of barney's public C API
- * (barney/include/barney.h) that a surface scene needs -- context, data
- * arrays, triangle / sphere / cylinder geometry, matte and physically-based
- * materials, groups, instances, perspective camera, renderer parameters,
- * frame buffer with color / depth / ID channels, render. Names, types,
- * parameter names and enum values are barney's, so an application written
- * against barney.h for such scenes compiles and links against
- * libbn_api.so. One device, one data slot (whichSlot must be 0). Volumes
- * and the other object types are reached through bnamd.h directly.
+ * (barney/include/barney.h) that surface scenes and structured volumes
+ * need -- context, data arrays, triangle / sphere / cylinder geometry,
+ * matte and physically-based materials, 3D float texture data, the
+ * "structured" scalar field, volumes with a transfer function, groups,
+ * instances, perspective camera, renderer parameters, frame buffer with
+ * color / depth / ID channels, render. Names, types, parameter names and
+ * enum values are barney's, so an application written against barney.h for
+ * such scenes compiles and links against libbn_api.so. One device, one data
+ * slot (whichSlot must be 0); a model holds either geometry or one volume
+ * (this backend's renderers are separate kernels). Other object types are
+ * reached through bnamd.h directly.
  * Objects belong to their context: bnRelease drops the application's
  * reference, storage is freed by bnContextDestroy. Errors throw
  * std::runtime_error from C++ callers' point of view, as barney does. */
@@ -29,6 +32,8 @@ typedef struct _BNRenderer : public _BNObject {} *BNRenderer;
 typedef struct _BNFrameBuffer : public _BNObject {} *BNFrameBuffer;
 typedef struct _BNCamera : public _BNObject {} *BNCamera;
 typedef struct _BNMaterial : public _BNObject {} *BNMaterial;
+typedef struct _BNTextureData : public _BNObject {} *BNTextureData;
+typedef struct _BNScalarField : public _BNObject {} *BNScalarField;
 
 struct bn_float2 { float x, y; };
 struct bn_float3 { float x, y, z; };
@@ -108,5 +113,15 @@ BN_API BNGroup bnGroupCreate(BNContext context, int whichSlot, BNGeom *geoms, in
 BN_API void bnGroupBuild(BNGroup group);
 BN_API BNGeom bnGeometryCreate(BNContext context, int whichSlot, const char *type);
 BN_API BNMaterial bnMaterialCreate(BNContext context, int whichSlot, const char *type);
+
+/* barney.h:440-446, 507-519: volumes. texelFormat BN_FLOAT32; the
+ * "structured" field takes "textureData", "gridOrigin", "gridSpacing"
+ * (StructuredData.cu:130-160) */
+BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BNDataType texelFormat, int width,
+                                           int height, int depth, const void *items);
+BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type);
+BN_API BNVolume bnVolumeCreate(BNContext context, int whichSlot, BNScalarField sf);
+BN_API void bnVolumeSetXF(BNVolume volume, bn_float2 domain, const bn_float4 *colorMap, int numColorMapValues,
+                          float densityAt1);
 
 #endif /* BN_API_H */

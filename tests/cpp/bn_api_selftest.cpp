@@ -146,6 +146,52 @@ int main() {
   bnamd_synchronize();
   bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
   if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("color after bnAccumReset");
+  // a structured volume through bn* equals bnamd_volume_* with the same
+  // field, transfer function and cellsPerMC 8
+  {
+    const int N = 17;
+    std::vector<float> field(N * N * N);
+    for (int z = 0; z < N; ++z)
+      for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+          const float dx = x - 8.f, dy = y - 7.f, dz = z - 9.f;
+          field[x + N * (y + N * z)] = std::exp(-(dx * dx + dy * dy + dz * dz) / 40.f);
+        }
+    const bn_float4 xf[3] = {{0.f, 0.f, 1.f, 0.f}, {1.f, .5f, 0.f, .5f}, {1.f, 1.f, 1.f, 1.f}};
+    BNTextureData td = bnTextureData3DCreate(ctx, 0, BN_FLOAT32, N, N, N, field.data());
+    BNScalarField sf = bnScalarFieldCreate(ctx, 0, "structured");
+    bnSetObject(sf, "textureData", td);
+    bnSet3f(sf, "gridOrigin", -2.f, -1.f, -2.f);
+    bnSet3f(sf, "gridSpacing", .25f, .25f, .25f);
+    bnCommit(sf);
+    BNVolume vol = bnVolumeCreate(ctx, 0, sf);
+    bnVolumeSetXF(vol, {0.f, 1.f}, xf, 3, 4.f);
+    BNGroup vg = bnGroupCreate(ctx, 0, nullptr, 0, &vol, 1);
+    BNModel vm = bnModelCreate(ctx);
+    bnSetInstances(vm, 0, &vg, &xfm, 1);
+    bnBuild(vm, 0);
+    bnAccumReset(fb);
+    bnRender(rnd, vm, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    void *dfield;
+    bnamd_device_malloc(&dfield, field.size() * 4);
+    bnamd_memcpy_h2d(dfield, field.data(), field.size() * 4);
+    bnamd_structured_desc d = {(const float *)dfield, {N, N, N}, {-2.f, -1.f, -2.f}, {.25f, .25f, .25f}, 8};
+    bnamd_volume *v = nullptr;
+    if (bnamd_volume_create(&d, &v) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_volume_set_tf(v, (const float *)xf, 3, 0.f, 1.f, 4.f);
+    p.accumID = 0;
+    bnamd_volume_stats vs;
+    if (bnamd_volume_render(v, &c, &p, (float *)acc, nullptr, &vs) != BNAMD_OK) return fail("volume render");
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("volume color differs");
+    if (vs.samples <= 0) return fail("volume not sampled");
+    std::printf("volume samples %lld\n", vs.samples);
+    bnamd_volume_destroy(v);
+    bnamd_device_free(dfield);
+  }
   // errors surface as exceptions
   bool threw = false;
   try {
