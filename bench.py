@@ -313,7 +313,7 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
 
 
 def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
-    """General unstructured cells (not a BASELINE workload; the same path as
+    """General unstructured cells (an extra workload beside the five; the same path as
     workload 5 with pyramids, prisms and hexahedra next to tets): the sheared
     mixed mesh of scenes.mixed_cells on an n^3 lattice, delta tracking plus
     an iso-surface at 0.5, 512x512, 4 paths per pixel."""
@@ -357,7 +357,7 @@ def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
 
 
 def amr_bench(api, n=128, block=16, steps=2, cpu=True, cpu_seconds=8.0):
-    """Block-structured AMR (not a BASELINE workload; the structured-volume
+    """Block-structured AMR (an extra workload beside the five; the structured-volume
     path tracer with BlockStructuredField's sampler): scenes.amr_field with a
     128^3-cell coarse level and a 128^3-cell fine level in 16^3 blocks
     (1,024 blocks, 4.2 M cells), delta tracking, 1024x1024, 4 paths per
@@ -516,7 +516,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by image tiles / views, see DESIGN.md)", file=sys.stderr)
+              "(the path shards by image tiles / views)", file=sys.stderr)
         sys.exit(2)
     if not (args.no_ea and args.no_fb):
         # torch must bring up the HIP runtime before libbnamd does when both
