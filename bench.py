@@ -72,10 +72,19 @@ def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
         log(f"volume step {k}: {o['stats']['ms']:.1f} ms, {o['stats']['samples'] / 1e6:.1f} M samples")
     api.capi.check(api.capi.load().bnamd_synchronize())
     wall = time.perf_counter() - t0
+    img = acc.numpy()
     res = {"samples": samples, "ddaSteps": dda, "rays": rays, "wall_s": wall, "kernel_ms": kernel_ms,
-           "steps": steps, "image_mean": float(acc.numpy()[..., :3].mean() / spp)}
+           "steps": steps, "image_mean": float(img[..., :3].mean() / spp)}
     if cpu:
-        res["cpu"] = volume_cpu_baseline(w2, cpu_seconds)
+        base, oimg = volume_cpu_baseline(w2, cpu_seconds)
+        y0, y1 = base.pop("_rows")
+        res["cpu"] = base
+        g = img[y0:y1]
+        # the full-size frame against the oracle on the band it rendered
+        res["vs_oracle"] = {"rows": [y0, y1 - 1],
+                            "pixels_bit_exact": float(np.all(g == oimg, axis=-1).mean()),
+                            "max_abs_diff": float(np.abs(g - oimg).max())}
+        log(f"volume vs oracle: {res['vs_oracle']}")
     return res
 
 
@@ -112,7 +121,8 @@ def band_timer(render_rows, H, seconds, unit, what, keep_rows=False):
 
 
 def volume_cpu_baseline(w2, seconds):
-    """The oracle on host cores over a band of rows of the same frame."""
+    """The oracle on host cores over a band of rows of the same frame;
+    returns the baseline and the oracle's image of the band."""
     from oracle import refcpu
     rv = refcpu.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
     rv.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
@@ -120,8 +130,10 @@ def volume_cpu_baseline(w2, seconds):
     cam = refcpu.camera(**w2["camera"])
     p = refcpu.render_params(W, H, spp=w2["spp"])
     acc = np.zeros((H, W, 4), np.float32)
-    return band_timer(lambda y0, y1: rv.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
-                      "Msamples/s", f"the {W}x{H}x{w2['spp']}spp frame")
+    base = band_timer(lambda y0, y1: rv.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Msamples/s", f"the {W}x{H}x{w2['spp']}spp frame", keep_rows=True)
+    y0, y1 = base["_rows"]
+    return base, acc[y0:y1]
 
 
 def surface_cpu_baseline(w1, seconds):
@@ -561,7 +573,8 @@ def main():
                      "alg_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": vb.get("cpu"),
         "extra": {"samples_per_step": int(per_launch_samples), "dda_steps_per_step": int(per_launch_dda),
-                  "rays_per_step": vb["rays"] // vb["steps"], "image_mean": vb["image_mean"]},
+                  "rays_per_step": vb["rays"] // vb["steps"], "image_mean": vb["image_mean"],
+                  "vs_oracle": vb.get("vs_oracle")},
     }
     if not args.no_surface:
         out["extra"]["surface_workload1"] = surface_bench(api, cpu=not args.no_cpu)
