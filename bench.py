@@ -146,8 +146,9 @@ def surface_cpu_baseline(w1, seconds):
     cam = refcpu.camera(**w1["camera"])
     p = refcpu.render_params(W, H, spp=1)
     acc = np.zeros((H, W, 4), np.float32)
-    return band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
-                      "Mrays/s", f"the {W}x{H} frame, 1 spp")
+    base = band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
+                      "Mrays/s", f"the {W}x{H} frame, 1 spp", keep_rows=True)
+    return base, acc[base["_rows"][0]:base["_rows"][1]]
 
 
 def _oracle_material(m):
@@ -180,6 +181,16 @@ def batch_cpu_baseline(w3, seconds):
     return base, acc[y0:y1]
 
 
+def band_vs_oracle(gpu_img, base, oimg):
+    """Per-pixel agreement of a full-size GPU frame with the oracle's band:
+    the fraction of pixels equal bit for bit, and within 1e-3 relative."""
+    y0, y1 = base.pop("_rows")
+    g = gpu_img[y0:y1]
+    rel = np.abs(g - oimg) / np.maximum(np.abs(oimg), 1e-3)
+    return {"rows": [y0, y1 - 1], "pixels_bit_exact": float(np.all(g == oimg, axis=-1).mean()),
+            "pixels_within_1e-3": float((rel.max(-1) <= 1e-3).mean())}
+
+
 def image_vs_oracle(g, r):
     """Error of a band of a GPU image against the oracle's (RGB, the
     accumulated 4-path sums): RMSE relative to the oracle's mean, and the
@@ -204,9 +215,10 @@ def umesh_cpu_baseline(w5, seconds):
     cam = refcpu.camera(**w5["camera"])
     p = refcpu.render_params(W, H, spp=w5["spp"], ambient=1.0, bg=(-1, -1, -1, -1))
     acc = np.zeros((H, W, 4), np.float32)
-    return band_timer(lambda y0, y1: u.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], y0=y0,
+    base = band_timer(lambda y0, y1: u.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], y0=y0,
                                               y1=y1, threads=cpu_threads(), accum=acc)[2], H, seconds,
-                      "Msamples/s", f"the {W}x{H}x{w5['spp']}spp frame")
+                      "Msamples/s", f"the {W}x{H}x{w5['spp']}spp frame", keep_rows=True)
+    return base, acc[base["_rows"][0]:base["_rows"][1]]
 
 
 def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
@@ -236,7 +248,9 @@ def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
            "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"],
            "roofline": surface_roofline(rays, nodes, pbytes, ms)}
     if cpu:
-        res["cpu_baseline"] = surface_cpu_baseline(w1, cpu_seconds)
+        base, oimg = surface_cpu_baseline(w1, cpu_seconds)
+        res["vs_oracle"] = band_vs_oracle(acc.numpy(), base, oimg)
+        res["cpu_baseline"] = base
     return res
 
 
@@ -315,13 +329,18 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
         ms.append(st["ms"])
     t = float(np.median(ms))
     log(f"workload 5: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M field samples")
-    return {"ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
+    res = {"ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
             "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"]),
             "Mrays_per_s": st["raysTraced"] / (t * 1e-3) / 1e6, "dda_steps_per_frame": int(st["ddaSteps"]),
             "tets": info["numTets"], "bvh_nodes": info["numNodes"], "bvh_depth": info["bvhDepth"],
             "mc_dims": list(info["mcDims"]), "bvh_build_ms": info["buildMs"], "mc_build_ms": info["mcMs"],
             "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"]),
-            "cpu_baseline": umesh_cpu_baseline(w5, cpu_seconds) if cpu else None}
+            "cpu_baseline": None}
+    if cpu:
+        base, oimg = umesh_cpu_baseline(w5, cpu_seconds)
+        res["vs_oracle"] = band_vs_oracle(acc.numpy(), base, oimg)
+        res["cpu_baseline"] = base
+    return res
 
 
 def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
