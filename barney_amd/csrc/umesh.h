@@ -19,6 +19,8 @@ struct UMeshDD {
   const BNode8 *nodes;
   const float4 *tets;       // leaf order: 4 records per tet, or 8 per element of a mixed mesh
   const uint8_t *elemTypes; // mixed meshes: ELEM_* per leaf position (null for tet meshes)
+  const int *cellIndices;   // mixed meshes: the caller's index stream (polyhedra read it)
+  const float4 *verts;      // mixed meshes: vertices {x, y, z, scalar}
   uint32_t numTets;         // elements
   int depth;
   vec3f worldLo, worldHi;  // bounds of the vertices
@@ -59,7 +61,7 @@ BN_HD bool tet_scalar(const float4 *t, vec3f P, float &val) {
 // also r + s); the value interpolates the vertex scalars with the shape
 // functions of the last step's starting point, as the reference does.
 // Vertices follow VTK order; an element is 8 float4 records {x, y, z, s}.
-enum : uint8_t { ELEM_TET = 0, ELEM_PYR = 1, ELEM_PRISM = 2, ELEM_HEX = 3 };
+enum : uint8_t { ELEM_TET = 0, ELEM_PYR = 1, ELEM_PRISM = 2, ELEM_HEX = 3, ELEM_POLY = 4 };
 
 // shape functions sf[N] and derivatives d[3N] (r-, s-, t-derivatives)
 BN_HD void pyr_shape(const float *pc, float *sf, float *d) {
@@ -208,6 +210,64 @@ BN_HD bool newton_scalar(const float4 *V, vec3f P, float &value) {
   return true;
 }
 
+// polyScalar (UMeshField.h:503-620): a polyhedron as a face stream at
+// indices[off]: numFaces, then per face numPts and its vertex indices
+// (validated when the mesh was built). Inside when a ray from P along +x
+// crosses its fan-triangulated faces an odd number of times (Moller-Trumbore
+// per triangle, |det| < 1e-12 skipped); the value is the inverse-squared-
+// distance weighted mean over the stream's vertices (shared ones counted
+// once per face).
+BN_D bool poly_scalar(const UMeshDD &u, int off, vec3f P, float &val) {
+#pragma clang fp contract(off)
+  const int *ix = u.cellIndices;
+  auto vert = [&](int i) {
+    const float4 v = u.verts[i];
+    return vec3f{v.x, v.y, v.z};
+  };
+  int pos = off;
+  const int nf = ix[pos++];
+  int crossings = 0;
+  for (int f = 0; f < nf; ++f) {
+    const int np = ix[pos++];
+    const vec3f a = vert(ix[pos]);
+    int v1 = ix[pos + 1];
+    for (int t = 0; t < np - 2; ++t) {
+      const int v2 = ix[pos + 2 + t];
+      const vec3f e1 = vert(v1) - a, e2 = vert(v2) - a;
+      v1 = v2;
+      const vec3f h = {0.f, -e2.z, e2.y};
+      const float det = dot(e1, h);
+      if (fabsf(det) < 1e-12f) continue;
+      const float inv = 1.f / det;
+      const vec3f s = P - a;
+      const float uu = inv * dot(s, h);
+      if (uu < 0.f || uu > 1.f) continue;
+      const vec3f q = cross(s, e1);
+      const float vv = inv * q.x;
+      if (vv < 0.f || uu + vv > 1.f) continue;
+      if (inv * dot(e2, q) > 0.f) ++crossings;
+    }
+    pos += np;
+  }
+  if ((crossings & 1) == 0) return false;
+  float ws = 0.f, vs = 0.f;
+  pos = off + 1;
+  for (int f = 0; f < nf; ++f) {
+    const int np = ix[pos++];
+    for (int k = 0; k < np; ++k) {
+      const float4 v = u.verts[ix[pos + k]];
+      const vec3f d = vec3f{v.x, v.y, v.z} - P;
+      const float w = 1.f / fmaxf(dot(d, d), 1e-20f);
+      ws += w;
+      vs += w * v.w;
+    }
+    pos += np;
+  }
+  if (ws <= 0.f) return false;
+  val = vs / ws;
+  return true;
+}
+
 // eltScalar (UMeshField.h:362-388) over the packed leaf-order elements of a
 // mixed mesh (kMixed), or tetScalar over packed tets.
 template <bool kMixed>
@@ -218,7 +278,8 @@ BN_D bool elem_scalar(const UMeshDD &u, uint32_t i, vec3f P, float &val) {
     case ELEM_TET: return tet_scalar(e, P, val);
     case ELEM_PYR: return newton_scalar<5, ELEM_PYR>(e, P, val);
     case ELEM_PRISM: return newton_scalar<6, ELEM_PRISM>(e, P, val);
-    default: return newton_scalar<8, ELEM_HEX>(e, P, val);
+    case ELEM_HEX: return newton_scalar<8, ELEM_HEX>(e, P, val);
+    default: return poly_scalar(u, __float_as_int(e[0].x), P, val);
   }
 }
 

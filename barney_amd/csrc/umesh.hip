@@ -16,6 +16,8 @@ struct UMesh {
   World w;                 // point-location BVH (prims freed after packing)
   DevBuf<float4> tets;     // leaf order: 4 per tet, or 8 per general element
   DevBuf<uint8_t> types;   // general elements: ELEM_* per leaf position
+  DevBuf<int> cellIndices; // general elements: copy of the index stream
+  DevBuf<float4> verts;    // general elements: vertices {x, y, z, scalar}
   DevBuf<uint2> rangesOrd; // ordered-uint min/max during rasterization
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
@@ -61,6 +63,7 @@ __device__ __forceinline__ int elem_kind(int type, int &nv) {
     case BNAMD_CELL_PYR: case 3: nv = 5; return ELEM_PYR;
     case BNAMD_CELL_PRISM: case 2: nv = 6; return ELEM_PRISM;
     case BNAMD_CELL_HEX: case 1: nv = 8; return ELEM_HEX;
+    case BNAMD_CELL_POLYHEDRON: nv = 0; return ELEM_POLY;
   }
   nv = 0;
   return -1;
@@ -84,6 +87,26 @@ __global__ void k_elem_boxes(CellsIn c, const float4 *__restrict v4, float4 *__r
   const int off = c.offsets[i];
   bool ok = k >= 0 && off >= 0 && (long long)off + nv <= (long long)c.numIndices;
   vec3f lo = vec3f(BN_INF), hi = vec3f(-BN_INF);
+  if (ok && k == ELEM_POLY) {  // FaceStreamReader's checks (UMeshField.h:179-262)
+    int pos = off;
+    const int nf = pos < c.numIndices ? c.indices[pos++] : 0;
+    ok = nf > 0 && nf <= 8192;
+    for (int f = 0; ok && f < nf; ++f) {
+      const int np = pos < c.numIndices ? c.indices[pos++] : 0;
+      ok = np >= 3 && np <= 1024 && (long long)pos + np <= (long long)c.numIndices;
+      for (int q = 0; ok && q < np; ++q) {
+        const int vi = c.indices[pos + q];
+        ok = vi >= 0 && vi < c.numVertices;
+        if (ok) {
+          const float4 p = v4[vi];
+          lo = vmin(lo, vec3f{p.x, p.y, p.z});
+          hi = vmax(hi, vec3f{p.x, p.y, p.z});
+        }
+      }
+      pos += np;
+    }
+    nv = 0;
+  }
   for (int q = 0; ok && q < nv; ++q) {
     const int vi = c.indices[off + q];
     if (vi < 0 || vi >= c.numVertices) {
@@ -110,9 +133,10 @@ __global__ void k_pack_elems(const PrimRec *__restrict prims, uint32_t n, CellsI
   if (p >= n) return;
   const uint32_t e = __float_as_uint(prims[p].a.w);
   const int k = kinds[e];
-  const int nv = k == ELEM_TET ? 4 : k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : 8;
+  const int nv = k == ELEM_TET ? 4 : k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : k == ELEM_HEX ? 8 : 0;
   const int off = c.offsets[e];
   for (int q = 0; q < 8; ++q) out[8ull * p + q] = q < nv ? v4[c.indices[off + q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (k == ELEM_POLY) out[8ull * p] = make_float4(__int_as_float(off), 0.f, 0.f, 0.f);  // its stream
   outKinds[p] = (uint8_t)k;
 }
 
@@ -198,15 +222,9 @@ __device__ void raster_tet(const MCRaster &g, const V4 &a, const V4 &b, const V4
 // rasterBox(grid, worldBounds, box4f) (MCGrid.cuh:67-91), the variant
 // umeshRasterCells uses for cells other than tets: only an inverted box is
 // skipped, so flat boxes are rasterized
-__device__ void raster_box_cells(const MCRaster &g, const float4 *v, int nv) {
+__device__ void raster_box_range(const MCRaster &g, float lx, float hx, float ly, float hy, float lz, float hz,
+                                 float lw, float hw) {
 #pragma clang fp contract(off)
-  float lx = v[0].x, ly = v[0].y, lz = v[0].z, lw = v[0].w, hx = lx, hy = ly, hz = lz, hw = lw;
-  for (int q = 1; q < nv; ++q) {
-    lx = fminf(lx, v[q].x); hx = fmaxf(hx, v[q].x);
-    ly = fminf(ly, v[q].y); hy = fmaxf(hy, v[q].y);
-    lz = fminf(lz, v[q].z); hz = fmaxf(hz, v[q].z);
-    lw = fminf(lw, v[q].w); hw = fmaxf(hw, v[q].w);
-  }
   if (lx > hx || ly > hy || lz > hz) return;
   int x0 = (int)((lx - g.origin.x) * g.rcpSpacing.x), x1 = (int)((hx - g.origin.x) * g.rcpSpacing.x);
   int y0 = (int)((ly - g.origin.y) * g.rcpSpacing.y), y1 = (int)((hy - g.origin.y) * g.rcpSpacing.y);
@@ -230,14 +248,35 @@ __global__ void k_raster_cells(MCRaster g, CellsIn c, const uint8_t *__restrict 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.numCells) return;
   const int k = kinds[i];
-  const int nv = k == ELEM_TET ? 4 : k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : 8;
-  float4 v[8];
-  for (int q = 0; q < nv; ++q) v[q] = v4[c.indices[c.offsets[i] + q]];
-  if (k == ELEM_TET)
+  const int off = c.offsets[i];
+  if (k == ELEM_TET) {
+    float4 v[4];
+    for (int q = 0; q < 4; ++q) v[q] = v4[c.indices[off + q]];
     raster_tet<5>(g, V4{v[0].x, v[0].y, v[0].z, v[0].w}, V4{v[1].x, v[1].y, v[1].z, v[1].w},
                   V4{v[2].x, v[2].y, v[2].z, v[2].w}, V4{v[3].x, v[3].y, v[3].z, v[3].w});
-  else
-    raster_box_cells(g, v, nv);
+    return;
+  }
+  // the cell's vertices (a polyhedron: every vertex of its face stream)
+  float lx = BN_INF, ly = BN_INF, lz = BN_INF, lw = BN_INF, hx = -BN_INF, hy = -BN_INF, hz = -BN_INF, hw = -BN_INF;
+  auto ext = [&](const float4 p) {
+    lx = fminf(lx, p.x); hx = fmaxf(hx, p.x);
+    ly = fminf(ly, p.y); hy = fmaxf(hy, p.y);
+    lz = fminf(lz, p.z); hz = fmaxf(hz, p.z);
+    lw = fminf(lw, p.w); hw = fmaxf(hw, p.w);
+  };
+  if (k == ELEM_POLY) {
+    int pos = off;
+    const int nf = c.indices[pos++];
+    for (int f = 0; f < nf; ++f) {
+      const int np = c.indices[pos++];
+      for (int q = 0; q < np; ++q) ext(v4[c.indices[pos + q]]);
+      pos += np;
+    }
+  } else {
+    const int nv = k == ELEM_PYR ? 5 : k == ELEM_PRISM ? 6 : 8;
+    for (int q = 0; q < nv; ++q) ext(v4[c.indices[off + q]]);
+  }
+  raster_box_range(g, lx, hx, ly, hy, lz, hz, lw, hw);
 }
 
 __global__ void k_raster_tets(MCRaster g, const int4 *__restrict tets, const float4 *__restrict v4, int n) {
@@ -544,7 +583,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
         int hbad = 0;
         BN_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
         BN_HIP(hipStreamSynchronize(s));
-        BN_REQUIRE(!hbad, "bnamd_umesh_create: a cell has an unknown type or an index out of range");
+        BN_REQUIRE(!hbad, "bnamd_umesh_create: a cell has an unknown type, an index out of range or a malformed face stream");
         g.kind = GEOM_ELEM_BOXES;
         g.vertices = (const float *)boxes.ptr;
       } else {
@@ -559,6 +598,12 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.w.materials.upload(&md, 1, s);
       build_world_bvh(m.w, s);
       if (cells) {
+        // polyhedra read the index stream and the vertices at sample time
+        m.cellIndices.alloc(desc->numIndices);
+        BN_HIP(hipMemcpyAsync(m.cellIndices.ptr, desc->indices, sizeof(int) * desc->numIndices,
+                              hipMemcpyDeviceToDevice, s));
+        m.verts.alloc(desc->numVertices);
+        BN_HIP(hipMemcpyAsync(m.verts.ptr, v4.ptr, sizeof(float4) * desc->numVertices, hipMemcpyDeviceToDevice, s));
         m.tets.alloc(8ull * numElems);
         m.types.alloc(numElems);
         k_pack_elems<<<div_up(numElems, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)numElems, c, kinds.ptr, v4.ptr,
@@ -610,6 +655,8 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.dd.nodes = m.w.nodes.ptr;
       m.dd.tets = m.tets.ptr;
       m.dd.elemTypes = cells ? m.types.ptr : nullptr;
+      m.dd.cellIndices = cells ? m.cellIndices.ptr : nullptr;
+      m.dd.verts = cells ? m.verts.ptr : nullptr;
       m.dd.numTets = (uint32_t)numElems;
       m.dd.depth = m.w.depth;
       m.dd.ranges = m.ranges.ptr;

@@ -376,14 +376,28 @@ _HEX_FACES = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 
 _HEX_TETS5 = ((0, 1, 3, 4), (1, 2, 3, 6), (1, 4, 5, 6), (3, 4, 6, 7), (1, 3, 4, 6))
 
 CELL_TET, CELL_HEX, CELL_PRISM, CELL_PYR = 10, 12, 13, 14
+CELL_POLYHEDRON = 42
 
 
-def mixed_cells(n=10, seed=0x5EED0005, blobs=6):
+def face_stream(faces):
+    """A polyhedron's index stream as UMeshField reads it: the number of
+    faces, then each face's vertex count and vertex indices."""
+    out = [len(faces)]
+    for f in faces:
+        out.append(len(f))
+        out.extend(int(v) for v in f)
+    return out
+
+
+def mixed_cells(n=10, seed=0x5EED0005, blobs=6, polyhedra=False):
     """A conforming mixed mesh in UMeshField's cell layout: an n^3 lattice
     of cells under a shear (an affine map, so every face stays planar); cell
     (i, j, k) is, by (i + 2j + 3k) mod 4, one hex, two prisms (split through
     the bottom / top diagonal), six pyramids on a center vertex, or five
     positively oriented tets. Per-vertex scalars: Gaussian blobs in [0, 1].
+    polyhedra=True writes every other hex (i + j + k even) as a polyhedron
+    with its six quad faces and the first prism of each split as one with
+    two triangles and three quads (VTK face streams).
     Returns vertices (float32 [V,3]), scalars (float32 [V]) and
     cells = (indices int32, offsets int32, types uint8)."""
     m = n + 1
@@ -415,8 +429,14 @@ def mixed_cells(n=10, seed=0x5EED0005, blobs=6):
         base = (k * m + j) * m + i
         h = [base + corner[_HEX_VTK[q]] for q in range(8)]
         kind = (i + 2 * j + 3 * k) % 4
-        if kind == 0:
+        if kind == 0 and polyhedra and (i + j + k) % 2 == 0:
+            add(CELL_POLYHEDRON, face_stream([[h[q] for q in f] for f in _HEX_FACES]))
+        elif kind == 0:
             add(CELL_HEX, h)
+        elif kind == 1 and polyhedra:
+            add(CELL_POLYHEDRON, face_stream([(h[0], h[1], h[2]), (h[4], h[5], h[6]), (h[0], h[1], h[5], h[4]),
+                                              (h[1], h[2], h[6], h[5]), (h[2], h[0], h[4], h[6])]))
+            add(CELL_PRISM, (h[0], h[2], h[3], h[4], h[6], h[7]))
         elif kind == 1:
             add(CELL_PRISM, (h[0], h[1], h[2], h[4], h[5], h[6]))
             add(CELL_PRISM, (h[0], h[2], h[3], h[4], h[6], h[7]))

@@ -300,12 +300,14 @@ int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera, const bna
  * UMeshField's layout (tets == NULL): cell c has type cellTypes[c] and its
  * vertex indices at indices[cellOffsets[c] ...] in VTK order. Types are
  * UMeshField's codes: VTK 10 tet, 12 hex, 13 prism (wedge), 14 pyramid, or
- * ANARI 0 tet, 1 hex, 2 prism, 3 pyramid. Polyhedra are not supported.
+ * ANARI 0 tet, 1 hex, 2 prism, 3 pyramid, and VTK 42 polyhedron, whose
+ * indices are a face stream (UMeshField.h:179-262 FaceStreamReader).
  * --------------------------------------------------------------------- */
 #define BNAMD_CELL_TET 10
 #define BNAMD_CELL_HEX 12
 #define BNAMD_CELL_PRISM 13
 #define BNAMD_CELL_PYR 14
+#define BNAMD_CELL_POLYHEDRON 42 /* face stream: numFaces, then numPts + indices per face */
 
 typedef struct {
   const float *vertices; /* device float3 [numVertices] */

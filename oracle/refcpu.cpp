@@ -2071,12 +2071,62 @@ bool newton_value(int kind, const float *const *V, V3 P, float &value) {
   return true;
 }
 
+// polyScalar (UMeshField.h:503-620): the face stream at ix (numFaces, then
+// per face numPts and its vertex indices). Inside when a ray from P along +x
+// crosses the fan-triangulated faces an odd number of times (Moller-Trumbore,
+// |det| < 1e-12 skipped; the ray direction's zero components written out);
+// the value is the inverse-squared-distance weighted mean over the stream's
+// vertices.
+bool poly_value(const ref_umesh_s *u, const int *ix, V3 P, float &val) {
+  auto vert = [&](int i) { return ld3(&u->v4[4 * (size_t)i]); };
+  int pos = 0;
+  const int nf = ix[pos++];
+  int crossings = 0;
+  for (int f = 0; f < nf; ++f) {
+    const int np = ix[pos++];
+    const V3 a = vert(ix[pos]);
+    for (int t = 0; t < np - 2; ++t) {
+      const V3 e1 = sub(vert(ix[pos + 1 + t]), a), e2 = sub(vert(ix[pos + 2 + t]), a);
+      const V3 h = mk(0.f, -e2.z, e2.y);  // cross((1, 0, 0), e2)
+      const float det = dot(e1, h);
+      if (std::fabs(det) < 1e-12f) continue;
+      const float inv = 1.f / det;
+      const V3 s = sub(P, a);
+      const float uu = inv * dot(s, h);
+      if (uu < 0.f || uu > 1.f) continue;
+      const V3 q = cross(s, e1);
+      const float vv = inv * q.x;  // dot((1, 0, 0), q)
+      if (vv < 0.f || uu + vv > 1.f) continue;
+      if (inv * dot(e2, q) > 0.f) ++crossings;
+    }
+    pos += np;
+  }
+  if ((crossings & 1) == 0) return false;
+  float ws = 0.f, vs = 0.f;
+  pos = 1;
+  for (int f = 0; f < nf; ++f) {
+    const int np = ix[pos++];
+    for (int k = 0; k < np; ++k) {
+      const float *v = &u->v4[4 * (size_t)ix[pos + k]];
+      const V3 d = sub(ld3(v), P);
+      const float w = 1.f / std::fmax(dot(d, d), 1e-20f);
+      ws += w;
+      vs += w * v[3];
+    }
+    pos += np;
+  }
+  if (ws <= 0.f) return false;
+  val = vs / ws;
+  return true;
+}
+
 // eltScalar (UMeshField.h:362-388)
 bool elem_value(const ref_umesh_s *u, int e, V3 P, float &val) {
   if (!u->cells) return tet_value(u, e, P, val);
   const int *ix = &u->indices[(size_t)u->offsets[e]];
   const int k = u->kinds[e];
   if (k == 0) return tet_value_ix(u, ix, P, val);
+  if (k == 4) return poly_value(u, ix, P, val);
   const float *V[8];
   const int N = k == 1 ? 5 : k == 2 ? 6 : 8;
   for (int i = 0; i < N; ++i) V[i] = &u->v4[4 * (size_t)ix[i]];
@@ -2294,33 +2344,46 @@ extern "C" {
 
 namespace {
 
-// vertex indices and count of element e
-void um_elem(const ref_umesh_s *u, int e, const int *&ix, int &nv) {
+// calls f(vertex index) for each vertex of element e (a polyhedron: every
+// entry of its face stream)
+template <class F> void um_each_vertex(const ref_umesh_s *u, int e, F f) {
   if (!u->cells) {
-    ix = &u->tets[4 * (size_t)e];
-    nv = 4;
+    for (int q = 0; q < 4; ++q) f(u->tets[4 * (size_t)e + q]);
     return;
   }
-  ix = &u->indices[(size_t)u->offsets[e]];
+  const int *ix = &u->indices[(size_t)u->offsets[e]];
   const int k = u->kinds[e];
-  nv = k == 0 ? 4 : k == 1 ? 5 : k == 2 ? 6 : 8;
+  if (k == 4) {
+    int pos = 0;
+    const int nf = ix[pos++];
+    for (int fc = 0; fc < nf; ++fc) {
+      const int np = ix[pos++];
+      for (int q = 0; q < np; ++q) f(ix[pos + q]);
+      pos += np;
+    }
+    return;
+  }
+  const int nv = k == 0 ? 4 : k == 1 ? 5 : k == 2 ? 6 : 8;
+  for (int q = 0; q < nv; ++q) f(ix[q]);
 }
+
+bool um_is_tet(const ref_umesh_s *u, int e) { return !u->cells || u->kinds[e] == 0; }
 
 // rasterBox(grid, worldBounds, box4f) (MCGrid.cuh:67-91): cells other than
 // tets; only an inverted box is skipped
-void raster_box_cells(ref_volume_s *g, const int *ix, int nv, const ref_umesh_s *u) {
+void raster_box_cells(ref_volume_s *g, int e, const ref_umesh_s *u) {
   float lo[4], hi[4];
   for (int k = 0; k < 4; ++k) {
     lo[k] = kInf;
     hi[k] = -kInf;
   }
-  for (int i = 0; i < nv; ++i) {
-    const float *c = &u->v4[4 * (size_t)ix[i]];
+  um_each_vertex(u, e, [&](int vi) {
+    const float *c = &u->v4[4 * (size_t)vi];
     for (int k = 0; k < 4; ++k) {
       lo[k] = std::fmin(lo[k], c[k]);
       hi[k] = std::fmax(hi[k], c[k]);
     }
-  }
+  });
   if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
   const float org[3] = {g->mcOrigin.x, g->mcOrigin.y, g->mcOrigin.z};
   const float sp[3] = {g->mcSpacing.x, g->mcSpacing.y, g->mcSpacing.z};
@@ -2352,16 +2415,13 @@ ref_umesh_s *um_build(ref_umesh_s *u, int numElems) {
       pr.lo[k] = kInf;
       pr.hi[k] = -kInf;
     }
-    const int *ix;
-    int nv;
-    um_elem(u, e, ix, nv);
-    for (int q = 0; q < nv; ++q) {
-      const float *p = &u->v4[4 * (size_t)ix[q]];
+    um_each_vertex(u, e, [&](int vi) {
+      const float *p = &u->v4[4 * (size_t)vi];
       for (int k = 0; k < 3; ++k) {
         pr.lo[k] = std::fmin(pr.lo[k], p[k]);
         pr.hi[k] = std::fmax(pr.hi[k], p[k]);
       }
-    }
+    });
     wlo = vmin(wlo, mk(pr.lo[0], pr.lo[1], pr.lo[2]));
     whi = vmax(whi, mk(pr.hi[0], pr.hi[1], pr.hi[2]));
     u->bvh.prims.push_back(pr);
@@ -2384,18 +2444,16 @@ ref_umesh_s *um_build(ref_umesh_s *u, int numElems) {
   g.rhi.assign(nc, -kInf);
   g.maj.assign(nc, 0.f);
   parallel_for(numElems, default_threads(0), [&](int e) {
-    const int *ix;
-    int nv;
-    um_elem(u, e, ix, nv);
-    if (nv == 4) {
+    if (um_is_tet(u, e)) {
       P4 p[4];
-      for (int q = 0; q < 4; ++q) {
-        const float *v = &u->v4[4 * (size_t)ix[q]];
-        p[q] = P4{v[0], v[1], v[2], v[3]};
-      }
+      int q = 0;
+      um_each_vertex(u, e, [&](int vi) {
+        const float *v = &u->v4[4 * (size_t)vi];
+        p[q++] = P4{v[0], v[1], v[2], v[3]};
+      });
       raster_tet(&g, 5, p[0], p[1], p[2], p[3]);
     } else {
-      raster_box_cells(&g, ix, nv, u);
+      raster_box_cells(&g, e, u);
     }
   });
   const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
@@ -2438,9 +2496,21 @@ ref_umesh *ref_umesh_create_cells(const float *vertices, const float *scalars, i
       case 14: case 3: k = 1; nv = 5; break;
       case 13: case 2: k = 2; nv = 6; break;
       case 12: case 1: k = 3; nv = 8; break;
+      case 42: k = 4; nv = 0; break;
     }
     bool ok = k >= 0 && offsets[c] >= 0 && (long long)offsets[c] + nv <= numIndices;
     for (int q = 0; ok && q < nv; ++q) ok = indices[offsets[c] + q] >= 0 && indices[offsets[c] + q] < numVertices;
+    if (ok && k == 4) {  // FaceStreamReader's checks (UMeshField.h:179-262)
+      long long pos = offsets[c];
+      const int nf = pos < numIndices ? indices[pos++] : 0;
+      ok = nf > 0 && nf <= 8192;
+      for (int f = 0; ok && f < nf; ++f) {
+        const int np = pos < numIndices ? indices[pos++] : 0;
+        ok = np >= 3 && np <= 1024 && pos + np <= numIndices;
+        for (int q = 0; ok && q < np; ++q) ok = indices[pos + q] >= 0 && indices[pos + q] < numVertices;
+        pos += np;
+      }
+    }
     if (!ok) {
       delete u;
       return nullptr;

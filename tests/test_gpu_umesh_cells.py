@@ -100,3 +100,62 @@ def test_mixed_renders_per_pixel(mixed, gpu, volume, iso, spp):
     assert close >= 0.999
     assert out["stats"]["raysTraced"] == nr
     assert ra[..., 3].mean() > 0.2
+
+
+@pytest.fixture(scope="module")
+def poly(gpu):
+    V, S, cells = scenes.mixed_cells(N, seed=0x5EED0505, polyhedra=True)
+    assert (cells[2] == scenes.CELL_POLYHEDRON).sum() > 20
+    g = gpu.UMesh(V, S, cells=cells)
+    r = refcpu.UMesh(V, S, cells=cells)
+    tf = scenes.ramp_tf()
+    g.set_tf(tf, (0.0, 1.0), 0.8)
+    r.set_tf(tf, (0.0, 1.0), 0.8)
+    return V, S, cells, g, r
+
+
+def test_polyhedra_point_samples(poly):
+    """Polyhedra (face streams): crossing parity + inverse-distance weights,
+    the same float operations on both sides."""
+    V, S, cells, g, r = poly
+    P = np.random.default_rng(14).uniform(V.min(0) - 0.3, V.max(0) + 0.3, (200_000, 3)).astype(np.float32)
+    a, b = g.sample(P), r.sample(P)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = ~np.isnan(a)
+    exact = (a[m] == b[m]).mean()
+    print(f"inside {m.mean():.3f} exact {exact:.6f}", flush=True)
+    assert m.mean() > 0.3 and exact >= 0.999
+    assert np.abs(a[m] - b[m]).max() <= 1e-6
+
+
+def test_polyhedra_macrocells_bit_exact(poly):
+    V, S, cells, g, r = poly
+    assert g.info()["mcDims"] == r.mc_dims()
+    gr, gm = g.macrocells()
+    rr, rm = r.macrocells()
+    assert np.array_equal(gr, rr) and np.array_equal(gm, rm)
+
+
+def test_polyhedra_render_per_pixel(poly, gpu):
+    V, S, cells, g, r = poly
+    W, H = 80, 60
+    pg = gpu.render_params(W, H, spp=4, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=0)
+    pr = refcpu.render_params(W, H, spp=4, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=0)
+    col = (0.9, 0.7, 0.4)
+    out = g.render(_cam(gpu), pg, volume=True, iso=0.5, iso_color=col)
+    ga = out["accum"].numpy()
+    ra, nr, ns = r.render(_cam(refcpu), pr, volume=True, iso=0.5, iso_color=col)
+    assert np.isfinite(ga).all()
+    rel = np.abs(ga - ra) / np.maximum(np.abs(ra), 1e-3)
+    close = (rel.max(-1) <= 1e-3).mean()
+    print(f"close {close:.5f} samples {out['stats']['samples']} {ns}", flush=True)
+    assert close >= 0.999
+    assert out["stats"]["raysTraced"] == nr
+    assert ra[..., 3].mean() > 0.2
+
+
+def test_bad_face_stream_rejected(gpu):
+    V, S, _ = scenes.mixed_cells(2, seed=3)
+    for bad in ([0], [1, 2, 0, 1], [1, 3, 0, 1, len(V)], [2, 3, 0, 1, 2]):
+        with pytest.raises(Exception):
+            gpu.UMesh(V, S, cells=(np.array(bad, np.int32), np.array([0], np.int32), np.array([42], np.uint8)))

@@ -11,7 +11,13 @@ against what the mathematics fixes:
     points clearly outside get NaN;
   * macrocell ranges bound each cell's vertex scalars (rasterBox of the
     cell's box);
-  * unknown cell types and out-of-range indices are rejected."""
+  * polyhedra (face streams, UMeshField.h:176-268, 503-620): a convex
+    hex written as a polyhedron is inside exactly where the hex is (away
+    from its faces), its value is the float64 inverse-squared-distance
+    mean of its face-stream vertices, and a constant field comes back as
+    that constant;
+  * unknown cell types, out-of-range indices and malformed face streams
+    are rejected."""
 import numpy as np
 import pytest
 from scipy.optimize import root
@@ -114,3 +120,61 @@ def test_bad_cells_rejected():
     badix[0] = len(V)
     with pytest.raises(ValueError):
         R.UMesh(V, S, cells=(badix, off, ty))
+
+
+def test_polyhedra_inside_and_weights():
+    n = 4
+    V, S, cells = scenes.mixed_cells(n, seed=31)
+    Vp, Sp, pcells = scenes.mixed_cells(n, seed=31, polyhedra=True)
+    assert np.array_equal(V, Vp) and np.array_equal(S, Sp)
+    ix, off, ty = pcells
+    assert (ty == scenes.CELL_POLYHEDRON).sum() > 10
+    hexes, polys = R.UMesh(V, S, cells=cells), R.UMesh(V, S, cells=pcells)
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 1.0, 0.15], [0.05, 0.0, 1.0]])
+    g = np.random.default_rng(41).uniform(-0.5, n + 0.5, (40000, 3))
+    P = (g @ A.T).astype(np.float32)
+    # away from the lattice planes (every cell face lies on one)
+    frac = g - np.round(g)
+    away = (np.abs(frac) > 1e-3).all(1)
+    a, b = hexes.sample(P), polys.sample(P)
+    assert np.array_equal(np.isnan(a[away]), np.isnan(b[away]))
+    # cells written as polyhedra: their value is the IDW mean of the stream
+    m = ~np.isnan(b) & away
+    ci = np.floor(g).astype(int)
+    kind = (ci[:, 0] + 2 * ci[:, 1] + 3 * ci[:, 2]) % 4
+    poly_hex = m & (kind == 0) & (ci.sum(1) % 2 == 0) & (ci >= 0).all(1) & (ci < n).all(1)
+    assert poly_hex.sum() > 500
+    # the stream of each hex cell: the one whose vertices are its corners
+    streams = {}
+    for e in np.nonzero(ty == scenes.CELL_POLYHEDRON)[0]:
+        o, nf, vs = off[e], ix[off[e]], []
+        p = o + 1
+        for _ in range(nf):
+            k = ix[p]
+            vs.extend(ix[p + 1:p + 1 + k].tolist())
+            p += 1 + k
+        if nf == 6:
+            lat = np.round(np.linalg.solve(A, V[vs].astype(np.float64).T).T).astype(int).min(0)
+            streams[(lat[0] * n + lat[1]) * n + lat[2]] = vs
+    checked = 0
+    for i in np.nonzero(poly_hex)[0][:800]:
+        vs = streams[(ci[i, 0] * n + ci[i, 1]) * n + ci[i, 2]]
+        d2 = ((V[vs].astype(np.float64) - P[i].astype(np.float64)) ** 2).sum(1)
+        w = 1.0 / np.maximum(d2, 1e-20)
+        want = (w * S[vs]).sum() / w.sum()
+        assert abs(b[i] - want) <= 2e-6 * max(1.0, abs(want)), (i, b[i], want)
+        checked += 1
+    assert checked > 500
+    # a constant field comes back as that constant inside every polyhedron
+    c = R.UMesh(V, np.full(len(V), 0.375, np.float32), cells=pcells).sample(P)
+    assert np.abs(c[~np.isnan(c)] - 0.375).max() <= 1e-6
+
+
+def test_bad_face_streams_rejected():
+    V, S, _ = scenes.mixed_cells(2, seed=3)
+    good = scenes.face_stream([(0, 1, 4, 3), (9, 10, 13, 12), (0, 1, 10, 9), (1, 4, 13, 10), (4, 3, 12, 13),
+                               (3, 0, 9, 12)])
+    R.UMesh(V, S, cells=(np.array(good, np.int32), np.array([0], np.int32), np.array([42], np.uint8)))
+    for bad in ([0], [1, 2, 0, 1], [1, 3, 0, 1, len(V)], [2, 3, 0, 1, 2], [8193] + good[1:]):
+        with pytest.raises(ValueError):
+            R.UMesh(V, S, cells=(np.array(bad, np.int32), np.array([0], np.int32), np.array([42], np.uint8)))
