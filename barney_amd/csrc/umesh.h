@@ -217,7 +217,7 @@ BN_HD bool newton_scalar(const float4 *V, vec3f P, float &value) {
 // per triangle, |det| < 1e-12 skipped); the value is the inverse-squared-
 // distance weighted mean over the stream's vertices (shared ones counted
 // once per face).
-BN_D bool poly_scalar(const UMeshDD &u, int off, vec3f P, float &val) {
+__device__ __attribute__((noinline)) bool poly_scalar(const UMeshDD &u, int off, vec3f P, float &val) {
 #pragma clang fp contract(off)
   const int *ix = u.cellIndices;
   auto vert = [&](int i) {
