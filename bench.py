@@ -343,15 +343,16 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     return res
 
 
-def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0):
+def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0, polyhedra=False):
     """General unstructured cells (an extra workload beside the five; the same path as
     workload 5 with pyramids, prisms and hexahedra next to tets): the sheared
     mixed mesh of scenes.mixed_cells on an n^3 lattice, delta tracking plus
-    an iso-surface at 0.5, 512x512, 4 paths per pixel."""
+    an iso-surface at 0.5, 512x512, 4 paths per pixel. polyhedra=True writes
+    half of the hexes and half of the prisms as VTK face-stream polyhedra."""
     from barney_amd import scenes
     from oracle import refcpu
     t0 = time.time()
-    V, S, cells = scenes.mixed_cells(n)
+    V, S, cells = scenes.mixed_cells(n, polyhedra=polyhedra)
     m = api.UMesh(V, S, cells=cells)
     tf = scenes.ramp_tf()
     m.set_tf(tf, (0.0, 1.0), 0.3)
@@ -602,6 +603,8 @@ def main():
     if not args.no_umesh:
         out["extra"]["umesh_workload5"] = umesh_bench(api, cpu=not args.no_cpu)
         out["extra"]["umesh_mixed_cells"] = umesh_cells_bench(api, cpu=not args.no_cpu)
+        out["extra"]["umesh_mixed_polyhedra"] = umesh_cells_bench(api, cpu=not args.no_cpu, cpu_seconds=4.0,
+                                                                  polyhedra=True)
         out["extra"]["amr_two_levels"] = amr_bench(api, cpu=not args.no_cpu)
     if not args.no_fb:
         out["extra"]["readback_3840x2160"] = fb_bench(api)
