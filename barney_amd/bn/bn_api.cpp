@@ -11,6 +11,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -166,7 +167,7 @@ bnamd_material make_material(const Object *m) {
   r.metallic = m->getf("metallic", 1.f);
   r.roughness = m->getf("roughness", 1.f);
   r.transmission = m->getf("transmission", 0.f);
-  r.ior = m->getf("ior", 1.5f);
+  r.ior = m->getf("ior", 1.45f);  // AnariPBR.h:57
   r.opacity = m->getf("opacity", 1.f);
   return r;
 }
@@ -484,9 +485,18 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   p.maxDiffuseBounces = 1;
   p.russianRoulette = 0;
   int *ids = (fb->ids && fb->paths == 0) ? (int *)fb->ids->ptr : nullptr;
-  if (m->volume)
+  if (m->volume) {
+    // the volume renderer writes no ID channels: a volume model reads back
+    // as all misses (primID/instID/objID -1, depth +inf)
+    if (ids) {
+      const size_t n = (size_t)fb->w * fb->h;
+      std::vector<int32_t> miss(4 * n, -1);
+      const float inf = std::numeric_limits<float>::infinity();
+      for (size_t q = 0; q < n; ++q) std::memcpy(&miss[4 * q + 3], &inf, 4);
+      check(bnamd_memcpy_h2d(ids, miss.data(), miss.size() * 4), "bnRender");
+    }
     check(bnamd_volume_render(m->volume, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
-  else
+  } else
     check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
   fb->paths += p.pathsPerPixel;
 }

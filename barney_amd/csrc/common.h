@@ -140,4 +140,26 @@ BN_HD uint64_t bn_hash(uint32_t v) { return 0xcbf29ce484222325ull ^ (0x100000001
 BN_HD uint64_t bn_hash(uint64_t h, uint32_t v) { return h * 0x100000001b3ull ^ v; }
 BN_HD uint64_t bn_hash3(uint32_t a, uint32_t b, uint32_t c) { return bn_hash(bn_hash(bn_hash(a), b), c); }
 
+// Kernels that give a pixel L lanes run its paths in rounds: in round r lane
+// 'sub' of the pixel's group traces path r*L + sub and holds v = (rgb sum,
+// alpha). This adds the round's first 'count' values to acc in path order
+// (lane base+0, base+1, ...), the order in which a sequential loop over the
+// pixel's paths adds them, so the float sums round alike. Every lane of the
+// wavefront must call it (it shuffles); each ends with its group's acc.
+template <int L>
+BN_D void add_round_in_path_order(float4 &acc, float4 v, int count) {
+  const int base = (int)(threadIdx.x & 63) & ~(L - 1);
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    const float x = __shfl(v.x, base + j), y = __shfl(v.y, base + j);
+    const float z = __shfl(v.z, base + j), w = __shfl(v.w, base + j);
+    if (j < count) {
+      acc.x += x;
+      acc.y += y;
+      acc.z += z;
+      acc.w += w;
+    }
+  }
+}
+
 }  // namespace bnamd

@@ -22,6 +22,7 @@
 #include "shade.h"
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 namespace bnamd {
@@ -35,6 +36,7 @@ constexpr int kMaxViews = 4096;
 
 struct ViewsDD {
   int numViews;
+  int pad;                // explicit: the wavefront graph key reads every byte
   const int *pixOffset;   // [numViews + 1]
   const int *tileOffset;  // [numViews + 1] (megakernel tiles)
   const int2 *size;       // [numViews]
@@ -158,10 +160,12 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
   unsigned long long nrays = 0, nshadow = 0;
   TraceCount tc;
-  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const CameraDD cam = vd.cams[view];
-  if (active) {
-    for (int p = sub; p < spp; p += L) {
+  for (int r = 0; r * L < spp; ++r) {
+    const int p = r * L + sub;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active && p < spp) {
       RenderDD rpp = rp;
       rpp.width = sz.x;
       rpp.height = sz.y;
@@ -192,26 +196,17 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
           add_frag(sum, bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
         }
       }
-      sr += sum.x;
-      sg += sum.y;
-      sb += sum.z;
-      sa += alpha;
+      v = make_float4(sum.x, sum.y, sum.z, alpha);
     }
-  }
-#pragma unroll
-  for (int off = L / 2; off > 0; off >>= 1) {
-    sr += __shfl_xor(sr, off);
-    sg += __shfl_xor(sg, off);
-    sb += __shfl_xor(sb, off);
-    sa += __shfl_xor(sa, off);
+    add_round_in_path_order<L>(acc, v, min(L, spp - r * L));
   }
   if (active && sub == 0) {
     const int pix = vd.pixOffset[view] + ix + iy * sz.x;
     if (rp.accumID == 0) {
-      accum[pix] = make_float4(sr, sg, sb, sa);
+      accum[pix] = acc;
     } else {
       float4 o = accum[pix];
-      o.x += sr; o.y += sg; o.z += sb; o.w += sa;
+      o.x += acc.x; o.y += acc.y; o.z += acc.z; o.w += acc.w;
       accum[pix] = o;
     }
   }
@@ -271,6 +266,7 @@ struct WfDD {
   int chunk;      // rays per dynamic work fetch (one atomic per fetch);
                   // 0: kWalkChunk in k_wf_walk, static 64-ray slices in the
                   // two-level passes
+  int pad;        // explicit tail: the graph key reads every byte
 };
 
 __device__ __forceinline__ unsigned sort_segment(unsigned n) {
@@ -764,6 +760,7 @@ ViewsDD upload_views(BatchState &st, const HostViews &h, hipStream_t s) {
   st.cams.upload(h.cams.data(), h.cams.size(), s);
   ViewsDD vd;
   vd.numViews = (int)h.sizes.size();
+  vd.pad = 0;
   vd.pixOffset = st.pixOffset.ptr;
   vd.tileOffset = st.tileOffset.ptr;
   vd.size = st.sizes.ptr;
@@ -856,6 +853,8 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
   int replays = 0;
   for (int p0 = 0; p0 < spp; p0 += sppBatch) {
     WfDD e;
+    // zeroed first: padding bytes take part in the graph cache key below
+    memset(&e, 0, sizeof e);
     e.w = w.dd();
     e.vd = vd;
     e.rp = rp0;
@@ -874,12 +873,14 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     e.cnt = st.cnt.ptr;
     e.chunk = tune.chunk;
     if (useGraph) {
-      // kernel arguments (accum0, buffers) are baked into the graph, so the
-      // cache key covers them
-      char key[512];
-      snprintf(key, sizeof key, "%p %p %p %d %d %d %d %d %d %p %d %d %zu", (void *)e.w.nodes, (void *)accum,
-               (void *)ids, e.numPixels, e.sppBatch, e.accum0, maxDepth, gridPersist, e.chunk,
-               (void *)st.cams.ptr, rp0.rrDepth, rp0.maxDiffuseBounces, st.q0.count);
+      // The graph bakes every kernel argument (the whole WfDD by value: world
+      // pointers and sizes, render parameters such as ambient and background,
+      // view buffers, queues) and the launch shapes, so the cache key is all
+      // of those bytes: a replay is then exactly the launches enqueue_wavefront
+      // would make.
+      std::string key((const char *)&e, sizeof e);
+      const int shape[4] = {maxDepth, gridPersist, (int)lds, w.twoLevel ? 1 : 0};
+      key.append((const char *)shape, sizeof shape);
       if (!st.exec || st.graphKey != key) {
         if (st.exec) {
           (void)hipGraphExecDestroy(st.exec);

@@ -9,6 +9,7 @@
 //   accumulation     shadeRays.cu:970-1038
 #pragma once
 #include <hip/hip_fp16.h>
+#include "fmath.h"
 #include "world.h"
 
 namespace bnamd {
@@ -167,7 +168,7 @@ BN_HD float sqr(float f) { return f * f; }
 BN_HD float lerp_r(float a, float b, float t) { return (1.f - t) * a + t * b; }
 BN_HD vec3f lerp_r(vec3f a, vec3f b, vec3f t) { return (vec3f(1.f) - t) * a + t * b; }
 BN_HD float saturate(float f) { return fmaxf(0.f, fminf(f, 1.f)); }
-BN_HD float schlick_weight(float c) { return powf(saturate(1.f - c), 5.f); }
+BN_HD float schlick_weight(float c) { return bnf::pow5(saturate(1.f - c)); }
 BN_HD bool same_hemisphere(vec3f wo, vec3f wi, vec3f n) { return dot(wo, n) * dot(wi, n) > 0.f; }
 // NVisii.h:138-145 (no alpha^2 numerator, single power in the denominator)
 BN_HD float gtr_2(float cos_theta_h, float alpha) {
@@ -199,7 +200,9 @@ BN_HD vec3f sample_gtr_2_h(vec3f n, vec3f v_x, vec3f v_y, float alpha, float ux,
   const float c2 = (1.f - uy) / (1.f + (alpha * alpha - 1.f) * uy);
   const float cos_h = sqrtf(c2);
   const float sin_h = 1.f - c2;
-  const vec3f hd = normalize(vec3f(sin_h * cosf(phi_h), sin_h * sinf(phi_h), cos_h));
+  float sphi, cphi;
+  bnf::sincos(phi_h, sphi, cphi);
+  const vec3f hd = normalize(vec3f(sin_h * cphi, sin_h * sphi, cos_h));
   return hd.x * v_x + hd.y * v_y + hd.z * n;
 }
 
@@ -277,7 +280,9 @@ BN_HD vec3f cos_sample_hemisphere(float ux, float uy) {
       theta = (float)M_PI / 2.f - (float)M_PI / 4.f * (sx / sy);
     }
   }
-  const float dx = radius * cosf(theta), dy = radius * sinf(theta);
+  float st, ct;
+  bnf::sincos(theta, st, ct);
+  const float dx = radius * ct, dy = radius * st;
   return {dx, dy, sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy))};
 }
 }  // namespace nvisii
@@ -416,7 +421,7 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedi
     const float cosTheta = sqrtf(sy);
     const float sinTheta = sqrtf(1.0f - sy);
     float sinPhi, cosPhi;
-    sincosf(phi, &sinPhi, &cosPhi);
+    bnf::sincos(phi, sinPhi, cosPhi);
     const vec3f local = {cosPhi * sinTheta, sinPhi * sinTheta, cosTheta};
     s.dir = xfmVector(make_frame(N), local);
     s.pdf = local.z * BN_ONE_OVER_PI;
@@ -489,8 +494,10 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedi
   } else {
     const float r1 = rng();
     const float r2 = rng();
-    const float x = cosf(BN_TWO_PI * r1) * sqrtf(r2 * (1.f - r2));
-    const float y = sinf(BN_TWO_PI * r1) * sqrtf(r2 * (1.f - r2));
+    float sphi, cphi;
+    bnf::sincos(BN_TWO_PI * r1, sphi, cphi);
+    const float x = cphi * sqrtf(r2 * (1.f - r2));
+    const float y = sphi * sqrtf(r2 * (1.f - r2));
     const float z = (1.f - 2.f * r2);
     s.pdf = BN_ONE_OVER_FOUR_PI;
     s.f_r = b.albedo * BN_ONE_OVER_FOUR_PI;

@@ -20,6 +20,7 @@ struct UMeshDD {
   const float4 *tets;       // leaf order: 4 records per tet, or 8 per element of a mixed mesh
   const uint8_t *elemTypes; // mixed meshes: ELEM_* per leaf position (null for tet meshes)
   const int *cellIndices;   // mixed meshes: the caller's index stream (polyhedra read it)
+  const float4 *elemBoxes;  // mixed meshes, per leaf position: {lo, cell index bits}, {hi, 0}
   const float4 *verts;      // mixed meshes: vertices {x, y, z, scalar}
   uint32_t numTets;         // elements
   int depth;
@@ -288,17 +289,27 @@ BN_D bool elem_scalar(const UMeshDD &u, uint32_t i, vec3f P, float &val) {
 // Child boxes are tested in the node's quantized frame: (P - origin) * 2^-e
 // against the 16-bit planes, which carry one quantum of slack on each side,
 // so the test is conservative.
-// 'hint' (a leaf position or ~0u) is tested first and updated to the tet
-// found: consecutive samples along a ray mostly fall into the same tet, and
-// otherwise often into one stored a few records away (tets are in BVH leaf
-// order, i.e. Morton order of their boxes), so kLocateWindow records on
-// either side are tested before a query from the root. The value differs
-// from the traversal-order answer only for points on a face shared by two
-// tets, where both interpolate the same plane.
+// Mixed meshes: the answer is the cell with the smallest caller index whose
+// exact box (of its vertices) holds P and whose inside test passes. All
+// candidates under the conservative quantized node boxes are visited; one
+// with a larger index than the best so far is not tested. The rule does not
+// depend on the BVH, so the oracle's binary BVH gives the same cell.
+// Tet meshes: 'hint' (a leaf position or ~0u) is tested first and
+// updated to the tet found: consecutive samples along a ray mostly fall into
+// the same tet, and otherwise often into one stored a few records away (tets
+// are in BVH leaf order, i.e. Morton order of their boxes), so kLocateWindow
+// records on either side are tested before a query from the root. For tets
+// the value differs from the traversal-order answer only for points on a
+// face shared by two tets, where both interpolate the same plane. Mixed
+// meshes skip the window: their inside tests overlap near shared faces
+// (Newton accepts parameters within 1e-6 outside [0,1], the polyhedron
+// crossing test is not watertight, and a polyhedron's inverse-distance value
+// is discontinuous with its neighbour's), so there the answer must be the
+// first containing cell in traversal order, as the reference's query gives.
 constexpr int kLocateWindow = 4;
 template <bool kMixed = false>
 BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
-  if (hint != ~0u) {
+  if (!kMixed && hint != ~0u) {
     float val;
     if (elem_scalar<kMixed>(u, hint, P, val)) return val;
     for (int d = 1; d <= kLocateWindow; ++d) {
@@ -313,6 +324,8 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
       }
     }
   }
+  uint32_t bestId = ~0u;  // mixed meshes
+  float best = __builtin_nanf("");
   uint32_t grpBase = 0;
   uint32_t grpBits = 1u | (1u << 8);  // root = internal slot 0
   uint32_t primBase = 0, primMask = 0;
@@ -373,7 +386,17 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
       const uint32_t i = __builtin_ctz(primMask);
       primMask &= primMask - 1u;
       float val;
-      if (elem_scalar<kMixed>(u, primBase + i, P, val)) {
+      if (kMixed) {
+        const float4 lo = u.elemBoxes[2ull * (primBase + i)];
+        const uint32_t id = __float_as_uint(lo.w);
+        if (id < bestId && P.x >= lo.x && P.y >= lo.y && P.z >= lo.z) {
+          const float4 hi = u.elemBoxes[2ull * (primBase + i) + 1];
+          if (P.x <= hi.x && P.y <= hi.y && P.z <= hi.z && elem_scalar<kMixed>(u, primBase + i, P, val)) {
+            bestId = id;
+            best = val;
+          }
+        }
+      } else if (elem_scalar<kMixed>(u, primBase + i, P, val)) {
         hint = primBase + i;
         return val;
       }
@@ -386,7 +409,7 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
       grpBits = e.y;
     }
   }
-  return __builtin_nanf("");
+  return kMixed ? best : __builtin_nanf("");
 }
 
 // The iso walk's range test through the per-render bit mask: one bit per

@@ -16,6 +16,7 @@ struct UMesh {
   World w;                 // point-location BVH (prims freed after packing)
   DevBuf<float4> tets;     // leaf order: 4 per tet, or 8 per general element
   DevBuf<uint8_t> types;   // general elements: ELEM_* per leaf position
+  DevBuf<float4> boxes;    // general elements: {lo, cell index bits}, {hi, 0} per leaf position
   DevBuf<int> cellIndices; // general elements: copy of the index stream
   DevBuf<float4> verts;    // general elements: vertices {x, y, z, scalar}
   DevBuf<uint2> rangesOrd; // ordered-uint min/max during rasterization
@@ -128,7 +129,8 @@ __global__ void k_elem_boxes(CellsIn c, const float4 *__restrict v4, float4 *__r
 
 // general elements in BVH leaf order: 8 records per element (unused ones 0)
 __global__ void k_pack_elems(const PrimRec *__restrict prims, uint32_t n, CellsIn c, const uint8_t *__restrict kinds,
-                             const float4 *__restrict v4, float4 *__restrict out, uint8_t *__restrict outKinds) {
+                             const float4 *__restrict v4, float4 *__restrict out, uint8_t *__restrict outKinds,
+                             float4 *__restrict outBoxes) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t e = __float_as_uint(prims[p].a.w);
@@ -138,6 +140,9 @@ __global__ void k_pack_elems(const PrimRec *__restrict prims, uint32_t n, CellsI
   for (int q = 0; q < 8; ++q) out[8ull * p + q] = q < nv ? v4[c.indices[off + q]] : make_float4(0.f, 0.f, 0.f, 0.f);
   if (k == ELEM_POLY) out[8ull * p] = make_float4(__int_as_float(off), 0.f, 0.f, 0.f);  // its stream
   outKinds[p] = (uint8_t)k;
+  const PrimRec r = prims[p];
+  outBoxes[2ull * p] = make_float4(r.a.x, r.a.y, r.a.z, __uint_as_float(e));
+  outBoxes[2ull * p + 1] = make_float4(r.b.x, r.b.y, r.b.z, 0.f);
 }
 
 __device__ __forceinline__ uint32_t ord_of(float f) {
@@ -426,10 +431,12 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
   VolCounters cnt{0, 0};
   unsigned long long nrays = 0;
   uint32_t hint = ~0u;
-  float sr = 0.f, sg = 0.f, sb = 0.f, sa = 0.f;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const bool active = ix < rp.width && iy < rp.height;
-  if (active) {
-    for (int p = sub; p < spp; p += L) {
+  for (int r = 0; r * L < spp; ++r) {
+    const int p = r * L + sub;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active && p < spp) {
       RenderDD rpp = rp;
       rpp.accumID = rp.accumID + p;
       PathRay ray = camera_ray(cam, rpp, ix, iy);
@@ -458,26 +465,17 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
           if (frag.z > 0.f) sum.z += frag.z;
         }
       }
-      sr += sum.x;
-      sg += sum.y;
-      sb += sum.z;
-      sa += alpha;
+      v = make_float4(sum.x, sum.y, sum.z, alpha);
     }
-  }
-#pragma unroll
-  for (int off = L / 2; off > 0; off >>= 1) {
-    sr += __shfl_xor(sr, off);
-    sg += __shfl_xor(sg, off);
-    sb += __shfl_xor(sb, off);
-    sa += __shfl_xor(sa, off);
+    add_round_in_path_order<L>(acc, v, min(L, spp - r * L));
   }
   if (active && sub == 0) {
     const int pix = ix + iy * rp.width;
     if (rp.accumID == 0) {
-      accum[pix] = make_float4(sr, sg, sb, sa);
+      accum[pix] = acc;
     } else {
       float4 a = accum[pix];
-      a.x += sr; a.y += sg; a.z += sb; a.w += sa;
+      a.x += acc.x; a.y += acc.y; a.z += acc.z; a.w += acc.w;
       accum[pix] = a;
     }
   }
@@ -606,8 +604,9 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
         BN_HIP(hipMemcpyAsync(m.verts.ptr, v4.ptr, sizeof(float4) * desc->numVertices, hipMemcpyDeviceToDevice, s));
         m.tets.alloc(8ull * numElems);
         m.types.alloc(numElems);
+        m.boxes.alloc(2ull * numElems);
         k_pack_elems<<<div_up(numElems, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)numElems, c, kinds.ptr, v4.ptr,
-                                                           m.tets.ptr, m.types.ptr);
+                                                           m.tets.ptr, m.types.ptr, m.boxes.ptr);
       } else {
         m.tets.alloc(4ull * numElems);
         k_pack_tets<<<div_up(numElems, 256), 256, 0, s>>>(m.w.prims.ptr, (uint32_t)numElems,
@@ -655,6 +654,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.dd.nodes = m.w.nodes.ptr;
       m.dd.tets = m.tets.ptr;
       m.dd.elemTypes = cells ? m.types.ptr : nullptr;
+      m.dd.elemBoxes = cells ? m.boxes.ptr : nullptr;
       m.dd.cellIndices = cells ? m.cellIndices.ptr : nullptr;
       m.dd.verts = cells ? m.verts.ptr : nullptr;
       m.dd.numTets = (uint32_t)numElems;

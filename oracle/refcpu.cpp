@@ -6,6 +6,9 @@
 // -ffp-contract=off so every fp32 expression rounds exactly as written.
 #include "refcpu.h"
 
+// the sine/cosine/x^5 sequences the kernels use (plain C++, no HIP)
+#include "../barney_amd/csrc/fmath.h"
+
 #include <algorithm>
 #include <atomic>
 #include <array>
@@ -583,7 +586,7 @@ Mat unpack(const HitInfo &h) {
 float saturate(float f) { return std::fmax(0.f, std::fmin(f, 1.f)); }
 float lerp_r(float a, float b, float t) { return (1.f - t) * a + t * b; }
 V3 lerp_r(V3 a, V3 b, float t) { return add(scl(1.f - t, a), scl(t, b)); }
-float schlick(float c) { return std::pow(saturate(1.f - c), 5.f); }
+float schlick(float c) { return bnf::pow5(saturate(1.f - c)); }
 bool same_hemi(V3 wo, V3 wi, V3 n) { return dot(wo, n) * dot(wi, n) > 0.f; }
 float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }  // DG.h:109
 V3 reflect(V3 v, V3 n) { return sub(v, scl(2.f * dot(v, n), n)); }                    // DG.h:115
@@ -638,7 +641,7 @@ V3 cos_hemi(float ux, float uy) {
       theta = (float)M_PI / 2.f - (float)M_PI / 4.f * (sx / sy);
     }
   }
-  const float dx = radius * std::cos(theta), dy = radius * std::sin(theta);
+  const float dx = radius * bnf::cos(theta), dy = radius * bnf::sin(theta);
   return mk(dx, dy, std::sqrt(std::fmax(0.f, 1.f - dx * dx - dy * dy)));
 }
 V3 on_frame(V3 h, V3 vx, V3 vy, V3 n) { return add(add(scl(h.x, vx), scl(h.y, vy)), scl(h.z, n)); }
@@ -647,7 +650,7 @@ V3 sample_gtr_2_h(V3 n, V3 vx, V3 vy, float a, float sx, float sy) {
   const float phi = 2.f * (float)M_PI * sx;
   const float c2 = (1.f - sy) / (1.f + (a * a - 1.f) * sy);
   const float ch = std::sqrt(c2), sh = 1.f - c2;
-  const V3 hd = nrm(mk(sh * std::cos(phi), sh * std::sin(phi), ch));
+  const V3 hd = nrm(mk(sh * bnf::cos(phi), sh * bnf::sin(phi), ch));
   return on_frame(hd, vx, vy, n);
 }
 // disney_brdf (:655-700) with the isotropic microfacet lobe (:480-503)
@@ -822,7 +825,7 @@ Scat bsdf_scatter(const HitInfo &h, V3 Ng, V3 wo, bool inMedium, Rng &rnd) {
     const float sx = rnd.uni(), sy = rnd.uni();
     const float phi = (2.f * kPi) * sx;
     const float ct = std::sqrt(sy), st = std::sqrt(1.0f - sy);
-    const V3 l = mk(std::cos(phi) * st, std::sin(phi) * st, ct);
+    const V3 l = mk(bnf::cos(phi) * st, bnf::sin(phi) * st, ct);
     s.dir = frame_xfm(N, l);
     s.pdf = l.z * (1.f / kPi);
     s.fr = h.albedo;
@@ -900,8 +903,8 @@ Scat bsdf_scatter(const HitInfo &h, V3 Ng, V3 wo, bool inMedium, Rng &rnd) {
     s.type = S_SPECULAR;
   } else {  // Phase.h:143-172
     const float r1 = rnd.uni(), r2 = rnd.uni();
-    const float x = std::cos(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
-    const float y = std::sin(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
+    const float x = bnf::cos(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
+    const float y = bnf::sin(2.f * kPi * r1) * std::sqrt(r2 * (1.f - r2));
     const float z = (1.f - 2.f * r2);
     s.pdf = 1.f / (4.f * kPi);
     s.fr = scl(s.pdf, h.albedo);
@@ -1105,6 +1108,14 @@ void ref_bsdf_eval(const ref_geom *g, const float *color, const float *Ng, const
   out[3] = bsdf_eval(h, ld3(Ng), ld3(wo), ld3(wi), v);
   out[0] = v.x; out[1] = v.y; out[2] = v.z;
   out[4] = bsdf_pdf(h, ld3(Ng), ld3(wo), ld3(wi));
+}
+
+// the shared sine / cosine / x^5 (fmath.h), for their own pins
+void ref_fmath_eval(const float *x, float *sinOut, float *cosOut, float *pow5Out, int n) {
+  for (int i = 0; i < n; ++i) {
+    bnf::sincos(x[i], sinOut[i], cosOut[i]);
+    pow5Out[i] = bnf::pow5(x[i]);
+  }
 }
 
 // Volume.h:174-184
@@ -2134,7 +2145,11 @@ bool elem_value(const ref_umesh_s *u, int e, V3 P, float &val) {
 }
 
 // UMeshCuBQLSampler.h:51-67: box query with the degenerate box {P, P}; the
-// first element whose inside test passes gives the value, NaN if none
+// first element whose inside test passes gives the value, NaN if none. For
+// mixed meshes, whose inside tests overlap near shared faces, the element
+// taken is the one with the smallest cell index whose own box holds P and
+// whose inside test passes (a rule independent of the BVH; the reference
+// takes the first its cuBQL query meets).
 float um_sample(const ref_umesh_s *u, V3 P) {
   const auto &nodes = u->bvh.nodes;
   if (nodes.empty()) return NAN;
@@ -2142,6 +2157,8 @@ float um_sample(const ref_umesh_s *u, V3 P) {
   int sp = 0;
   stack[sp++] = 0;
   const float p[3] = {P.x, P.y, P.z};
+  int bestE = -1;
+  float best = NAN;
   while (sp) {
     const auto &n = nodes[stack[--sp]];
     bool in = true;
@@ -2149,15 +2166,23 @@ float um_sample(const ref_umesh_s *u, V3 P) {
     if (!in) continue;
     if (n.left < 0) {
       for (int i = 0; i < n.count; ++i) {
+        const Prim &pr = u->bvh.prims[n.first + i];
+        const int e = pr.prim;
         float v;
-        if (elem_value(u, u->bvh.prims[n.first + i].prim, P, v)) return v;
+        if (!u->cells) {
+          if (elem_value(u, e, P, v)) return v;
+        } else if ((bestE < 0 || e < bestE) && p[0] >= pr.lo[0] && p[1] >= pr.lo[1] && p[2] >= pr.lo[2] &&
+                   p[0] <= pr.hi[0] && p[1] <= pr.hi[1] && p[2] <= pr.hi[2] && elem_value(u, e, P, v)) {
+          bestE = e;
+          best = v;
+        }
       }
     } else {
       stack[sp++] = n.left + 1;
       stack[sp++] = n.left;
     }
   }
-  return NAN;
+  return best;
 }
 
 // rasterBox(grid, box4f) (MCGrid.cuh:92-122) and rasterTet<D>
@@ -2577,6 +2602,32 @@ void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, f
     o[1] = h.hit ? h.N.x : 0.f;
     o[2] = h.hit ? h.N.y : 0.f;
     o[3] = h.hit ? h.N.z : 0.f;
+  });
+}
+
+// Brute force: the first element in index order whose inside test passes
+// (for mixed meshes, whose own box also holds P: the smallest index that
+// um_sample's BVH query must also return)
+void ref_umesh_sample_brute(const ref_umesh *u, const float *points, float *out, int n, int threads) {
+  const int numElems = u->cells ? (int)u->kinds.size() : (int)u->tets.size() / 4;
+  // element boxes by element index (the BVH holds them in its own order)
+  std::vector<const Prim *> box(numElems);
+  for (const Prim &pr : u->bvh.prims) box[pr.prim] = &pr;
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const V3 P = ld3(points + 3 * (size_t)i);
+    const float p[3] = {P.x, P.y, P.z};
+    out[i] = NAN;
+    for (int e = 0; e < numElems; ++e) {
+      const Prim &pr = *box[e];
+      if (u->cells && !(p[0] >= pr.lo[0] && p[1] >= pr.lo[1] && p[2] >= pr.lo[2] && p[0] <= pr.hi[0] &&
+                        p[1] <= pr.hi[1] && p[2] <= pr.hi[2]))
+        continue;
+      float v;
+      if (elem_value(u, e, P, v)) {
+        out[i] = v;
+        break;
+      }
+    }
   });
 }
 

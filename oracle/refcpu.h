@@ -25,6 +25,7 @@ void ref_rng_sequence(uint64_t *state, uint32_t *out, int n);
 /* Random2(a,b) then n floats */
 void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n);
 uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c);
+void ref_fmath_eval(const float *x, float *sinOut, float *cosOut, float *pow5Out, int n);
 float ref_fast_log(float f);
 
 /* --- geometry input (host arrays) --- */
@@ -160,6 +161,7 @@ void ref_umesh_mc_dims(const ref_umesh *u, int *dims);
 void ref_umesh_get_mcs(const ref_umesh *u, float *ranges, float *majorants);
 /* point location + barycentric interpolation; NaN outside */
 void ref_umesh_sample(const ref_umesh *u, const float *points, float *out, int n, int threads);
+void ref_umesh_sample_brute(const ref_umesh *u, const float *points, float *out, int n, int threads);
 /* iso-surface hit per ray (8 floats org,tmin,dir,tmax): out = t (inf if
  * none), normal[3] */
 void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, float *out, int n, int threads);

@@ -98,6 +98,8 @@ def load():
         "ref_umesh_mc_dims": (None, [P, P]),
         "ref_umesh_get_mcs": (None, [P, P, P]),
         "ref_umesh_sample": (None, [P, P, P, C.c_int, C.c_int]),
+        "ref_fmath_eval": (None, [P, P, P, P, C.c_int]),
+        "ref_umesh_sample_brute": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_umesh_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
         "ref_umesh_render": (C.c_longlong, [P, P, P, C.c_int, C.c_int, C.c_float, P, P, C.c_int, C.c_int, C.c_int,
                                             P]),
@@ -386,6 +388,13 @@ class UMesh:
         load().ref_umesh_sample(self.handle, p.ctypes.data, out.ctypes.data, len(p), threads)
         return out
 
+    def sample_brute(self, points, threads=0):
+        """Every element in index order; the first containing P gives the value."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(len(p), np.float32)
+        load().ref_umesh_sample_brute(self.handle, p.ctypes.data, out.ctypes.data, len(p), threads)
+        return out
+
     def iso_rays(self, rays, iso, threads=0):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         out = np.empty((len(r), 4), np.float32)
@@ -541,3 +550,11 @@ def fb_to_fixed8(rgba, srgb):
     out = np.zeros(a.shape[:-1], np.uint32)
     load().ref_fb_to_fixed8(a.ctypes.data, a.size // 4, int(bool(srgb)), out.ctypes.data)
     return out
+
+
+def fmath(x):
+    """sin, cos and x^5 of the shared fmath.h sequences, elementwise."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    s, c, p = (np.empty_like(x) for _ in range(3))
+    load().ref_fmath_eval(x.ctypes.data, s.ctypes.data, c.ctypes.data, p.ctypes.data, len(x))
+    return s, c, p

@@ -146,6 +146,50 @@ int main() {
   bnamd_synchronize();
   bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
   if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("color after bnAccumReset");
+  // a physicallyBased material that sets only baseColor and transmission
+  // takes AnariPBR's default ior 1.45 (AnariPBR.h:57), which makes it Glass
+  {
+    BNMaterial glass = bnMaterialCreate(ctx, 0, "physicallyBased");
+    bnSet3f(glass, "baseColor", .9f, .9f, .9f);
+    bnSet1f(glass, "transmission", 1.f);
+    BNGeom gs = bnGeometryCreate(ctx, 0, "spheres");
+    bnSetData(gs, "origins", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 3, so));
+    bnSetData(gs, "radii", bnDataCreate(ctx, 0, BN_FLOAT32, 3, sr));
+    bnSetObject(gs, "material", glass);
+    bnSet1i(gs, "userID", 9);
+    bnCommit(gs);
+    BNGeom gg[2] = {tri, gs};
+    BNGroup ggrp = bnGroupCreate(ctx, 0, gg, 2, nullptr, 0);
+    bnGroupBuild(ggrp);
+    BNModel gm = bnModelCreate(ctx);
+    bnSetInstances(gm, 0, &ggrp, &xfm, 1);
+    bnBuild(gm, 0);
+    bnAccumReset(fb);
+    bnRender(rnd, gm, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    bnamd_material mg[2] = {m[0], m[1]};
+    mg[1].color[0] = mg[1].color[1] = mg[1].color[2] = .9f;
+    mg[1].metallic = 1.f; mg[1].roughness = 1.f; mg[1].transmission = 1.f; mg[1].ior = 1.45f;
+    bnamd_world *gw = nullptr;
+    if (bnamd_world_build(&grp, 1, &in, 1, mg, 2, nullptr, &gw) != BNAMD_OK) return fail(bnamd_last_error());
+    p.accumID = 0;
+    bnamd_render_surface(gw, &c, &p, (float *)acc, nullptr, nullptr, nullptr);
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("PBR default ior differs from 1.45");
+    // ior 1.5 renders a different image: the default is what decided it
+    mg[1].ior = 1.5f;
+    bnamd_world *gw2 = nullptr;
+    if (bnamd_world_build(&grp, 1, &in, 1, mg, 2, nullptr, &gw2) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_render_surface(gw2, &c, &p, (float *)acc, nullptr, nullptr, nullptr);
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) == 0) return fail("ior 1.5 and 1.45 render alike");
+    bnamd_world_destroy(gw);
+    bnamd_world_destroy(gw2);
+  }
   // a structured volume through bn* equals bnamd_volume_* with the same
   // field, transfer function and cellsPerMC 8
   {
@@ -188,6 +232,11 @@ int main() {
     bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
     if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("volume color differs");
     if (vs.samples <= 0) return fail("volume not sampled");
+    // a volume model has no surface hits: its ID channels read as misses
+    bnFrameBufferRead(fb, BN_FB_PRIMID, prim.data(), BN_INT32);
+    bnFrameBufferRead(fb, BN_FB_DEPTH, depth.data(), BN_FLOAT32);
+    for (int q = 0; q < W * H; ++q)
+      if (prim[q] != -1 || !std::isinf(depth[q])) return fail("volume model ID channels not cleared");
     std::printf("volume samples %lld\n", vs.samples);
     bnamd_volume_destroy(v);
     bnamd_device_free(dfield);

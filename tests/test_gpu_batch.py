@@ -68,3 +68,23 @@ def test_autotune_writes_choice(gpu, tmp_path):
     assert d["traceBlocksPerCU"] in (0, 2, 3, 4) and d["chunk"] in (0, 64, 256)  # flattened world
     out = gpu.render_batch(world, views, _params(gpu, w3, spp=1))["accum"].numpy()
     assert np.isfinite(out).all()
+
+
+def test_graph_replay_follows_render_parameters(gpu):
+    """The captured wavefront graph bakes its kernel arguments, so two frames
+    of the same world and views that differ only in ambient radiance and
+    background must not replay the first frame's values: each wavefront
+    frame equals the megakernel frame with the same parameters, and the two
+    frames differ."""
+    world, views, w3 = _scene(gpu, views=3)
+    frames = []
+    for ambient, bg in ((1.0, (-1, -1, -1, -1)), (0.25, (0.1, 0.3, 0.7, 1.0))):
+        p = gpu.render_params(0, 0, spp=2, ambient=ambient, bg=bg, max_depth=w3["max_depth"],
+                              max_diffuse=w3["max_diffuse"], russian_roulette=w3["rr_depth"])
+        wave = gpu.render_batch(world, views, p, gpu.BATCH_WAVEFRONT)
+        mega = gpu.render_batch(world, views, p, gpu.BATCH_MEGAKERNEL)
+        assert wave["stats"]["graphReplays"] >= 1
+        a, b = mega["accum"].numpy(), wave["accum"].numpy()
+        np.testing.assert_allclose(b, a, rtol=1e-4, atol=1e-5)
+        frames.append(b)
+    assert np.abs(frames[0] - frames[1]).max() > 0.05

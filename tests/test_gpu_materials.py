@@ -5,20 +5,22 @@ the medium flag; Russian roulette; many diffuse bounces).
 
 Every view is rendered by the oracle on its own with the same per-path RNG
 streams. The stream of each bounce is hashed from the bits of the hit
-distance (Random(ray.rngSeed, t), shadeRays.cu:610), so a last-ulp
-difference between device and host libm (sincosf, powf, sqrtf) re-seeds the
-rest of that path. Measured per-pixel mismatch rates after one bounce are
-uniform over materials and shapes: ~8% for Lambertian scattering (sincosf), ~3% for
-NVisii (powf, sinf, cosf) and 0 for Glass, whose arithmetic is exactly
-rounded on both sides. Hence three bars:
-  * Glass only: per-pixel 1e-3 relative on >= 99.9% of pixels, at depth 1
-    and 8;
-  * one bounce (max depth 1), mixed materials: per-pixel 1e-3 relative on
-    >= 90% of pixels;
-  * long paths (depth 8, Russian roulette from depth 2, 8 diffuse bounces):
-    no bias -- the mean of the per-pixel differences of every view and
-    channel within 4.5 standard errors of zero, alpha bit-exact, and the
-    number of traced rays within 3%."""
+distance (Random(ray.rngSeed, t), shadeRays.cu:610), so a path stays on the
+oracle's stream only if every value it computes rounds alike on both sides.
+The kernels and the oracle therefore share one sine / cosine / x^5 sequence
+(fmath.h: correctly rounded operations only, contraction off), and a
+pixel's paths are added in path order on both. Bars:
+  * same streams, every material mix, depth 1 and depth 8 with Russian
+    roulette: per-pixel 1e-3 relative on >= 99.9 % of pixels, alpha
+    bit-exact, ray counts within 0.1 %;
+  * independent streams (GPU paths 0..N-1, oracle paths N..2N-1, N = 256 in
+    16 frames): every pixel's converged mean within 3 sigma of the
+    oracle's on >= 99 % of pixels (the normal rate outside 3 sigma is
+    0.27 %) and within 6 sigma on all;
+  * converged RMSE (the workload's target, "<= 1 % vs the CPU reference"):
+    at 256 paths per pixel, the RMSE of the per-pixel means against the
+    oracle's, relative to the oracle's mean, <= 1 % for the megakernel and
+    the wavefront renderer."""
 import numpy as np
 import pytest
 
@@ -68,43 +70,99 @@ def _render_both(gpu, w3, world, ref, views, ws, hs, max_depth, max_diffuse, rr,
     return pairs, out["stats"], rays
 
 
-def test_one_bounce_per_pixel(gpu):
-    w3, world, ref, views, ws, hs = _setup(gpu)
-    pairs, stats, rays = _render_both(gpu, w3, world, ref, views, ws, hs, max_depth=1, max_diffuse=8, rr=0)
+def _per_pixel(pairs, ws, hs, what, bar=0.999):
     for v, (g, r) in enumerate(pairs):
         rel = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
         close = (rel.max(-1) <= 1e-3).mean()
-        print(f"view {v} {ws[v]}x{hs[v]} per-pixel close {close:.4f}", flush=True)
-        assert close >= 0.90, (v, close)
+        exact = np.all(g == r, axis=-1).mean()
+        print(f"{what} view {v} {ws[v]}x{hs[v]} per-pixel close {close:.5f} bit-exact {exact:.5f}", flush=True)
+        assert close >= bar, (v, close)
         assert np.array_equal(g[..., 3], r[..., 3])
-    assert abs(stats["raysTraced"] - rays) <= 0.01 * rays
-
-
-def _unbiased(pairs, ws, hs):
-    for v, (g, r) in enumerate(pairs):
-        assert np.array_equal(g[..., 3], r[..., 3])
-        d = (g[..., :3] - r[..., :3]).reshape(-1, 3).astype(np.float64)
-        z = d.mean(0) / (d.std(0) / np.sqrt(len(d)) + 1e-12)
-        print(f"view {v} {ws[v]}x{hs[v]} z {np.round(z, 2)} mean g {g[..., :3].mean():.4f} "
-              f"r {r[..., :3].mean():.4f}", flush=True)
-        assert np.all(np.abs(z) <= 4.5), (v, z)
         assert r[..., :3].mean() > 0.05
+
+
+def test_one_bounce_per_pixel(gpu):
+    w3, world, ref, views, ws, hs = _setup(gpu)
+    pairs, stats, rays = _render_both(gpu, w3, world, ref, views, ws, hs, max_depth=1, max_diffuse=8, rr=0)
+    _per_pixel(pairs, ws, hs, "depth 1")
+    assert abs(stats["raysTraced"] - rays) <= 1e-3 * rays
 
 
 def test_long_paths_mixed_materials_roulette(gpu):
     w3, world, ref, views, ws, hs = _setup(gpu)
     pairs, stats, rays = _render_both(gpu, w3, world, ref, views, ws, hs, max_depth=w3["max_depth"],
                                       max_diffuse=w3["max_diffuse"], rr=w3["rr_depth"])
-    _unbiased(pairs, ws, hs)
-    assert abs(stats["raysTraced"] - rays) <= 0.03 * rays
+    _per_pixel(pairs, ws, hs, "depth 8 + RR")
+    assert abs(stats["raysTraced"] - rays) <= 1e-3 * rays
 
 
 def test_long_paths_matte_no_roulette(gpu):
     w3, world, ref, views, ws, hs = _setup(
         gpu, lambda mats: [dict(type="matte", color=m.get("color", (0.8, 0.8, 0.8))) for m in mats])
     pairs, stats, rays = _render_both(gpu, w3, world, ref, views, ws, hs, max_depth=8, max_diffuse=8, rr=0)
-    _unbiased(pairs, ws, hs)
-    assert abs(stats["raysTraced"] - rays) <= 0.03 * rays
+    _per_pixel(pairs, ws, hs, "matte depth 8")
+    assert abs(stats["raysTraced"] - rays) <= 1e-3 * rays
+
+
+def _frames(render, n_frames, spp, first):
+    """Per-pixel mean and standard error over n_frames frames of spp paths
+    (accumIDs first, first + spp, ...): each frame is an independent
+    estimate of the pixel."""
+    f = np.stack([render(first + k * spp, spp)[..., :3] / spp for k in range(n_frames)]).astype(np.float64)
+    return f.mean(0), f.std(0, ddof=1) / np.sqrt(n_frames)
+
+
+def _view0_band(gpu, w3, world, ref, views, ws, hs, mode):
+    """View 0 of the reduced scene, depth 8 with Russian roulette: a GPU
+    render function (the whole batch, view 0 cut out) and the oracle's."""
+    c = w3["cameras"][0]
+
+    def gpu_render(accum_id, spp):
+        p = gpu.render_params(0, 0, spp=spp, accum_id=accum_id, ambient=1.0, bg=(-1, -1, -1, -1),
+                              max_depth=w3["max_depth"], max_diffuse=w3["max_diffuse"],
+                              russian_roulette=w3["rr_depth"])
+        # a frame with accum_id > 0 adds to the buffer: start it at zero
+        acc = gpu.DeviceArray(np.zeros((views.num_pixels, 4), np.float32))
+        out = gpu.render_batch(world, views, p, mode, accum=acc)
+        return views.image(out["accum"].numpy(), 0)
+
+    def ref_render(accum_id, spp):
+        rp = refcpu.render_params(ws[0], hs[0], spp=spp, accum_id=accum_id, ambient=1.0, bg=(-1, -1, -1, -1),
+                                  max_depth=w3["max_depth"], max_diffuse=w3["max_diffuse"],
+                                  russian_roulette=w3["rr_depth"])
+        return ref.render(refcpu.camera(c["position"], c["direction"], fovy=c["fovy"]), rp)[0]
+
+    return gpu_render, ref_render
+
+
+def test_converged_per_pixel_mean_within_3_sigma(gpu):
+    """Independent streams: the GPU's paths 0..255 against the oracle's paths
+    256..511, 16 frames of 16 each; per pixel, the difference of the means
+    over its standard error."""
+    w3, world, ref, views, ws, hs = _setup(gpu)
+    gr, rr = _view0_band(gpu, w3, world, ref, views, ws, hs, gpu.BATCH_MEGAKERNEL)
+    gm, gs = _frames(gr, 16, 16, 0)
+    rm, rs = _frames(rr, 16, 16, 256)
+    z = np.abs(gm - rm) / np.sqrt(gs ** 2 + rs ** 2 + 1e-12)
+    out3 = (z > 3.0).mean()
+    print(f"view 0 {ws[0]}x{hs[0]}: beyond 3 sigma {out3:.4f}, max z {z.max():.2f}, "
+          f"means {gm.mean():.4f} {rm.mean():.4f}", flush=True)
+    assert out3 <= 0.01 and z.max() <= 6.0
+    assert rm.mean() > 0.05
+
+
+@pytest.mark.parametrize("mode", ["megakernel", "wavefront"])
+def test_converged_rmse_within_1_percent(gpu, mode):
+    """256 paths per pixel, the same streams on both sides: RMSE of the
+    per-pixel means relative to the oracle's image mean."""
+    w3, world, ref, views, ws, hs = _setup(gpu)
+    m = gpu.BATCH_MEGAKERNEL if mode == "megakernel" else gpu.BATCH_WAVEFRONT
+    gr, rr = _view0_band(gpu, w3, world, ref, views, ws, hs, m)
+    g = gr(0, 256)[..., :3].astype(np.float64) / 256
+    r = rr(0, 256)[..., :3].astype(np.float64) / 256
+    rmse = np.sqrt(((g - r) ** 2).mean()) / r.mean()
+    print(f"{mode} view 0 {ws[0]}x{hs[0]} 256 spp: relative RMSE {rmse:.2e}", flush=True)
+    assert rmse <= 0.01
 
 
 @pytest.mark.parametrize("depth", [1, 8])

@@ -1,10 +1,10 @@
 """GPU surface path-tracing parity (workload-1 style, reduced): the
 megakernel against the oracle's restatement of generateRays + bounce().
 
-Both sides follow each ray's own RNG stream, so per-pixel values agree up to
-the last-ulp differences of sqrtf/sincosf between device and host libm; the
-bar is per-pixel 1e-3 relative on >= 99% of pixels, the ID channel
-bit-exact, and the image mean within 3 sigma."""
+Both sides follow each ray's own RNG stream with the same arithmetic (the
+bounce's sine and cosine are the shared fmath.h sequence, not libm), so the
+bar is per-pixel 1e-3 relative on >= 99.9 % of pixels (bit-exact is
+reported), the ID channel bit-exact, and the image mean within 3 sigma."""
 import numpy as np
 import pytest
 
@@ -29,7 +29,8 @@ def test_render_matches_oracle(gpu):
     assert np.array_equal(out["ids"][..., 3], rids[..., 3])
     rel = np.abs(gacc - racc) / np.maximum(np.abs(racc), 1e-3)
     frac_close = (rel.max(-1) <= 1e-3).mean()
-    assert frac_close >= 0.99, frac_close
+    print(f"close {frac_close:.5f} bit-exact {np.all(gacc == racc, axis=-1).mean():.5f}", flush=True)
+    assert frac_close >= 0.999, frac_close
     # statistical agreement of the image mean
     g = gacc[..., :3].reshape(-1, 3) / spp
     r = racc[..., :3].reshape(-1, 3) / spp
@@ -70,4 +71,4 @@ def test_every_pixel_written_any_frame_size(gpu, W, H):
     assert np.isfinite(gacc).all()
     assert np.array_equal(out["ids"], rids)
     rel = np.abs(gacc - racc) / np.maximum(np.abs(racc), 1e-3)
-    assert (rel.max(-1) <= 1e-3).mean() >= 0.99
+    assert (rel.max(-1) <= 1e-3).mean() >= 0.999

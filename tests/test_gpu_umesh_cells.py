@@ -4,12 +4,14 @@ mesh, UMeshField's layout) on the GPU against the oracle.
 The element tests (tetScalar and the Newton inversions of
 ElementIntersection.h), the cell boxes, the macrocell rasterization
 (rasterTet<5> for tets, rasterBox for other cells) and the walks are
-exactly rounded on both sides; only the point-location order differs (BVH8
-vs the oracle's binary BVH), which matters only on faces two cells share:
+exactly rounded on both sides, and where inside tests overlap (near faces
+two cells share) both sides take the containing cell with the smallest
+index, whatever their BVHs' orders:
   * point samples on non-affine (jittered) single elements: bit-exact,
     NaN outside included;
-  * point samples on the sheared mixed mesh: equal on >= 99.9 % of points,
-    all within 1e-6;
+  * point samples on the sheared mixed mesh, random and within 1e-6 of
+    shared faces (hex / polyhedron / prism / pyramid / tet neighbours):
+    bit-exact;
   * macrocell ranges and majorants bit-exact;
   * volume-only and iso-only renders (NEE only) equal per pixel on >= 99.9 %
     of pixels, with the same ray counts."""
@@ -59,12 +61,36 @@ def test_mixed_point_samples(mixed):
     rng = np.random.default_rng(13)
     P = rng.uniform(V.min(0) - 0.3, V.max(0) + 0.3, (200_000, 3)).astype(np.float32)
     a, b = g.sample(P), r.sample(P)
-    assert np.array_equal(np.isnan(a), np.isnan(b))
     m = ~np.isnan(a)
-    exact = (a[m] == b[m]).mean()
-    print(f"inside {m.mean():.3f} exact {exact:.6f}", flush=True)
-    assert m.mean() > 0.3 and exact >= 0.999
-    assert np.abs(a[m] - b[m]).max() <= 1e-6
+    print(f"inside {m.mean():.3f} exact {(a[m] == b[m]).mean():.6f}", flush=True)
+    assert m.mean() > 0.3
+    assert np.array_equal(a, b, equal_nan=True)
+
+
+def _near_shared_faces(n, count, seed):
+    """Points within 1e-6 of the lattice faces that neighbouring cells share
+    (one grid coordinate an interior integer), mapped by the mesh's shear."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(0.0, float(n), (count, 3))
+    axis = rng.integers(0, 3, count)
+    g[np.arange(count), axis] = rng.integers(1, n, count)
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 1.0, 0.15], [0.05, 0.0, 1.0]])
+    P = g @ A.T + rng.uniform(-1e-6, 1e-6, (count, 3))
+    return P.astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["mixed", "poly"])
+def test_shared_face_points_bit_exact(which, request):
+    """Near faces shared by two cells (a polyhedron and a hex among them) the
+    overlapping inside tests are resolved by the smallest cell index on both
+    sides, so the value does not depend on either side's BVH order."""
+    V, S, cells, g, r = request.getfixturevalue(which)
+    P = _near_shared_faces(N, 100_000, 15)
+    a, b = g.sample(P), r.sample(P)
+    m = ~np.isnan(a)
+    print(f"{which}: inside {m.mean():.3f} exact {(a[m] == b[m]).mean():.6f}", flush=True)
+    assert m.mean() > 0.9
+    assert np.array_equal(a, b, equal_nan=True)
 
 
 def test_mixed_macrocells_bit_exact(mixed):
@@ -120,12 +146,10 @@ def test_polyhedra_point_samples(poly):
     V, S, cells, g, r = poly
     P = np.random.default_rng(14).uniform(V.min(0) - 0.3, V.max(0) + 0.3, (200_000, 3)).astype(np.float32)
     a, b = g.sample(P), r.sample(P)
-    assert np.array_equal(np.isnan(a), np.isnan(b))
     m = ~np.isnan(a)
-    exact = (a[m] == b[m]).mean()
-    print(f"inside {m.mean():.3f} exact {exact:.6f}", flush=True)
-    assert m.mean() > 0.3 and exact >= 0.999
-    assert np.abs(a[m] - b[m]).max() <= 1e-6
+    print(f"inside {m.mean():.3f} exact {(a[m] == b[m]).mean():.6f}", flush=True)
+    assert m.mean() > 0.3
+    assert np.array_equal(a, b, equal_nan=True)
 
 
 def test_polyhedra_macrocells_bit_exact(poly):
