@@ -133,7 +133,7 @@ def volume_cpu_baseline(w2, seconds):
     base = band_timer(lambda y0, y1: rv.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
                       "Msamples/s", f"the {W}x{H}x{w2['spp']}spp frame", keep_rows=True)
     y0, y1 = base["_rows"]
-    return base, acc[y0:y1]
+    return base, acc[y0:y1], ref
 
 
 def surface_cpu_baseline(w1, seconds):
@@ -178,7 +178,7 @@ def batch_cpu_baseline(w3, seconds):
     base = band_timer(lambda y0, y1: ref.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
                       "Mrays/s", f"view 0 ({W}x{H}, {w3['spp']} spp)", keep_rows=True)
     y0, y1 = base["_rows"]
-    return base, acc[y0:y1]
+    return base, acc[y0:y1], ref
 
 
 def band_vs_oracle(gpu_img, base, oimg):
@@ -296,13 +296,43 @@ def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     res["pixels"] = views.num_pixels
     res["bvh_build_ms"] = info["buildMs"]
     if cpu:
-        base, oimg = batch_cpu_baseline(w3, cpu_seconds)
+        base, oimg, ref = batch_cpu_baseline(w3, cpu_seconds)
         y0, y1 = base.pop("_rows")
         res["cpu_baseline"] = base
         res["vs_oracle"] = {"rows": [y0, y1 - 1], **{k: image_vs_oracle(view0[k][y0:y1], oimg)
                                                      for k in ("megakernel", "wavefront")}}
         log(f"workload 3 vs oracle (view 0 rows {y0}..{y1 - 1}): {res['vs_oracle']}")
+        res["converged_vs_oracle"] = batch_converged(api, w3, world, ref)
+        log(f"workload 3 converged (256 paths per pixel) vs oracle: {res['converged_vs_oracle']}")
     return res
+
+
+def batch_converged(api, w3, world, ref, spp=256, rows=16):
+    """The workload's target measure: view 0 rendered with 256 paths per
+    pixel by both GPU renderers (view 0 alone in the batch) and, over a band
+    of rows, by the oracle; the RMSE of the per-pixel means against the
+    oracle's, relative to the oracle's band mean (the same per-path streams
+    on both sides)."""
+    from oracle import refcpu
+    W, H = w3["widths"][0], w3["heights"][0]
+    y0 = max(0, H // 2 - rows // 2)
+    y1 = min(H, y0 + rows)
+    c = w3["cameras"][0]
+    views0 = api.Views([api.camera(**c)], [W], [H])
+    kw = dict(ambient=1.0, bg=(-1, -1, -1, -1), max_depth=w3["max_depth"], max_diffuse=w3["max_diffuse"],
+              russian_roulette=w3["rr_depth"])
+    racc = np.zeros((H, W, 4), np.float32)
+    t0 = time.time()
+    ref.render(refcpu.camera(**c), refcpu.render_params(W, H, spp=spp, **kw), y0, y1, cpu_threads(), accum=racc)
+    r = racc[y0:y1, :, :3].astype(np.float64) / spp
+    out = {"spp": spp, "view": 0, "rows": [y0, y1 - 1], "oracle_s": round(time.time() - t0, 2)}
+    for name, mode in (("megakernel", api.BATCH_MEGAKERNEL), ("wavefront", api.BATCH_WAVEFRONT)):
+        acc = api.DeviceArray.empty((W * H, 4), np.float32)
+        o = api.render_batch(world, views0, api.render_params(0, 0, spp=spp, **kw), mode, accum=acc)
+        g = views0.image(acc.numpy(), 0)[y0:y1, :, :3].astype(np.float64) / spp
+        out[name] = {"rel_rmse": float(np.sqrt(((g - r) ** 2).mean()) / r.mean()),
+                     "mean_rel_diff": float((g.mean() - r.mean()) / r.mean()), "ms": o["stats"]["ms"]}
+    return out
 
 
 def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
@@ -524,11 +554,40 @@ def ea_bench(steps=3):
             bwd.append(setup.last_backward["ms"])
     f, b = float(np.median(fwd)), float(np.median(bwd))
     samples = setup.last_forward["samples"]
-    return {"forward_ms": f, "adjoint_ms": b, "adjoint_over_forward": b / f, "samples_per_iter": samples,
-            "forward_Msamples_per_s": samples / (f * 1e-3) / 1e6,
-            "forward_GBps_alg": 32.0 * samples / (f * 1e-3) / 1e9,
-            "adjoint_GBps_alg": 96.0 * samples / (b * 1e-3) / 1e9,
-            "loss": float(loss), "grad_tf_norm": float(tf.grad.norm()), "grad_density_norm": float(dens.grad.norm())}
+    res = {"forward_ms": f, "adjoint_ms": b, "adjoint_over_forward": b / f, "samples_per_iter": samples,
+           "forward_Msamples_per_s": samples / (f * 1e-3) / 1e6,
+           "forward_GBps_alg": 32.0 * samples / (f * 1e-3) / 1e9,
+           "adjoint_GBps_alg": 96.0 * samples / (b * 1e-3) / 1e9,
+           "loss": float(loss.detach()), "grad_tf_norm": float(tf.grad.norm()),
+           "grad_density_norm": float(dens.grad.norm())}
+    res["gradient_check"] = ea_directional_check(ea, setup, dens, tf, target)
+    log(f"workload 4 gradient check: {res['gradient_check']}")
+    return res
+
+
+def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
+    """The full-size adjoint against central differences along a seeded
+    random direction v (entries uniform in [-1, 1]) of the transfer
+    function, then of the density grid: <grad L, v> from the adjoint vs
+    (L(x + eps v) - L(x - eps v)) / (2 eps), the loss summed in float64."""
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(11)
+
+    def loss(d, t):
+        with torch.no_grad():
+            return float(((ea.ea_render(d, t, setup) - target).double() ** 2).sum())
+
+    out = {}
+    for name, x in (("tf", tf), ("density", dens)):
+        v = torch.rand(x.shape, generator=gen, device="cuda") * 2 - 1
+        dot = float((x.grad.double() * v.double()).sum())
+        xp, xm = (x.detach() + eps * v), (x.detach() - eps * v)
+        if name == "tf":
+            fd = (loss(dens.detach(), xp) - loss(dens.detach(), xm)) / (2 * eps)
+        else:
+            fd = (loss(xp, tf.detach()) - loss(xm, tf.detach())) / (2 * eps)
+        out[name] = {"adjoint": dot, "central_difference": fd, "rel_diff": abs(fd - dot) / abs(fd)}
+    return out
 
 
 def main():

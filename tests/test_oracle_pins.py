@@ -327,18 +327,22 @@ def test_woodcock_homogeneous_transmittance():
 # Shading chain: Lambertian plane under uniform ambient light, closed form
 # ---------------------------------------------------------------------------
 
-def test_lambert_plane_furnace_closed_form():
+@pytest.mark.parametrize("rr", [0, 1])
+def test_lambert_plane_furnace_closed_form(rr):
     # big plane y=0 facing +y, camera looking down; one diffuse bounce then
     # escape. E[pixel] = E_unif[4 a alb mu^2/(1+4mu)] + E_cos[pi alb a 4mu/(4mu+1)]
     # with alb = 0.85 c / pi (AnariMatte) -- see shadeRays.cu:676-693, 833-871.
+    # rr = 1: Russian roulette on the bounced ray (survival q = max(tp) =
+    # 0.85 c, weight 1/q) leaves the expectation unchanged; a survivor
+    # without the 1/q weight lowers the bounce term by the factor q.
     c, amb = 0.6, 0.8
     S = 1e4
     v = np.array([[-S, 0, -S], [S, 0, -S], [S, 0, S], [-S, 0, S]], np.float32)
     t = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
     s = refcpu.Scene([dict(kind=0, vertices=v, indices=t, color=(c, c, c))])
-    W, H, spp = 40, 30, 16
+    W, H, spp = 40, 30, 16 if rr == 0 else 64
     acc, _, _ = s.render(refcpu.camera((0, 10, 0), (0.01, -1, 0.02), fovy=30),
-                         refcpu.render_params(W, H, spp=spp, ambient=amb))
+                         refcpu.render_params(W, H, spp=spp, ambient=amb, russian_roulette=rr))
     pix = acc[..., 0].ravel() / spp
     alb = 0.85 * c / math.pi
     mu = np.linspace(0, 1, 200001)
@@ -348,3 +352,94 @@ def test_lambert_plane_furnace_closed_form():
     se = pix.std() / math.sqrt(len(pix))
     assert abs(pix.mean() - exp) < 4 * se + 1e-3, (pix.mean(), exp, se)
     assert np.allclose(acc[..., 3], spp)  # alpha: every primary ray hits
+
+
+def _open_box():
+    """Five matte quads of a 10-unit box, open at the top (y = 10)."""
+    q = [[(0, 0, 0), (10, 0, 0), (10, 0, 10), (0, 0, 10)],      # floor
+         [(0, 0, 0), (0, 10, 0), (10, 10, 0), (10, 0, 0)],      # z = 0
+         [(0, 0, 10), (10, 0, 10), (10, 10, 10), (0, 10, 10)],  # z = 10
+         [(0, 0, 0), (0, 0, 10), (0, 10, 10), (0, 10, 0)],      # x = 0
+         [(10, 0, 0), (10, 10, 0), (10, 10, 10), (10, 0, 10)]]  # x = 10
+    v = np.array(q, np.float32).reshape(-1, 3)
+    t = np.array([[4 * k, 4 * k + 1, 4 * k + 2] for k in range(5)] +
+                 [[4 * k, 4 * k + 2, 4 * k + 3] for k in range(5)], np.int32)
+    return refcpu.Scene([dict(kind=0, vertices=v, indices=t, color=(0.8, 0.75, 0.7))])
+
+
+def test_russian_roulette_is_unbiased_over_many_bounces():
+    """Paths bounce up to 8 times inside an open box (up to 8 diffuse
+    bounces; the only light is the ambient sky seen through the opening).
+    Russian roulette from depth 2 kills paths with probability 1 - q, q =
+    min(max(tp), 0.95), and weights survivors by 1/q: the expected image is
+    the one without roulette. Per band of rows, the mean over 32 independent
+    frames with roulette equals the mean without within 3.5 standard errors;
+    roulette does end paths early (fewer rays). Dropping the 1/q weight, or
+    killing with probability q instead of 1 - q, moves the bands by many
+    standard errors."""
+    s = _open_box()
+    cam = refcpu.camera((5.0, 24.0, -6.0), (0.0, -18.0, 11.0), fovy=35)
+    W, H, spp, frames = 24, 18, 8, 32
+
+    def frame_bands(rr, first):
+        out, rays = [], 0
+        for f in range(frames):
+            acc, _, n = s.render(cam, refcpu.render_params(W, H, spp=spp, accum_id=first + f * spp, ambient=1.0,
+                                                            max_depth=8, max_diffuse=8, russian_roulette=rr))
+            rays += n
+            px = acc[..., :3].sum(-1) / spp
+            out.append(px.reshape(6, -1).mean(1))  # six bands of three rows
+        return np.array(out, np.float64), rays
+
+    on, rays_on = frame_bands(2, 0)
+    off, rays_off = frame_bands(0, 100_000)
+    z = (on.mean(0) - off.mean(0)) / np.sqrt(on.var(0, ddof=1) / frames + off.var(0, ddof=1) / frames)
+    print(f"bands off {np.round(off.mean(0), 4)} z {np.round(z, 2)} rays {rays_on} vs {rays_off}")
+    assert np.all(np.abs(z) <= 3.5), z
+    assert off.mean() > 0.1
+    assert rays_on < 0.9 * rays_off
+
+
+@pytest.mark.parametrize("cpm", [4, 3])
+def test_structured_macrocells_bound_every_sample(cpm):
+    """StructuredData_computeMCs (StructuredData.cu:30-61) + mapMCs: the
+    range of macrocell c spans the (cpm+1)^3 voxels its cells touch, so every
+    trilinear sample taken inside c lies in c's range, and c's majorant is
+    at least the transfer function's opacity at every such sample. A white-
+    noise field makes the face voxels matter (a cpm^3 corner range misses
+    them) and a transfer function peaked between its knots makes interior
+    knots matter (a majorant of the range's end values misses the peak).
+    cpm = 3 does not divide the 20 cells: the last macrocell is partial."""
+    n = 21
+    field = np.random.default_rng(5).random((n, n, n)).astype(np.float32)
+    origin, spacing = np.array([0.5, -1.0, 2.0]), np.array([0.5, 0.25, 1.0])
+    v = refcpu.Volume(field, origin=tuple(origin), spacing=tuple(spacing), cells_per_mc=cpm)
+    x = np.linspace(0, 1, 9)
+    tf = np.stack([x, 1 - x, 0.5 + 0 * x, np.where(np.arange(9) == 4, 1.0, 0.1)], -1).astype(np.float32)
+    v.set_tf(tf, (0.0, 1.0), 2.0)
+    ranges, maj = v.macrocells()
+    mx, my, mz = v.mc_dims()
+    assert (mx, my, mz) == ((n - 1 + cpm - 1) // cpm,) * 3
+    rng = np.random.default_rng(6)
+    hi = origin + spacing * (n - 1)
+    P = rng.uniform(origin, hi, (6000, 3))
+    # and points on and next to macrocell faces (x = k cpm cells)
+    k = rng.integers(1, mx, 3000)
+    Q = rng.uniform(origin, hi, (3000, 3))
+    Q[:, 0] = origin[0] + spacing[0] * cpm * k + rng.choice([-1e-4, 0.0, 1e-4], 3000)
+    P = np.concatenate([P, Q]).astype(np.float32)
+    bad_range = bad_maj = 0
+    for p in P:
+        s = v.sample(p)
+        if np.isnan(s):
+            continue
+        c = np.minimum(((p.astype(np.float64) - origin) / (spacing * cpm)).astype(int), [mx - 1, my - 1, mz - 1])
+        lo_, hi_ = ranges[c[2], c[1], c[0]]
+        bad_range += not (lo_ <= s <= hi_)
+        bad_maj += not (v.tf_map(s)[3] <= maj[c[2], c[1], c[0]])
+    assert bad_range == 0 and bad_maj == 0, (bad_range, bad_maj)
+    # the majorant is the transfer function's maximum over the range, times
+    # the base density (TransferFunction.h:92-116): the peak knot when the
+    # range covers 0.5
+    cover = (ranges[..., 0] <= 0.5) & (ranges[..., 1] >= 0.5)
+    assert cover.any() and np.allclose(maj[cover], 2.0)
