@@ -329,7 +329,8 @@ class Volume:
         check(capi.load().bnamd_volume_render(self.handle, C.byref(cam), C.byref(params), acc.addr, None,
                                               C.byref(st)))
         return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
-                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+                                        "raysTraced": st.raysTraced, "ms": st.ms,
+                                        "zeroMajorantSteps": st.zeroMajorantSteps}}
 
     def destroy(self):
         if self.handle is not None and self.handle.value:
@@ -396,7 +397,8 @@ class Amr:
         st = capi.bnamd_volume_stats()
         check(capi.load().bnamd_amr_render(self.handle, C.byref(cam), C.byref(params), acc.addr, None, C.byref(st)))
         return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
-                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+                                        "raysTraced": st.raysTraced, "ms": st.ms,
+                                        "zeroMajorantSteps": st.zeroMajorantSteps}}
 
     def __del__(self):
         try:
@@ -489,7 +491,8 @@ class UMesh:
         check(capi.load().bnamd_umesh_render(self.handle, C.byref(cam), C.byref(params), C.byref(o), acc.addr, None,
                                              C.byref(st)))
         return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
-                                        "raysTraced": st.raysTraced, "ms": st.ms}}
+                                        "raysTraced": st.raysTraced, "ms": st.ms,
+                                        "zeroMajorantSteps": st.zeroMajorantSteps}}
 
     def destroy(self):
         if self.handle is not None and self.handle.value:

@@ -72,7 +72,7 @@ class bnamd_amr_info(C.Structure):
 
 class bnamd_volume_stats(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("ddaSteps", C.c_longlong), ("raysTraced", C.c_longlong),
-                ("ms", C.c_float)]
+                ("ms", C.c_float), ("zeroMajorantSteps", C.c_longlong)]
 
 
 class bnamd_umesh_desc(C.Structure):

@@ -428,7 +428,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
   const int ix = (tile % tilesX) * TW + (q % TW);
   const int iy = (tile / tilesX) * TH + (q / TW);
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
-  VolCounters cnt{0, 0};
+  VolCounters cnt{0, 0, 0};
   unsigned long long nrays = 0;
   uint32_t hint = ~0u;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -481,6 +481,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
   }
   wave_sum_u64(cnt.samples, counters + 0);
   wave_sum_u64(cnt.ddaSteps, counters + 1);
+  wave_sum_u64(cnt.zeroSteps, counters + 3);
   wave_sum_u64(nrays, counters + 2);
 }
 
@@ -626,6 +627,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.dd.mx = 1 + (int)(size.x * f);
       m.dd.my = 1 + (int)(size.y * f);
       m.dd.mz = 1 + (int)(size.z * f);
+      BN_REQUIRE((long long)m.dd.my * m.dd.mz < (1ll << 24), "bnamd_umesh_create: over 2^24 macrocells in y*z");
       m.dd.mcOrigin = lo;
       m.dd.mcSpacing = size * rcp(vec3f((float)m.dd.mx, (float)m.dd.my, (float)m.dd.mz));
       m.dd.worldLo = lo;
@@ -769,8 +771,8 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     o.isoValue = opts->isoValue;
     o.isoColor = {opts->isoColor[0], opts->isoColor[1], opts->isoColor[2]};
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> cnt(3);
-    BN_HIP(hipMemsetAsync(cnt.ptr, 0, 3 * sizeof(unsigned long long), s));
+    DevBuf<unsigned long long> cnt(4);
+    BN_HIP(hipMemsetAsync(cnt.ptr, 0, cnt.bytes(), s));
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_render: BVH too deep for the LDS stack");
     hipEvent_t e0, e1;
@@ -796,13 +798,14 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     BN_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned long long h[3];
+    unsigned long long h[4];
     BN_HIP(hipMemcpy(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = ms;
+      stats->zeroMajorantSteps = (long long)h[3];
     }
   });
 }

@@ -19,8 +19,20 @@ struct TFDD {
   float baseDensity;
 };
 
+// The structured field is sampled from a bricked copy: kBrick^3 voxels per
+// brick (2 KB), bricks x-fastest; inside a brick every 128-byte cache line
+// holds a 4 x 4 x 2 block of voxels (16 lines: x-halves fastest, then
+// y-halves, then z-pairs; x fastest inside a line). A trilinear sample's 2 x
+// 2 x 2 corners then lie in about 2.1 lines on average, against 2.6 for
+// x-fastest rows of 8 in a brick and 4 in the x-fastest array. Voxel
+// (x, y, z) is at X(x) + Y(y) + Z(z), one term per axis.
+constexpr int kBrickShift = 3, kBrick = 1 << kBrickShift, kBrickVoxels = kBrick * kBrick * kBrick;
+
 struct VolumeDD {
-  const float *scalars;
+  const float *scalars;    // the caller's x-fastest array (macrocell build)
+  const float *bricks;     // the bricked copy (sampling)
+  uint32_t brickStrideY;   // voxels per row of bricks: bricksX * kBrickVoxels
+  uint32_t brickStrideZ;   // voxels per slab of bricks: bricksX * bricksY * kBrickVoxels
   int nx, ny, nz;          // numScalars
   vec3f origin, spacing;   // cell grid
   vec3f worldLo, worldHi;  // origin + spacing * numCells
@@ -63,6 +75,21 @@ BN_HD float lerp_l(float f, float a, float b) {
   return (1.f - f) * a + f * b;
 }
 
+// Offsets of voxel coordinate x, y, z in the bricked copy (their sum):
+// brick, then line within the brick (32 voxels each), then voxel in the line.
+BN_HD uint32_t brick_x(int x) {
+  const uint32_t u = (uint32_t)x;
+  return (u >> 3) * kBrickVoxels + ((u >> 2) & 1u) * 32u + (u & 3u);
+}
+BN_HD uint32_t brick_y(int y, uint32_t strideY) {
+  const uint32_t u = (uint32_t)y;
+  return (u >> 3) * strideY + ((u >> 2) & 1u) * 64u + (u & 3u) * 4u;
+}
+BN_HD uint32_t brick_z(int z, uint32_t strideZ) {
+  const uint32_t u = (uint32_t)z;
+  return (u >> 3) * strideZ + ((u >> 1) & 3u) * 128u + (u & 1u) * 16u;
+}
+
 // Trilinear sample of the scalar field at world point P; NaN outside.
 BN_HD float sample_field(const VolumeDD &v, vec3f P) {
 #pragma clang fp contract(off)
@@ -78,16 +105,18 @@ BN_HD float sample_field(const VolumeDD &v, vec3f P) {
   lerp_coords(v.nx, rel.x + .5f, x0, x1, fx);
   lerp_coords(v.ny, rel.y + .5f, y0, y1, fy);
   lerp_coords(v.nz, rel.z + .5f, z0, z1, fz);
-  const size_t sx = 1, sy = (size_t)v.nx, sz = (size_t)v.nx * v.ny;
-  const float *s = v.scalars;
-  const float v000 = s[x0 * sx + y0 * sy + z0 * sz];
-  const float v001 = s[x1 * sx + y0 * sy + z0 * sz];
-  const float v010 = s[x0 * sx + y1 * sy + z0 * sz];
-  const float v011 = s[x1 * sx + y1 * sy + z0 * sz];
-  const float v100 = s[x0 * sx + y0 * sy + z1 * sz];
-  const float v101 = s[x1 * sx + y0 * sy + z1 * sz];
-  const float v110 = s[x0 * sx + y1 * sy + z1 * sz];
-  const float v111 = s[x1 * sx + y1 * sy + z1 * sz];
+  const uint32_t X0 = brick_x(x0), X1 = brick_x(x1);
+  const uint32_t Y0 = brick_y(y0, v.brickStrideY), Y1 = brick_y(y1, v.brickStrideY);
+  const uint32_t Z0 = brick_z(z0, v.brickStrideZ), Z1 = brick_z(z1, v.brickStrideZ);
+  const float *s = v.bricks;
+  const float v000 = s[X0 + Y0 + Z0];
+  const float v001 = s[X1 + Y0 + Z0];
+  const float v010 = s[X0 + Y1 + Z0];
+  const float v011 = s[X1 + Y1 + Z0];
+  const float v100 = s[X0 + Y0 + Z1];
+  const float v101 = s[X1 + Y0 + Z1];
+  const float v110 = s[X0 + Y1 + Z1];
+  const float v111 = s[X1 + Y1 + Z1];
   const float v00 = lerp_l(fx, v000, v001);
   const float v01 = lerp_l(fx, v010, v011);
   const float v10 = lerp_l(fx, v100, v101);
@@ -148,6 +177,7 @@ BN_HD VolSample sample_and_map(const VolumeDD &v, vec3f P) {
 struct VolCounters {
   uint32_t samples;
   uint32_t ddaSteps;
+  uint32_t zeroSteps;  // macrocells visited whose majorant is zero (no draw)
 };
 
 struct VolHit {
@@ -175,11 +205,14 @@ struct DeltaTrack {
   Random2 rng;
   vec3f t_next, t_step;
   int cx, cy, cz;
-  uint32_t neg;           // bit k: the DDA steps down along axis k
+  uint32_t neg;           // bit k: the DDA steps down along axis k; bits 3-5:
+                          // the axes the next step crosses; bit 6: the next
+                          // cell lies in the grid (both found on entering)
   float ray_t0, tMax;     // DDA origin offset, clipped segment end
-  float next_cell_begin;
+  float next_cell_begin;  // where the next cell begins (t_closest on entering)
   float majorant, nextMajorant;
   float t, upper;         // Woodcock position and end of the current cell
+  float leaveDepth;       // optical depth beyond which a draw leaves the cell
   float pendingR;         // the draw of a collision distance awaiting sample()
   int mode;
   bool hit;
@@ -254,14 +287,25 @@ struct DeltaTrack {
 
   // One cheap unit of work: a collision distance drawn in the current cell,
   // or a macrocell step of the DDA (leaving the cell, then entering the next
-  // one, with its majorant fetched one cell ahead). The distance's exact
-  // value (barney's fastLog and correctly rounded divisions) only matters
-  // when the collision falls inside the cell: a draw that certainly leaves
-  // the cell is decided with hardware reciprocals and a margin of 1e-5 of
-  // the distance plus 1e-6 of t (the approximation's error stays below an
-  // eighth of that margin), and t, which the next cell does not use, is not
-  // computed. Every other draw waits in SAMPLE, where sample() finishes it
-  // exactly. The RNG draws keep their order.
+  // one, with its majorant fetched one cell ahead).
+  //
+  // A draw's exact distance (barney's fastLog and correctly rounded
+  // divisions) only matters when the collision falls inside the cell. The
+  // draw is compared in optical depth: tau = -fastLog(1 - r), evaluated with
+  // a hardware reciprocal and within 6e-7 of the exact float value, against
+  // leaveDepth = majorant ((upper - t) + 1e-6 (|t| + |upper|)), set when the
+  // cell is entered and after each rejected collision. tau (1 - 1e-5) >=
+  // leaveDepth implies that the exact t + dt rounds to upper or beyond (the
+  // margins exceed the reciprocal, division and addition rounding), so such
+  // a draw moves on to the next cell without computing t, which the next
+  // cell does not use. Every other draw waits in SAMPLE, where sample()
+  // finishes it exactly. The RNG draws keep their order.
+  //
+  // Entering a cell finds the axes whose crossing ends it (t_next equal to
+  // their minimum), the next cell and whether it lies in the grid, for the
+  // majorant fetched ahead; leaving the cell then applies exactly that step
+  // (the same t_next additions as dda3's loop, which ends when a stepped
+  // coordinate leaves the grid).
   template <bool kExactLog, class V>
   BN_HD void advance(const V &v, VolCounters &cnt) {
 #pragma clang fp contract(off)
@@ -272,9 +316,8 @@ struct DeltaTrack {
         const float f = 1.f - r;
         const float q = (f - 1.f) * __builtin_amdgcn_rcpf(f + 1.f);
         const float q2 = q * q;
-        const float dtA = -2.f * (q * (1.f + q2 * (1.f / 3.f + q2 * (1.f / 5.f)))) * __builtin_amdgcn_rcpf(majorant);
-        const float tA = t + dtA;
-        leaves = tA - (1e-5f * dtA + 1e-6f * fabsf(tA)) >= upper;
+        const float tau = -2.f * (q * (1.f + q2 * (1.f / 3.f + q2 * (1.f / 5.f))));
+        leaves = tau * (1.f - 1e-5f) >= leaveDepth;
       }
       if (!leaves) {
         pendingR = r;
@@ -284,41 +327,44 @@ struct DeltaTrack {
       mode = ADVANCE;
     }
     if (mode == ADVANCE) {
-      const float t_closest = reduce_min(t_next);
       majorant = nextMajorant;
-      next_cell_begin = t_closest;
-      if (t_next.x == t_closest) {
+      if (!(neg & 64u)) {
+        mode = DONE;
+        return;
+      }
+      if (neg & 8u) {
         t_next.x += t_step.x;
         cx += (neg & 1u) ? -1 : 1;
-        if (cx == ((neg & 1u) ? -1 : v.mx)) { mode = DONE; return; }
       }
-      if (t_next.y == t_closest) {
+      if (neg & 16u) {
         t_next.y += t_step.y;
         cy += (neg & 2u) ? -1 : 1;
-        if (cy == ((neg & 2u) ? -1 : v.my)) { mode = DONE; return; }
       }
-      if (t_next.z == t_closest) {
+      if (neg & 32u) {
         t_next.z += t_step.z;
         cz += (neg & 4u) ? -1 : 1;
-        if (cz == ((neg & 4u) ? -1 : v.mz)) { mode = DONE; return; }
       }
     }
-    // ENTER: the cell's segment, and the next cell's majorant fetched ahead
+    // ENTER: the cell's segment, its exit step, and the next cell's majorant
     const float t_closest = reduce_min(t_next);
     const float cell_t0 = ray_t0 + next_cell_begin;
     const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
+    next_cell_begin = t_closest;
     cnt.ddaSteps++;
-    int ncx = cx, ncy = cy, ncz = cz;
-    if (t_next.x == t_closest) ncx += (neg & 1u) ? -1 : 1;
-    if (t_next.y == t_closest) ncy += (neg & 2u) ? -1 : 1;
-    if (t_next.z == t_closest) ncz += (neg & 4u) ? -1 : 1;
+    const bool sx = t_next.x == t_closest, sy = t_next.y == t_closest, sz = t_next.z == t_closest;
+    const int ncx = sx ? cx + ((neg & 1u) ? -1 : 1) : cx;
+    const int ncy = sy ? cy + ((neg & 2u) ? -1 : 1) : cy;
+    const int ncz = sz ? cz + ((neg & 4u) ? -1 : 1) : cz;
     const bool inside = ncx >= 0 && ncx < v.mx && ncy >= 0 && ncy < v.my && ncz >= 0 && ncz < v.mz;
+    neg = (neg & 7u) | (sx ? 8u : 0u) | (sy ? 16u : 0u) | (sz ? 32u : 0u) | (inside ? 64u : 0u);
     nextMajorant = inside ? v.majorants[ncx + v.mx * (ncy + v.my * ncz)] : 0.f;
     if (majorant != 0.f) {
       t = cell_t0;
       upper = fminf(cell_t1, rayTMax);
+      leaveDepth = majorant * ((upper - t) + 1e-6f * (fabsf(t) + fabsf(upper)));
       mode = WOODCOCK;
     } else {
+      cnt.zeroSteps++;
       mode = ADVANCE;
     }
   }
@@ -344,6 +390,7 @@ struct DeltaTrack {
       rgb = {s.r, s.g, s.b};
       mode = DONE;
     } else {
+      leaveDepth = majorant * ((upper - t) + 1e-6f * (fabsf(t) + fabsf(upper)));
       mode = WOODCOCK;
     }
   }

@@ -14,6 +14,7 @@ namespace bnamd {
 
 struct Volume {
   VolumeDD dd;
+  DevBuf<float> bricks;
   int cellsPerMC = 8;
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
@@ -48,6 +49,18 @@ __global__ void k_mc_ranges(VolumeDD v, int cpm, float2 *__restrict ranges) {
   ranges[tid] = make_float2(lo, hi);
 }
 
+// The bricked copy of the field: one thread per voxel of the caller's array
+// (coalesced reads), written to its place in the brick layout. Voxels of the
+// padded brick grid past the field's end stay unwritten: sampling clamps its
+// corner coordinates into the field and never reads them.
+__global__ void k_brick_field(VolumeDD v, float *__restrict out) {
+  const size_t total = (size_t)v.nx * v.ny * v.nz;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % v.nx), y = (int)((i / v.nx) % v.ny), z = (int)(i / ((size_t)v.nx * v.ny));
+    out[brick_x(x) + brick_y(y, v.brickStrideY) + brick_z(z, v.brickStrideZ)] = v.scalars[i];
+  }
+}
+
 template <class F>
 __global__ void k_majorants(F v, const float2 *__restrict ranges, float *__restrict maj) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -78,6 +91,8 @@ constexpr int kPoolPaths = 2048;
 // steps per lane, so it is worth running for fewer lanes than the others.
 constexpr int kPhaseWeight[5] = {1, 3, 3, 3, 3};
 constexpr int kCheapSteps = 12;
+// Wavefronts per SIMD the kernel is built for (a register budget of 128).
+constexpr int kVolumeWaves = 4;
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long long *dst) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -93,30 +108,31 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long
 // (rgb, alpha) goes to an LDS slot; when the pool is used up, each pixel adds
 // its paths in path order, as barney accumulates them.
 //
-// A path is a sequence of small kinds of work: camera ray, walk set-up
-// (DeltaTrack::init), cheap walk steps (macrocell DDA step or collision
-// distance), field samples at collisions, shading of a collision (barney's
-// bounce(): NEE shadow ray toward the environment) and shading of an escaped
-// ray (background, or the unoccluded shadow contribution). Every wavefront
-// iteration runs ONE kind, the one most lanes are waiting for (weighted by
-// kPhaseWeight), for all of them, so no iteration pays for code its lanes do
-// not need; camera rays and walk set-ups share a kind, since every camera
-// ray is set up next. Cheap steps, most of a path's work, run several to an
-// iteration (kCheapSteps): 8.4 ms per 1920x1080x16spp frame with one step
-// per iteration and equal weights, 5.9 ms with twelve and the weights above.
-// Per path the rays, RNG draws and fragment order are those of barney's
-// generate / trace / shade loop.
+// A path is barney's generate / trace / shade loop for a volume-only scene
+// with SCI_VIS_MODE shading: the camera ray's walk either escapes (the pixel
+// shows the background) or collides; at a collision bounce() (NEE toward
+// the environment light with the phase function) spawns one shadow ray and
+// ends the path ray, since scattering in a volume ends a path there
+// (shadeRays.cu:787-790; the phase function's pdf, 1/(4 pi), never fails
+// bounce()'s pdf test, so that branch is always reached). The path's value is
+// then the shadow ray's weight if its walk escapes, else zero. Its work comes
+// in small kinds: camera ray with walk set-up (DeltaTrack::init), cheap walk
+// steps (macrocell DDA step or collision distance), field samples at
+// collisions, shading of a collision, and shading of an escaped ray. Every
+// wavefront iteration runs ONE kind, the one most lanes are waiting for
+// (weighted by kPhaseWeight), for all of them; cheap steps, most of a path's
+// work, run several to an iteration (kCheapSteps).
 //
-// Occupancy: the kernel is issue-bound, and three wavefronts per SIMD
-// (<= 168 VGPRs) measured 8.8 ms per 1920x1080x16spp frame against 12.4 ms
-// at the two that the unconstrained allocation (178 VGPRs) leaves; the cap
-// costs a few dwords of spill in the shading code.
+// A lane carries only the walk and {path index, stage, shadow weight}: the
+// camera ray is generated again (same arithmetic) where shading needs it
+// instead of being kept across the walk, which keeps the register count at
+// the four wavefronts per SIMD that this issue-bound kernel needs.
 //
 // F is the field: VolumeDD (structured data) or AmrDD (block-structured
 // AMR); both carry the macrocell grid and transfer function, and
 // sample_and_map(F, P) gives the mapped sample.
 template <int PB, class F>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kVolumeWaves)))
 k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
                 unsigned long long *__restrict counters) {
   constexpr int TW = (PB >= 1024) ? 32 : (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
@@ -131,7 +147,7 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
   const int lane = threadIdx.x & 63;
   const int tx0 = (tile % tilesX) * TW, ty0 = (tile / tilesX) * TH;
   const unsigned long long lanesBelow = (1ull << lane) - 1ull;
-  VolCounters cnt{0, 0};
+  VolCounters cnt{0, 0, 0};
   unsigned nrays = 0;
   const auto sampler = [&](vec3f P) { return sample_and_map(v, P); };
   const uint64_t geomHash = bn_hash3(0u, 0u, 0u);
@@ -141,18 +157,23 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
     const int nPaths = PB * min(kUnit, spp - s0);
     if (threadIdx.x == 0) poolNext = 0;
     __syncthreads();
-    int k = 0;                    // this lane's path: pixel k % PB, sample s0 + k / PB
+    int k = 0;                      // this lane's path: pixel k % PB, sample s0 + k / PB
     int stage = VS_GEN;
-    bool inShadow = false;        // the walk holds the path's shadow ray
-    int gen = 0;
-    RenderDD rpp = rp;
-    PathRay ray;                  // the path's current (non-shadow) ray
-    vec3f shadowTp = vec3f(0.f);  // contribution of the shadow ray if unoccluded
-    float shadowMis = 0.f;
-    vec3f sum = vec3f(0.f);
-    float alpha = 0.f;
+    bool inShadow = false;          // the walk holds the path's shadow ray
+    vec3f shadowW = vec3f(0.f);     // the shadow ray's contribution if it escapes
     DeltaTrack tr;
     tr.mode = DeltaTrack::DONE;
+    // the path's pixel and camera ray (generateRays), computed where needed
+    auto pixelOf = [&](int kk, int &ix, int &iy) {
+      const int j = kk % PB;
+      ix = tx0 + j % TW;
+      iy = ty0 + j / TW;
+    };
+    auto pathParams = [&](int kk) {
+      RenderDD r = rp;
+      r.accumID = rp.accumID + s0 + kk / PB;
+      return r;
+    };
 
     for (;;) {
       const bool march = stage == VS_MARCH;
@@ -176,7 +197,6 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
           most = waiting[w];
           best = w + 1;
         }
-      bool pathDone = false;
       if (best == 0) {
         if (wCheap) {  // steps until the walk needs the field or ends
           int i = 0;
@@ -194,20 +214,18 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
           if (lane == leader) base = atomicAdd(&poolNext, (int)__popcll(genMask));
           base = __shfl(base, leader);
         }
+        RNGSeed seed;
         if (wGen) {  // take the next path of the pool; generateRays for it
           k = base + (int)__popcll(genMask & lanesBelow);
-          const int j = k % PB;
-          const int ix = tx0 + j % TW, iy = ty0 + j / TW;
+          int ix, iy;
+          pixelOf(k, ix, iy);
           if (k >= nPaths) {
             stage = VS_DEAD;
           } else if (ix >= rp.width || iy >= rp.height) {
             slots[k] = make_float4(0.f, 0.f, 0.f, 0.f);  // outside the frame: take another path
           } else {
-            rpp.accumID = rp.accumID + s0 + k / PB;
-            ray = camera_ray(cam, rpp, ix, iy);
-            sum = vec3f(0.f);
-            alpha = 0.f;
-            gen = 0;
+            const PathRay ray = camera_ray(cam, pathParams(k), ix, iy);
+            seed = ray.rng;
             inShadow = false;
             tr.org = ray.org;
             tr.dir = fix_dir(ray.dir);
@@ -215,84 +233,60 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
             stage = VS_INIT;
           }
         }
-        // the walks of the new camera rays and of the rays shading left
-        if (stage == VS_INIT) {  // a shadow ray's seed waits in tr.rng; a path ray's is ray.rng
-          RNGSeed seed;
-          seed.state = inShadow ? tr.rng.state : ray.rng.state;
+        // the walks of the new camera rays and of the shadow rays shading
+        // left (whose seed waits in tr.rng)
+        if (stage == VS_INIT) {
+          if (inShadow) seed.state = tr.rng.state;
           tr.init(v, tr.org, tr.dir, 0.f, tr.rayTMax, tr.rayTMax, seed, geomHash);
-          if (!inShadow) ray.rng.state = seed.state;
           stage = VS_MARCH;
         }
       } else if (best == 3) {
-        if (wHit) {  // a collision of the path ray: shade, spawn the shadow ray
+        if (wHit) {  // a collision of the camera ray: bounce() spawns the shadow ray
           ++nrays;
+          int ix, iy;
+          pixelOf(k, ix, iy);
+          const RenderDD rpp = pathParams(k);
+          PathRay ray = camera_ray(cam, rpp, ix, iy);
+          ray.rng.next();  // as the walk's set-up advanced it (Random2(seed, hash))
           ShadeHit sh;
           sh.hit = true;
           sh.t = tr.t;
           sh.P = tr.org + tr.t * tr.dir;
           sh.N = vec3f(0.f);
           sh.bsdf = phase_bsdf(tr.rgb);
-          if (gen == 0) alpha = 1.f;
           PathRay sray;
-          const vec3f frag = vmin(bounce(rpp, ray, sh, sray, gen), 10.f * (1 + rpp.accumID));
-          if (frag.x > 0.f) sum.x += frag.x;
-          if (frag.y > 0.f) sum.y += frag.y;
-          if (frag.z > 0.f) sum.z += frag.z;
+          (void)bounce(rpp, ray, sh, sray, 0);  // no fragment at a volume collision
           if (sray.tMax > 0.f) {
-            shadowTp = sray.tp;
-            shadowMis = sray.mis;
+            shadowW = sray.mis * sray.tp;  // bounce() of the shadow ray if unoccluded
             tr.org = sray.org;
             tr.dir = fix_dir(sray.dir);
             tr.rayTMax = sray.tMax;
             tr.rng.state = sray.rng.state;
             inShadow = true;
             stage = VS_INIT;
-          } else if (ray.tMax > 0.f) {
-            ++gen;
-            tr.org = ray.org;
-            tr.dir = fix_dir(ray.dir);
-            tr.rayTMax = ray.tMax;
-            stage = VS_INIT;
           } else {
-            pathDone = true;
+            slots[k] = make_float4(0.f, 0.f, 0.f, 1.f);
+            stage = VS_GEN;
           }
         }
       } else {
-        if (wMiss) {  // an escaped path ray, or a finished shadow ray
+        if (wMiss) {  // an escaped camera ray, or a finished shadow ray
           ++nrays;
+          int ix, iy;
+          pixelOf(k, ix, iy);
+          const RenderDD rpp = pathParams(k);
           vec3f frag;
+          float alpha = 1.f;
           if (inShadow) {
-            // bounce() of a shadow ray: its contribution when unoccluded
-            frag = tr.hit ? vec3f(0.f) : shadowMis * shadowTp;
-            inShadow = false;
-          } else {
-            ShadeHit sh;
-            sh.hit = false;
-            // camera_ray's background, recomputed here rather than carried
-            // through the walk
-            miss_color(rpp, ty0 + (k % PB) / TW, ray.missColor, ray.missAlpha);
-            if (gen == 0) alpha = ray.missAlpha;
-            PathRay sray;
-            frag = bounce(rpp, ray, sh, sray, gen);
+            frag = tr.hit ? vec3f(0.f) : shadowW;
+          } else {  // bounce() of a primary miss: the background
+            miss_color(rpp, iy, frag, alpha);
           }
           frag = vmin(frag, 10.f * (1 + rpp.accumID));
-          if (frag.x > 0.f) sum.x += frag.x;
-          if (frag.y > 0.f) sum.y += frag.y;
-          if (frag.z > 0.f) sum.z += frag.z;
-          if (ray.tMax > 0.f) {
-            ++gen;
-            tr.org = ray.org;
-            tr.dir = fix_dir(ray.dir);
-            tr.rayTMax = ray.tMax;
-            stage = VS_INIT;
-          } else {
-            pathDone = true;
-          }
+          slots[k] = make_float4(frag.x > 0.f ? frag.x : 0.f, frag.y > 0.f ? frag.y : 0.f,
+                                 frag.z > 0.f ? frag.z : 0.f, alpha);
+          stage = VS_GEN;
         }
-      }
-      if (pathDone) {
-        slots[k] = make_float4(sum.x, sum.y, sum.z, alpha);
-        stage = VS_GEN;
       }
     }
     // the workgroup's paths are done: each pixel adds its paths in path order
@@ -325,6 +319,7 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
   wave_add_u64(cnt.samples, counters + 0);
   wave_add_u64(cnt.ddaSteps, counters + 1);
   wave_add_u64((unsigned long long)nrays, counters + 2);
+  wave_add_u64(cnt.zeroSteps, counters + 3);
 }
 
 template <int PB, class F>
@@ -368,7 +363,7 @@ __global__ void k_volume_rays(VolumeDD v, const bnamd_ray *__restrict rays, cons
   const bnamd_ray r = rays[i];
   RNGSeed seed;
   seed.state = seeds[i];
-  VolCounters cnt{0, 0};
+  VolCounters cnt{0, 0, 0};
   const vec3f dir = fix_dir({r.dir[0], r.dir[1], r.dir[2]});
   const VolHit h = volume_intersect(v, {r.org[0], r.org[1], r.org[2]}, dir, r.tmin, r.tmax, r.tmax, seed,
                                     exactLog != 0, cnt);
@@ -428,6 +423,7 @@ extern "C" int bnamd_volume_create(const bnamd_structured_desc *desc, bnamd_volu
       const int cpm = desc->cellsPerMC;
       V.cellsPerMC = cpm;
       d.mx = div_up(d.nx - 1, cpm); d.my = div_up(d.ny - 1, cpm); d.mz = div_up(d.nz - 1, cpm);
+      BN_REQUIRE((long long)d.my * d.mz < (1ll << 24), "bnamd_volume_create: over 2^24 macrocells in y*z");
       d.mcOrigin = d.worldLo;
       d.mcSpacing = vec3f((float)cpm) * d.spacing;
       const long long nmc = (long long)d.mx * d.my * d.mz;
@@ -439,6 +435,18 @@ extern "C" int bnamd_volume_create(const bnamd_structured_desc *desc, bnamd_volu
       d.tf = TFDD{nullptr, 0, 0.f, 1.f, 1.f};
       k_mc_ranges<<<div_up(nmc, 128), 128>>>(d, cpm, V.ranges.ptr);
       BN_HIP(hipGetLastError());
+      // the bricked copy the renderers sample (32-bit voxel offsets)
+      const uint32_t bx = (uint32_t)div_up(d.nx, kBrick), by = (uint32_t)div_up(d.ny, kBrick),
+                     bz = (uint32_t)div_up(d.nz, kBrick);
+      const size_t total = (size_t)bx * by * bz * kBrickVoxels;
+      BN_REQUIRE(total <= 0xffffffffull, "bnamd_volume_create: field over 2^32 voxels (bricked)");
+      V.bricks.alloc(total);
+      d.brickStrideY = bx * kBrickVoxels;
+      d.brickStrideZ = bx * by * kBrickVoxels;
+      const size_t nvox = (size_t)d.nx * d.ny * d.nz;
+      k_brick_field<<<(unsigned)std::min<size_t>((nvox + 255) / 256, 65536), 256>>>(d, V.bricks.ptr);
+      BN_HIP(hipGetLastError());
+      d.bricks = V.bricks.ptr;
       BN_HIP(hipDeviceSynchronize());
     } catch (...) {
       delete vo;
@@ -494,7 +502,7 @@ extern "C" int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *
     BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
                "bnamd_volume_render: bad frame size or pathsPerPixel");
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counters(3);
+    DevBuf<unsigned long long> counters(4);
     BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
@@ -510,13 +518,14 @@ extern "C" int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *
     BN_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned long long h[3];
+    unsigned long long h[4];
     BN_HIP(hipMemcpy(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = ms;
+      stats->zeroMajorantSteps = (long long)h[3];
     }
   });
 }
@@ -684,6 +693,7 @@ extern "C" int bnamd_amr_create(const bnamd_amr_desc *desc, void *stream, bnamd_
         dd.mx = 1 + (int)(size.x * f);
         dd.my = 1 + (int)(size.y * f);
         dd.mz = 1 + (int)(size.z * f);
+        BN_REQUIRE((long long)dd.my * dd.mz < (1ll << 24), "bnamd_amr_create: over 2^24 macrocells in y*z");
         dd.mcOrigin = wlo;
         dd.mcSpacing = size * rcp(vec3f((float)dd.mx, (float)dd.my, (float)dd.mz));
       }
@@ -819,7 +829,7 @@ extern "C" int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera
     BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
                "bnamd_amr_render: bad frame size or pathsPerPixel");
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counters(3);
+    DevBuf<unsigned long long> counters(4);
     BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
@@ -835,13 +845,14 @@ extern "C" int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera
     BN_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned long long h[3];
+    unsigned long long h[4];
     BN_HIP(hipMemcpy(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = ms;
+      stats->zeroMajorantSteps = (long long)h[3];
     }
   });
 }

@@ -60,20 +60,22 @@ def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
     for k in range(warmup):
         vol.render(cam, api.render_params(W, H, spp=spp, accum_id=0), accum=acc)
     api.capi.check(api.capi.load().bnamd_synchronize())
-    samples = dda = rays = 0
+    samples = dda = rays = zero = 0
     kernel_ms = 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         o = vol.render(cam, api.render_params(W, H, spp=spp, accum_id=0), accum=acc)
         samples += o["stats"]["samples"]
         dda += o["stats"]["ddaSteps"]
+        zero += o["stats"]["zeroMajorantSteps"]
         rays += o["stats"]["raysTraced"]
         kernel_ms += o["stats"]["ms"]
         log(f"volume step {k}: {o['stats']['ms']:.1f} ms, {o['stats']['samples'] / 1e6:.1f} M samples")
     api.capi.check(api.capi.load().bnamd_synchronize())
     wall = time.perf_counter() - t0
     img = acc.numpy()
-    res = {"samples": samples, "ddaSteps": dda, "rays": rays, "wall_s": wall, "kernel_ms": kernel_ms,
+    res = {"samples": samples, "ddaSteps": dda, "zeroSteps": zero, "rays": rays, "wall_s": wall,
+           "kernel_ms": kernel_ms,
            "steps": steps, "image_mean": float(img[..., :3].mean() / spp)}
     if cpu:
         base, oimg = volume_cpu_baseline(w2, cpu_seconds)
@@ -133,7 +135,7 @@ def volume_cpu_baseline(w2, seconds):
     base = band_timer(lambda y0, y1: rv.render(cam, p, y0, y1, cpu_threads(), accum=acc)[2], H, seconds,
                       "Msamples/s", f"the {W}x{H}x{w2['spp']}spp frame", keep_rows=True)
     y0, y1 = base["_rows"]
-    return base, acc[y0:y1], ref
+    return base, acc[y0:y1]
 
 
 def surface_cpu_baseline(w1, seconds):
@@ -652,6 +654,7 @@ def main():
                      "alg_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": vb.get("cpu"),
         "extra": {"samples_per_step": int(per_launch_samples), "dda_steps_per_step": int(per_launch_dda),
+                  "zero_majorant_steps_per_step": vb["zeroSteps"] // vb["steps"],
                   "rays_per_step": vb["rays"] // vb["steps"], "image_mean": vb["image_mean"],
                   "vs_oracle": vb.get("vs_oracle")},
     }

@@ -192,7 +192,11 @@ int bnamd_render_surface(const bnamd_world *world, const bnamd_camera *camera,
  * MCVolumeAccel (barney/volume/).
  * --------------------------------------------------------------------- */
 typedef struct {
-  const float *scalars; /* device float[dims.x*dims.y*dims.z], x fastest */
+  /* device float[dims.x*dims.y*dims.z], x fastest. Read during
+   * bnamd_volume_create only: the volume keeps its own copy in 8^3-voxel
+   * bricks (as StructuredData::commit keeps a texture of the data), so the
+   * array may be freed or changed afterwards. */
+  const float *scalars;
   int dims[3];          /* number of scalars (cells = dims-1) */
   float gridOrigin[3];
   float gridSpacing[3];
@@ -216,6 +220,7 @@ typedef struct {
   long long ddaSteps; /* macrocells visited */
   long long raysTraced;
   float ms;
+  long long zeroMajorantSteps; /* of ddaSteps: cells with majorant 0 (no draw) */
 } bnamd_volume_stats;
 
 int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *camera,
