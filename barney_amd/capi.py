@@ -159,6 +159,9 @@ SIGNATURES = {
     "bnamd_umesh_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
                                      C.POINTER(bnamd_umesh_render_opts), C.c_void_p, C.c_void_p,
                                      C.POINTER(bnamd_volume_stats)]),
+    "bnamd_umesh_render_noalloc": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                             C.POINTER(bnamd_umesh_render_opts), C.c_void_p, C.c_void_p,
+                                             C.POINTER(bnamd_volume_stats)]),
     "bnamd_render_batch": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),
                                      C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(bnamd_batch_stats)]),
     "bnamd_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(bnamd_views), C.POINTER(bnamd_render_params),

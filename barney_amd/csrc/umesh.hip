@@ -5,6 +5,8 @@
 // (mapMCs, MCGrid.cu:94-116), and the volume + iso-surface renderer.
 #include "shade.h"
 #include "umesh.h"
+#include <mutex>
+#include <string>
 #include <chrono>
 #include <cmath>
 
@@ -538,6 +540,63 @@ UMeshDD iso_dd(const UMesh &m, float isoValue, hipStream_t s) {
   return d;
 }
 
+// Lanes per pixel from the paths per pixel (general cells: 4 from 4 spp,
+// else 1); the kernel instance for them.
+void render_umesh_dispatch(const UMeshDD &dd, const UMeshOpts &o, const CameraDD &cam, const RenderDD &rp, int spp,
+                           float4 *accum, unsigned long long *cnt, size_t lds, hipStream_t s) {
+  int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
+  if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
+  if (dd.elemTypes) {
+    if (lanes >= 4) launch_render_umesh<4, true>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+    else launch_render_umesh<1, true>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  }
+  else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  else launch_render_umesh<1>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  BN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of bnamd_umesh_render_noalloc, which allocates nothing: one array
+// per GPU for the life of the process, shared by every call on that GPU (the
+// last call's contents stay; calls that overlap on one GPU race). Words
+// [0, 64): the four 64-bit counters (256 bytes); then the iso-surface bit
+// mask, one bit per macrocell, at most kNoAllocMaskWords words (32 M
+// macrocells, 4 MiB).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocHeaderWords = 64;
+constexpr size_t kNoAllocMaskWords = size_t(1) << 20;
+static __device__ __align__(256) uint32_t scratch[kNoAllocHeaderWords + kNoAllocMaskWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+uint32_t *gScratchPtr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch (the array's name is a host-side
+// stand-in and must not be passed to kernels), looked up once per device
+// with that device current.
+uint32_t *noalloc_scratch() {
+  int dev = 0;
+  BN_HIP(hipGetDevice(&dev));
+  BN_REQUIRE(dev >= 0 && dev < kMaxDevices, "bnamd_umesh_render_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratchPtr[dev]) {
+    void *p = nullptr;
+    BN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BN_REQUIRE(p, "bnamd_umesh_render_noalloc: no scratch address");
+    gScratchPtr[dev] = (uint32_t *)p;
+  }
+  return gScratchPtr[dev];
+}
+
+// Zeroes n 64-bit counters (a kernel, not a memset node, so that graph
+// captures of the call replay correctly).
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
 }  // namespace bnamd
 
@@ -780,18 +839,7 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     BN_HIP(hipEventCreate(&e1));
     BN_HIP(hipEventRecord(e0, s));
     const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
-    const int spp = params->pathsPerPixel;
-    int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
-    if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
-    if (dd.elemTypes) {  // general cells: 4 lanes per pixel from 4 spp, else 1
-      if (lanes >= 4) launch_render_umesh<4, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-      else launch_render_umesh<1, true>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    }
-    else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    else launch_render_umesh<1>(dd, o, cam, rp, spp, (float4 *)accum, cnt.ptr, lds, s);
-    BN_HIP(hipGetLastError());
+    render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, cnt.ptr, lds, s);
     BN_HIP(hipEventRecord(e1, s));
     BN_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -805,6 +853,56 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = ms;
+      stats->zeroMajorantSteps = (long long)h[3];
+    }
+  });
+}
+
+extern "C" int bnamd_umesh_render_noalloc(const bnamd_umesh *mesh, const bnamd_camera *camera,
+                                          const bnamd_render_params *params, const bnamd_umesh_render_opts *opts,
+                                          float *accum, void *stream, bnamd_volume_stats *stats) {
+  return guarded([&] {
+    BN_REQUIRE(mesh && camera && params && opts && accum, "bnamd_umesh_render_noalloc: NULL argument");
+    BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
+               "bnamd_umesh_render_noalloc: bad frame size");
+    BN_REQUIRE(!opts->volume || mesh->m.haveTF,
+               "bnamd_umesh_render_noalloc: volume rendering needs a transfer function");
+    const UMesh &m = mesh->m;
+    const size_t ncell = (size_t)m.dd.mx * m.dd.my * m.dd.mz;
+    const size_t maskWords = (ncell + 31) / 32;
+    BN_REQUIRE(!opts->iso || maskWords <= kNoAllocMaskWords,
+               "bnamd_umesh_render_noalloc: the iso-surface mask of this mesh's " + std::to_string(ncell) +
+                   " macrocells exceeds the scratch maximum of " + std::to_string(kNoAllocMaskWords * 32) +
+                   " macrocells; use bnamd_umesh_render");
+    const size_t lds = trace_lds_bytes(m.dd.depth, 256);
+    BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_render_noalloc: BVH too deep for the LDS stack");
+    const CameraDD cam = make_camera(*camera);
+    const RenderDD rp = make_render_dd(*params);
+    UMeshOpts o;
+    o.volume = opts->volume;
+    o.iso = opts->iso;
+    o.isoValue = opts->isoValue;
+    o.isoColor = {opts->isoColor[0], opts->isoColor[1], opts->isoColor[2]};
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *sp = noalloc_scratch();
+    unsigned long long *cnt = (unsigned long long *)sp;
+    k_zero_u64<<<1, 64, 0, s>>>(cnt, 4);
+    UMeshDD dd = m.dd;
+    if (o.iso) {  // this call's mask, whatever an earlier call left there
+      uint32_t *mask = sp + kNoAllocHeaderWords;
+      k_iso_mask<<<(int)std::min<size_t>(div_up(maskWords, 256), 4096), 256, 0, s>>>(m.ranges.ptr, ncell,
+                                                                                    o.isoValue, mask);
+      dd.isoMask = mask;
+    }
+    render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, cnt, lds, s);
+    if (stats) {  // read back on the stream (not for graph capture: pass NULL there)
+      unsigned long long h[4];
+      BN_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+      BN_HIP(hipStreamSynchronize(s));
+      stats->samples = (long long)h[0];
+      stats->ddaSteps = (long long)h[1];
+      stats->raysTraced = (long long)h[2];
+      stats->ms = -1.f;
       stats->zeroMajorantSteps = (long long)h[3];
     }
   });

@@ -373,6 +373,21 @@ int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *camera,
                        const bnamd_render_params *params, const bnamd_umesh_render_opts *opts,
                        float *accum, void *stream, bnamd_volume_stats *stats);
 
+/* bnamd_umesh_render without any allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its counters and its iso-surface mask live in one fixed per-GPU array the
+ * library holds until the process ends (4 MiB + 256 B): meshes whose
+ * macrocell grid exceeds 32 M cells are rejected when opts->iso is set
+ * (use bnamd_umesh_render). Every call on a GPU shares that array, so calls
+ * that overlap on one GPU race. The image equals bnamd_umesh_render's.
+ * stats may be NULL (enqueue only, as under capture); when given, the call
+ * waits for the stream to read the counters back, and stats->ms is -1 (not
+ * timed). */
+int bnamd_umesh_render_noalloc(const bnamd_umesh *mesh, const bnamd_camera *camera,
+                               const bnamd_render_params *params, const bnamd_umesh_render_opts *opts,
+                               float *accum, void *stream, bnamd_volume_stats *stats);
+
 /* ------------------------------------------------------------------------
  * Emission-absorption volume rendering and its adjoint (differentiable
  * form; not in the reference). Fixed-step march over a structured density
