@@ -34,7 +34,8 @@ class bnamd_material(C.Structure):
 
 class bnamd_world_info(C.Structure):
     _fields_ = [("numPrims", C.c_longlong), ("numNodes", C.c_longlong), ("depth", C.c_int),
-                ("buildMs", C.c_float), ("sceneLower", C.c_float * 3), ("sceneUpper", C.c_float * 3)]
+                ("buildMs", C.c_float), ("sceneLower", C.c_float * 3), ("sceneUpper", C.c_float * 3),
+                ("numBlas", C.c_int)]
 
 
 class bnamd_camera(C.Structure):
@@ -127,6 +128,13 @@ SIGNATURES = {
     "bnamd_world_build": (C.c_int, [C.POINTER(bnamd_group), C.c_int, C.POINTER(bnamd_instance), C.c_int,
                                     C.POINTER(bnamd_material), C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "bnamd_world_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_blas_build": (C.c_int, [C.POINTER(bnamd_group), C.POINTER(bnamd_material), C.c_int, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    "bnamd_blas_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_world_build_instances": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(bnamd_instance), C.c_int,
+                                              C.POINTER(bnamd_material), C.c_int, C.c_void_p,
+                                              C.POINTER(C.c_void_p)]),
+    "bnamd_world_set_transforms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_void_p]),
     "bnamd_world_info_get": (C.c_int, [C.c_void_p, C.POINTER(bnamd_world_info)]),
     "bnamd_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_render_surface": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),

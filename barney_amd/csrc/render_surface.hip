@@ -147,8 +147,6 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counter(3);
-    BN_HIP(hipMemsetAsync(counter.ptr, 0, 3 * sizeof(unsigned long long), s));
     const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
     // xcd_remap hands XCD x the tiles [x*per, (x+1)*per): the grid must hold
     // 8*per workgroups (a multiple of 8) to cover every tile
@@ -156,25 +154,16 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     const int block = 256;
     const size_t lds = trace_lds_bytes(w.depth, block);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_surface: BVH too deep for the LDS stack");
-    hipEvent_t e0, e1;
-    BN_HIP(hipEventCreate(&e0));
-    BN_HIP(hipEventCreate(&e1));
-    BN_HIP(hipEventRecord(e0, s));
-    if (w.twoLevel)
-      k_render_surface<true><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                                       (int4 *)ids, counter.ptr);
-    else
-      k_render_surface<false><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                                        (int4 *)ids, counter.ptr);
-    BN_HIP(hipGetLastError());
-    BN_HIP(hipEventRecord(e1, s));
-    BN_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     unsigned long long n[3] = {0, 0, 0};
-    BN_HIP(hipMemcpy(n, counter.ptr, sizeof(n), hipMemcpyDeviceToHost));
+    const float ms = render_on_stream(s, w.counters, 3, stats != nullptr, n, [&] {
+      unsigned long long *c = w.counters.ptr;
+      if (w.twoLevel)
+        k_render_surface<true><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+                                                         (int4 *)ids, c);
+      else
+        k_render_surface<false><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
+                                                          (int4 *)ids, c);
+    });
     if (stats) {
       stats->raysTraced = (long long)n[0];
       stats->nodeVisits = (long long)n[1];

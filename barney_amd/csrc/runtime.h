@@ -82,4 +82,37 @@ struct DevBuf {
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// A render entry point's work on stream s: counters (numCounters 64-bit
+// words, owned by the rendered object) are zeroed, then launch() enqueues
+// the kernels. Without stats requested the call only enqueues: the host
+// does not wait, and the counters are left for nobody. With stats, the
+// launch is timed with events, the host waits for the stream and h receives
+// the counters; the return value is the time in ms.
+template <class F>
+float render_on_stream(hipStream_t s, DevBuf<unsigned long long> &counters, int numCounters, bool wantStats,
+                       unsigned long long *h, F &&launch) {
+  counters.ensure(numCounters);
+  BN_HIP(hipMemsetAsync(counters.ptr, 0, numCounters * sizeof(unsigned long long), s));
+  if (!wantStats) {
+    launch();
+    BN_HIP(hipGetLastError());
+    return 0.f;
+  }
+  hipEvent_t e0, e1;
+  BN_HIP(hipEventCreate(&e0));
+  BN_HIP(hipEventCreate(&e1));
+  BN_HIP(hipEventRecord(e0, s));
+  launch();
+  BN_HIP(hipGetLastError());
+  BN_HIP(hipEventRecord(e1, s));
+  BN_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  BN_HIP(hipMemcpyAsync(h, counters.ptr, numCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  BN_HIP(hipStreamSynchronize(s));
+  return ms;
+}
+
 }  // namespace bnamd

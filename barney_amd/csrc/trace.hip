@@ -4,6 +4,7 @@
 #include <hip/hip_fp16.h>
 #include <cmath>
 #include <algorithm>
+#include <memory>
 
 namespace bnamd {
 
@@ -173,38 +174,66 @@ void xfm_box(const float *m, const float *lo, const float *hi, float4 &blo, floa
 // Two levels: one BVH8 per referenced group in object space, and a BVH8
 // over the instances' world boxes (InstanceGroup::buildAccel,
 // rtcore/cuda/cudaGroup.cu:144-258).
-void build_two_level(World &w, const bnamd_group *groups, int numGroups, const bnamd_instance *instances,
-                     int numInstances, const std::vector<MaterialDev> &mats, int numMaterials, hipStream_t s) {
-  w.twoLevel = true;
-  std::vector<int> blasOf(numGroups, -1);
+// One group's bottom level in object space (Group::buildAccel of a
+// triangles or user-geometry group, rtcore/cuda/cudaGroup.cu:261-562). The
+// material table is part of the build: a triangle primitive records whether
+// its material makes it transparent to shadow rays (Glass), and the BVH
+// build reads the table for that.
+std::unique_ptr<World> build_blas(const bnamd_group &grp, const std::vector<MaterialDev> &mats, int numMaterials,
+                                  hipStream_t s) {
+  auto b = std::make_unique<World>();
+  uint64_t offset = 0;
+  for (int gi = 0; gi < grp.numGeoms; ++gi) {
+    const bnamd_geom &src = grp.geoms[gi];
+    check_geom(src, numMaterials);
+    if (src.numPrims == 0) continue;
+    b->hgeoms.push_back(make_geom(src, gi, -1, 0, offset));
+    offset += (uint64_t)src.numPrims;
+  }
+  BN_REQUIRE(offset < 0xfffffff0ull, "bnamd_world_build: too many primitives in a group");
+  b->numPrims = (uint32_t)offset;
+  b->materials.upload(mats.data(), mats.size(), s);
+  build_world_bvh(*b, s);
+  b->hmats = mats;
+  return b;
+}
+
+bool shadow_transparent(const MaterialDev &m) {
+  return m.type == BNAMD_MAT_PBR && m.ior != 1.f && m.transmission >= 1e-3f;
+}
+
+// A bottom level built with one material table used in a world with
+// another: the IDs must be in range, and every triangle geom must be as
+// transparent to shadow rays under the world's table as its primitives
+// recorded at the build.
+void check_materials(const World &b, const std::vector<MaterialDev> &mats, int numMaterials) {
+  for (const GeomDev &g : b.hgeoms) {
+    BN_REQUIRE(g.materialID >= 0 && g.materialID < std::max(numMaterials, 1),
+               "bnamd_world_build: materialID out of range");
+    if (g.kind == BNAMD_GEOM_TRIANGLES)
+      BN_REQUIRE(shadow_transparent(b.hmats[g.materialID]) == shadow_transparent(mats[g.materialID]),
+                 "bnamd_world_build_instances: a triangle geom's material is glass in one material table and "
+                 "not in the other (its BLAS was built with a different table)");
+  }
+}
+
+// The instance level over w.instBlas with the given transforms: instance
+// records, their world boxes, and a BVH8 over those boxes
+// (InstanceGroup::buildAccel, rtcore/cuda/cudaGroup.cu:144-258).
+void assemble_tlas(World &w, const float *xfms, hipStream_t s) {
+  const int numInstances = (int)w.instBlas.size();
   uint32_t maxPrims = 1;
   int maxDepth = 1;
   uint64_t total = 0, nodes = 0;
-  for (int ii = 0; ii < numInstances; ++ii) {
-    const int gidx = instances[ii].group;
-    if (blasOf[gidx] < 0) {
-      auto b = std::make_unique<World>();
-      const bnamd_group &grp = groups[gidx];
-      uint64_t offset = 0;
-      for (int gi = 0; gi < grp.numGeoms; ++gi) {
-        const bnamd_geom &src = grp.geoms[gi];
-        check_geom(src, numMaterials);
-        if (src.numPrims == 0) continue;
-        b->hgeoms.push_back(make_geom(src, gi, -1, 0, offset));
-        offset += (uint64_t)src.numPrims;
-      }
-      BN_REQUIRE(offset < 0xfffffff0ull, "bnamd_world_build: too many primitives in a group");
-      b->numPrims = (uint32_t)offset;
-      b->materials.upload(mats.data(), mats.size(), s);
-      build_world_bvh(*b, s);
-      maxPrims = std::max(maxPrims, b->numPrims);
-      maxDepth = std::max(maxDepth, b->depth);
-      blasOf[gidx] = (int)w.blas.size();
-      w.blas.push_back(std::move(b));
+  std::vector<const World *> distinct;
+  for (const World *b : w.instBlas) {
+    maxPrims = std::max(maxPrims, b->numPrims);
+    maxDepth = std::max(maxDepth, b->depth);
+    total += b->numPrims;
+    if (std::find(distinct.begin(), distinct.end(), b) == distinct.end()) {
+      distinct.push_back(b);
+      nodes += b->numNodes;
     }
-    const World &b = *w.blas[blasOf[gidx]];
-    total += b.numPrims;
-    nodes += b.numNodes;
   }
   int bits = 1;
   while (bits < 32 && (1ull << bits) < (uint64_t)maxPrims) ++bits;
@@ -214,18 +243,18 @@ void build_two_level(World &w, const bnamd_group *groups, int numGroups, const b
   std::vector<InstDev> insts(numInstances);
   std::vector<float4> boxes(2ull * numInstances);
   for (int ii = 0; ii < numInstances; ++ii) {
-    const bnamd_instance &in = instances[ii];
-    const World &b = *w.blas[blasOf[in.group]];
+    const World &b = *w.instBlas[ii];
+    const float *xfm = xfms + 12 * (size_t)ii;
     InstDev &I = insts[ii];
-    for (int k = 0; k < 12; ++k) I.o2w[k] = in.xfm[k];
-    invert_affine(in.xfm, I.w2o);
+    for (int k = 0; k < 12; ++k) I.o2w[k] = xfm[k];
+    invert_affine(xfm, I.w2o);
     I.nodes = b.nodes.ptr;
     I.prims = b.prims.ptr;
     I.geoms = b.geoms.ptr;
     I.numGeoms = (int)b.hgeoms.size();
-    I.instID = in.instID;
+    I.instID = w.instIDs[ii];
     const float zero[3] = {0.f, 0.f, 0.f};
-    xfm_box(in.xfm, b.numPrims ? b.lo : zero, b.numPrims ? b.hi : zero, boxes[2 * ii], boxes[2 * ii + 1]);
+    xfm_box(xfm, b.numPrims ? b.lo : zero, b.numPrims ? b.hi : zero, boxes[2 * ii], boxes[2 * ii + 1]);
   }
   w.insts.upload(insts.data(), insts.size(), s);
   w.instBoxes.upload(boxes.data(), boxes.size(), s);
@@ -234,14 +263,42 @@ void build_two_level(World &w, const bnamd_group *groups, int numGroups, const b
   top.numPrims = numInstances;
   top.primOffset = 0;
   top.vertices = (const float *)w.instBoxes.ptr;
-  w.hgeoms.push_back(top);
+  w.hgeoms.assign(1, top);
   w.numPrims = (uint32_t)numInstances;
-  w.materials.upload(mats.data(), mats.size(), s);
+  w.twoLevel = true;
   build_world_bvh(w, s);
   w.tlasDepth = w.depth;
   w.depth = w.tlasDepth + maxDepth + 1;
+  w.numBlas = (int)distinct.size();
   w.totalPrims = total;
   w.totalNodes = nodes + w.numNodes;
+}
+
+std::vector<float> instance_xfms(const bnamd_instance *instances, int numInstances) {
+  std::vector<float> x(12 * (size_t)numInstances);
+  for (int ii = 0; ii < numInstances; ++ii)
+    for (int k = 0; k < 12; ++k) x[12 * (size_t)ii + k] = instances[ii].xfm[k];
+  return x;
+}
+
+// Two levels: one BVH8 per referenced group in object space, and a BVH8
+// over the instances' world boxes.
+void build_two_level(World &w, const bnamd_group *groups, int numGroups, const bnamd_instance *instances,
+                     int numInstances, const std::vector<MaterialDev> &mats, int numMaterials, hipStream_t s) {
+  std::vector<int> blasOf(numGroups, -1);
+  w.instBlas.clear();
+  w.instIDs.clear();
+  for (int ii = 0; ii < numInstances; ++ii) {
+    const int gidx = instances[ii].group;
+    if (blasOf[gidx] < 0) {
+      blasOf[gidx] = (int)w.blas.size();
+      w.blas.push_back(build_blas(groups[gidx], mats, numMaterials, s));
+    }
+    w.instBlas.push_back(w.blas[blasOf[gidx]].get());
+    w.instIDs.push_back(instances[ii].instID);
+  }
+  w.materials.upload(mats.data(), mats.size(), s);
+  assemble_tlas(w, instance_xfms(instances, numInstances).data(), s);
 }
 
 }  // namespace
@@ -300,6 +357,76 @@ extern "C" int bnamd_world_destroy(bnamd_world *world) {
   return guarded([&] { delete world; });
 }
 
+struct bnamd_blas_s {
+  std::unique_ptr<World> b;
+};
+
+extern "C" int bnamd_blas_build(const bnamd_group *group, const bnamd_material *materials, int numMaterials,
+                                void *stream, bnamd_blas **out) {
+  return guarded([&] {
+    BN_REQUIRE(group && out, "bnamd_blas_build: NULL argument");
+    BN_REQUIRE(group->numGeoms >= 0 && (group->numGeoms == 0 || group->geoms), "bnamd_blas_build: bad group");
+    BN_REQUIRE(numMaterials >= 0 && (numMaterials == 0 || materials), "bnamd_blas_build: bad material table");
+    hipStream_t s = (hipStream_t)stream;
+    auto *bo = new bnamd_blas_s;
+    try {
+      bo->b = build_blas(*group, make_materials(materials, numMaterials), numMaterials, s);
+      BN_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      delete bo;
+      throw;
+    }
+    *out = bo;
+  });
+}
+
+extern "C" int bnamd_blas_destroy(bnamd_blas *blas) {
+  return guarded([&] { delete blas; });
+}
+
+extern "C" int bnamd_world_build_instances(const bnamd_blas *const *blas, int numBlas,
+                                           const bnamd_instance *instances, int numInstances,
+                                           const bnamd_material *materials, int numMaterials, void *stream,
+                                           bnamd_world **out) {
+  return guarded([&] {
+    BN_REQUIRE(out, "bnamd_world_build_instances: out is NULL");
+    BN_REQUIRE(numBlas >= 0 && numInstances >= 0, "bnamd_world_build_instances: negative counts");
+    BN_REQUIRE(numInstances == 0 || (blas && instances), "bnamd_world_build_instances: NULL array");
+    auto *wo = new bnamd_world_s;
+    try {
+      World &w = wo->w;
+      hipStream_t s = (hipStream_t)stream;
+      const std::vector<MaterialDev> mats = make_materials(materials, numMaterials);
+      for (int ii = 0; ii < numInstances; ++ii) {
+        const int g = instances[ii].group;
+        BN_REQUIRE(g >= 0 && g < numBlas && blas[g], "bnamd_world_build_instances: bad group index");
+        check_materials(*blas[g]->b, mats, numMaterials);
+        w.instBlas.push_back(blas[g]->b.get());
+        w.instIDs.push_back(instances[ii].instID);
+      }
+      w.materials.upload(mats.data(), mats.size(), s);
+      assemble_tlas(w, instance_xfms(instances, numInstances).data(), s);
+      BN_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      delete wo;
+      throw;
+    }
+    *out = wo;
+  });
+}
+
+extern "C" int bnamd_world_set_transforms(bnamd_world *world, const float *xfms, int numInstances, void *stream) {
+  return guarded([&] {
+    BN_REQUIRE(world && (xfms || numInstances == 0), "bnamd_world_set_transforms: NULL argument");
+    World &w = world->w;
+    BN_REQUIRE(w.twoLevel, "bnamd_world_set_transforms: the world has no instance level (built flattened)");
+    BN_REQUIRE(numInstances == (int)w.instBlas.size(), "bnamd_world_set_transforms: one transform per instance");
+    hipStream_t s = (hipStream_t)stream;
+    assemble_tlas(w, xfms, s);
+    BN_HIP(hipStreamSynchronize(s));
+  });
+}
+
 extern "C" int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info) {
   return guarded([&] {
     BN_REQUIRE(world && info, "bnamd_world_info_get: NULL argument");
@@ -307,6 +434,7 @@ extern "C" int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *
     info->numNodes = (long long)world->w.totalNodes;
     info->depth = world->w.depth;
     info->buildMs = world->w.buildMs;
+    info->numBlas = world->w.twoLevel ? world->w.numBlas : 0;
     for (int k = 0; k < 3; ++k) {
       info->sceneLower[k] = world->w.lo[k];
       info->sceneUpper[k] = world->w.hi[k];

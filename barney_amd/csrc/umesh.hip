@@ -15,6 +15,7 @@ namespace bnamd {
 size_t trace_lds_bytes(int depth, int block);
 
 struct UMesh {
+  mutable DevBuf<unsigned long long> counters;  // the render's counters
   World w;                 // point-location BVH (prims freed after packing)
   DevBuf<float4> tets;     // leaf order: 4 per tet, or 8 per general element
   DevBuf<uint8_t> types;   // general elements: ELEM_* per leaf position
@@ -830,24 +831,13 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     o.isoValue = opts->isoValue;
     o.isoColor = {opts->isoColor[0], opts->isoColor[1], opts->isoColor[2]};
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> cnt(4);
-    BN_HIP(hipMemsetAsync(cnt.ptr, 0, cnt.bytes(), s));
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_render: BVH too deep for the LDS stack");
-    hipEvent_t e0, e1;
-    BN_HIP(hipEventCreate(&e0));
-    BN_HIP(hipEventCreate(&e1));
-    BN_HIP(hipEventRecord(e0, s));
-    const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
-    render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, cnt.ptr, lds, s);
-    BN_HIP(hipEventRecord(e1, s));
-    BN_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    unsigned long long h[4];
-    BN_HIP(hipMemcpy(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    const float ms = render_on_stream(s, mesh->m.counters, 4, stats != nullptr, h, [&] {
+      const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
+      render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, mesh->m.counters.ptr, lds, s);
+    });
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];

@@ -15,6 +15,7 @@ namespace bnamd {
 struct Volume {
   VolumeDD dd;
   DevBuf<float> bricks;
+  mutable DevBuf<unsigned long long> counters;  // the render's counters
   int cellsPerMC = 8;
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
@@ -502,24 +503,12 @@ extern "C" int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *
     BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
                "bnamd_volume_render: bad frame size or pathsPerPixel");
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counters(4);
-    BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
-    hipEvent_t e0, e1;
-    BN_HIP(hipEventCreate(&e0));
-    BN_HIP(hipEventCreate(&e1));
-    BN_HIP(hipEventRecord(e0, s));
-    render_volume_dispatch(vol->v.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, counters.ptr, s);
-    BN_HIP(hipGetLastError());
-    BN_HIP(hipEventRecord(e1, s));
-    BN_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    unsigned long long h[4];
-    BN_HIP(hipMemcpy(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    const float ms = render_on_stream(s, vol->v.counters, 4, stats != nullptr, h, [&] {
+      render_volume_dispatch(vol->v.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, vol->v.counters.ptr, s);
+    });
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
@@ -567,6 +556,7 @@ namespace bnamd {
 
 struct Amr {
   AmrDD dd{};
+  mutable DevBuf<unsigned long long> counters;  // the render's counters
   int numBlocks = 0;
   int lookup[3] = {0, 0, 0};
   DevBuf<float4> dom;
@@ -829,24 +819,12 @@ extern "C" int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera
     BN_REQUIRE(params->width > 0 && params->height > 0 && params->pathsPerPixel >= 1,
                "bnamd_amr_render: bad frame size or pathsPerPixel");
     hipStream_t s = (hipStream_t)stream;
-    DevBuf<unsigned long long> counters(4);
-    BN_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes(), s));
     const CameraDD cam = make_camera(*camera);
     const RenderDD rp = make_render_dd(*params);
-    hipEvent_t e0, e1;
-    BN_HIP(hipEventCreate(&e0));
-    BN_HIP(hipEventCreate(&e1));
-    BN_HIP(hipEventRecord(e0, s));
-    render_volume_dispatch(amr->a.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, counters.ptr, s);
-    BN_HIP(hipGetLastError());
-    BN_HIP(hipEventRecord(e1, s));
-    BN_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    BN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    unsigned long long h[4];
-    BN_HIP(hipMemcpy(h, counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    const float ms = render_on_stream(s, amr->a.counters, 4, stats != nullptr, h, [&] {
+      render_volume_dispatch(amr->a.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, amr->a.counters.ptr, s);
+    });
     if (stats) {
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];

@@ -86,6 +86,7 @@ struct World {
   DevBuf<PrimRec> prims;
   DevBuf<GeomDev> geoms;
   DevBuf<MaterialDev> materials;
+  std::vector<MaterialDev> hmats;  // bottom levels: the table they were built with
   std::vector<GeomDev> hgeoms;
   uint32_t numPrims = 0, numNodes = 0;
   int depth = 1;
@@ -94,7 +95,11 @@ struct World {
   // two-level worlds: one bottom level per group, instance records, and the
   // instance level in nodes/prims above
   bool twoLevel = false;
-  std::vector<std::unique_ptr<World>> blas;
+  std::vector<std::unique_ptr<World>> blas;  // bottom levels this world owns
+  std::vector<const World *> instBlas;       // each instance's bottom level (owned or the caller's)
+  std::vector<int> instIDs;
+  int numBlas = 0;                           // distinct bottom levels referenced
+  mutable DevBuf<unsigned long long> counters;  // the surface render's counters
   DevBuf<InstDev> insts;
   DevBuf<float4> instBoxes;
   int tlasDepth = 0, localBits = 0;

@@ -33,18 +33,41 @@ void Buffer::resize(size_t newNumBytes) {
 }
 
 // ----------------------------------------------------------- TextureData
+namespace {
+size_t texel_bytes(DataType f) {
+  switch (f) {
+    case UCHAR: return 1;
+    case UCHAR4: return 4;
+    case USHORT: return 2;
+    case FLOAT: return 4;
+    case FLOAT2: return 8;
+    case FLOAT3: return 12;
+    case FLOAT4: return 16;
+    default: throw std::runtime_error("rtc::amd::TextureData: unsupported texel format");
+  }
+}
+}  // namespace
+
 TextureData::TextureData(Device *device, vec3i dims, DataType format, const void *texels)
-    : device(device), dims(dims) {
-  if (format != DataType::FLOAT) throw std::runtime_error("rtc::amd::TextureData: only FLOAT texels");
-  const size_t bytes = sizeof(float) * (size_t)dims.x * dims.y * dims.z;
-  void *p = nullptr;
-  check(bnamd_device_malloc(&p, bytes), "rtc::amd::TextureData");
-  d_texels = (float *)p;
-  if (texels) check(bnamd_memcpy_h2d(d_texels, texels, bytes), "rtc::amd::TextureData");
+    : device(device), dims(dims), format(format) {
+  if (dims.x <= 0 || dims.y <= 0 || dims.z < 0) throw std::runtime_error("rtc::amd::TextureData: bad dims");
+  numBytes = texel_bytes(format) * (size_t)dims.x * dims.y * (dims.z > 0 ? dims.z : 1);
+  check(bnamd_device_malloc(&d_texels, numBytes), "rtc::amd::TextureData");
+  if (texels) check(bnamd_memcpy_h2d(d_texels, texels, numBytes), "rtc::amd::TextureData");
 }
 
 TextureData::~TextureData() {
   if (d_texels) (void)bnamd_device_free(d_texels);
+}
+
+Texture *TextureData::createTexture(const TextureDesc &desc) { return new Texture(this, desc); }
+
+Texture::Texture(TextureData *data, const TextureDesc &desc) : device(data->device), data(data), desc(desc) {
+  // the structured-volume sampler filters trilinearly with clamping on
+  // texel centres (StructuredData.cu:154-173 asks for exactly that)
+  if (data->dims.z > 0 && (desc.addressMode[0] != CLAMP || desc.addressMode[1] != CLAMP ||
+                           desc.addressMode[2] != CLAMP || desc.normalizedCoords))
+    throw std::runtime_error("rtc::amd::Texture: 3D textures are sampled with CLAMP, unnormalized coordinates");
 }
 
 // -------------------------------------------------------- GeomType / Geom
@@ -53,6 +76,49 @@ GeomType::GeomType(Device *device, GeomKind kind, const char *name) : device(dev
 Geom *GeomType::createGeom() { return new Geom(this); }
 
 Geom::Geom(GeomType *gt) : gt(gt) {}
+
+void Geom::setDD(const void *dd) {
+  if (!dd) throw std::runtime_error("rtc::amd::Geom::setDD: NULL");
+  barney_dd::Geometry base;
+  std::memcpy(&base, dd, barney_dd::kGeometryDDSize);
+  const uint8_t *fields = (const uint8_t *)dd + barney_dd::kGeometryDDSize;
+  userID = base.userID;
+  materialID = base.materialID;
+  const barney_dd::GeometryAttribute &col = base.attributes.colorAttribute;
+  ddColors = (col.scope == barney_dd::SCOPE_PER_PRIM && col.fromArray.type == barney_dd::BN_FLOAT32_VEC3)
+                 ? col.fromArray.ptr : nullptr;
+  ddNormals = nullptr;
+  switch (gt->kind) {
+    case GeomKind::Triangles: {
+      barney_dd::TrianglesFields f;
+      std::memcpy(&f, fields, sizeof(f));
+      ddIndices = f.indices;
+      ddVertices = f.vertices;
+      ddNormals = f.normals;
+      ddRadii = nullptr;
+      break;
+    }
+    case GeomKind::Spheres: {
+      barney_dd::SpheresFields f;
+      std::memcpy(&f, fields, sizeof(f));
+      ddVertices = f.origins;
+      ddRadii = f.radii;
+      if (f.colors) ddColors = f.colors;
+      ddIndices = nullptr;
+      defaultRadius = f.defaultRadius;
+      break;
+    }
+    case GeomKind::Cylinders: {
+      barney_dd::CylindersFields f;
+      std::memcpy(&f, fields, sizeof(f));
+      ddVertices = f.vertices;
+      ddIndices = f.indices;
+      ddRadii = f.radii;
+      break;
+    }
+  }
+  haveDD = true;
+}
 
 void Geom::setVertices(Buffer *v, int n) {
   vertices = v;
@@ -91,13 +157,37 @@ bnamd_geom Geom::desc() const {
   g.numPrims = primCount;
   g.numVertices = numVertices;
   g.materialID = materialID;
-  g.vertices = vertices ? (const float *)vertices->getDD() : nullptr;
-  g.indices = indices ? (const int *)indices->getDD() : nullptr;
-  g.radii = radii ? (const float *)radii->getDD() : nullptr;
-  g.colors = colors ? (const float *)colors->getDD() : nullptr;
+  if (haveDD) {
+    g.vertices = (const float *)ddVertices;
+    g.indices = (const int *)ddIndices;
+    g.radii = (const float *)ddRadii;
+    g.colors = (const float *)ddColors;
+  } else {
+    g.vertices = vertices ? (const float *)vertices->getDD() : nullptr;
+    g.indices = indices ? (const int *)indices->getDD() : nullptr;
+    g.radii = radii ? (const float *)radii->getDD() : nullptr;
+    g.colors = colors ? (const float *)colors->getDD() : nullptr;
+  }
   g.defaultRadius = defaultRadius;
   g.userID = userID;
   return g;
+}
+
+// -------------------------------------------------------------- GeomGroup
+GeomGroup::~GeomGroup() {
+  if (blas) (void)bnamd_blas_destroy(blas);
+}
+
+void GeomGroup::buildAccel() {
+  std::vector<bnamd_geom> gs;
+  for (Geom *g : geoms) gs.push_back(g->desc());
+  bnamd_group grp{gs.data(), (int)gs.size()};
+  bnamd_blas *b = nullptr;
+  const std::vector<bnamd_material> &mats = device->materials;
+  check(bnamd_blas_build(&grp, mats.empty() ? nullptr : mats.data(), (int)mats.size(), nullptr, &b),
+        "rtc::amd::GeomGroup::buildAccel");
+  if (blas) (void)bnamd_blas_destroy(blas);
+  blas = b;
 }
 
 // ---------------------------------------------------------- InstanceGroup
@@ -120,30 +210,61 @@ void InstanceGroup::setTransforms(const std::vector<affine3f> &newXfms) {
   xfms = newXfms;
 }
 
+namespace {
+std::vector<float> flat_xfms(const std::vector<affine3f> &xfms) {
+  std::vector<float> out;
+  for (const affine3f &x : xfms) {
+    const float m[12] = {x.vx.x, x.vx.y, x.vx.z, x.vy.x, x.vy.y, x.vy.z, x.vz.x, x.vz.y, x.vz.z, x.p.x, x.p.y, x.p.z};
+    out.insert(out.end(), m, m + 12);
+  }
+  return out;
+}
+}  // namespace
+
 void InstanceGroup::buildAccel() {
-  // each instance references its own group; geometry lists per group
-  std::vector<std::vector<bnamd_geom>> geoms(groups.size());
-  std::vector<bnamd_group> gs(groups.size());
+  // one bottom level per distinct group (built here if its group has not
+  // built it yet), referenced by every instance of that group
+  std::vector<GeomGroup *> distinct;
   std::vector<bnamd_instance> inst(groups.size());
+  const std::vector<float> m = flat_xfms(xfms);
   for (size_t i = 0; i < groups.size(); ++i) {
     auto *gg = dynamic_cast<GeomGroup *>(groups[i]);
     if (!gg) throw std::runtime_error("rtc::amd::InstanceGroup::buildAccel: nested instance groups");
-    for (Geom *g : gg->geoms) geoms[i].push_back(g->desc());
-    gs[i].geoms = geoms[i].data();
-    gs[i].numGeoms = (int)geoms[i].size();
-    inst[i].group = (int)i;
+    size_t k = 0;
+    while (k < distinct.size() && distinct[k] != gg) ++k;
+    if (k == distinct.size()) {
+      if (!gg->blas) gg->buildAccel();
+      distinct.push_back(gg);
+    }
+    inst[i].group = (int)k;
     inst[i].instID = instIDs.empty() ? (int)i : instIDs[i];
-    const affine3f &x = xfms[i];
-    const float m[12] = {x.vx.x, x.vx.y, x.vx.z, x.vy.x, x.vy.y, x.vy.z,
-                         x.vz.x, x.vz.y, x.vz.z, x.p.x,  x.p.y,  x.p.z};
-    std::memcpy(inst[i].xfm, m, sizeof(m));
+    std::memcpy(inst[i].xfm, &m[12 * i], 12 * sizeof(float));
   }
+  std::vector<const bnamd_blas *> blas;
+  for (GeomGroup *gg : distinct) blas.push_back(gg->blas);
+  const std::vector<bnamd_material> &mats = materials.empty() ? device->materials : materials;
   bnamd_world *w = nullptr;
-  check(bnamd_world_build(gs.data(), (int)gs.size(), inst.data(), (int)inst.size(),
-                          materials.empty() ? nullptr : materials.data(), (int)materials.size(), nullptr, &w),
+  check(bnamd_world_build_instances(blas.data(), (int)blas.size(), inst.data(), (int)inst.size(),
+                                    mats.empty() ? nullptr : mats.data(), (int)mats.size(), nullptr, &w),
         "rtc::amd::InstanceGroup::buildAccel");
   if (world) (void)bnamd_world_destroy(world);
   world = w;
+}
+
+void InstanceGroup::refitAccel() {
+  if (!world) {
+    buildAccel();
+    return;
+  }
+  const std::vector<float> m = flat_xfms(xfms);
+  check(bnamd_world_set_transforms(world, m.data(), (int)xfms.size(), nullptr), "rtc::amd::InstanceGroup::refitAccel");
+}
+
+int InstanceGroup::numBottomLevels() const {
+  if (!world) return 0;
+  bnamd_world_info info{};
+  check(bnamd_world_info_get(world, &info), "rtc::amd::InstanceGroup::numBottomLevels");
+  return info.numBlas;
 }
 
 void InstanceGroup::traceRays(const bnamd_ray *d_rays, bnamd_hit *d_hits, int numRays) {
@@ -198,6 +319,14 @@ TextureData *Device::createTextureData(vec3i dims, DataType format, const void *
 }
 
 void Device::freeTextureData(TextureData *t) { delete t; }
+
+void Device::freeTexture(Texture *t) { delete t; }
+
+AccelHandle getAccelHandle(Group *ig) {
+  auto *g = dynamic_cast<InstanceGroup *>(ig);
+  if (!g || !g->world) throw std::runtime_error("rtc::amd::getAccelHandle: a built InstanceGroup expected");
+  return (AccelHandle)g->world;
+}
 
 Buffer *Device::createBuffer(size_t numBytes, const void *initValues) { return new Buffer(this, numBytes, initValues); }
 

@@ -10,9 +10,10 @@
 //
 // Geometry programs are native in libbnamd: the geometry types barney
 // registers with RTC_EXPORT_USER_GEOM / RTC_EXPORT_TRIANGLES_GEOM
-// (Triangles, Spheres, Cylinders) come from createGeomType_<Name>() below,
-// and their device data is given through the typed setters of Geom rather
-// than as raw DD bytes.
+// (Triangles, Spheres, Cylinders) come from createGeomType_<Name>() below.
+// Their device data arrives as barney writes it: Geom::setDD(&dd) with the
+// bytes of barney's Triangles::DD / Spheres::DD / Cylinders::DD, whose
+// layout barney_dd below restates (or through the typed setters).
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -41,7 +42,87 @@ struct affine3f {
 // throws std::runtime_error with libbnamd's message when status != 0
 void check(int status, const char *what);
 
-enum class DataType { FLOAT };
+// rtc::DataType, AddressMode, FilterMode, ColorSpace, TextureDesc and the
+// opaque handle types of rtcore/common/rtcore-common.h, same values.
+enum DataType {
+  UCHAR = 0, UCHAR4, INT = 10, INT2, INT3, INT4, LONG = 20, LONG2, LONG3, LONG4,
+  FLOAT = 30, FLOAT2, FLOAT3, FLOAT4, USHORT = 40
+};
+enum AddressMode { WRAP, CLAMP, BORDER, MIRROR };
+enum FilterMode { FILTER_MODE_POINT, FILTER_MODE_LINEAR };
+enum ColorSpace { COLOR_SPACE_LINEAR, COLOR_SPACE_SRGB };
+struct vec4f {
+  float x, y, z, w;
+};
+struct TextureDesc {
+  FilterMode filterMode = FILTER_MODE_LINEAR;
+  AddressMode addressMode[3] = {CLAMP, CLAMP, CLAMP};
+  vec4f borderColor = {0.f, 0.f, 0.f, 0.f};
+  bool normalizedCoords = true;
+  ColorSpace colorSpace = COLOR_SPACE_LINEAR;
+};
+typedef struct _TextureObject *TextureObject;
+typedef struct _AccelHandle *AccelHandle;
+
+// The device data barney hands to Geom::setDD: Geometry::DD
+// (barney/geometry/Geometry.h:34-45, with GeometryAttributes::DD of
+// barney/render/GeometryAttributes.h:15-60 and numAttributes = 4 from
+// barney/render/HitAttributes.h:12) followed by each geometry's fields
+// (Triangles.h:28-34, Spheres.h:14-20, Cylinders.h:19-25). The vec4f in
+// GeometryAttribute::DD is 16-byte aligned, so each attribute record is 48
+// bytes; the derived structs' fields follow at kGeometryDDSize, the base's
+// data size (its vec4f has constructors, so the base is not POD for layout
+// and its tail padding is reused). barney built against this backend can
+// check the assumption: offsetof(Spheres::DD, origins) == kGeometryDDSize,
+// and likewise for Triangles::DD::indices and Cylinders::DD::vertices.
+namespace barney_dd {
+struct alignas(16) Vec4 {
+  float x, y, z, w;
+};
+struct AttributeArray {
+  const void *ptr;
+  int type;  // BNDataType
+  int size;
+};
+struct GeometryAttribute {
+  Vec4 value;
+  AttributeArray fromArray;
+  int scope;  // INVALID, CONSTANT, PER_PRIM, PER_VERTEX, FACE_VARYING
+};
+enum { SCOPE_INVALID = 0, SCOPE_CONSTANT, SCOPE_PER_PRIM, SCOPE_PER_VERTEX, SCOPE_FACE_VARYING };
+enum { BN_FLOAT32_VEC3 = 182 };  // barney/include/barney.h:147-150
+constexpr int kNumAttributes = 4;
+struct GeometryAttributes {
+  GeometryAttribute attribute[kNumAttributes];
+  GeometryAttribute colorAttribute;
+  GeometryAttribute normalAttribute;
+};
+struct Geometry {
+  GeometryAttributes attributes;
+  int userID;
+  int materialID;
+};
+constexpr size_t kGeometryDDSize = sizeof(GeometryAttributes) + 2 * sizeof(int);
+static_assert(sizeof(GeometryAttribute) == 48, "GeometryAttribute::DD is 48 bytes");
+static_assert(kGeometryDDSize == 296, "Geometry::DD holds 296 bytes of data");
+struct TrianglesFields {
+  const int *indices;       // vec3i
+  const float *vertices;    // vec3f
+  const float *normals;     // vec3f, may be null
+  const float *texcoords;   // vec2f, may be null
+};
+struct SpheresFields {
+  const float *origins;  // vec3f
+  const float *radii;    // may be null: defaultRadius
+  const float *colors;   // vec3f, may be null
+  float defaultRadius;
+};
+struct CylindersFields {
+  const float *vertices;  // vec3f
+  const int *indices;     // vec2i
+  const float *radii;
+};
+}  // namespace barney_dd
 
 struct Device;
 struct Geom;
@@ -58,14 +139,31 @@ struct Buffer {
   Device *const device;
 };
 
-// 3D scalar texture data (structured volumes); LINEAR/CLAMP filtering is
-// done by the sampler of the volume renderer, so this is device memory.
+struct Texture;
+
+// Texel data (rtcore/cudaCommon/TextureData.h:15-31): 3D scalar fields
+// (dims.z > 0, structured volumes) and 2D images (dims.z == 0). Filtering
+// is done by the renderers' own samplers, so this is plain device memory.
 struct TextureData {
   TextureData(Device *device, vec3i dims, DataType format, const void *texels);
   ~TextureData();
+  Texture *createTexture(const TextureDesc &desc);
   Device *const device;
-  vec3i dims;
-  float *d_texels = nullptr;
+  const vec3i dims;
+  const DataType format;
+  void *d_texels = nullptr;
+  size_t numBytes = 0;
+};
+
+// A view of texel data with its sampling description
+// (rtcore/cudaCommon/Texture.h:12-24). getDD() is the TextureObject the
+// device side of barney stores: here the device address of the texels.
+struct Texture {
+  Texture(TextureData *data, const TextureDesc &desc);
+  TextureObject getDD() const { return (TextureObject)data->d_texels; }
+  Device *const device;
+  TextureData *const data;
+  const TextureDesc desc;
 };
 
 enum class GeomKind { Triangles = BNAMD_GEOM_TRIANGLES, Spheres = BNAMD_GEOM_SPHERES,
@@ -83,6 +181,11 @@ struct GeomType {
 struct Geom {
   explicit Geom(GeomType *gt);
   virtual ~Geom() = default;
+  // the bytes of barney's Triangles::DD / Spheres::DD / Cylinders::DD
+  // (rtcore/cuda/Geom.h:16-69: Geom::setDD copies sizeof(DD) bytes): the
+  // geometry's buffers, userID, materialID, and per-primitive float3 colors
+  // from the color attribute (other attribute scopes are not read)
+  void setDD(const void *dd);
   // Triangles (TrianglesGeom::setVertices / setIndices): float3 / int3
   // device buffers; numIndices = number of triangles
   void setVertices(Buffer *vertices, int numVertices);
@@ -101,6 +204,10 @@ struct Geom {
   int numVertices = 0, numIndices = 0, primCount = 0;
   float defaultRadius = 1.f;
   int materialID = 0, userID = 0;
+  // device pointers given through setDD (take precedence over the Buffers)
+  const void *ddVertices = nullptr, *ddIndices = nullptr, *ddRadii = nullptr, *ddColors = nullptr;
+  const void *ddNormals = nullptr;
+  bool haveDD = false;
 };
 
 struct Group {
@@ -111,10 +218,15 @@ struct Group {
   Device *const device;
 };
 
+// A triangles or user-geometry group: buildAccel builds its bottom-level
+// BVH8 (rtcore/cuda/cudaGroup.cu:261-562), which every instance of the
+// group shares.
 struct GeomGroup : public Group {
   GeomGroup(Device *device, const std::vector<Geom *> &geoms) : Group(device), geoms(geoms) {}
-  void buildAccel() override {}  // built as part of the instance group
+  ~GeomGroup() override;
+  void buildAccel() override;
   const std::vector<Geom *> geoms;
+  bnamd_blas *blas = nullptr;
 };
 struct TrianglesGeomGroup : public GeomGroup {
   using GeomGroup::GeomGroup;
@@ -123,17 +235,24 @@ struct UserGeomGroup : public GeomGroup {
   using GeomGroup::GeomGroup;
 };
 
+// Instances of geometry groups (rtcore/cuda/cudaGroup.cu:144-258): one
+// bottom level per distinct group (built on demand), instances over them.
+// refitAccel after setTransforms rebuilds the instance level only.
 struct InstanceGroup : public Group {
   InstanceGroup(Device *device, const std::vector<Group *> &groups, const std::vector<int> &instIDs,
                 const std::vector<affine3f> &xfms);
   ~InstanceGroup() override;
   void buildAccel() override;
+  void refitAccel() override;
   void setTransforms(const std::vector<affine3f> &xfms);
+  // distinct bottom levels the built world references
+  int numBottomLevels() const;
   // closest hit per ray (TraceInterface::traceRay over this accel);
   // rays/hits are device arrays
   void traceRays(const bnamd_ray *d_rays, bnamd_hit *d_hits, int numRays);
   bnamd_world *getDD() const { return world; }
-  std::vector<bnamd_material> materials;  // the world's material table
+  // the world's material table; empty: the device's
+  std::vector<bnamd_material> materials;
   const std::vector<Group *> groups;
   const std::vector<int> instIDs;
   std::vector<affine3f> xfms;
@@ -164,6 +283,7 @@ struct Device {
 
   TextureData *createTextureData(vec3i dims, DataType format, const void *texels);
   void freeTextureData(TextureData *);
+  void freeTexture(Texture *);
 
   Buffer *createBuffer(size_t numBytes, const void *initValues = nullptr);
   void freeBuffer(Buffer *);
@@ -180,7 +300,15 @@ struct Device {
   Denoiser *createDenoiser() { return nullptr; }
 
   const int physicalID;
+  // the material table of this device's groups and worlds (the
+  // bnamd_material view of barney's per-device material registry); bottom
+  // levels are built with it (it marks glass triangles for shadow rays)
+  std::vector<bnamd_material> materials;
 };
+
+// The acceleration handle barney's launch parameters carry
+// (rtcore/cuda/Device.h:88): the instance group's world.
+AccelHandle getAccelHandle(Group *instanceGroup);
 
 // the geometry types barney imports (RTC_IMPORT_*_GEOM)
 GeomType *createGeomType_Triangles(Device *device);

@@ -114,6 +114,7 @@ typedef struct {
   float buildMs;
   float sceneLower[3];
   float sceneUpper[3];
+  int numBlas; /* distinct bottom levels of a two-level world (0: flattened) */
 } bnamd_world_info;
 
 int bnamd_world_build(const bnamd_group *groups, int numGroups,
@@ -122,6 +123,28 @@ int bnamd_world_build(const bnamd_group *groups, int numGroups,
                       void *stream, bnamd_world **out);
 int bnamd_world_destroy(bnamd_world *world);
 int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info);
+
+/* Bottom levels built on their own: rtc::GeomGroup::buildAccel of a
+ * triangles or user-geometry group (rtcore/cuda/cudaGroup.cu:261-562). A
+ * BLAS keeps the group's geometry pointers (the caller's buffers). The
+ * material table decides which triangles shadow rays pass (Glass); a world
+ * using the BLAS must agree with it on that (checked). */
+typedef struct bnamd_blas_s bnamd_blas;
+int bnamd_blas_build(const bnamd_group *group, const bnamd_material *materials, int numMaterials,
+                     void *stream, bnamd_blas **out);
+int bnamd_blas_destroy(bnamd_blas *blas);
+/* rtc::InstanceGroup::buildAccel over built groups (cudaGroup.cu:144-258):
+ * instances[i].group indexes 'blas'; every instance of one BLAS shares it.
+ * The world references the BLASes, which must outlive it. */
+int bnamd_world_build_instances(const bnamd_blas *const *blas, int numBlas,
+                                const bnamd_instance *instances, int numInstances,
+                                const bnamd_material *materials, int numMaterials,
+                                void *stream, bnamd_world **out);
+/* rtc::InstanceGroup::setTransforms + refitAccel: new object-to-world
+ * transforms (12 floats each, bnamd_instance.xfm layout) for every instance
+ * of a two-level world; rebuilds the instance level only. */
+int bnamd_world_set_transforms(bnamd_world *world, const float *xfms, int numInstances,
+                               void *stream);
 
 /* ------------------------------------------------------------------------
  * Trace: closest hit for a batch of rays (rtc::TraceInterface::traceRay

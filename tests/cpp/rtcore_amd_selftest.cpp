@@ -2,7 +2,9 @@
 // librtcore_amd.so / libbnamd.so): device memory and pinned host memory,
 // the active-device guard, Triangles / Spheres / Cylinders geometry types,
 // groups, an instance group, closest-hit tracing with analytic answers,
-// and error reporting as std::runtime_error.
+// setDD with barney's DD bytes, one bottom level shared by 1,000 instances
+// and a top-level refit, getAccelHandle, texture data and textures, and
+// error reporting as std::runtime_error.
 // Exit codes: 0 = all checks passed, 1 = a check failed, 3 = no usable GPU.
 #include <cmath>
 #include <cstdio>
@@ -165,6 +167,121 @@ int main() {
     threw = std::strstr(e.what(), "singular") != nullptr;
   }
   CHECK(threw);
+
+  // setDD: the bytes barney's Spheres::commit / Cylinders::commit /
+  // Triangles::commit write (Geometry::DD head, then the geometry's fields)
+  // give the same geometry as the typed setters
+  {
+    using namespace barney_dd;
+    auto ddBytes = [](int userID, int materialID, const void *fields, size_t n) {
+      std::vector<uint8_t> b(kGeometryDDSize + n, 0);
+      Geometry g;
+      std::memset(&g, 0, sizeof(g));
+      g.userID = userID;
+      g.materialID = materialID;
+      std::memcpy(b.data(), &g, kGeometryDDSize);
+      std::memcpy(b.data() + kGeometryDDSize, fields, n);
+      return b;
+    };
+    SpheresFields sf{(const float *)sv->getDD(), (const float *)sr->getDD(), nullptr, 1.f};
+    CylindersFields cf{(const float *)cv->getDD(), (const int *)ci->getDD(), (const float *)cr->getDD()};
+    TrianglesFields tf{(const int *)qi->getDD(), (const float *)qv->getDD(), nullptr, nullptr};
+    Geom *sph2 = sphT->createGeom();
+    sph2->setPrimCount(1);
+    const auto sb = ddBytes(41, 0, &sf, sizeof(sf));
+    sph2->setDD(sb.data());
+    Geom *cyl2 = cylT->createGeom();
+    cyl2->setPrimCount(1);
+    const auto cb = ddBytes(42, 0, &cf, sizeof(cf));
+    cyl2->setDD(cb.data());
+    Geom *quad2 = triT->createGeom();
+    quad2->setVertices(qv, 4);
+    quad2->setIndices(qi, 2);
+    const auto tb = ddBytes(43, 0, &tf, sizeof(tf));
+    quad2->setDD(tb.data());
+    const bnamd_geom d1 = sph2->desc(), d2 = cyl2->desc(), d3 = quad2->desc();
+    CHECK(d1.userID == 41 && d1.vertices == sf.origins && d1.radii == sf.radii && d1.numPrims == 1);
+    CHECK(d2.userID == 42 && d2.vertices == cf.vertices && d2.indices == cf.indices && d2.radii == cf.radii);
+    CHECK(d3.userID == 43 && d3.vertices == tf.vertices && d3.indices == tf.indices && d3.numPrims == 2);
+    Group *tg2 = dev->createTrianglesGroup({quad2});
+    Group *ug2 = dev->createUserGeomsGroup({sph2, cyl2});
+    auto *ig2 = (InstanceGroup *)dev->createInstanceGroup({tg2, ug2}, {7, 8}, {affine3f(), affine3f()});
+    ig2->buildAccel();
+    ig->setTransforms({affine3f(), affine3f()});
+    ig->buildAccel();
+    std::vector<bnamd_hit> h1(rays.size()), h2(rays.size());
+    ig->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, (int)rays.size());
+    dev->copy(h1.data(), dHits, h1.size() * sizeof(bnamd_hit));
+    ig2->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, (int)rays.size());
+    dev->copy(h2.data(), dHits, h2.size() * sizeof(bnamd_hit));
+    CHECK(std::memcmp(h1.data(), h2.data(), h1.size() * sizeof(bnamd_hit)) == 0);
+    CHECK((void *)getAccelHandle(ig2) == (void *)ig2->world);
+    dev->freeGroup(ig2);
+    dev->freeGroup(tg2);
+    dev->freeGroup(ug2);
+    for (Geom *g : {sph2, cyl2, quad2}) dev->freeGeom(g);
+  }
+  // 1,000 instances of one group share one bottom level; moving them
+  // (setTransforms + refitAccel) rebuilds only the instance level
+  {
+    const int n = 1000;
+    std::vector<Group *> gs(n, ug);
+    std::vector<int> ids(n);
+    std::vector<affine3f> xs(n);
+    for (int i = 0; i < n; ++i) {
+      ids[i] = 1000 + i;
+      xs[i].p = {0.f, 0.f, 10.f * (float)i};  // instance i's sphere at (3, 0, 5 + 10 i)
+    }
+    auto *many = (InstanceGroup *)dev->createInstanceGroup(gs, ids, xs);
+    many->buildAccel();
+    std::printf("1000 instances: %d bottom level(s)\n", many->numBottomLevels());
+    CHECK(many->numBottomLevels() == 1);
+    // a ray along +z through the sphere column at x = 3 meets instance 0
+    bnamd_ray r{};
+    r.org[0] = 3.f;
+    r.org[2] = -10.f;
+    r.dir[2] = 1.f;
+    r.tmax = 1e30f;
+    dev->copy(dRays, &r, sizeof(r));
+    bnamd_hit h{};
+    many->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, 1);
+    dev->copy(&h, dHits, sizeof(h));
+    CHECK(h.instID == 1000 && near(h.t, 14.f, 1e-5f));
+    // drop the first 500 instances far away: the ray now meets instance 500
+    for (int i = 0; i < 500; ++i) xs[i].p.y = 1000.f;
+    many->setTransforms(xs);
+    many->refitAccel();
+    many->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, 1);
+    dev->copy(&h, dHits, sizeof(h));
+    std::printf("after refit: inst=%d t=%g\n", h.instID, h.t);
+    CHECK(h.instID == 1500 && near(h.t, 5014.f, 1e-5f));
+    CHECK(many->numBottomLevels() == 1);
+    dev->freeGroup(many);
+  }
+  // texture data and textures: a 3D scalar field sampled with CLAMP and
+  // unnormalized coordinates (StructuredData's description)
+  {
+    const vec3i dims = {4, 3, 2};
+    std::vector<float> texels(24);
+    for (int i = 0; i < 24; ++i) texels[i] = 0.5f * (float)i;
+    TextureData *td = dev->createTextureData(dims, FLOAT, texels.data());
+    TextureDesc desc;
+    desc.normalizedCoords = false;
+    Texture *tex = td->createTexture(desc);
+    CHECK((void *)tex->getDD() == td->d_texels && td->numBytes == 24 * sizeof(float));
+    std::vector<float> back(24, -1.f);
+    dev->copy(back.data(), (const void *)tex->getDD(), 24 * sizeof(float));
+    CHECK(back == texels);
+    threw = false;
+    try {
+      delete td->createTexture(TextureDesc());  // normalized coordinates on a 3D field
+    } catch (const std::runtime_error &) {
+      threw = true;
+    }
+    CHECK(threw);
+    dev->freeTexture(tex);
+    dev->freeTextureData(td);
+  }
 
   dev->freeMem(dRays);
   dev->freeMem(dHits);
