@@ -80,12 +80,14 @@ class Geom:
     """Host description of one geometry (triangles / spheres / cylinders)."""
 
     def __init__(self, kind, vertices, indices=None, radii=None, colors=None, default_radius=1.0,
-                 material=0, user_id=0):
+                 material=0, user_id=0, normals=None):
+        """normals: triangles only, one float3 per vertex (shading normals)."""
         self.kind = kind
         self.vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         self.indices = None if indices is None else np.ascontiguousarray(indices, dtype=np.int32)
         self.radii = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
         self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 3)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         self.default_radius = float(default_radius)
         self.material = int(material)
         self.user_id = int(user_id)
@@ -99,11 +101,11 @@ class Geom:
         return len(self.vertices)
 
 
-def matte(color):
+def matte(color, opacity=1.0):
     m = capi.bnamd_material()
     m.type = MAT_MATTE
     m.color[:] = [float(c) for c in color]
-    m.opacity = 1.0
+    m.opacity = float(opacity)
     return m
 
 
@@ -121,7 +123,8 @@ class World:
                 di = DeviceArray(g.indices) if g.indices is not None else None
                 dr = DeviceArray(g.radii) if g.radii is not None else None
                 dc = DeviceArray(g.colors) if g.colors is not None else None
-                self._keep += [x for x in (dv, di, dr, dc) if x is not None]
+                dn = DeviceArray(g.normals) if getattr(g, "normals", None) is not None else None
+                self._keep += [x for x in (dv, di, dr, dc, dn) if x is not None]
                 e = arr[k]
                 e.kind, e.numPrims, e.numVertices = g.kind, g.num_prims, len(g.vertices)
                 e.materialID, e.userID = g.material, g.user_id
@@ -129,6 +132,7 @@ class World:
                 e.indices = di.addr if di else None
                 e.radii = dr.addr if dr else None
                 e.colors = dc.addr if dc else None
+                e.normals = dn.addr if dn else None
                 e.defaultRadius = g.default_radius
             cgroups[gi].geoms = C.cast(arr, C.POINTER(capi.bnamd_geom))
             cgroups[gi].numGeoms = len(geoms)
@@ -190,13 +194,100 @@ def camera(position, direction, up=(0, 1, 0), fovy=60.0):
 
 
 def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
-                  max_diffuse=1, russian_roulette=0):
+                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None):
+    """env: an EnvMap (HDRI environment light) or None (uniform ambient);
+    cut_plane: (nx, ny, nz, d) or None (disabled)."""
     p = capi.bnamd_render_params()
     p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
     p.ambientRadiance = float(ambient)
     p.bgColor[:] = [float(x) for x in bg]
     p.maxPathDepth, p.maxDiffuseBounces, p.russianRoulette = max_depth, max_diffuse, russian_roulette
+    p.envMap = env.handle.value if env is not None else None
+    p._env = env  # keeps the map alive as long as the parameters
+    p.cutPlane[:] = [float(x) for x in (cut_plane if cut_plane is not None else (0.0, 0.0, 0.0, -1e30))]
     return p
+
+
+TEXEL_FORMATS = {"float4": 33, "float": 30, "uchar4": 1, "uchar": 0}
+ADDRESS_MODES = {"wrap": 0, "clamp": 1, "border": 2, "mirror": 3}
+
+
+class Texture2D:
+    """A 2D texture (rtc::TextureData + rtc::Texture for images): texels
+    (H, W, 4) / (H, W) float32 or uint8; filter 'linear' or 'point';
+    address modes per axis ('wrap', 'clamp', 'border', 'mirror')."""
+
+    def __init__(self, texels, filter="linear", address=("wrap", "clamp")):
+        lib = init()
+        t = np.ascontiguousarray(texels)
+        if t.dtype != np.uint8:
+            t = np.ascontiguousarray(t, np.float32)
+        self.height, self.width = t.shape[:2]
+        d = capi.bnamd_texture2d_desc()
+        d.width, d.height = self.width, self.height
+        d.format = TEXEL_FORMATS[("uchar" if t.dtype == np.uint8 else "float") + ("4" if t.ndim == 3 else "")]
+        d.texels = t.ctypes.data
+        d.filterMode = 1 if filter == "linear" else 0
+        d.addressMode[:] = [ADDRESS_MODES[a] for a in address]
+        self.handle = C.c_void_p()
+        check(lib.bnamd_texture2d_create(C.byref(d), None, C.byref(self.handle)))
+
+    def sample(self, uv):
+        uv = DeviceArray(np.ascontiguousarray(uv, np.float32).reshape(-1, 2))
+        out = DeviceArray.empty((uv.shape[0], 4), np.float32)
+        check(capi.load().bnamd_texture2d_sample(self.handle, uv.addr, out.addr, uv.shape[0], None))
+        return out.numpy()
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            capi.load().bnamd_texture2d_destroy(self.handle)
+            self.handle = None
+
+
+class EnvMap:
+    """The HDRI environment light (EnvMapLight) over a Texture2D."""
+
+    def __init__(self, texture, direction=(1, 0, 0), up=(0, 0, 1), scale=1.0):
+        lib = init()
+        self.texture = texture  # the map reads the texture's texels
+        d = capi.bnamd_envmap_desc()
+        d.direction[:] = [float(x) for x in direction]
+        d.up[:] = [float(x) for x in up]
+        d.scale = float(scale)
+        self.handle = C.c_void_p()
+        check(lib.bnamd_envmap_create(texture.handle, C.byref(d), None, C.byref(self.handle)))
+
+    def cdfs(self):
+        h, w = self.texture.height, self.texture.width
+        y = np.empty(h, np.float32)
+        x = np.empty((h, w), np.float32)
+        check(capi.load().bnamd_envmap_get_cdfs(self.handle, y.ctypes.data, x.ctypes.data))
+        return y, x
+
+    def frame(self):
+        w = np.empty(9, np.float32)
+        l = np.empty(9, np.float32)
+        check(capi.load().bnamd_envmap_get_frame(self.handle, w.ctypes.data, l.ctypes.data))
+        return w.reshape(3, 3), l.reshape(3, 3)
+
+    def eval(self, dirs):
+        d = DeviceArray(np.ascontiguousarray(dirs, np.float32).reshape(-1, 3))
+        n = d.shape[0]
+        pdf = DeviceArray.empty((n,), np.float32)
+        rad = DeviceArray.empty((n, 3), np.float32)
+        check(capi.load().bnamd_envmap_eval(self.handle, d.addr, pdf.addr, rad.addr, n, None))
+        return pdf.numpy(), rad.numpy()
+
+    def sample(self, n, seed=0):
+        out = DeviceArray.empty((n, 7), np.float32)
+        check(capi.load().bnamd_envmap_sample(self.handle, seed, out.addr, n, None))
+        o = out.numpy()
+        return o[:, :3], o[:, 3:6], o[:, 6]
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            capi.load().bnamd_envmap_destroy(self.handle)
+            self.handle = None
 
 
 def render_surface(world, cam, params, with_ids=False, accum=None):
@@ -262,11 +353,11 @@ def batch_autotune(world, views, params, path=None):
                                            None if path is None else path.encode()))
 
 
-def pbr(color, metallic=0.0, roughness=0.5, transmission=0.0, ior=1.45):
+def pbr(color, metallic=0.0, roughness=0.5, transmission=0.0, ior=1.45, opacity=1.0):
     m = capi.bnamd_material()
     m.type = MAT_PBR
     m.color[:] = [float(c) for c in color]
-    m.opacity = 1.0
+    m.opacity = float(opacity)
     m.metallic, m.roughness, m.transmission, m.ior, m.specular = metallic, roughness, transmission, ior, 0.5
     return m
 

@@ -35,6 +35,7 @@ SOURCES = [
     "render_batch.hip",
     "umesh.hip",
     "fb.hip",
+    "envmap.hip",
 ]
 
 # -packed-fp32-ops: packed FP32 forms are kept out of the generated code.

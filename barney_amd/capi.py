@@ -15,7 +15,7 @@ class bnamd_geom(C.Structure):
     _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("numVertices", C.c_int),
                 ("materialID", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
                 ("radii", C.c_void_p), ("colors", C.c_void_p), ("defaultRadius", C.c_float),
-                ("userID", C.c_int)]
+                ("userID", C.c_int), ("normals", C.c_void_p)]
 
 
 class bnamd_group(C.Structure):
@@ -47,7 +47,17 @@ class bnamd_render_params(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int),
                 ("pathsPerPixel", C.c_int), ("ambientRadiance", C.c_float),
                 ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
-                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int)]
+                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int), ("envMap", C.c_void_p),
+                ("cutPlane", C.c_float * 4)]
+
+
+class bnamd_texture2d_desc(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("format", C.c_int), ("texels", C.c_void_p),
+                ("filterMode", C.c_int), ("addressMode", C.c_int * 2)]
+
+
+class bnamd_envmap_desc(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("up", C.c_float * 3), ("scale", C.c_float)]
 
 
 class bnamd_render_stats(C.Structure):
@@ -182,6 +192,16 @@ SIGNATURES = {
                                          C.c_void_p]),
     "bnamd_fb_resolve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p, C.c_void_p]),
     "bnamd_fb_to_fixed8": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "bnamd_texture2d_create": (C.c_int, [C.POINTER(bnamd_texture2d_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bnamd_texture2d_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_texture2d_sample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_envmap_create": (C.c_int, [C.c_void_p, C.POINTER(bnamd_envmap_desc), C.c_void_p,
+                                      C.POINTER(C.c_void_p)]),
+    "bnamd_envmap_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_envmap_get_cdfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_envmap_get_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_envmap_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_envmap_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 _lib = None

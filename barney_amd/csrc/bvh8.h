@@ -16,6 +16,10 @@ enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2
 // triangle anyHit ignores it for shadow rays (Glass::getOpacity returns 0
 // for shadow rays, Triangles.dev.cu:125-133).
 constexpr uint32_t PRIM_SHADOW_TRANSPARENT = 1u << 8;
+// PRIM_CUTOUT: a triangle whose material's opacity alpha (b.w of its record)
+// is below 1; barney's triangle anyHit ignores a candidate hit when a random
+// draw exceeds alpha (Triangles.dev.cu:121-133).
+constexpr uint32_t PRIM_CUTOUT = 1u << 9;
 
 // One node = one 128-byte line. Child boxes are quantized to 16 bits per
 // plane relative to the node origin with a power-of-two scale per axis.
@@ -40,7 +44,7 @@ struct alignas(128) BNode8 {
 static_assert(sizeof(BNode8) == 128, "BNode8 must be one 128B line");
 
 // Primitive record (48 B, three float4) stored in BVH leaf order.
-//   triangle: a = v0 | flat,  b = e1 = v1-v0 | -, c = e2 = v2-v0 | type
+//   triangle: a = v0 | flat,  b = e1 = v1-v0 | alpha, c = e2 = v2-v0 | type
 //   sphere:   a = center | flat, b = (radius,0,0) | -, c = 0 | type
 //   cylinder: a = v0 | flat, b = v1 | -, c = (radius,0,0) | type
 //   tet:      a = box lo | flat, b = box hi | -, c = 0 | type (point-location
@@ -66,6 +70,54 @@ BN_HD bool better(float t, uint32_t flat, const HitRec &best) {
 }
 
 // ---------------------------------------------------------------------------
+// Candidate filters: what barney's anyHit / intersection programs do with a
+// candidate before it is reported. NoFilt accepts every candidate.
+// HitFilter applies the renderer's cut plane (OptixGlobals.h:33-42: the
+// candidate's world-space point P = o + t d is dropped when
+// dot(n, P) + d < 0; Triangles.dev.cu:39-43, Spheres.dev.cu:170-174,
+// Cylinders.dev.cu:257-259) and, for PRIM_CUTOUT triangles, stochastic
+// opacity (Triangles.dev.cu:121-133: the ray seed advanced by the bits of
+// the object-space point, then Random(seed, 290374) > alpha drops it).
+// Each candidate's draw depends on the ray's seed at the start of the trace
+// and on that candidate only (barney chains the seed through the
+// candidates in traversal order), so the closest accepted hit does not
+// depend on the order in which the BVH is walked.
+// ---------------------------------------------------------------------------
+struct NoFilt {
+  BN_HD bool any(float) const { return true; }
+  BN_HD bool tri(float, float, float, vec3f, vec3f, vec3f, float, uint32_t) const { return true; }
+};
+
+struct HitFilter {
+  vec3f o, d;       // world-space ray as traced (the cut plane is a world-space test)
+  float4 cut;       // (nx, ny, nz, d)
+  uint64_t seed;    // the ray's RNG state when the trace starts
+  bool cutOn;
+  bool stochastic;  // false: opacity ignored (the ID channel's closest candidate)
+  BN_HD bool any(float t) const {
+#pragma clang fp contract(off)
+    if (!cutOn) return true;
+    const vec3f P = o + t * d;
+    return !(dot(vec3f(cut.x, cut.y, cut.z), P) + cut.w < 0.f);
+  }
+  BN_HD bool tri(float t, float u, float v, vec3f v0, vec3f e1, vec3f e2, float alpha, uint32_t bits) const {
+#pragma clang fp contract(off)
+    if (!any(t)) return false;
+    if (stochastic && (bits & PRIM_CUTOUT)) {
+      const vec3f P = v0 + (u * e1 + v * e2);
+      RNGSeed s;
+      s.state = seed;
+      s.next_hash(float_as_uint(P.x));
+      s.next_hash(float_as_uint(P.y));
+      s.next_hash(float_as_uint(P.z));
+      Random2 rng(s, 290374ull);
+      if (rng() > alpha) return false;
+    }
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Primitive intersectors. Each computes the candidate t exactly as the
 // reference program does (same fp32 operation order, no FMA contraction),
 // tested against the ray's original [tmin, tmax]; acceptance against the
@@ -74,8 +126,10 @@ BN_HD bool better(float t, uint32_t flat, const HitRec &best) {
 
 // rtcore/cuda/TraceInterface.h:32-98 (barney's own plane + 2D-projection
 // triangle test).
+template <class Filt = NoFilt>
 BN_HD void isect_triangle(vec3f org, vec3f dir, float tmax, vec3f v0, vec3f e1, vec3f e2,
-                          uint32_t flat, HitRec &best) {
+                          uint32_t flat, HitRec &best, const Filt &filt = Filt(), float alpha = 1.f,
+                          uint32_t bits = 0u) {
 #pragma clang fp contract(off)
   const vec3f N = cross(e1, e2);
   const float dd = dot(dir, N);
@@ -102,6 +156,7 @@ BN_HD void isect_triangle(vec3f org, vec3f dir, float tmax, vec3f v0, vec3f e1, 
   const float v = (e1u * Pv - e1v * Pu) / den;
   if ((u < 0.f) || (v < 0.f) || ((u + v) >= 1.f)) return;
   if (!better(t, flat, best)) return;
+  if (!filt.tri(t, u, v, v0, e1, e2, alpha, bits)) return;
   best.t = t;
   best.u = u;
   best.v = v;
@@ -111,8 +166,9 @@ BN_HD void isect_triangle(vec3f org, vec3f dir, float tmax, vec3f v0, vec3f e1, 
 // barney/geometry/Spheres.dev.cu:116-170 ("move the origin" trick).
 // Returns the shifted-frame hit in best.u so the closest-hit epilogue can
 // rebuild the object-space point bit-identically.
+template <class Filt = NoFilt>
 BN_HD void isect_sphere(vec3f org, vec3f dir, float tmin, float tmax, vec3f center, float radius,
-                        uint32_t flat, HitRec &best) {
+                        uint32_t flat, HitRec &best, const Filt &filt = Filt()) {
 #pragma clang fp contract(off)
   const float t_move = fmaxf(0.f, length(center - org) - 3.f * radius);
   const vec3f o = org + t_move * dir;
@@ -138,6 +194,7 @@ BN_HD void isect_sphere(vec3f org, vec3f dir, float tmin, float tmax, vec3f cent
   if (!(hit_t < t_max)) return;
   const float t = hit_t + t_move;
   if (!better(t, flat, best)) return;
+  if (!filt.any(t)) return;
   best.t = t;
   best.u = hit_t;
   best.v = t_move;
@@ -212,11 +269,13 @@ BN_HD bool cylinder_hit(vec3f ray_org, vec3f ray_dir, float ray_tmin, float ray_
   return true;
 }
 
+template <class Filt = NoFilt>
 BN_HD void isect_cylinder(vec3f org, vec3f dir, float tmin, float tmax, vec3f v0, vec3f v1,
-                          float radius, uint32_t flat, HitRec &best) {
+                          float radius, uint32_t flat, HitRec &best, const Filt &filt = Filt()) {
   float t;
   if (!cylinder_hit(org, dir, tmin, tmax, v0, v1, radius, t, nullptr)) return;
   if (!better(t, flat, best)) return;
+  if (!filt.any(t)) return;
   best.t = t;
   best.u = 0.f;
   best.v = 0.f;
@@ -235,34 +294,37 @@ BN_HD vec3f fix_dir(vec3f d) {
 
 // keyBase: high bits of the tie-break key (the instance index in a
 // two-level world, 0 otherwise), OR-ed onto the record's flat index.
+template <class Filt = NoFilt>
 BN_D void test_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, vec3f dir,
-                    float tmin, float tmax, HitRec &best, uint32_t keyBase = 0u) {
+                    float tmin, float tmax, HitRec &best, uint32_t keyBase = 0u, const Filt &filt = Filt()) {
   const float4 a = prims[idx].a;
   const float4 b = prims[idx].b;
   const float4 c = prims[idx].c;
-  const uint32_t type = __float_as_uint(c.w) & 0xffu;
+  const uint32_t bits = __float_as_uint(c.w);
+  const uint32_t type = bits & 0xffu;
   const uint32_t flat = keyBase | __float_as_uint(a.w);
   const vec3f p0 = {a.x, a.y, a.z};
   if (type == PRIM_TRIANGLE) {
-    isect_triangle(org, dir, tmax, p0, {b.x, b.y, b.z}, {c.x, c.y, c.z}, flat, best);
+    isect_triangle(org, dir, tmax, p0, {b.x, b.y, b.z}, {c.x, c.y, c.z}, flat, best, filt, b.w, bits);
   } else if (type == PRIM_SPHERE) {
-    isect_sphere(org, dir, tmin, tmax, p0, b.x, flat, best);
+    isect_sphere(org, dir, tmin, tmax, p0, b.x, flat, best, filt);
   } else {
-    isect_cylinder(org, dir, tmin, tmax, p0, {b.x, b.y, b.z}, c.x, flat, best);
+    isect_cylinder(org, dir, tmin, tmax, p0, {b.x, b.y, b.z}, c.x, flat, best, filt);
   }
 }
 
 // Occlusion test of one primitive for shadow rays: any accepted hit in the
 // ray's range (shadow-transparent primitives skipped).
+template <class Filt = NoFilt>
 BN_D bool occludes_prim(const PrimRec *__restrict prims, uint32_t idx, vec3f org, vec3f dir, float tmin,
-                        float tmax) {
+                        float tmax, const Filt &filt = Filt()) {
   const float4 c = prims[idx].c;
   const uint32_t bits = __float_as_uint(c.w);
   if (bits & PRIM_SHADOW_TRANSPARENT) return false;
   HitRec h;
   h.t = tmax;
   h.flat = 0xffffffffu;
-  test_prim(prims, idx, org, dir, tmin, tmax, h);
+  test_prim(prims, idx, org, dir, tmin, tmax, h, 0u, filt);
   return h.flat != 0xffffffffu;
 }
 
@@ -361,10 +423,10 @@ BN_D uint32_t ray_octant(vec3f dir) {
 // rays; returns true when occluded). Otherwise: closest hit into 'best'
 // (returns false only when the stack would overflow, which the builder's
 // reported depth rules out).
-template <bool kAny>
+template <bool kAny, class Filt = NoFilt>
 BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
                         float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, uint32_t keyBase,
-                        TraceCount *tc = nullptr) {
+                        TraceCount *tc = nullptr, const Filt &filt = Filt()) {
   const vec3f inv = rcp(dir);
   const uint32_t oct = ray_octant(dir);
   uint32_t grpBase = 0;
@@ -391,9 +453,9 @@ BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restric
       primMask &= primMask - 1u;
       if (tc) tc->prim(prims, primBase + i);
       if (kAny) {
-        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) return true;
+        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax, filt)) return true;
       } else {
-        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, keyBase);
+        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, keyBase, filt);
       }
     }
     if ((grpBits & 0xffu) == 0) {
@@ -413,7 +475,7 @@ BN_D bool traverse_from(const BNode8 *__restrict nodes, const PrimRec *__restric
 // of the wavefront keeps walking. Closest-hit acceptance (better()) is a
 // total order on (t, flat index), so the hit found does not depend on the
 // order in which rays or nodes are visited.
-template <bool kAny>
+template <bool kAny, class Filt = NoFilt>
 struct BvhWalk {
   vec3f org, dir, inv;
   uint32_t oct;
@@ -423,6 +485,7 @@ struct BvhWalk {
   int sp;
   bool occluded;  // kAny: an occluder was found (or the stack overflowed)
   TraceCount count;  // over every ray this walk has taken (init keeps it)
+  Filt filt;         // the candidate filter of the current ray
 
   BN_D void init(vec3f o, vec3f d, float t0, float t1) {
     org = o;
@@ -466,12 +529,12 @@ struct BvhWalk {
       primMask &= primMask - 1u;
       count.prim(prims, primBase + i);
       if (kAny) {
-        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax)) {
+        if (occludes_prim(prims, primBase + i, org, dir, tmin, tmax, filt)) {
           occluded = true;
           return true;
         }
       } else {
-        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, 0u);
+        test_prim(prims, primBase + i, org, dir, tmin, tmax, best, 0u, filt);
       }
     }
     if ((grpBits & 0xffu) == 0) {
@@ -486,13 +549,14 @@ struct BvhWalk {
 };
 
 // Closest-hit (or occlusion) traversal of one BVH8 from scratch.
-template <bool kAny>
+template <bool kAny, class Filt = NoFilt>
 BN_D bool traverse(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,
-                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, TraceCount *tc = nullptr) {
+                   float tmin, float tmax, HitRec &best, const TraverseCtx &ctx, TraceCount *tc = nullptr,
+                   const Filt &filt = Filt()) {
   best.t = tmax;
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
-  return traverse_from<kAny>(nodes, prims, org, dir, tmin, tmax, best, ctx, 0u, tc);
+  return traverse_from<kAny>(nodes, prims, org, dir, tmin, tmax, best, ctx, 0u, tc, filt);
 }
 
 BN_D bool traverse_closest(const BNode8 *__restrict nodes, const PrimRec *__restrict prims, vec3f org, vec3f dir,

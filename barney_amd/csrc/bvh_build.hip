@@ -39,11 +39,13 @@ __global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, const Ma
     const vec3f v1 = load3(gd.vertices + 3 * ix[1]);
     const vec3f v2 = load3(gd.vertices + 3 * ix[2]);
     const vec3f e1 = v1 - v0, e2 = v2 - v0;
-    r.a = make_float4(v0.x, v0.y, v0.z, __uint_as_float(i));
-    r.b = make_float4(e1.x, e1.y, e1.z, 0.f);
     const MaterialDev &m = mats[gd.materialID];
     const bool glass = m.type == BNAMD_MAT_PBR && m.ior != 1.f && m.transmission >= 1e-3f;
-    r.c = make_float4(e2.x, e2.y, e2.z, __uint_as_float(PRIM_TRIANGLE | (glass ? PRIM_SHADOW_TRANSPARENT : 0u)));
+    const bool cutout = !glass && m.alpha < 1.f;
+    r.a = make_float4(v0.x, v0.y, v0.z, __uint_as_float(i));
+    r.b = make_float4(e1.x, e1.y, e1.z, cutout ? m.alpha : 1.f);
+    r.c = make_float4(e2.x, e2.y, e2.z, __uint_as_float(PRIM_TRIANGLE | (glass ? PRIM_SHADOW_TRANSPARENT : 0u) |
+                                                       (cutout ? PRIM_CUTOUT : 0u)));
     lo = vmin(v0, vmin(v1, v2));
     hi = vmax(v0, vmax(v1, v2));
   } else if (gd.kind == BNAMD_GEOM_SPHERES) {

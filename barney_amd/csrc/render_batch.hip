@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 namespace bnamd {
 
@@ -134,10 +135,10 @@ __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long lon
 // Megakernel: L lanes per pixel (each looping over its share of the paths),
 // 4 wavefronts per workgroup covering a TW x TH pixel tile of one view.
 // ===========================================================================
-template <int L, bool TL>
+template <int L, bool TL, bool kEnv, bool kHit>
 __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, RenderDD rp, int spp,
                                                     float4 *__restrict accum, int4 *__restrict ids,
-                                                    unsigned long long *__restrict counters) {
+                                                    unsigned long long *__restrict counters, bool cutouts) {
   extern __shared__ uint2 lds_stack[];
   constexpr int PW = 64 / L, PB = 4 * PW;
   constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
@@ -174,26 +175,40 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
       const float clampMax = 10.f * (1 + rpp.accumID);
       vec3f sum = vec3f(0.f);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
+        if (kHit)
+          world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc,
+                                    hit_filter(rpp, ray.org, dir, ray.rng.state));
+        else
+          world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
         const ShadeHit sh = shade_hit<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) {
           alpha = sh.hit ? 1.f : ray.missAlpha;
-          if (ids && rpp.accumID == 0)
-            ids[vd.pixOffset[view] + ix + iy * sz.x] = hit_ids<TL>(w, ray.org, dir, ray.tMax, h);
+          if (ids && rpp.accumID == 0) {
+            HitRec hid = h;  // closest candidate whatever its opacity
+            if (kHit && cutouts)
+              world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, hid, ctx, nullptr,
+                                        hit_filter(rpp, ray.org, dir, ray.rng.state, false));
+            ids[vd.pixOffset[view] + ix + iy * sz.x] = hit_ids<TL>(w, ray.org, dir, ray.tMax, hid);
+          }
         }
         PathRay shadow;
-        add_frag(sum, bounce(rpp, ray, sh, shadow, gen), clampMax);
+        add_frag(sum, bounce<kEnv>(rpp, ray, sh, shadow, gen), clampMax);
         if (shadow.tMax > 0.f) {
           ShadeHit shs;
           HitRec hs;
-          shs.hit = world_traverse<TL, true>(w, shadow.org, fix_dir(shadow.dir), 0.f, shadow.tMax, hs, ctx, &tc);
+          const vec3f sdir = fix_dir(shadow.dir);
+          if (kHit)
+            shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc,
+                                               hit_filter(rpp, shadow.org, sdir, shadow.rng.state));
+          else
+            shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
           ++nshadow;
           PathRay dummy;
-          add_frag(sum, bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          add_frag(sum, bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1), clampMax);
         }
       }
       v = make_float4(sum.x, sum.y, sum.z, alpha);
@@ -229,6 +244,7 @@ struct ShadowRec {
   float4 a;  // org, tMax
   float4 b;  // dir, path id
   float4 c;  // contribution (mis * throughput), 0
+  uint4 d;   // rng lo, rng hi (the seed of its anyHit draws), 0, 0
 };
 
 // device counters (one slot per bounce depth)
@@ -266,7 +282,8 @@ struct WfDD {
   int chunk;      // rays per dynamic work fetch (one atomic per fetch);
                   // 0: kWalkChunk in k_wf_walk, static 64-ray slices in the
                   // two-level passes
-  int pad;        // explicit tail: the graph key reads every byte
+  int cutouts;    // the world has cutout triangles (explicit tail: the graph
+                  // key reads every byte)
 };
 
 __device__ __forceinline__ unsigned sort_segment(unsigned n) {
@@ -335,7 +352,7 @@ __global__ void k_wf_generate(WfDD e) {
   if (pid == 0) e.cnt->count[0] = total;
 }
 
-template <bool TL>
+template <bool TL, bool kHit>
 __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->count[depth];
@@ -348,13 +365,44 @@ __global__ void __launch_bounds__(256) k_wf_trace(WfDD e, int depth) {
     const vec3f org = {a.x, a.y, a.z};
     const vec3f dir = fix_dir({b.x, b.y, b.z});
     HitRec h;
-    world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx, &tc);
+    if (kHit) {
+      const uint4 d = q[i].d;
+      world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx, &tc,
+                                hit_filter(e.rp, org, dir, (uint64_t)d.x | ((uint64_t)d.y << 32)));
+    } else {
+      world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx, &tc);
+    }
     e.hits[i] = h;
     ++nr;
   });
   wave_add(nr, &e.cnt->rays);
   wave_add(tc.nodes, &e.cnt->nodeVisits);
   wave_add(tc.primBytes, &e.cnt->primBytes);
+}
+
+// The ID channel of a world with cutout triangles: the closest candidate of
+// each primary ray whatever its opacity (Triangles.dev.cu:45-60 writes the
+// IDs before the opacity test), for the paths with accumID 0.
+template <bool TL>
+__global__ void __launch_bounds__(256) k_wf_ids(WfDD e) {
+  extern __shared__ uint2 lds_stack[];
+  const unsigned n = e.cnt->count[0];
+  const PathRec *__restrict q = e.q[0];
+  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, e.w.depth + 1};
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned pid = __float_as_uint(q[i].c.w);
+    int sidx, view, ix, iy, pix;
+    path_pixel(e, pid, sidx, view, ix, iy, pix);
+    if (e.accum0 + sidx != 0) continue;
+    const float4 a = q[i].a, b = q[i].b;
+    const uint4 d = q[i].d;
+    const vec3f org = {a.x, a.y, a.z};
+    const vec3f dir = fix_dir({b.x, b.y, b.z});
+    HitRec h;
+    world_traverse<TL, false>(e.w, org, dir, 0.f, a.w, h, ctx, nullptr,
+                              hit_filter(e.rp, org, dir, (uint64_t)d.x | ((uint64_t)d.y << 32), false));
+    e.ids[pix] = hit_ids<TL>(e.w, org, dir, a.w, h);
+  }
 }
 
 template <bool TL>
@@ -472,7 +520,7 @@ __device__ __forceinline__ void accum_add(float4 *accum, int pix, vec3f f, float
   if (f.z > 0.f) atomicAdd(p + 2, f.z);
 }
 
-template <bool TL>
+template <bool TL, bool kEnv>
 __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   const unsigned n = e.cnt->count[depth];
   const PathRec *__restrict qin = e.q[depth & 1];
@@ -512,9 +560,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
       if (depth == 0) {
         const float alpha = sh.hit ? 1.f : ray.missAlpha;
         if (alpha > 0.f) atomicAdd(((float *)(e.accum + pix)) + 3, alpha);
-        if (e.ids && rpp.accumID == 0) e.ids[pix] = hit_ids<TL>(e.w, ray.org, dir, ray.tMax, h);
+        // with cutout triangles k_wf_ids wrote the closest candidate instead
+        if (e.ids && rpp.accumID == 0 && !e.cutouts) e.ids[pix] = hit_ids<TL>(e.w, ray.org, dir, ray.tMax, h);
       }
-      const vec3f frag = bounce(rpp, ray, sh, shadow, depth);
+      const vec3f frag = bounce<kEnv>(rpp, ray, sh, shadow, depth);
       accum_add(e.accum, pix, frag, clampMax);
       wantShadow = shadow.tMax > 0.f;
       wantCont = ray.tMax > 0.f;
@@ -527,6 +576,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
       srec.a = make_float4(shadow.org.x, shadow.org.y, shadow.org.z, shadow.tMax);
       srec.b = make_float4(shadow.dir.x, shadow.dir.y, shadow.dir.z, __uint_as_float(pid));
       srec.c = make_float4(c.x, c.y, c.z, 0.f);
+      srec.d = make_uint4((uint32_t)shadow.rng.state, (uint32_t)(shadow.rng.state >> 32), 0u, 0u);
       e.sq[si] = srec;
     }
     if (wantCont) {
@@ -541,7 +591,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   }
 }
 
-template <bool TL>
+template <bool TL, bool kHit>
 __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = e.cnt->shadow[depth];
@@ -553,7 +603,10 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
     const vec3f org = {s.a.x, s.a.y, s.a.z};
     const vec3f dir = fix_dir({s.b.x, s.b.y, s.b.z});
     HitRec hs;
-    const bool occluded = world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx, &tc);
+    const bool occluded =
+        kHit ? world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx, &tc,
+                                        hit_filter(e.rp, org, dir, (uint64_t)s.d.x | ((uint64_t)s.d.y << 32)))
+             : world_traverse<TL, true>(e.w, org, dir, 0.f, s.a.w, hs, ctx, &tc);
     ++nr;
     if (!occluded) {
       const unsigned pid = __float_as_uint(s.b.w);
@@ -580,7 +633,7 @@ __global__ void __launch_bounds__(256) k_wf_shadow(WfDD e, int depth) {
 constexpr unsigned kWalkChunk = 128;
 constexpr int kRefillLanes = 16;
 
-template <bool kAny>
+template <bool kAny, bool kHit>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_wf_walk(WfDD e, int depth) {
   extern __shared__ uint2 lds_stack[];
   const unsigned n = kAny ? e.cnt->shadow[depth] : e.cnt->count[depth];
@@ -590,7 +643,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned chunk = e.chunk > 0 ? (unsigned)e.chunk : kWalkChunk;
-  BvhWalk<kAny> wk;
+  using Filt = typename std::conditional<kHit, HitFilter, NoFilt>::type;
+  BvhWalk<kAny, Filt> wk;
   unsigned rid = ~0u;             // this lane's ray (~0u: none)
   bool dry = false;               // the queue is used up (wavefront-uniform)
   unsigned cNext = 0, cEnd = 0;   // rest of the wavefront's chunk
@@ -630,6 +684,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
           b = q[rid].b;
         }
         wk.init({a.x, a.y, a.z}, fix_dir({b.x, b.y, b.z}), 0.f, a.w);
+        if constexpr (kHit) {
+          const uint4 d = kAny ? e.sq[rid].d : q[rid].d;
+          wk.filt = hit_filter(e.rp, wk.org, wk.dir, (uint64_t)d.x | ((uint64_t)d.y << 32));
+        }
       }
     }
     if (__ballot(rid != ~0u) == 0ull) {
@@ -713,11 +771,11 @@ int num_cus() {
 int trace_grid(int blocksPerCU, size_t lds, bool twoLevel) {
   int occT = 0, occS = 0;
   if (twoLevel) {
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<true>, 256, lds));
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<true>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_trace<true, false>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_shadow<true, false>, 256, lds));
   } else {
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_walk<false>, 256, lds));
-    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_walk<true>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_wf_walk<false, false>, 256, lds));
+    BN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_wf_walk<true, false>, 256, lds));
   }
   int occ = std::max(1, std::min(occT, occS));
   if (blocksPerCU > 0) occ = std::min(occ, blocksPerCU);
@@ -780,10 +838,19 @@ void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
   const int tiles = hv.tileOffset.back();
   const int grid = ((tiles + 7) / 8) * 8;
   const size_t lds = trace_lds_bytes(w.depth, 256);
-  if (w.twoLevel)
-    k_batch_mega<L, true><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt);
-  else
-    k_batch_mega<L, false><<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt);
+  auto go = [&](auto kernel) {
+    kernel<<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt, w.hasCutouts);
+  };
+  switch ((w.twoLevel ? 4 : 0) | (has_env(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0)) {
+    case 0: go(k_batch_mega<L, false, false, false>); break;
+    case 1: go(k_batch_mega<L, false, false, true>); break;
+    case 2: go(k_batch_mega<L, false, true, false>); break;
+    case 3: go(k_batch_mega<L, false, true, true>); break;
+    case 4: go(k_batch_mega<L, true, false, false>); break;
+    case 5: go(k_batch_mega<L, true, false, true>); break;
+    case 6: go(k_batch_mega<L, true, true, false>); break;
+    default: go(k_batch_mega<L, true, true, true>); break;
+  }
   BN_HIP(hipGetLastError());
 }
 
@@ -796,27 +863,37 @@ void render_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
 }
 
 // Records the kernels of one batch (generate + maxDepth+1 bounces) on s.
-template <bool TL>
+// kHit: a cut plane or cutout triangles (the traces filter candidates).
+template <bool TL, bool kHit>
 void enqueue_wavefront_t(const WfDD &e, int maxDepth, int gridPersist, size_t lds, hipStream_t s) {
   const unsigned total = (unsigned)e.numPixels * (unsigned)e.sppBatch;
   k_wf_reset<<<1, 256, 0, s>>>(e.cnt);
   k_wf_generate<<<div_up(total, 256), 256, 0, s>>>(e);
   const int gridStream = std::min(div_up(total, 256), 8 * num_cus());
   for (int d = 0; d <= maxDepth; ++d) {
-    if (TL) k_wf_trace<TL><<<gridPersist, 256, lds, s>>>(e, d);
-    else k_wf_walk<false><<<gridPersist, 256, lds, s>>>(e, d);
+    if (TL) k_wf_trace<TL, kHit><<<gridPersist, 256, lds, s>>>(e, d);
+    else k_wf_walk<false, kHit><<<gridPersist, 256, lds, s>>>(e, d);
+    if (d == 0 && kHit && e.cutouts && e.ids && e.accum0 == 0) k_wf_ids<TL><<<gridStream, 256, lds, s>>>(e);
     k_wf_sort_count<TL><<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_sort_scan<<<1, kSortBlocks, 0, s>>>(e);
     k_wf_sort_scatter<<<kSortBlocks, 256, 0, s>>>(e, d);
-    k_wf_shade<TL><<<gridStream, 256, 0, s>>>(e, d);
-    if (TL) k_wf_shadow<TL><<<gridPersist, 256, lds, s>>>(e, d);
-    else k_wf_walk<true><<<gridPersist, 256, lds, s>>>(e, d);
+    if (has_env(e.rp))
+      k_wf_shade<TL, true><<<gridStream, 256, 0, s>>>(e, d);
+    else
+      k_wf_shade<TL, false><<<gridStream, 256, 0, s>>>(e, d);
+    if (TL) k_wf_shadow<TL, kHit><<<gridPersist, 256, lds, s>>>(e, d);
+    else k_wf_walk<true, kHit><<<gridPersist, 256, lds, s>>>(e, d);
   }
 }
 
 void enqueue_wavefront(const WfDD &e, int maxDepth, int gridPersist, size_t lds, bool twoLevel, hipStream_t s) {
-  if (twoLevel) enqueue_wavefront_t<true>(e, maxDepth, gridPersist, lds, s);
-  else enqueue_wavefront_t<false>(e, maxDepth, gridPersist, lds, s);
+  const bool hit = e.rp.cutOn || e.cutouts;
+  if (twoLevel)
+    hit ? enqueue_wavefront_t<true, true>(e, maxDepth, gridPersist, lds, s)
+        : enqueue_wavefront_t<true, false>(e, maxDepth, gridPersist, lds, s);
+  else
+    hit ? enqueue_wavefront_t<false, true>(e, maxDepth, gridPersist, lds, s)
+        : enqueue_wavefront_t<false, false>(e, maxDepth, gridPersist, lds, s);
 }
 
 float render_wavefront(const World &w, const bnamd_views *views, const RenderDD &rp0, int spp, float4 *accum,
@@ -872,6 +949,7 @@ float render_wavefront(const World &w, const bnamd_views *views, const RenderDD 
     e.ids = ids;
     e.cnt = st.cnt.ptr;
     e.chunk = tune.chunk;
+    e.cutouts = w.hasCutouts ? 1 : 0;
     if (useGraph) {
       // The graph bakes every kernel argument (the whole WfDD by value: world
       // pointers and sizes, render parameters such as ambient and background,

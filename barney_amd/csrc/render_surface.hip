@@ -51,10 +51,13 @@ __device__ __forceinline__ void add_fragment(vec3f &sum, vec3f frag, float clamp
   if (frag.z > 0.f) sum.z += frag.z;
 }
 
-template <bool TL>
+// kEnv: an HDRI map lights the scene; kHit: a cut plane or cutout
+// triangles, so candidate hits pass the anyHit filter (kernels without
+// either keep the plain traversal).
+template <bool TL, bool kEnv, bool kHit>
 __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam, RenderDD rp, int spp,
                                                         float4 *__restrict accum, int4 *__restrict ids,
-                                                        unsigned long long *__restrict counters) {
+                                                        unsigned long long *__restrict counters, bool cutouts) {
   extern __shared__ uint2 lds_stack[];
   const int tilesX = (rp.width + 15) >> 4, tilesY = (rp.height + 15) >> 4;
   const int tile = xcd_remap(blockIdx.x, tilesX * tilesY);
@@ -77,27 +80,45 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
       const float clampMax = 10.f * (1 + rpp.accumID);
       vec3f sum = vec3f(0.f);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      // a primary ray clipped away by the cut plane (tMax = 0) is still
+      // traced and shows the background
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         HitRec h;
-        world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
+        if (kHit)
+          world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc,
+                                    hit_filter(rpp, ray.org, dir, ray.rng.state));
+        else
+          world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
         const ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
         if (gen == 0) {
           alpha = sh.hit ? 1.f : ray.missAlpha;
-          if (ids && p == 0 && accum0 == 0) ids[pix] = hit_ids<TL>(w, ray.org, dir, ray.tMax, h);
+          if (ids && p == 0 && accum0 == 0) {
+            // the ID channel records the closest candidate whatever its
+            // opacity (Triangles.dev.cu:45-60 writes it before the test)
+            HitRec hid = h;
+            if (kHit && cutouts)
+              world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, hid, ctx, nullptr,
+                                        hit_filter(rpp, ray.org, dir, ray.rng.state, false));
+            ids[pix] = hit_ids<TL>(w, ray.org, dir, ray.tMax, hid);
+          }
         }
         PathRay shadow;
-        vec3f frag = bounce(rpp, ray, sh, shadow, gen);
+        vec3f frag = bounce<kEnv>(rpp, ray, sh, shadow, gen);
         add_fragment(sum, frag, clampMax);
         if (shadow.tMax > 0.f) {
           const vec3f sdir = fix_dir(shadow.dir);
           ShadeHit shs;
           HitRec hs;
-          shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
+          if (kHit)
+            shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc,
+                                               hit_filter(rpp, shadow.org, sdir, shadow.rng.state));
+          else
+            shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
           ++nrays;
           PathRay dummy;
-          frag = bounce(rpp, shadow, shs, dummy, gen + 1);
+          frag = bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1);
           add_fragment(sum, frag, clampMax);
         }
       }
@@ -157,12 +178,21 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
     unsigned long long n[3] = {0, 0, 0};
     const float ms = render_on_stream(s, w.counters, 3, stats != nullptr, n, [&] {
       unsigned long long *c = w.counters.ptr;
-      if (w.twoLevel)
-        k_render_surface<true><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                                         (int4 *)ids, c);
-      else
-        k_render_surface<false><<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum,
-                                                          (int4 *)ids, c);
+      auto go = [&](auto kernel) {
+        kernel<<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, c,
+                                        w.hasCutouts);
+      };
+      const int variant = (w.twoLevel ? 4 : 0) | (has_env(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0);
+      switch (variant) {
+        case 0: go(k_render_surface<false, false, false>); break;
+        case 1: go(k_render_surface<false, false, true>); break;
+        case 2: go(k_render_surface<false, true, false>); break;
+        case 3: go(k_render_surface<false, true, true>); break;
+        case 4: go(k_render_surface<true, false, false>); break;
+        case 5: go(k_render_surface<true, false, true>); break;
+        case 6: go(k_render_surface<true, true, false>); break;
+        default: go(k_render_surface<true, true, true>); break;
+      }
     });
     if (stats) {
       stats->raysTraced = (long long)n[0];

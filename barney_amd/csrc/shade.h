@@ -3,12 +3,15 @@
 //   camera           barney/Camera.cu:81-100
 //   ray generation   barney/kernels/generateRays.cu:73-129, 199-210
 //   bounce()         barney/kernels/shadeRays.cu:476-878
-//   env light        shadeRays.cu:267-293 (sampleEnvLight), 295-404 (sampleLights)
+//   env light        shadeRays.cu:267-293 (sampleEnvLight), 295-404 (sampleLights),
+//                    437-470 (radianceFromEnv, primaryRayMissColor); the
+//                    HDRI map in envmap.h
 //   BSDFs            packedBSDF/Lambertian.h:55-87, Phase.h:120-172
 //   materials        material/AnariMatte.h:51-63
 //   accumulation     shadeRays.cu:970-1038
 #pragma once
 #include <hip/hip_fp16.h>
+#include "envmap.h"
 #include "fmath.h"
 #include "world.h"
 
@@ -40,6 +43,9 @@ struct RenderDD {
   float ambient;
   float bg[4];
   int maxPathDepth, maxDiffuseBounces, rrDepth;
+  int cutOn;        // cut plane enabled
+  float4 cut;       // Renderer "cutPlane" (nx, ny, nz, d)
+  EnvDD env;        // env.tex.texels == nullptr: uniform ambient radiance
 };
 
 inline RenderDD make_render_dd(const bnamd_render_params &p) {
@@ -52,7 +58,32 @@ inline RenderDD make_render_dd(const bnamd_render_params &p) {
   r.maxPathDepth = p.maxPathDepth > 0 ? p.maxPathDepth : 10;
   r.maxDiffuseBounces = p.maxDiffuseBounces >= 0 ? p.maxDiffuseBounces : 1;
   r.rrDepth = p.russianRoulette;
+  r.cut = make_float4(p.cutPlane[0], p.cutPlane[1], p.cutPlane[2], p.cutPlane[3]);
+  // disabled when d <= -1e28 (Renderer.h:59); an all-zero plane rejects
+  // nothing either (0 < 0 never holds), so it is skipped the same way
+  r.cutOn = (p.cutPlane[3] > -1e28f) &&
+            !(p.cutPlane[0] == 0.f && p.cutPlane[1] == 0.f && p.cutPlane[2] == 0.f && p.cutPlane[3] == 0.f);
+  if (p.envMap)
+    r.env = p.envMap->dd;
+  else
+    r.env = EnvDD{};
   return r;
+}
+
+BN_HD bool has_env(const RenderDD &rp) { return rp.env.tex.texels != nullptr; }
+
+// The candidate filter of a surface trace: the renderer's cut plane and
+// (stochastic = true) the stochastic opacity of cutout triangles, drawn
+// from the ray's seed at the start of the trace.
+BN_HD HitFilter hit_filter(const RenderDD &rp, vec3f org, vec3f dir, uint64_t seed, bool stochastic = true) {
+  HitFilter f;
+  f.o = org;
+  f.d = dir;
+  f.cut = rp.cut;
+  f.cutOn = rp.cutOn != 0;
+  f.seed = seed;
+  f.stochastic = stochastic;
+  return f;
 }
 
 enum BsdfType : int { BSDF_NONE = 0, BSDF_LAMBERT = 1, BSDF_PHASE = 2, BSDF_NVISII = 3, BSDF_GLASS = 4 };
@@ -127,6 +158,21 @@ BN_HD PathRay camera_ray(const CameraDD &cam, const RenderDD &rp, int ix, int iy
   ray.org = cam.org;
   ray.dir = normalize(d);
   ray.tMax = 1e30f;
+  if (rp.cutOn) {
+    // the cutting plane (generateRays.cu:171-193): clip where the ray
+    // leaves the visible half-space, or start it where it enters; a ray
+    // that never enters it keeps tMax = 0 and shows the background
+    const vec3f N = vec3f(rp.cut.x, rp.cut.y, rp.cut.z);
+    const float distOrg = dot(N, ray.org) + rp.cut.w;
+    const float denom = dot(N, ray.dir);
+    if (distOrg >= 0.f) {
+      if (denom < 0.f) ray.tMax = fminf(ray.tMax, -distOrg / denom);
+    } else if (denom > 0.f) {
+      ray.org = ray.org + (-distOrg / denom) * ray.dir;
+    } else {
+      ray.tMax = 0.f;
+    }
+  }
   ray.tp = vec3f(1.f);
   ray.mis = 0.f;
   ray.numDiffuse = 0;
@@ -507,9 +553,13 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedi
   return s;
 }
 
-// shadeRays.cu:476-878 for env/ambient lighting. Returns the fragment this
+// shadeRays.cu:476-878 for the environment light (HDRI map or uniform
+// ambient radiance). Returns the fragment this
 // ray adds to its pixel. On return ray.tMax > 0 means the path continues with
 // the updated ray; shadow.tMax > 0 means a shadow ray was spawned.
+// kEnv: rp carries an HDRI map (kernels are instantiated for both cases so
+// that the map's code costs the ambient-light kernels no registers).
+template <bool kEnv>
 BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay &shadow, int pathDepth) {
 #pragma clang fp contract(off)
   const float EPS = 1e-4f;
@@ -527,10 +577,11 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   const bool hitWasOnFront = dot(ray.dir, Ng) < 0.f;
   const vec3f Ngff = hitWasOnFront ? Ng : -Ng;
   if (!h.hit) {
+    // primaryRayMissColor / radianceFromEnv (shadeRays.cu:437-470, 556-575)
     if (pathDepth == 0)
-      fragment = ray.missColor;
+      fragment = kEnv ? env_radiance(rp.env, ray.dir) : ray.missColor;
     else
-      fragment = ray.tp * vec3f(rp.ambient) * ray.mis;
+      fragment = ray.tp * (kEnv ? env_radiance(rp.env, ray.dir) : vec3f(rp.ambient)) * ray.mis;
     ray.tMax = -1.f;
     return fragment;
   }
@@ -541,30 +592,41 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   const float offsetEps = fmaxf(EPS, 1e-6f * reduce_max(vabs(P)));
   const vec3f ffOffset = isVolumeHit ? wo : Ngff;
 
-  // ---- sampleLights(): only the environment light is present ----
-  vec3f lsDir = random_direction(random);
+  // ---- sampleEnvLight (shadeRays.cu:267-293): the map's importance
+  // sample, or randomDirection into the normal's hemisphere ----
+  vec3f lsDir;
   vec3f lsRad;
   float lsPdf;
-  if (!(Ngff == vec3f(0.f))) {
-    lsRad = vec3f(BN_TWO_PI * rp.ambient);
-    if (dot(lsDir, Ngff) < 0.f) lsDir = -lsDir;
-    lsPdf = BN_ONE_OVER_TWO_PI;
+  if (kEnv) {
+    env_sample(rp.env, random, lsDir, lsRad, lsPdf);
   } else {
-    lsRad = vec3f(BN_FOUR_PI * rp.ambient);
-    lsPdf = BN_ONE_OVER_FOUR_PI;
+    lsDir = random_direction(random);
+    if (!(Ngff == vec3f(0.f))) {
+      lsRad = vec3f(BN_TWO_PI * rp.ambient);
+      if (dot(lsDir, Ngff) < 0.f) lsDir = -lsDir;
+      lsPdf = BN_ONE_OVER_TWO_PI;
+    } else {
+      lsRad = vec3f(BN_FOUR_PI * rp.ambient);
+      lsPdf = BN_ONE_OVER_FOUR_PI;
+    }
   }
+  // ---- sampleLights (shadeRays.cu:295-404): the area, point and
+  // directional lights are absent (weight 0, pdf 0); a draw at or below the
+  // area weight or above the environment's share picks no light ----
   bool haveLight = false;
   bool lightNeedsMIS = false;
   {
-    float elsWeight = reduce_max(lsRad);
-    const float sumWeights = elsWeight;
+    const float elsWeight = reduce_max(lsRad);
+    const float sumWeights = ((0.f + 0.f) + elsWeight) + 0.f;
     if (sumWeights != 0.f) {
-      elsWeight *= 1.f / sumWeights;
+      const float alsW = 0.f * (1.f / sumWeights);
+      const float elsW = elsWeight * (1.f / sumWeights);
       const float r = random();
-      (void)r;  // r <= 0 selects the (absent) area light in barney; env otherwise
-      lsRad *= (1.f / elsWeight);
-      lightNeedsMIS = true;
-      haveLight = !(lsPdf <= 0.f) && !(lsPdf != lsPdf);
+      if (!(r <= alsW) && r <= alsW + elsW) {
+        lsRad *= (1.f / elsW);
+        lightNeedsMIS = true;
+        haveLight = !(lsPdf <= 0.f) && !(lsPdf != lsPdf);
+      }
     }
   }
   if (haveLight) {
@@ -586,7 +648,8 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
       shadow.numDiffuse = ray.numDiffuse;
       shadow.isInMedium = ray.isInMedium;
       if (lightNeedsMIS) {
-        const float pl = BN_ONE_OVER_FOUR_PI;  // EnvMapLight::DD::pdf without a texture
+        // EnvMapLight::DD::pdf (1/4pi without a texture)
+        const float pl = kEnv ? env_pdf(rp.env, lsDir) : BN_ONE_OVER_FOUR_PI;
         const float pb = bsdf_pdf(bsdf, Ng, lsDir, wo);
         shadow.mis = pl / (pl + pb + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tMax = -1.f;
@@ -621,7 +684,7 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   if (sr.changedMedium) ray.isInMedium = !ray.isInMedium;
   if (lightNeedsMIS && !is_inf(sr.pdf)) {
     const float pb = bsdf_pdf(bsdf, Ng, ray.dir, wo);
-    const float pl = BN_ONE_OVER_FOUR_PI;
+    const float pl = kEnv ? env_pdf(rp.env, ray.dir) : BN_ONE_OVER_FOUR_PI;
     ray.mis = pb / (pb + pl);
   } else {
     ray.mis = 1.f;

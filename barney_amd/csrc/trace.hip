@@ -98,6 +98,7 @@ GeomDev make_geom(const bnamd_geom &src, int geomID, int instIndex, int instID, 
   g.indices = src.indices;
   g.radii = src.radii;
   g.colors = src.colors;
+  g.normals = src.kind == BNAMD_GEOM_TRIANGLES ? src.normals : nullptr;
   g.defaultRadius = src.defaultRadius;
   return g;
 }
@@ -112,17 +113,22 @@ std::vector<MaterialDev> make_materials(const bnamd_material *materials, int num
       md.color[0] = s.color[0]; md.color[1] = s.color[1]; md.color[2] = s.color[2];
       md.opacity = s.opacity; md.metallic = s.metallic; md.roughness = s.roughness;
       md.transmission = s.transmission; md.ior = s.ior; md.specular = s.specular;
+      md.alpha = 1.f * s.opacity;
       if (s.type == BNAMD_MAT_PBR && !(s.ior != 1.f && s.transmission >= 1e-3f)) {
         // NVisii stores its parameters as halfs (NVisii.h:1062-1083);
         // roughness is clamped to [.001, .9] (AnariPBR.h:86)
         md.metallic = round_half(s.metallic);
         md.roughness = round_half(std::min(std::max(s.roughness, .001f), .9f));
         md.ior = round_half(s.ior);
+        md.alpha = round_half(((1.f - s.transmission) * 1.f) * s.opacity);
+      } else if (s.type == BNAMD_MAT_PBR) {
+        md.alpha = 1.f;  // Glass
       }
     } else {
       md.type = BNAMD_MAT_MATTE;
       md.color[0] = md.color[1] = md.color[2] = 0.8f;
       md.opacity = 1.f;
+      md.alpha = 1.f;
     }
     mats[m] = md;
   }
@@ -179,6 +185,19 @@ void xfm_box(const float *m, const float *lo, const float *hi, float4 &blo, floa
 // material table is part of the build: a triangle primitive records whether
 // its material makes it transparent to shadow rays (Glass), and the BVH
 // build reads the table for that.
+bool shadow_transparent(const MaterialDev &m) {
+  return m.type == BNAMD_MAT_PBR && m.ior != 1.f && m.transmission >= 1e-3f;
+}
+
+// Whether some triangle's material makes its hits stochastic (alpha < 1,
+// not glass): the BVH records carry PRIM_CUTOUT then.
+bool has_cutouts(const std::vector<GeomDev> &geoms, const std::vector<MaterialDev> &mats) {
+  for (const GeomDev &g : geoms)
+    if (g.kind == BNAMD_GEOM_TRIANGLES && !shadow_transparent(mats[g.materialID]) && mats[g.materialID].alpha < 1.f)
+      return true;
+  return false;
+}
+
 std::unique_ptr<World> build_blas(const bnamd_group &grp, const std::vector<MaterialDev> &mats, int numMaterials,
                                   hipStream_t s) {
   auto b = std::make_unique<World>();
@@ -195,11 +214,8 @@ std::unique_ptr<World> build_blas(const bnamd_group &grp, const std::vector<Mate
   b->materials.upload(mats.data(), mats.size(), s);
   build_world_bvh(*b, s);
   b->hmats = mats;
+  b->hasCutouts = has_cutouts(b->hgeoms, mats);
   return b;
-}
-
-bool shadow_transparent(const MaterialDev &m) {
-  return m.type == BNAMD_MAT_PBR && m.ior != 1.f && m.transmission >= 1e-3f;
 }
 
 // A bottom level built with one material table used in a world with
@@ -210,10 +226,14 @@ void check_materials(const World &b, const std::vector<MaterialDev> &mats, int n
   for (const GeomDev &g : b.hgeoms) {
     BN_REQUIRE(g.materialID >= 0 && g.materialID < std::max(numMaterials, 1),
                "bnamd_world_build: materialID out of range");
-    if (g.kind == BNAMD_GEOM_TRIANGLES)
+    if (g.kind == BNAMD_GEOM_TRIANGLES) {
       BN_REQUIRE(shadow_transparent(b.hmats[g.materialID]) == shadow_transparent(mats[g.materialID]),
                  "bnamd_world_build_instances: a triangle geom's material is glass in one material table and "
                  "not in the other (its BLAS was built with a different table)");
+      BN_REQUIRE(b.hmats[g.materialID].alpha == mats[g.materialID].alpha,
+                 "bnamd_world_build_instances: a triangle geom's material opacity differs from the table its "
+                 "BLAS was built with");
+    }
   }
 }
 
@@ -226,7 +246,9 @@ void assemble_tlas(World &w, const float *xfms, hipStream_t s) {
   int maxDepth = 1;
   uint64_t total = 0, nodes = 0;
   std::vector<const World *> distinct;
+  w.hasCutouts = false;
   for (const World *b : w.instBlas) {
+    w.hasCutouts = w.hasCutouts || b->hasCutouts;
     maxPrims = std::max(maxPrims, b->numPrims);
     maxDepth = std::max(maxDepth, b->depth);
     total += b->numPrims;
@@ -341,6 +363,7 @@ extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const
         w.numPrims = (uint32_t)offset;
         w.materials.upload(mats.data(), mats.size(), s);
         build_world_bvh(w, s);
+        w.hasCutouts = has_cutouts(w.hgeoms, mats);
         w.totalPrims = w.numPrims;
         w.totalNodes = w.numNodes;
       }

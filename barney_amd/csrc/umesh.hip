@@ -413,7 +413,7 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
 // at six wavefronts and 8 lanes per pixel, against 232 ms at the three
 // that the unconstrained allocation (168 VGPRs) leaves with 4 lanes; five
 // and eight wavefronts measured 156 and 159 ms.
-template <int L, bool kVol, bool kIso, bool kMixed>
+template <int L, bool kVol, bool kIso, bool kMixed, bool kEnv>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
                unsigned long long *__restrict counters) {
@@ -446,13 +446,13 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
       const float clampMax = 10.f * (1 + rpp.accumID);
       vec3f sum = vec3f(0.f);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const vec3f dir = fix_dir(ray.dir);
         const ShadeHit sh = umesh_trace<kVol, kIso, kMixed>(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt, hint);
         ++nrays;
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
         PathRay shadow;
-        vec3f frag = vmin(bounce(rpp, ray, sh, shadow, gen), clampMax);
+        vec3f frag = vmin(bounce<kEnv>(rpp, ray, sh, shadow, gen), clampMax);
         if (frag.x > 0.f) sum.x += frag.x;
         if (frag.y > 0.f) sum.y += frag.y;
         if (frag.z > 0.f) sum.z += frag.z;
@@ -462,7 +462,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
                                               hint);
           ++nrays;
           PathRay dummy;
-          frag = vmin(bounce(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          frag = vmin(bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1), clampMax);
           if (frag.x > 0.f) sum.x += frag.x;
           if (frag.y > 0.f) sum.y += frag.y;
           if (frag.z > 0.f) sum.z += frag.z;
@@ -496,14 +496,16 @@ void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &c
   constexpr int TH = PB / TW;
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   const int grid = ((tiles + 7) / 8) * 8;
+  auto go = [&](auto kernel) { kernel<<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters); };
+  const bool env = has_env(rp);
   if (o.volume && o.iso)
-    k_render_umesh<L, true, true, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    env ? go(k_render_umesh<L, true, true, kMixed, true>) : go(k_render_umesh<L, true, true, kMixed, false>);
   else if (o.volume)
-    k_render_umesh<L, true, false, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    env ? go(k_render_umesh<L, true, false, kMixed, true>) : go(k_render_umesh<L, true, false, kMixed, false>);
   else if (o.iso)
-    k_render_umesh<L, false, true, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    env ? go(k_render_umesh<L, false, true, kMixed, true>) : go(k_render_umesh<L, false, true, kMixed, false>);
   else
-    k_render_umesh<L, false, false, kMixed><<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters);
+    env ? go(k_render_umesh<L, false, false, kMixed, true>) : go(k_render_umesh<L, false, false, kMixed, false>);
 }
 
 void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi, float base) {

@@ -31,6 +31,7 @@ struct GeomDev {
   const int *indices;
   const float *radii;
   const float *colors;
+  const float *normals;  // triangles: per-vertex normals or nullptr
   float defaultRadius;
   int pad;
 };
@@ -39,6 +40,12 @@ struct MaterialDev {
   int type;
   float color[3];
   float opacity, metallic, roughness, transmission, ior, specular;
+  // the BSDF's alpha that barney's triangle anyHit tests (getOpacity):
+  // Lambertian baseColor.w * opacity (AnariMatte.h:63), NVisii
+  // (1 - transmission) * baseColor.w * opacity stored as a half
+  // (AnariPBR.h:88-91, NVisii.h:1080); Glass is handled by
+  // PRIM_SHADOW_TRANSPARENT instead (1 here)
+  float alpha;
 };
 
 // One instance of a two-level world: affine transforms as 12 floats
@@ -99,6 +106,7 @@ struct World {
   std::vector<const World *> instBlas;       // each instance's bottom level (owned or the caller's)
   std::vector<int> instIDs;
   int numBlas = 0;                           // distinct bottom levels referenced
+  bool hasCutouts = false;                   // some triangle's material has alpha < 1
   mutable DevBuf<unsigned long long> counters;  // the surface render's counters
   DevBuf<InstDev> insts;
   DevBuf<float4> instBoxes;
@@ -148,7 +156,8 @@ struct SurfaceHit {
 
 // Rebuilds hit position and normal in the space of the geometry table
 // 'geoms' for the accepted primitive with group-local flat index 'flat',
-// following Triangles.dev.cu:57-74 (geometric normal, interpolated P),
+// following Triangles.dev.cu:57-75 (geometric or interpolated vertex
+// normal, interpolated P),
 // Spheres.dev.cu:57-90 (normal P - center; the object-space point is the
 // shifted-frame hit) and Cylinders.dev.cu:208-216 (normal from intersect).
 BN_D SurfaceHit object_epilogue(const GeomDev *__restrict geoms, int numGeoms, vec3f org, vec3f dir, float tmin,
@@ -167,7 +176,13 @@ BN_D SurfaceHit object_epilogue(const GeomDev *__restrict geoms, int numGeoms, v
     const vec3f v0 = load3(gd.vertices + 3 * ix[0]);
     const vec3f v1 = load3(gd.vertices + 3 * ix[1]);
     const vec3f v2 = load3(gd.vertices + 3 * ix[2]);
-    s.Ng = normalize(cross(v1 - v0, v2 - v0));
+    // n = normalize(cross), or the normalized interpolated vertex normal;
+    // the object normal is normalize(n) (Triangles.dev.cu:65-75)
+    vec3f n = normalize(cross(v1 - v0, v2 - v0));
+    if (gd.normals)
+      n = normalize((1.f - h.u - h.v) * load3(gd.normals + 3 * ix[0]) + h.u * load3(gd.normals + 3 * ix[1]) +
+                    h.v * load3(gd.normals + 3 * ix[2]));
+    s.Ng = normalize(n);
     s.P = (1.f - h.u - h.v) * v0 + h.u * v1 + h.v * v2;
   } else if (gd.kind == BNAMD_GEOM_SPHERES) {
     const vec3f c = load3(gd.vertices + 3 * primID);
@@ -242,10 +257,10 @@ BN_D const GeomDev &hit_geom(const WorldDev &w, uint32_t flat) {
 // the ray goes to object space (W2O; the direction is not renormalized, so
 // t is the same parameter in both spaces) and its group's BVH continues
 // from the best hit so far, with tie keys prefixed by the instance index.
-template <bool TL, bool kAny>
+template <bool TL, bool kAny, class Filt = NoFilt>
 BN_D bool world_traverse(const WorldDev &w, vec3f org, vec3f dir, float tmin, float tmax, HitRec &best,
-                         const TraverseCtx &ctx, TraceCount *tc = nullptr) {
-  if (!TL) return traverse<kAny>(w.nodes, w.prims, org, dir, tmin, tmax, best, ctx, tc);
+                         const TraverseCtx &ctx, TraceCount *tc = nullptr, const Filt &filt = Filt()) {
+  if (!TL) return traverse<kAny>(w.nodes, w.prims, org, dir, tmin, tmax, best, ctx, tc, filt);
   best.t = tmax;
   best.flat = 0xffffffffu;
   best.u = best.v = 0.f;
@@ -283,9 +298,9 @@ BN_D bool world_traverse(const WorldDev &w, vec3f org, vec3f dir, float tmin, fl
         HitRec tmp;
         tmp.t = tmax;
         tmp.flat = 0xffffffffu;
-        if (traverse_from<true>(I.nodes, I.prims, o, d, tmin, tmax, tmp, sub, 0u, tc)) return true;
+        if (traverse_from<true>(I.nodes, I.prims, o, d, tmin, tmax, tmp, sub, 0u, tc, filt)) return true;
       } else {
-        traverse_from<false>(I.nodes, I.prims, o, d, tmin, tmax, best, sub, inst << w.localBits, tc);
+        traverse_from<false>(I.nodes, I.prims, o, d, tmin, tmax, best, sub, inst << w.localBits, tc, filt);
       }
     }
     if ((grpBits & 0xffu) == 0) {

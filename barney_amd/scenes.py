@@ -303,9 +303,10 @@ def product_materials(api, mats):
     out = []
     for m in mats:
         if m["type"] == "matte":
-            out.append(api.matte(m["color"]))
+            out.append(api.matte(m["color"], opacity=m.get("opacity", 1.0)))
         elif m["type"] == "pbr":
-            out.append(api.pbr(m["color"], metallic=m.get("metallic", 0.0), roughness=m["roughness"]))
+            out.append(api.pbr(m["color"], metallic=m.get("metallic", 0.0), roughness=m["roughness"],
+                               transmission=m.get("transmission", 0.0), opacity=m.get("opacity", 1.0)))
         else:
             out.append(api.pbr((1.0, 1.0, 1.0), transmission=1.0, ior=m["ior"]))
     return out

@@ -77,6 +77,8 @@ typedef struct {
   const float *colors;   /* device float3[numPrims] per-primitive color or NULL */
   float defaultRadius;
   int userID;            /* 'objID' written to the ID channel */
+  const float *normals;  /* triangles: device float3[numVertices] vertex normals, interpolated at
+                          * hits as the shading normal (Triangles.dev.cu:70-74), or NULL */
 } bnamd_geom;
 
 typedef struct {
@@ -192,7 +194,59 @@ typedef struct {
   int maxPathDepth;     /* barney: 10 */
   int maxDiffuseBounces;/* barney: 1 */
   int russianRoulette;  /* 0: off (barney); >0: start depth for RR */
+  /* HDRI environment light (barney's EnvMapLight, light/EnvMap.cu) or
+   * NULL: the uniform ambientRadiance (World::DD without an env map) */
+  const struct bnamd_envmap_s *envMap;
+  /* Renderer "cutPlane" (render/Renderer.h:59, generateRays.cu:171-193,
+   * OptixGlobals.h:33-42): (nx, ny, nz, d); surface hits with
+   * dot(n, P) + d < 0 are ignored and primary rays are clipped to the
+   * visible half-space. Disabled when d <= -1e28 (or all four are 0). */
+  float cutPlane[4];
 } bnamd_render_params;
+
+/* ------------------------------------------------------------------------
+ * 2D textures and the HDRI environment light.
+ * bnamd_texture2d replaces rtc::TextureData / rtc::Texture for 2D images
+ * (rtcore/cudaCommon/TextureData.h:15-31, Texture.h:12-24); it is sampled
+ * as the embree backend's tex2D does (rtcore/embree/Texture.cpp:26-289),
+ * texels widened to float4 at creation. bnamd_envmap replaces
+ * EnvMapLight::commit / computeCDFs (barney/light/EnvMap.cu:59-150,
+ * 178-230) and its DD (EnvMap.h:97-246); renders use it through
+ * bnamd_render_params.envMap.
+ * --------------------------------------------------------------------- */
+typedef struct bnamd_texture2d_s bnamd_texture2d;
+typedef struct {
+  int width, height;
+  int format;          /* rtc::DataType: 33 FLOAT4, 30 FLOAT, 1 UCHAR4, 0 UCHAR */
+  const void *texels;  /* host, row-major, width*height texels */
+  int filterMode;      /* rtc::FilterMode: 0 point, 1 linear */
+  int addressMode[2];  /* rtc::AddressMode per axis: 0 wrap, 1 clamp, 2 border, 3 mirror */
+} bnamd_texture2d_desc;
+int bnamd_texture2d_create(const bnamd_texture2d_desc *desc, void *stream, bnamd_texture2d **out);
+int bnamd_texture2d_destroy(bnamd_texture2d *tex);
+/* tex2D<float4> at n (u, v) pairs: uv device float2[n], out device float4[n] */
+int bnamd_texture2d_sample(const bnamd_texture2d *tex, const float *uv, float *out, int n, void *stream);
+
+typedef struct bnamd_envmap_s bnamd_envmap;
+typedef struct {
+  float direction[3];  /* EnvMapLight "direction" (default 1,0,0) */
+  float up[3];         /* "up" (default 0,0,1) */
+  float scale;         /* "scale" (default 1) */
+} bnamd_envmap_desc;
+/* The map references tex's texels: tex must outlive it. */
+int bnamd_envmap_create(const bnamd_texture2d *tex, const bnamd_envmap_desc *desc, void *stream,
+                        bnamd_envmap **out);
+int bnamd_envmap_destroy(bnamd_envmap *env);
+/* host copies: cdfY[height], cdfX[height*width]; frames 3x3 column-major */
+int bnamd_envmap_get_cdfs(const bnamd_envmap *env, float *cdfY, float *cdfX);
+int bnamd_envmap_get_frame(const bnamd_envmap *env, float *toWorld, float *toLocal);
+/* EnvMapLight::DD::pdf and radianceFromEnv for n directions: dirs device
+ * float3[n], pdf device float[n], radiance device float3[n] */
+int bnamd_envmap_eval(const bnamd_envmap *env, const float *dirs, float *pdf, float *radiance, int n,
+                      void *stream);
+/* EnvMapLight::DD::sample with Random2(i * 0x9E3779B9, seed) for i < n:
+ * out device float[n][7] = direction[3], radiance[3], pdf */
+int bnamd_envmap_sample(const bnamd_envmap *env, uint32_t seed, float *out, int n, void *stream);
 
 typedef struct {
   long long raysTraced; /* all rays traced: primary, shadow and bounce */

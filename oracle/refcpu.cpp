@@ -126,11 +126,241 @@ inline bool better(float t, uint32_t key, const Best &b) { return t < b.t || (t 
 
 }  // namespace
 
+// ---- 2D textures and the HDRI environment light ----------------------------
+// A 2D texture as the embree backend samples it (rtcore/embree/Texture.cpp):
+// texels widened to float4 at creation (getTexel, :166-214: float -> (v,v,v,v),
+// uchar -> v / 255), nearest (:232-240) or bilinear (:266-289) filtering with
+// the per-axis addressing of computeAddress (:26-84).
+struct ref_envmap_s {
+  std::vector<float> tex;  // float4 texels, row-major
+  int w = 0, h = 0, filter = 1, addr[2] = {1, 1};
+  V3 toWorld[3], toLocal[3];  // columns
+  float scale = 1.f;
+  std::vector<float> cdfY, cdfX;
+};
+
+namespace {
+
+struct V4 {
+  float x, y, z, w;
+};
+
+struct LerpAddr {
+  int i0, i1;
+  float f;
+};
+
+// rtcore/embree/Texture.cpp:26-84 (modes: 0 wrap, 1 clamp, 2 border, 3 mirror)
+LerpAddr tex_address(int mode, float tc, int N) {
+  LerpAddr o{0, 0, 0.f};
+  if (mode == 0) {
+    tc = tc * (float)N - .5f;
+    tc = std::fmod(tc, (float)N);
+    if (tc < 0.f) tc += (float)N;
+    const float fc = std::floor(tc);
+    const int ic = (int)fc;
+    o.f = tc - ic;
+    // tc + N can round up to N itself; the texel index then wraps to 0
+    // (the embree sampler would read one texel past the row)
+    o.i0 = ic >= N ? ic - N : ic;
+    o.i1 = (int)std::fmod(tc + 1.f, (float)N);
+  } else if (mode == 3) {
+    tc = tc * N + .5f;
+    const float fc = std::floor(tc);
+    const int ic = (int)fc;
+    o.f = tc - ic;
+    int i0 = ic - 1, i1 = ic;
+    if (i0 < 0) i0 = -i0;
+    if (i1 < 0) i1 = -i1;
+    i0 %= 2 * N;
+    i1 %= 2 * N;
+    if (i0 >= N) i0 = 2 * N - 1 - i0;
+    if (i1 >= N) i1 = 2 * N - 1 - i1;
+    o.i0 = i0;
+    o.i1 = i1;
+  } else if (mode == 1) {
+    tc = tc * N;
+    if (tc - .5f <= 0.f) {
+      o.i0 = o.i1 = 0;
+      o.f = 0.f;
+    } else if (tc - .5f >= N - 1) {
+      o.i0 = o.i1 = N - 1;
+      o.f = 0.f;
+    } else {
+      o.f = (tc - .5f) - (int)(tc - .5f);
+      o.i0 = (int)(tc - .5f);
+      o.i1 = o.i0 + 1;
+    }
+  }
+  // border: the embree sampler leaves the record zeroed (texel 0, weight 0)
+  return o;
+}
+
+V4 texel(const ref_envmap_s *e, int i) {
+  const float *t = e->tex.data() + 4 * (size_t)i;
+  return V4{t[0], t[1], t[2], t[3]};
+}
+
+V4 lerp4(float f, V4 a, V4 b) {
+  return V4{(1.f - f) * a.x + f * b.x, (1.f - f) * a.y + f * b.y, (1.f - f) * a.z + f * b.z,
+            (1.f - f) * a.w + f * b.w};
+}
+
+V4 tex2d(const ref_envmap_s *e, float u, float v) {
+  if (e->filter == 0) {  // Texture.cpp:232-240
+    const int ix = (int)((uint32_t)(std::fabs(u) * e->w) % (uint32_t)e->w);
+    const int iy = (int)((uint32_t)(std::fabs(v) * e->h) % (uint32_t)e->h);
+    return texel(e, ix + iy * e->w);
+  }
+  const LerpAddr lx = tex_address(e->addr[0], u, e->w), ly = tex_address(e->addr[1], v, e->h);
+  const V4 v00 = texel(e, lx.i0 + ly.i0 * e->w), v01 = texel(e, lx.i1 + ly.i0 * e->w);
+  const V4 v10 = texel(e, lx.i0 + ly.i1 * e->w), v11 = texel(e, lx.i1 + ly.i1 * e->w);
+  return lerp4(ly.f, lerp4(lx.f, v00, v01), lerp4(lx.f, v10, v11));
+}
+
+// linear3f * vector: columns (common/math, xfmVector)
+V3 xfm_lin(const V3 *m, V3 v) { return add(add(scl(v.x, m[0]), scl(v.y, m[1])), scl(v.z, m[2])); }
+
+const float kTwoPi = 2.f * kPi;
+const float kOneOverPi = 1.f / kPi;
+const float kOneOverTwoPi = 1.f / kTwoPi;
+
+// render/DG.h:97-106 with the shared acos / atan2
+float sph_theta(V3 v) { return bnf::acos(std::fmax(std::fmin(v.z, 1.f), -1.f)); }
+float sph_phi(V3 v) {
+  const float p = bnf::atan2(v.y, v.x);
+  return (p < 0.f) ? (p + float(2.f * M_PI)) : p;
+}
+float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
+
+// EnvMap.h:97-133
+float cdf_pdf(int pos, const float *cdf, int N) { return N * (cdf[pos] - (pos ? cdf[pos - 1] : 0.f)); }
+int sample_cdf(const float *cdf, int N, float v, float &pdf) {
+  int begin = 0, end = N;
+  while (end - begin > 1) {
+    const int mid = (begin + end) / 2;
+    if (v < cdf[mid - 1])
+      end = mid;
+    else
+      begin = mid;
+  }
+  pdf = cdf_pdf(begin, cdf, N);
+  return begin;
+}
+
+// EnvMapLight::DD::worldToPixel (EnvMap.h:161-177)
+void env_pixel(const ref_envmap_s *e, V3 dir, int &ix, int &iy) {
+  const V3 l = xfm_lin(e->toLocal, dir);
+  const float u = sph_phi(l) * kOneOverTwoPi, v = sph_theta(l) * kOneOverPi;
+  ix = (int)clampf(u * (float)e->w, 0.f, e->w - 1.f);
+  iy = (int)clampf(v * (float)e->h, 0.f, e->h - 1.f);
+}
+
+// EnvMapLight::DD::pdf (EnvMap.h:135-159)
+float env_pdf(const ref_envmap_s *e, V3 dir) {
+  int ix, iy;
+  env_pixel(e, dir, ix, iy);
+  const float py = cdf_pdf(iy, e->cdfY.data(), e->h);
+  const float px = cdf_pdf(ix, e->cdfX.data() + (size_t)iy * e->w, e->w);
+  const float theta = kPi * ((iy + .5f) / e->h);
+  return px * py * 1.f / (kTwoPi * kPi * bnf::sin(theta));
+}
+
+// EnvMapLight::DD::uvToWorld (EnvMap.h:195-206)
+V3 env_uv_dir(const ref_envmap_s *e, float fx, float fy) {
+  const float phi = kTwoPi * fx, theta = kPi * fy;
+  float sp, cp, st, ct;
+  bnf::sincos(phi, sp, cp);
+  bnf::sincos(theta, st, ct);
+  return xfm_lin(e->toWorld, mk(cp * st, sp * st, ct));
+}
+
+// EnvMapLight::DD::sample (EnvMap.h:208-246): direction, radiance, pdf
+void env_sample(const ref_envmap_s *e, Rng &r, V3 &dir, V3 &rad, float &pdf) {
+  const float ry = r.uni();
+  const float rx = r.uni();
+  float py, px;
+  const int iy = sample_cdf(e->cdfY.data(), e->h, ry, py);
+  const int ix = sample_cdf(e->cdfX.data() + (size_t)e->w * iy, e->w, rx, px);
+  const float sx = (ix + r.uni()) / e->w;
+  const float sy = (iy + r.uni()) / e->h;
+  const V4 c = tex2d(e, sx, sy);
+  rad = mk(e->scale * c.x, e->scale * c.y, e->scale * c.z);
+  dir = env_uv_dir(e, sx, sy);
+  pdf = px * py * 1.f / (kTwoPi * kPi * bnf::sin(kPi * sy));
+}
+
+// radianceFromEnv with a texture (shadeRays.cu:437-456)
+V3 env_radiance(const ref_envmap_s *e, V3 rayDir) {
+  const V3 d = xfm_lin(e->toLocal, nrm(rayDir));
+  const float theta = sph_theta(d), phi = sph_phi(d);
+  const V4 c = tex2d(e, phi * kOneOverTwoPi, theta * kOneOverPi);
+  return mk(e->scale * c.x, e->scale * c.y, e->scale * c.z);
+}
+
+// EnvMapLight::commit + computeCDFs (EnvMap.cu:59-150, 178-230): the frame
+// from direction and up, its inverse (adjugate times the reciprocal
+// determinant), per-texel importance (max channel over a 3x3 half-texel
+// stencil), per-row CDFs and the row CDF weighted by sin(theta).
+void env_build(ref_envmap_s *e, const float *direction, const float *up) {
+  const V3 vz = neg(nrm(ld3(up)));
+  const V3 vy = neg(nrm(cross(vz, ld3(direction))));
+  const V3 vx = nrm(cross(vy, vz));
+  e->toWorld[0] = vx;
+  e->toWorld[1] = vy;
+  e->toWorld[2] = vz;
+  const V3 a = cross(vy, vz), b = cross(vz, vx), c = cross(vx, vy);
+  const float r = 1.f / dot(vx, a);
+  e->toLocal[0] = scl(r, mk(a.x, b.x, c.x));
+  e->toLocal[1] = scl(r, mk(a.y, b.y, c.y));
+  e->toLocal[2] = scl(r, mk(a.z, b.z, c.z));
+  const int W = e->w, H = e->h;
+  e->cdfX.assign((size_t)W * H, 0.f);
+  e->cdfY.assign(H, 0.f);
+  for (int iy = 0; iy < H; ++iy)
+    for (int ix = 0; ix < W; ++ix) {
+      float wgt = 0.f;
+      for (int jy = 0; jy <= 2; ++jy)
+        for (int jx = 0; jx <= 2; ++jx) {
+          const V4 t = tex2d(e, (ix + jx * .5f) / W, (iy + jy * .5f) / H);
+          wgt = std::max(wgt, std::max(std::max(t.x, t.y), t.z));
+        }
+      e->cdfX[ix + (size_t)W * iy] = wgt;
+    }
+  for (int y = 0; y < H; ++y) {
+    float *line = e->cdfX.data() + (size_t)y * W;
+    float sum = 0.f;
+    for (int x = 0; x < W; ++x) sum += line[x];
+    const float rs = 1.f / sum;
+    sum = 0.f;
+    for (int x = 0; x < W; ++x) {
+      sum += line[x];
+      line[x] = sum * rs;
+    }
+    line[W - 1] = 1.f;
+    e->cdfY[y] = sum * bnf::sin(kPi * ((y + .5f) / H));
+  }
+  float sum = 0.f;
+  for (int i = 0; i < H; ++i) sum += e->cdfY[i];
+  const float rs = 1.f / sum;
+  sum = 0.f;
+  for (int i = 0; i < H; ++i) {
+    sum += e->cdfY[i];
+    e->cdfY[i] = sum * rs;
+  }
+  e->cdfY[H - 1] = 1.f;
+}
+
+}  // namespace
+
 struct ref_scene_s {
   std::vector<ref_geom> geoms;
   // triangles whose material is Glass: Glass::getOpacity returns 0 for
   // shadow rays and the triangle anyHit ignores them (Triangles.dev.cu:119-133)
   std::vector<uint8_t> shadowTransparent;
+  // per geom: the BSDF alpha its triangle anyHit tests (1 = opaque)
+  std::vector<float> alpha;
+  bool cutouts = false;
   std::vector<Prim> prims;  // reordered into BVH order
   struct Node {
     float lo[3], hi[3];
@@ -142,8 +372,60 @@ struct ref_scene_s {
 
 namespace {
 
+// What barney's anyHit / intersection programs do with a candidate before
+// reporting it: the renderer's cut plane (OptixGlobals.h:33-42, the world
+// point o + t d on the invisible side is dropped) and, for triangles whose
+// material alpha is below 1, stochastic opacity (Triangles.dev.cu:121-133:
+// the seed advanced by the bits of the object-space point, then
+// Random(seed, 290374) > alpha drops the candidate). Each candidate's draw
+// uses the ray's seed at the start of the trace (barney chains the seed in
+// traversal order), the point v0 + (u e1 + v e2) with e1 = v1 - v0,
+// e2 = v2 - v0 as the kernels' primitive records hold them.
+struct Filt {
+  bool cutOn;
+  float cut[4];
+  V3 o, d;
+  uint64_t seed;
+  bool stochastic;
+  bool any(float t) const {
+    if (!cutOn) return true;
+    const V3 P = add(o, scl(t, d));
+    return !(dot(mk(cut[0], cut[1], cut[2]), P) + cut[3] < 0.f);
+  }
+  bool tri(float t, float u, float v, V3 v0, V3 e1, V3 e2, float alpha) const {
+    if (!any(t)) return false;
+    if (stochastic && alpha < 1.f) {
+      const V3 P = add(v0, add(scl(u, e1), scl(v, e2)));
+      Rng s;
+      s.s = seed;
+      s.mix(fbits(P.x));
+      s.mix(fbits(P.y));
+      s.mix(fbits(P.z));
+      Rng r = child(s, 290374ull);
+      if (r.uni() > alpha) return false;
+    }
+    return true;
+  }
+};
+
+bool cut_enabled(const float *c) {
+  return c[3] > -1e28f && !(c[0] == 0.f && c[1] == 0.f && c[2] == 0.f && c[3] == 0.f);
+}
+
+Filt make_filt(const ref_render_params *p, V3 o, V3 d, uint64_t seed, bool stochastic = true) {
+  Filt f;
+  f.cutOn = cut_enabled(p->cutPlane);
+  for (int k = 0; k < 4; ++k) f.cut[k] = p->cutPlane[k];
+  f.o = o;
+  f.d = d;
+  f.seed = seed;
+  f.stochastic = stochastic;
+  return f;
+}
+
 // rtcore/cuda/TraceInterface.h:32-98 (barney's plane + 2D-projection test)
-void hit_triangle(V3 org, V3 dir, float tmax, V3 v0, V3 v1, V3 v2, uint32_t key, int g, int p, Best &b) {
+void hit_triangle(V3 org, V3 dir, float tmax, V3 v0, V3 v1, V3 v2, uint32_t key, int g, int p, Best &b,
+                  const Filt *f = nullptr, float alpha = 1.f) {
   const V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
   const V3 N = cross(e1, e2);
   const float dd = dot(dir, N);
@@ -168,11 +450,13 @@ void hit_triangle(V3 org, V3 dir, float tmax, V3 v0, V3 v1, V3 v2, uint32_t key,
   const float v = (e1u * Pv - e1v * Pu) / den;
   if (u < 0.f || v < 0.f || (u + v) >= 1.f) return;
   if (!better(t, key, b)) return;
+  if (f && !f->tri(t, u, v, v0, e1, e2, alpha)) return;
   b.t = t; b.u = u; b.v = v; b.key = key; b.geom = g; b.prim = p;
 }
 
 // barney/geometry/Spheres.dev.cu:116-170
-void hit_sphere(V3 org, V3 dir, float tmin, float tmax, V3 c, float r, uint32_t key, int g, int p, Best &b) {
+void hit_sphere(V3 org, V3 dir, float tmin, float tmax, V3 c, float r, uint32_t key, int g, int p, Best &b,
+                const Filt *f = nullptr) {
   const float t_move = std::fmax(0.f, len(sub(c, org)) - 3.f * r);
   const V3 o = add(org, scl(t_move, dir));
   const float t_max = tmax - t_move;
@@ -190,6 +474,7 @@ void hit_sphere(V3 org, V3 dir, float tmin, float tmax, V3 c, float r, uint32_t 
   if (!(hit_t < t_max)) return;
   const float t = hit_t + t_move;
   if (!better(t, key, b)) return;
+  if (f && !f->any(t)) return;
   b.t = t; b.u = hit_t; b.v = t_move; b.key = key; b.geom = g; b.prim = p;
 }
 
@@ -245,21 +530,22 @@ bool cyl(V3 ro, V3 rd, float tmin, float tmax, V3 v0, V3 v1, float radius, float
   return true;
 }
 
-void test_prim(const ref_scene *sc, const Prim &pr, V3 org, V3 dir, float tmin, float tmax, Best &b) {
+void test_prim(const ref_scene *sc, const Prim &pr, V3 org, V3 dir, float tmin, float tmax, Best &b,
+               const Filt *f = nullptr) {
   const ref_geom &g = sc->geoms[pr.geom];
   const int p = pr.prim;
   if (g.kind == 0) {
     const int *ix = g.indices + 3 * p;
     hit_triangle(org, dir, tmax, ld3(g.vertices + 3 * ix[0]), ld3(g.vertices + 3 * ix[1]),
-                 ld3(g.vertices + 3 * ix[2]), pr.key, pr.geom, p, b);
+                 ld3(g.vertices + 3 * ix[2]), pr.key, pr.geom, p, b, f, sc->alpha[pr.geom]);
   } else if (g.kind == 1) {
     const float r = g.radii ? g.radii[p] : g.defaultRadius;
-    hit_sphere(org, dir, tmin, tmax, ld3(g.vertices + 3 * p), r, pr.key, pr.geom, p, b);
+    hit_sphere(org, dir, tmin, tmax, ld3(g.vertices + 3 * p), r, pr.key, pr.geom, p, b, f);
   } else {
     const int *ix = g.indices + 2 * p;
     float t;
     if (cyl(org, dir, tmin, tmax, ld3(g.vertices + 3 * ix[0]), ld3(g.vertices + 3 * ix[1]), g.radii[p], t, nullptr) &&
-        better(t, pr.key, b)) {
+        better(t, pr.key, b) && (!f || f->any(t))) {
       b.t = t; b.u = 0.f; b.v = 0.f; b.key = pr.key; b.geom = pr.geom; b.prim = p;
     }
   }
@@ -279,12 +565,12 @@ inline bool box_hit(const float *lo, const float *hi, const double *o, const dou
 }
 
 Best closest(const ref_scene *sc, V3 org, V3 dir, float tmin, float tmax, bool brute, long long *tests,
-             bool shadowRay = false) {
+             bool shadowRay = false, const Filt *f = nullptr) {
   Best b;
   b.t = tmax; b.u = b.v = 0.f; b.key = 0xffffffffu; b.geom = -1; b.prim = -1;
   auto test = [&](const Prim &p) {
     if (shadowRay && sc->shadowTransparent[p.geom]) return;
-    test_prim(sc, p, org, dir, tmin, tmax, b);
+    test_prim(sc, p, org, dir, tmin, tmax, b, f);
   };
   if (brute) {
     for (const Prim &p : sc->prims) test(p);
@@ -423,7 +709,13 @@ void hit_point_normal(const ref_scene *sc, const Best &b, V3 org, V3 dir, float 
   if (g.kind == 0) {
     const int *ix = g.indices + 3 * p;
     const V3 v0 = ld3(g.vertices + 3 * ix[0]), v1 = ld3(g.vertices + 3 * ix[1]), v2 = ld3(g.vertices + 3 * ix[2]);
-    N = nrm(cross(sub(v1, v0), sub(v2, v0)));
+    // Triangles.dev.cu:65-75: n = normalize(cross) or the normalized
+    // interpolated vertex normal; the object normal is normalize(n)
+    V3 n = nrm(cross(sub(v1, v0), sub(v2, v0)));
+    if (g.normals)
+      n = nrm(add(add(scl(1.f - b.u - b.v, ld3(g.normals + 3 * ix[0])), scl(b.u, ld3(g.normals + 3 * ix[1]))),
+                  scl(b.v, ld3(g.normals + 3 * ix[2]))));
+    N = nrm(n);
     P = add(add(scl(1.f - b.u - b.v, v0), scl(b.u, v1)), scl(b.v, v2));
   } else if (g.kind == 1) {
     const V3 c = ld3(g.vertices + 3 * p);
@@ -481,6 +773,18 @@ PRay cam_ray(const Cam &c, const ref_render_params *p, int ix, int iy, int accum
   r.org = c.org;
   r.dir = nrm(d);
   r.tMax = 1e30f;
+  if (cut_enabled(p->cutPlane)) {  // generateRays.cu:171-193
+    const V3 N = ld3(p->cutPlane);
+    const float distOrg = dot(N, r.org) + p->cutPlane[3];
+    const float denom = dot(N, r.dir);
+    if (distOrg >= 0.f) {
+      if (denom < 0.f) r.tMax = std::fmin(r.tMax, -distOrg / denom);
+    } else if (denom > 0.f) {
+      r.org = add(r.org, scl(-distOrg / denom, r.dir));
+    } else {
+      r.tMax = 0.f;
+    }
+  }
   r.tp = mk(1, 1, 1);
   r.mis = 0.f;
   r.nDiff = 0;
@@ -914,12 +1218,14 @@ Scat bsdf_scatter(const HitInfo &h, V3 Ng, V3 wo, bool inMedium, Rng &rnd) {
   return s;
 }
 
-// shadeRays.cu:476-878 with only the environment light present
+// shadeRays.cu:476-878: the environment light is the only light, an HDRI
+// map (EnvMapLight) or the uniform ambientRadiance
 V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int depth, int maxDepth, int maxDiff) {
   sh.tMax = -1.f;
   V3 frag = mk(0, 0, 0);
   const V3 tp0 = ray.tp;
   const float amb = p->ambientRadiance;
+  const ref_envmap_s *env = p->envMap;
   if (ray.shadow) {
     if (!h.hit) frag = scl(ray.mis, ray.tp);
     ray.tMax = -1.f;
@@ -930,7 +1236,11 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
   if (!vol) Ng = nrm(Ng);
   const V3 Ngff = dot(ray.dir, Ng) < 0.f ? Ng : neg(Ng);
   if (!h.hit) {
-    frag = depth == 0 ? ray.miss : scl(ray.mis, mul(ray.tp, mk(amb, amb, amb)));
+    // primaryRayMissColor / radianceFromEnv (shadeRays.cu:437-470, 556-575)
+    if (depth == 0)
+      frag = env ? env_radiance(env, ray.dir) : ray.miss;
+    else
+      frag = scl(ray.mis, mul(ray.tp, env ? env_radiance(env, ray.dir) : mk(amb, amb, amb)));
     ray.tMax = -1.f;
     return frag;
   }
@@ -939,36 +1249,46 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
   const V3 wo = neg(nrm(ray.dir));
   const float eps = std::fmax(1e-4f, 1e-6f * maxc(vabs(P)));
   const V3 off = vol ? wo : Ngff;
-  // sampleEnvLight: randomDirection (DG.h:184-194)
-  V3 ld;
-  while (true) {
-    V3 v;
-    v.x = 1.f - 2.f * rnd.uni();
-    v.y = 1.f - 2.f * rnd.uni();
-    v.z = 1.f - 2.f * rnd.uni();
-    if (dot(v, v) <= 1.f) { ld = nrm(v); break; }
-  }
-  V3 lrad;
+  // sampleEnvLight (shadeRays.cu:267-293): the map's importance sample, or
+  // randomDirection (DG.h:184-194) into the hemisphere of the normal
+  V3 ld, lrad;
   float lpdf;
-  if (!eq(Ngff, mk(0, 0, 0))) {
-    lrad = mk(2.f * kPi * amb, 2.f * kPi * amb, 2.f * kPi * amb);
-    if (dot(ld, Ngff) < 0.f) ld = neg(ld);
-    lpdf = 1.f / (2.f * kPi);
+  if (env) {
+    env_sample(env, rnd, ld, lrad, lpdf);
   } else {
-    const float r4 = 4.f * kPi * amb;
-    lrad = mk(r4, r4, r4);
-    lpdf = 1.f / (4.f * kPi);
+    while (true) {
+      V3 v;
+      v.x = 1.f - 2.f * rnd.uni();
+      v.y = 1.f - 2.f * rnd.uni();
+      v.z = 1.f - 2.f * rnd.uni();
+      if (dot(v, v) <= 1.f) { ld = nrm(v); break; }
+    }
+    if (!eq(Ngff, mk(0, 0, 0))) {
+      lrad = mk(2.f * kPi * amb, 2.f * kPi * amb, 2.f * kPi * amb);
+      if (dot(ld, Ngff) < 0.f) ld = neg(ld);
+      lpdf = 1.f / (2.f * kPi);
+    } else {
+      const float r4 = 4.f * kPi * amb;
+      lrad = mk(r4, r4, r4);
+      lpdf = 1.f / (4.f * kPi);
+    }
   }
+  // sampleLights (shadeRays.cu:295-404): the area, point and directional
+  // lights are absent (weight 0, pdf 0), so a draw at or below the area
+  // weight or above the environment's share picks no light
   bool needMIS = false, have = false;
   {
-    float w = maxc(lrad);
-    const float sum = w;
+    const float w = maxc(lrad);
+    const float sum = ((0.f + 0.f) + w) + 0.f;
     if (sum != 0.f) {
-      w *= 1.f / sum;
-      (void)rnd.uni();  // light-selection draw
-      lrad = scl(1.f / w, lrad);
-      needMIS = true;
-      have = !(lpdf <= 0.f) && !std::isnan(lpdf);
+      const float alsW = 0.f * (1.f / sum);
+      const float elsW = w * (1.f / sum);
+      const float r = rnd.uni();
+      if (!(r <= alsW) && r <= alsW + elsW) {
+        lrad = scl(1.f / elsW, lrad);
+        needMIS = true;
+        have = !(lpdf <= 0.f) && !std::isnan(lpdf);
+      }
     }
   }
   if (have) {
@@ -992,7 +1312,7 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
       sh.nDiff = ray.nDiff;
       sh.inMedium = ray.inMedium;
       if (needMIS) {
-        const float pl = 1.f / (4.f * kPi);
+        const float pl = env ? env_pdf(env, ld) : 1.f / (4.f * kPi);
         const float pb = bsdf_pdf(h, Ng, wo, ld);
         sh.mis = pl / (pl + pb + 1e-10f);
         if (sh.mis < 1e-5f) sh.tMax = -1.f;
@@ -1026,7 +1346,7 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
   ray.tp = mul(ray.tp, sf);
   if (sr.medium) ray.inMedium = !ray.inMedium;
   if (needMIS && !std::isinf(sr.pdf)) {
-    const float pb = bsdf_pdf(h, Ng, wo, ray.dir), pl = 1.f / (4.f * kPi);
+    const float pb = bsdf_pdf(h, Ng, wo, ray.dir), pl = env ? env_pdf(env, ray.dir) : 1.f / (4.f * kPi);
     ray.mis = pb / (pb + pl);
   } else {
     ray.mis = 1.f;
@@ -1074,6 +1394,74 @@ void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n) {
   for (int i = 0; i < n; ++i) out[i] = r.uni();
 }
 uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c) { return fnv3(a, b, c); }
+// ---- environment map (EnvMap.cu, EnvMap.h; textures: embree Texture.cpp) ----
+ref_envmap *ref_envmap_create(const void *texels, int format, int w, int h, int filter, const int *addr,
+                              const float *direction, const float *up, float scale) {
+  if (w <= 0 || h <= 0 || !texels) return nullptr;
+  auto *e = new ref_envmap_s;
+  e->w = w;
+  e->h = h;
+  e->filter = filter;
+  e->addr[0] = addr[0];
+  e->addr[1] = addr[1];
+  e->scale = scale;
+  const size_t n = (size_t)w * h;
+  e->tex.resize(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    float *t = e->tex.data() + 4 * i;
+    if (format == 33) {  // rtc::FLOAT4
+      for (int k = 0; k < 4; ++k) t[k] = ((const float *)texels)[4 * i + k];
+    } else if (format == 30) {  // rtc::FLOAT
+      t[0] = t[1] = t[2] = t[3] = ((const float *)texels)[i];
+    } else if (format == 1) {  // rtc::UCHAR4
+      for (int k = 0; k < 4; ++k) t[k] = (float)((const uint8_t *)texels)[4 * i + k] * 1.f / 255.f;
+    } else if (format == 0) {  // rtc::UCHAR
+      t[0] = t[1] = t[2] = t[3] = (float)((const uint8_t *)texels)[i] * 1.f / 255.f;
+    } else {
+      delete e;
+      return nullptr;
+    }
+  }
+  env_build(e, direction, up);
+  return e;
+}
+void ref_envmap_destroy(ref_envmap *e) { delete e; }
+void ref_envmap_get_cdfs(const ref_envmap *e, float *cdfY, float *cdfX) {
+  std::copy(e->cdfY.begin(), e->cdfY.end(), cdfY);
+  std::copy(e->cdfX.begin(), e->cdfX.end(), cdfX);
+}
+void ref_envmap_frame(const ref_envmap *e, float *toWorld, float *toLocal) {
+  for (int k = 0; k < 3; ++k) {
+    toWorld[3 * k] = e->toWorld[k].x; toWorld[3 * k + 1] = e->toWorld[k].y; toWorld[3 * k + 2] = e->toWorld[k].z;
+    toLocal[3 * k] = e->toLocal[k].x; toLocal[3 * k + 1] = e->toLocal[k].y; toLocal[3 * k + 2] = e->toLocal[k].z;
+  }
+}
+void ref_envmap_tex2d(const ref_envmap *e, const float *uv, float *out, int n) {
+  for (int i = 0; i < n; ++i) {
+    const V4 v = tex2d(e, uv[2 * i], uv[2 * i + 1]);
+    out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+  }
+}
+void ref_envmap_eval(const ref_envmap *e, const float *dirs, float *pdf, float *rad, int n) {
+  for (int i = 0; i < n; ++i) {
+    const V3 d = ld3(dirs + 3 * i);
+    pdf[i] = env_pdf(e, d);
+    const V3 r = env_radiance(e, d);
+    rad[3 * i] = r.x; rad[3 * i + 1] = r.y; rad[3 * i + 2] = r.z;
+  }
+}
+void ref_envmap_sample(const ref_envmap *e, uint32_t seed, float *out, int n) {
+  for (int i = 0; i < n; ++i) {
+    Rng r;
+    r.seed((uint32_t)i * 0x9E3779B9u, seed);
+    V3 d, rad;
+    float pdf;
+    env_sample(e, r, d, rad, pdf);
+    float *o = out + 7 * (size_t)i;
+    o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = rad.x; o[4] = rad.y; o[5] = rad.z; o[6] = pdf;
+  }
+}
+
 
 float ref_half_round(float x) { return half_round(x); }
 
@@ -1118,6 +1506,14 @@ void ref_fmath_eval(const float *x, float *sinOut, float *cosOut, float *pow5Out
   }
 }
 
+// the shared arc cosine and two-argument arctangent (fmath.h)
+void ref_fmath_eval_inv(const float *y, const float *x, float *atan2Out, float *acosOut, int n) {
+  for (int i = 0; i < n; ++i) {
+    atan2Out[i] = bnf::atan2(y[i], x[i]);
+    acosOut[i] = bnf::acos(x[i]);
+  }
+}
+
 // Volume.h:174-184
 float ref_fast_log(float f) {
   f = (f - 1.f) / (f + 1.f);
@@ -1133,9 +1529,19 @@ float ref_fast_log(float f) {
 ref_scene *ref_scene_create(const ref_geom *geoms, int numGeoms) {
   auto *s = new ref_scene_s;
   s->geoms.assign(geoms, geoms + numGeoms);
-  for (int g = 0; g < numGeoms; ++g)
-    s->shadowTransparent.push_back(geoms[g].kind == 0 && geoms[g].matType == 1 && geoms[g].ior != 1.f &&
-                                   geoms[g].transmission >= 1e-3f);
+  for (int g = 0; g < numGeoms; ++g) {
+    const ref_geom &gd = geoms[g];
+    const bool glass = gd.kind == 0 && gd.matType == 1 && gd.ior != 1.f && gd.transmission >= 1e-3f;
+    s->shadowTransparent.push_back(glass);
+    // Lambertian alpha = baseColor.w * opacity (AnariMatte.h:63); NVisii
+    // alpha = (1 - transmission) * baseColor.w * opacity as a half
+    // (AnariPBR.h:88-91, NVisii.h:1080); glass: the flag above
+    float a = 1.f;
+    if (gd.kind == 0 && !glass)
+      a = gd.matType == 1 ? half_round(((1.f - gd.transmission) * 1.f) * gd.opacity) : 1.f * gd.opacity;
+    s->alpha.push_back(a);
+    s->cutouts = s->cutouts || a < 1.f;
+  }
   uint32_t key = 0;
   for (int g = 0; g < numGeoms; ++g) {
     const ref_geom &gd = geoms[g];
@@ -1172,6 +1578,31 @@ ref_scene *ref_scene_create(const ref_geom *geoms, int numGeoms) {
 }
 
 void ref_scene_destroy(ref_scene *s) { delete s; }
+
+// Closest hit (shadow = 0) or occlusion (shadow = 1: glass skipped, hit =
+// an accepted occluder) with the renderer's candidate filter: p's cut plane
+// and stochastic opacity drawn from seeds[i] (stochastic = 0: opacity
+// ignored, as for the ID channel).
+void ref_trace_filtered(const ref_scene *s, const ref_render_params *p, const float *rays, const uint64_t *seeds,
+                        ref_hit *hits, int n, int brute, int shadow, int stochastic, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * i;
+    const V3 org = ld3(r), dir = fix_dir(ld3(r + 4));
+    const Filt f = make_filt(p, org, dir, seeds[i], stochastic != 0);
+    const Best b = closest(s, org, dir, r[3], r[7], brute != 0, nullptr, shadow != 0, &f);
+    ref_hit &h = hits[i];
+    h.t = b.t; h.u = b.u; h.v = b.v;
+    if (b.geom < 0) {
+      h.primID = h.geomID = h.instID = h.instIndex = -1;
+      h.N[0] = h.N[1] = h.N[2] = 0.f;
+    } else {
+      V3 P, N;
+      hit_point_normal(s, b, org, dir, r[3], r[7], P, N);
+      h.primID = b.prim; h.geomID = b.geom; h.instID = 0; h.instIndex = 0;
+      h.N[0] = N.x; h.N[1] = N.y; h.N[2] = N.z;
+    }
+  });
+}
 
 void ref_trace(const ref_scene *s, const float *rays, ref_hit *hits, int n, int brute, int threads, long long *primTests) {
   std::atomic<long long> tests(0);
@@ -1239,9 +1670,11 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
       const float cmax = 10.f * (1 + a);
       V3 sum = mk(0, 0, 0);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const V3 d = fix_dir(ray.dir);
-        const Best b = closest(s, ray.org, d, 0.f, ray.tMax, false, nullptr);
+        const Filt f = make_filt(p, ray.org, d, ray.rng.s);
+        const Best b0 = closest(s, ray.org, d, 0.f, ray.tMax, false, nullptr, false, &f);
+        const Best &b = b0;
         ++nr;
         HitInfo hi;
         hi.hit = b.geom >= 0;
@@ -1256,7 +1689,14 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
           alpha = hi.hit ? 1.f : ray.missA;
           if (ids && pp == 0 && p->accumID == 0) {
             int *rec = ids + 4 * pix;
-            if (hi.hit) {
+            // the closest candidate whatever its opacity (Triangles.dev.cu:
+            // 45-60 writes the IDs before the opacity test)
+            Best b = b0;
+            if (s->cutouts) {
+              const Filt fid = make_filt(p, ray.org, d, ray.rng.s, false);
+              b = closest(s, ray.org, d, 0.f, ray.tMax, false, nullptr, false, &fid);
+            }
+            if (b.geom >= 0) {
               const ref_geom &g = s->geoms[b.geom];
               rec[0] = b.prim;
               rec[1] = g.kind == 1 ? 13 : 0;
@@ -1272,7 +1712,9 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
         PRay sh;
         add_frag(sum, bounce(p, ray, hi, sh, gen, maxDepth, maxDiff), cmax);
         if (sh.tMax > 0.f) {
-          const Best bs = closest(s, sh.org, fix_dir(sh.dir), 0.f, sh.tMax, false, nullptr, true);
+          const V3 sd = fix_dir(sh.dir);
+          const Filt fs = make_filt(p, sh.org, sd, sh.rng.s);
+          const Best bs = closest(s, sh.org, sd, 0.f, sh.tMax, false, nullptr, true, &fs);
           ++nr;
           HitInfo hs;
           hs.hit = bs.geom >= 0;
@@ -1621,7 +2063,7 @@ long long render_volume_g(const ref_volume_s *v, const Field &field, const ref_c
       const float cmax = 10.f * (1 + a);
       V3 sum = mk(0, 0, 0);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const V3 d = fix_dir(ray.dir);
         const VHit vh = vol_isect(v, ray.org, d, 0.f, ray.tMax, ray.tMax, ray.rng, false, cnt);
         ++nr;
@@ -2654,7 +3096,7 @@ long long ref_umesh_render(const ref_umesh *u, const ref_camera *cam, const ref_
       const float cmax = 10.f * (1 + a);
       V3 sum = mk(0, 0, 0);
       float alpha = 0.f;
-      for (int gen = 0; ray.tMax > 0.f; ++gen) {
+      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
         const V3 d = fix_dir(ray.dir);
         const HitInfo hi = um_trace(u, volume != 0, iso != 0, isoValue, icol, ray.org, d, ray.tMax, ray.rng, cnt);
         ++nr;

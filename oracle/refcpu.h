@@ -26,6 +26,8 @@ void ref_rng_sequence(uint64_t *state, uint32_t *out, int n);
 void ref_random2_floats(uint32_t a, uint32_t b, float *out, int n);
 uint64_t ref_hash3(uint32_t a, uint32_t b, uint32_t c);
 void ref_fmath_eval(const float *x, float *sinOut, float *cosOut, float *pow5Out, int n);
+/* atan2(y[i], x[i]) and acos(x[i]) of the shared fmath.h sequences */
+void ref_fmath_eval_inv(const float *y, const float *x, float *atan2Out, float *acosOut, int n);
 float ref_fast_log(float f);
 
 /* --- geometry input (host arrays) --- */
@@ -43,6 +45,11 @@ typedef struct {
    * transmission >= 1e-3, else NVisii; AnariPBR.h:68-94) */
   int matType;
   float metallic, roughness, transmission, ior;
+  /* material "opacity" (AnariMatte / AnariPBR; 1 = opaque): triangles with
+   * an alpha below 1 ignore candidate hits at random (Triangles.dev.cu:
+   * 121-133) */
+  float opacity;
+  const float *normals;   /* triangles: float3 per vertex (shading normals) or NULL */
 } ref_geom;
 
 typedef struct ref_scene_s ref_scene;
@@ -95,7 +102,34 @@ typedef struct {
   /* Russian roulette from this path depth on (0 = off; not in barney):
    * survive with p = min(max throughput, 0.95), throughput /= p */
   int russianRoulette;
+  /* HDRI environment light (EnvMapLight) or NULL: uniform ambientRadiance */
+  const struct ref_envmap_s *envMap;
+  /* Renderer "cutPlane" (nx, ny, nz, d): hits with dot(n, P) + d < 0 are
+   * ignored; disabled when d <= -1e28 (Renderer.h:59) */
+  float cutPlane[4];
 } ref_render_params;
+
+/* --- HDRI environment light (barney/light/EnvMap.cu, EnvMap.h) --- */
+typedef struct ref_envmap_s ref_envmap;
+/* texels: w*h texels of rtc::DataType format (33 FLOAT4, 30 FLOAT, 1 UCHAR4,
+ * 0 UCHAR), row-major; filter 0 point / 1 linear; addr[2] rtc::AddressMode
+ * (0 wrap, 1 clamp, 2 border, 3 mirror); direction / up / scale as
+ * EnvMapLight's parameters. NULL on a bad format or size. */
+ref_envmap *ref_envmap_create(const void *texels, int format, int w, int h, int filter, const int *addr,
+                              const float *direction, const float *up, float scale);
+void ref_envmap_destroy(ref_envmap *e);
+void ref_envmap_get_cdfs(const ref_envmap *e, float *cdfY, float *cdfX);
+/* 3x3 column-major toWorld and toLocal */
+void ref_envmap_frame(const ref_envmap *e, float *toWorld, float *toLocal);
+/* tex2D at n (u, v) pairs -> float4 */
+void ref_envmap_tex2d(const ref_envmap *e, const float *uv, float *out, int n);
+/* pdf(dir) and radianceFromEnv(dir) for n directions */
+void ref_envmap_eval(const ref_envmap *e, const float *dirs, float *pdf, float *rad, int n);
+/* sample i with Random2(i * 0x9E3779B9, seed): dir[3], radiance[3], pdf */
+void ref_envmap_sample(const ref_envmap *e, uint32_t seed, float *out, int n);
+
+void ref_trace_filtered(const ref_scene *s, const ref_render_params *p, const float *rays, const uint64_t *seeds,
+                        ref_hit *hits, int n, int brute, int shadow, int stochastic, int threads);
 
 /* primary ray for pixel (ix,iy): out = org[3], dir[3], seed (as 2 floats bits) */
 void ref_camera_ray(const ref_camera *cam, const ref_render_params *p, int ix, int iy, float *org,

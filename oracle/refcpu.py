@@ -24,7 +24,8 @@ class ref_geom(C.Structure):
     _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
                 ("radii", C.c_void_p), ("defaultRadius", C.c_float), ("colors", C.c_void_p),
                 ("color", C.c_float * 3), ("userID", C.c_int), ("matType", C.c_int), ("metallic", C.c_float),
-                ("roughness", C.c_float), ("transmission", C.c_float), ("ior", C.c_float)]
+                ("roughness", C.c_float), ("transmission", C.c_float), ("ior", C.c_float),
+                ("opacity", C.c_float), ("normals", C.c_void_p)]
 
 
 class ref_hit(C.Structure):
@@ -40,7 +41,8 @@ class ref_camera(C.Structure):
 class ref_render_params(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int), ("pathsPerPixel", C.c_int),
                 ("ambientRadiance", C.c_float), ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
-                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int)]
+                ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int), ("envMap", C.c_void_p),
+                ("cutPlane", C.c_float * 4)]
 
 
 _lib = None
@@ -99,6 +101,15 @@ def load():
         "ref_umesh_get_mcs": (None, [P, P, P]),
         "ref_umesh_sample": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_fmath_eval": (None, [P, P, P, P, C.c_int]),
+        "ref_fmath_eval_inv": (None, [P, P, P, P, C.c_int]),
+        "ref_trace_filtered": (None, [P, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ref_envmap_create": (P, [P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, C.c_float]),
+        "ref_envmap_destroy": (None, [P]),
+        "ref_envmap_get_cdfs": (None, [P, P, P]),
+        "ref_envmap_frame": (None, [P, P, P]),
+        "ref_envmap_tex2d": (None, [P, P, P, C.c_int]),
+        "ref_envmap_eval": (None, [P, P, P, P, C.c_int]),
+        "ref_envmap_sample": (None, [P, C.c_uint32, P, C.c_int]),
         "ref_umesh_sample_brute": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_umesh_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
         "ref_umesh_render": (C.c_longlong, [P, P, P, C.c_int, C.c_int, C.c_float, P, P, C.c_int, C.c_int, C.c_int,
@@ -128,8 +139,9 @@ class Scene:
 
     def __init__(self, geoms):
         """geoms: list of dicts kind, vertices, indices, radii, colors,
-        default_radius, color, user_id and optionally material = dict(type
-        "matte" | "pbr", metallic, roughness, transmission, ior)."""
+        normals (triangles: per-vertex), default_radius, color, user_id and
+        optionally material = dict(type "matte" | "pbr", metallic,
+        roughness, transmission, ior, opacity)."""
         L = load()
         self._keep = []
         arr = (ref_geom * max(len(geoms), 1))()
@@ -138,7 +150,8 @@ class Scene:
             i = None if g.get("indices") is None else np.ascontiguousarray(g["indices"], np.int32)
             r = None if g.get("radii") is None else np.ascontiguousarray(g["radii"], np.float32)
             c = None if g.get("colors") is None else np.ascontiguousarray(g["colors"], np.float32)
-            self._keep += [v, i, r, c]
+            nr = None if g.get("normals") is None else np.ascontiguousarray(g["normals"], np.float32)
+            self._keep += [v, i, r, c, nr]
             kind = g["kind"]
             n = len(v.reshape(-1, 3)) if kind == 1 else len(i.reshape(-1, 3 if kind == 0 else 2))
             e = arr[k]
@@ -153,10 +166,27 @@ class Scene:
             e.roughness = float(m.get("roughness", 0.5))
             e.transmission = float(m.get("transmission", 0.0))
             e.ior = float(m.get("ior", 1.45))
+            e.opacity = float(m.get("opacity", 1.0))
+            e.normals = _ptr(nr)
             if "color" in m:
                 e.color[:] = [float(x) for x in m["color"]]
         self._arr = arr
         self.handle = L.ref_scene_create(arr, len(geoms))
+
+    def trace_filtered(self, rays, params, seeds, brute=False, shadow=False, stochastic=True, threads=0):
+        """Closest hit (or occlusion) with the renderer's candidate filter:
+        params' cut plane, stochastic opacity drawn from seeds (uint64 per
+        ray)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        n = len(rays)
+        hits = (ref_hit * max(n, 1))()
+        load().ref_trace_filtered(self.handle, C.byref(params), rays.ctypes.data, seeds.ctypes.data, hits, n,
+                                  int(brute), int(shadow), int(stochastic), threads)
+        raw = np.frombuffer(hits, dtype=np.float32, count=10 * n).reshape(n, 10)
+        ints = raw.view(np.int32)
+        return {"t": raw[:, 0].copy(), "primID": ints[:, 3].copy(), "geomID": ints[:, 4].copy(),
+                "N": raw[:, 6:9].copy()}
 
     def trace(self, rays, brute=False, threads=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -429,14 +459,91 @@ def camera(position, direction, up=(0, 1, 0), fovy=60.0):
 
 
 def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
-                  max_diffuse=1, russian_roulette=0):
+                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None):
+    """env: an EnvMap (HDRI environment light) or None; cut_plane: (nx, ny,
+    nz, d) or None (disabled)."""
     p = ref_render_params()
     p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
     p.ambientRadiance = float(ambient)
     p.bgColor[:] = [float(x) for x in bg]
     p.maxPathDepth, p.maxDiffuseBounces = max_depth, max_diffuse
     p.russianRoulette = russian_roulette
+    p.envMap = env.handle if env is not None else None
+    p._env = env  # keeps the map alive as long as the parameters
+    p.cutPlane[:] = [float(x) for x in (cut_plane if cut_plane is not None else (0.0, 0.0, 0.0, -1e30))]
     return p
+
+
+def camera_ray(cam, params, ix, iy):
+    """The primary ray of pixel (ix, iy): origin, direction, RNG state."""
+    o = np.empty(3, np.float32)
+    d = np.empty(3, np.float32)
+    seed = np.zeros(1, np.uint64)
+    load().ref_camera_ray(C.byref(cam), C.byref(params), ix, iy, o.ctypes.data, d.ctypes.data, seed.ctypes.data)
+    return o, d, int(seed[0])
+
+
+TEXEL_FORMATS = {"float4": 33, "float": 30, "uchar4": 1, "uchar": 0}
+ADDRESS_MODES = {"wrap": 0, "clamp": 1, "border": 2, "mirror": 3}
+
+
+class EnvMap:
+    """The oracle's HDRI environment light: texels (H, W, 4) float32 /
+    (H, W) float32 / (H, W, 4) or (H, W) uint8; filter 'linear' or 'point';
+    address modes per axis."""
+
+    def __init__(self, texels, direction=(1, 0, 0), up=(0, 0, 1), scale=1.0, filter="linear",
+                 address=("wrap", "clamp")):
+        t = np.ascontiguousarray(texels)
+        self.h, self.w = t.shape[:2]
+        if t.dtype == np.uint8:
+            fmt = TEXEL_FORMATS["uchar4" if t.ndim == 3 else "uchar"]
+        else:
+            t = np.ascontiguousarray(t, np.float32)
+            fmt = TEXEL_FORMATS["float4" if t.ndim == 3 else "float"]
+        self._texels = t
+        addr = (C.c_int * 2)(*[ADDRESS_MODES[a] for a in address])
+        d = (C.c_float * 3)(*map(float, direction))
+        u = (C.c_float * 3)(*map(float, up))
+        self.handle = load().ref_envmap_create(t.ctypes.data, fmt, self.w, self.h,
+                                               1 if filter == "linear" else 0, addr, d, u, float(scale))
+        if not self.handle:
+            raise ValueError("ref_envmap_create failed")
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.ref_envmap_destroy(self.handle)
+            self.handle = None
+
+    def cdfs(self):
+        y = np.empty(self.h, np.float32)
+        x = np.empty((self.h, self.w), np.float32)
+        load().ref_envmap_get_cdfs(self.handle, y.ctypes.data, x.ctypes.data)
+        return y, x
+
+    def frame(self):
+        w = np.empty(9, np.float32)
+        l = np.empty(9, np.float32)
+        load().ref_envmap_frame(self.handle, w.ctypes.data, l.ctypes.data)
+        return w.reshape(3, 3), l.reshape(3, 3)  # rows = columns of the matrices
+
+    def tex2d(self, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.empty((len(uv), 4), np.float32)
+        load().ref_envmap_tex2d(self.handle, uv.ctypes.data, out.ctypes.data, len(uv))
+        return out
+
+    def eval(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        pdf = np.empty(len(d), np.float32)
+        rad = np.empty((len(d), 3), np.float32)
+        load().ref_envmap_eval(self.handle, d.ctypes.data, pdf.ctypes.data, rad.ctypes.data, len(d))
+        return pdf, rad
+
+    def sample(self, n, seed=0):
+        out = np.empty((n, 7), np.float32)
+        load().ref_envmap_sample(self.handle, seed, out.ctypes.data, n)
+        return out[:, :3], out[:, 3:6], out[:, 6]
 
 
 def half_round(x):
@@ -558,3 +665,12 @@ def fmath(x):
     s, c, p = (np.empty_like(x) for _ in range(3))
     load().ref_fmath_eval(x.ctypes.data, s.ctypes.data, c.ctypes.data, p.ctypes.data, len(x))
     return s, c, p
+
+
+def fmath_inv(y, x):
+    """atan2(y, x) and acos(x) of the shared fmath.h sequences, elementwise."""
+    y = np.ascontiguousarray(y, np.float32).ravel()
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    a, c = np.empty_like(x), np.empty_like(x)
+    load().ref_fmath_eval_inv(y.ctypes.data, x.ctypes.data, a.ctypes.data, c.ctypes.data, len(x))
+    return a, c

@@ -61,3 +61,38 @@ def test_pow5_within_3_ulp():
     assert err.max() <= 3.0
     _, _, p = R.fmath(np.array([0.0, 1.0, 0.5, 0.25, 2.0], np.float32))
     assert list(p) == [0.0, 1.0, 0.5 ** 5, 0.25 ** 5, 32.0]
+
+
+def test_acos_within_3_ulp():
+    x = np.concatenate([np.linspace(-1.0, 1.0, 2_000_001), np.random.default_rng(5).uniform(-1, 1, 500_000),
+                        1.0 - np.logspace(-8, -1, 1000), -1.0 + np.logspace(-8, -1, 1000)]).astype(np.float32)
+    _, c = R.fmath_inv(np.zeros_like(x), x)
+    err = _ulps(c, np.arccos(x.astype(np.float64)))
+    print(f"max ulp acos {err.max():.3f}")
+    assert err.max() <= 3.0
+    _, c = R.fmath_inv(np.zeros(6, np.float32), np.array([1.0, -1.0, 0.0, 1.0000001, -2.0, np.nan], np.float32))
+    assert c[0] == 0.0 and c[1] == np.float32(np.pi) and c[2] == np.float32(np.pi / 2)
+    assert np.isnan(c[3:]).all()
+
+
+def test_atan2_within_3_ulp_and_quadrants():
+    rng = np.random.default_rng(6)
+    # directions on the unit sphere (the environment map's use) and wide ranges
+    v = rng.normal(size=(1_000_000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    ys = [v[:, 1], rng.uniform(-1e3, 1e3, 500_000), np.sin(np.linspace(-np.pi, np.pi, 100_001))]
+    xs = [v[:, 0], rng.uniform(-1e-3, 1e-3, 500_000), np.cos(np.linspace(-np.pi, np.pi, 100_001))]
+    for y, x in zip(ys, xs):
+        y, x = y.astype(np.float32), x.astype(np.float32)
+        a, _ = R.fmath_inv(y, x)
+        err = _ulps(a, np.arctan2(y.astype(np.float64), x.astype(np.float64)))
+        print(f"max ulp atan2 {err.max():.3f}")
+        assert err.max() <= 3.0
+    # the signed zero and infinity cases of atan2
+    y = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, np.inf, np.inf, 1.0, 1.0, 0.0], np.float32)
+    x = np.array([0.0, 0.0, -0.0, -0.0, 0.0, -0.0, np.inf, 1.0, np.inf, -np.inf, -1.0], np.float32)
+    a, _ = R.fmath_inv(y, x)
+    want = np.arctan2(y.astype(np.float64), x.astype(np.float64)).astype(np.float32)
+    assert np.array_equal(a, want) and np.array_equal(np.signbit(a), np.signbit(want))
+    a, _ = R.fmath_inv(np.array([np.nan, 1.0], np.float32), np.array([1.0, np.nan], np.float32))
+    assert np.isnan(a).all()
