@@ -516,15 +516,34 @@ def fb_bench(api, reps=20):
     return res
 
 
+# the sources a kernel's code is built from: a traffic figure measured on
+# other sources is not this kernel's
+KERNEL_SOURCES = {"k_render_volume": ["volume.hip", "volume.h", "shade.h", "envmap.h", "fmath.h", "common.h",
+                                      "world.h", "bvh8.h", "runtime.h", "amr.h"]}
+
+
+def sources_sha256(kernel):
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES[kernel]:
+        with open(os.path.join(ROOT, "barney_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
 def profile_traffic(kernel):
     """HBM bytes per launch of 'kernel' from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), each counter
-    scaled by what it reads for a copy kernel of known size."""
+    scaled by what it reads for a copy kernel of known size; None when the
+    figures were measured on other kernel sources than the ones built."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)[kernel]
     except (OSError, KeyError, ValueError):
+        return None
+    if d.get("sources_sha256") != sources_sha256(kernel):
+        log(f"profiles/traffic.json: {kernel} was measured on other sources; traffic left null")
         return None
     return int((d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024)
 

@@ -30,7 +30,7 @@ def test_render_matches_oracle(gpu):
     rel = np.abs(gacc - racc) / np.maximum(np.abs(racc), 1e-3)
     frac_close = (rel.max(-1) <= 1e-3).mean()
     print(f"close {frac_close:.5f} bit-exact {np.all(gacc == racc, axis=-1).mean():.5f}", flush=True)
-    assert frac_close >= 0.999, frac_close
+    assert np.array_equal(gacc.view(np.uint32), racc.view(np.uint32)), frac_close
     # statistical agreement of the image mean
     g = gacc[..., :3].reshape(-1, 3) / spp
     r = racc[..., :3].reshape(-1, 3) / spp
@@ -70,5 +70,4 @@ def test_every_pixel_written_any_frame_size(gpu, W, H):
     gacc = out["accum"].numpy()
     assert np.isfinite(gacc).all()
     assert np.array_equal(out["ids"], rids)
-    rel = np.abs(gacc - racc) / np.maximum(np.abs(racc), 1e-3)
-    assert (rel.max(-1) <= 1e-3).mean() >= 0.999
+    assert np.array_equal(gacc.view(np.uint32), racc.view(np.uint32))

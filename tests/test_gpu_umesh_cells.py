@@ -123,7 +123,7 @@ def test_mixed_renders_per_pixel(mixed, gpu, volume, iso, spp):
     rel = np.abs(ga - ra) / np.maximum(np.abs(ra), 1e-3)
     close = (rel.max(-1) <= 1e-3).mean()
     print(f"volume {volume} iso {iso} close {close:.5f} samples {out['stats']['samples']} {ns}", flush=True)
-    assert close >= 0.999
+    assert np.array_equal(ga.view(np.uint32), ra.view(np.uint32))
     assert out["stats"]["raysTraced"] == nr
     assert ra[..., 3].mean() > 0.2
 
@@ -173,7 +173,7 @@ def test_polyhedra_render_per_pixel(poly, gpu):
     rel = np.abs(ga - ra) / np.maximum(np.abs(ra), 1e-3)
     close = (rel.max(-1) <= 1e-3).mean()
     print(f"close {close:.5f} samples {out['stats']['samples']} {ns}", flush=True)
-    assert close >= 0.999
+    assert np.array_equal(ga.view(np.uint32), ra.view(np.uint32))
     assert out["stats"]["raysTraced"] == nr
     assert ra[..., 3].mean() > 0.2
 
