@@ -1,11 +1,14 @@
-// barney's bn* host API (barney/include/barney.h, barney/api/barney.cu) for
-// surface scenes, over the bnamd C ABI: objects hold staged parameters
-// (Object::set*, barney/Object.cpp), bnBuild turns the model's instances of
-// groups of geometries into one bnamd world (ModelSlot::build,
-// barney/ModelSlot.cpp:210-216), bnRender runs the fused surface path
-// tracer into the frame buffer's accumulation buffer (Context::render,
-// barney/Context.cpp:142-185) and bnFrameBufferRead resolves it
-// (FrameBuffer::read, barney/fb/FrameBuffer.cu:188-450).
+// barney's bn* host API (barney/include/barney.h, barney/api/barney.cu) over
+// the bnamd C ABI: objects hold staged parameters (Object::set*,
+// barney/Object.cpp), bnBuild turns the model's instances of groups of
+// geometries into one bnamd world (ModelSlot::build,
+// barney/ModelSlot.cpp:210-216) or builds its one volume (structured,
+// unstructured or block-structured AMR field), and its groups' env-map
+// light (ModelSlot::updateWorldLightsFromInstances, ModelSlot.cpp:90-125);
+// bnRender runs the fused path tracer into the frame buffer's accumulation
+// buffer (Context::render, barney/Context.cpp:142-185) and
+// bnFrameBufferRead resolves it (FrameBuffer::read,
+// barney/fb/FrameBuffer.cu:188-450).
 #include "bn_api.h"
 
 #include <array>
@@ -50,6 +53,9 @@ struct Object : _BNObject {
   int appRefs = 1;
   std::map<std::string, std::array<float, 4>> f;
   std::map<std::string, int> i;
+  std::map<std::string, std::array<int, 4>> iv;  // bnSet2i / 3i / 4i
+  std::map<std::string, std::array<float, 16>> mat;  // bnSet4x3fv / 4x4fv
+  std::map<std::string, std::string> str;
   std::map<std::string, Object *> o;
   float getf(const std::string &n, float def) const {
     auto it = f.find(n);
@@ -73,20 +79,78 @@ struct Data : Object {
   Data(Context *c, BNDataType t, size_t n, const void *items) : Object(c, "data"), dataType(t), count(n) {
     const size_t sz = elem_size(t);
     if (!sz) throw std::runtime_error("bnDataCreate: unsupported data type");
-    bytes.assign((const uint8_t *)items, (const uint8_t *)items + sz * n);
+    if (n && !items) throw std::runtime_error("bnDataCreate: NULL items");
+    if (t == BN_OBJECT)
+      objs.assign((Object *const *)items, (Object *const *)items + n);
+    else
+      bytes.assign((const uint8_t *)items, (const uint8_t *)items + sz * n);
   }
   static size_t elem_size(BNDataType t) {
     switch (t) {
-      case BN_INT32: case BN_FLOAT32: return 4;
-      case BN_INT32_VEC2: return 8;
+      case BN_OBJECT: return sizeof(void *);
+      case BN_UINT8: return 1;
+      case BN_UINT8_VEC4: case BN_INT32: case BN_UINT32: case BN_FLOAT32: return 4;
+      case BN_INT32_VEC2: case BN_FLOAT32_VEC2: case BN_INT64: case BN_UINT64: return 8;
       case BN_INT32_VEC3: case BN_FLOAT32_VEC3: return 12;
-      case BN_FLOAT32_VEC4: return 16;
+      case BN_INT32_VEC4: case BN_FLOAT32_VEC4: return 16;
       default: return 0;
     }
+  }
+  // integer items widened to 64 bits (offsets may come as any integer type)
+  std::vector<unsigned long long> as_u64() const {
+    std::vector<unsigned long long> r(count);
+    for (size_t k = 0; k < count; ++k) {
+      switch (dataType) {
+        case BN_INT32: r[k] = (unsigned long long)((const int32_t *)bytes.data())[k]; break;
+        case BN_UINT32: r[k] = ((const uint32_t *)bytes.data())[k]; break;
+        case BN_INT64: r[k] = (unsigned long long)((const int64_t *)bytes.data())[k]; break;
+        case BN_UINT64: r[k] = ((const uint64_t *)bytes.data())[k]; break;
+        default: throw std::runtime_error("bnBuild: integer data expected");
+      }
+    }
+    return r;
   }
   BNDataType dataType;
   size_t count;
   std::vector<uint8_t> bytes;
+  std::vector<Object *> objs;  // BN_OBJECT items
+};
+
+// bnTexture2DCreate (barney/common/Texture.cpp:85-135): host texels and the
+// sampling description; the device texture is made when a model uses it
+struct Texture2D : Object {
+  Texture2D(Context *c, BNDataType fmt, int w, int h, const void *texels, BNTextureFilterMode filter,
+            BNTextureAddressMode ax, BNTextureAddressMode ay)
+      : Object(c, "texture2D") {
+    desc.width = w;
+    desc.height = h;
+    size_t bpp;
+    switch (fmt) {  // rtc::DataType codes of the ABI
+      case BN_FLOAT32_VEC4: desc.format = 33; bpp = 16; break;
+      case BN_FLOAT32: desc.format = 30; bpp = 4; break;
+      case BN_UFIXED8_RGBA: case BN_UINT8_VEC4: desc.format = 1; bpp = 4; break;
+      case BN_UFIXED8: case BN_UINT8: desc.format = 0; bpp = 1; break;
+      default: throw std::runtime_error("bnTexture2DCreate: unsupported texel format");
+    }
+    if (w <= 0 || h <= 0 || !texels) throw std::runtime_error("bnTexture2DCreate: bad size or NULL texels");
+    bytes.assign((const uint8_t *)texels, (const uint8_t *)texels + bpp * (size_t)w * h);
+    desc.filterMode = filter == BN_TEXTURE_NEAREST ? 0 : 1;
+    desc.addressMode[0] = (int)ax;  // BNTextureAddressMode = rtc::AddressMode order
+    desc.addressMode[1] = (int)ay;
+  }
+  ~Texture2D() override {
+    if (tex) (void)bnamd_texture2d_destroy(tex);
+  }
+  bnamd_texture2d *device() {
+    if (!tex) {
+      desc.texels = bytes.data();
+      check(bnamd_texture2d_create(&desc, nullptr, &tex), "bnTexture2DCreate");
+    }
+    return tex;
+  }
+  bnamd_texture2d_desc desc{};
+  std::vector<uint8_t> bytes;
+  bnamd_texture2d *tex = nullptr;
 };
 
 struct TexData : Object {
@@ -114,11 +178,24 @@ struct Model : Object {
   std::vector<BNTransform> xfms;
   bnamd_world *world = nullptr;
   bnamd_volume *volume = nullptr;
+  bnamd_umesh *umesh = nullptr;
+  bnamd_amr *amr = nullptr;
+  bnamd_envmap *env = nullptr;
   std::vector<std::unique_ptr<DevMem>> mem;
-  ~Model() override {
+  void release() {
     if (world) (void)bnamd_world_destroy(world);
     if (volume) (void)bnamd_volume_destroy(volume);
+    if (umesh) (void)bnamd_umesh_destroy(umesh);
+    if (amr) (void)bnamd_amr_destroy(amr);
+    if (env) (void)bnamd_envmap_destroy(env);
+    world = nullptr;
+    volume = nullptr;
+    umesh = nullptr;
+    amr = nullptr;
+    env = nullptr;
+    mem.clear();
   }
+  ~Model() override { release(); }
 };
 struct FrameBuffer : Object {
   using Object::Object;
@@ -227,9 +304,39 @@ BN_API void bnSetData(BNObject target, const char *paramName, BNData value) {
 BN_API void bnSetObject(BNObject target, const char *paramName, const BNObject value) {
   obj(target)->o[paramName] = value ? obj(value) : nullptr;
 }
+BN_API void bnSetString(BNObject target, const char *paramName, const char *value) {
+  obj(target)->str[paramName] = value ? value : "";
+}
+BN_API void bnSetLight(BNObject target, const char *paramName, BNLight value) {
+  obj(target)->o[paramName] = value ? obj(value) : nullptr;
+}
 BN_API void bnSet1i(BNObject target, const char *paramName, int value) { obj(target)->i[paramName] = value; }
+BN_API void bnSet2i(BNObject target, const char *paramName, int x, int y) {
+  obj(target)->iv[paramName] = {x, y, 0, 0};
+}
+BN_API void bnSet3i(BNObject target, const char *paramName, int x, int y, int z) {
+  obj(target)->iv[paramName] = {x, y, z, 0};
+}
+BN_API void bnSet4i(BNObject target, const char *paramName, int x, int y, int z, int w) {
+  obj(target)->iv[paramName] = {x, y, z, w};
+}
 BN_API void bnSet1f(BNObject target, const char *paramName, float value) {
   obj(target)->f[paramName] = {value, 0.f, 0.f, 0.f};
+}
+BN_API void bnSet2f(BNObject target, const char *paramName, float x, float y) {
+  obj(target)->f[paramName] = {x, y, 0.f, 0.f};
+}
+BN_API void bnSet4x3fv(BNObject target, const char *paramName, const BNTransform *x) {
+  if (!x) throw std::runtime_error("bnSet4x3fv: NULL matrix");
+  obj(target)->mat[paramName] = {x->l.vx.x, x->l.vx.y, x->l.vx.z, x->l.vy.x, x->l.vy.y, x->l.vy.z,
+                                 x->l.vz.x, x->l.vz.y, x->l.vz.z, x->p.x,    x->p.y,    x->p.z,
+                                 0.f,       0.f,       0.f,       1.f};
+}
+BN_API void bnSet4x4fv(BNObject target, const char *paramName, const bn_float4 *x) {
+  if (!x) throw std::runtime_error("bnSet4x4fv: NULL matrix");
+  std::array<float, 16> m{};
+  std::memcpy(m.data(), x, sizeof m);
+  obj(target)->mat[paramName] = m;
 }
 BN_API void bnSet3f(BNObject target, const char *paramName, float x, float y, float z) {
   obj(target)->f[paramName] = {x, y, z, 0.f};
@@ -252,6 +359,29 @@ BN_API BNGeom bnGeometryCreate(BNContext context, int whichSlot, const char *typ
   return static_cast<BNGeom>(static_cast<_BNObject *>(c->add(new Object(c, t))));
 }
 
+// Light::create (barney/light/Light.cpp:22-35)
+BN_API BNLight bnLightCreate(BNContext context, int whichSlot, const char *type) {
+  slot0(whichSlot);
+  const std::string t = type ? type : "";
+  if (t != "envmap" && t != "directional" && t != "point" && t != "quad")
+    throw std::runtime_error("bnLightCreate: unknown type " + t);
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNLight>(static_cast<_BNObject *>(c->add(new Object(c, t))));
+}
+
+BN_API BNTexture2D bnTexture2DCreate(BNContext context, int whichSlot, BNDataType texelFormat, uint32_t size_x,
+                                     uint32_t size_y, const void *texels, BNTextureFilterMode filterMode,
+                                     BNTextureAddressMode addressMode_x, BNTextureAddressMode addressMode_y,
+                                     BNTextureColorSpace colorSpace) {
+  slot0(whichSlot);
+  if (colorSpace != BN_COLOR_SPACE_LINEAR)
+    throw std::runtime_error("bnTexture2DCreate: only the linear color space");
+  auto *c = reinterpret_cast<Context *>(context);
+  return static_cast<BNTexture2D>(static_cast<_BNObject *>(
+      c->add(new Texture2D(c, texelFormat, (int)size_x, (int)size_y, texels, filterMode, addressMode_x,
+                           addressMode_y))));
+}
+
 BN_API BNMaterial bnMaterialCreate(BNContext context, int whichSlot, const char *type) {
   slot0(whichSlot);
   auto *c = reinterpret_cast<Context *>(context);
@@ -270,7 +400,8 @@ BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BND
 BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type) {
   slot0(whichSlot);
   const std::string t = type ? type : "";
-  if (t != "structured") throw std::runtime_error("bnScalarFieldCreate: unknown type " + t);
+  if (t != "structured" && t != "unstructured" && t != "BlockStructuredAMR")
+    throw std::runtime_error("bnScalarFieldCreate: unknown type " + t);
   auto *c = reinterpret_cast<Context *>(context);
   return static_cast<BNScalarField>(static_cast<_BNObject *>(c->add(new Object(c, t))));
 }
@@ -315,36 +446,22 @@ BN_API void bnSetInstances(BNModel model, int whichSlot, BNGroup *groups, BNTran
   }
 }
 
-BN_API void bnBuild(BNModel model, int whichDataSlot) {
-  slot0(whichDataSlot);
-  auto *m = static_cast<Model *>(obj(model));
-  if (m->world) {
-    check(bnamd_world_destroy(m->world), "bnBuild");
-    m->world = nullptr;
-  }
-  if (m->volume) {
-    check(bnamd_volume_destroy(m->volume), "bnBuild");
-    m->volume = nullptr;
-  }
-  m->mem.clear();
-  // a volume model: one structured volume (StructuredData::commit and
-  // buildMCs with barney's 8 cells per macrocell, StructuredData.cu:25-95)
-  size_t nvol = 0, ngeom = 0;
-  const VolumeObj *vol = nullptr;
-  for (const Group *g : m->groups) {
-    nvol += g->volumes.size();
-    ngeom += g->geoms.size();
-    if (!g->volumes.empty()) vol = g->volumes[0];
-  }
-  if (nvol > 0) {
-    if (nvol != 1 || ngeom != 0 || m->groups.size() != 1)
-      throw std::runtime_error("bnBuild: a model holds either geometry or one volume in this backend");
-    const Object *f = vol->field;
+namespace {
+
+// the structured, unstructured or block-structured field of a volume model
+void build_volume(Model *m, const VolumeObj *vol) {
+  const Object *f = vol->field;
+  auto upload = [&](const void *p, size_t bytes) {
+    m->mem.emplace_back(new DevMem(p, bytes));
+    return m->mem.back()->ptr;
+  };
+  if (f->type == "structured") {
+    // StructuredData::commit and buildMCs with barney's 8 cells per
+    // macrocell (StructuredData.cu:25-95)
     const TexData *td = dynamic_cast<const TexData *>(f->geto("textureData"));
     if (!td) throw std::runtime_error("bnBuild: structured field without 'textureData'");
-    m->mem.emplace_back(new DevMem(td->texels.data(), td->texels.size() * 4));
     bnamd_structured_desc d{};
-    d.scalars = (const float *)m->mem.back()->ptr;
+    d.scalars = (const float *)upload(td->texels.data(), td->texels.size() * 4);
     const auto o = f->get4("gridOrigin", {0.f, 0.f, 0.f, 0.f});
     const auto sp = f->get4("gridSpacing", {1.f, 1.f, 1.f, 0.f});
     for (int k = 0; k < 3; ++k) {
@@ -357,6 +474,109 @@ BN_API void bnBuild(BNModel model, int whichDataSlot) {
     check(bnamd_volume_set_tf(m->volume, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
                               vol->densityAt1),
           "bnBuild");
+  } else if (f->type == "unstructured") {
+    // UMeshField::setData / commit (UMeshField.cu:204-236)
+    if (f->geto("cell.data")) throw std::runtime_error("bnBuild: per-cell scalars ('cell.data') are not supported");
+    const Data *v = data_param(f, "vertex.position", BN_FLOAT32_VEC3, true);
+    const Data *sc = data_param(f, "vertex.data", BN_FLOAT32, true);
+    const Data *ix = data_param(f, "index", BN_INT32, true);
+    const Data *co = data_param(f, "cell.index", BN_INT32, true);
+    const Data *ct = data_param(f, "cell.type", BN_UINT8, true);
+    if (sc->count != v->count || ct->count != co->count)
+      throw std::runtime_error("bnBuild: unstructured field arrays of mismatched sizes");
+    bnamd_umesh_desc d{};
+    d.vertices = (const float *)upload(v->bytes.data(), v->bytes.size());
+    d.scalars = (const float *)upload(sc->bytes.data(), sc->bytes.size());
+    d.numVertices = (int)v->count;
+    d.indices = (const int *)upload(ix->bytes.data(), ix->bytes.size());
+    d.numIndices = (int)ix->count;
+    d.cellOffsets = (const int *)upload(co->bytes.data(), co->bytes.size());
+    d.cellTypes = (const unsigned char *)upload(ct->bytes.data(), ct->bytes.size());
+    d.numCells = (int)co->count;
+    check(bnamd_umesh_create(&d, nullptr, &m->umesh), "bnBuild");
+    check(bnamd_umesh_set_tf(m->umesh, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
+                             vol->densityAt1),
+          "bnBuild");
+  } else {
+    // BlockStructuredField::setData / commit (BlockStructuredField.cu:225-300)
+    const Data *orig = data_param(f, "grid.origins", BN_INT32_VEC3, true);
+    const Data *dims = data_param(f, "grid.dims", BN_INT32_VEC3, true);
+    const Data *lev = data_param(f, "grid.levels", BN_INT32, true);
+    const Data *sc = data_param(f, "scalars", BN_FLOAT32, true);
+    const Data *ref = data_param(f, "level.refinements", BN_INT32, true);
+    const Object *offo = f->geto("grid.offsets");
+    const Data *off = dynamic_cast<const Data *>(offo);
+    if (!off) throw std::runtime_error("bnBuild: BlockStructuredAMR field without 'grid.offsets'");
+    const std::vector<unsigned long long> offsets = off->as_u64();
+    const size_t nb = orig->count;
+    if (dims->count != nb || lev->count != nb || offsets.size() != nb)
+      throw std::runtime_error("bnBuild: BlockStructuredAMR arrays of mismatched sizes");
+    bnamd_amr_desc d{};
+    d.numBlocks = (int)nb;
+    d.origins = (const int *)orig->bytes.data();
+    d.dims = (const int *)dims->bytes.data();
+    d.levels = (const int *)lev->bytes.data();
+    d.offsets = offsets.data();
+    d.numLevels = (int)ref->count;
+    d.refinements = (const int *)ref->bytes.data();
+    d.scalars = (const float *)upload(sc->bytes.data(), sc->bytes.size());
+    d.numScalars = sc->count;
+    check(bnamd_amr_create(&d, nullptr, &m->amr), "bnBuild");
+    check(bnamd_amr_set_tf(m->amr, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
+                           vol->densityAt1),
+          "bnBuild");
+  }
+}
+
+// the env-map light of the model's groups (the last one met, as
+// ModelSlot::updateWorldLightsFromInstances keeps it); EnvMapLight::commit
+// with its "texture", "direction", "up", "scale" (EnvMap.cu:157-171, 280-330)
+void build_lights(Model *m) {
+  const Object *env = nullptr;
+  for (const Group *g : m->groups) {
+    const Data *lights = dynamic_cast<const Data *>(g->geto("lights"));
+    if (!lights) continue;
+    if (lights->dataType != BN_OBJECT) throw std::runtime_error("bnBuild: 'lights' must be BN_OBJECT data");
+    for (const Object *l : lights->objs) {
+      if (!l) continue;
+      if (l->type != "envmap")
+        throw std::runtime_error("bnBuild: this backend renders the environment light only ('" + l->type +
+                                 "' lights are not supported)");
+      env = l;
+    }
+  }
+  if (!env) return;
+  auto *tex = dynamic_cast<Texture2D *>(env->geto("texture"));
+  if (!tex) throw std::runtime_error("bnBuild: envmap light without a 2D 'texture'");
+  bnamd_envmap_desc d{};
+  const auto dir = env->get4("direction", {1.f, 0.f, 0.f, 0.f});
+  const auto up = env->get4("up", {0.f, 0.f, 1.f, 0.f});
+  for (int k = 0; k < 3; ++k) {
+    d.direction[k] = dir[k];
+    d.up[k] = up[k];
+  }
+  d.scale = env->getf("scale", 1.f);
+  check(bnamd_envmap_create(tex->device(), &d, nullptr, &m->env), "bnBuild");
+}
+
+}  // namespace
+
+BN_API void bnBuild(BNModel model, int whichDataSlot) {
+  slot0(whichDataSlot);
+  auto *m = static_cast<Model *>(obj(model));
+  m->release();
+  build_lights(m);
+  size_t nvol = 0, ngeom = 0;
+  const VolumeObj *vol = nullptr;
+  for (const Group *g : m->groups) {
+    nvol += g->volumes.size();
+    ngeom += g->geoms.size();
+    if (!g->volumes.empty()) vol = g->volumes[0];
+  }
+  if (nvol > 0) {
+    if (nvol != 1 || ngeom != 0 || m->groups.size() != 1)
+      throw std::runtime_error("bnBuild: a model holds either geometry or one volume in this backend");
+    build_volume(m, vol);
     return;
   }
   std::vector<Group *> groups;  // distinct groups, in first-use order
@@ -398,18 +618,24 @@ BN_API void bnBuild(BNModel model, int whichDataSlot) {
       if (g->type == "triangles") {  // Triangles.cpp:29-40
         const Data *v = data_param(g, "vertices", BN_FLOAT32_VEC3, true);
         const Data *ix = data_param(g, "indices", BN_INT32_VEC3, true);
+        const Data *nr = data_param(g, "normals", BN_FLOAT32_VEC3, false);  // Triangles.cpp:37-40
+        if (nr && nr->count != v->count) throw std::runtime_error("bnBuild: 'normals' needs one per vertex");
         e.kind = BNAMD_GEOM_TRIANGLES;
         e.numPrims = (int)ix->count;
         e.numVertices = (int)v->count;
         e.vertices = (const float *)upload(v);
         e.indices = (const int *)upload(ix);
+        e.normals = (const float *)upload(nr);
       } else if (g->type == "spheres") {  // Spheres.cpp:48-70, default radius .1
         const Data *v = data_param(g, "origins", BN_FLOAT32_VEC3, true);
         const Data *r = data_param(g, "radii", BN_FLOAT32, false);
+        const Data *col = data_param(g, "colors", BN_FLOAT32_VEC3, false);  // Spheres.cpp:59-62
+        if (col && col->count != v->count) throw std::runtime_error("bnBuild: 'colors' needs one per sphere");
         e.kind = BNAMD_GEOM_SPHERES;
         e.numPrims = e.numVertices = (int)v->count;
         e.vertices = (const float *)upload(v);
         e.radii = (const float *)upload(r);
+        e.colors = (const float *)upload(col);
         e.defaultRadius = g->getf("radius", .1f);
       } else {  // cylinders: Cylinders.cpp:60-71
         const Data *v = data_param(g, "vertices", BN_FLOAT32_VEC3, true);
@@ -459,7 +685,7 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   const Object *r = obj(renderer), *cam = obj(camera);
   auto *m = static_cast<Model *>(obj(model));
   auto *fb = static_cast<FrameBuffer *>(obj(fbh));
-  if (!m->world && !m->volume) throw std::runtime_error("bnRender: model not built (bnBuild)");
+  if (!m->world && !m->volume && !m->umesh && !m->amr) throw std::runtime_error("bnRender: model not built (bnBuild)");
   if (!fb->accum) throw std::runtime_error("bnRender: frame buffer not sized (bnFrameBufferResize)");
   // PerspectiveCamera parameters (Camera.cu:30-80)
   bnamd_camera c{};
@@ -484,8 +710,11 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   p.maxPathDepth = 10;
   p.maxDiffuseBounces = 1;
   p.russianRoulette = 0;
+  p.envMap = m->env;
+  const auto cut = r->get4("cutPlane", {0.f, 0.f, 0.f, -1e30f});  // Renderer.cpp:95-98
+  for (int k = 0; k < 4; ++k) p.cutPlane[k] = cut[k];
   int *ids = (fb->ids && fb->paths == 0) ? (int *)fb->ids->ptr : nullptr;
-  if (m->volume) {
+  if (!m->world) {
     // the volume renderer writes no ID channels: a volume model reads back
     // as all misses (primID/instID/objID -1, depth +inf)
     if (ids) {
@@ -495,7 +724,15 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
       for (size_t q = 0; q < n; ++q) std::memcpy(&miss[4 * q + 3], &inf, 4);
       check(bnamd_memcpy_h2d(ids, miss.data(), miss.size() * 4), "bnRender");
     }
-    check(bnamd_volume_render(m->volume, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+    if (m->volume)
+      check(bnamd_volume_render(m->volume, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+    else if (m->amr)
+      check(bnamd_amr_render(m->amr, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+    else {
+      bnamd_umesh_render_opts o{};
+      o.volume = 1;
+      check(bnamd_umesh_render(m->umesh, &c, &p, &o, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+    }
   } else
     check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
   fb->paths += p.pathsPerPixel;

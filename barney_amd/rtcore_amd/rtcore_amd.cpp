@@ -54,6 +54,7 @@ TextureData::TextureData(Device *device, vec3i dims, DataType format, const void
   numBytes = texel_bytes(format) * (size_t)dims.x * dims.y * (dims.z > 0 ? dims.z : 1);
   check(bnamd_device_malloc(&d_texels, numBytes), "rtc::amd::TextureData");
   if (texels) check(bnamd_memcpy_h2d(d_texels, texels, numBytes), "rtc::amd::TextureData");
+  if (dims.z == 0 && texels) hostTexels.assign((const unsigned char *)texels, (const unsigned char *)texels + numBytes);
 }
 
 TextureData::~TextureData() {
@@ -68,6 +69,42 @@ Texture::Texture(TextureData *data, const TextureDesc &desc) : device(data->devi
   if (data->dims.z > 0 && (desc.addressMode[0] != CLAMP || desc.addressMode[1] != CLAMP ||
                            desc.addressMode[2] != CLAMP || desc.normalizedCoords))
     throw std::runtime_error("rtc::amd::Texture: 3D textures are sampled with CLAMP, unnormalized coordinates");
+  if (data->dims.z == 0) {
+    // 2D images: barney creates them with normalized coordinates
+    // (barney/common/Texture.cpp:98-101), which is what tex2D samples
+    if (!desc.normalizedCoords)
+      throw std::runtime_error("rtc::amd::Texture: 2D textures are sampled with normalized coordinates");
+    if (data->hostTexels.empty()) throw std::runtime_error("rtc::amd::Texture: 2D texture without texels");
+    bnamd_texture2d_desc td{};
+    td.width = data->dims.x;
+    td.height = data->dims.y;
+    td.format = (int)data->format;
+    td.texels = data->hostTexels.data();
+    td.filterMode = (int)desc.filterMode;
+    td.addressMode[0] = (int)desc.addressMode[0];
+    td.addressMode[1] = (int)desc.addressMode[1];
+    check(bnamd_texture2d_create(&td, nullptr, &tex2d), "rtc::amd::Texture");
+  }
+}
+
+Texture::~Texture() {
+  if (tex2d) (void)bnamd_texture2d_destroy(tex2d);
+}
+
+EnvMapLight::EnvMapLight(Texture *texture, const float direction[3], const float up[3], float scale)
+    : texture(texture) {
+  if (!texture || !texture->tex2d) throw std::runtime_error("rtc::amd::EnvMapLight: needs a 2D texture");
+  bnamd_envmap_desc d{};
+  for (int k = 0; k < 3; ++k) {
+    d.direction[k] = direction[k];
+    d.up[k] = up[k];
+  }
+  d.scale = scale;
+  check(bnamd_envmap_create(texture->tex2d, &d, nullptr, &handle), "rtc::amd::EnvMapLight");
+}
+
+EnvMapLight::~EnvMapLight() {
+  if (handle) (void)bnamd_envmap_destroy(handle);
 }
 
 // -------------------------------------------------------- GeomType / Geom
@@ -321,6 +358,12 @@ TextureData *Device::createTextureData(vec3i dims, DataType format, const void *
 void Device::freeTextureData(TextureData *t) { delete t; }
 
 void Device::freeTexture(Texture *t) { delete t; }
+
+EnvMapLight *Device::createEnvMapLight(Texture *texture, const float direction[3], const float up[3], float scale) {
+  return new EnvMapLight(texture, direction, up, scale);
+}
+
+void Device::freeEnvMapLight(EnvMapLight *l) { delete l; }
 
 AccelHandle getAccelHandle(Group *ig) {
   auto *g = dynamic_cast<InstanceGroup *>(ig);

@@ -142,8 +142,9 @@ struct Buffer {
 struct Texture;
 
 // Texel data (rtcore/cudaCommon/TextureData.h:15-31): 3D scalar fields
-// (dims.z > 0, structured volumes) and 2D images (dims.z == 0). Filtering
-// is done by the renderers' own samplers, so this is plain device memory.
+// (dims.z > 0, structured volumes) and 2D images (dims.z == 0). A 3D
+// field's samplers read the device texels directly; a 2D image keeps a host
+// copy from which each Texture builds its sampled bnamd_texture2d.
 struct TextureData {
   TextureData(Device *device, vec3i dims, DataType format, const void *texels);
   ~TextureData();
@@ -153,17 +154,32 @@ struct TextureData {
   const DataType format;
   void *d_texels = nullptr;
   size_t numBytes = 0;
+  std::vector<unsigned char> hostTexels;  // 2D images only
 };
 
 // A view of texel data with its sampling description
 // (rtcore/cudaCommon/Texture.h:12-24). getDD() is the TextureObject the
-// device side of barney stores: here the device address of the texels.
+// device side of barney stores: the device address of the texels for a 3D
+// field, the bnamd_texture2d handle (tex2D as the embree backend samples
+// it) for a 2D image.
 struct Texture {
   Texture(TextureData *data, const TextureDesc &desc);
-  TextureObject getDD() const { return (TextureObject)data->d_texels; }
+  ~Texture();
+  TextureObject getDD() const { return tex2d ? (TextureObject)tex2d : (TextureObject)data->d_texels; }
   Device *const device;
   TextureData *const data;
   const TextureDesc desc;
+  bnamd_texture2d *tex2d = nullptr;
+};
+
+// barney's EnvMapLight (barney/light/EnvMap.cu:59-230) over a 2D texture:
+// commit() becomes the constructor (frame from direction / up, importance
+// CDFs); handle goes into bnamd_render_params.envMap for the renders.
+struct EnvMapLight {
+  EnvMapLight(Texture *texture, const float direction[3], const float up[3], float scale);
+  ~EnvMapLight();
+  Texture *const texture;
+  bnamd_envmap *handle = nullptr;
 };
 
 enum class GeomKind { Triangles = BNAMD_GEOM_TRIANGLES, Spheres = BNAMD_GEOM_SPHERES,
@@ -284,6 +300,8 @@ struct Device {
   TextureData *createTextureData(vec3i dims, DataType format, const void *texels);
   void freeTextureData(TextureData *);
   void freeTexture(Texture *);
+  EnvMapLight *createEnvMapLight(Texture *texture, const float direction[3], const float up[3], float scale);
+  void freeEnvMapLight(EnvMapLight *);
 
   Buffer *createBuffer(size_t numBytes, const void *initValues = nullptr);
   void freeBuffer(Buffer *);

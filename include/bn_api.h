@@ -1,15 +1,18 @@
-/* The bn* host API over this backend: the part of barney's public C API
- * (barney/include/barney.h) that surface scenes and structured volumes
- * need -- context, data arrays, triangle / sphere / cylinder geometry,
- * matte and physically-based materials, 3D float texture data, the
- * "structured" scalar field, volumes with a transfer function, groups,
- * instances, perspective camera, renderer parameters, frame buffer with
- * color / depth / ID channels, render. Names, types, parameter names and
- * enum values are barney's, so an application written against barney.h for
- * such scenes compiles and links against libbn_api.so. One device, one data
- * slot (whichSlot must be 0); a model holds either geometry or one volume
- * (this backend's renderers are separate kernels). Other object types are
- * reached through bnamd.h directly.
+/* The bn* host API over this backend: barney's public C API
+ * (barney/include/barney.h) for surface and volume scenes -- context, data
+ * arrays (including object lists), triangle / sphere / cylinder geometry
+ * (vertex normals, per-primitive colors), matte and physically-based
+ * materials (opacity), 2D textures, the environment-map light, 3D float
+ * texture data, the "structured", "unstructured" and "BlockStructuredAMR"
+ * scalar fields, volumes with a transfer function, groups with lights,
+ * instances, perspective camera, renderer parameters (including the cut
+ * plane), frame buffer with color / depth / ID channels, render, and every
+ * bnSet* setter. Names, types, parameter names and enum values are
+ * barney's, so an application written against barney.h for such scenes
+ * compiles and links against libbn_api.so. One device, one data slot
+ * (whichSlot must be 0); a model holds either geometry or one volume (this
+ * backend's renderers are separate kernels); of the light types only
+ * "envmap" is rendered (bnBuild rejects the others).
  * Objects belong to their context: bnRelease drops the application's
  * reference, storage is freed by bnContextDestroy. Errors throw
  * std::runtime_error from C++ callers' point of view, as barney does. */
@@ -34,6 +37,8 @@ typedef struct _BNCamera : public _BNObject {} *BNCamera;
 typedef struct _BNMaterial : public _BNObject {} *BNMaterial;
 typedef struct _BNTextureData : public _BNObject {} *BNTextureData;
 typedef struct _BNScalarField : public _BNObject {} *BNScalarField;
+typedef struct _BNLight : public _BNObject {} *BNLight;
+typedef struct _BNTexture2D : public _BNObject {} *BNTexture2D;
 
 struct bn_float2 { float x, y; };
 struct bn_float3 { float x, y, z; };
@@ -60,16 +65,37 @@ typedef enum {
 /* the values of barney.h:104-182 that this API accepts */
 typedef enum {
   BN_DATA_UNDEFINED = 0,
+  BN_OBJECT = 2,
+  BN_UINT8 = 110,
+  BN_UINT8_VEC4 = 113,
   BN_INT32 = 140,
   BN_INT32_VEC2 = 141,
   BN_INT32_VEC3 = 142,
+  BN_INT32_VEC4 = 143,
+  BN_UINT32 = 150,
+  BN_INT64 = 160,
+  BN_UINT64 = 170,
   BN_FLOAT32 = 180,
+  BN_FLOAT32_VEC2 = 181,
   BN_FLOAT32_VEC3 = 182,
   BN_FLOAT32_VEC4 = 183,
   BN_FLOAT4 = BN_FLOAT32_VEC4,
+  BN_UFIXED8 = 300,
   BN_UFIXED8_RGBA = 301,
   BN_UFIXED8_RGBA_SRGB = 302,
 } BNDataType;
+
+/* barney.h:185-213 */
+typedef enum {
+  BN_UNSTRUCTURED_TET = 10,
+  BN_UNSTRUCTURED_HEX = 12,
+  BN_UNSTRUCTURED_PRISM = 13,
+  BN_UNSTRUCTURED_PYRAMID = 14,
+  BN_UNSTRUCTURED_POLYHEDRON = 42
+} BNUnstructuredElementType;
+typedef enum { BN_TEXTURE_NEAREST, BN_TEXTURE_LINEAR } BNTextureFilterMode;
+typedef enum { BN_TEXTURE_WRAP, BN_TEXTURE_CLAMP, BN_TEXTURE_BORDER, BN_TEXTURE_MIRROR } BNTextureAddressMode;
+typedef enum { BN_COLOR_SPACE_LINEAR, BN_COLOR_SPACE_SRGB } BNTextureColorSpace;
 
 /* barney.h:308-323 */
 BN_API BNContext bnContextCreate(const int *dataRanksOnThisContext = 0, int numDataRanksOnThisContext = 1,
@@ -89,10 +115,18 @@ BN_API BNRenderer bnRendererCreate(BNContext ctx, const char *type = "default");
 BN_API void bnCommit(BNObject target);
 BN_API void bnSetData(BNObject target, const char *paramName, BNData value);
 BN_API void bnSetObject(BNObject target, const char *paramName, const BNObject value);
+BN_API void bnSetString(BNObject target, const char *paramName, const char *value);
+BN_API void bnSetLight(BNObject target, const char *paramName, BNLight value);
 BN_API void bnSet1i(BNObject target, const char *paramName, int value);
+BN_API void bnSet2i(BNObject target, const char *paramName, int x, int y);
+BN_API void bnSet3i(BNObject target, const char *paramName, int x, int y, int z);
+BN_API void bnSet4i(BNObject target, const char *paramName, int x, int y, int z, int w);
 BN_API void bnSet1f(BNObject target, const char *paramName, float value);
+BN_API void bnSet2f(BNObject target, const char *paramName, float x, float y);
 BN_API void bnSet3f(BNObject target, const char *paramName, float x, float y, float z);
 BN_API void bnSet4f(BNObject target, const char *paramName, float x, float y, float z, float w);
+BN_API void bnSet4x3fv(BNObject target, const char *paramName, const BNTransform *affineMatrix);
+BN_API void bnSet4x4fv(BNObject target, const char *paramName, const bn_float4 *xfm);
 
 /* barney.h:346-400 */
 BN_API void bnBuild(BNModel model, int whichDataSlot);
@@ -114,9 +148,25 @@ BN_API void bnGroupBuild(BNGroup group);
 BN_API BNGeom bnGeometryCreate(BNContext context, int whichSlot, const char *type);
 BN_API BNMaterial bnMaterialCreate(BNContext context, int whichSlot, const char *type);
 
+/* barney.h:449-475: lights ("envmap" with "texture", "direction", "up",
+ * "scale"; attached to a group as bnSetData(group, "lights", BN_OBJECT
+ * data)) and 2D textures (texelFormat BN_FLOAT32_VEC4, BN_FLOAT32,
+ * BN_UFIXED8_RGBA or BN_UFIXED8; sampled as barney's embree backend does) */
+BN_API BNLight bnLightCreate(BNContext context, int whichSlot, const char *type);
+BN_API BNTexture2D bnTexture2DCreate(BNContext context, int whichSlot, BNDataType texelFormat, uint32_t size_x,
+                                     uint32_t size_y, const void *texels,
+                                     BNTextureFilterMode filterMode = BN_TEXTURE_LINEAR,
+                                     BNTextureAddressMode addressMode_x = BN_TEXTURE_WRAP,
+                                     BNTextureAddressMode addressMode_y = BN_TEXTURE_WRAP,
+                                     BNTextureColorSpace colorSpace = BN_COLOR_SPACE_LINEAR);
+
 /* barney.h:440-446, 507-519: volumes. texelFormat BN_FLOAT32; the
  * "structured" field takes "textureData", "gridOrigin", "gridSpacing"
- * (StructuredData.cu:130-160) */
+ * (StructuredData.cu:130-160); "unstructured" takes "vertex.position",
+ * "vertex.data", "index", "cell.index", "cell.type" (UMeshField.cu:204-236;
+ * per-vertex scalars); "BlockStructuredAMR" takes "grid.origins",
+ * "grid.dims", "grid.levels", "grid.offsets", "scalars",
+ * "level.refinements" (BlockStructuredField.cu:225-256) */
 BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BNDataType texelFormat, int width,
                                            int height, int depth, const void *items);
 BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type);

@@ -241,6 +241,219 @@ int main() {
     bnamd_volume_destroy(v);
     bnamd_device_free(dfield);
   }
+  // environment-map light over a 2D texture, the renderer's cut plane,
+  // triangle vertex normals, per-sphere colors and a cutout (opacity .5)
+  // material through bn* equal the same through the C ABI
+  {
+    const int TW = 16, TH = 8;
+    std::vector<float> texels(4 * TW * TH);
+    for (int i = 0; i < TW * TH; ++i) {
+      texels[4 * i] = 0.3f + 0.05f * (i % 7);
+      texels[4 * i + 1] = 0.2f + 0.1f * ((i / TW) % 3);
+      texels[4 * i + 2] = (i == 37) ? 30.f : 0.6f;
+      texels[4 * i + 3] = 1.f;
+    }
+    const float nrm[12] = {0.1f, 1, 0, 0, 1, 0.1f, -0.1f, 1, 0, 0, 1, -0.1f};
+    const float scol[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    BNTexture2D tex = bnTexture2DCreate(ctx, 0, BN_FLOAT32_VEC4, TW, TH, texels.data(), BN_TEXTURE_LINEAR,
+                                        BN_TEXTURE_WRAP, BN_TEXTURE_CLAMP);
+    BNLight env = bnLightCreate(ctx, 0, "envmap");
+    bnSetObject(env, "texture", tex);
+    bnSet3f(env, "direction", 0.f, 0.f, 1.f);
+    bnSet3f(env, "up", 0.f, 1.f, 0.f);
+    bnSet1f(env, "scale", 1.5f);
+    bnCommit(env);
+    BNMaterial cut = bnMaterialCreate(ctx, 0, "matte");
+    bnSet3f(cut, "color", .7f, .6f, .5f);
+    bnSet1f(cut, "opacity", .5f);
+    BNGeom t2 = bnGeometryCreate(ctx, 0, "triangles");
+    bnSetData(t2, "vertices", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 4, tv));
+    bnSetData(t2, "indices", bnDataCreate(ctx, 0, BN_INT32_VEC3, 2, ti));
+    bnSetData(t2, "normals", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 4, nrm));
+    bnSetObject(t2, "material", cut);
+    bnSet1i(t2, "userID", 7);
+    BNGeom s2 = bnGeometryCreate(ctx, 0, "spheres");
+    bnSetData(s2, "origins", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 3, so));
+    bnSetData(s2, "radii", bnDataCreate(ctx, 0, BN_FLOAT32, 3, sr));
+    bnSetData(s2, "colors", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 3, scol));
+    bnSetObject(s2, "material", floorMat);
+    bnSet1i(s2, "userID", 9);
+    BNGeom g2[2] = {t2, s2};
+    BNGroup lg = bnGroupCreate(ctx, 0, g2, 2, nullptr, 0);
+    BNObject lights[1] = {env};
+    bnSetData(lg, "lights", bnDataCreate(ctx, 0, BN_OBJECT, 1, lights));
+    BNModel lm = bnModelCreate(ctx);
+    bnSetInstances(lm, 0, &lg, &xfm, 1);
+    bnBuild(lm, 0);
+    BNRenderer r2 = bnRendererCreate(ctx, "default");
+    bnSet1i(r2, "pathsPerPixel", SPP);
+    bnSet4f(r2, "bgColor", 0.1f, 0.2f, 0.3f, 1.f);
+    bnSet4f(r2, "cutPlane", 0.2f, 0.f, 1.f, 0.5f);
+    // setters this backend stores without using: callable, no effect
+    bnSet2i(r2, "crosshairs", 1, 2);
+    bnSet3i(r2, "unused3i", 1, 2, 3);
+    bnSet4i(r2, "unused4i", 1, 2, 3, 4);
+    bnSet2f(r2, "unused2f", 1.f, 2.f);
+    bnSetString(r2, "name", "renderer");
+    bnSet4x3fv(r2, "unusedXfm", &xfm);
+    bnAccumReset(fb);
+    bnRender(r2, lm, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    bnFrameBufferRead(fb, BN_FB_PRIMID, prim.data(), BN_INT32);
+    // the C ABI: the same texture, light, geometry and parameters
+    bnamd_texture2d_desc tdsc{};
+    tdsc.width = TW; tdsc.height = TH; tdsc.format = 33; tdsc.texels = texels.data();
+    tdsc.filterMode = 1; tdsc.addressMode[0] = 0; tdsc.addressMode[1] = 1;
+    bnamd_texture2d *t2d = nullptr;
+    if (bnamd_texture2d_create(&tdsc, nullptr, &t2d) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_envmap_desc edsc = {{0.f, 0.f, 1.f}, {0.f, 1.f, 0.f}, 1.5f};
+    bnamd_envmap *em = nullptr;
+    if (bnamd_envmap_create(t2d, &edsc, nullptr, &em) != BNAMD_OK) return fail(bnamd_last_error());
+    void *dn, *dc;
+    bnamd_device_malloc(&dn, sizeof nrm);
+    bnamd_device_malloc(&dc, sizeof scol);
+    bnamd_memcpy_h2d(dn, nrm, sizeof nrm);
+    bnamd_memcpy_h2d(dc, scol, sizeof scol);
+    bnamd_geom gl[2] = {g[0], g[1]};
+    gl[0].normals = (const float *)dn;
+    gl[1].colors = (const float *)dc;
+    gl[1].materialID = 0;
+    bnamd_group lgrp = {gl, 2};
+    bnamd_material ml[1] = {m[0]};
+    ml[0].opacity = .5f;
+    bnamd_world *lw = nullptr;
+    if (bnamd_world_build(&lgrp, 1, &in, 1, ml, 1, nullptr, &lw) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_render_params pl = p;
+    pl.accumID = 0;
+    pl.ambientRadiance = 1.f;
+    pl.envMap = em;
+    pl.cutPlane[0] = 0.2f; pl.cutPlane[1] = 0.f; pl.cutPlane[2] = 1.f; pl.cutPlane[3] = 0.5f;
+    if (bnamd_render_surface(lw, &c, &pl, (float *)acc, (int *)ids, nullptr, nullptr) != BNAMD_OK)
+      return fail("env render");
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("env-map render differs");
+    // the cut plane removes part of the scene: fewer hits than without it
+    int cutHits = 0;
+    for (int q = 0; q < W * H; ++q) cutHits += prim[q] >= 0;
+    std::printf("env-map + cut plane: %d hits (without the plane %d)\n", cutHits, hits);
+    if (cutHits <= 0 || cutHits >= hits) return fail("cut plane has no effect");
+    bnamd_world_destroy(lw);
+    bnamd_envmap_destroy(em);
+    bnamd_texture2d_destroy(t2d);
+    bnamd_device_free(dn);
+    bnamd_device_free(dc);
+    // lights other than the env map are not rendered: bnBuild says so
+    BNLight dl = bnLightCreate(ctx, 0, "directional");
+    BNObject dls[1] = {dl};
+    bnSetData(lg, "lights", bnDataCreate(ctx, 0, BN_OBJECT, 1, dls));
+    bool thr = false;
+    try {
+      bnBuild(lm, 0);
+    } catch (const std::runtime_error &) {
+      thr = true;
+    }
+    if (!thr) return fail("directional light accepted");
+  }
+  // an unstructured field (two hexahedra and a tetrahedron) and a
+  // block-structured AMR field through bn* equal bnamd_umesh / bnamd_amr
+  {
+    const float uv[] = {-1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1,
+                        3, -1, -1, 3, 1, -1, 3, -1, 1, 3, 1, 1, 1, 3, 0};
+    const float us[] = {0.1f, 0.5f, 0.9f, 0.3f, 0.2f, 0.8f, 1.f, 0.4f, 0.6f, 0.7f, 0.3f, 0.95f, 0.5f};
+    const int ui[] = {0, 1, 2, 3, 4, 5, 6, 7, 1, 8, 9, 2, 5, 10, 11, 6, 2, 3, 6, 12};
+    const int uo[] = {0, 8, 16};
+    const uint8_t ut[] = {BN_UNSTRUCTURED_HEX, BN_UNSTRUCTURED_HEX, BN_UNSTRUCTURED_TET};
+    const bn_float4 xf[3] = {{0.f, 0.f, 1.f, 0.f}, {1.f, .5f, 0.f, .5f}, {1.f, 1.f, 1.f, 1.f}};
+    BNScalarField uf = bnScalarFieldCreate(ctx, 0, "unstructured");
+    bnSetData(uf, "vertex.position", bnDataCreate(ctx, 0, BN_FLOAT32_VEC3, 13, uv));
+    bnSetData(uf, "vertex.data", bnDataCreate(ctx, 0, BN_FLOAT32, 13, us));
+    bnSetData(uf, "index", bnDataCreate(ctx, 0, BN_INT32, 20, ui));
+    bnSetData(uf, "cell.index", bnDataCreate(ctx, 0, BN_INT32, 3, uo));
+    bnSetData(uf, "cell.type", bnDataCreate(ctx, 0, BN_UINT8, 3, ut));
+    BNVolume uvol = bnVolumeCreate(ctx, 0, uf);
+    bnVolumeSetXF(uvol, {0.f, 1.f}, xf, 3, 2.f);
+    BNGroup ug = bnGroupCreate(ctx, 0, nullptr, 0, &uvol, 1);
+    BNModel um = bnModelCreate(ctx);
+    bnSetInstances(um, 0, &ug, &xfm, 1);
+    bnBuild(um, 0);
+    bnAccumReset(fb);
+    bnRender(rnd, um, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    void *b0, *b1, *b2, *b3, *b4;
+    bnamd_device_malloc(&b0, sizeof uv);
+    bnamd_device_malloc(&b1, sizeof us);
+    bnamd_device_malloc(&b2, sizeof ui);
+    bnamd_device_malloc(&b3, sizeof uo);
+    bnamd_device_malloc(&b4, sizeof ut);
+    bnamd_memcpy_h2d(b0, uv, sizeof uv);
+    bnamd_memcpy_h2d(b1, us, sizeof us);
+    bnamd_memcpy_h2d(b2, ui, sizeof ui);
+    bnamd_memcpy_h2d(b3, uo, sizeof uo);
+    bnamd_memcpy_h2d(b4, ut, sizeof ut);
+    bnamd_umesh_desc ud{};
+    ud.vertices = (const float *)b0; ud.scalars = (const float *)b1; ud.numVertices = 13;
+    ud.indices = (const int *)b2; ud.numIndices = 20; ud.cellOffsets = (const int *)b3;
+    ud.cellTypes = (const unsigned char *)b4; ud.numCells = 3;
+    bnamd_umesh *mesh = nullptr;
+    if (bnamd_umesh_create(&ud, nullptr, &mesh) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_umesh_set_tf(mesh, (const float *)xf, 3, 0.f, 1.f, 2.f);
+    bnamd_umesh_render_opts uo2{};
+    uo2.volume = 1;
+    p.accumID = 0;
+    bnamd_volume_stats us2;
+    if (bnamd_umesh_render(mesh, &c, &p, &uo2, (float *)acc, nullptr, &us2) != BNAMD_OK) return fail("umesh render");
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("unstructured color differs");
+    if (us2.samples <= 0) return fail("unstructured field not sampled");
+    std::printf("unstructured samples %lld\n", us2.samples);
+    bnamd_umesh_destroy(mesh);
+    for (void *b : {b0, b1, b2, b3, b4}) bnamd_device_free(b);
+    // AMR: a 4^3 level-0 block and a 4^3 level-1 block inside it
+    const int ao[] = {0, 0, 0, 2, 2, 2};
+    const int ad[] = {4, 4, 4, 4, 4, 4};
+    const int al[] = {0, 1};
+    const uint64_t aoff[] = {0, 64};
+    const int aref[] = {2, 2};
+    std::vector<float> asc(128);
+    for (int i = 0; i < 128; ++i) asc[i] = 0.5f + 0.45f * std::sin(0.37f * (float)i);
+    BNScalarField af = bnScalarFieldCreate(ctx, 0, "BlockStructuredAMR");
+    bnSetData(af, "grid.origins", bnDataCreate(ctx, 0, BN_INT32_VEC3, 2, ao));
+    bnSetData(af, "grid.dims", bnDataCreate(ctx, 0, BN_INT32_VEC3, 2, ad));
+    bnSetData(af, "grid.levels", bnDataCreate(ctx, 0, BN_INT32, 2, al));
+    bnSetData(af, "grid.offsets", bnDataCreate(ctx, 0, BN_UINT64, 2, aoff));
+    bnSetData(af, "scalars", bnDataCreate(ctx, 0, BN_FLOAT32, 128, asc.data()));
+    bnSetData(af, "level.refinements", bnDataCreate(ctx, 0, BN_INT32, 2, aref));
+    BNVolume avol = bnVolumeCreate(ctx, 0, af);
+    bnVolumeSetXF(avol, {0.f, 1.f}, xf, 3, 2.f);
+    BNGroup ag = bnGroupCreate(ctx, 0, nullptr, 0, &avol, 1);
+    BNModel am = bnModelCreate(ctx);
+    bnSetInstances(am, 0, &ag, &xfm, 1);
+    bnBuild(am, 0);
+    bnAccumReset(fb);
+    bnRender(rnd, am, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    void *dsc;
+    bnamd_device_malloc(&dsc, asc.size() * 4);
+    bnamd_memcpy_h2d(dsc, asc.data(), asc.size() * 4);
+    const unsigned long long aoffu[] = {0, 64};
+    bnamd_amr_desc ad2 = {2, ao, ad, al, aoffu, 2, aref, (const float *)dsc, 128};
+    bnamd_amr *amr = nullptr;
+    if (bnamd_amr_create(&ad2, nullptr, &amr) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_amr_set_tf(amr, (const float *)xf, 3, 0.f, 1.f, 2.f);
+    bnamd_volume_stats as2;
+    if (bnamd_amr_render(amr, &c, &p, (float *)acc, nullptr, &as2) != BNAMD_OK) return fail("amr render");
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("AMR color differs");
+    std::printf("amr samples %lld\n", as2.samples);
+    bnamd_amr_destroy(amr);
+    bnamd_device_free(dsc);
+  }
   // errors surface as exceptions
   bool threw = false;
   try {

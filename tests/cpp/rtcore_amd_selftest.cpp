@@ -282,6 +282,27 @@ int main() {
     dev->freeTexture(tex);
     dev->freeTextureData(td);
   }
+  // a 2D image (normalized coordinates, WRAP x CLAMP) and the environment
+  // light over it: the light's importance CDFs end at 1
+  {
+    const vec3i dims = {8, 4, 0};
+    std::vector<float> texels(8 * 4 * 4);
+    for (size_t i = 0; i < texels.size(); ++i) texels[i] = 0.25f + 0.01f * (float)(i % 37);
+    TextureData *td = dev->createTextureData(dims, FLOAT4, texels.data());
+    TextureDesc desc;
+    desc.addressMode[0] = WRAP;
+    Texture *tex = td->createTexture(desc);
+    CHECK(tex->tex2d != nullptr && (void *)tex->getDD() == (void *)tex->tex2d);
+    const float dir[3] = {1.f, 0.f, 0.f}, up[3] = {0.f, 0.f, 1.f};
+    EnvMapLight *env = dev->createEnvMapLight(tex, dir, up, 2.f);
+    std::vector<float> cy(4), cx(32);
+    CHECK(bnamd_envmap_get_cdfs(env->handle, cy.data(), cx.data()) == 0);
+    CHECK(cy[3] == 1.f && cx[7] == 1.f && cx[31] == 1.f && cy[0] > 0.f && cy[0] < cy[1]);
+    std::printf("env map: cdf_y %g %g %g %g\n", cy[0], cy[1], cy[2], cy[3]);
+    dev->freeEnvMapLight(env);
+    dev->freeTexture(tex);
+    dev->freeTextureData(td);
+  }
 
   dev->freeMem(dRays);
   dev->freeMem(dHits);
