@@ -149,6 +149,10 @@ class World:
         check(lib.bnamd_world_build(cgroups, len(groups), cinst, len(instances), cmats, len(materials), None,
                                     C.byref(self.handle)))
 
+    def validate(self):
+        """Host-side structural check of the BVH8s (raises on a violation)."""
+        check(capi.load().bnamd_world_validate(self.handle))
+
     def info(self):
         inf = capi.bnamd_world_info()
         check(capi.load().bnamd_world_info_get(self.handle, C.byref(inf)))
@@ -305,7 +309,7 @@ def render_surface(world, cam, params, with_ids=False, accum=None):
     return out
 
 
-BATCH_MEGAKERNEL, BATCH_WAVEFRONT = 0, 1
+BATCH_MEGAKERNEL, BATCH_WAVEFRONT, BATCH_AUTO = 0, 1, 2
 
 
 class Views:

@@ -192,6 +192,7 @@ SIGNATURES = {
                                          C.c_void_p]),
     "bnamd_fb_resolve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p, C.c_void_p]),
     "bnamd_fb_to_fixed8": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "bnamd_world_validate": (C.c_int, [C.c_void_p]),
     "bnamd_texture2d_create": (C.c_int, [C.POINTER(bnamd_texture2d_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
     "bnamd_texture2d_destroy": (C.c_int, [C.c_void_p]),
     "bnamd_texture2d_sample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

@@ -213,10 +213,11 @@ __global__ void k_refit(const float4 *__restrict boxes, const uint32_t *__restri
   if (leaf >= n) return;
   int p = parent[n - 1 + leaf];
   while (p >= 0) {
-    __threadfence();
-    if (atomicAdd(&flags[p], 1u) == 0) return;  // first child to arrive stops
-    __threadfence();
-    BinNode nd = nodes[p];
+    // the first child to arrive stops; the second goes on with both boxes.
+    // acquire-release at device scope: it publishes the box this thread
+    // wrote one level down and makes the sibling's box visible here
+    if (__hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const BinNode nd = nodes[p];
     float l[3], h[3];
     for (int side = 0; side < 2; ++side) {
       const int c = side ? nd.right : nd.left;
@@ -227,17 +228,14 @@ __global__ void k_refit(const float4 *__restrict boxes, const uint32_t *__restri
         cl[0] = lo.x; cl[1] = lo.y; cl[2] = lo.z;
         ch[0] = hi.x; ch[1] = hi.y; ch[2] = hi.z;
       } else {
-        // written by another thread: read through L2
-        const volatile BinNode *vn = nodes + c;
-        for (int k = 0; k < 3; ++k) { cl[k] = vn->lo[k]; ch[k] = vn->hi[k]; }
+        for (int k = 0; k < 3; ++k) { cl[k] = nodes[c].lo[k]; ch[k] = nodes[c].hi[k]; }
       }
       for (int k = 0; k < 3; ++k) {
         l[k] = side ? fminf(l[k], cl[k]) : cl[k];
         h[k] = side ? fmaxf(h[k], ch[k]) : ch[k];
       }
     }
-    volatile BinNode *vp = nodes + p;
-    for (int k = 0; k < 3; ++k) { vp->lo[k] = l[k]; vp->hi[k] = h[k]; }
+    for (int k = 0; k < 3; ++k) { nodes[p].lo[k] = l[k]; nodes[p].hi[k] = h[k]; }
     p = parent[p];
   }
 }
@@ -463,11 +461,11 @@ bool box_inside(const float ilo[3], const float ihi[3], const float olo[3], cons
 // range, every primitive referenced exactly once, and every decoded child
 // box containing what it bounds (the primitives of a leaf slot, all slots of
 // an inner child node). Guards the traversal kernels against out-of-bounds
-// fetches and the quantization against dropping geometry; disable with
-// BNAMD_SKIP_BVH_CHECK=1 for very large scenes once trusted.
+// fetches and the quantization against dropping geometry. It copies the
+// whole tree to the host, so builds run it only with BNAMD_VALIDATE_BVH=1;
+// bnamd_world_validate runs it on demand (the tests do, on every kind of
+// world they build).
 void validate_bvh(const World &w) {
-  const char *skip = getenv("BNAMD_SKIP_BVH_CHECK");
-  if (skip && skip[0] == '1') return;
   std::vector<BNode8> nodes(w.numNodes);
   BN_HIP(hipMemcpy(nodes.data(), w.nodes.ptr, nodes.size() * sizeof(BNode8), hipMemcpyDeviceToHost));
   std::vector<PrimRec> prims(w.numPrims);
@@ -555,6 +553,8 @@ void validate_bvh(const World &w) {
 }
 
 }  // namespace
+
+void check_world_bvh(const World &w) { validate_bvh(w); }
 
 void build_world_bvh(World &w, hipStream_t stream) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -645,7 +645,8 @@ void build_world_bvh(World &w, hipStream_t stream) {
   BN_REQUIRE(hc[1] == n, "BVH8 collapse lost primitives");
   w.numNodes = hc[0];
   w.depth = depth;
-  validate_bvh(w);
+  const char *check = getenv("BNAMD_VALIDATE_BVH");
+  if (check && check[0] == '1') validate_bvh(w);
   uint32_t hb[12];
   BN_HIP(hipMemcpy(hb, bounds.ptr, sizeof(hb), hipMemcpyDeviceToHost));
   for (int k = 0; k < 3; ++k) {

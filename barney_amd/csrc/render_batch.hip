@@ -724,11 +724,7 @@ __global__ void k_fill_f4(float4 *p, size_t n) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-struct TuneChoice {
-  int traceBlocksPerCU = 0;  // trace/shadow workgroups per CU; 0 = as many as are resident
-  int chunk = 0;             // 0 = static 64-ray slices, else rays per dynamic fetch
-  int sppBatch = 0;          // 0 = all paths of the frame in one batch
-};
+using TuneChoice = BatchTune;
 
 struct BatchState {
   DevBuf<int> pixOffset, tileOffset;
@@ -740,8 +736,6 @@ struct BatchState {
   DevBuf<ShadowRec> sq;
   DevBuf<WfCounters> cnt;
   DevBuf<unsigned long long> megaCnt;
-  TuneChoice tune;
-  bool tuned = false;
   // graph cache
   hipGraphExec_t exec = nullptr;
   std::string graphKey;
@@ -854,11 +848,14 @@ void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
   BN_HIP(hipGetLastError());
 }
 
+// lanes per pixel: the tuned value, else the largest power of two <= spp
+// (up to 8); a pixel's paths run in rounds of 'lanes'
 void render_mega(const World &w, const bnamd_views *views, const RenderDD &rp, int spp, float4 *accum,
-                 int4 *ids, unsigned long long *cnt, hipStream_t s) {
-  if (spp >= 8) launch_mega<8>(w, views, rp, spp, accum, ids, cnt, s);
-  else if (spp >= 4) launch_mega<4>(w, views, rp, spp, accum, ids, cnt, s);
-  else if (spp >= 2) launch_mega<2>(w, views, rp, spp, accum, ids, cnt, s);
+                 int4 *ids, unsigned long long *cnt, hipStream_t s, int lanes = 0) {
+  if (lanes <= 0) lanes = spp >= 8 ? 8 : spp >= 4 ? 4 : spp >= 2 ? 2 : 1;
+  if (lanes >= 8) launch_mega<8>(w, views, rp, spp, accum, ids, cnt, s);
+  else if (lanes >= 4) launch_mega<4>(w, views, rp, spp, accum, ids, cnt, s);
+  else if (lanes >= 2) launch_mega<2>(w, views, rp, spp, accum, ids, cnt, s);
   else launch_mega<1>(w, views, rp, spp, accum, ids, cnt, s);
 }
 
@@ -1016,10 +1013,12 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
   return guarded([&] {
     BN_REQUIRE(world && views && params && accum, "bnamd_render_batch: NULL argument");
     BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_render_batch: pathsPerPixel must be >= 1");
-    BN_REQUIRE(mode == BNAMD_BATCH_MEGAKERNEL || mode == BNAMD_BATCH_WAVEFRONT, "bnamd_render_batch: bad mode");
+    BN_REQUIRE(mode == BNAMD_BATCH_MEGAKERNEL || mode == BNAMD_BATCH_WAVEFRONT || mode == BNAMD_BATCH_AUTO,
+               "bnamd_render_batch: bad mode");
     const World &w = world_of(world);
     hipStream_t s = (hipStream_t)stream;
     RenderDD rp = make_render_dd(*params);
+    if (mode == BNAMD_BATCH_AUTO) mode = w.tune.mode;
     if (mode == BNAMD_BATCH_MEGAKERNEL) {
       BatchState &st = state();
       st.megaCnt.ensure(4);
@@ -1028,7 +1027,8 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
       BN_HIP(hipEventCreate(&e0));
       BN_HIP(hipEventCreate(&e1));
       BN_HIP(hipEventRecord(e0, s));
-      render_mega(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, st.megaCnt.ptr, s);
+      render_mega(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, st.megaCnt.ptr, s,
+                  w.tune.megaLanes);
       BN_HIP(hipEventRecord(e1, s));
       BN_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1051,23 +1051,57 @@ extern "C" int bnamd_render_batch(const bnamd_world *world, const bnamd_views *v
     } else {
       BatchState &st = state();
       const char *ng = getenv("BNAMD_NO_GRAPH");
-      render_wavefront(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, s, stats, st.tune,
+      render_wavefront(w, views, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, s, stats, w.tune,
                        !(ng && ng[0] == '1'));
     }
   });
 }
 
+// Times the batched renderers' launch configurations on this world and views
+// and keeps the fastest in the world's BatchTune:
+//   1. wavefront trace grid (workgroups per CU) x work-fetch chunk, on one
+//      path per pixel;
+//   2. wavefront queue size: paths per pixel per batch (sppBatch), on the
+//      parameters' paths per pixel;
+//   3. megakernel tile: lanes per pixel (1, 2, 4, 8 up to spp), likewise;
+//   4. the faster of the two renderers at their best settings.
 extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views *views,
                                     const bnamd_render_params *params, const char *path) {
   return guarded([&] {
     BN_REQUIRE(world && views && params, "bnamd_batch_autotune: NULL argument");
+    BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_batch_autotune: pathsPerPixel must be >= 1");
     const World &w = world_of(world);
     BatchState &st = state();
     RenderDD rp = make_render_dd(*params);
     rp.accumID = 0;
+    const int spp = params->pathsPerPixel;
     long long px = 0;
     for (int i = 0; i < views->numViews; ++i) px += (long long)views->widths[i] * views->heights[i];
     DevBuf<float4> accum((size_t)px);
+    // the better of two runs keeps one noisy run from deciding
+    auto wave = [&](const TuneChoice &t, int n) {
+      float ms = render_wavefront(w, views, rp, n, accum.ptr, nullptr, 0, nullptr, t, false);
+      return std::min(ms, render_wavefront(w, views, rp, n, accum.ptr, nullptr, 0, nullptr, t, false));
+    };
+    st.megaCnt.ensure(4);
+    auto mega = [&](int lanes) {
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        hipEvent_t e0, e1;
+        BN_HIP(hipEventCreate(&e0));
+        BN_HIP(hipEventCreate(&e1));
+        BN_HIP(hipEventRecord(e0, 0));
+        render_mega(w, views, rp, spp, accum.ptr, nullptr, st.megaCnt.ptr, 0, lanes);
+        BN_HIP(hipEventRecord(e1, 0));
+        BN_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        best = std::min(best, ms);
+      }
+      return best;
+    };
     const int blocks[] = {2, 3, 4, 0};
     // chunk: rays per work fetch (flattened worlds: the walk passes' fetch,
     // 0 = 128; two-level worlds: 0 = static 64-ray slices)
@@ -1075,30 +1109,53 @@ extern "C" int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views 
     const int chunksTwoLevel[] = {0, 256, 1024};
     const int *chunks = w.twoLevel ? chunksTwoLevel : chunksFlat;
     TuneChoice best;
-    float bestMs = 1e30f;
+    float best1 = 1e30f;
     for (int b : blocks)
       for (int ci = 0; ci < 3; ++ci) {
         TuneChoice t;
         t.traceBlocksPerCU = b;
         t.chunk = chunks[ci];
-        t.sppBatch = params->pathsPerPixel;
-        // one spp per candidate keeps tuning short; the better of two runs
-        // keeps one noisy run from deciding
-        float ms = render_wavefront(w, views, rp, 1, accum.ptr, nullptr, 0, nullptr, t, false);
-        ms = std::min(ms, render_wavefront(w, views, rp, 1, accum.ptr, nullptr, 0, nullptr, t, false));
-        if (ms < bestMs) {
-          bestMs = ms;
+        t.sppBatch = 1;
+        const float ms = wave(t, 1);
+        if (ms < best1) {
+          best1 = ms;
           best = t;
         }
       }
-    best.sppBatch = 0;
-    st.tune = best;
-    st.tuned = true;
+    float bestWave = 1e30f;
+    int bestBatch = spp;
+    for (int b = spp; b >= 1; b = (b > 1 ? (b + 1) / 2 : 0)) {
+      TuneChoice t = best;
+      t.sppBatch = b;
+      const float ms = wave(t, spp);
+      if (ms < bestWave) {
+        bestWave = ms;
+        bestBatch = b;
+      }
+      if (b == 1) break;
+    }
+    best.sppBatch = bestBatch == spp ? 0 : bestBatch;
+    float bestMega = 1e30f;
+    int bestLanes = 0;
+    for (int lanes = 1; lanes <= 8 && lanes <= spp; lanes *= 2) {
+      const float ms = mega(lanes);
+      if (ms < bestMega) {
+        bestMega = ms;
+        bestLanes = lanes;
+      }
+    }
+    best.megaLanes = bestLanes;
+    best.mode = bestMega < bestWave ? BNAMD_BATCH_MEGAKERNEL : BNAMD_BATCH_WAVEFRONT;
+    best.tuned = true;
+    w.tune = best;
     if (path) {
       FILE *f = fopen(path, "w");
       BN_REQUIRE(f, std::string("bnamd_batch_autotune: cannot write ") + path);
-      fprintf(f, "{\"traceBlocksPerCU\": %d, \"chunk\": %d, \"bestMs1spp\": %.4f}\n", best.traceBlocksPerCU,
-              best.chunk, bestMs);
+      fprintf(f,
+              "{\"traceBlocksPerCU\": %d, \"chunk\": %d, \"bestMs1spp\": %.4f, \"sppBatch\": %d, "
+              "\"wavefrontMs\": %.4f, \"megaLanes\": %d, \"megakernelMs\": %.4f, \"mode\": \"%s\"}\n",
+              best.traceBlocksPerCU, best.chunk, best1, bestBatch, bestWave, bestLanes, bestMega,
+              best.mode == BNAMD_BATCH_MEGAKERNEL ? "megakernel" : "wavefront");
       fclose(f);
     }
   });

@@ -376,6 +376,21 @@ extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const
   });
 }
 
+extern "C" int bnamd_world_validate(const bnamd_world *world) {
+  return guarded([&] {
+    BN_REQUIRE(world, "bnamd_world_validate: NULL world");
+    const World &w = world->w;
+    BN_HIP(hipDeviceSynchronize());
+    check_world_bvh(w);
+    std::vector<const World *> seen;
+    for (const World *b : w.instBlas)
+      if (std::find(seen.begin(), seen.end(), b) == seen.end()) {
+        seen.push_back(b);
+        check_world_bvh(*b);
+      }
+  });
+}
+
 extern "C" int bnamd_world_destroy(bnamd_world *world) {
   return guarded([&] { delete world; });
 }

@@ -88,6 +88,17 @@ BN_HD vec3f xfm_vector(const float *m, vec3f v) {
           m[2] * v.x + m[5] * v.y + m[8] * v.z};
 }
 
+// Launch configuration of the batched renderers for one world, chosen by
+// bnamd_batch_autotune (defaults until it runs).
+struct BatchTune {
+  int traceBlocksPerCU = 0;  // wavefront trace/shadow workgroups per CU; 0 = as many as are resident
+  int chunk = 0;             // rays per dynamic work fetch; 0 = the kernels' default
+  int sppBatch = 0;          // paths per pixel per wavefront batch (queue size); 0 = all in one batch
+  int megaLanes = 0;         // megakernel lanes per pixel (tile = 256 / lanes pixels); 0 = from spp
+  int mode = 1;              // the faster renderer (BNAMD_BATCH_MEGAKERNEL 0 / WAVEFRONT 1)
+  bool tuned = false;
+};
+
 struct World {
   DevBuf<BNode8> nodes;
   DevBuf<PrimRec> prims;
@@ -108,6 +119,7 @@ struct World {
   int numBlas = 0;                           // distinct bottom levels referenced
   bool hasCutouts = false;                   // some triangle's material has alpha < 1
   mutable DevBuf<unsigned long long> counters;  // the surface render's counters
+  mutable BatchTune tune;                        // batched renderers' launch configuration
   DevBuf<InstDev> insts;
   DevBuf<float4> instBoxes;
   int tlasDepth = 0, localBits = 0;
@@ -131,6 +143,8 @@ struct World {
 
 // Builds the BVH8 over the given prim table (implemented in bvh_build.hip).
 void build_world_bvh(World &w, hipStream_t stream);
+// Host-side structural check of a built BVH8 (throws on a violation).
+void check_world_bvh(const World &w);
 
 // Maps a flat prim index back to its geom (binary search over primOffset).
 BN_D int find_geom(const GeomDev *__restrict geoms, int numGeoms, uint32_t flat) {

@@ -590,7 +590,11 @@ def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
     """The full-size adjoint against central differences along a seeded
     random direction v (entries uniform in [-1, 1]) of the transfer
     function, then of the density grid: <grad L, v> from the adjoint vs
-    (L(x + eps v) - L(x - eps v)) / (2 eps), the loss summed in float64."""
+    (L(x + eps v) - L(x - eps v)) / (2 eps), the loss summed in float64.
+    The density direction is zero on voxels within 2 eps of the transfer
+    function's domain ends: the lookup clamps there (most of the grid is
+    empty space at exactly 0), so L has a kink that a central difference
+    halves while the adjoint takes one side."""
     import torch
     gen = torch.Generator(device="cuda").manual_seed(11)
 
@@ -601,6 +605,9 @@ def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
     out = {}
     for name, x in (("tf", tf), ("density", dens)):
         v = torch.rand(x.shape, generator=gen, device="cuda") * 2 - 1
+        if name == "density":
+            xd = x.detach()
+            v = torch.where((xd > 2 * eps) & (xd < 1 - 2 * eps), v, torch.zeros_like(v))
         dot = float((x.grad.double() * v.double()).sum())
         xp, xm = (x.detach() + eps * v), (x.detach() - eps * v)
         if name == "tf":

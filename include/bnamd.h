@@ -124,6 +124,12 @@ int bnamd_world_build(const bnamd_group *groups, int numGroups,
                       const bnamd_material *materials, int numMaterials,
                       void *stream, bnamd_world **out);
 int bnamd_world_destroy(bnamd_world *world);
+/* Copies the world's BVH8s (both levels) to the host and checks them: child
+ * and primitive references in range, every primitive referenced exactly
+ * once, every quantized child box containing what it bounds. Builds skip
+ * this check unless BNAMD_VALIDATE_BVH=1. No reference counterpart (a
+ * test hook). */
+int bnamd_world_validate(const bnamd_world *world);
 int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info);
 
 /* Bottom levels built on their own: rtc::GeomGroup::buildAccel of a
@@ -526,6 +532,7 @@ int bnamd_ea_backward(const bnamd_ea_volume *vol, const bnamd_ea_views *views, c
  * --------------------------------------------------------------------- */
 #define BNAMD_BATCH_MEGAKERNEL 0
 #define BNAMD_BATCH_WAVEFRONT 1
+#define BNAMD_BATCH_AUTO 2 /* the renderer bnamd_batch_autotune found faster (wavefront untuned) */
 
 typedef struct {
   int numViews;
@@ -556,9 +563,13 @@ int bnamd_render_batch(const bnamd_world *world, const bnamd_views *views,
                        const bnamd_render_params *params, float *accum, int *ids, int mode,
                        void *stream, bnamd_batch_stats *stats);
 
-/* Times candidate wavefront launch configurations on this world and views
- * and keeps the fastest for later bnamd_render_batch calls; if path is
- * non-NULL the choice is also written there as JSON. */
+/* Times launch configurations on this world and views and keeps the fastest
+ * with the world for later bnamd_render_batch calls: the wavefront trace
+ * grid and work-fetch chunk (one path per pixel), the wavefront queue size
+ * (paths per pixel per batch) and the megakernel tile (lanes per pixel) at
+ * params->pathsPerPixel, and which of the two renderers is faster
+ * (BNAMD_BATCH_AUTO). If path is non-NULL the choice is also written there
+ * as JSON. */
 int bnamd_batch_autotune(const bnamd_world *world, const bnamd_views *views,
                          const bnamd_render_params *params, const char *path);
 

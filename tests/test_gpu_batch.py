@@ -18,6 +18,7 @@ def _scene(gpu, views=5):
     geoms = [gpu.Geom(g["kind"], g["vertices"], g.get("indices"), g.get("radii"), material=g["material"])
              for g in w3["geoms"]]
     world = gpu.World([geoms], materials=scenes.product_materials(gpu, w3["materials"]))
+    world.validate()
     cams = [gpu.camera(**c) for c in w3["cameras"]]
     ws = [max(16, w // 8) for w in w3["widths"]]
     hs = [max(16, h // 8) for h in w3["heights"]]
@@ -60,14 +61,26 @@ def test_wavefront_without_graph_and_progressive(gpu):
 
 
 def test_autotune_writes_choice(gpu, tmp_path):
+    """The tuner searches the wavefront grid and fetch chunk, the wavefront
+    queue size (paths per pixel per batch) and the megakernel tile (lanes
+    per pixel), and picks the faster renderer; every tuned renderer still
+    equals the untuned megakernel's image (wavefront within 1e-4)."""
     world, views, w3 = _scene(gpu, views=2)
     path = str(tmp_path / "tune.json")
-    gpu.batch_autotune(world, views, _params(gpu, w3, spp=1), path)
+    ref = gpu.render_batch(world, views, _params(gpu, w3, spp=4), gpu.BATCH_MEGAKERNEL)["accum"].numpy()
+    gpu.batch_autotune(world, views, _params(gpu, w3, spp=4), path)
     import json
     d = json.load(open(path))
+    print(d)
     assert d["traceBlocksPerCU"] in (0, 2, 3, 4) and d["chunk"] in (0, 64, 256)  # flattened world
-    out = gpu.render_batch(world, views, _params(gpu, w3, spp=1))["accum"].numpy()
-    assert np.isfinite(out).all()
+    assert d["sppBatch"] in (1, 2, 4) and d["megaLanes"] in (1, 2, 4) and d["mode"] in ("megakernel", "wavefront")
+    assert d["wavefrontMs"] > 0 and d["megakernelMs"] > 0
+    mega = gpu.render_batch(world, views, _params(gpu, w3, spp=4), gpu.BATCH_MEGAKERNEL)["accum"].numpy()
+    assert np.array_equal(mega, ref)  # lanes change the schedule, not the sums (path order)
+    for mode in (gpu.BATCH_WAVEFRONT, gpu.BATCH_AUTO):
+        out = gpu.render_batch(world, views, _params(gpu, w3, spp=4), mode)["accum"].numpy()
+        assert np.isfinite(out).all()
+        np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-5)
 
 
 def test_graph_replay_follows_render_parameters(gpu):

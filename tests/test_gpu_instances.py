@@ -57,6 +57,7 @@ def _worlds(gpu):
     gg = [[gpu.Geom(g["kind"], g["vertices"], g.get("indices"), g.get("radii"), material=0) for g in grp]
           for grp in groups]
     world = gpu.World(gg, materials=[gpu.matte((0.7, 0.6, 0.5))], instances=INSTANCES)
+    world.validate()  # both levels
     ref = refcpu.World([refcpu.Scene(grp) for grp in groups], INSTANCES)
     return world, ref
 

@@ -28,7 +28,9 @@ def _compare(gpu_hits, ref_hits, label):
 
 def _world(gpu, geoms):
     pg, mats = to_product_geoms(geoms)
-    return gpu.World([pg], materials=mats)
+    w = gpu.World([pg], materials=mats)
+    w.validate()  # structure: references in range, each primitive once, boxes contain their content
+    return w
 
 
 def test_triangle_spheres_random_rays(gpu):
