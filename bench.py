@@ -588,9 +588,14 @@ def ea_bench(steps=3):
 
 def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
     """The full-size adjoint against central differences along a seeded
-    random direction v (entries uniform in [-1, 1]) of the transfer
-    function, then of the density grid: <grad L, v> from the adjoint vs
-    (L(x + eps v) - L(x - eps v)) / (2 eps), the loss summed in float64.
+    direction v of the transfer function, then of the density grid:
+    <grad L, v> from the adjoint vs (L(x + eps v) - L(x - eps v)) / (2 eps),
+    the loss summed in float64. v_i = sign(g_i) u_i with u uniform in
+    [0.5, 1] and g the adjoint's gradient: a direction of random signs makes
+    <grad L, v> a sum of millions of cancelling terms, far below the
+    terms' own scale, so any rounding in the difference quotient swamps
+    it; with the gradient's signs the terms add, and an entry whose sign
+    the adjoint got wrong still counts against the difference quotient.
     The density direction is zero on voxels within 2 eps of the transfer
     function's domain ends: the lookup clamps there (most of the grid is
     empty space at exactly 0), so L has a kink that a central difference
@@ -604,7 +609,8 @@ def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
 
     out = {}
     for name, x in (("tf", tf), ("density", dens)):
-        v = torch.rand(x.shape, generator=gen, device="cuda") * 2 - 1
+        u = torch.rand(x.shape, generator=gen, device="cuda") * 0.5 + 0.5
+        v = torch.sign(x.grad) * u
         if name == "density":
             xd = x.detach()
             v = torch.where((xd > 2 * eps) & (xd < 1 - 2 * eps), v, torch.zeros_like(v))
@@ -614,7 +620,7 @@ def ea_directional_check(ea, setup, dens, tf, target, eps=1e-3):
             fd = (loss(dens.detach(), xp) - loss(dens.detach(), xm)) / (2 * eps)
         else:
             fd = (loss(xp, tf.detach()) - loss(xm, tf.detach())) / (2 * eps)
-        out[name] = {"adjoint": dot, "central_difference": fd, "rel_diff": abs(fd - dot) / abs(fd)}
+        out[name] = {"adjoint": dot, "central_difference": fd, "rel_diff": abs(fd - dot) / abs(fd), "eps": eps}
     return out
 
 
