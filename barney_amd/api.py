@@ -575,19 +575,23 @@ class UMesh:
         check(capi.load().bnamd_umesh_iso_rays(self.handle, dr.addr, float(iso), do.addr, len(r), None))
         return do.numpy()
 
-    def render(self, cam, params, volume=True, iso=None, iso_color=(0.8, 0.8, 0.8), accum=None):
+    def render(self, cam, params, volume=True, iso=None, iso_color=(0.8, 0.8, 0.8), accum=None, count=False):
+        """count: the instrumented kernel, which also reports point
+        location's node visits, element tests and their bytes."""
         acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
         o = capi.bnamd_umesh_render_opts()
         o.volume = int(bool(volume))
         o.iso = int(iso is not None)
         o.isoValue = float(iso if iso is not None else 0.0)
         o.isoColor[:] = [float(c) for c in iso_color]
+        o.countTraversal = int(bool(count))
         st = capi.bnamd_volume_stats()
         check(capi.load().bnamd_umesh_render(self.handle, C.byref(cam), C.byref(params), C.byref(o), acc.addr, None,
                                              C.byref(st)))
         return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
                                         "raysTraced": st.raysTraced, "ms": st.ms,
-                                        "zeroMajorantSteps": st.zeroMajorantSteps}}
+                                        "zeroMajorantSteps": st.zeroMajorantSteps, "nodeVisits": st.nodeVisits,
+                                        "cellTests": st.cellTests, "cellBytes": st.cellBytes}}
 
     def destroy(self):
         if self.handle is not None and self.handle.value:

@@ -34,6 +34,7 @@ SOURCES = [
     "ea.hip",
     "render_batch.hip",
     "umesh.hip",
+    "umesh_count.hip",
     "fb.hip",
     "envmap.hip",
 ]

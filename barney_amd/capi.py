@@ -83,7 +83,8 @@ class bnamd_amr_info(C.Structure):
 
 class bnamd_volume_stats(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("ddaSteps", C.c_longlong), ("raysTraced", C.c_longlong),
-                ("ms", C.c_float), ("zeroMajorantSteps", C.c_longlong)]
+                ("ms", C.c_float), ("zeroMajorantSteps", C.c_longlong),
+                ("nodeVisits", C.c_longlong), ("cellTests", C.c_longlong), ("cellBytes", C.c_longlong)]
 
 
 class bnamd_umesh_desc(C.Structure):
@@ -99,7 +100,8 @@ class bnamd_umesh_info(C.Structure):
 
 
 class bnamd_umesh_render_opts(C.Structure):
-    _fields_ = [("volume", C.c_int), ("iso", C.c_int), ("isoValue", C.c_float), ("isoColor", C.c_float * 3)]
+    _fields_ = [("volume", C.c_int), ("iso", C.c_int), ("isoValue", C.c_float), ("isoColor", C.c_float * 3),
+                ("countTraversal", C.c_int)]
 
 
 class bnamd_views(C.Structure):

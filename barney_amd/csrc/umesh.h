@@ -306,19 +306,63 @@ BN_D bool elem_scalar(const UMeshDD &u, uint32_t i, vec3f P, float &val) {
 // crossing test is not watertight, and a polyhedron's inverse-distance value
 // is discontinuous with its neighbour's), so there the answer must be the
 // first containing cell in traversal order, as the reference's query gives.
+//
+// With kCount, 'tc' counts the BVH nodes visited, the element inside tests
+// and their algorithmic bytes, for the roofline of point location: 128 B per
+// node; per test the bytes of the element in indexed storage (per vertex a
+// 4-byte index, a 12-byte position and a 4-byte scalar: 80 B per tet, 100,
+// 120 and 160 per pyramid, prism and hexahedron; a polyhedron's face stream
+// at 4 B per entry plus 16 B per vertex reference) and, for mixed meshes,
+// the 16-byte halves of the element boxes read before the test.
+struct TravCount {
+  uint32_t nodes, tests, bytes;
+};
+
+template <bool kMixed>
+BN_D void count_test(const UMeshDD &u, uint32_t i, TravCount &tc) {
+  ++tc.tests;
+  if (!kMixed) {
+    tc.bytes += 80u;
+    return;
+  }
+  switch (u.elemTypes[i]) {
+    case ELEM_TET: tc.bytes += 80u; break;
+    case ELEM_PYR: tc.bytes += 100u; break;
+    case ELEM_PRISM: tc.bytes += 120u; break;
+    case ELEM_HEX: tc.bytes += 160u; break;
+    default: {
+      const int *ix = u.cellIndices;
+      int pos = __float_as_int(u.tets[8ull * i].x);
+      const int nf = ix[pos++];
+      uint32_t b = 4u;
+      for (int f = 0; f < nf; ++f) {
+        const int np = ix[pos++];
+        b += 4u + 20u * (uint32_t)np;
+        pos += np;
+      }
+      tc.bytes += b;
+    }
+  }
+}
+
 constexpr int kLocateWindow = 4;
-template <bool kMixed = false>
-BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
+template <bool kMixed = false, bool kCount = false>
+BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint,
+                        TravCount *tc = nullptr) {
+  auto test = [&](uint32_t i, float &val) {
+    if (kCount) count_test<kMixed>(u, i, *tc);
+    return elem_scalar<kMixed>(u, i, P, val);
+  };
   if (!kMixed && hint != ~0u) {
     float val;
-    if (elem_scalar<kMixed>(u, hint, P, val)) return val;
+    if (test(hint, val)) return val;
     for (int d = 1; d <= kLocateWindow; ++d) {
       const uint32_t a = hint + (uint32_t)d, b = hint - (uint32_t)d;
-      if (a < u.numTets && elem_scalar<kMixed>(u, a, P, val)) {
+      if (a < u.numTets && test(a, val)) {
         hint = a;
         return val;
       }
-      if (hint >= (uint32_t)d && elem_scalar<kMixed>(u, b, P, val)) {
+      if (hint >= (uint32_t)d && test(b, val)) {
         hint = b;
         return val;
       }
@@ -342,6 +386,7 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
         ++sp;
       }
       const uint4 *np = (const uint4 *)(u.nodes + nodeIdx);
+      if (kCount) ++tc->nodes;
       const uint4 n0 = np[0];
       const uint4 n1 = np[1];
       const uint32_t ex = n0.w & 0xffu, ey = (n0.w >> 8) & 0xffu, ez = (n0.w >> 16) & 0xffu;
@@ -389,14 +434,16 @@ BN_D float umesh_locate(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint3
       if (kMixed) {
         const float4 lo = u.elemBoxes[2ull * (primBase + i)];
         const uint32_t id = __float_as_uint(lo.w);
+        if (kCount) tc->bytes += 16u;
         if (id < bestId && P.x >= lo.x && P.y >= lo.y && P.z >= lo.z) {
           const float4 hi = u.elemBoxes[2ull * (primBase + i) + 1];
-          if (P.x <= hi.x && P.y <= hi.y && P.z <= hi.z && elem_scalar<kMixed>(u, primBase + i, P, val)) {
+          if (kCount) tc->bytes += 16u;
+          if (P.x <= hi.x && P.y <= hi.y && P.z <= hi.z && test(primBase + i, val)) {
             bestId = id;
             best = val;
           }
         }
-      } else if (elem_scalar<kMixed>(u, primBase + i, P, val)) {
+      } else if (test(primBase + i, val)) {
         hint = primBase + i;
         return val;
       }
@@ -423,9 +470,10 @@ BN_HD bool iso_cell_active(const UMeshDD &u, int idx, float isoValue) {
   return (u.isoMask[(uint32_t)idx >> 5] >> ((uint32_t)idx & 31u)) & 1u;
 }
 
-template <bool kMixed = false>
-BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint) {
-  const float f = umesh_locate<kMixed>(u, P, ctx, hint);
+template <bool kMixed = false, bool kCount = false>
+BN_D VolSample umesh_sample_and_map(const UMeshDD &u, vec3f P, const TraverseCtx &ctx, uint32_t &hint,
+                                    TravCount *tc = nullptr) {
+  const float f = umesh_locate<kMixed, kCount>(u, P, ctx, hint, tc);
   if (f != f) return VolSample{0.f, 0.f, 0.f, 0.f};
   return tf_map(u.tf, f);
 }

@@ -5,12 +5,26 @@
 // (mapMCs, MCGrid.cu:94-116), and the volume + iso-surface renderer.
 #include "shade.h"
 #include "umesh.h"
+#include "umesh_render.h"
 #include <mutex>
 #include <string>
 #include <chrono>
 #include <cmath>
 
 namespace bnamd {
+
+// The instrumented variants are compiled in umesh_count.hip.
+#define BNAMD_UMESH_COUNT_VARIANT(L, M)                                                                        \
+  extern template void launch_render_umesh_t<L, M, true>(const UMeshDD &, const UMeshOpts &, const CameraDD &, \
+                                                         const RenderDD &, int, float4 *, unsigned long long *, \
+                                                         size_t, hipStream_t);
+BNAMD_UMESH_COUNT_VARIANT(8, false)
+BNAMD_UMESH_COUNT_VARIANT(4, false)
+BNAMD_UMESH_COUNT_VARIANT(2, false)
+BNAMD_UMESH_COUNT_VARIANT(1, false)
+BNAMD_UMESH_COUNT_VARIANT(4, true)
+BNAMD_UMESH_COUNT_VARIANT(1, true)
+#undef BNAMD_UMESH_COUNT_VARIANT
 
 size_t trace_lds_bytes(int depth, int block);
 
@@ -349,163 +363,12 @@ __global__ void __launch_bounds__(256) k_umesh_iso_rays(UMeshDD u, const bnamd_r
   out[i] = h.hit ? make_float4(h.t, h.N.x, h.N.y, h.N.z) : make_float4(BN_INF, 0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  const int per = (nb + 7) >> 3;
-  const int t = (bid & 7) * per + (bid >> 3);
-  return t < nb ? t : -1;
-}
-
-__device__ __forceinline__ void wave_sum_u64(unsigned long long v, unsigned long long *dst) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
-}
-
-struct UMeshOpts {
-  int volume, iso;
-  float isoValue;
-  vec3f isoColor;
-};
-
-// Closest of the volume (geometry 0) and iso-surface (geometry 1) hits for
-// one ray; each intersection program advances the ray's seed once.
-template <bool kVol, bool kIso, bool kMixed>
-BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f dir, float tMax,
-                                RNGSeed &seed, const TraverseCtx &ctx, VolCounters &cnt, uint32_t &hint) {
-  ShadeHit sh;
-  sh.hit = false;
-  float tClosest = tMax;
-  if (kVol) {
-    const VolHit vh = volume_intersect_g(
-        u, [&](vec3f P) { return umesh_sample_and_map<kMixed>(u, P, ctx, hint); }, org, dir, 0.f, tMax, tMax, seed, false,
-        cnt, bn_hash3(0u, 0u, 0u));
-    if (vh.hit) {
-      sh.hit = true;
-      sh.P = org + vh.t * dir;
-      sh.N = vec3f(0.f);
-      sh.t = vh.t;
-      sh.bsdf = phase_bsdf(vh.rgb);
-      tClosest = vh.t;
-    }
-  }
-  if (kIso) {
-    const IsoHit ih = iso_intersect(
-        u,
-        [&](vec3f P) {
-          cnt.samples++;
-          return umesh_locate<kMixed>(u, P, ctx, hint);
-        },
-        o.isoValue, org, dir, 0.f, tMax, tClosest, seed, bn_hash3(0u, 1u, 0u));
-    if (ih.hit) {
-      sh.hit = true;
-      sh.P = ih.P;
-      sh.N = ih.N;
-      sh.t = ih.t;
-      sh.bsdf = matte_bsdf(o.isoColor);
-    }
-  }
-  return sh;
-}
-
-// L lanes per pixel, each looping over its share of the pixel's paths.
-// The kernel waits on memory (point-location loads, 19 % VALU issue), so
-// it is built for six wavefronts per SIMD (<= 80 VGPRs, ~300 bytes of
-// spill per lane): 1024x1024x8 spp with volume + iso-surface takes 147 ms
-// at six wavefronts and 8 lanes per pixel, against 232 ms at the three
-// that the unconstrained allocation (168 VGPRs) leaves with 4 lanes; five
-// and eight wavefronts measured 156 and 159 ms.
-template <int L, bool kVol, bool kIso, bool kMixed, bool kEnv>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
-k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
-               unsigned long long *__restrict counters) {
-  extern __shared__ uint2 lds_stack[];
-  constexpr int PW = 64 / L;
-  constexpr int PB = 4 * PW;
-  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
-  constexpr int TH = PB / TW;
-  const int tilesX = (rp.width + TW - 1) / TW, tilesY = (rp.height + TH - 1) / TH;
-  const int tile = xcd_tile(blockIdx.x, tilesX * tilesY);
-  if (tile < 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int sub = lane & (L - 1);
-  const int q = wave * PW + lane / L;
-  const int ix = (tile % tilesX) * TW + (q % TW);
-  const int iy = (tile / tilesX) * TH + (q / TW);
-  TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, u.depth + 1};
-  VolCounters cnt{0, 0, 0};
-  unsigned long long nrays = 0;
-  uint32_t hint = ~0u;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const bool active = ix < rp.width && iy < rp.height;
-  for (int r = 0; r * L < spp; ++r) {
-    const int p = r * L + sub;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (active && p < spp) {
-      RenderDD rpp = rp;
-      rpp.accumID = rp.accumID + p;
-      PathRay ray = camera_ray(cam, rpp, ix, iy);
-      const float clampMax = 10.f * (1 + rpp.accumID);
-      vec3f sum = vec3f(0.f);
-      float alpha = 0.f;
-      for (int gen = 0; gen == 0 || ray.tMax > 0.f; ++gen) {
-        const vec3f dir = fix_dir(ray.dir);
-        const ShadeHit sh = umesh_trace<kVol, kIso, kMixed>(u, o, ray.org, dir, ray.tMax, ray.rng, ctx, cnt, hint);
-        ++nrays;
-        if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
-        PathRay shadow;
-        vec3f frag = vmin(bounce<kEnv>(rpp, ray, sh, shadow, gen), clampMax);
-        if (frag.x > 0.f) sum.x += frag.x;
-        if (frag.y > 0.f) sum.y += frag.y;
-        if (frag.z > 0.f) sum.z += frag.z;
-        if (shadow.tMax > 0.f) {
-          const ShadeHit shs =
-              umesh_trace<kVol, kIso, kMixed>(u, o, shadow.org, fix_dir(shadow.dir), shadow.tMax, shadow.rng, ctx, cnt,
-                                              hint);
-          ++nrays;
-          PathRay dummy;
-          frag = vmin(bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1), clampMax);
-          if (frag.x > 0.f) sum.x += frag.x;
-          if (frag.y > 0.f) sum.y += frag.y;
-          if (frag.z > 0.f) sum.z += frag.z;
-        }
-      }
-      v = make_float4(sum.x, sum.y, sum.z, alpha);
-    }
-    add_round_in_path_order<L>(acc, v, min(L, spp - r * L));
-  }
-  if (active && sub == 0) {
-    const int pix = ix + iy * rp.width;
-    if (rp.accumID == 0) {
-      accum[pix] = acc;
-    } else {
-      float4 a = accum[pix];
-      a.x += acc.x; a.y += acc.y; a.z += acc.z; a.w += acc.w;
-      accum[pix] = a;
-    }
-  }
-  wave_sum_u64(cnt.samples, counters + 0);
-  wave_sum_u64(cnt.ddaSteps, counters + 1);
-  wave_sum_u64(cnt.zeroSteps, counters + 3);
-  wave_sum_u64(nrays, counters + 2);
-}
 
 template <int L, bool kMixed = false>
 void launch_render_umesh(const UMeshDD &u, const UMeshOpts &o, const CameraDD &cam, const RenderDD &rp, int spp,
-                         float4 *accum, unsigned long long *counters, size_t lds, hipStream_t s) {
-  constexpr int PB = 4 * (64 / L);
-  constexpr int TW = (PB >= 256) ? 16 : (PB >= 64 ? 8 : (PB >= 16 ? 4 : (PB >= 4 ? 2 : 1)));
-  constexpr int TH = PB / TW;
-  const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
-  const int grid = ((tiles + 7) / 8) * 8;
-  auto go = [&](auto kernel) { kernel<<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters); };
-  const bool env = has_env(rp);
-  if (o.volume && o.iso)
-    env ? go(k_render_umesh<L, true, true, kMixed, true>) : go(k_render_umesh<L, true, true, kMixed, false>);
-  else if (o.volume)
-    env ? go(k_render_umesh<L, true, false, kMixed, true>) : go(k_render_umesh<L, true, false, kMixed, false>);
-  else if (o.iso)
-    env ? go(k_render_umesh<L, false, true, kMixed, true>) : go(k_render_umesh<L, false, true, kMixed, false>);
-  else
-    env ? go(k_render_umesh<L, false, false, kMixed, true>) : go(k_render_umesh<L, false, false, kMixed, false>);
+                         float4 *accum, unsigned long long *counters, size_t lds, hipStream_t s, bool count) {
+  if (count) launch_render_umesh_t<L, kMixed, true>(u, o, cam, rp, spp, accum, counters, lds, s);
+  else launch_render_umesh_t<L, kMixed, false>(u, o, cam, rp, spp, accum, counters, lds, s);
 }
 
 void upload_tf(UMesh &m, const float *values, int numValues, float lo, float hi, float base) {
@@ -546,17 +409,17 @@ UMeshDD iso_dd(const UMesh &m, float isoValue, hipStream_t s) {
 // Lanes per pixel from the paths per pixel (general cells: 4 from 4 spp,
 // else 1); the kernel instance for them.
 void render_umesh_dispatch(const UMeshDD &dd, const UMeshOpts &o, const CameraDD &cam, const RenderDD &rp, int spp,
-                           float4 *accum, unsigned long long *cnt, size_t lds, hipStream_t s) {
+                           float4 *accum, unsigned long long *cnt, size_t lds, hipStream_t s, bool count) {
   int lanes = spp >= 8 ? 8 : (spp >= 4 ? 4 : (spp >= 2 ? 2 : 1));
   if (const char *e = getenv("BNAMD_UMESH_LANES")) lanes = atoi(e);
   if (dd.elemTypes) {
-    if (lanes >= 4) launch_render_umesh<4, true>(dd, o, cam, rp, spp, accum, cnt, lds, s);
-    else launch_render_umesh<1, true>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+    if (lanes >= 4) launch_render_umesh<4, true>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
+    else launch_render_umesh<1, true>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
   }
-  else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, accum, cnt, lds, s);
-  else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, accum, cnt, lds, s);
-  else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, accum, cnt, lds, s);
-  else launch_render_umesh<1>(dd, o, cam, rp, spp, accum, cnt, lds, s);
+  else if (lanes >= 8) launch_render_umesh<8>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
+  else if (lanes >= 4) launch_render_umesh<4>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
+  else if (lanes >= 2) launch_render_umesh<2>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
+  else launch_render_umesh<1>(dd, o, cam, rp, spp, accum, cnt, lds, s, count);
   BN_HIP(hipGetLastError());
 }
 
@@ -564,7 +427,7 @@ void render_umesh_dispatch(const UMeshDD &dd, const UMeshOpts &o, const CameraDD
 // Scratch of bnamd_umesh_render_noalloc, which allocates nothing: one array
 // per GPU for the life of the process, shared by every call on that GPU (the
 // last call's contents stay; calls that overlap on one GPU race). Words
-// [0, 64): the four 64-bit counters (256 bytes); then the iso-surface bit
+// [0, 64): the seven 64-bit counters (in 256 bytes); then the iso-surface bit
 // mask, one bit per macrocell, at most kNoAllocMaskWords words (32 M
 // macrocells, 4 MiB).
 // ---------------------------------------------------------------------------
@@ -835,17 +698,23 @@ extern "C" int bnamd_umesh_render(const bnamd_umesh *mesh, const bnamd_camera *c
     hipStream_t s = (hipStream_t)stream;
     const size_t lds = trace_lds_bytes(mesh->m.dd.depth, 256);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_umesh_render: BVH too deep for the LDS stack");
-    unsigned long long h[4] = {0, 0, 0, 0};
-    const float ms = render_on_stream(s, mesh->m.counters, 4, stats != nullptr, h, [&] {
+    const bool count = opts->countTraversal != 0;
+    unsigned long long h[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float ms = render_on_stream(s, mesh->m.counters, 7, stats != nullptr, h, [&] {
       const UMeshDD dd = o.iso ? iso_dd(mesh->m, o.isoValue, s) : mesh->m.dd;
-      render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, mesh->m.counters.ptr, lds, s);
+      render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, mesh->m.counters.ptr, lds, s,
+                            count);
     });
     if (stats) {
+      *stats = bnamd_volume_stats{};
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = ms;
       stats->zeroMajorantSteps = (long long)h[3];
+      stats->nodeVisits = (long long)h[4];
+      stats->cellTests = (long long)h[5];
+      stats->cellBytes = (long long)h[6];
     }
   });
 }
@@ -878,7 +747,8 @@ extern "C" int bnamd_umesh_render_noalloc(const bnamd_umesh *mesh, const bnamd_c
     hipStream_t s = (hipStream_t)stream;
     uint32_t *sp = noalloc_scratch();
     unsigned long long *cnt = (unsigned long long *)sp;
-    k_zero_u64<<<1, 64, 0, s>>>(cnt, 4);
+    const bool count = opts->countTraversal != 0;
+    k_zero_u64<<<1, 64, 0, s>>>(cnt, 7);
     UMeshDD dd = m.dd;
     if (o.iso) {  // this call's mask, whatever an earlier call left there
       uint32_t *mask = sp + kNoAllocHeaderWords;
@@ -886,16 +756,20 @@ extern "C" int bnamd_umesh_render_noalloc(const bnamd_umesh *mesh, const bnamd_c
                                                                                     o.isoValue, mask);
       dd.isoMask = mask;
     }
-    render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, cnt, lds, s);
+    render_umesh_dispatch(dd, o, cam, rp, params->pathsPerPixel, (float4 *)accum, cnt, lds, s, count);
     if (stats) {  // read back on the stream (not for graph capture: pass NULL there)
-      unsigned long long h[4];
+      unsigned long long h[7];
       BN_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
       BN_HIP(hipStreamSynchronize(s));
+      *stats = bnamd_volume_stats{};
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
       stats->ms = -1.f;
       stats->zeroMajorantSteps = (long long)h[3];
+      stats->nodeVisits = (long long)h[4];
+      stats->cellTests = (long long)h[5];
+      stats->cellBytes = (long long)h[6];
     }
   });
 }

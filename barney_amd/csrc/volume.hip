@@ -514,6 +514,7 @@ extern "C" int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *
       render_volume_dispatch(vol->v.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, vol->v.counters.ptr, s);
     });
     if (stats) {
+      *stats = bnamd_volume_stats{};
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];
@@ -830,6 +831,7 @@ extern "C" int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera
       render_volume_dispatch(amr->a.dd, cam, rp, params->pathsPerPixel, (float4 *)accum, amr->a.counters.ptr, s);
     });
     if (stats) {
+      *stats = bnamd_volume_stats{};
       stats->samples = (long long)h[0];
       stats->ddaSteps = (long long)h[1];
       stats->raysTraced = (long long)h[2];

@@ -39,6 +39,30 @@ def surface_roofline(rays, node_visits, prim_bytes, ms):
     return {"alg_bytes": int(b), "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
             "node_visits_per_ray": round(node_visits / max(rays, 1), 2),
             "prim_bytes_per_ray": round(prim_bytes / max(rays, 1), 1)}
+
+
+def umesh_roofline(counted, ms):
+    """Algorithmic bytes of unstructured-mesh sampling: 128 B per BVH node
+    visited by point location + the element tests' bytes (80 B per tet: four
+    indices, positions and scalars; more for larger cells), counted by the
+    instrumented kernel over one frame of the same paths; as GB/s over the
+    uncounted kernel's frame time and fraction of HBM peak."""
+    b = 128 * counted["nodeVisits"] + counted["cellBytes"]
+    gbps = b / (ms * 1e-3) / 1e9
+    return {"alg_bytes": int(b), "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+            "node_visits": int(counted["nodeVisits"]), "cell_tests": int(counted["cellTests"]),
+            "node_visits_per_sample": round(counted["nodeVisits"] / max(counted["samples"], 1), 2),
+            "cell_tests_per_sample": round(counted["cellTests"] / max(counted["samples"], 1), 2)}
+
+
+def volume_roofline(samples, dda_steps, ms):
+    """Structured-volume algorithmic bytes (32 B per field sample: eight fp32
+    corners; 4 B per macrocell step) as GB/s and fraction of HBM peak."""
+    b = 32 * samples + 4 * dda_steps
+    gbps = b / (ms * 1e-3) / 1e9
+    return {"alg_bytes": int(b), "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
 METRIC = "Mrays/s + volume Msamples/s on 1 MI355X; adjoint ms/iter vs forward"
 
 
@@ -223,7 +247,7 @@ def umesh_cpu_baseline(w5, seconds):
     return base, acc[base["_rows"][0]:base["_rows"][1]]
 
 
-def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
+def surface_bench(api, steps=10, warmup=2, cpu=True, cpu_seconds=10.0):
     from barney_amd import scenes
     w1 = scenes.workload1()
     geoms, mats = [], []
@@ -237,18 +261,16 @@ def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
     acc = api.DeviceArray.empty((H, W, 4), np.float32)
     for _ in range(warmup):
         api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
-    rays = nodes = pbytes = 0
-    ms = 0.0
-    for _ in range(steps):
+    ms = []
+    for _ in range(steps):  # every frame traces the same rays
         o = api.render_surface(world, cam, api.render_params(W, H, spp=1), accum=acc)
-        rays += o["stats"]["raysTraced"]
-        nodes += o["stats"]["nodeVisits"]
-        pbytes += o["stats"]["primBytes"]
-        ms += o["stats"]["ms"]
-    res = {"Mrays_per_s": rays / (ms * 1e-3) / 1e6, "rays_per_frame": rays // steps,
-           "ms_per_frame": ms / steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
+        rays, nodes, pbytes = o["stats"]["raysTraced"], o["stats"]["nodeVisits"], o["stats"]["primBytes"]
+        ms.append(o["stats"]["ms"])
+    t = float(np.median(ms))
+    res = {"Mrays_per_s": rays / (t * 1e-3) / 1e6, "rays_per_frame": rays,
+           "ms_per_frame": t, "frames": steps, "bvh_build_ms": info["buildMs"], "prims": info["numPrims"],
            "bvh_nodes": info["numNodes"], "bvh_depth": info["depth"],
-           "roofline": surface_roofline(rays, nodes, pbytes, ms)}
+           "roofline": surface_roofline(rays, nodes, pbytes, t)}
     if cpu:
         base, oimg = surface_cpu_baseline(w1, cpu_seconds)
         res["vs_oracle"] = band_vs_oracle(acc.numpy(), base, oimg)
@@ -256,7 +278,7 @@ def surface_bench(api, steps=3, warmup=1, cpu=True, cpu_seconds=10.0):
     return res
 
 
-def batch_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
+def batch_bench(api, steps=10, cpu=True, cpu_seconds=10.0):
     """Workload 3: 10M-triangle heightfield + 200k spheres + 100k cylinders,
     60/30/10 diffuse/glossy/dielectric, depth <= 8 with Russian roulette,
     64 views of uneven size, 4 paths per pixel: megakernel vs wavefront."""
@@ -337,7 +359,7 @@ def batch_converged(api, w3, world, ref, spp=256, rows=16):
     return out
 
 
-def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
+def umesh_bench(api, steps=10, cpu=True, cpu_seconds=10.0):
     """Workload 5: 128^3 jittered hex grid split into 5 tets per hex (10.5M
     tets), per-vertex Gaussian-blob scalars; BVH point location with
     barycentric sampling, rasterTet macrocell majorants, delta tracking plus
@@ -353,6 +375,7 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     cam = api.camera(**w5["camera"])
     p = api.render_params(W, H, spp=w5["spp"], ambient=1.0, bg=(-1, -1, -1, -1))
     acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    counted = m.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], accum=acc, count=True)["stats"]
     m.render(cam, p, volume=True, iso=w5["iso"], iso_color=w5["iso_color"], accum=acc)
     ms, st = [], None
     for _ in range(steps):
@@ -360,6 +383,7 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
         st = o["stats"]
         ms.append(st["ms"])
     t = float(np.median(ms))
+    assert counted["samples"] == st["samples"], "the counted frame took other paths"
     log(f"workload 5: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M field samples")
     res = {"ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
             "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"]),
@@ -367,6 +391,7 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
             "tets": info["numTets"], "bvh_nodes": info["numNodes"], "bvh_depth": info["bvhDepth"],
             "mc_dims": list(info["mcDims"]), "bvh_build_ms": info["buildMs"], "mc_build_ms": info["mcMs"],
             "image_mean": float(acc.numpy()[..., :3].mean() / w5["spp"]),
+            "roofline": umesh_roofline(counted, t), "counted_frame_ms": counted["ms"],
             "cpu_baseline": None}
     if cpu:
         base, oimg = umesh_cpu_baseline(w5, cpu_seconds)
@@ -375,7 +400,7 @@ def umesh_bench(api, steps=2, cpu=True, cpu_seconds=10.0):
     return res
 
 
-def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0, polyhedra=False):
+def umesh_cells_bench(api, n=40, steps=10, cpu=True, cpu_seconds=8.0, polyhedra=False):
     """General unstructured cells (an extra workload beside the five; the same path as
     workload 5 with pyramids, prisms and hexahedra next to tets): the sheared
     mixed mesh of scenes.mixed_cells on an n^3 lattice, delta tracking plus
@@ -397,6 +422,7 @@ def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0, polyhedra=F
     p = api.render_params(W, H, spp=spp, ambient=1.0, bg=(-1, -1, -1, -1))
     acc = api.DeviceArray.empty((H, W, 4), np.float32)
     cam = api.camera(**camargs)
+    counted = m.render(cam, p, volume=True, iso=0.5, accum=acc, count=True)["stats"]
     m.render(cam, p, volume=True, iso=0.5, accum=acc)
     ms, st = [], None
     for _ in range(steps):
@@ -404,10 +430,12 @@ def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0, polyhedra=F
         st = o["stats"]
         ms.append(st["ms"])
     t = float(np.median(ms))
+    assert counted["samples"] == st["samples"], "the counted frame took other paths"
     log(f"mixed cells: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M field samples")
     res = {"cells": info["numTets"], "types": {int(k): int(v) for k, v in zip(*np.unique(cells[2], return_counts=True))},
            "ms_per_frame": t, "field_samples_per_frame": int(st["samples"]),
-           "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"])}
+           "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6, "rays_per_frame": int(st["raysTraced"]),
+           "roofline": umesh_roofline(counted, t)}
     if cpu:
         r = refcpu.UMesh(V, S, cells=cells)
         r.set_tf(tf, (0.0, 1.0), 0.3)
@@ -420,7 +448,7 @@ def umesh_cells_bench(api, n=40, steps=2, cpu=True, cpu_seconds=8.0, polyhedra=F
     return res
 
 
-def amr_bench(api, n=128, block=16, steps=2, cpu=True, cpu_seconds=8.0):
+def amr_bench(api, n=128, block=16, steps=10, cpu=True, cpu_seconds=8.0):
     """Block-structured AMR (an extra workload beside the five; the structured-volume
     path tracer with BlockStructuredField's sampler): scenes.amr_field with a
     128^3-cell coarse level and a 128^3-cell fine level in 16^3 blocks
@@ -453,7 +481,8 @@ def amr_bench(api, n=128, block=16, steps=2, cpu=True, cpu_seconds=8.0):
     log(f"amr: {t:.1f} ms/frame, {st['samples'] / 1e6:.1f} M samples")
     res = {"blocks": info["numBlocks"], "cells": int(len(f["scalars"])), "ms_per_frame": t,
            "samples_per_frame": int(st["samples"]), "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6,
-           "dda_steps_per_frame": int(st["ddaSteps"]), "rays_per_frame": int(st["raysTraced"])}
+           "dda_steps_per_frame": int(st["ddaSteps"]), "rays_per_frame": int(st["raysTraced"]),
+           "roofline": volume_roofline(st["samples"], st["ddaSteps"], t)}
     if cpu:
         r = refcpu.Amr(*args)
         r.set_tf(tf, (0.0, 1.0), 0.2)
@@ -548,7 +577,7 @@ def profile_traffic(kernel):
     return int((d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024)
 
 
-def ea_bench(steps=3):
+def ea_bench(steps=10):
     """Workload 4: 256^3 density, 64 orbit views at 512^2, L2 loss against
     images of a second transfer function; forward and checkpointed adjoint."""
     import torch

@@ -304,6 +304,15 @@ typedef struct {
   long long raysTraced;
   float ms;
   long long zeroMajorantSteps; /* of ddaSteps: cells with majorant 0 (no draw) */
+  /* unstructured meshes rendered with opts->countTraversal (else 0):
+   * point location's BVH node visits, element inside tests, and the tests'
+   * algorithmic bytes (per vertex a 4-byte index, 12-byte position and
+   * 4-byte scalar, e.g. 80 B per tet; a polyhedron's face stream at 4 B per
+   * entry plus 16 B per vertex reference; mixed meshes add the 16-byte
+   * element-box halves read before a test) */
+  long long nodeVisits;
+  long long cellTests;
+  long long cellBytes;
 } bnamd_volume_stats;
 
 int bnamd_volume_render(const bnamd_volume *vol, const bnamd_camera *camera,
@@ -446,6 +455,9 @@ typedef struct {
   int iso;           /* iso-surface on/off */
   float isoValue;
   float isoColor[3]; /* matte (AnariMatte) color of the iso-surface */
+  int countTraversal; /* nonzero: run the instrumented kernel that also fills
+                       * stats->nodeVisits / cellTests / cellBytes (the
+                       * same image; slower, for measurement) */
 } bnamd_umesh_render_opts;
 
 /* Volume and/or iso-surface render with the env light (same bounce as the

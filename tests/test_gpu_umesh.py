@@ -136,3 +136,20 @@ def test_workload5_reduced_renders(gpu):
     a = out["accum"].numpy()
     assert np.isfinite(a).all() and a[..., :3].mean() > 0.01
     assert out["stats"]["samples"] > 0
+
+
+def test_counted_render_equals_plain(gpu, meshes):
+    """The instrumented kernel (point location's node visits, tet tests and
+    their bytes counted for the roofline) renders the same image with the
+    same samples as the plain one; a tet test is 80 bytes."""
+    V, S, T, g, r = meshes
+    p = gpu.render_params(64, 48, spp=8, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=1)
+    a = g.render(_cam(gpu), p, volume=True, iso=0.5)
+    b = g.render(_cam(gpu), p, volume=True, iso=0.5, count=True)
+    assert np.array_equal(a["accum"].numpy().view(np.uint32), b["accum"].numpy().view(np.uint32))
+    sa, sb = a["stats"], b["stats"]
+    assert sa["samples"] == sb["samples"] and sa["raysTraced"] == sb["raysTraced"]
+    assert sa["nodeVisits"] == sa["cellTests"] == sa["cellBytes"] == 0
+    print(f"nodes {sb['nodeVisits']} tests {sb['cellTests']} samples {sb['samples']}", flush=True)
+    assert sb["nodeVisits"] > 0 and sb["cellTests"] > 0
+    assert sb["cellBytes"] == 80 * sb["cellTests"]

@@ -183,3 +183,19 @@ def test_bad_face_stream_rejected(gpu):
     for bad in ([0], [1, 2, 0, 1], [1, 3, 0, 1, len(V)], [2, 3, 0, 1, 2]):
         with pytest.raises(Exception):
             gpu.UMesh(V, S, cells=(np.array(bad, np.int32), np.array([0], np.int32), np.array([42], np.uint8)))
+
+
+@pytest.mark.parametrize("which", ["mixed", "poly"])
+def test_counted_render_equals_plain(which, request, gpu):
+    """The instrumented kernel renders the same image with the same samples
+    as the plain one; every test reads at least a tet's 80 bytes."""
+    V, S, cells, g, r = request.getfixturevalue(which)
+    p = gpu.render_params(64, 48, spp=4, ambient=1.0, bg=(-1, -1, -1, -1), max_diffuse=0)
+    a = g.render(_cam(gpu), p, volume=True, iso=0.5)
+    b = g.render(_cam(gpu), p, volume=True, iso=0.5, count=True)
+    assert np.array_equal(a["accum"].numpy().view(np.uint32), b["accum"].numpy().view(np.uint32))
+    sb = b["stats"]
+    assert a["stats"]["samples"] == sb["samples"]
+    print(f"nodes {sb['nodeVisits']} tests {sb['cellTests']} bytes {sb['cellBytes']}", flush=True)
+    assert sb["nodeVisits"] > 0 and sb["cellTests"] > 0
+    assert sb["cellBytes"] >= 80 * sb["cellTests"]
