@@ -2339,6 +2339,277 @@ long long ref_amr_render(const ref_amr *a, const ref_camera *cam, const ref_rend
 }  // extern "C"
 
 // ============================================================================
+// NanoVDB fields (barney/volume/NanoVDB.h:98-127, NanoVDB.cu:28-102,
+// 204-235) over the grid buffer of NanoVDB 32.x (the layout of
+// 3rdParty/nanovdb/include/nanovdb/NanoVDB.h: GridData 672 B, TreeData 64 B,
+// RootData 64 B + 32-byte tiles, upper 32^3 and lower 16^3 internal nodes,
+// 8^3 leaves of float or of Fp4/Fp8/Fp16/FpN codes)
+// ============================================================================
+struct ref_nvdb_s {
+  ref_volume_s grid;           // macrocell grid, world box and transfer function
+  std::vector<uint64_t> words; // the grid buffer, 8-byte aligned
+  const uint8_t *buf = nullptr;
+  size_t bytes = 0;
+  int type = 0;
+  size_t root = 0;             // byte offset of RootData
+  int ilo[3], ihi[3];          // index bounding box (the root's)
+  float invMatF[9], vecF[3];   // Map::mInvMatF, mVecF
+};
+
+namespace {
+
+const size_t kNvUpperSize = 270400, kNvLowerSize = 33856;
+
+template <class T>
+inline T nv_ld(const uint8_t *p) {
+  T v;
+  std::memcpy(&v, p, sizeof(T));
+  return v;
+}
+
+// leaf byte size by grid type (FpN: from the leaf's flags)
+inline size_t nv_leaf_size(int type, const uint8_t *leaf) {
+  switch (type) {
+    case 1: return 2144;   // float
+    case 13: return 352;   // Fp4
+    case 14: return 608;   // Fp8
+    case 15: return 1120;  // Fp16
+    default: return 96 + ((size_t)1 << (leaf[15] >> 5)) * 64;  // FpN
+  }
+}
+
+// LeafData<T>::getValue for the leaf-local offset n
+inline float nv_leaf_value(int type, const uint8_t *leaf, uint32_t n) {
+  if (type == 1) return nv_ld<float>(leaf + 96 + 4 * n);
+  const float mn = nv_ld<float>(leaf + 80), q = nv_ld<float>(leaf + 84);
+  const uint8_t *code = leaf + 96;
+  uint32_t c;
+  switch (type) {
+    case 13: c = (code[n >> 1] >> ((n & 1) << 2)) & 15u; break;
+    case 14: c = code[n]; break;
+    case 15: c = nv_ld<uint16_t>(code + 2 * n); break;
+    default: {
+      const int b = leaf[15] >> 5;  // bit width 1 << b
+      switch (b) {
+        case 0: c = (code[n >> 3] >> (n & 7)) & 1u; break;
+        case 1: c = (code[n >> 2] >> ((n & 3) << 1)) & 3u; break;
+        case 2: c = (code[n >> 1] >> ((n & 1) << 2)) & 15u; break;
+        case 3: c = code[n]; break;
+        default: c = nv_ld<uint16_t>(code + 2 * n); break;
+      }
+    }
+  }
+  return (float)c * q + mn;
+}
+
+// ReadAccessor::getValue: root tile by key, then upper, lower, leaf
+float nv_value(const ref_nvdb_s *g, int x, int y, int z) {
+  const uint8_t *root = g->buf + g->root;
+  const uint64_t key = ((uint64_t)((uint32_t)z >> 12)) | ((uint64_t)((uint32_t)y >> 12) << 21) |
+                       ((uint64_t)((uint32_t)x >> 12) << 42);
+  const uint32_t nt = nv_ld<uint32_t>(root + 24);
+  const uint8_t *tile = nullptr;
+  for (uint32_t i = 0; i < nt; ++i)
+    if (nv_ld<uint64_t>(root + 64 + 32 * (size_t)i) == key) {
+      tile = root + 64 + 32 * (size_t)i;
+      break;
+    }
+  if (!tile) return nv_ld<float>(root + 28);  // background
+  const int64_t child = nv_ld<int64_t>(tile + 8);
+  if (child == 0) return nv_ld<float>(tile + 20);
+  const uint8_t *up = root + child;
+  uint32_t n = (((uint32_t)x & 4095u) >> 7 << 10) | (((uint32_t)y & 4095u) >> 7 << 5) | (((uint32_t)z & 4095u) >> 7);
+  if (!((nv_ld<uint64_t>(up + 4128 + 8 * (n >> 6)) >> (n & 63)) & 1u)) return nv_ld<float>(up + 8256 + 8 * (size_t)n);
+  const uint8_t *lo = up + nv_ld<int64_t>(up + 8256 + 8 * (size_t)n);
+  n = (((uint32_t)x & 127u) >> 3 << 8) | (((uint32_t)y & 127u) >> 3 << 4) | (((uint32_t)z & 127u) >> 3);
+  if (!((nv_ld<uint64_t>(lo + 544 + 8 * (n >> 6)) >> (n & 63)) & 1u)) return nv_ld<float>(lo + 1088 + 8 * (size_t)n);
+  const uint8_t *leaf = lo + nv_ld<int64_t>(lo + 1088 + 8 * (size_t)n);
+  return nv_leaf_value(g->type, leaf, ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u));
+}
+
+// NanoVDBDataSampler::DD::sample (NanoVDB.h:98-127): Map::applyInverseMapF
+// on the double position (double difference, then fp32 fmaf rows), minus
+// half a voxel, clamped to the index box, then SampleFromVoxels<1>'s floor,
+// stencil and nested lerps a + float(w) * (b - a) (z innermost, then y, x).
+float nv_sample(const ref_nvdb_s *g, V3 P) {
+  const double d[3] = {(double)P.x - (double)g->vecF[0], (double)P.y - (double)g->vecF[1],
+                       (double)P.z - (double)g->vecF[2]};
+  const float f[3] = {(float)d[0], (float)d[1], (float)d[2]};
+  const float *m = g->invMatF;
+  double ip[3] = {(double)std::fma(f[0], m[0], std::fma(f[1], m[1], f[2] * m[2])),
+                  (double)std::fma(f[0], m[3], std::fma(f[1], m[4], f[2] * m[5])),
+                  (double)std::fma(f[0], m[6], std::fma(f[1], m[7], f[2] * m[8]))};
+  int c[3];
+  double w[3];
+  for (int k = 0; k < 3; ++k) {
+    ip[k] -= 0.5;
+    ip[k] = std::min(std::max(ip[k], (double)g->ilo[k]), (double)g->ihi[k]);
+    const double fl = std::floor(ip[k]);
+    c[k] = (int32_t)fl;
+    w[k] = ip[k] - fl;
+  }
+  auto lerp = [](float a, float b, double t) { return a + (float)t * (b - a); };
+  const float v000 = nv_value(g, c[0], c[1], c[2]), v001 = nv_value(g, c[0], c[1], c[2] + 1);
+  const float v010 = nv_value(g, c[0], c[1] + 1, c[2]), v011 = nv_value(g, c[0], c[1] + 1, c[2] + 1);
+  const float v100 = nv_value(g, c[0] + 1, c[1], c[2]), v101 = nv_value(g, c[0] + 1, c[1], c[2] + 1);
+  const float v110 = nv_value(g, c[0] + 1, c[1] + 1, c[2]), v111 = nv_value(g, c[0] + 1, c[1] + 1, c[2] + 1);
+  return lerp(lerp(lerp(v000, v001, w[2]), lerp(v010, v011, w[2]), w[1]),
+              lerp(lerp(v100, v101, w[2]), lerp(v110, v111, w[2]), w[1]), w[0]);
+}
+
+// every node reached from the root lies inside the buffer
+bool nv_check(const ref_nvdb_s *g) {
+  const uint8_t *b = g->buf;
+  auto in = [&](const uint8_t *p, size_t n) { return p >= b && (size_t)(p - b) + n <= g->bytes; };
+  const uint8_t *root = b + g->root;
+  const uint32_t nt = nv_ld<uint32_t>(root + 24);
+  if (!in(root, 64 + 32 * (size_t)nt)) return false;
+  for (uint32_t i = 0; i < nt; ++i) {
+    const int64_t ch = nv_ld<int64_t>(root + 64 + 32 * (size_t)i + 8);
+    if (!ch) continue;
+    const uint8_t *up = root + ch;
+    if (!in(up, kNvUpperSize)) return false;
+    for (uint32_t n = 0; n < 32768; ++n) {
+      if (!((nv_ld<uint64_t>(up + 4128 + 8 * (n >> 6)) >> (n & 63)) & 1u)) continue;
+      const uint8_t *lo = up + nv_ld<int64_t>(up + 8256 + 8 * (size_t)n);
+      if (!in(lo, kNvLowerSize)) return false;
+      for (uint32_t m = 0; m < 4096; ++m) {
+        if (!((nv_ld<uint64_t>(lo + 544 + 8 * (m >> 6)) >> (m & 63)) & 1u)) continue;
+        const uint8_t *leaf = lo + nv_ld<int64_t>(lo + 1088 + 8 * (size_t)m);
+        if (!in(leaf, 96) || !in(leaf, nv_leaf_size(g->type, leaf))) return false;
+        if (g->type == 16 && (leaf[15] >> 5) > 4) return false;
+      }
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+ref_nvdb *ref_nvdb_create(const void *data, unsigned long long bytes) {
+  if (!data || bytes < 672 + 64 + 64) return nullptr;
+  auto *g = new ref_nvdb_s;
+  g->words.resize((bytes + 7) / 8);
+  std::memcpy(g->words.data(), data, bytes);
+  g->buf = reinterpret_cast<const uint8_t *>(g->words.data());
+  g->bytes = bytes;
+  const uint8_t *b = g->buf;
+  const uint64_t magic = nv_ld<uint64_t>(b);
+  const uint32_t version = nv_ld<uint32_t>(b + 16);
+  g->type = (int)nv_ld<uint32_t>(b + 636);
+  const int64_t rootOff = nv_ld<int64_t>(b + 672 + 24);
+  const bool typeOk = g->type == 1 || (g->type >= 13 && g->type <= 16);
+  if ((magic & 0x00FFFFFFFFFFFFFFull) != 0x004244566f6e614eull || (version >> 21) != 32u || !typeOk ||
+      rootOff != 64 || nv_ld<uint64_t>(b + 32) > bytes) {
+    delete g;
+    return nullptr;
+  }
+  g->root = 672 + (size_t)rootOff;
+  if (!nv_check(g)) {
+    delete g;
+    return nullptr;
+  }
+  for (int k = 0; k < 3; ++k) {
+    g->ilo[k] = nv_ld<int32_t>(b + g->root + 4 * k);
+    g->ihi[k] = nv_ld<int32_t>(b + g->root + 12 + 4 * k);
+  }
+  std::memcpy(g->invMatF, b + 296 + 36, 36);
+  std::memcpy(g->vecF, b + 296 + 72, 12);
+  // NanoVDBData::commit: the world box as floats, the grid size
+  ref_volume_s &v = g->grid;
+  const double *wb = reinterpret_cast<const double *>(b + 560);
+  v.wlo = mk((float)wb[0], (float)wb[1], (float)wb[2]);
+  v.whi = mk((float)wb[3], (float)wb[4], (float)wb[5]);
+  const int gs[3] = {g->ihi[0] - g->ilo[0] + 1, g->ihi[1] - g->ilo[1] + 1, g->ihi[2] - g->ilo[2] + 1};
+  // buildMCs (NanoVDB.cu:82-102): divRoundUp(gridSize, 16) cells over the
+  // world box, spacing = size * rcp(dims)
+  for (int k = 0; k < 3; ++k) v.m[k] = (gs[k] + 15) / 16;
+  v.mcOrigin = v.wlo;
+  v.mcSpacing = mul(sub(v.whi, v.wlo), inv(mk((float)v.m[0], (float)v.m[1], (float)v.m[2])));
+  // NanoVDBData_computeMCs (NanoVDB.cu:28-62): the range of the voxels
+  // lower + id * gridSize / dims .. lower + (id + 1) * gridSize / dims + 1
+  const size_t nc = (size_t)v.m[0] * v.m[1] * v.m[2];
+  v.rlo.assign(nc, kInf);
+  v.rhi.assign(nc, -kInf);
+  v.maj.assign(nc, 0.f);
+  parallel_for((int)nc, default_threads(0), [&](int tid) {
+    const int id[3] = {tid % v.m[0], (tid / v.m[0]) % v.m[1], tid / (v.m[0] * v.m[1])};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = g->ilo[k] + (id[k] * gs[k]) / v.m[k];
+      hi[k] = g->ilo[k] + ((id[k] + 1) * gs[k]) / v.m[k] + 1;
+    }
+    float rl = kInf, rh = -kInf;
+    for (int x = lo[0]; x <= hi[0]; ++x)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int z = lo[2]; z <= hi[2]; ++z) {
+          const float s = nv_value(g, x, y, z);
+          rl = std::fmin(rl, s);
+          rh = std::fmax(rh, s);
+        }
+    v.rlo[tid] = rl;
+    v.rhi[tid] = rh;
+  });
+  const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  v.tf.assign(one, one + 8);
+  v.numTF = 2;
+  return g;
+}
+
+void ref_nvdb_destroy(ref_nvdb *g) { delete g; }
+
+void ref_nvdb_set_tf(ref_nvdb *g, const float *values, int numValues, float lo, float hi, float base) {
+  ref_volume_s &v = g->grid;
+  v.tf.assign(values, values + 4 * (size_t)numValues);
+  v.numTF = numValues;
+  v.dlo = lo;
+  v.dhi = hi;
+  v.base = base;
+  for (size_t i = 0; i < v.maj.size(); ++i) v.maj[i] = tfmaj(&v, v.rlo[i], v.rhi[i]);
+}
+
+void ref_nvdb_mc_dims(const ref_nvdb *g, int *dims) {
+  for (int k = 0; k < 3; ++k) dims[k] = g->grid.m[k];
+}
+
+void ref_nvdb_get_mcs(const ref_nvdb *g, float *ranges, float *maj) {
+  const ref_volume_s &v = g->grid;
+  for (size_t i = 0; i < v.maj.size(); ++i) {
+    if (ranges) { ranges[2 * i] = v.rlo[i]; ranges[2 * i + 1] = v.rhi[i]; }
+    if (maj) maj[i] = v.maj[i];
+  }
+}
+
+void ref_nvdb_boxes(const ref_nvdb *g, float *worldLoHi, int *indexLoHi, float *mcOriginSpacing) {
+  const ref_volume_s &v = g->grid;
+  const float w[6] = {v.wlo.x, v.wlo.y, v.wlo.z, v.whi.x, v.whi.y, v.whi.z};
+  std::memcpy(worldLoHi, w, sizeof(w));
+  for (int k = 0; k < 3; ++k) {
+    indexLoHi[k] = g->ilo[k];
+    indexLoHi[3 + k] = g->ihi[k];
+  }
+  const float m[6] = {v.mcOrigin.x, v.mcOrigin.y, v.mcOrigin.z, v.mcSpacing.x, v.mcSpacing.y, v.mcSpacing.z};
+  std::memcpy(mcOriginSpacing, m, sizeof(m));
+}
+
+void ref_nvdb_sample(const ref_nvdb *g, const float *points, float *out, int n, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) { out[i] = nv_sample(g, ld3(points + 3 * (size_t)i)); });
+}
+
+void ref_nvdb_values(const ref_nvdb *g, const int *coords, float *out, int n) {
+  for (int i = 0; i < n; ++i) out[i] = nv_value(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+}
+
+long long ref_nvdb_render(const ref_nvdb *g, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                          int y1, int threads, long long *samples) {
+  return render_volume_g(&g->grid, [&](V3 P) { return nv_sample(g, P); }, cam, p, accum, y0, y1, threads, samples);
+}
+
+}  // extern "C"
+
+// ============================================================================
 // Unstructured tetrahedral volumes (barney/umesh)
 // ============================================================================
 struct ref_umesh_s {

@@ -179,6 +179,23 @@ void ref_amr_sample(const ref_amr *a, const float *points, float *out, int n, in
 long long ref_amr_render(const ref_amr *a, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
                          int y1, int threads, long long *samples);
 
+/* --- NanoVDB fields (barney/volume/NanoVDB.*) --- */
+typedef struct ref_nvdb_s ref_nvdb;
+/* a NanoVDB 32.x grid buffer of float, Fp4, Fp8, Fp16 or FpN values; NULL
+ * when the header is not such a grid or a node lies outside the buffer */
+ref_nvdb *ref_nvdb_create(const void *data, unsigned long long bytes);
+void ref_nvdb_destroy(ref_nvdb *g);
+void ref_nvdb_set_tf(ref_nvdb *g, const float *values, int numValues, float lo, float hi, float baseDensity);
+void ref_nvdb_mc_dims(const ref_nvdb *g, int *dims);
+void ref_nvdb_get_mcs(const ref_nvdb *g, float *ranges, float *majorants);
+/* world box (lo, hi), index box (lo, hi), macrocell origin and spacing */
+void ref_nvdb_boxes(const ref_nvdb *g, float *worldLoHi, int *indexLoHi, float *mcOriginSpacing);
+void ref_nvdb_sample(const ref_nvdb *g, const float *points, float *out, int n, int threads);
+/* ReadAccessor::getValue at integer coordinates (int3 each) */
+void ref_nvdb_values(const ref_nvdb *g, const int *coords, float *out, int n);
+long long ref_nvdb_render(const ref_nvdb *g, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
+                          int y1, int threads, long long *samples);
+
 /* --- unstructured volume (barney/umesh) --- */
 typedef struct ref_umesh_s ref_umesh;
 /* vertices float3, per-vertex scalars, tets int4 (positively oriented) */

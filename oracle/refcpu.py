@@ -13,10 +13,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "librefcpu.so")
 
 
+# the NanoVDB headers the reference vendors; present in the build container
+# only (the golden-vector generator built on them is not needed elsewhere)
+NANOVDB_INC = "/root/reference/3rdParty/nanovdb/include"
+
+
 def build():
     r = subprocess.run(["make", "-C", HERE, "-s"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    if os.path.isdir(NANOVDB_INC):
+        r = subprocess.run(["make", "-C", HERE, "-s", "_ref/nvdb_ref", f"NANOVDB_INC={NANOVDB_INC}"],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("oracle/_ref build failed:\n" + r.stdout + r.stderr)
     return LIB
 
 
@@ -94,6 +104,15 @@ def load():
         "ref_amr_get_mcs": (None, [P, P, P]),
         "ref_amr_sample": (None, [P, P, P, C.c_int, C.c_int, C.c_int]),
         "ref_amr_render": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_nvdb_create": (P, [P, C.c_ulonglong]),
+        "ref_nvdb_destroy": (None, [P]),
+        "ref_nvdb_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "ref_nvdb_mc_dims": (None, [P, P]),
+        "ref_nvdb_get_mcs": (None, [P, P, P]),
+        "ref_nvdb_boxes": (None, [P, P, P, P]),
+        "ref_nvdb_sample": (None, [P, P, P, C.c_int, C.c_int]),
+        "ref_nvdb_values": (None, [P, P, P, C.c_int]),
+        "ref_nvdb_render": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_umesh_create_cells": (P, [P, P, C.c_int, P, C.c_int, P, P, C.c_int]),
         "ref_umesh_destroy": (None, [P]),
         "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -370,6 +389,69 @@ class Amr:
     def __del__(self):
         try:
             load().ref_amr_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class NanoVDB:
+    """Oracle NanoVDB field over a grid buffer (bytes / uint8 array): the
+    macrocell grid, transfer function, point sampling and render."""
+
+    def __init__(self, grid):
+        self._g = np.frombuffer(bytes(grid), np.uint8) if not isinstance(grid, np.ndarray) else \
+            np.ascontiguousarray(grid, np.uint8)
+        self.handle = load().ref_nvdb_create(self._g.ctypes.data, self._g.size)
+        if not self.handle:
+            raise ValueError("ref_nvdb_create: not a readable NanoVDB grid")
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
+        load().ref_nvdb_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]), float(domain[1]),
+                               float(base_density))
+
+    def mc_dims(self):
+        d = np.zeros(3, np.int32)
+        load().ref_nvdb_mc_dims(self.handle, d.ctypes.data)
+        return tuple(int(x) for x in d)
+
+    def boxes(self):
+        w = np.zeros(6, np.float32)
+        i = np.zeros(6, np.int32)
+        m = np.zeros(6, np.float32)
+        load().ref_nvdb_boxes(self.handle, w.ctypes.data, i.ctypes.data, m.ctypes.data)
+        return dict(world_lo=w[:3], world_hi=w[3:], index_lo=i[:3], index_hi=i[3:], mc_origin=m[:3],
+                    mc_spacing=m[3:])
+
+    def macrocells(self):
+        mx, my, mz = self.mc_dims()
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        load().ref_nvdb_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data)
+        return ranges, maj
+
+    def sample(self, points, threads=0):
+        P = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(len(P), np.float32)
+        load().ref_nvdb_sample(self.handle, P.ctypes.data, out.ctypes.data, len(P), threads)
+        return out
+
+    def values(self, coords):
+        c = np.ascontiguousarray(coords, np.int32).reshape(-1, 3)
+        out = np.empty(len(c), np.float32)
+        load().ref_nvdb_values(self.handle, c.ctypes.data, out.ctypes.data, len(c))
+        return out
+
+    def render(self, cam, params, y0=0, y1=None, threads=0, accum=None):
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        ns = C.c_longlong(0)
+        n = load().ref_nvdb_render(self.handle, C.byref(cam), C.byref(params), acc.ctypes.data, y0, y1, threads,
+                                   C.byref(ns))
+        return acc, n, ns.value
+
+    def __del__(self):
+        try:
+            load().ref_nvdb_destroy(self.handle)
         except Exception:
             pass
 
