@@ -439,6 +439,78 @@ class Volume:
             pass
 
 
+class NanoVDB:
+    """NanoVDB field (barney's "NanoVDB" scalar field): a NanoVDB grid buffer
+    (bytes or a uint8 array, e.g. from barney_amd.nanovdb.from_dense) of
+    float, Fp4, Fp8, Fp16 or FpN values; macrocell grid, transfer function,
+    point samples, iso-surface rays and the delta-tracked render."""
+
+    def __init__(self, grid, on_device=False):
+        lib = init()
+        g = np.frombuffer(bytes(grid), np.uint8) if not isinstance(grid, np.ndarray) else \
+            np.ascontiguousarray(grid, np.uint8).reshape(-1)
+        d = capi.bnamd_nanovdb_desc()
+        if on_device:
+            dg = DeviceArray(g)
+            d.data, d.bytes, d.dataOnDevice = dg.addr, g.size, 1
+        else:
+            d.data, d.bytes, d.dataOnDevice = g.ctypes.data, g.size, 0
+        self.handle = C.c_void_p()
+        check(lib.bnamd_nanovdb_create(C.byref(d), None, C.byref(self.handle)))
+
+    def info(self):
+        i = capi.bnamd_nanovdb_info()
+        check(capi.load().bnamd_nanovdb_info_get(self.handle, C.byref(i)))
+        return {"gridType": i.gridType, "indexLower": list(i.indexLower), "indexUpper": list(i.indexUpper),
+                "mcDims": tuple(i.mcDims), "lower": list(i.worldLower), "upper": list(i.worldUpper),
+                "mcOrigin": list(i.mcOrigin), "mcSpacing": list(i.mcSpacing), "voxelSize": list(i.voxelSize),
+                "gridBytes": i.gridBytes, "buildMs": i.buildMs}
+
+    def set_tf(self, values, domain=(0.0, 1.0), base_density=1.0):
+        values = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 4)
+        check(capi.load().bnamd_nanovdb_set_tf(self.handle, values.ctypes.data, len(values), float(domain[0]),
+                                               float(domain[1]), float(base_density)))
+
+    def macrocells(self):
+        mx, my, mz = self.info()["mcDims"]
+        ranges = np.empty((mz, my, mx, 2), np.float32)
+        maj = np.empty((mz, my, mx), np.float32)
+        check(capi.load().bnamd_nanovdb_get_mcs(self.handle, ranges.ctypes.data, maj.ctypes.data))
+        return ranges, maj
+
+    def sample(self, points):
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        dp = DeviceArray(p)
+        do = DeviceArray.empty((len(p),), np.float32)
+        check(capi.load().bnamd_nanovdb_sample_points(self.handle, dp.addr, do.addr, len(p), None))
+        return do.numpy()
+
+    def iso_rays(self, rays, iso):
+        """rays: [n, 8] (org, tmin, dir, tmax) -> [n, 4] (t or inf, normal)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        dr = DeviceArray(r)
+        do = DeviceArray.empty((len(r), 4), np.float32)
+        check(capi.load().bnamd_nanovdb_iso_rays(self.handle, dr.addr, float(iso), do.addr, len(r), None))
+        return do.numpy()
+
+    def render(self, cam, params, accum=None):
+        acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+        st = capi.bnamd_volume_stats()
+        check(capi.load().bnamd_nanovdb_render(self.handle, C.byref(cam), C.byref(params), acc.addr, None,
+                                               C.byref(st)))
+        return {"accum": acc, "stats": {"samples": st.samples, "ddaSteps": st.ddaSteps,
+                                        "raysTraced": st.raysTraced, "ms": st.ms,
+                                        "zeroMajorantSteps": st.zeroMajorantSteps}}
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value:
+                capi.load().bnamd_nanovdb_destroy(self.handle)
+            self.handle = None
+        except Exception:
+            pass
+
+
 class Amr:
     """Block-structured AMR volume (BlockStructuredField's layout): blocks
     (origins, dims int3; levels; offsets of their first scalar), per-level

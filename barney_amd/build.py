@@ -31,6 +31,7 @@ SOURCES = [
     "trace.hip",
     "render_surface.hip",
     "volume.hip",
+    "nanovdb.hip",
     "ea.hip",
     "render_batch.hip",
     "umesh.hip",

@@ -81,6 +81,17 @@ class bnamd_amr_info(C.Structure):
                 ("worldLower", C.c_float * 3), ("worldUpper", C.c_float * 3), ("buildMs", C.c_float)]
 
 
+class bnamd_nanovdb_desc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("bytes", C.c_ulonglong), ("dataOnDevice", C.c_int)]
+
+
+class bnamd_nanovdb_info(C.Structure):
+    _fields_ = [("gridType", C.c_int), ("indexLower", C.c_int * 3), ("indexUpper", C.c_int * 3),
+                ("mcDims", C.c_int * 3), ("worldLower", C.c_float * 3), ("worldUpper", C.c_float * 3),
+                ("mcOrigin", C.c_float * 3), ("mcSpacing", C.c_float * 3), ("voxelSize", C.c_float * 3),
+                ("gridBytes", C.c_ulonglong), ("buildMs", C.c_float)]
+
+
 class bnamd_volume_stats(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("ddaSteps", C.c_longlong), ("raysTraced", C.c_longlong),
                 ("ms", C.c_float), ("zeroMajorantSteps", C.c_longlong),
@@ -166,6 +177,15 @@ SIGNATURES = {
     "bnamd_amr_sample_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_amr_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
                                    C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
+    "bnamd_nanovdb_create": (C.c_int, [C.POINTER(bnamd_nanovdb_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bnamd_nanovdb_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_nanovdb_info_get": (C.c_int, [C.c_void_p, C.POINTER(bnamd_nanovdb_info)]),
+    "bnamd_nanovdb_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "bnamd_nanovdb_get_mcs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bnamd_nanovdb_sample_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bnamd_nanovdb_render": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                       C.c_void_p, C.c_void_p, C.POINTER(bnamd_volume_stats)]),
+    "bnamd_nanovdb_iso_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_volume_sample_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                            C.c_void_p]),
     "bnamd_umesh_create": (C.c_int, [C.POINTER(bnamd_umesh_desc), C.c_void_p, C.POINTER(C.c_void_p)]),

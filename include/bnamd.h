@@ -387,6 +387,58 @@ int bnamd_amr_render(const bnamd_amr *amr, const bnamd_camera *camera, const bna
                      float *accum, void *stream, bnamd_volume_stats *stats);
 
 /* ------------------------------------------------------------------------
+ * NanoVDB fields (barney/volume/NanoVDB.h, NanoVDB.cu): one NanoVDB grid
+ * (format major version 32: GridData, TreeData, root tiles, 32^3 and 16^3
+ * internal nodes, 8^3 leaves) of float, Fp4, Fp8, Fp16 or FpN values, as
+ * barney's "NanoVDB" scalar field takes it in its "data" member. Sampled as
+ * NanoVDBDataSampler::DD::sample (NanoVDB.h:98-127: worldToIndexF, minus
+ * half a voxel, clamped to the index box, SampleFromVoxels<1> trilinear),
+ * with the macrocell grid of NanoVDBData::buildMCs / computeMCs
+ * (NanoVDB.cu:28-102: divRoundUp(gridSize, 16) cells over the world box),
+ * MCVolumeAccel delta tracking and MCIsoSurfaceAccel iso-surfaces.
+ * --------------------------------------------------------------------- */
+typedef struct {
+  const void *data;         /* the grid buffer, 8-byte aligned */
+  unsigned long long bytes;
+  int dataOnDevice;         /* 0: host memory, 1: device memory */
+} bnamd_nanovdb_desc;
+
+typedef struct bnamd_nanovdb_s bnamd_nanovdb;
+
+typedef struct {
+  int gridType; /* nanovdb::GridType: 1 float, 13 Fp4, 14 Fp8, 15 Fp16, 16 FpN */
+  int indexLower[3], indexUpper[3];
+  int mcDims[3];
+  float worldLower[3], worldUpper[3];
+  float mcOrigin[3], mcSpacing[3];
+  float voxelSize[3];
+  unsigned long long gridBytes;
+  float buildMs;
+} bnamd_nanovdb_info;
+
+/* Replaces NanoVDBData::setData("data") + commit + buildMCs (NanoVDB.cu:82-102,
+ * 190-235). The buffer is read during the call only (the field keeps its own
+ * copy in device memory); a buffer that is not such a grid, or whose nodes
+ * reach outside it, gives BNAMD_ERROR with a message. */
+int bnamd_nanovdb_create(const bnamd_nanovdb_desc *desc, void *stream, bnamd_nanovdb **out);
+int bnamd_nanovdb_destroy(bnamd_nanovdb *g);
+int bnamd_nanovdb_info_get(const bnamd_nanovdb *g, bnamd_nanovdb_info *info);
+/* values: host float4[numValues]; recomputes the majorants */
+int bnamd_nanovdb_set_tf(bnamd_nanovdb *g, const float *values, int numValues, float domainLo, float domainHi,
+                         float baseDensity);
+/* host copies of the macrocell ranges (float2 per cell) and majorants */
+int bnamd_nanovdb_get_mcs(const bnamd_nanovdb *g, float *ranges, float *majorants);
+/* out[i] = field value at points[i] (device float3) */
+int bnamd_nanovdb_sample_points(const bnamd_nanovdb *g, const float *points, float *out, int numPoints,
+                                void *stream);
+/* delta-tracked render, as bnamd_volume_render */
+int bnamd_nanovdb_render(const bnamd_nanovdb *g, const bnamd_camera *camera, const bnamd_render_params *params,
+                         float *accum, void *stream, bnamd_volume_stats *stats);
+/* iso-surface query per ray, as bnamd_volume_iso_rays */
+int bnamd_nanovdb_iso_rays(const bnamd_nanovdb *g, const bnamd_ray *rays, float isoValue, float *out,
+                           int numRays, void *stream);
+
+/* ------------------------------------------------------------------------
  * Unstructured volumes (barney/umesh): UMeshField with per-vertex scalars,
  * UMeshCuBQLSampler point location (UMeshCuBQLSampler.h:51-67,
  * UMeshField.h:362-501 eltScalar for tets, pyramids, prisms and hexahedra),

@@ -3303,6 +3303,23 @@ void ref_volume_iso_rays(const ref_volume *v, const float *rays, float isoValue,
   });
 }
 
+// MCIsoSurfaceAccel over a NanoVDB field (MCAccelerator.h:338-567 with
+// NanoVDBDataSampler, NanoVDB.h:98-127) over its macrocell value ranges.
+void ref_nvdb_iso_rays(const ref_nvdb *g, const float *rays, float isoValue, float *out, int n, int threads) {
+  parallel_for(n, default_threads(threads), [&](int i) {
+    const float *r = rays + 8 * (size_t)i;
+    Rng seed;
+    auto field = [&](V3 P) { return nv_sample(g, P); };
+    const V3 org = ld3(r), dir = fix_dir(ld3(r + 4));
+    const IHit h = iso_isect(&g->grid, field, isoValue, org, dir, r[3], r[7], r[7], seed, fnv3(0, 0, 0));
+    float *o = out + 4 * (size_t)i;
+    o[0] = h.hit ? h.t : kInf;
+    o[1] = h.hit ? h.N.x : 0.f;
+    o[2] = h.hit ? h.N.y : 0.f;
+    o[3] = h.hit ? h.N.z : 0.f;
+  });
+}
+
 void ref_umesh_iso_rays(const ref_umesh *u, const float *rays, float isoValue, float *out, int n, int threads) {
   parallel_for(n, default_threads(threads), [&](int i) {
     const float *r = rays + 8 * (size_t)i;

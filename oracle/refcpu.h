@@ -193,6 +193,8 @@ void ref_nvdb_boxes(const ref_nvdb *g, float *worldLoHi, int *indexLoHi, float *
 void ref_nvdb_sample(const ref_nvdb *g, const float *points, float *out, int n, int threads);
 /* ReadAccessor::getValue at integer coordinates (int3 each) */
 void ref_nvdb_values(const ref_nvdb *g, const int *coords, float *out, int n);
+/* iso-surface rays, as ref_volume_iso_rays */
+void ref_nvdb_iso_rays(const ref_nvdb *g, const float *rays, float isoValue, float *out, int n, int threads);
 long long ref_nvdb_render(const ref_nvdb *g, const ref_camera *cam, const ref_render_params *p, float *accum, int y0,
                           int y1, int threads, long long *samples);
 

@@ -113,6 +113,7 @@ def load():
         "ref_nvdb_sample": (None, [P, P, P, C.c_int, C.c_int]),
         "ref_nvdb_values": (None, [P, P, P, C.c_int]),
         "ref_nvdb_render": (C.c_longlong, [P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
+        "ref_nvdb_iso_rays": (None, [P, P, C.c_float, P, C.c_int, C.c_int]),
         "ref_umesh_create_cells": (P, [P, P, C.c_int, P, C.c_int, P, P, C.c_int]),
         "ref_umesh_destroy": (None, [P]),
         "ref_umesh_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -439,6 +440,13 @@ class NanoVDB:
         c = np.ascontiguousarray(coords, np.int32).reshape(-1, 3)
         out = np.empty(len(c), np.float32)
         load().ref_nvdb_values(self.handle, c.ctypes.data, out.ctypes.data, len(c))
+        return out
+
+    def iso_rays(self, rays, iso, threads=0):
+        """rays: [n, 8] (org, tmin, dir, tmax) -> [n, 4] (t or inf, normal)."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((len(r), 4), np.float32)
+        load().ref_nvdb_iso_rays(self.handle, r.ctypes.data, float(iso), out.ctypes.data, len(r), threads)
         return out
 
     def render(self, cam, params, y0=0, y1=None, threads=0, accum=None):
