@@ -445,11 +445,12 @@ class NanoVDB:
     float, Fp4, Fp8, Fp16 or FpN values; macrocell grid, transfer function,
     point samples, iso-surface rays and the delta-tracked render."""
 
-    def __init__(self, grid, on_device=False):
+    def __init__(self, grid, on_device=False, leaf_table=True):
         lib = init()
         g = np.frombuffer(bytes(grid), np.uint8) if not isinstance(grid, np.ndarray) else \
             np.ascontiguousarray(grid, np.uint8).reshape(-1)
         d = capi.bnamd_nanovdb_desc()
+        d.noLeafTable = 0 if leaf_table else 1
         if on_device:
             dg = DeviceArray(g)
             d.data, d.bytes, d.dataOnDevice = dg.addr, g.size, 1

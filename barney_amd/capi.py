@@ -82,7 +82,7 @@ class bnamd_amr_info(C.Structure):
 
 
 class bnamd_nanovdb_desc(C.Structure):
-    _fields_ = [("data", C.c_void_p), ("bytes", C.c_ulonglong), ("dataOnDevice", C.c_int)]
+    _fields_ = [("data", C.c_void_p), ("bytes", C.c_ulonglong), ("dataOnDevice", C.c_int), ("noLeafTable", C.c_int)]
 
 
 class bnamd_nanovdb_info(C.Structure):

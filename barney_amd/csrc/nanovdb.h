@@ -40,6 +40,13 @@ struct NvdbDD {
   int type;
   int ilo[3], ihi[3];         // index box (the root's)
   float invMatF[9], vecF[3];  // Map::mInvMatF, mVecF
+  // leaf lookup: per 8^3 block over the index box (grown by one voxel),
+  // the block's leaf offset from the grid start in 32-byte units, 0 where
+  // no leaf holds it (tiles, background: the tree walk answers); NULL when
+  // the grid has no table
+  const uint8_t *base;
+  const uint32_t *leafTable;
+  int tlo[3], tdim[3];
 };
 
 template <class T>
@@ -108,8 +115,23 @@ BN_HD NvSpot nv_locate(const NvdbDD &g, int x, int y, int z) {
   return NvSpot{leaf, ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u), 0.f};
 }
 
+// nv_locate through the leaf table where it covers the block: one L2-held
+// load instead of the root / upper / lower walk (the same leaf either way)
+BN_HD NvSpot nv_find(const NvdbDD &g, int x, int y, int z) {
+  if (g.leafTable) {
+    const uint32_t bx = (uint32_t)((x >> 3) - g.tlo[0]), by = (uint32_t)((y >> 3) - g.tlo[1]),
+                   bz = (uint32_t)((z >> 3) - g.tlo[2]);
+    if (bx < (uint32_t)g.tdim[0] && by < (uint32_t)g.tdim[1] && bz < (uint32_t)g.tdim[2]) {
+      const uint32_t e = g.leafTable[bx + (size_t)g.tdim[0] * (by + (size_t)g.tdim[1] * bz)];
+      if (e) return NvSpot{g.base + 32 * (size_t)e, ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u),
+                           0.f};
+    }
+  }
+  return nv_locate(g, x, y, z);
+}
+
 BN_HD float nv_value(const NvdbDD &g, int x, int y, int z) {
-  const NvSpot s = nv_locate(g, x, y, z);
+  const NvSpot s = nv_find(g, x, y, z);
   return s.leaf ? nv_leaf_value(g.type, s.leaf, s.n) : s.value;
 }
 
@@ -139,7 +161,7 @@ BN_HD float nvdb_sample(const NvdbDD &g, vec3f P) {
     w[k] = (float)(q - fl);
   }
   float v[8];  // v[dx * 4 + dy * 2 + dz]
-  const NvSpot s = nv_locate(g, c[0], c[1], c[2]);
+  const NvSpot s = nv_find(g, c[0], c[1], c[2]);
   if ((c[0] & 7) != 7 && (c[1] & 7) != 7 && (c[2] & 7) != 7) {
     if (s.leaf) {
 #pragma unroll

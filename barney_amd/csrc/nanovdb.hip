@@ -20,6 +20,7 @@ struct Nvdb {
   DevBuf<float2> ranges;
   DevBuf<float> majorants;
   DevBuf<float4> tf;
+  DevBuf<uint32_t> leafTable;
   bool haveTF = false;
   double voxelSize[3] = {0, 0, 0};
   float buildMs = 0.f;
@@ -39,7 +40,8 @@ T host_load(const uint8_t *p) {
 // Checks that b holds one NanoVDB 32.x grid of a supported value type whose
 // nodes, reached from the root, all lie inside the buffer (so that device
 // walks stay in bounds).
-void check_grid(const uint8_t *b, size_t bytes, int &type) {
+template <class LeafFn>
+void check_grid(const uint8_t *b, size_t bytes, int &type, LeafFn &&onLeaf) {
   BN_REQUIRE(bytes >= 672 + 64 + 64, "bnamd_nanovdb_create: buffer smaller than a grid header");
   const uint64_t magic = host_load<uint64_t>(b);
   BN_REQUIRE((magic & 0x00FFFFFFFFFFFFFFull) == 0x004244566f6e614eull,
@@ -61,6 +63,11 @@ void check_grid(const uint8_t *b, size_t bytes, int &type) {
   for (uint32_t i = 0; i < nt; ++i) {
     const int64_t ch = host_load<int64_t>(root + 64 + 32 * (size_t)i + 8);
     if (!ch) continue;
+    // the upper node's origin from its tile key (RootData::KeyToCoord)
+    const uint64_t key = host_load<uint64_t>(root + 64 + 32 * (size_t)i);
+    const int ux = (int)(uint32_t)(((key >> 42) & 0x1FFFFFu) << 12);
+    const int uy = (int)(uint32_t)(((key >> 21) & 0x1FFFFFu) << 12);
+    const int uz = (int)(uint32_t)((key & 0x1FFFFFu) << 12);
     const uint8_t *up = root + ch;
     BN_REQUIRE(ch > 0 && inside(up, kUpperBytes), "bnamd_nanovdb_create: upper node outside the buffer");
     for (uint32_t n = 0; n < 32768; ++n) {
@@ -86,6 +93,10 @@ void check_grid(const uint8_t *b, size_t bytes, int &type) {
           }
         }
         BN_REQUIRE(inside(leaf, size), "bnamd_nanovdb_create: leaf outside the buffer");
+        // the leaf's origin: the upper origin plus the two table slots
+        onLeaf(leaf, ux + (int)((n >> 10) << 7) + (int)((m >> 8) << 3),
+               uy + (int)(((n >> 5) & 31) << 7) + (int)(((m >> 4) & 15) << 3),
+               uz + (int)((n & 31) << 7) + (int)((m & 15) << 3));
       }
     }
   }
@@ -175,7 +186,12 @@ extern "C" int bnamd_nanovdb_create(const bnamd_nanovdb_desc *desc, void *stream
     }
     const uint8_t *b = reinterpret_cast<const uint8_t *>(host.data());
     int type = 0;
-    check_grid(b, bytes, type);
+    struct LeafAt {
+      const uint8_t *p;
+      int x, y, z;
+    };
+    std::vector<LeafAt> leaves;
+    check_grid(b, bytes, type, [&](const uint8_t *leaf, int x, int y, int z) { leaves.push_back({leaf, x, y, z}); });
     auto *ug = new bnamd_nanovdb_s;
     try {
       Nvdb &G = ug->g;
@@ -192,6 +208,30 @@ extern "C" int bnamd_nanovdb_create(const bnamd_nanovdb_desc *desc, void *stream
         d.ilo[k] = host_load<int32_t>(root + 4 * k);
         d.ihi[k] = host_load<int32_t>(root + 12 + 4 * k);
         BN_REQUIRE(d.ihi[k] >= d.ilo[k], "bnamd_nanovdb_create: empty index box");
+      }
+      // the leaf table over the index box grown by one voxel (the
+      // stencil's far corners), when it stays within 2^24 blocks and every
+      // leaf offset is a multiple of 32 bytes (as NanoVDB writers align them)
+      {
+        long long tcells = 1;
+        for (int k = 0; k < 3; ++k) {
+          d.tlo[k] = d.ilo[k] >> 3;
+          d.tdim[k] = ((d.ihi[k] + 1) >> 3) - d.tlo[k] + 1;
+          tcells *= d.tdim[k];
+        }
+        bool aligned = true;
+        for (const LeafAt &l : leaves) aligned = aligned && ((l.p - b) % 32 == 0) && (l.p - b) / 32 < (1ll << 32);
+        if (!desc->noLeafTable && tcells <= (1ll << 24) && aligned) {
+          std::vector<uint32_t> table((size_t)tcells, 0u);
+          for (const LeafAt &l : leaves) {
+            const int bx = (l.x >> 3) - d.tlo[0], by = (l.y >> 3) - d.tlo[1], bz = (l.z >> 3) - d.tlo[2];
+            if (bx < 0 || by < 0 || bz < 0 || bx >= d.tdim[0] || by >= d.tdim[1] || bz >= d.tdim[2]) continue;
+            table[bx + (size_t)d.tdim[0] * (by + (size_t)d.tdim[1] * bz)] = (uint32_t)((l.p - b) / 32);
+          }
+          G.leafTable.upload(table.data(), table.size(), s);
+          d.leafTable = G.leafTable.ptr;
+        }
+        d.base = reinterpret_cast<const uint8_t *>(G.grid.ptr);
       }
       std::memcpy(d.invMatF, b + 296 + 36, sizeof(d.invMatF));
       std::memcpy(d.vecF, b + 296 + 72, sizeof(d.vecF));

@@ -141,11 +141,9 @@ def from_dense(values, voxel_size=1.0, origin=(0.0, 0.0, 0.0), index_origin=(0, 
     vdev = np.float32(active_vals.astype(np.float64).std())
 
     # ---- leaves
-    leaf_pos = {}  # leaf index -> byte offset
     lv_o, la_o, org_o = lv[leaf_order], la[leaf_order], leaf_org[leaf_order]
-    offs = leaf_off + lsize * np.arange(nF, dtype=np.int64)
-    for r, li in enumerate(leaf_order):
-        leaf_pos[int(li)] = int(offs[r])
+    leaf_pos = np.empty(nF, np.int64)  # leaf index -> byte offset
+    leaf_pos[leaf_order] = leaf_off + lsize * np.arange(nF, dtype=np.int64)
     head = np.zeros((nF, 96), np.uint8)
     head_i = head.view(np.int32)  # [nF, 24]
     head_i[:, 0:3] = org_o.astype(np.int32)
@@ -186,12 +184,13 @@ def from_dense(values, voxel_size=1.0, origin=(0.0, 0.0, 0.0), index_origin=(0, 
     buf[leaf_off:leaf_off + nF * lsize] = leaves.ravel()
 
     # ---- lower nodes
-    low_pos = {}
-    for r, li in enumerate(low_order):
-        low_pos[int(li)] = low_off + _LOWER_SIZE * r
+    low_pos = np.empty(nL, np.int64)
+    low_pos[low_order] = low_off + _LOWER_SIZE * np.arange(nL, dtype=np.int64)
+    by_low = np.argsort(low_idx, kind="stable")
+    bounds = np.searchsorted(low_idx[by_low], np.arange(nL + 1))
     for li in range(nL):
-        base = low_pos[li]
-        sel = np.nonzero(low_idx == li)[0]
+        base = int(low_pos[li])
+        sel = by_low[bounds[li]:bounds[li + 1]]
         node = np.zeros(_LOWER_SIZE, np.uint8)
         ni = node[:32].view(np.int32)
         ni[0:3] = lows[li].astype(np.int32)
@@ -203,8 +202,7 @@ def from_dense(values, voxel_size=1.0, origin=(0.0, 0.0, 0.0), index_origin=(0, 
         table = np.zeros(4096, np.int64)
         table_f = table.view(np.float32).reshape(4096, 2)
         table_f[:, 0] = bg
-        for s_, lf in zip(leaf_slot[sel], sel):
-            table[s_] = leaf_pos[int(lf)] - base
+        table[leaf_slot[sel]] = leaf_pos[sel] - base
         node[1056:1072] = np.array([vmin, vmax, vavg, vdev], np.float32).view(np.uint8)
         node[1088:] = table.view(np.uint8)
         buf[base:base + _LOWER_SIZE] = node
@@ -224,8 +222,7 @@ def from_dense(values, voxel_size=1.0, origin=(0.0, 0.0, 0.0), index_origin=(0, 
         node[4128:8224] = np.packbits(child, bitorder="little")
         table = np.zeros(32768, np.int64)
         table.view(np.float32).reshape(32768, 2)[:, 0] = bg
-        for s_, lw in zip(low_slot[sel], sel):
-            table[s_] = low_pos[int(lw)] - base
+        table[low_slot[sel]] = low_pos[sel] - base
         node[8224:8240] = np.array([vmin, vmax, vavg, vdev], np.float32).view(np.uint8)
         node[8256:] = table.view(np.uint8)
         buf[base:base + _UPPER_SIZE] = node

@@ -55,10 +55,10 @@ def umesh_roofline(counted, ms):
             "cell_tests_per_sample": round(counted["cellTests"] / max(counted["samples"], 1), 2)}
 
 
-def volume_roofline(samples, dda_steps, ms):
-    """Structured-volume algorithmic bytes (32 B per field sample: eight fp32
-    corners; 4 B per macrocell step) as GB/s and fraction of HBM peak."""
-    b = 32 * samples + 4 * dda_steps
+def volume_roofline(samples, dda_steps, ms, value_bytes=4):
+    """Volume algorithmic bytes (eight corner values per field sample, 32 B
+    at fp32; 4 B per macrocell step) as GB/s and fraction of HBM peak."""
+    b = 8 * value_bytes * samples + 4 * dda_steps
     gbps = b / (ms * 1e-3) / 1e9
     return {"alg_bytes": int(b), "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
 
@@ -70,11 +70,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
+_W2 = {}
+
+
+def workload2(n):
+    """Workload 2's field, camera and transfer function (made once)."""
     from barney_amd import scenes
-    t0 = time.time()
-    w2 = scenes.workload2(n=n)
-    log(f"field {n}^3 generated in {time.time() - t0:.1f}s")
+    if n not in _W2:
+        t0 = time.time()
+        _W2[n] = scenes.workload2(n=n)
+        log(f"field {n}^3 generated in {time.time() - t0:.1f}s")
+    return _W2[n]
+
+
+def volume_bench(api, steps, warmup, n=512, cpu=True, cpu_seconds=15.0):
+    w2 = workload2(n)
     vol = api.Volume(w2["field"], cells_per_mc=w2["cells_per_mc"])
     vol.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
     W, H = w2["size"]
@@ -494,6 +504,57 @@ def amr_bench(api, n=128, block=16, steps=10, cpu=True, cpu_seconds=8.0):
     return res
 
 
+def nanovdb_bench(api, value_type, n=512, steps=10, cpu=True, cpu_seconds=6.0):
+    """NanoVDB field (an extra workload beside the five): workload 2's 512^3
+    field written as a NanoVDB grid of 'value_type' values (one voxel per
+    scalar, unit voxels), its camera, transfer function and frame
+    (1920x1080, 16 paths per pixel), rendered by the volume path tracer with
+    barney's NanoVDB sampler; the oracle renders a band of the same frame."""
+    from barney_amd import nanovdb
+    from oracle import refcpu
+    w2 = workload2(n)
+    t0 = time.time()
+    grid = nanovdb.from_dense(w2["field"], value_type=value_type)
+    g = api.NanoVDB(grid)
+    g.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
+    info = g.info()
+    log(f"nanovdb {value_type}: {grid.size / 1e6:.0f} MB grid written and loaded in {time.time() - t0:.1f}s "
+        f"(macrocells {info['mcDims']}, build {info['buildMs']:.0f} ms)")
+    W, H = w2["size"]
+    spp = w2["spp"]
+    p = api.render_params(W, H, spp=spp)
+    cam = api.camera(**w2["camera"])
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    g.render(cam, p, accum=acc)
+    ms, st = [], None
+    for _ in range(steps):
+        o = g.render(cam, p, accum=acc)
+        st = o["stats"]
+        ms.append(st["ms"])
+    t = float(np.median(ms))
+    vb = {"float": 4, "fp16": 2, "fp8": 1, "fp4": 0.5}.get(value_type, 4)
+    log(f"nanovdb {value_type}: {t:.2f} ms/frame, {st['samples'] / 1e6:.1f} M samples")
+    res = {"value_type": value_type, "grid_bytes": int(grid.size), "ms_per_frame": t,
+           "samples_per_frame": int(st["samples"]), "Msamples_per_s": st["samples"] / (t * 1e-3) / 1e6,
+           "dda_steps_per_frame": int(st["ddaSteps"]), "rays_per_frame": int(st["raysTraced"]),
+           "build_ms": info["buildMs"], "roofline": volume_roofline(st["samples"], st["ddaSteps"], t, vb)}
+    if cpu:
+        r = refcpu.NanoVDB(grid)
+        r.set_tf(w2["tf"], (0.0, 1.0), w2["base_density"])
+        rp = refcpu.render_params(W, H, spp=spp)
+        rcam = refcpu.camera(**w2["camera"])
+        racc = np.zeros((H, W, 4), np.float32)
+        base = band_timer(lambda y0, y1: r.render(rcam, rp, y0, y1, cpu_threads(), accum=racc)[2], H,
+                          cpu_seconds, "Msamples/s", f"the {W}x{H}x{spp}spp frame", keep_rows=True)
+        y0, y1 = base.pop("_rows")
+        res["cpu_baseline"] = base
+        img = acc.numpy()[y0:y1]
+        res["vs_oracle"] = {"rows": [y0, y1 - 1],
+                            "pixels_bit_exact": float(np.all(img == racc[y0:y1], axis=-1).mean())}
+        log(f"nanovdb {value_type} vs oracle: {res['vs_oracle']}")
+    return res
+
+
 def fb_bench(api, reps=20):
     """Frame readback of a 3840x2160 frame (8160 tiles of 32x32): tiled
     accumulation -> linear float4 + normals, -> sRGB RGBA8, and the resolve of
@@ -664,6 +725,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-umesh", action="store_true")
     ap.add_argument("--no-fb", action="store_true")
+    ap.add_argument("--no-nanovdb", action="store_true")
     ap.add_argument("--n", type=int, default=512, help="volume resolution (default: the workload's 512)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB streaming copy kernel (HBM byte-counter calibration)")
@@ -729,6 +791,9 @@ def main():
         out["extra"]["umesh_mixed_polyhedra"] = umesh_cells_bench(api, cpu=not args.no_cpu, cpu_seconds=4.0,
                                                                   polyhedra=True)
         out["extra"]["amr_two_levels"] = amr_bench(api, cpu=not args.no_cpu)
+    if not args.no_nanovdb:
+        out["extra"]["nanovdb_float"] = nanovdb_bench(api, "float", n=args.n, cpu=not args.no_cpu)
+        out["extra"]["nanovdb_fp8"] = nanovdb_bench(api, "fp8", n=args.n, cpu=not args.no_cpu)
     if not args.no_fb:
         out["extra"]["readback_3840x2160"] = fb_bench(api)
     if not args.no_ea:

@@ -401,6 +401,11 @@ typedef struct {
   const void *data;         /* the grid buffer, 8-byte aligned */
   unsigned long long bytes;
   int dataOnDevice;         /* 0: host memory, 1: device memory */
+  int noLeafTable;          /* 0: a per-8^3-block leaf lookup table over the
+                             * index box (4 B per block, made when it has at
+                             * most 2^24 blocks) spares most samples the tree
+                             * walk; 1: every lookup walks the tree. Samples
+                             * are the same either way. */
 } bnamd_nanovdb_desc;
 
 typedef struct bnamd_nanovdb_s bnamd_nanovdb;

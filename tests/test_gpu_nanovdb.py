@@ -47,6 +47,15 @@ def test_device_buffer_equals_host_buffer(gpu):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("case", ["tiles", "fp4"])
+def test_tree_walk_without_leaf_table(gpu, case):
+    f = fixture(case)
+    g = gpu.NanoVDB(f["grid"], leaf_table=False)
+    assert np.array_equal(g.sample(f["points"]).view(np.uint32), f["samples"].view(np.uint32))
+    ranges, _ = g.macrocells()
+    assert np.array_equal(ranges.reshape(-1, 2), f["ranges"])
+
+
 def test_malformed_buffers_rejected(gpu):
     from barney_amd.capi import BnamdError
     g = fixture("float")["grid"]
