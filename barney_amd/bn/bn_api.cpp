@@ -3,7 +3,7 @@
 // barney/Object.cpp), bnBuild turns the model's instances of groups of
 // geometries into one bnamd world (ModelSlot::build,
 // barney/ModelSlot.cpp:210-216) or builds its one volume (structured,
-// unstructured or block-structured AMR field), and its groups' env-map
+// unstructured, block-structured AMR or NanoVDB field), and its groups' env-map
 // light (ModelSlot::updateWorldLightsFromInstances, ModelSlot.cpp:90-125);
 // bnRender runs the fused path tracer into the frame buffer's accumulation
 // buffer (Context::render, barney/Context.cpp:142-185) and
@@ -180,6 +180,7 @@ struct Model : Object {
   bnamd_volume *volume = nullptr;
   bnamd_umesh *umesh = nullptr;
   bnamd_amr *amr = nullptr;
+  bnamd_nanovdb *nvdb = nullptr;
   bnamd_envmap *env = nullptr;
   std::vector<std::unique_ptr<DevMem>> mem;
   void release() {
@@ -187,11 +188,13 @@ struct Model : Object {
     if (volume) (void)bnamd_volume_destroy(volume);
     if (umesh) (void)bnamd_umesh_destroy(umesh);
     if (amr) (void)bnamd_amr_destroy(amr);
+    if (nvdb) (void)bnamd_nanovdb_destroy(nvdb);
     if (env) (void)bnamd_envmap_destroy(env);
     world = nullptr;
     volume = nullptr;
     umesh = nullptr;
     amr = nullptr;
+    nvdb = nullptr;
     env = nullptr;
     mem.clear();
   }
@@ -400,7 +403,7 @@ BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BND
 BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type) {
   slot0(whichSlot);
   const std::string t = type ? type : "";
-  if (t != "structured" && t != "unstructured" && t != "BlockStructuredAMR")
+  if (t != "structured" && t != "unstructured" && t != "BlockStructuredAMR" && t != "NanoVDB")
     throw std::runtime_error("bnScalarFieldCreate: unknown type " + t);
   auto *c = reinterpret_cast<Context *>(context);
   return static_cast<BNScalarField>(static_cast<_BNObject *>(c->add(new Object(c, t))));
@@ -448,7 +451,8 @@ BN_API void bnSetInstances(BNModel model, int whichSlot, BNGroup *groups, BNTran
 
 namespace {
 
-// the structured, unstructured or block-structured field of a volume model
+// the structured, unstructured, block-structured or NanoVDB field of a
+// volume model
 void build_volume(Model *m, const VolumeObj *vol) {
   const Object *f = vol->field;
   auto upload = [&](const void *p, size_t bytes) {
@@ -496,6 +500,17 @@ void build_volume(Model *m, const VolumeObj *vol) {
     check(bnamd_umesh_create(&d, nullptr, &m->umesh), "bnBuild");
     check(bnamd_umesh_set_tf(m->umesh, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
                              vol->densityAt1),
+          "bnBuild");
+  } else if (f->type == "NanoVDB") {
+    // NanoVDBData::setData("data") / commit: the grid buffer as BN_UINT8
+    // data (NanoVDB.cu:190-235; anari/SpatialField.cpp hands it so)
+    const Data *g = data_param(f, "data", BN_UINT8, true);
+    bnamd_nanovdb_desc d{};
+    d.data = g->bytes.data();
+    d.bytes = g->bytes.size();
+    check(bnamd_nanovdb_create(&d, nullptr, &m->nvdb), "bnBuild");
+    check(bnamd_nanovdb_set_tf(m->nvdb, vol->xf.data(), (int)vol->xf.size() / 4, vol->domain[0], vol->domain[1],
+                               vol->densityAt1),
           "bnBuild");
   } else {
     // BlockStructuredField::setData / commit (BlockStructuredField.cu:225-300)
@@ -685,7 +700,8 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   const Object *r = obj(renderer), *cam = obj(camera);
   auto *m = static_cast<Model *>(obj(model));
   auto *fb = static_cast<FrameBuffer *>(obj(fbh));
-  if (!m->world && !m->volume && !m->umesh && !m->amr) throw std::runtime_error("bnRender: model not built (bnBuild)");
+  if (!m->world && !m->volume && !m->umesh && !m->amr && !m->nvdb)
+    throw std::runtime_error("bnRender: model not built (bnBuild)");
   if (!fb->accum) throw std::runtime_error("bnRender: frame buffer not sized (bnFrameBufferResize)");
   // PerspectiveCamera parameters (Camera.cu:30-80)
   bnamd_camera c{};
@@ -728,6 +744,8 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
       check(bnamd_volume_render(m->volume, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
     else if (m->amr)
       check(bnamd_amr_render(m->amr, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
+    else if (m->nvdb)
+      check(bnamd_nanovdb_render(m->nvdb, &c, &p, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
     else {
       bnamd_umesh_render_opts o{};
       o.volume = 1;

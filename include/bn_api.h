@@ -3,8 +3,8 @@
  * arrays (including object lists), triangle / sphere / cylinder geometry
  * (vertex normals, per-primitive colors), matte and physically-based
  * materials (opacity), 2D textures, the environment-map light, 3D float
- * texture data, the "structured", "unstructured" and "BlockStructuredAMR"
- * scalar fields, volumes with a transfer function, groups with lights,
+ * texture data, the "structured", "unstructured", "BlockStructuredAMR" and
+ * "NanoVDB" scalar fields, volumes with a transfer function, groups with lights,
  * instances, perspective camera, renderer parameters (including the cut
  * plane), frame buffer with color / depth / ID channels, render, and every
  * bnSet* setter. Names, types, parameter names and enum values are
@@ -166,7 +166,8 @@ BN_API BNTexture2D bnTexture2DCreate(BNContext context, int whichSlot, BNDataTyp
  * "vertex.data", "index", "cell.index", "cell.type" (UMeshField.cu:204-236;
  * per-vertex scalars); "BlockStructuredAMR" takes "grid.origins",
  * "grid.dims", "grid.levels", "grid.offsets", "scalars",
- * "level.refinements" (BlockStructuredField.cu:225-256) */
+ * "level.refinements" (BlockStructuredField.cu:225-256); "NanoVDB" takes
+ * "data", the grid buffer as BN_UINT8 data (NanoVDB.cu:190-235) */
 BN_API BNTextureData bnTextureData3DCreate(BNContext context, int whichSlot, BNDataType texelFormat, int width,
                                            int height, int depth, const void *items);
 BN_API BNScalarField bnScalarFieldCreate(BNContext context, int whichSlot, const char *type);

@@ -3,10 +3,12 @@ program uses it: libbn_api.so exports every declared function; without a
 GPU bnContextCreate throws with the runtime's message; on the GPU a scene
 built and rendered through bn* equals, bit for bit, the same scene through
 the bnamd C ABI (color over two progressive frames, depth and ID channels,
-bnAccumReset), and unknown types throw."""
+bnAccumReset; the volume field kinds, NanoVDB among them), and unknown types
+throw."""
 import os
 import re
 import subprocess
+import tempfile
 
 import pytest
 
@@ -21,7 +23,14 @@ HDR = os.path.join(ROOT, "include", "bn_api.h")
 def _run():
     if not os.path.exists(EXE):
         pytest.skip("self-test not built (python -m barney_amd.build)")
-    return subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    # a NanoVDB grid around the origin for the "NanoVDB" scalar field case
+    from barney_amd import nanovdb, scenes
+    f = scenes.structured_field(24, blobs=4, sigma=(3.0, 8.0), seed=0x5EED0A0A)
+    grid = nanovdb.from_dense(f, voxel_size=0.125, origin=(-1.5, -1.5, -1.5))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "grid.nvdb")
+        grid.tofile(path)
+        return subprocess.run([EXE, path], capture_output=True, text=True, timeout=120)
 
 
 def test_exports_every_declared_function():
