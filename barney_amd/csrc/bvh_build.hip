@@ -1,6 +1,7 @@
 // GPU BVH8 builder: primitive gather -> 63-bit Morton codes -> radix sort ->
-// Karras binary radix tree -> bottom-up boxes -> top-down collapse into
-// 128-byte 8-wide nodes with 16-bit quantized child boxes.
+// binary tree by PLOC clustering of the sorted primitives (or the Karras
+// radix tree with bottom-up boxes) -> top-down collapse into 128-byte 8-wide
+// nodes with 16-bit quantized child boxes.
 //
 // Replaces the cuBQL builds of rtcore/cuda/cudaGroup.cu:261-413 (triangles),
 // :418-562 (user geoms) and :144-258 (instances); the primitive bounds follow
@@ -237,6 +238,139 @@ __global__ void k_refit(const float4 *__restrict boxes, const uint32_t *__restri
     }
     for (int k = 0; k < 3; ++k) { nodes[p].lo[k] = l[k]; nodes[p].hi[k] = h[k]; }
     p = parent[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PLOC: parallel locally-ordered clustering (Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 2018). The clusters start as the Morton-sorted primitives; each
+// round every cluster finds its nearest neighbour among the kPlocRadius
+// clusters on either side (the smallest surface area of the union, ties to
+// the smaller index, a strict order on pairs, so the closest pair in any
+// window is mutual and every round merges), mutual nearest neighbours merge
+// into a new binary node, and the survivors are compacted in order. The
+// binary tree it makes has a far lower SAH cost than the radix tree of the
+// Morton codes alone. A node's leaves are then renumbered in depth-first
+// order so that every subtree's primitives are contiguous (range).
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
+  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_ploc_init(const float4 *__restrict boxes, const uint32_t *__restrict sortedIdx, int n,
+                            int *__restrict cid, float4 *__restrict cbox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pi = sortedIdx[i];
+  cid[i] = ~i;
+  cbox[2 * i] = boxes[2 * pi];
+  cbox[2 * i + 1] = boxes[2 * pi + 1];
+}
+
+// nearest neighbour of each cluster within the window; the window's boxes
+// are staged in LDS (256 clusters and the radius on either side)
+__global__ void __launch_bounds__(256) k_ploc_nn(const float4 *__restrict cbox, int nc, int *__restrict nn) {
+  __shared__ float4 lo[256 + 2 * kPlocRadius], hi[256 + 2 * kPlocRadius];
+  const int base = blockIdx.x * 256 - kPlocRadius;
+  for (int t = threadIdx.x; t < 256 + 2 * kPlocRadius; t += 256) {
+    const int c = base + t;
+    if (c >= 0 && c < nc) {
+      lo[t] = cbox[2 * c];
+      hi[t] = cbox[2 * c + 1];
+    }
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  const int li = threadIdx.x + kPlocRadius;
+  const float4 a = lo[li], b = hi[li];
+  float best = FLT_MAX;
+  int bj = -1;
+  const int j0 = max(0, i - kPlocRadius), j1 = min(nc - 1, i + kPlocRadius);
+  for (int j = j0; j <= j1; ++j) {
+    if (j == i) continue;
+    const int lj = j - base;
+    const float4 ul = make_float4(fminf(a.x, lo[lj].x), fminf(a.y, lo[lj].y), fminf(a.z, lo[lj].z), 0.f);
+    const float4 uh = make_float4(fmaxf(b.x, hi[lj].x), fmaxf(b.y, hi[lj].y), fmaxf(b.z, hi[lj].z), 0.f);
+    const float d = half_area(ul, uh);
+    if (d < best) {  // j ascends, so a tie keeps the smaller index
+      best = d;
+      bj = j;
+    }
+  }
+  nn[i] = bj;
+}
+
+// Mutual nearest neighbours i < j merge into a new node held by cluster i;
+// cluster j is dropped (keep[j] = 0). count: leaves under each new node.
+__global__ void k_ploc_merge(int *__restrict cid, float4 *__restrict cbox, const int *__restrict nn, int nc,
+                             BinNode *__restrict nodes, int *__restrict count, uint32_t *__restrict nodeCounter,
+                             uint32_t *__restrict keep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const int j = nn[i];
+  if (j < 0 || nn[j] != i) {
+    keep[i] = 1u;
+    return;
+  }
+  if (j < i) {
+    keep[i] = 0u;
+    return;
+  }
+  const uint32_t node = atomicAdd(nodeCounter, 1u);
+  const int a = cid[i], b = cid[j];
+  const float4 al = cbox[2 * i], ah = cbox[2 * i + 1], bl = cbox[2 * j], bh = cbox[2 * j + 1];
+  BinNode nd;
+  nd.lo[0] = fminf(al.x, bl.x); nd.lo[1] = fminf(al.y, bl.y); nd.lo[2] = fminf(al.z, bl.z);
+  nd.hi[0] = fmaxf(ah.x, bh.x); nd.hi[1] = fmaxf(ah.y, bh.y); nd.hi[2] = fmaxf(ah.z, bh.z);
+  nd.left = a;
+  nd.right = b;
+  nodes[node] = nd;
+  count[node] = (a < 0 ? 1 : count[a]) + (b < 0 ? 1 : count[b]);
+  cid[i] = (int)node;
+  cbox[2 * i] = make_float4(nd.lo[0], nd.lo[1], nd.lo[2], 0.f);
+  cbox[2 * i + 1] = make_float4(nd.hi[0], nd.hi[1], nd.hi[2], 0.f);
+  keep[i] = 1u;
+}
+
+__global__ void k_ploc_compact(const int *__restrict cid, const float4 *__restrict cbox,
+                               const uint32_t *__restrict keep, const uint32_t *__restrict pos, int nc,
+                               int *__restrict cid2, float4 *__restrict cbox2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc || !keep[i]) return;
+  const uint32_t o = pos[i];
+  cid2[o] = cid[i];
+  cbox2[2 * o] = cbox[2 * i];
+  cbox2[2 * o + 1] = cbox[2 * i + 1];
+}
+
+// Depth-first leaf numbering, one round of nodes at a time from the root's
+// round down (a node's parent is made in a later round): a node's leaves
+// start at first[node], its left subtree's first; leaves get their new
+// sorted position (newPos), nodes their range {first, count}.
+__global__ void k_ploc_first(const BinNode *__restrict nodes, const int *__restrict count, uint32_t begin,
+                             uint32_t end, int *__restrict first, int *__restrict newPos, int2 *__restrict range) {
+  const uint32_t p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= end) return;
+  const int f = first[p];
+  const BinNode nd = nodes[p];
+  const int cl = nd.left < 0 ? 1 : count[nd.left];
+  if (nd.left < 0) newPos[~nd.left] = f; else first[nd.left] = f;
+  if (nd.right < 0) newPos[~nd.right] = f + cl; else first[nd.right] = f + cl;
+  range[p] = make_int2(f, count[p]);
+}
+
+// leaves renumbered: node child references and the sorted primitive order
+__global__ void k_ploc_renumber(BinNode *__restrict nodes, int numNodes, const int *__restrict newPos,
+                                const uint32_t *__restrict sortedIdx, int n, uint32_t *__restrict sorted2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sorted2[newPos[i]] = sortedIdx[i];
+  if (i < numNodes) {
+    if (nodes[i].left < 0) nodes[i].left = ~newPos[~nodes[i].left];
+    if (nodes[i].right < 0) nodes[i].right = ~newPos[~nodes[i].right];
   }
 }
 
@@ -556,7 +690,16 @@ void validate_bvh(const World &w) {
 
 void check_world_bvh(const World &w) { validate_bvh(w); }
 
-void build_world_bvh(World &w, hipStream_t stream) {
+// Binary tree builder under the 8-wide collapse (see world.h).
+enum { kBuildPloc = 0, kBuildRadix = 1 };
+int build_mode(bool mortonOrder) {
+  const char *m = getenv("BNAMD_BVH_BUILD");
+  if (m && std::string(m) == "radix") return kBuildRadix;
+  if (m && std::string(m) == "ploc") return kBuildPloc;
+  return mortonOrder ? kBuildRadix : kBuildPloc;
+}
+
+void build_world_bvh(World &w, hipStream_t stream, bool mortonOrder) {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t n = w.numPrims;
   const int numGeoms = (int)w.hgeoms.size();
@@ -600,18 +743,73 @@ void build_world_bvh(World &w, hipStream_t stream) {
 
   const int nInner = (int)n - 1;
   DevBuf<BinNode> bnodes(std::max(nInner, 1));
-  DevBuf<int> parent(2ull * n);
   DevBuf<int2> range(std::max(nInner, 1));
-  DevBuf<uint32_t> flags(std::max(nInner, 1));
-  BN_HIP(hipMemsetAsync(parent.ptr, 0xff, parent.bytes(), stream));
-  BN_HIP(hipMemsetAsync(flags.ptr, 0, flags.bytes(), stream));
-  if (nInner > 0) {
+  int rootBin = ~0;
+  uint32_t *sortedPrims = valsSorted.ptr;
+  DevBuf<uint32_t> sorted2;
+  if (nInner > 0 && build_mode(mortonOrder) == kBuildRadix) {
+    // the radix tree of the Morton codes with bottom-up boxes
+    DevBuf<int> parent(2ull * n);
+    DevBuf<uint32_t> flags(nInner);
+    BN_HIP(hipMemsetAsync(parent.ptr, 0xff, parent.bytes(), stream));
+    BN_HIP(hipMemsetAsync(flags.ptr, 0, flags.bytes(), stream));
     k_karras<<<div_up(nInner, 256), 256, 0, stream>>>(keysSorted.ptr, (int)n, bnodes.ptr, parent.ptr,
                                                      range.ptr);
     BN_HIP(hipGetLastError());
     k_refit<<<div_up(n, 256), 256, 0, stream>>>(boxes.ptr, valsSorted.ptr, (int)n, bnodes.ptr, parent.ptr,
                                                flags.ptr);
     BN_HIP(hipGetLastError());
+    rootBin = 0;
+  } else if (nInner > 0) {
+    // PLOC over the Morton-sorted primitives
+    DevBuf<int> cidA(n), cidB(n), nn(n), count(nInner), first(nInner), newPos(n);
+    DevBuf<float4> boxA(2ull * n), boxB(2ull * n);
+    DevBuf<uint32_t> keep(n), pos(n), nodeCounter(1);
+    BN_HIP(hipMemsetAsync(nodeCounter.ptr, 0, 4, stream));
+    k_ploc_init<<<div_up(n, 256), 256, 0, stream>>>(boxes.ptr, valsSorted.ptr, (int)n, cidA.ptr, boxA.ptr);
+    BN_HIP(hipGetLastError());
+    size_t scanBytes = 0;
+    BN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, keep.ptr, pos.ptr, (int)n, stream));
+    DevBuf<uint8_t> scanTemp(scanBytes);
+    std::vector<uint32_t> roundEnd;  // node counter after each round
+    int nc = (int)n;
+    int *cid = cidA.ptr, *cid2 = cidB.ptr;
+    float4 *cb = boxA.ptr, *cb2 = boxB.ptr;
+    while (nc > 1) {
+      BN_REQUIRE(roundEnd.size() < 4096, "PLOC made no progress");
+      k_ploc_nn<<<div_up(nc, 256), 256, 0, stream>>>(cb, nc, nn.ptr);
+      k_ploc_merge<<<div_up(nc, 256), 256, 0, stream>>>(cid, cb, nn.ptr, nc, bnodes.ptr, count.ptr,
+                                                        nodeCounter.ptr, keep.ptr);
+      BN_HIP(hipGetLastError());
+      BN_HIP(hipcub::DeviceScan::ExclusiveSum(scanTemp.ptr, scanBytes, keep.ptr, pos.ptr, nc, stream));
+      k_ploc_compact<<<div_up(nc, 256), 256, 0, stream>>>(cid, cb, keep.ptr, pos.ptr, nc, cid2, cb2);
+      BN_HIP(hipGetLastError());
+      uint32_t tail[3];
+      BN_HIP(hipMemcpyAsync(&tail[0], pos.ptr + nc - 1, 4, hipMemcpyDeviceToHost, stream));
+      BN_HIP(hipMemcpyAsync(&tail[1], keep.ptr + nc - 1, 4, hipMemcpyDeviceToHost, stream));
+      BN_HIP(hipMemcpyAsync(&tail[2], nodeCounter.ptr, 4, hipMemcpyDeviceToHost, stream));
+      BN_HIP(hipStreamSynchronize(stream));
+      nc = (int)(tail[0] + tail[1]);
+      roundEnd.push_back(tail[2]);
+      std::swap(cid, cid2);
+      std::swap(cb, cb2);
+    }
+    BN_REQUIRE(roundEnd.back() == (uint32_t)nInner, "PLOC node count");
+    // the root is the last node made; depth-first leaf numbering from it
+    rootBin = nInner - 1;
+    const int zero = 0;
+    BN_HIP(hipMemcpyAsync(first.ptr + rootBin, &zero, 4, hipMemcpyHostToDevice, stream));
+    for (size_t r = roundEnd.size(); r-- > 0;) {
+      const uint32_t begin = r ? roundEnd[r - 1] : 0u, end = roundEnd[r];
+      if (end > begin)
+        k_ploc_first<<<div_up(end - begin, 256), 256, 0, stream>>>(bnodes.ptr, count.ptr, begin, end, first.ptr,
+                                                                  newPos.ptr, range.ptr);
+    }
+    sorted2.alloc(n);
+    k_ploc_renumber<<<div_up(n, 256), 256, 0, stream>>>(bnodes.ptr, nInner, newPos.ptr, valsSorted.ptr, (int)n,
+                                                        sorted2.ptr);
+    BN_HIP(hipGetLastError());
+    sortedPrims = sorted2.ptr;
   }
 
   // top-down collapse, one launch per BVH8 level
@@ -622,7 +820,7 @@ void build_world_bvh(World &w, hipStream_t stream) {
   uint32_t hc[3] = {1u, 0u, 0u};
   BN_HIP(hipMemcpyAsync(counters.ptr, hc, sizeof(hc), hipMemcpyHostToDevice, stream));
   CollapseTask root;
-  root.bin = nInner > 0 ? 0 : ~0;
+  root.bin = rootBin;
   root.out = 0;
   BN_HIP(hipMemcpyAsync(taskA.ptr, &root, sizeof(root), hipMemcpyHostToDevice, stream));
   uint32_t numTasks = 1;
@@ -634,7 +832,7 @@ void build_world_bvh(World &w, hipStream_t stream) {
     const uint32_t zero = 0;
     BN_HIP(hipMemcpyAsync(counters.ptr + 2, &zero, 4, hipMemcpyHostToDevice, stream));
     k_collapse<<<div_up(numTasks, 64), 64, 0, stream>>>(cur, numTasks, bnodes.ptr, range.ptr, boxes.ptr,
-                                                       valsSorted.ptr, recs.ptr, w.nodes.ptr, w.prims.ptr,
+                                                       sortedPrims, recs.ptr, w.nodes.ptr, w.prims.ptr,
                                                        counters.ptr, nxt);
     BN_HIP(hipGetLastError());
     BN_HIP(hipMemcpyAsync(hc, counters.ptr, sizeof(hc), hipMemcpyDeviceToHost, stream));

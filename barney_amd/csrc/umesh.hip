@@ -520,7 +520,7 @@ extern "C" int bnamd_umesh_create(const bnamd_umesh_desc *desc, void *stream, bn
       m.w.numPrims = (uint32_t)numElems;
       MaterialDev md{};
       m.w.materials.upload(&md, 1, s);
-      build_world_bvh(m.w, s);
+      build_world_bvh(m.w, s, /*mortonOrder=*/true);
       if (cells) {
         // polyhedra read the index stream and the vertices at sample time
         m.cellIndices.alloc(desc->numIndices);

@@ -142,7 +142,12 @@ struct World {
 };
 
 // Builds the BVH8 over the given prim table (implemented in bvh_build.hip).
-void build_world_bvh(World &w, hipStream_t stream);
+// BVH8 of w's primitives. The binary tree under the 8-wide collapse is
+// PLOC's (surfaces: fewer node visits per ray) or, with mortonOrder, the
+// radix tree of the Morton codes, whose leaf order follows the space-filling
+// curve (tet meshes: point location tries the leaves next to the last hit
+// first); BNAMD_BVH_BUILD=ploc|radix overrides the choice.
+void build_world_bvh(World &w, hipStream_t stream, bool mortonOrder = false);
 // Host-side structural check of a built BVH8 (throws on a violation).
 void check_world_bvh(const World &w);
 
