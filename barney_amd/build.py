@@ -24,6 +24,8 @@ BN_SRC = os.path.join(HERE, "bn", "bn_api.cpp")
 BN_LIB = os.path.join(HERE, "libbn_api.so")
 BN_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "bn_api_selftest.cpp")
 BN_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "bn_api_selftest")
+PROG_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_programs.hip")
+PROG_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "rtcore_amd_programs")
 
 SOURCES = [
     "runtime.hip",
@@ -103,8 +105,14 @@ def build_host_layer(verbose=False):
     _run([CXX, *common, inc, BN_TEST_SRC, "-o", tmp, "-L" + HERE, "-lbn_api", "-lbnamd",
           "-Wl,-rpath,$ORIGIN/../../barney_amd"], "g++ (bn_api selftest)")
     os.replace(tmp, BN_TEST)
+    # programs and kernels against rtc::amd's device interface (hipcc: the
+    # traversal and the programs share the program's code object)
+    tmp = PROG_TEST + ".tmp"
+    _run([HIPCC, *FLAGS, PROG_TEST_SRC, "-o", tmp, "-L" + HERE, "-lrtcore_amd", "-lbnamd",
+          "-Wl,-rpath,$ORIGIN/../../barney_amd"], "hipcc (rtcore_amd programs)")
+    os.replace(tmp, PROG_TEST)
     if verbose:
-        print(f"built {RTC_LIB}, {SELFTEST}, {BN_LIB} and {BN_TEST}")
+        print(f"built {RTC_LIB}, {SELFTEST}, {BN_LIB}, {BN_TEST} and {PROG_TEST}")
 
 
 if __name__ == "__main__":

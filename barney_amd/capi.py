@@ -15,7 +15,7 @@ class bnamd_geom(C.Structure):
     _fields_ = [("kind", C.c_int), ("numPrims", C.c_int), ("numVertices", C.c_int),
                 ("materialID", C.c_int), ("vertices", C.c_void_p), ("indices", C.c_void_p),
                 ("radii", C.c_void_p), ("colors", C.c_void_p), ("defaultRadius", C.c_float),
-                ("userID", C.c_int), ("normals", C.c_void_p)]
+                ("userID", C.c_int), ("normals", C.c_void_p), ("programs", C.c_void_p)]
 
 
 class bnamd_group(C.Structure):
@@ -215,6 +215,7 @@ SIGNATURES = {
     "bnamd_fb_resolve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p, C.c_void_p]),
     "bnamd_fb_to_fixed8": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "bnamd_world_validate": (C.c_int, [C.c_void_p]),
+    "bnamd_world_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "bnamd_texture2d_create": (C.c_int, [C.POINTER(bnamd_texture2d_desc), C.c_void_p, C.POINTER(C.c_void_p)]),
     "bnamd_texture2d_destroy": (C.c_int, [C.c_void_p]),
     "bnamd_texture2d_sample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

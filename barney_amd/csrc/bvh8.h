@@ -10,7 +10,10 @@
 
 namespace bnamd {
 
-enum PrimType : uint32_t { PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2, PRIM_TET = 3, PRIM_INSTANCE = 4 };
+enum PrimType : uint32_t {
+  PRIM_TRIANGLE = 0, PRIM_SPHERE = 1, PRIM_CYLINDER = 2, PRIM_TET = 3, PRIM_INSTANCE = 4,
+  PRIM_USER = 5  // a user geometry's box; its intersect program finds the hit
+};
 // c.w of a PrimRec: type in bits 0-7, flags above.
 // PRIM_SHADOW_TRANSPARENT: a triangle whose material is Glass; barney's
 // triangle anyHit ignores it for shadow rays (Glass::getOpacity returns 0

@@ -57,14 +57,15 @@ __global__ void k_gather(const GeomDev *__restrict geoms, int numGeoms, const Ma
     r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(PRIM_SPHERE));
     lo = c - vec3f(rad);
     hi = c + vec3f(rad);
-  } else if (gd.kind == GEOM_INSTANCES || gd.kind == GEOM_ELEM_BOXES) {
+  } else if (gd.kind == GEOM_INSTANCES || gd.kind == GEOM_ELEM_BOXES || gd.kind == BNAMD_GEOM_USER) {
     const float4 *bx = (const float4 *)gd.vertices;
     const float4 l = bx[2 * primID], h = bx[2 * primID + 1];
     lo = {l.x, l.y, l.z};
     hi = {h.x, h.y, h.z};
     r.a = make_float4(lo.x, lo.y, lo.z, __uint_as_float(i));
     r.b = make_float4(hi.x, hi.y, hi.z, 0.f);
-    r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(gd.kind == GEOM_INSTANCES ? PRIM_INSTANCE : PRIM_TET));
+    const uint32_t type = gd.kind == GEOM_INSTANCES ? PRIM_INSTANCE : (gd.kind == BNAMD_GEOM_USER ? PRIM_USER : PRIM_TET);
+    r.c = make_float4(0.f, 0.f, 0.f, __uint_as_float(type));
   } else if (gd.kind == GEOM_TETS) {
     const int4 ix = ((const int4 *)gd.indices)[primID];
     const float4 *v4 = (const float4 *)gd.vertices;

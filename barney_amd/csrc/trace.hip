@@ -73,11 +73,13 @@ namespace bnamd {
 namespace {
 
 void check_geom(const bnamd_geom &src, int numMaterials) {
-  BN_REQUIRE(src.kind >= 0 && src.kind <= 2, "bnamd_world_build: unknown geom kind");
+  BN_REQUIRE((src.kind >= 0 && src.kind <= 2) || src.kind == BNAMD_GEOM_USER, "bnamd_world_build: unknown geom kind");
   BN_REQUIRE(src.numPrims >= 0, "bnamd_world_build: negative prim count");
   BN_REQUIRE(src.numPrims == 0 || src.vertices, "bnamd_world_build: geom without vertices");
-  BN_REQUIRE(src.kind == BNAMD_GEOM_SPHERES || src.numPrims == 0 || src.indices,
+  BN_REQUIRE(src.kind == BNAMD_GEOM_SPHERES || src.kind == BNAMD_GEOM_USER || src.numPrims == 0 || src.indices,
              "bnamd_world_build: geom without indices");
+  BN_REQUIRE(src.kind != BNAMD_GEOM_USER || src.programs,
+             "bnamd_world_build: a user geometry needs its program record (intersect program)");
   BN_REQUIRE(src.kind != BNAMD_GEOM_CYLINDERS || src.numPrims == 0 || src.radii,
              "bnamd_world_build: cylinders need per-primitive radii");
   BN_REQUIRE(src.materialID >= 0 && src.materialID < std::max(numMaterials, 1),
@@ -100,7 +102,14 @@ GeomDev make_geom(const bnamd_geom &src, int geomID, int instIndex, int instID, 
   g.colors = src.colors;
   g.normals = src.kind == BNAMD_GEOM_TRIANGLES ? src.normals : nullptr;
   g.defaultRadius = src.defaultRadius;
+  g.programs = src.programs;
   return g;
+}
+
+bool has_user(const std::vector<GeomDev> &geoms) {
+  for (const GeomDev &g : geoms)
+    if (g.kind == BNAMD_GEOM_USER) return true;
+  return false;
 }
 
 std::vector<MaterialDev> make_materials(const bnamd_material *materials, int numMaterials) {
@@ -215,6 +224,7 @@ std::unique_ptr<World> build_blas(const bnamd_group &grp, const std::vector<Mate
   build_world_bvh(*b, s);
   b->hmats = mats;
   b->hasCutouts = has_cutouts(b->hgeoms, mats);
+  b->hasUser = has_user(b->hgeoms);
   return b;
 }
 
@@ -247,8 +257,10 @@ void assemble_tlas(World &w, const float *xfms, hipStream_t s) {
   uint64_t total = 0, nodes = 0;
   std::vector<const World *> distinct;
   w.hasCutouts = false;
+  w.hasUser = false;
   for (const World *b : w.instBlas) {
     w.hasCutouts = w.hasCutouts || b->hasCutouts;
+    w.hasUser = w.hasUser || b->hasUser;
     maxPrims = std::max(maxPrims, b->numPrims);
     maxDepth = std::max(maxDepth, b->depth);
     total += b->numPrims;
@@ -364,6 +376,7 @@ extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const
         w.materials.upload(mats.data(), mats.size(), s);
         build_world_bvh(w, s);
         w.hasCutouts = has_cutouts(w.hgeoms, mats);
+        w.hasUser = has_user(w.hgeoms);
         w.totalPrims = w.numPrims;
         w.totalNodes = w.numNodes;
       }
@@ -373,6 +386,17 @@ extern "C" int bnamd_world_build(const bnamd_group *groups, int numGroups, const
       throw;
     }
     *out = wo;
+  });
+}
+
+extern "C" int bnamd_world_device(const bnamd_world *world, const void **deviceRecord) {
+  return guarded([&] {
+    BN_REQUIRE(world && deviceRecord, "bnamd_world_device: NULL argument");
+    const World &w = world->w;
+    const WorldDev d = w.dd();
+    w.devRecord.upload(&d, 1);
+    BN_HIP(hipDeviceSynchronize());
+    *deviceRecord = w.devRecord.ptr;
   });
 }
 
@@ -487,6 +511,8 @@ extern "C" int bnamd_trace(const bnamd_world *world, const bnamd_ray *rays, bnam
     BN_REQUIRE(numRays >= 0, "bnamd_trace: negative ray count");
     if (numRays == 0) return;
     BN_REQUIRE(rays && hits, "bnamd_trace: NULL ray or hit buffer");
+    BN_REQUIRE(!world->w.hasUser, "bnamd_trace: the world holds user geometry (BNAMD_GEOM_USER): only the "
+                                  "programmable traversal (rtc::amd::TraceInterface::traceRay) can trace it");
     const int block = 256;
     const size_t lds = trace_lds_bytes(world->w.depth, block);
     BN_REQUIRE(lds <= 64 * 1024, "bnamd_trace: BVH too deep for the LDS stack");
@@ -502,5 +528,12 @@ extern "C" int bnamd_trace(const bnamd_world *world, const bnamd_ray *rays, bnam
 
 // Used by other translation units (world accessors).
 namespace bnamd {
-const World &world_of(const bnamd_world *w) { return w->w; }
+// The world of a fused renderer: its programs are the native ones, so a
+// world with user geometry (whose hits only its intersect program finds)
+// is traced by rtc::amd::TraceInterface instead.
+const World &world_of(const bnamd_world *w) {
+  BN_REQUIRE(!w->w.hasUser, "the world holds user geometry (BNAMD_GEOM_USER): only the programmable traversal "
+                            "(rtc::amd::TraceInterface::traceRay) can trace it");
+  return w->w;
+}
 }

@@ -34,6 +34,7 @@ struct GeomDev {
   const float *normals;  // triangles: per-vertex normals or nullptr
   float defaultRadius;
   int pad;
+  const void *programs;  // program record (programmable traversal) or nullptr
 };
 
 struct MaterialDev {
@@ -118,6 +119,8 @@ struct World {
   std::vector<int> instIDs;
   int numBlas = 0;                           // distinct bottom levels referenced
   bool hasCutouts = false;                   // some triangle's material has alpha < 1
+  bool hasUser = false;                      // user geometry (programmable traversal only)
+  mutable DevBuf<WorldDev> devRecord;        // dd() in device memory (bnamd_world_device)
   mutable DevBuf<unsigned long long> counters;  // the surface render's counters
   mutable BatchTune tune;                        // batched renderers' launch configuration
   DevBuf<InstDev> insts;

@@ -110,12 +110,31 @@ EnvMapLight::~EnvMapLight() {
 // -------------------------------------------------------- GeomType / Geom
 GeomType::GeomType(Device *device, GeomKind kind, const char *name) : device(device), kind(kind), name(name) {}
 
+GeomType::GeomType(Device *device, GeomKind kind, const char *name, size_t sizeOfDD, const ProgramRecord &programs,
+                   BoundsFct bounds)
+    : device(device), kind(kind), name(name), programmable(true), sizeOfDD(sizeOfDD), programs(programs),
+      bounds(bounds) {
+  if (kind != GeomKind::Triangles && kind != GeomKind::User)
+    throw std::runtime_error("rtc::amd::GeomType: programmable types are triangles or user geometry");
+  if (kind == GeomKind::User && (!bounds || !programs.intersect))
+    throw std::runtime_error("rtc::amd::GeomType: user geometry needs bounds and intersect programs");
+}
+
 Geom *GeomType::createGeom() { return new Geom(this); }
 
 Geom::Geom(GeomType *gt) : gt(gt) {}
 
+Geom::~Geom() {
+  if (d_record) (void)bnamd_device_free(d_record);
+  if (d_boxes) (void)bnamd_device_free(d_boxes);
+}
+
 void Geom::setDD(const void *dd) {
   if (!dd) throw std::runtime_error("rtc::amd::Geom::setDD: NULL");
+  if (gt->programmable) {
+    ddBytes.assign((const uint8_t *)dd, (const uint8_t *)dd + gt->sizeOfDD);
+    if (gt->kind == GeomKind::User) return;  // its programs read the DD; the boxes come from bounds
+  }
   barney_dd::Geometry base;
   std::memcpy(&base, dd, barney_dd::kGeometryDDSize);
   const uint8_t *fields = (const uint8_t *)dd + barney_dd::kGeometryDDSize;
@@ -153,8 +172,31 @@ void Geom::setDD(const void *dd) {
       ddRadii = f.radii;
       break;
     }
+    case GeomKind::User:
+      break;
   }
   haveDD = true;
+}
+
+void Geom::prepare() {
+  if (!gt->programmable) return;
+  const size_t bytes = sizeof(ProgramRecord) + gt->sizeOfDD;
+  std::vector<uint8_t> rec(bytes, 0);
+  std::memcpy(rec.data(), &gt->programs, sizeof(ProgramRecord));
+  if (!ddBytes.empty()) std::memcpy(rec.data() + sizeof(ProgramRecord), ddBytes.data(), ddBytes.size());
+  if (d_record) check(bnamd_device_free(d_record), "rtc::amd::Geom::prepare");
+  d_record = nullptr;
+  check(bnamd_device_malloc(&d_record, bytes), "rtc::amd::Geom::prepare");
+  check(bnamd_memcpy_h2d(d_record, rec.data(), bytes), "rtc::amd::Geom::prepare");
+  if (gt->kind == GeomKind::User) {
+    if (d_boxes) check(bnamd_device_free(d_boxes), "rtc::amd::Geom::prepare");
+    d_boxes = nullptr;
+    if (primCount > 0) {
+      check(bnamd_device_malloc(&d_boxes, 32 * (size_t)primCount), "rtc::amd::Geom::prepare");
+      gt->bounds((const uint8_t *)d_record + sizeof(ProgramRecord), d_boxes, primCount);
+      check(bnamd_synchronize(), "rtc::amd::Geom::prepare (bounds program)");
+    }
+  }
 }
 
 void Geom::setVertices(Buffer *v, int n) {
@@ -194,6 +236,12 @@ bnamd_geom Geom::desc() const {
   g.numPrims = primCount;
   g.numVertices = numVertices;
   g.materialID = materialID;
+  g.programs = d_record;
+  if (gt->kind == GeomKind::User) {
+    g.vertices = (const float *)d_boxes;
+    g.userID = userID;
+    return g;
+  }
   if (haveDD) {
     g.vertices = (const float *)ddVertices;
     g.indices = (const int *)ddIndices;
@@ -217,7 +265,10 @@ GeomGroup::~GeomGroup() {
 
 void GeomGroup::buildAccel() {
   std::vector<bnamd_geom> gs;
-  for (Geom *g : geoms) gs.push_back(g->desc());
+  for (Geom *g : geoms) {
+    g->prepare();
+    gs.push_back(g->desc());
+  }
   bnamd_group grp{gs.data(), (int)gs.size()};
   bnamd_blas *b = nullptr;
   const std::vector<bnamd_material> &mats = device->materials;
@@ -368,7 +419,30 @@ void Device::freeEnvMapLight(EnvMapLight *l) { delete l; }
 AccelHandle getAccelHandle(Group *ig) {
   auto *g = dynamic_cast<InstanceGroup *>(ig);
   if (!g || !g->world) throw std::runtime_error("rtc::amd::getAccelHandle: a built InstanceGroup expected");
-  return (AccelHandle)g->world;
+  const void *rec = nullptr;
+  check(bnamd_world_device(g->world, &rec), "rtc::amd::getAccelHandle");
+  return (AccelHandle)rec;
+}
+
+// ------------------------------------------------------------ TraceKernel2D
+TraceKernel2D::TraceKernel2D(Device *device, size_t sizeOfLP, TraceLaunchFct traceLaunchFct)
+    : device(device), traceLaunchFct(traceLaunchFct), sizeOfLP(sizeOfLP) {
+  if (sizeOfLP) check(bnamd_device_malloc(&d_lpData, sizeOfLP), "rtc::amd::TraceKernel2D");
+}
+
+TraceKernel2D::~TraceKernel2D() {
+  if (d_lpData) (void)bnamd_device_free(d_lpData);
+}
+
+void TraceKernel2D::launch(vec2i launchDims, const void *kernelData) {
+  // the launch parameters (host or device memory) are copied before the
+  // call returns, so the caller may reuse its buffer at once; the kernel
+  // follows on the same (default) stream
+  if (sizeOfLP) {
+    check(bnamd_memcpy_async(d_lpData, kernelData, sizeOfLP, nullptr), "rtc::amd::TraceKernel2D::launch");
+    check(bnamd_synchronize(), "rtc::amd::TraceKernel2D::launch");
+  }
+  if (launchDims.x > 0 && launchDims.y > 0) traceLaunchFct(device, launchDims, d_lpData);
 }
 
 Buffer *Device::createBuffer(size_t numBytes, const void *initValues) { return new Buffer(this, numBytes, initValues); }

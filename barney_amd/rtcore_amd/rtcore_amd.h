@@ -8,12 +8,18 @@
 // C ABI of libbnamd (include/bnamd.h); no HIP or torch type appears here, so
 // this layer builds with any C++17 host compiler.
 //
-// Geometry programs are native in libbnamd: the geometry types barney
-// registers with RTC_EXPORT_USER_GEOM / RTC_EXPORT_TRIANGLES_GEOM
-// (Triangles, Spheres, Cylinders) come from createGeomType_<Name>() below.
-// Their device data arrives as barney writes it: Geom::setDD(&dd) with the
-// bytes of barney's Triangles::DD / Spheres::DD / Cylinders::DD, whose
-// layout barney_dd below restates (or through the typed setters).
+// Two ways to use it. Native: the geometry types barney imports
+// (Triangles, Spheres, Cylinders) come from createGeomType_<Name>() below,
+// with programs native in libbnamd, and barney's frame loop calls the fused
+// renderers; their device data arrives as barney writes it: Geom::setDD(&dd)
+// with the bytes of barney's Triangles::DD / Spheres::DD / Cylinders::DD,
+// whose layout barney_dd below restates (or through the typed setters).
+// Programmable: barney's own device programs and kernels, compiled with
+// hipcc against rtcore_amd_device.h, register through its RTC_EXPORT_*
+// macros (geometry types with anyHit / closestHit / intersect / bounds
+// programs, trace and compute kernels); TraceKernel2D and ComputeKernel*D
+// below launch the kernels, and TraceInterface::traceRay walks the BVH8s
+// built here and calls the programs.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -32,6 +38,18 @@ struct vec3f {
 };
 struct vec3i {
   int x, y, z;
+};
+struct vec2i {
+  int x, y;
+};
+struct vec2ui {
+  unsigned x, y;
+};
+struct vec3ui {
+  unsigned x, y, z;
+};
+struct box3f {
+  vec3f lower, upper;
 };
 // affine3f as rtcore uses it: linear part as three columns (vx, vy, vz),
 // then the translation p (object to world)
@@ -183,20 +201,44 @@ struct EnvMapLight {
 };
 
 enum class GeomKind { Triangles = BNAMD_GEOM_TRIANGLES, Spheres = BNAMD_GEOM_SPHERES,
-                      Cylinders = BNAMD_GEOM_CYLINDERS };
+                      Cylinders = BNAMD_GEOM_CYLINDERS, User = BNAMD_GEOM_USER };
+
+// A geometry's program record in device memory, as TraceInterface::traceRay
+// reads it (the SBT record of rtcore/cuda/Geom.h:16-69): the device addresses
+// of its anyHit, closestHit and intersect programs (0: none), then the
+// geometry's DD bytes, which getProgramData() returns.
+struct ProgramRecord {
+  uint64_t anyHit, closestHit, intersect, reserved;
+};
+static_assert(sizeof(ProgramRecord) == 32, "program record header is 32 bytes");
+
+// Launches a user geometry's bounds program over its primitives: the boxes
+// of numPrims primitives of the geometry whose DD is at deviceDD, written to
+// deviceBoxes as float4 pairs (lo, hi); RTC_EXPORT_USER_GEOM defines it.
+typedef void (*BoundsFct)(const void *deviceDD, void *deviceBoxes, int numPrims);
 
 struct GeomType {
+  // a native geometry type (programs in libbnamd)
   GeomType(Device *device, GeomKind kind, const char *name);
+  // a programmable one (RTC_EXPORT_TRIANGLES_GEOM / RTC_EXPORT_USER_GEOM):
+  // the size of its DD, its programs' addresses and, for user geometry, the
+  // launcher of its bounds program
+  GeomType(Device *device, GeomKind kind, const char *name, size_t sizeOfDD, const ProgramRecord &programs,
+           BoundsFct bounds);
   virtual ~GeomType() = default;
   Geom *createGeom();
   Device *const device;
   const GeomKind kind;
   const std::string name;
+  const bool programmable = false;
+  const size_t sizeOfDD = 0;
+  const ProgramRecord programs{};
+  const BoundsFct bounds = nullptr;
 };
 
 struct Geom {
   explicit Geom(GeomType *gt);
-  virtual ~Geom() = default;
+  virtual ~Geom();
   // the bytes of barney's Triangles::DD / Spheres::DD / Cylinders::DD
   // (rtcore/cuda/Geom.h:16-69: Geom::setDD copies sizeof(DD) bytes): the
   // geometry's buffers, userID, materialID, and per-primitive float3 colors
@@ -214,6 +256,9 @@ struct Geom {
   void setRadii(Buffer *radii, float defaultRadius);
   void setColors(Buffer *colors);
   void setMaterial(int materialID, int userID);
+  // programmable geometry: writes its program record (programs + DD) to
+  // device memory and, for user geometry, runs its bounds program
+  void prepare();
   bnamd_geom desc() const;
   GeomType *const gt;
   Buffer *vertices = nullptr, *indices = nullptr, *radii = nullptr, *colors = nullptr;
@@ -224,6 +269,11 @@ struct Geom {
   const void *ddVertices = nullptr, *ddIndices = nullptr, *ddRadii = nullptr, *ddColors = nullptr;
   const void *ddNormals = nullptr;
   bool haveDD = false;
+  // programmable geometry: the DD bytes given to setDD, the device program
+  // record and the primitive boxes (user geometry)
+  std::vector<uint8_t> ddBytes;
+  void *d_record = nullptr;
+  void *d_boxes = nullptr;
 };
 
 struct Group {
@@ -275,6 +325,36 @@ struct InstanceGroup : public Group {
   bnamd_world *world = nullptr;
 };
 
+// Trace kernels (rtcore/cuda/TraceKernel.h:16-28): launch(dims, lp) copies
+// the sizeOfLP bytes of the launch parameters to device memory and runs the
+// kernel RTC_EXPORT_TRACE2D defined (16x16 threads per block over dims).
+typedef void (*TraceLaunchFct)(Device *device, vec2i dims, const void *d_lpData);
+struct TraceKernel2D {
+  TraceKernel2D(Device *device, size_t sizeOfLP, TraceLaunchFct traceLaunchFct);
+  ~TraceKernel2D();
+  void launch(vec2i launchDims, const void *kernelData);
+  Device *const device;
+  const TraceLaunchFct traceLaunchFct;
+  const size_t sizeOfLP;
+  void *d_lpData = nullptr;
+};
+
+// Compute kernels (rtcore/cudaCommon/ComputeKernel.h:16-32): the kernel
+// RTC_EXPORT_COMPUTE*D defined runs dd.run(ComputeInterface) per thread
+// over nb blocks of bs threads, the kernel data passed by value.
+struct ComputeKernel1D {
+  virtual ~ComputeKernel1D() = default;
+  virtual void launch(unsigned int nb, unsigned int bs, const void *pKernelData) = 0;
+};
+struct ComputeKernel2D {
+  virtual ~ComputeKernel2D() = default;
+  virtual void launch(vec2ui nb, vec2ui bs, const void *pKernelData) = 0;
+};
+struct ComputeKernel3D {
+  virtual ~ComputeKernel3D() = default;
+  virtual void launch(vec3ui nb, vec3ui bs, const void *pKernelData) = 0;
+};
+
 struct Denoiser;
 
 struct Device {
@@ -313,7 +393,7 @@ struct Device {
   Group *createInstanceGroup(const std::vector<Group *> &groups, const std::vector<int> &instIDs,
                              const std::vector<affine3f> &xfms);
   void freeGroup(Group *);
-  void buildPipeline() {}  // programs are native: nothing to link
+  void buildPipeline() {}  // nothing to link: program addresses are read when a geometry type is made
   void buildSBT() {}
   Denoiser *createDenoiser() { return nullptr; }
 
@@ -325,7 +405,8 @@ struct Device {
 };
 
 // The acceleration handle barney's launch parameters carry
-// (rtcore/cuda/Device.h:88): the instance group's world.
+// (rtcore/cuda/Device.h:88): the device address of the instance group's
+// world record, which TraceInterface::traceRay reads.
 AccelHandle getAccelHandle(Group *instanceGroup);
 
 // the geometry types barney imports (RTC_IMPORT_*_GEOM)

@@ -65,6 +65,12 @@ int bnamd_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 #define BNAMD_GEOM_TRIANGLES 0
 #define BNAMD_GEOM_SPHERES 1
 #define BNAMD_GEOM_CYLINDERS 2
+/* user geometry of the programmable path (RTC_EXPORT_USER_GEOM): its
+ * primitives are boxes ('vertices' = device float4 pairs lo, hi per
+ * primitive, as the bounds program writes them) whose hits its intersect
+ * program finds; only rtc::amd::TraceInterface::traceRay traverses worlds
+ * holding them (the fused renderers and bnamd_trace reject such worlds) */
+#define BNAMD_GEOM_USER 6
 
 typedef struct {
   int kind;              /* BNAMD_GEOM_* */
@@ -79,6 +85,9 @@ typedef struct {
   int userID;            /* 'objID' written to the ID channel */
   const float *normals;  /* triangles: device float3[numVertices] vertex normals, interpolated at
                           * hits as the shading normal (Triangles.dev.cu:70-74), or NULL */
+  const void *programs;  /* device program record of the geometry's anyHit / closestHit / intersect
+                          * programs and its DD (rtc::amd::ProgramRecord) for the programmable
+                          * traversal, or NULL (native programs only) */
 } bnamd_geom;
 
 typedef struct {
@@ -130,6 +139,12 @@ int bnamd_world_destroy(bnamd_world *world);
  * this check unless BNAMD_VALIDATE_BVH=1. No reference counterpart (a
  * test hook). */
 int bnamd_world_validate(const bnamd_world *world);
+/* The device address of the world's traversal record (the AccelHandle that
+ * rtc::amd::getAccelHandle returns and TraceInterface::traceRay reads on the
+ * device; rtcore/cuda/Device.h:88 getAccelHandle returns the instance
+ * group's device record the same way). Valid until the world is rebuilt,
+ * refit or destroyed. */
+int bnamd_world_device(const bnamd_world *world, const void **deviceRecord);
 int bnamd_world_info_get(const bnamd_world *world, bnamd_world_info *info);
 
 /* Bottom levels built on their own: rtc::GeomGroup::buildAccel of a

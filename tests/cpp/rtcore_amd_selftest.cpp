@@ -215,7 +215,11 @@ int main() {
     ig2->traceRays((const bnamd_ray *)dRays, (bnamd_hit *)dHits, (int)rays.size());
     dev->copy(h2.data(), dHits, h2.size() * sizeof(bnamd_hit));
     CHECK(std::memcmp(h1.data(), h2.data(), h1.size() * sizeof(bnamd_hit)) == 0);
-    CHECK((void *)getAccelHandle(ig2) == (void *)ig2->world);
+    {  // the handle is the world's device record
+      const void *rec = nullptr;
+      CHECK(bnamd_world_device(ig2->world, &rec) == BNAMD_OK);
+      CHECK((const void *)getAccelHandle(ig2) == rec && rec != nullptr);
+    }
     dev->freeGroup(ig2);
     dev->freeGroup(tg2);
     dev->freeGroup(ug2);

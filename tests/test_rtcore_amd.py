@@ -1,7 +1,8 @@
 """The rtc::amd C++ host layer (barney's rtcore backend classes over the C
 ABI), exercised by a g++-built program: without a GPU it must fail loudly
 with the runtime's message; on the GPU it checks memory, groups, geometry
-types, tracing against analytic hits, and error reporting."""
+types, tracing against analytic hits, and error reporting. A hipcc-built
+program checks the device side (programs, trace and compute kernels)."""
 import os
 import subprocess
 
@@ -41,6 +42,27 @@ def test_without_gpu_fails_loudly():
 @pytest.mark.gpu
 def test_selftest_on_gpu(gpu):
     r = _run()
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
+
+
+PROGRAMS = os.path.join(ROOT, "tests", "cpp", "rtcore_amd_programs")
+
+
+@pytest.mark.gpu
+def test_programmable_traversal_on_gpu(gpu):
+    """Device programs and kernels written against rtcore_amd_device.h
+    (RTC_EXPORT_TRIANGLES_GEOM / USER_GEOM / TRACE2D / COMPUTE1D), launched
+    through TraceKernel2D / ComputeKernel1D: every ray's closest hit through
+    TraceInterface::traceRay (anyHit ignoring a fifth of the triangles, user
+    spheres by their intersect program, a two-level world) equals the native
+    trace of the kept triangles and a float64 brute force over the spheres,
+    t bit for bit, with the IDs and instance transforms the closestHit
+    program reads."""
+    if not os.path.exists(PROGRAMS):
+        pytest.skip("programs test not built (python -m barney_amd.build)")
+    r = subprocess.run([PROGRAMS], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
