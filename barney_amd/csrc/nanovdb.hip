@@ -56,30 +56,39 @@ void check_grid(const uint8_t *b, size_t bytes, int &type, LeafFn &&onLeaf) {
   const uint8_t *root = b + 672 + 64;
   const uint32_t nt = host_load<uint32_t>(root + 24);
   BN_REQUIRE(nt > 0, "bnamd_nanovdb_create: empty grid");
-  auto inside = [&](const uint8_t *p, size_t n) {
-    return p >= b && (size_t)(p - b) <= bytes && n <= bytes - (size_t)(p - b);
+  // a node of n bytes at byte offset 'base + off' lies inside the buffer
+  // (checked on the offsets, before any pointer is formed from them)
+  auto inside = [&](size_t base, int64_t off, size_t n) {
+    if (off < 0 || (uint64_t)off > bytes) return false;
+    const size_t at = base + (size_t)off;
+    return at <= bytes && n <= bytes - at;
   };
-  BN_REQUIRE(inside(root, 64 + 32 * (size_t)nt), "bnamd_nanovdb_create: root tiles outside the buffer");
+  const size_t rootAt = 672 + 64;
+  BN_REQUIRE(nt <= (bytes - rootAt) / 32 && inside(rootAt, 0, 64 + 32 * (size_t)nt),
+             "bnamd_nanovdb_create: root tiles outside the buffer");
   for (uint32_t i = 0; i < nt; ++i) {
     const int64_t ch = host_load<int64_t>(root + 64 + 32 * (size_t)i + 8);
     if (!ch) continue;
+    BN_REQUIRE(ch > 0 && inside(rootAt, ch, kUpperBytes), "bnamd_nanovdb_create: upper node outside the buffer");
     // the upper node's origin from its tile key (RootData::KeyToCoord)
     const uint64_t key = host_load<uint64_t>(root + 64 + 32 * (size_t)i);
     const int ux = (int)(uint32_t)(((key >> 42) & 0x1FFFFFu) << 12);
     const int uy = (int)(uint32_t)(((key >> 21) & 0x1FFFFFu) << 12);
     const int uz = (int)(uint32_t)((key & 0x1FFFFFu) << 12);
-    const uint8_t *up = root + ch;
-    BN_REQUIRE(ch > 0 && inside(up, kUpperBytes), "bnamd_nanovdb_create: upper node outside the buffer");
+    const size_t upAt = rootAt + (size_t)ch;
+    const uint8_t *up = b + upAt;
     for (uint32_t n = 0; n < 32768; ++n) {
       if (!((host_load<uint64_t>(up + 4128 + 8 * (n >> 6)) >> (n & 63)) & 1u)) continue;
       const int64_t lc = host_load<int64_t>(up + 8256 + 8 * (size_t)n);
-      const uint8_t *lo = up + lc;
-      BN_REQUIRE(lc > 0 && inside(lo, kLowerBytes), "bnamd_nanovdb_create: lower node outside the buffer");
+      BN_REQUIRE(lc > 0 && inside(upAt, lc, kLowerBytes), "bnamd_nanovdb_create: lower node outside the buffer");
+      const size_t loAt = upAt + (size_t)lc;
+      const uint8_t *lo = b + loAt;
       for (uint32_t m = 0; m < 4096; ++m) {
         if (!((host_load<uint64_t>(lo + 544 + 8 * (m >> 6)) >> (m & 63)) & 1u)) continue;
         const int64_t fc = host_load<int64_t>(lo + 1088 + 8 * (size_t)m);
-        const uint8_t *leaf = lo + fc;
-        BN_REQUIRE(fc > 0 && inside(leaf, 96), "bnamd_nanovdb_create: leaf outside the buffer");
+        BN_REQUIRE(fc > 0 && inside(loAt, fc, 96), "bnamd_nanovdb_create: leaf outside the buffer");
+        const size_t leafAt = loAt + (size_t)fc;
+        const uint8_t *leaf = b + leafAt;
         size_t size = 0;
         switch (type) {
           case kNvFloat: size = 2144; break;
@@ -92,7 +101,7 @@ void check_grid(const uint8_t *b, size_t bytes, int &type, LeafFn &&onLeaf) {
             size = 96 + ((size_t)1 << bw) * 64;
           }
         }
-        BN_REQUIRE(inside(leaf, size), "bnamd_nanovdb_create: leaf outside the buffer");
+        BN_REQUIRE(inside(leafAt, 0, size), "bnamd_nanovdb_create: leaf outside the buffer");
         // the leaf's origin: the upper origin plus the two table slots
         onLeaf(leaf, ux + (int)((n >> 10) << 7) + (int)((m >> 8) << 3),
                uy + (int)(((n >> 5) & 31) << 7) + (int)(((m >> 4) & 15) << 3),
