@@ -198,9 +198,10 @@ def camera(position, direction, up=(0, 1, 0), fovy=60.0):
 
 
 def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
-                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None):
+                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None, lights=None):
     """env: an EnvMap (HDRI environment light) or None (uniform ambient);
-    cut_plane: (nx, ny, nz, d) or None (disabled)."""
+    cut_plane: (nx, ny, nz, d) or None (disabled); lights: Lights (quad,
+    directional and point lights) or None."""
     p = capi.bnamd_render_params()
     p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
     p.ambientRadiance = float(ambient)
@@ -209,7 +210,31 @@ def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0
     p.envMap = env.handle.value if env is not None else None
     p._env = env  # keeps the map alive as long as the parameters
     p.cutPlane[:] = [float(x) for x in (cut_plane if cut_plane is not None else (0.0, 0.0, 0.0, -1e30))]
+    p.lights = lights.handle.value if lights is not None else None
+    p._lights = lights  # keeps the lists alive as long as the parameters
     return p
+
+
+class Lights:
+    """Quad, directional and point lights on the device (World::DD's light
+    lists). quads (n, 12) = corner, edge0, edge1, emission; dirs (n, 7) =
+    direction, color, radiance; points (n, 8) = position, color, power,
+    intensity (NaN: radiance = color * power / 4pi)."""
+
+    def __init__(self, quads=(), dirs=(), points=()):
+        lib = init()
+        self.quads = np.ascontiguousarray(np.asarray(quads, np.float32).reshape(-1, 12))
+        self.dirs = np.ascontiguousarray(np.asarray(dirs, np.float32).reshape(-1, 7))
+        self.points = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 8))
+        self.handle = C.c_void_p()
+        check(lib.bnamd_lights_create(self.quads.ctypes.data, len(self.quads), self.dirs.ctypes.data,
+                                      len(self.dirs), self.points.ctypes.data, len(self.points),
+                                      C.byref(self.handle)))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            capi.load().bnamd_lights_destroy(self.handle)
+            self.handle = C.c_void_p()
 
 
 TEXEL_FORMATS = {"float4": 33, "float": 30, "uchar4": 1, "uchar": 0}

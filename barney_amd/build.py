@@ -34,6 +34,7 @@ SOURCES = [
     "render_surface.hip",
     "volume.hip",
     "nanovdb.hip",
+    "lights.hip",
     "ea.hip",
     "render_batch.hip",
     "umesh.hip",

@@ -48,7 +48,7 @@ class bnamd_render_params(C.Structure):
                 ("pathsPerPixel", C.c_int), ("ambientRadiance", C.c_float),
                 ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
                 ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int), ("envMap", C.c_void_p),
-                ("cutPlane", C.c_float * 4)]
+                ("cutPlane", C.c_float * 4), ("lights", C.c_void_p)]
 
 
 class bnamd_texture2d_desc(C.Structure):
@@ -222,6 +222,9 @@ SIGNATURES = {
     "bnamd_envmap_create": (C.c_int, [C.c_void_p, C.POINTER(bnamd_envmap_desc), C.c_void_p,
                                       C.POINTER(C.c_void_p)]),
     "bnamd_envmap_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_lights_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                      C.POINTER(C.c_void_p)]),
+    "bnamd_lights_destroy": (C.c_int, [C.c_void_p]),
     "bnamd_envmap_get_cdfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bnamd_envmap_get_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bnamd_envmap_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

@@ -135,7 +135,7 @@ __device__ __forceinline__ void wave_add(unsigned long long v, unsigned long lon
 // Megakernel: L lanes per pixel (each looping over its share of the paths),
 // 4 wavefronts per workgroup covering a TW x TH pixel tile of one view.
 // ===========================================================================
-template <int L, bool TL, bool kEnv, bool kHit>
+template <int L, bool TL, bool kLights, bool kHit>
 __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, RenderDD rp, int spp,
                                                     float4 *__restrict accum, int4 *__restrict ids,
                                                     unsigned long long *__restrict counters, bool cutouts) {
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
           }
         }
         PathRay shadow;
-        add_frag(sum, bounce<kEnv>(rpp, ray, sh, shadow, gen), clampMax);
+        add_frag(sum, bounce<kLights>(rpp, ray, sh, shadow, gen), clampMax);
         if (shadow.tMax > 0.f) {
           ShadeHit shs;
           HitRec hs;
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(256) k_batch_mega(WorldDev w, ViewsDD vd, Rend
             shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
           ++nshadow;
           PathRay dummy;
-          add_frag(sum, bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          add_frag(sum, bounce<kLights>(rpp, shadow, shs, dummy, gen + 1), clampMax);
         }
       }
       v = make_float4(sum.x, sum.y, sum.z, alpha);
@@ -520,7 +520,7 @@ __device__ __forceinline__ void accum_add(float4 *accum, int pix, vec3f f, float
   if (f.z > 0.f) atomicAdd(p + 2, f.z);
 }
 
-template <bool TL, bool kEnv>
+template <bool TL, bool kLights>
 __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
   const unsigned n = e.cnt->count[depth];
   const PathRec *__restrict qin = e.q[depth & 1];
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(WfDD e, int depth) {
         // with cutout triangles k_wf_ids wrote the closest candidate instead
         if (e.ids && rpp.accumID == 0 && !e.cutouts) e.ids[pix] = hit_ids<TL>(e.w, ray.org, dir, ray.tMax, h);
       }
-      const vec3f frag = bounce<kEnv>(rpp, ray, sh, shadow, depth);
+      const vec3f frag = bounce<kLights>(rpp, ray, sh, shadow, depth);
       accum_add(e.accum, pix, frag, clampMax);
       wantShadow = shadow.tMax > 0.f;
       wantCont = ray.tMax > 0.f;
@@ -835,7 +835,7 @@ void launch_mega(const World &w, const bnamd_views *views, const RenderDD &rp, i
   auto go = [&](auto kernel) {
     kernel<<<grid, 256, lds, s>>>(w.dd(), vd, rp, spp, accum, ids, cnt, w.hasCutouts);
   };
-  switch ((w.twoLevel ? 4 : 0) | (has_env(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0)) {
+  switch ((w.twoLevel ? 4 : 0) | (has_lights(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0)) {
     case 0: go(k_batch_mega<L, false, false, false>); break;
     case 1: go(k_batch_mega<L, false, false, true>); break;
     case 2: go(k_batch_mega<L, false, true, false>); break;
@@ -874,7 +874,7 @@ void enqueue_wavefront_t(const WfDD &e, int maxDepth, int gridPersist, size_t ld
     k_wf_sort_count<TL><<<kSortBlocks, 256, 0, s>>>(e, d);
     k_wf_sort_scan<<<1, kSortBlocks, 0, s>>>(e);
     k_wf_sort_scatter<<<kSortBlocks, 256, 0, s>>>(e, d);
-    if (has_env(e.rp))
+    if (has_lights(e.rp))
       k_wf_shade<TL, true><<<gridStream, 256, 0, s>>>(e, d);
     else
       k_wf_shade<TL, false><<<gridStream, 256, 0, s>>>(e, d);

@@ -51,10 +51,10 @@ __device__ __forceinline__ void add_fragment(vec3f &sum, vec3f frag, float clamp
   if (frag.z > 0.f) sum.z += frag.z;
 }
 
-// kEnv: an HDRI map lights the scene; kHit: a cut plane or cutout
+// kLights: an HDRI map or light lists light the scene; kHit: a cut plane or cutout
 // triangles, so candidate hits pass the anyHit filter (kernels without
 // either keep the plain traversal).
-template <bool TL, bool kEnv, bool kHit>
+template <bool TL, bool kLights, bool kHit>
 __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam, RenderDD rp, int spp,
                                                         float4 *__restrict accum, int4 *__restrict ids,
                                                         unsigned long long *__restrict counters, bool cutouts) {
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
           }
         }
         PathRay shadow;
-        vec3f frag = bounce<kEnv>(rpp, ray, sh, shadow, gen);
+        vec3f frag = bounce<kLights>(rpp, ray, sh, shadow, gen);
         add_fragment(sum, frag, clampMax);
         if (shadow.tMax > 0.f) {
           const vec3f sdir = fix_dir(shadow.dir);
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
             shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
           ++nrays;
           PathRay dummy;
-          frag = bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1);
+          frag = bounce<kLights>(rpp, shadow, shs, dummy, gen + 1);
           add_fragment(sum, frag, clampMax);
         }
       }
@@ -182,7 +182,7 @@ extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera
         kernel<<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, c,
                                         w.hasCutouts);
       };
-      const int variant = (w.twoLevel ? 4 : 0) | (has_env(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0);
+      const int variant = (w.twoLevel ? 4 : 0) | (has_lights(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0);
       switch (variant) {
         case 0: go(k_render_surface<false, false, false>); break;
         case 1: go(k_render_surface<false, false, true>); break;

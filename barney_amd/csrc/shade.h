@@ -1,18 +1,21 @@
 // Camera rays and the shade/bounce step of barney's path tracer with the
-// environment (ambient) light and single scattering:
+// environment (ambient or HDRI) light, quad / directional / point lights and
+// single scattering:
 //   camera           barney/Camera.cu:81-100
 //   ray generation   barney/kernels/generateRays.cu:73-129, 199-210
 //   bounce()         barney/kernels/shadeRays.cu:476-878
 //   env light        shadeRays.cu:267-293 (sampleEnvLight), 295-404 (sampleLights),
 //                    437-470 (radianceFromEnv, primaryRayMissColor); the
-//                    HDRI map in envmap.h
+//                    HDRI map in envmap.h; the light lists in lights.h
 //   BSDFs            packedBSDF/Lambertian.h:55-87, Phase.h:120-172
 //   materials        material/AnariMatte.h:51-63
 //   accumulation     shadeRays.cu:970-1038
 #pragma once
 #include <hip/hip_fp16.h>
+#include <cstring>
 #include "envmap.h"
 #include "fmath.h"
+#include "lights.h"
 #include "world.h"
 
 namespace bnamd {
@@ -46,10 +49,12 @@ struct RenderDD {
   int cutOn;        // cut plane enabled
   float4 cut;       // Renderer "cutPlane" (nx, ny, nz, d)
   EnvDD env;        // env.tex.texels == nullptr: uniform ambient radiance
+  LightsDD lights;  // quad, directional and point lights (counts 0: none)
 };
 
 inline RenderDD make_render_dd(const bnamd_render_params &p) {
   RenderDD r;
+  memset(&r, 0, sizeof r);  // padding too: graph cache keys compare these bytes
   r.width = p.width;
   r.height = p.height;
   r.accumID = p.accumID;
@@ -67,10 +72,15 @@ inline RenderDD make_render_dd(const bnamd_render_params &p) {
     r.env = p.envMap->dd;
   else
     r.env = EnvDD{};
+  r.lights = p.lights ? p.lights->dd : LightsDD{};
   return r;
 }
 
-BN_HD bool has_env(const RenderDD &rp) { return rp.env.tex.texels != nullptr; }
+BN_HD bool has_map(const RenderDD &rp) { return rp.env.tex.texels != nullptr; }
+// Kernels are instantiated with kLights = has_lights(rp): an HDRI map or a
+// light list, so that scenes lit by the uniform ambient radiance alone run
+// code without the map's and the lists' samplers.
+BN_HD bool has_lights(const RenderDD &rp) { return has_map(rp) || has_light_lists(rp.lights); }
 
 // The candidate filter of a surface trace: the renderer's cut plane and
 // (stochastic = true) the stochastic opacity of cutout triangles, drawn
@@ -553,19 +563,22 @@ BN_HD ScatterRes bsdf_scatter(const Bsdf &b, vec3f Ng, vec3f wo, bool insideMedi
   return s;
 }
 
-// shadeRays.cu:476-878 for the environment light (HDRI map or uniform
-// ambient radiance). Returns the fragment this
-// ray adds to its pixel. On return ray.tMax > 0 means the path continues with
-// the updated ray; shadow.tMax > 0 means a shadow ray was spawned.
-// kEnv: rp carries an HDRI map (kernels are instantiated for both cases so
-// that the map's code costs the ambient-light kernels no registers).
-template <bool kEnv>
+// shadeRays.cu:476-878 with sampleLights (:295-404) over the environment
+// (HDRI map or uniform ambient radiance) and the quad, directional and
+// point lights. Returns the fragment this ray adds to its pixel. On return
+// ray.tMax > 0 means the path continues with the updated ray; shadow.tMax > 0
+// means a shadow ray was spawned.
+// kLights: rp carries an HDRI map or light lists (has_lights); kernels are
+// instantiated for both cases so that the samplers cost the ambient-light
+// kernels no registers.
+template <bool kLights>
 BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay &shadow, int pathDepth) {
 #pragma clang fp contract(off)
   const float EPS = 1e-4f;
   shadow.tMax = -1.f;
   vec3f fragment = vec3f(0.f);
   const vec3f incomingTp = ray.tp;
+  const bool map = kLights && has_map(rp);
   if (ray.isShadow) {
     if (!h.hit) fragment = ray.mis * ray.tp;
     ray.tMax = -1.f;
@@ -579,9 +592,9 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   if (!h.hit) {
     // primaryRayMissColor / radianceFromEnv (shadeRays.cu:437-470, 556-575)
     if (pathDepth == 0)
-      fragment = kEnv ? env_radiance(rp.env, ray.dir) : ray.missColor;
+      fragment = map ? env_radiance(rp.env, ray.dir) : ray.missColor;
     else
-      fragment = ray.tp * (kEnv ? env_radiance(rp.env, ray.dir) : vec3f(rp.ambient)) * ray.mis;
+      fragment = ray.tp * (map ? env_radiance(rp.env, ray.dir) : vec3f(rp.ambient)) * ray.mis;
     ray.tMax = -1.f;
     return fragment;
   }
@@ -597,7 +610,8 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   vec3f lsDir;
   vec3f lsRad;
   float lsPdf;
-  if (kEnv) {
+  float lsDist = BN_INF;
+  if (map) {
     env_sample(rp.env, random, lsDir, lsRad, lsPdf);
   } else {
     lsDir = random_direction(random);
@@ -610,22 +624,55 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
       lsPdf = BN_ONE_OVER_FOUR_PI;
     }
   }
-  // ---- sampleLights (shadeRays.cu:295-404): the area, point and
-  // directional lights are absent (weight 0, pdf 0); a draw at or below the
-  // area weight or above the environment's share picks no light ----
+  // ---- sampleLights (shadeRays.cu:295-404): the environment's sample, then
+  // the area, directional and point lights' (in that draw order); one is
+  // picked by its share of the summed weights. A light whose sampler found
+  // nothing has weight 0: picking it (a draw at a boundary, or past the
+  // rounded sum) gives no light, as the default Light::Sample's pdf 0 does.
   bool haveLight = false;
   bool lightNeedsMIS = false;
   {
     const float elsWeight = reduce_max(lsRad);
-    const float sumWeights = ((0.f + 0.f) + elsWeight) + 0.f;
+    LightSample als, dls, pls;
+    float alsWeight = 0.f, dlsWeight = 0.f, plsWeight = 0.f;
+    bool alsOk = false, dlsOk = false, plsOk = false;
+    if (kLights) {
+      alsOk = sample_quad_lights(rp.lights, P, Ngff, random, als);
+      if (alsOk) alsWeight = reduce_max(als.rad) / als.pdf;
+      dlsOk = sample_dir_lights(rp.lights, Ngff, random, dls);
+      if (dlsOk) dlsWeight = reduce_max(dls.rad);
+      plsOk = sample_point_lights(rp.lights, P, Ngff, random, pls);
+      if (plsOk) plsWeight = reduce_max(pls.rad);
+    }
+    const float sumWeights = ((alsWeight + dlsWeight) + elsWeight) + plsWeight;
     if (sumWeights != 0.f) {
-      const float alsW = 0.f * (1.f / sumWeights);
+      const float alsW = alsWeight * (1.f / sumWeights);
       const float elsW = elsWeight * (1.f / sumWeights);
+      const float dlsW = dlsWeight * (1.f / sumWeights);
+      const float plsW = plsWeight * (1.f / sumWeights);
       const float r = random();
-      if (!(r <= alsW) && r <= alsW + elsW) {
+      if (r <= alsW) {
+        if (kLights && alsOk) {
+          lsDir = als.dir;
+          lsRad = als.rad * (1.f / alsW);
+          lsDist = als.dist;
+          lsPdf = als.pdf;
+          haveLight = !(lsPdf <= 0.f) && !(lsPdf != lsPdf);
+        }
+      } else if (r <= alsW + elsW) {
         lsRad *= (1.f / elsW);
         lightNeedsMIS = true;
         haveLight = !(lsPdf <= 0.f) && !(lsPdf != lsPdf);
+      } else if (kLights) {
+        const bool point = r <= (alsW + elsW) + plsW;
+        const LightSample &ls = point ? pls : dls;
+        if (point ? plsOk : dlsOk) {
+          lsDir = ls.dir;
+          lsRad = ls.rad * (1.f / (point ? plsW : dlsW));
+          lsDist = ls.dist;
+          lsPdf = ls.pdf;  // inf
+          haveLight = !(lsPdf <= 0.f) && !(lsPdf != lsPdf);
+        }
       }
     }
   }
@@ -640,7 +687,7 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
       shadow.isShadow = true;
       shadow.dir = lsDir;
       shadow.org = P + offsetEps * ffOffset;
-      shadow.tMax = BN_INF * (1.f - 2.f * offsetEps);
+      shadow.tMax = lsDist * (1.f - 2.f * offsetEps);
       shadow.tp = tp_sr;
       shadow.rng = ray.rng;
       ray.rng.next_hash((uint64_t)float_as_uint(h.t));
@@ -649,7 +696,7 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
       shadow.isInMedium = ray.isInMedium;
       if (lightNeedsMIS) {
         // EnvMapLight::DD::pdf (1/4pi without a texture)
-        const float pl = kEnv ? env_pdf(rp.env, lsDir) : BN_ONE_OVER_FOUR_PI;
+        const float pl = map ? env_pdf(rp.env, lsDir) : BN_ONE_OVER_FOUR_PI;
         const float pb = bsdf_pdf(bsdf, Ng, lsDir, wo);
         shadow.mis = pl / (pl + pb + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tMax = -1.f;
@@ -684,7 +731,7 @@ BN_HD vec3f bounce(const RenderDD &rp, PathRay &ray, const ShadeHit &h, PathRay 
   if (sr.changedMedium) ray.isInMedium = !ray.isInMedium;
   if (lightNeedsMIS && !is_inf(sr.pdf)) {
     const float pb = bsdf_pdf(bsdf, Ng, ray.dir, wo);
-    const float pl = kEnv ? env_pdf(rp.env, ray.dir) : BN_ONE_OVER_FOUR_PI;
+    const float pl = map ? env_pdf(rp.env, ray.dir) : BN_ONE_OVER_FOUR_PI;
     ray.mis = pb / (pb + pl);
   } else {
     ray.mis = 1.f;

@@ -77,7 +77,7 @@ BN_D ShadeHit umesh_trace(const UMeshDD &u, const UMeshOpts &o, vec3f org, vec3f
 // visits, element tests and their bytes (counters 4, 5, 6); it renders the
 // same image with the same paths, so one counted frame gives the
 // uncounted kernel's per-frame figures.
-template <int L, bool kVol, bool kIso, bool kMixed, bool kEnv, bool kCount = false>
+template <int L, bool kVol, bool kIso, bool kMixed, bool kLights, bool kCount = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
                unsigned long long *__restrict counters) {
@@ -118,7 +118,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
         ++nrays;
         if (gen == 0) alpha = sh.hit ? 1.f : ray.missAlpha;
         PathRay shadow;
-        vec3f frag = vmin(bounce<kEnv>(rpp, ray, sh, shadow, gen), clampMax);
+        vec3f frag = vmin(bounce<kLights>(rpp, ray, sh, shadow, gen), clampMax);
         if (frag.x > 0.f) sum.x += frag.x;
         if (frag.y > 0.f) sum.y += frag.y;
         if (frag.z > 0.f) sum.z += frag.z;
@@ -128,7 +128,7 @@ k_render_umesh(UMeshDD u, UMeshOpts o, CameraDD cam, RenderDD rp, int spp, float
                                                       ctx, cnt, hint, tc);
           ++nrays;
           PathRay dummy;
-          frag = vmin(bounce<kEnv>(rpp, shadow, shs, dummy, gen + 1), clampMax);
+          frag = vmin(bounce<kLights>(rpp, shadow, shs, dummy, gen + 1), clampMax);
           if (frag.x > 0.f) sum.x += frag.x;
           if (frag.y > 0.f) sum.y += frag.y;
           if (frag.z > 0.f) sum.z += frag.z;
@@ -168,7 +168,7 @@ void launch_render_umesh_t(const UMeshDD &u, const UMeshOpts &o, const CameraDD 
   const int tiles = div_up(rp.width, TW) * div_up(rp.height, TH);
   const int grid = ((tiles + 7) / 8) * 8;
   auto go = [&](auto kernel) { kernel<<<grid, 256, lds, s>>>(u, o, cam, rp, spp, accum, counters); };
-  const bool env = has_env(rp);
+  const bool env = has_lights(rp);
   if (o.volume && o.iso)
     env ? go(k_render_umesh<L, true, true, kMixed, true, kCount>) : go(k_render_umesh<L, true, true, kMixed, false, kCount>);
   else if (o.volume)

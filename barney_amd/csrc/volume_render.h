@@ -81,7 +81,7 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long
 // F is the field: VolumeDD (structured data) or AmrDD (block-structured
 // AMR); both carry the macrocell grid and transfer function, and
 // sample_and_map(F, P) gives the mapped sample.
-template <int PB, class F, bool kEnv>
+template <int PB, class F, bool kLights>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kVolumeWaves)))
 k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accum,
                 unsigned long long *__restrict counters) {
@@ -205,7 +205,7 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
           sh.N = vec3f(0.f);
           sh.bsdf = phase_bsdf(tr.rgb);
           PathRay sray;
-          (void)bounce<kEnv>(rpp, ray, sh, sray, 0);  // no fragment at a volume collision
+          (void)bounce<kLights>(rpp, ray, sh, sray, 0);  // no fragment at a volume collision
           if (sray.tMax > 0.f) {
             shadowW = sray.mis * sray.tp;  // bounce() of the shadow ray if unoccluded
             tr.org = sray.org;
@@ -231,7 +231,7 @@ k_render_volume(F v, CameraDD cam, RenderDD rp, int spp, float4 *__restrict accu
             frag = tr.hit ? vec3f(0.f) : shadowW;
           } else {  // bounce() of a primary miss: the background, or the map
             miss_color(rpp, iy, frag, alpha);
-            if (kEnv) frag = env_radiance(rpp.env, camera_ray(cam, rpp, ix, iy).dir);
+            if (kLights && has_map(rpp)) frag = env_radiance(rpp.env, camera_ray(cam, rpp, ix, iy).dir);
           }
           frag = vmin(frag, 10.f * (1 + rpp.accumID));
           slots[k] = make_float4(frag.x > 0.f ? frag.x : 0.f, frag.y > 0.f ? frag.y : 0.f,
@@ -282,7 +282,7 @@ void launch_render_volume(const F &v, const CameraDD &cam, const RenderDD &rp, i
   // xcd_remap_v hands XCD x the tiles [x*per, (x+1)*per): the grid must
   // hold 8*per workgroups (a multiple of 8) to cover every tile
   const int grid = ((tiles + 7) / 8) * 8;
-  if (has_env(rp))
+  if (has_lights(rp))
     k_render_volume<PB, F, true><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);
   else
     k_render_volume<PB, F, false><<<grid, 256, 0, s>>>(v, cam, rp, spp, accum, counters);

@@ -223,6 +223,9 @@ typedef struct {
    * dot(n, P) + d < 0 are ignored and primary rays are clipped to the
    * visible half-space. Disabled when d <= -1e28 (or all four are 0). */
   float cutPlane[4];
+  /* quad, directional and point lights sampled next to the environment
+   * (World::DD's light lists, render/World.h:28-35) or NULL: none */
+  const struct bnamd_lights_s *lights;
 } bnamd_render_params;
 
 /* ------------------------------------------------------------------------
@@ -268,6 +271,38 @@ int bnamd_envmap_eval(const bnamd_envmap *env, const float *dirs, float *pdf, fl
 /* EnvMapLight::DD::sample with Random2(i * 0x9E3779B9, seed) for i < n:
  * out device float[n][7] = direction[3], radiance[3], pdf */
 int bnamd_envmap_sample(const bnamd_envmap *env, uint32_t seed, float *out, int n, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Quad, directional and point lights: World::set of the light lists
+ * (barney/render/World.cpp:80-120) with each light's getDD
+ * (light/QuadLight.cu:9-18, DirLight.cu:10-20, PointLight.cu:9-17); the
+ * instance transform is applied by the caller. Sampled by bounce() as
+ * sampleLights does (kernels/shadeRays.cu:46-404): a reservoir of 8 quad
+ * candidates, of 2 directional and of 2 point candidates, next to the
+ * environment's sample.
+ * --------------------------------------------------------------------- */
+typedef struct {
+  float corner[3], edge0[3], edge1[3]; /* QuadLight "corner", "edge0", "edge1" */
+  float emission[3];                   /* "emission" (radiance) */
+} bnamd_quad_light;
+typedef struct {
+  float direction[3]; /* DirLight "direction": the way the light travels */
+  float color[3];     /* Light "color" */
+  float radiance;     /* "irradiance" if set, else "radiance" (DirLight.cu:15-18) */
+} bnamd_dir_light;
+typedef struct {
+  float position[3];
+  float color[3];
+  float power;     /* "power" (default 1) */
+  float intensity; /* "intensity"; NaN: radiance = color * power / (4 pi) */
+} bnamd_point_light;
+typedef struct bnamd_lights_s bnamd_lights;
+/* Copies the lists to the device (each count >= 0, arrays may be NULL when
+ * their count is 0). A quad's edges must span a non-zero area; a
+ * directional light's direction must be non-zero. */
+int bnamd_lights_create(const bnamd_quad_light *quads, int numQuads, const bnamd_dir_light *dirs, int numDirs,
+                        const bnamd_point_light *points, int numPoints, bnamd_lights **out);
+int bnamd_lights_destroy(bnamd_lights *lights);
 
 typedef struct {
   long long raysTraced; /* all rays traced: primary, shadow and bounce */

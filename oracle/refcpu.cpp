@@ -353,6 +353,172 @@ void env_build(ref_envmap_s *e, const float *direction, const float *up) {
 
 }  // namespace
 
+// ---- quad, directional and point lights ------------------------------------
+// Each light's DD as its getDD leaves it (light/QuadLight.cu:9-18,
+// DirLight.cu:10-20, PointLight.cu:9-17); the quad's normal, which getDD
+// leaves unset, is normalize(edge0 x edge1).
+struct RefQuad {
+  V3 corner, edge0, edge1, emission, normal;
+  float area;
+};
+struct RefDir {
+  V3 direction, color;
+  float radiance;
+};
+struct RefPoint {
+  V3 position, color;
+  float power, intensity;
+};
+struct ref_lights_s {
+  std::vector<RefQuad> quad;
+  std::vector<RefDir> dir;
+  std::vector<RefPoint> point;
+};
+
+namespace {
+
+// Light::Sample (light/Light.h); pdf 0 = nothing sampled
+struct LSample {
+  V3 dir = mk(0, 0, 0), rad = mk(0, 0, 0);
+  float dist = 0.f, pdf = 0.f;
+};
+
+// sampleAreaLights (kernels/shadeRays.cu:46-139)
+bool sample_area(const ref_lights_s *L, V3 P, V3 N, Rng &rnd, LSample &ls) {
+  const int n = L ? (int)L->quad.size() : 0;
+  if (n == 0) return false;
+  const int R = 8;
+  int id[R];
+  float u[R], v[R], w[R];
+  float sum = 0.f;
+  for (int i = 0; i < R; ++i) {
+    id[i] = std::min((int)(rnd.uni() * (float)n), n - 1);
+    w[i] = 0.f;
+    const RefQuad &q = L->quad[id[i]];
+    u[i] = rnd.uni();
+    v[i] = rnd.uni();
+    const V3 LP = add(add(q.corner, scl(u[i], q.edge0)), scl(v[i], q.edge1));
+    V3 d = sub(LP, P);
+    const float dist = len(d);
+    if (dist < 1e-3f) continue;
+    d = scl(1.f / dist, d);
+    float weight = eq(N, mk(0, 0, 0)) ? 1.f : dot(d, N);
+    if (weight <= 1e-3f) continue;
+    weight *= -dot(d, q.normal);
+    if (weight <= 1e-3f) continue;
+    weight *= (1.f / (dist * dist)) * q.area * maxc(q.emission);
+    sum += weight;
+    w[i] = weight;
+  }
+  if (sum == 0.f) return false;
+  float r = rnd.uni() * sum;
+  int i = 0;
+  while (i < R && r >= w[i]) {
+    r -= w[i];
+    ++i;
+  }
+  if (i == R) return false;
+  const RefQuad &q = L->quad[id[i]];
+  const V3 LD = sub(add(add(q.corner, scl(u[i], q.edge0)), scl(v[i], q.edge1)), P);
+  ls.dir = nrm(LD);
+  ls.dist = len(LD);
+  ls.rad = scl(q.area * -dot(q.normal, ls.dir) / (ls.dist * ls.dist), q.emission);
+  ls.pdf = w[i] / sum * ((float)R / (float)n);
+  return true;
+}
+
+// sampleDirLights (shadeRays.cu:141-193)
+bool sample_dir(const ref_lights_s *L, V3 N, Rng &rnd, LSample &ls) {
+  const int n = L ? (int)L->dir.size() : 0;
+  if (n == 0) return false;
+  const int R = 2;
+  int id[R];
+  float w[R];
+  float sum = 0.f;
+  for (int i = 0; i < R; ++i) {
+    id[i] = std::min((int)(rnd.uni() * (float)n), n - 1);
+    w[i] = 0.f;
+    const RefDir &d = L->dir[id[i]];
+    const V3 lrad = scl(d.radiance, d.color);
+    float weight = eq(N, mk(0, 0, 0)) ? 1.f : dot(neg(d.direction), N);
+    if (weight <= 1e-3f) continue;
+    weight *= maxc(lrad);
+    if (weight <= 1e-3f) continue;
+    w[i] = weight;
+    sum += weight;
+  }
+  if (sum == 0.f) return false;
+  float r = rnd.uni() * sum;
+  int i = 0;
+  while (i < R && r >= w[i]) {
+    r -= w[i];
+    ++i;
+  }
+  if (i == R) return false;
+  const RefDir &d = L->dir[id[i]];
+  ls.dir = neg(d.direction);
+  ls.dist = kInf;
+  const float q = w[i] / sum * ((float)R / (float)n);
+  const V3 c = scl(1.f, scl(d.radiance, d.color));
+  ls.rad = mk(c.x / q, c.y / q, c.z / q);
+  ls.pdf = kInf;
+  return w[i] != 0.f;
+}
+
+// PointLight::DD::radianceTowards (PointLight.h:14-20)
+V3 point_radiance(const RefPoint &p) {
+  return std::isnan(p.intensity) ? scl(1.f / (4.f * kPi), scl(p.power, p.color)) : scl(p.intensity, p.color);
+}
+
+// samplePointLights (shadeRays.cu:195-265): falloff c0 + c1 d + c2 d^2 with
+// (0, 0, 1)
+bool sample_point(const ref_lights_s *L, V3 P, V3 N, Rng &rnd, LSample &ls) {
+  const int n = L ? (int)L->point.size() : 0;
+  if (n == 0) return false;
+  const int R = 2;
+  const float c0 = 0.f, c1 = 0.f, c2 = 1.f;
+  int id[R];
+  float w[R];
+  float sum = 0.f;
+  for (int i = 0; i < R; ++i) {
+    id[i] = std::min((int)(rnd.uni() * (float)n), n - 1);
+    w[i] = 0.f;
+    const RefPoint &p = L->point[id[i]];
+    V3 d = sub(p.position, P);
+    const float dist = len(d);
+    const float falloff = c0 + c1 * dist + c2 * dist * dist;
+    d = nrm(d);
+    const V3 lrad = scl(1.f / falloff, point_radiance(p));
+    float weight = eq(N, mk(0, 0, 0)) ? 1.f : dot(d, N);
+    if (weight <= 1e-3f) continue;
+    weight *= maxc(lrad);
+    if (weight <= 1e-3f) continue;
+    w[i] = weight;
+    sum += weight;
+  }
+  if (sum == 0.f) return false;
+  float r = rnd.uni() * sum;
+  int i = 0;
+  while (i < R && r >= w[i]) {
+    r -= w[i];
+    ++i;
+  }
+  if (i == R) return false;
+  const RefPoint &p = L->point[id[i]];
+  V3 d = sub(p.position, P);
+  const float dist = len(d);
+  ls.dir = nrm(d);
+  const float falloff = c0 + c1 * dist + c2 * dist * dist;
+  ls.dist = dist * .9999f;
+  const float q = w[i] / sum * ((float)R / (float)n);
+  const V3 c = scl(1.f, scl(1.f / falloff, point_radiance(p)));
+  ls.rad = mk(c.x / q, c.y / q, c.z / q);
+  ls.pdf = kInf;
+  return w[i] != 0.f;
+}
+
+}  // namespace
+
 struct ref_scene_s {
   std::vector<ref_geom> geoms;
   // triangles whose material is Glass: Glass::getOpacity returns 0 for
@@ -1218,8 +1384,8 @@ Scat bsdf_scatter(const HitInfo &h, V3 Ng, V3 wo, bool inMedium, Rng &rnd) {
   return s;
 }
 
-// shadeRays.cu:476-878: the environment light is the only light, an HDRI
-// map (EnvMapLight) or the uniform ambientRadiance
+// shadeRays.cu:476-878: the environment light (an HDRI map, EnvMapLight, or
+// the uniform ambientRadiance) and the quad, directional and point lights
 V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int depth, int maxDepth, int maxDiff) {
   sh.tMax = -1.f;
   V3 frag = mk(0, 0, 0);
@@ -1273,22 +1439,46 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
       lpdf = 1.f / (4.f * kPi);
     }
   }
-  // sampleLights (shadeRays.cu:295-404): the area, point and directional
-  // lights are absent (weight 0, pdf 0), so a draw at or below the area
-  // weight or above the environment's share picks no light
+  // sampleLights (shadeRays.cu:295-404): the environment's sample (above),
+  // then the area, directional and point lights' samples; one is picked by
+  // its share of the summed weights, its radiance divided by that share.
+  // Picking a light whose sampler returned false gives Light::Sample's
+  // default pdf 0, i.e. no light.
   bool needMIS = false, have = false;
+  float ldist = kInf;
   {
-    const float w = maxc(lrad);
-    const float sum = ((0.f + 0.f) + w) + 0.f;
+    const float elsW0 = maxc(lrad);
+    LSample als, dls, pls;
+    const float alsW0 = sample_area(p->lights, P, Ngff, rnd, als) ? maxc(als.rad) / als.pdf : 0.f;
+    const float dlsW0 = sample_dir(p->lights, Ngff, rnd, dls) ? maxc(dls.rad) : 0.f;
+    const float plsW0 = sample_point(p->lights, P, Ngff, rnd, pls) ? maxc(pls.rad) : 0.f;
+    const float sum = alsW0 + dlsW0 + elsW0 + plsW0;
     if (sum != 0.f) {
-      const float alsW = 0.f * (1.f / sum);
-      const float elsW = w * (1.f / sum);
+      const float alsW = alsW0 * (1.f / sum), elsW = elsW0 * (1.f / sum);
+      const float dlsW = dlsW0 * (1.f / sum), plsW = plsW0 * (1.f / sum);
       const float r = rnd.uni();
-      if (!(r <= alsW) && r <= alsW + elsW) {
-        lrad = scl(1.f / elsW, lrad);
+      LSample ls;
+      if (r <= alsW) {
+        ls = als;
+        ls.rad = scl(1.f / alsW, ls.rad);
+      } else if (r <= alsW + elsW) {
+        ls.dir = ld;
+        ls.rad = scl(1.f / elsW, lrad);
+        ls.dist = kInf;
+        ls.pdf = lpdf;
         needMIS = true;
-        have = !(lpdf <= 0.f) && !std::isnan(lpdf);
+      } else if (r <= alsW + elsW + plsW) {
+        ls = pls;
+        ls.rad = scl(1.f / plsW, ls.rad);
+      } else {
+        ls = dls;
+        ls.rad = scl(1.f / dlsW, ls.rad);
       }
+      have = !(std::isnan(ls.pdf) || ls.pdf <= 0.f);
+      ld = ls.dir;
+      lrad = ls.rad;
+      lpdf = ls.pdf;
+      ldist = ls.dist;
     }
   }
   if (have) {
@@ -1304,7 +1494,7 @@ V3 bounce(const ref_render_params *p, PRay &ray, const HitInfo &h, PRay &sh, int
       sh.shadow = true;
       sh.dir = ld;
       sh.org = add(P, scl(eps, off));
-      sh.tMax = kInf * (1.f - 2.f * eps);
+      sh.tMax = ldist * (1.f - 2.f * eps);
       sh.tp = tsr;
       sh.rng = ray.rng;
       ray.rng.mix((uint64_t)fbits(h.t));
@@ -1426,6 +1616,42 @@ ref_envmap *ref_envmap_create(const void *texels, int format, int w, int h, int 
   return e;
 }
 void ref_envmap_destroy(ref_envmap *e) { delete e; }
+
+ref_lights *ref_lights_create(const float *quads, int nq, const float *dirs, int nd, const float *points, int np) {
+  auto *L = new ref_lights_s;
+  for (int i = 0; i < nq; ++i) {
+    const float *q = quads + 12 * i;
+    RefQuad d;
+    d.corner = ld3(q);
+    d.edge0 = ld3(q + 3);
+    d.edge1 = ld3(q + 6);
+    d.emission = ld3(q + 9);
+    const V3 c = cross(d.edge0, d.edge1);
+    d.area = len(c);  // QuadLight::getDD: length(cross(edge0, edge1))
+    d.normal = nrm(c);
+    L->quad.push_back(d);
+  }
+  for (int i = 0; i < nd; ++i) {
+    const float *q = dirs + 7 * i;
+    RefDir d;
+    d.direction = nrm(ld3(q));  // DirLight::getDD normalizes
+    d.color = ld3(q + 3);
+    d.radiance = q[6];
+    L->dir.push_back(d);
+  }
+  for (int i = 0; i < np; ++i) {
+    const float *q = points + 8 * i;
+    RefPoint d;
+    d.position = ld3(q);
+    d.color = ld3(q + 3);
+    d.power = q[6];
+    d.intensity = q[7];
+    L->point.push_back(d);
+  }
+  return L;
+}
+
+void ref_lights_destroy(ref_lights *l) { delete l; }
 void ref_envmap_get_cdfs(const ref_envmap *e, float *cdfY, float *cdfX) {
   std::copy(e->cdfY.begin(), e->cdfY.end(), cdfY);
   std::copy(e->cdfX.begin(), e->cdfX.end(), cdfX);

@@ -107,6 +107,8 @@ typedef struct {
   /* Renderer "cutPlane" (nx, ny, nz, d): hits with dot(n, P) + d < 0 are
    * ignored; disabled when d <= -1e28 (Renderer.h:59) */
   float cutPlane[4];
+  /* quad, directional and point lights (World::DD's lists) or NULL */
+  const struct ref_lights_s *lights;
 } ref_render_params;
 
 /* --- HDRI environment light (barney/light/EnvMap.cu, EnvMap.h) --- */
@@ -118,6 +120,14 @@ typedef struct ref_envmap_s ref_envmap;
 ref_envmap *ref_envmap_create(const void *texels, int format, int w, int h, int filter, const int *addr,
                               const float *direction, const float *up, float scale);
 void ref_envmap_destroy(ref_envmap *e);
+
+/* --- quad, directional and point lights (barney/light/, World.cpp:80-120) ---
+ * quads: n x 12 floats (corner, edge0, edge1, emission); dirs: n x 7
+ * (direction, color, radiance); points: n x 8 (position, color, power,
+ * intensity; NaN intensity: color * power / (4 pi)) */
+typedef struct ref_lights_s ref_lights;
+ref_lights *ref_lights_create(const float *quads, int nq, const float *dirs, int nd, const float *points, int np);
+void ref_lights_destroy(ref_lights *l);
 void ref_envmap_get_cdfs(const ref_envmap *e, float *cdfY, float *cdfX);
 /* 3x3 column-major toWorld and toLocal */
 void ref_envmap_frame(const ref_envmap *e, float *toWorld, float *toLocal);

@@ -52,7 +52,7 @@ class ref_render_params(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("accumID", C.c_int), ("pathsPerPixel", C.c_int),
                 ("ambientRadiance", C.c_float), ("bgColor", C.c_float * 4), ("maxPathDepth", C.c_int),
                 ("maxDiffuseBounces", C.c_int), ("russianRoulette", C.c_int), ("envMap", C.c_void_p),
-                ("cutPlane", C.c_float * 4)]
+                ("cutPlane", C.c_float * 4), ("lights", C.c_void_p)]
 
 
 _lib = None
@@ -125,6 +125,8 @@ def load():
         "ref_trace_filtered": (None, [P, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ref_envmap_create": (P, [P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, C.c_float]),
         "ref_envmap_destroy": (None, [P]),
+        "ref_lights_create": (P, [P, C.c_int, P, C.c_int, P, C.c_int]),
+        "ref_lights_destroy": (None, [P]),
         "ref_envmap_get_cdfs": (None, [P, P, P]),
         "ref_envmap_frame": (None, [P, P, P]),
         "ref_envmap_tex2d": (None, [P, P, P, C.c_int]),
@@ -549,9 +551,10 @@ def camera(position, direction, up=(0, 1, 0), fovy=60.0):
 
 
 def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0.0, 1.0), max_depth=10,
-                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None):
+                  max_diffuse=1, russian_roulette=0, env=None, cut_plane=None, lights=None):
     """env: an EnvMap (HDRI environment light) or None; cut_plane: (nx, ny,
-    nz, d) or None (disabled)."""
+    nz, d) or None (disabled); lights: Lights (quad, directional and point
+    lights) or None."""
     p = ref_render_params()
     p.width, p.height, p.accumID, p.pathsPerPixel = width, height, accum_id, spp
     p.ambientRadiance = float(ambient)
@@ -561,7 +564,34 @@ def render_params(width, height, spp=1, accum_id=0, ambient=1.0, bg=(0.0, 0.0, 0
     p.envMap = env.handle if env is not None else None
     p._env = env  # keeps the map alive as long as the parameters
     p.cutPlane[:] = [float(x) for x in (cut_plane if cut_plane is not None else (0.0, 0.0, 0.0, -1e30))]
+    p.lights = lights.handle if lights is not None else None
+    p._lights = lights
     return p
+
+
+def light_arrays(quads=(), dirs=(), points=()):
+    """The three light lists as float32 arrays: quads (n, 12) = corner,
+    edge0, edge1, emission; dirs (n, 7) = direction, color, radiance; points
+    (n, 8) = position, color, power, intensity (NaN: color * power / 4pi)."""
+    q = np.ascontiguousarray(np.asarray(quads, np.float32).reshape(-1, 12))
+    d = np.ascontiguousarray(np.asarray(dirs, np.float32).reshape(-1, 7))
+    pt = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 8))
+    return q, d, pt
+
+
+class Lights:
+    """Quad, directional and point lights (light/QuadLight, DirLight,
+    PointLight), lists as light_arrays takes them."""
+
+    def __init__(self, quads=(), dirs=(), points=()):
+        self.quads, self.dirs, self.points = light_arrays(quads, dirs, points)
+        self.handle = load().ref_lights_create(self.quads.ctypes.data, len(self.quads), self.dirs.ctypes.data,
+                                               len(self.dirs), self.points.ctypes.data, len(self.points))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.ref_lights_destroy(self.handle)
+            self.handle = None
 
 
 def camera_ray(cam, params, ix, iy):
