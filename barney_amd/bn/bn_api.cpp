@@ -3,8 +3,8 @@
 // barney/Object.cpp), bnBuild turns the model's instances of groups of
 // geometries into one bnamd world (ModelSlot::build,
 // barney/ModelSlot.cpp:210-216) or builds its one volume (structured,
-// unstructured, block-structured AMR or NanoVDB field), and its groups' env-map
-// light (ModelSlot::updateWorldLightsFromInstances, ModelSlot.cpp:90-125);
+// unstructured, block-structured AMR or NanoVDB field), and its groups'
+// lights (ModelSlot::updateWorldLightsFromInstances, ModelSlot.cpp:90-125);
 // bnRender runs the fused path tracer into the frame buffer's accumulation
 // buffer (Context::render, barney/Context.cpp:142-185) and
 // bnFrameBufferRead resolves it (FrameBuffer::read,
@@ -182,6 +182,7 @@ struct Model : Object {
   bnamd_amr *amr = nullptr;
   bnamd_nanovdb *nvdb = nullptr;
   bnamd_envmap *env = nullptr;
+  bnamd_lights *lights = nullptr;
   std::vector<std::unique_ptr<DevMem>> mem;
   void release() {
     if (world) (void)bnamd_world_destroy(world);
@@ -190,12 +191,14 @@ struct Model : Object {
     if (amr) (void)bnamd_amr_destroy(amr);
     if (nvdb) (void)bnamd_nanovdb_destroy(nvdb);
     if (env) (void)bnamd_envmap_destroy(env);
+    if (lights) (void)bnamd_lights_destroy(lights);
     world = nullptr;
     volume = nullptr;
     umesh = nullptr;
     amr = nullptr;
     nvdb = nullptr;
     env = nullptr;
+    lights = nullptr;
     mem.clear();
   }
   ~Model() override { release(); }
@@ -543,23 +546,82 @@ void build_volume(Model *m, const VolumeObj *vol) {
   }
 }
 
-// the env-map light of the model's groups (the last one met, as
-// ModelSlot::updateWorldLightsFromInstances keeps it); EnvMapLight::commit
-// with its "texture", "direction", "up", "scale" (EnvMap.cu:157-171, 280-330)
+// affine3f applied to a point / a vector (owl's xfmPoint / xfmVector)
+std::array<float, 3> xfm_vector(const BNTransform &x, const std::array<float, 4> &v) {
+  return {x.l.vx.x * v[0] + x.l.vy.x * v[1] + x.l.vz.x * v[2], x.l.vx.y * v[0] + x.l.vy.y * v[1] + x.l.vz.y * v[2],
+          x.l.vx.z * v[0] + x.l.vy.z * v[1] + x.l.vz.z * v[2]};
+}
+std::array<float, 3> xfm_point(const BNTransform &x, const std::array<float, 4> &v) {
+  const auto r = xfm_vector(x, v);
+  return {r[0] + x.p.x, r[1] + x.p.y, r[2] + x.p.z};
+}
+
+// ModelSlot::updateWorldLightsFromInstances (ModelSlot.cpp:90-125): each
+// instance's group lights, quad / directional / point lights through their
+// getDD with the instance transform (QuadLight.cu:9-18, DirLight.cu:10-20,
+// PointLight.cu:9-17), and the env-map light (the last one met);
+// EnvMapLight::commit with its "texture", "direction", "up", "scale"
+// (EnvMap.cu:157-171, 280-330)
 void build_lights(Model *m) {
+#pragma clang fp contract(off)
   const Object *env = nullptr;
-  for (const Group *g : m->groups) {
+  std::vector<bnamd_quad_light> quads;
+  std::vector<bnamd_dir_light> dirs;
+  std::vector<bnamd_point_light> points;
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  for (size_t k = 0; k < m->groups.size(); ++k) {
+    const Group *g = m->groups[k];
+    const BNTransform &x = m->xfms[k];
     const Data *lights = dynamic_cast<const Data *>(g->geto("lights"));
     if (!lights) continue;
     if (lights->dataType != BN_OBJECT) throw std::runtime_error("bnBuild: 'lights' must be BN_OBJECT data");
     for (const Object *l : lights->objs) {
       if (!l) continue;
-      if (l->type != "envmap")
-        throw std::runtime_error("bnBuild: this backend renders the environment light only ('" + l->type +
-                                 "' lights are not supported)");
-      env = l;
+      const auto color = l->get4("color", {1.f, 1.f, 1.f, 0.f});  // Light::color
+      if (l->type == "envmap") {
+        env = l;
+      } else if (l->type == "quad") {  // QuadLight::DD defaults (QuadLight.h:15-22)
+        bnamd_quad_light q{};
+        const auto c = xfm_point(x, l->get4("corner", {0.f, 0.f, 0.f, 0.f}));
+        const auto e0 = xfm_vector(x, l->get4("edge0", {1.f, 0.f, 0.f, 0.f}));
+        const auto e1 = xfm_vector(x, l->get4("edge1", {0.f, 1.f, 0.f, 0.f}));
+        const auto em = l->get4("emission", {1.f, 1.f, 1.f, 0.f});
+        for (int j = 0; j < 3; ++j) {
+          q.corner[j] = c[j];
+          q.edge0[j] = e0[j];
+          q.edge1[j] = e1[j];
+          q.emission[j] = em[j];
+        }
+        quads.push_back(q);
+      } else if (l->type == "directional") {  // DirLight.h defaults: (0,0,-1), radiance 1
+        bnamd_dir_light d{};
+        const auto v = xfm_vector(x, l->get4("direction", {0.f, 0.f, -1.f, 0.f}));
+        for (int j = 0; j < 3; ++j) {
+          d.direction[j] = v[j];
+          d.color[j] = color[j];
+        }
+        const float irr = l->getf("irradiance", nan);
+        d.radiance = std::isnan(irr) ? l->getf("radiance", 1.f) : irr;
+        dirs.push_back(d);
+      } else if (l->type == "point") {  // PointLight.h defaults: origin, power 1
+        bnamd_point_light pt{};
+        const auto v = xfm_point(x, l->get4("position", {0.f, 0.f, 0.f, 0.f}));
+        for (int j = 0; j < 3; ++j) {
+          pt.position[j] = v[j];
+          pt.color[j] = color[j];
+        }
+        pt.power = l->getf("power", 1.f);
+        pt.intensity = l->getf("intensity", nan);
+        points.push_back(pt);
+      } else {
+        throw std::runtime_error("bnBuild: un-handled type of light '" + l->type + "'");
+      }
     }
   }
+  if (!quads.empty() || !dirs.empty() || !points.empty())
+    check(bnamd_lights_create(quads.data(), (int)quads.size(), dirs.data(), (int)dirs.size(), points.data(),
+                              (int)points.size(), &m->lights),
+          "bnBuild");
   if (!env) return;
   auto *tex = dynamic_cast<Texture2D *>(env->geto("texture"));
   if (!tex) throw std::runtime_error("bnBuild: envmap light without a 2D 'texture'");
@@ -727,6 +789,7 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
   p.maxDiffuseBounces = 1;
   p.russianRoulette = 0;
   p.envMap = m->env;
+  p.lights = m->lights;
   const auto cut = r->get4("cutPlane", {0.f, 0.f, 0.f, -1e30f});  // Renderer.cpp:95-98
   for (int k = 0; k < 4; ++k) p.cutPlane[k] = cut[k];
   int *ids = (fb->ids && fb->paths == 0) ? (int *)fb->ids->ptr : nullptr;

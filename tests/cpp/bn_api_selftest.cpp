@@ -344,19 +344,62 @@ int main(int argc, char **argv) {
     bnamd_world_destroy(lw);
     bnamd_envmap_destroy(em);
     bnamd_texture2d_destroy(t2d);
+    // quad, directional and point lights of a translated instance through
+    // bn* equal the same lights, moved by the translation, through the C ABI
+    BNLight ql = bnLightCreate(ctx, 0, "quad");
+    bnSet3f(ql, "corner", -1.f, 3.f, -1.f);
+    bnSet3f(ql, "edge0", 2.f, 0.f, 0.f);
+    bnSet3f(ql, "edge1", 0.f, 0.f, 2.f);  // edge0 x edge1 = -y: lights the floor
+    bnSet3f(ql, "emission", 4.f, 3.f, 2.f);
+    bnCommit(ql);
+    BNLight dl = bnLightCreate(ctx, 0, "directional");
+    bnSet3f(dl, "direction", 0.3f, -1.f, 0.2f);
+    bnSet3f(dl, "color", 1.f, .9f, .8f);
+    bnSet1f(dl, "irradiance", 2.f);
+    bnCommit(dl);
+    BNLight ptl = bnLightCreate(ctx, 0, "point");
+    bnSet3f(ptl, "position", 0.5f, 2.f, 0.5f);
+    bnSet1f(ptl, "power", 30.f);
+    bnCommit(ptl);
+    BNObject ls3[3] = {ql, dl, ptl};
+    bnSetData(lg, "lights", bnDataCreate(ctx, 0, BN_OBJECT, 3, ls3));
+    BNTransform xt = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {0.5f, 0.f, -0.25f}};
+    BNModel lm2 = bnModelCreate(ctx);
+    bnSetInstances(lm2, 0, &lg, &xt, 1);
+    bnBuild(lm2, 0);
+    bnAccumReset(fb);
+    bnRender(r2, lm2, cam, fb);
+    bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+    bnamd_quad_light cq = {{-1.f + 0.5f, 3.f, -1.f - 0.25f}, {2.f, 0.f, 0.f}, {0.f, 0.f, 2.f}, {4.f, 3.f, 2.f}};
+    bnamd_dir_light cd = {{0.3f, -1.f, 0.2f}, {1.f, .9f, .8f}, 2.f};
+    bnamd_point_light cp = {{0.5f + 0.5f, 2.f, 0.5f - 0.25f}, {1.f, 1.f, 1.f}, 30.f, std::nanf("")};
+    bnamd_lights *cl = nullptr;
+    if (bnamd_lights_create(&cq, 1, &cd, 1, &cp, 1, &cl) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_instance it = {0, 0, {1, 0, 0, 0, 1, 0, 0, 0, 1, 0.5f, 0.f, -0.25f}};
+    bnamd_world *tw = nullptr;
+    if (bnamd_world_build(&lgrp, 1, &it, 1, ml, 1, nullptr, &tw) != BNAMD_OK) return fail(bnamd_last_error());
+    bnamd_render_params pq = pl;
+    pq.envMap = nullptr;
+    pq.lights = cl;
+    if (bnamd_render_surface(tw, &c, &pq, (float *)acc, nullptr, nullptr, nullptr) != BNAMD_OK)
+      return fail("lights render");
+    bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+    bnamd_synchronize();
+    bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+    if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("quad/dir/point light render differs");
+    std::printf("quad, directional and point lights of a translated instance: bn* = C ABI\n");
+    bnamd_world_destroy(tw);
+    bnamd_lights_destroy(cl);
     bnamd_device_free(dn);
     bnamd_device_free(dc);
-    // lights other than the env map are not rendered: bnBuild says so
-    BNLight dl = bnLightCreate(ctx, 0, "directional");
-    BNObject dls[1] = {dl};
-    bnSetData(lg, "lights", bnDataCreate(ctx, 0, BN_OBJECT, 1, dls));
+    // a light type barney does not have is rejected
     bool thr = false;
     try {
-      bnBuild(lm, 0);
+      (void)bnLightCreate(ctx, 0, "spot");
     } catch (const std::runtime_error &) {
       thr = true;
     }
-    if (!thr) return fail("directional light accepted");
+    if (!thr) return fail("unknown light type accepted");
   }
   // an unstructured field (two hexahedra and a tetrahedron) and a
   // block-structured AMR field through bn* equal bnamd_umesh / bnamd_amr
