@@ -334,6 +334,24 @@ def render_surface(world, cam, params, with_ids=False, accum=None):
     return out
 
 
+def render_mixed(world, volume, cam, params, with_ids=False, accum=None):
+    """A scene of the world's surfaces and a structured Volume: each ray's
+    closest surface hit, then delta tracking through the volume up to it."""
+    n = params.width * params.height
+    acc = accum if accum is not None else DeviceArray.empty((params.height, params.width, 4), np.float32)
+    ids = DeviceArray.empty((params.height, params.width, 4), np.int32) if with_ids else None
+    st = capi.bnamd_render_stats()
+    ns = C.c_longlong(0)
+    check(capi.load().bnamd_render_mixed(world.handle, volume.handle, C.byref(cam), C.byref(params), acc.addr,
+                                         ids.addr if ids else None, None, C.byref(st), C.byref(ns)))
+    out = {"accum": acc, "stats": {"raysTraced": st.raysTraced, "primaryRays": st.primaryRays, "ms": st.ms,
+                                   "nodeVisits": st.nodeVisits, "primBytes": st.primBytes, "samples": ns.value},
+           "pixels": n}
+    if ids is not None:
+        out["ids"] = ids.numpy()
+    return out
+
+
 BATCH_MEGAKERNEL, BATCH_WAVEFRONT, BATCH_AUTO = 0, 1, 2
 
 

@@ -2,8 +2,9 @@
 // the bnamd C ABI: objects hold staged parameters (Object::set*,
 // barney/Object.cpp), bnBuild turns the model's instances of groups of
 // geometries into one bnamd world (ModelSlot::build,
-// barney/ModelSlot.cpp:210-216) or builds its one volume (structured,
-// unstructured, block-structured AMR or NanoVDB field), and its groups'
+// barney/ModelSlot.cpp:210-216) and/or builds its one volume (structured,
+// unstructured, block-structured AMR or NanoVDB field; next to geometry, a
+// structured one), and its groups'
 // lights (ModelSlot::updateWorldLightsFromInstances, ModelSlot.cpp:90-125);
 // bnRender runs the fused path tracer into the frame buffer's accumulation
 // buffer (Context::render, barney/Context.cpp:142-185) and
@@ -650,15 +651,24 @@ BN_API void bnBuild(BNModel model, int whichDataSlot) {
     ngeom += g->geoms.size();
     if (!g->volumes.empty()) vol = g->volumes[0];
   }
-  if (nvol > 0) {
-    if (nvol != 1 || ngeom != 0 || m->groups.size() != 1)
-      throw std::runtime_error("bnBuild: a model holds either geometry or one volume in this backend");
+  if (nvol > 0 && ngeom == 0) {
+    if (nvol != 1 || m->groups.size() != 1)
+      throw std::runtime_error("bnBuild: a volume model holds one volume in one instance in this backend");
     build_volume(m, vol);
     return;
+  }
+  if (nvol > 0) {
+    // geometry and a volume in one model: the surface world plus one
+    // structured volume, rendered by bnamd_render_mixed
+    if (nvol != 1 || vol->field->type != "structured")
+      throw std::runtime_error("bnBuild: a model mixing geometry and volumes takes one structured volume in this "
+                               "backend");
+    build_volume(m, vol);
   }
   std::vector<Group *> groups;  // distinct groups, in first-use order
   std::vector<bnamd_instance> insts;
   for (size_t k = 0; k < m->groups.size(); ++k) {
+    if (m->groups[k]->geoms.empty()) continue;  // the volume's own instance
     size_t gi = 0;
     while (gi < groups.size() && groups[gi] != m->groups[k]) ++gi;
     if (gi == groups.size()) groups.push_back(m->groups[k]);
@@ -814,7 +824,10 @@ BN_API void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFram
       o.volume = 1;
       check(bnamd_umesh_render(m->umesh, &c, &p, &o, (float *)fb->accum->ptr, nullptr, nullptr), "bnRender");
     }
-  } else
+  } else if (m->volume)
+    check(bnamd_render_mixed(m->world, m->volume, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr, nullptr),
+          "bnRender");
+  else
     check(bnamd_render_surface(m->world, &c, &p, (float *)fb->accum->ptr, ids, nullptr, nullptr), "bnRender");
   fb->paths += p.pathsPerPixel;
 }

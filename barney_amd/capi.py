@@ -162,6 +162,9 @@ SIGNATURES = {
     "bnamd_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "bnamd_render_surface": (C.c_int, [C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(bnamd_render_stats)]),
+    "bnamd_render_mixed": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(bnamd_camera), C.POINTER(bnamd_render_params),
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(bnamd_render_stats),
+                                     C.POINTER(C.c_longlong)]),
     "bnamd_volume_create": (C.c_int, [C.POINTER(bnamd_structured_desc), C.POINTER(C.c_void_p)]),
     "bnamd_volume_destroy": (C.c_int, [C.c_void_p]),
     "bnamd_volume_set_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),

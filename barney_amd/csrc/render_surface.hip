@@ -4,11 +4,18 @@
 // generation loop (barney/Context.cpp:142-185) into one thread per pixel:
 // each ray of a path only depends on its own RNG state, so per-pixel
 // depth-first order computes the same values as the queue-based order.
+// With a structured volume (a model holding geometry and a volume), each
+// ray's closest surface hit is found first and the volume is delta-tracked
+// up to it (MCVolumeAccel::isProg with the surface hit as tMax,
+// barney/volume/MCAccelerator.h:569-708): a collision before the surface is
+// a volume hit; shadow rays no surface blocks are tracked through the volume.
 #include "shade.h"
+#include "volume.h"
 
 namespace bnamd {
 
 size_t trace_lds_bytes(int depth, int block);
+const VolumeDD &volume_dd(const bnamd_volume *vol, const char *who);
 
 namespace {
 
@@ -53,11 +60,12 @@ __device__ __forceinline__ void add_fragment(vec3f &sum, vec3f frag, float clamp
 
 // kLights: an HDRI map or light lists light the scene; kHit: a cut plane or cutout
 // triangles, so candidate hits pass the anyHit filter (kernels without
-// either keep the plain traversal).
-template <bool TL, bool kLights, bool kHit>
+// either keep the plain traversal); kVol: 'vol' is in the scene.
+template <bool TL, bool kLights, bool kHit, bool kVol>
 __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam, RenderDD rp, int spp,
                                                         float4 *__restrict accum, int4 *__restrict ids,
-                                                        unsigned long long *__restrict counters, bool cutouts) {
+                                                        unsigned long long *__restrict counters, bool cutouts,
+                                                        VolumeDD vol) {
   extern __shared__ uint2 lds_stack[];
   const int tilesX = (rp.width + 15) >> 4, tilesY = (rp.height + 15) >> 4;
   const int tile = xcd_remap(blockIdx.x, tilesX * tilesY);
@@ -69,6 +77,7 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
   TraverseCtx ctx{lds_stack + threadIdx.x, (int)blockDim.x, w.depth + 1};
   unsigned long long nrays = 0;
   TraceCount tc;
+  VolCounters vc{0, 0, 0};
   if (active) {
     const int pix = ix + iy * rp.width;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -91,7 +100,18 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
         else
           world_traverse<TL, false>(w, ray.org, dir, 0.f, ray.tMax, h, ctx, &tc);
         ++nrays;
-        const ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
+        ShadeHit sh = shade_surface<TL>(w, ray.org, dir, ray.tMax, h);
+        if (kVol) {
+          const float tS = sh.hit ? sh.t : ray.tMax;
+          const VolHit vh = volume_intersect(vol, ray.org, dir, 0.f, tS, tS, ray.rng, false, vc);
+          if (vh.hit) {
+            sh.hit = true;
+            sh.P = ray.org + vh.t * dir;
+            sh.N = vec3f(0.f);
+            sh.t = vh.t;
+            sh.bsdf = phase_bsdf(vh.rgb);
+          }
+        }
         if (gen == 0) {
           alpha = sh.hit ? 1.f : ray.missAlpha;
           if (ids && p == 0 && accum0 == 0) {
@@ -116,6 +136,8 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
                                                hit_filter(rpp, shadow.org, sdir, shadow.rng.state));
           else
             shs.hit = world_traverse<TL, true>(w, shadow.org, sdir, 0.f, shadow.tMax, hs, ctx, &tc);
+          if (kVol && !shs.hit)
+            shs.hit = volume_intersect(vol, shadow.org, sdir, 0.f, shadow.tMax, shadow.tMax, shadow.rng, false, vc).hit;
           ++nrays;
           PathRay dummy;
           frag = bounce<kLights>(rpp, shadow, shs, dummy, gen + 1);
@@ -142,10 +164,14 @@ __global__ void __launch_bounds__(256) k_render_surface(WorldDev w, CameraDD cam
     nv += __shfl_xor(nv, off);
     pb += __shfl_xor(pb, off);
   }
+  unsigned long long ns = vc.samples;
+  if (kVol)
+    for (int off = 32; off > 0; off >>= 1) ns += __shfl_xor(ns, off);
   if (lane == 0) {
     atomicAdd(counters + 0, nrays);
     atomicAdd(counters + 1, nv);
     atomicAdd(counters + 2, pb);
+    if (kVol) atomicAdd(counters + 3, ns);
   }
 }
 
@@ -157,49 +183,87 @@ const World &world_of(const bnamd_world *w);
 
 using namespace bnamd;
 
+namespace {
+
+void render_surface_impl(const char *who, const bnamd_world *world, const bnamd_volume *volume,
+                         const bnamd_camera *camera, const bnamd_render_params *params, float *accum, int *ids,
+                         void *stream, bnamd_render_stats *stats, long long *volumeSamples) {
+  BN_REQUIRE(world && camera && params && accum, std::string(who) + ": NULL argument");
+  BN_REQUIRE(params->width > 0 && params->height > 0, std::string(who) + ": empty frame");
+  BN_REQUIRE(params->pathsPerPixel >= 1, std::string(who) + ": pathsPerPixel must be >= 1");
+  const World &w = world_of(world);
+  const bool withVol = volume != nullptr;
+  const VolumeDD vol = withVol ? volume_dd(volume, who) : VolumeDD{};
+  const CameraDD cam = make_camera(*camera);
+  const RenderDD rp = make_render_dd(*params);
+  hipStream_t s = (hipStream_t)stream;
+  const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
+  // xcd_remap hands XCD x the tiles [x*per, (x+1)*per): the grid must hold
+  // 8*per workgroups (a multiple of 8) to cover every tile
+  const int grid = ((tiles + 7) / 8) * 8;
+  const int block = 256;
+  const size_t lds = trace_lds_bytes(w.depth, block);
+  BN_REQUIRE(lds <= 64 * 1024, std::string(who) + ": BVH too deep for the LDS stack");
+  unsigned long long n[4] = {0, 0, 0, 0};
+  const int numCounters = withVol ? 4 : 3;
+  const bool wantStats = stats != nullptr || volumeSamples != nullptr;
+  const float ms = render_on_stream(s, w.counters, numCounters, wantStats, n, [&] {
+    unsigned long long *c = w.counters.ptr;
+    auto go = [&](auto kernel) {
+      kernel<<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, c,
+                                      w.hasCutouts, vol);
+    };
+    const int variant = (w.twoLevel ? 4 : 0) | (has_lights(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0);
+    if (withVol) {
+      switch (variant) {
+        case 0: go(k_render_surface<false, false, false, true>); break;
+        case 1: go(k_render_surface<false, false, true, true>); break;
+        case 2: go(k_render_surface<false, true, false, true>); break;
+        case 3: go(k_render_surface<false, true, true, true>); break;
+        case 4: go(k_render_surface<true, false, false, true>); break;
+        case 5: go(k_render_surface<true, false, true, true>); break;
+        case 6: go(k_render_surface<true, true, false, true>); break;
+        default: go(k_render_surface<true, true, true, true>); break;
+      }
+      return;
+    }
+    switch (variant) {
+      case 0: go(k_render_surface<false, false, false, false>); break;
+      case 1: go(k_render_surface<false, false, true, false>); break;
+      case 2: go(k_render_surface<false, true, false, false>); break;
+      case 3: go(k_render_surface<false, true, true, false>); break;
+      case 4: go(k_render_surface<true, false, false, false>); break;
+      case 5: go(k_render_surface<true, false, true, false>); break;
+      case 6: go(k_render_surface<true, true, false, false>); break;
+      default: go(k_render_surface<true, true, true, false>); break;
+    }
+  });
+  if (stats) {
+    stats->raysTraced = (long long)n[0];
+    stats->nodeVisits = (long long)n[1];
+    stats->primBytes = (long long)n[2];
+    stats->primaryRays = (long long)params->width * params->height * params->pathsPerPixel;
+    stats->ms = ms;
+  }
+  if (volumeSamples) *volumeSamples = (long long)n[3];
+}
+
+}  // namespace
+
 extern "C" int bnamd_render_surface(const bnamd_world *world, const bnamd_camera *camera,
                                     const bnamd_render_params *params, float *accum, int *ids, void *stream,
                                     bnamd_render_stats *stats) {
   return guarded([&] {
-    BN_REQUIRE(world && camera && params && accum, "bnamd_render_surface: NULL argument");
-    BN_REQUIRE(params->width > 0 && params->height > 0, "bnamd_render_surface: empty frame");
-    BN_REQUIRE(params->pathsPerPixel >= 1, "bnamd_render_surface: pathsPerPixel must be >= 1");
-    const World &w = world_of(world);
-    const CameraDD cam = make_camera(*camera);
-    const RenderDD rp = make_render_dd(*params);
-    hipStream_t s = (hipStream_t)stream;
-    const int tiles = div_up(params->width, 16) * div_up(params->height, 16);
-    // xcd_remap hands XCD x the tiles [x*per, (x+1)*per): the grid must hold
-    // 8*per workgroups (a multiple of 8) to cover every tile
-    const int grid = ((tiles + 7) / 8) * 8;
-    const int block = 256;
-    const size_t lds = trace_lds_bytes(w.depth, block);
-    BN_REQUIRE(lds <= 64 * 1024, "bnamd_render_surface: BVH too deep for the LDS stack");
-    unsigned long long n[3] = {0, 0, 0};
-    const float ms = render_on_stream(s, w.counters, 3, stats != nullptr, n, [&] {
-      unsigned long long *c = w.counters.ptr;
-      auto go = [&](auto kernel) {
-        kernel<<<grid, block, lds, s>>>(w.dd(), cam, rp, params->pathsPerPixel, (float4 *)accum, (int4 *)ids, c,
-                                        w.hasCutouts);
-      };
-      const int variant = (w.twoLevel ? 4 : 0) | (has_lights(rp) ? 2 : 0) | ((rp.cutOn || w.hasCutouts) ? 1 : 0);
-      switch (variant) {
-        case 0: go(k_render_surface<false, false, false>); break;
-        case 1: go(k_render_surface<false, false, true>); break;
-        case 2: go(k_render_surface<false, true, false>); break;
-        case 3: go(k_render_surface<false, true, true>); break;
-        case 4: go(k_render_surface<true, false, false>); break;
-        case 5: go(k_render_surface<true, false, true>); break;
-        case 6: go(k_render_surface<true, true, false>); break;
-        default: go(k_render_surface<true, true, true>); break;
-      }
-    });
-    if (stats) {
-      stats->raysTraced = (long long)n[0];
-      stats->nodeVisits = (long long)n[1];
-      stats->primBytes = (long long)n[2];
-      stats->primaryRays = (long long)params->width * params->height * params->pathsPerPixel;
-      stats->ms = ms;
-    }
+    render_surface_impl("bnamd_render_surface", world, nullptr, camera, params, accum, ids, stream, stats, nullptr);
+  });
+}
+
+extern "C" int bnamd_render_mixed(const bnamd_world *world, const bnamd_volume *volume, const bnamd_camera *camera,
+                                  const bnamd_render_params *params, float *accum, int *ids, void *stream,
+                                  bnamd_render_stats *stats, long long *volumeSamples) {
+  return guarded([&] {
+    BN_REQUIRE(volume, "bnamd_render_mixed: NULL volume");
+    render_surface_impl("bnamd_render_mixed", world, volume, camera, params, accum, ids, stream, stats,
+                        volumeSamples);
   });
 }

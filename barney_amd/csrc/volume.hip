@@ -108,6 +108,16 @@ struct bnamd_volume_s {
   Volume v;
 };
 
+namespace bnamd {
+// The device view of a structured volume for the other renderers (a model
+// holding geometry and a volume renders through the surface path tracer).
+const VolumeDD &volume_dd(const bnamd_volume *vol, const char *who) {
+  BN_REQUIRE(vol, std::string(who) + ": NULL volume");
+  BN_REQUIRE(vol->v.haveTF, std::string(who) + ": no transfer function set");
+  return vol->v.dd;
+}
+}  // namespace bnamd
+
 extern "C" int bnamd_volume_create(const bnamd_structured_desc *desc, bnamd_volume **out) {
   return guarded([&] {
     BN_REQUIRE(desc && out, "bnamd_volume_create: NULL argument");

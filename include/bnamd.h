@@ -320,6 +320,19 @@ int bnamd_render_surface(const bnamd_world *world, const bnamd_camera *camera,
                          const bnamd_render_params *params, float *accum, int *ids,
                          void *stream, bnamd_render_stats *stats);
 
+/* A model holding geometry and a structured volume (barney's volumes are
+ * user geometry of the same world, traced by MCVolumeAccel::isProg,
+ * barney/volume/MCAccelerator.h:569-708, next to the surfaces): the surface
+ * path tracer of bnamd_render_surface where each ray's closest surface hit
+ * is found first and the volume is then delta-tracked up to it (a collision
+ * before the surface is a volume hit, shaded with the Phase BSDF); shadow
+ * rays that no surface blocks are tracked through the volume. The volume
+ * needs its transfer function; the ID channels record surface hits.
+ * *volumeSamples (may be NULL) receives the volume samples taken. */
+int bnamd_render_mixed(const bnamd_world *world, const struct bnamd_volume_s *volume, const bnamd_camera *camera,
+                       const bnamd_render_params *params, float *accum, int *ids, void *stream,
+                       bnamd_render_stats *stats, long long *volumeSamples);
+
 /* ------------------------------------------------------------------------
  * Structured volumes: StructuredData + MCGrid + TransferFunction +
  * MCVolumeAccel (barney/volume/).

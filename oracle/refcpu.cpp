@@ -1877,9 +1877,26 @@ void ref_camera_ray(const ref_camera *cam, const ref_render_params *p, int ix, i
   *seed = r.rng.s;
 }
 
-long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const ref_render_params *p, float *accum,
-                             int *ids, int y0, int y1, int threads) {
+}  // extern "C"
+
+struct ref_volume_s;
+
+namespace {
+
+// the structured volume's MCVolumeAccel::isProg (defined with the volumes)
+bool vol_isect_fwd(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed,
+                   long long &samples, float &t, V3 &rgb);
+
+// The surface path tracer; with a volume (a model holding geometry and a
+// structured volume), each ray's closest surface hit is found first and the
+// volume is then delta-tracked up to it (MCVolumeAccel::isProg with the
+// surface hit as the ray's tMax): a collision before the surface is a volume
+// hit. Shadow rays not blocked by a surface are tracked through the volume.
+long long render_surface_g(const ref_scene *s, const ref_volume_s *vol, const ref_camera *cam,
+                           const ref_render_params *p, float *accum, int *ids, int y0, int y1, int threads,
+                           long long *samples) {
   const Cam c = make_cam(cam);
+  std::atomic<long long> nsamples(0);
   const int W = p->width;
   const int rows = y1 - y0;
   std::atomic<long long> nrays(0);
@@ -1889,7 +1906,7 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
     const int ix = li % W, iy = y0 + li / W;
     const int pix = ix + iy * W;
     float acc[4] = {0, 0, 0, 0};
-    long long nr = 0;
+    long long nr = 0, vcnt = 0;
     for (int pp = 0; pp < p->pathsPerPixel; ++pp) {
       const int a = p->accumID + pp;
       PRay ray = cam_ray(c, p, ix, iy, a);
@@ -1910,6 +1927,19 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
           const ref_geom &g = s->geoms[b.geom];
           const V3 col = g.colors ? ld3(g.colors + 3 * b.prim) : ld3(g.color);
           material_hit(g, col, hi);
+        }
+        if (vol) {
+          const float tS = hi.hit ? hi.t : ray.tMax;
+          float vt;
+          V3 rgb;
+          if (vol_isect_fwd(vol, ray.org, d, 0.f, tS, tS, ray.rng, vcnt, vt, rgb)) {
+            hi.hit = true;
+            hi.P = add(ray.org, scl(vt, d));
+            hi.N = mk(0, 0, 0);
+            hi.t = vt;
+            hi.type = 2;
+            hi.albedo = scl(.7f, rgb);  // Phase(color), Phase.h:137-141
+          }
         }
         if (gen == 0) {
           alpha = hi.hit ? 1.f : ray.missA;
@@ -1944,6 +1974,11 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
           ++nr;
           HitInfo hs;
           hs.hit = bs.geom >= 0;
+          if (vol && !hs.hit) {
+            float vt;
+            V3 rgb;
+            hs.hit = vol_isect_fwd(vol, sh.org, sd, 0.f, sh.tMax, sh.tMax, sh.rng, vcnt, vt, rgb);
+          }
           PRay dummy;
           add_frag(sum, bounce(p, sh, hs, dummy, gen + 1, maxDepth, maxDiff), cmax);
         }
@@ -1957,8 +1992,25 @@ long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const re
       o[0] += acc[0]; o[1] += acc[1]; o[2] += acc[2]; o[3] += acc[3];
     }
     nrays += nr;
+    nsamples += vcnt;
   });
+  if (samples) *samples = nsamples.load();
   return nrays.load();
+}
+
+}  // namespace
+
+extern "C" {
+
+long long ref_render_surface(const ref_scene *s, const ref_camera *cam, const ref_render_params *p, float *accum,
+                             int *ids, int y0, int y1, int threads) {
+  return render_surface_g(s, nullptr, cam, p, accum, ids, y0, y1, threads, nullptr);
+}
+
+long long ref_render_mixed(const ref_scene *s, const ref_volume *v, const ref_camera *cam,
+                           const ref_render_params *p, float *accum, int *ids, int y0, int y1, int threads,
+                           long long *samples) {
+  return render_surface_g(s, v, cam, p, accum, ids, y0, y1, threads, samples);
 }
 
 }  // extern "C"
@@ -2167,6 +2219,14 @@ VHit vol_isect(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, fl
                long long &samples) {
   return vol_isect_g(v, [&](V3 P) { return vsample(v, P); }, org, dir, tmin, tmax, rayTMax, seed, exactLog, samples,
                      fnv3(0, 0, 0));
+}
+
+bool vol_isect_fwd(const ref_volume_s *v, V3 org, V3 dir, float tmin, float tmax, float rayTMax, Rng &seed,
+                   long long &samples, float &t, V3 &rgb) {
+  const VHit h = vol_isect(v, org, dir, tmin, tmax, rayTMax, seed, false, samples);
+  t = h.t;
+  rgb = mk(h.rgb[0], h.rgb[1], h.rgb[2]);
+  return h.hit;
 }
 
 }  // namespace

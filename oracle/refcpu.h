@@ -171,6 +171,13 @@ int ref_dda_cells(const float *org, const float *dir, float tMax, const int *gri
                   float *ts, int maxCells);
 long long ref_render_volume(const ref_volume *v, const ref_camera *cam, const ref_render_params *p,
                             float *accum, int y0, int y1, int threads, long long *samples);
+/* A model holding geometry and a structured volume: the surface path
+ * tracer, each ray delta-tracked through the volume up to its closest
+ * surface hit; shadow rays not blocked by a surface are tracked through the
+ * volume. Returns rays traced; *samples = volume samples. */
+long long ref_render_mixed(const ref_scene *s, const ref_volume *v, const ref_camera *cam,
+                           const ref_render_params *p, float *accum, int *ids, int y0, int y1, int threads,
+                           long long *samples);
 
 /* --- block-structured AMR (barney/amr) --- */
 typedef struct ref_amr_s ref_amr;

@@ -81,6 +81,7 @@ def load():
         "ref_triangle_t_f64": (C.c_double, [P, P, P, P, P]),
         "ref_camera_ray": (None, [P, P, C.c_int, C.c_int, P, P, P]),
         "ref_render_surface": (C.c_longlong, [P, P, P, P, P, C.c_int, C.c_int, C.c_int]),
+        "ref_render_mixed": (C.c_longlong, [P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, P]),
         "ref_volume_create": (P, [P, P, P, P, C.c_int]),
         "ref_volume_destroy": (None, [P]),
         "ref_volume_set_tf": (None, [P, P, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -229,6 +230,18 @@ class Scene:
         n = load().ref_render_surface(self.handle, C.byref(cam), C.byref(params), acc.ctypes.data, _ptr(ids), y0,
                                       y1, threads)
         return acc, ids, n
+
+    def render_mixed(self, volume, cam, params, y0=0, y1=None, threads=0, with_ids=False, accum=None):
+        """The scene together with a structured Volume: each ray's closest
+        surface hit first, then delta tracking through the volume up to it.
+        Returns (accum, ids, rays traced, volume samples)."""
+        y1 = params.height if y1 is None else y1
+        acc = accum if accum is not None else np.zeros((params.height, params.width, 4), np.float32)
+        ids = np.zeros((params.height, params.width, 4), np.int32) if with_ids else None
+        ns = C.c_longlong(0)
+        n = load().ref_render_mixed(self.handle, volume.handle, C.byref(cam), C.byref(params), acc.ctypes.data,
+                                    _ptr(ids), y0, y1, threads, C.byref(ns))
+        return acc, ids, n, ns.value
 
     def __del__(self):
         try:

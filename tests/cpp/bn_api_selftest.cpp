@@ -240,6 +240,27 @@ int main(int argc, char **argv) {
     for (int q = 0; q < W * H; ++q)
       if (prim[q] != -1 || !std::isinf(depth[q])) return fail("volume model ID channels not cleared");
     std::printf("volume samples %lld\n", vs.samples);
+    // geometry and the volume in one model (two instances) through bn*
+    // equal bnamd_render_mixed with the same world and volume
+    {
+      BNGroup both[2] = {group, vg};
+      BNTransform bx[2] = {xfm, xfm};
+      BNModel mm = bnModelCreate(ctx);
+      bnSetInstances(mm, 0, both, bx, 2);
+      bnBuild(mm, 0);
+      bnAccumReset(fb);
+      bnRender(rnd, mm, cam, fb);
+      bnFrameBufferRead(fb, BN_FB_COLOR, color.data(), BN_FLOAT4);
+      long long ms = 0;
+      if (bnamd_render_mixed(world, v, &c, &p, (float *)acc, nullptr, nullptr, nullptr, &ms) != BNAMD_OK)
+        return fail(bnamd_last_error());
+      bnamd_fb_resolve((const float *)acc, W, H, 1.f / SPP, BNAMD_FB_FLOAT4, out, nullptr);
+      bnamd_synchronize();
+      bnamd_memcpy_d2h(rc.data(), out, rc.size() * 4);
+      if (std::memcmp(rc.data(), color.data(), rc.size() * 4) != 0) return fail("mixed model differs");
+      if (ms <= 0) return fail("mixed model: volume not sampled");
+      std::printf("geometry + volume model: %lld volume samples, bn* = C ABI\n", ms);
+    }
     bnamd_volume_destroy(v);
     bnamd_device_free(dfield);
   }
