@@ -609,7 +609,7 @@ def fb_bench(api, reps=20):
 # the sources a kernel's code is built from: a traffic figure measured on
 # other sources is not this kernel's
 KERNEL_SOURCES = {"k_render_volume": ["volume.hip", "volume_render.h", "volume.h", "shade.h", "envmap.h", "fmath.h",
-                                      "common.h", "world.h", "bvh8.h", "runtime.h", "amr.h"]}
+                                      "common.h", "world.h", "bvh8.h", "runtime.h", "amr.h", "lights.h"]}
 
 
 def sources_sha256(kernel):
