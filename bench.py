@@ -288,6 +288,60 @@ def surface_bench(api, steps=10, warmup=2, cpu=True, cpu_seconds=10.0):
     return res
 
 
+def lit_mixed_bench(api, steps=10, warmup=2, spp=4, cpu=True, cpu_seconds=8.0):
+    """Workload 1's spheres lit by a quad, a directional and two point lights
+    next to the ambient environment, with a 128^3 blob-and-noise structured
+    volume among them (a model holding geometry and a volume):
+    bnamd_render_mixed at 1024x768, 4 paths per pixel."""
+    from barney_amd import scenes
+    from oracle import refcpu
+    w1 = scenes.workload1()
+    geoms, mats = [], []
+    for k, (v, t, c) in enumerate(w1["geoms"]):
+        geoms.append(api.Geom(api.GEOM_TRIANGLES, v, t, material=k, user_id=k))
+        mats.append(api.matte(c))
+    world = api.World([geoms], materials=mats)
+    quads = [[20.0, 110.0, 20.0, 60.0, 0.0, 0.0, 0.0, 0.0, 60.0, 3.0, 3.0, 2.5]]
+    dirs = [[0.3, -1.0, 0.4, 1.0, 0.95, 0.9, 1.5]]
+    points = [[50.0, 60.0, 50.0, 1.0, 0.8, 0.6, 4.0e4, float("nan")], [10.0, 30.0, 80.0, 0.5, 0.7, 1.0, 1.0, 300.0]]
+    n = 128
+    f = scenes.structured_field(n=n, blobs=12, sigma=(4.0, 16.0), octaves=3, seed=0x5EED0906)
+    vkw = dict(origin=(30.0, 20.0, 30.0), spacing=(40.0 / (n - 1),) * 3, cells_per_mc=8)
+    vol = api.Volume(f, **vkw)
+    vol.set_tf(scenes.ramp_tf(), (0.0, 1.0), 0.3)
+    lights = api.Lights(quads, dirs, points)
+    W, H = w1["size"]
+    cam = api.camera(**w1["camera"])
+    acc = api.DeviceArray.empty((H, W, 4), np.float32)
+    params = api.render_params(W, H, spp=spp, ambient=0.5, lights=lights)
+    for _ in range(warmup):
+        api.render_mixed(world, vol, cam, params, accum=acc)
+    ms = []
+    for _ in range(steps):  # every frame traces the same paths
+        o = api.render_mixed(world, vol, cam, params, accum=acc)
+        ms.append(o["stats"]["ms"])
+    t = float(np.median(ms))
+    rays, samples = o["stats"]["raysTraced"], o["stats"]["samples"]
+    res = {"ms_per_frame": t, "frames": steps, "spp": spp, "rays_per_frame": rays,
+           "Mrays_per_s": rays / (t * 1e-3) / 1e6, "volume_samples_per_frame": samples,
+           "lights": {"quad": 1, "directional": 1, "point": 2}, "volume": n}
+    if cpu:
+        ref = refcpu.Scene([dict(kind=0, vertices=v, indices=t, color=c, user_id=k)
+                            for k, (v, t, c) in enumerate(w1["geoms"])])
+        rv = refcpu.Volume(f, **vkw)
+        rv.set_tf(scenes.ramp_tf(), (0.0, 1.0), 0.3)
+        rl = refcpu.Lights(quads, dirs, points)
+        rp = refcpu.render_params(W, H, spp=spp, ambient=0.5, lights=rl)
+        racc = np.zeros((H, W, 4), np.float32)
+        rcam = refcpu.camera(**w1["camera"])
+        base = band_timer(lambda y0, y1: ref.render_mixed(rv, rcam, rp, y0, y1, cpu_threads(), accum=racc)[2], H,
+                          cpu_seconds, "Mrays/s", f"the {W}x{H} frame, {spp} spp", keep_rows=True)
+        y0, y1 = base["_rows"]
+        res["vs_oracle"] = band_vs_oracle(acc.numpy(), base, racc[y0:y1])
+        res["cpu_baseline"] = base
+    return res
+
+
 def batch_bench(api, steps=10, cpu=True, cpu_seconds=10.0):
     """Workload 3: 10M-triangle heightfield + 200k spheres + 100k cylinders,
     60/30/10 diffuse/glossy/dielectric, depth <= 8 with Russian roulette,
@@ -783,6 +837,7 @@ def main():
     }
     if not args.no_surface:
         out["extra"]["surface_workload1"] = surface_bench(api, cpu=not args.no_cpu)
+        out["extra"]["lit_mixed_workload1"] = lit_mixed_bench(api, cpu=not args.no_cpu)
     if not args.no_batch:
         out["extra"]["batch_workload3"] = batch_bench(api, cpu=not args.no_cpu)
     if not args.no_umesh:
