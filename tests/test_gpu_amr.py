@@ -65,3 +65,23 @@ def test_render_per_pixel(field, gpu, spp):
     assert out["stats"]["raysTraced"] == nr
     assert out["stats"]["samples"] == ns
     assert ra[..., 3].mean() > 0.2 * spp
+
+
+def test_render_with_lights_bit_exact(field, gpu):
+    """Quad, directional and point lights next to the ambient light: the
+    lit instantiation of the shared volume path tracer equals the oracle."""
+    f, g, r = field
+    e = float(f["extent"])
+    c = e / 2.0
+    quads = [[0.0, 1.5 * e, 0.0, e, 0.0, 0.0, 0.0, 0.0, e, 3.0, 3.0, 3.0]]
+    dirs = [[0.3, -1.0, 0.2, 1.0, 0.9, 0.8, 1.5]]
+    points = [[c, c, c, 1.0, 1.0, 1.0, 4.0 * e * e, float("nan")]]
+    W, H, spp = 64, 48, 4
+    pos = (c - 1.3 * e, c + 0.7 * e, c - 1.6 * e)
+    d = (c - pos[0], c - pos[1], c - pos[2])
+    out = g.render(gpu.camera(pos, d, fovy=35.0),
+                   gpu.render_params(W, H, spp=spp, ambient=0.3, lights=gpu.Lights(quads, dirs, points)))
+    ra, nr, ns = r.render(refcpu.camera(pos, d, fovy=35.0),
+                          refcpu.render_params(W, H, spp=spp, ambient=0.3, lights=refcpu.Lights(quads, dirs, points)))
+    assert np.array_equal(out["accum"].numpy().view(np.uint32), ra.view(np.uint32))
+    assert out["stats"]["raysTraced"] == nr and out["stats"]["samples"] == ns

@@ -128,3 +128,24 @@ def test_render_bit_exact(blob, gpu, spp):
     assert np.array_equal(ga.view(np.uint32), ra.view(np.uint32))
     assert out["stats"]["raysTraced"] == nr and out["stats"]["samples"] == ns
     assert ra[..., 3].mean() > 0.1 * spp and ns > 1000
+
+
+def test_render_with_lights_bit_exact(blob, gpu):
+    """Quad, directional and point lights next to the ambient light: the
+    lit instantiation of the shared volume path tracer equals the oracle."""
+    vt, n, g, r = blob
+    b = r.boxes()
+    c = (b["world_lo"] + b["world_hi"]) / 2
+    ext = float((b["world_hi"] - b["world_lo"]).max())
+    quads = [[c[0] - ext / 2, c[1] + ext, c[2] - ext / 2, ext, 0.0, 0.0, 0.0, 0.0, ext, 3.0, 3.0, 3.0]]
+    dirs = [[0.3, -1.0, 0.2, 1.0, 0.9, 0.8, 1.5]]
+    points = [[c[0], c[1], c[2], 1.0, 1.0, 1.0, 4.0 * ext * ext, float("nan")]]
+    W, H, spp = 48, 36, 4
+    pos = (c[0] - 1.2 * ext, c[1] + 0.6 * ext, c[2] - 1.5 * ext)
+    d = (c[0] - pos[0], c[1] - pos[1], c[2] - pos[2])
+    out = g.render(gpu.camera(pos, d, fovy=40.0),
+                   gpu.render_params(W, H, spp=spp, ambient=0.3, lights=gpu.Lights(quads, dirs, points)))
+    ra, nr, ns = r.render(refcpu.camera(pos, d, fovy=40.0),
+                          refcpu.render_params(W, H, spp=spp, ambient=0.3, lights=refcpu.Lights(quads, dirs, points)))
+    assert np.array_equal(out["accum"].numpy().view(np.uint32), ra.view(np.uint32))
+    assert out["stats"]["raysTraced"] == nr and out["stats"]["samples"] == ns
