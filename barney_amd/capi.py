@@ -225,6 +225,8 @@ SIGNATURES = {
     "bnamd_envmap_create": (C.c_int, [C.c_void_p, C.POINTER(bnamd_envmap_desc), C.c_void_p,
                                       C.POINTER(C.c_void_p)]),
     "bnamd_envmap_destroy": (C.c_int, [C.c_void_p]),
+    "bnamd_texture2d_device_view": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                              C.POINTER(C.c_int)]),
     "bnamd_lights_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                       C.POINTER(C.c_void_p)]),
     "bnamd_lights_destroy": (C.c_int, [C.c_void_p]),

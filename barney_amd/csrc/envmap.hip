@@ -193,6 +193,16 @@ extern "C" int bnamd_texture2d_destroy(bnamd_texture2d *tex) {
   return guarded([&] { delete tex; });
 }
 
+extern "C" int bnamd_texture2d_device_view(const bnamd_texture2d *tex, const void **texels4, int *width,
+                                           int *height) {
+  return guarded([&] {
+    BN_REQUIRE(tex && texels4 && width && height, "bnamd_texture2d_device_view: NULL argument");
+    *texels4 = tex->dd.texels;
+    *width = tex->dd.w;
+    *height = tex->dd.h;
+  });
+}
+
 extern "C" int bnamd_texture2d_sample(const bnamd_texture2d *tex, const float *uv, float *out, int n, void *stream) {
   return guarded([&] {
     check_tex(tex, "bnamd_texture2d_sample");

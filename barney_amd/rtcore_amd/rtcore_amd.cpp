@@ -85,10 +85,29 @@ Texture::Texture(TextureData *data, const TextureDesc &desc) : device(data->devi
     td.addressMode[1] = (int)desc.addressMode[1];
     check(bnamd_texture2d_create(&td, nullptr, &tex2d), "rtc::amd::Texture");
   }
+  // the device descriptor getDD() points to
+  TextureObjectDD o{};
+  o.filter = (int)desc.filterMode;
+  for (int k = 0; k < 3; ++k) o.addr[k] = (int)desc.addressMode[k];
+  if (tex2d) {
+    o.dim = 2;
+    check(bnamd_texture2d_device_view(tex2d, &o.texels, &o.w, &o.h), "rtc::amd::Texture");
+    o.d = 1;
+  } else {
+    if (data->format != FLOAT) throw std::runtime_error("rtc::amd::Texture: 3D textures hold FLOAT texels");
+    o.dim = 3;
+    o.w = data->dims.x;
+    o.h = data->dims.y;
+    o.d = data->dims.z;
+    o.texels = data->d_texels;
+  }
+  check(bnamd_device_malloc(&d_obj, sizeof o), "rtc::amd::Texture");
+  check(bnamd_memcpy_h2d(d_obj, &o, sizeof o), "rtc::amd::Texture");
 }
 
 Texture::~Texture() {
   if (tex2d) (void)bnamd_texture2d_destroy(tex2d);
+  if (d_obj) (void)bnamd_device_free(d_obj);
 }
 
 EnvMapLight::EnvMapLight(Texture *texture, const float direction[3], const float up[3], float scale)

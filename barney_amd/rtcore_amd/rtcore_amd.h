@@ -175,19 +175,32 @@ struct TextureData {
   std::vector<unsigned char> hostTexels;  // 2D images only
 };
 
+// What a TextureObject points to on the device: the sampling description
+// and the texels, so that device code samples it with tex2D / tex3D
+// (rtcore_amd_device.h) as barney's programs do.
+struct TextureObjectDD {
+  int dim;               // 2: image (normalized coordinates), 3: scalar field
+  int w, h, d;
+  int filter;            // FilterMode
+  int addr[3];           // AddressMode per axis
+  const void *texels;    // 2D: w*h float4 (widened as tex2D reads them); 3D: w*h*d float, x fastest
+};
+
 // A view of texel data with its sampling description
 // (rtcore/cudaCommon/Texture.h:12-24). getDD() is the TextureObject the
-// device side of barney stores: the device address of the texels for a 3D
-// field, the bnamd_texture2d handle (tex2D as the embree backend samples
-// it) for a 2D image.
+// device side of barney stores: the device address of a TextureObjectDD.
+// The native entry points take the texels themselves (texels3D() for a
+// structured field) or the bnamd_texture2d handle (tex2d, for an env map).
 struct Texture {
   Texture(TextureData *data, const TextureDesc &desc);
   ~Texture();
-  TextureObject getDD() const { return tex2d ? (TextureObject)tex2d : (TextureObject)data->d_texels; }
+  TextureObject getDD() const { return (TextureObject)d_obj; }
+  const float *texels3D() const { return data->dims.z > 0 ? (const float *)data->d_texels : nullptr; }
   Device *const device;
   TextureData *const data;
   const TextureDesc desc;
   bnamd_texture2d *tex2d = nullptr;
+  void *d_obj = nullptr;  // the TextureObjectDD
 };
 
 // barney's EnvMapLight (barney/light/EnvMap.cu:59-230) over a 2D texture:

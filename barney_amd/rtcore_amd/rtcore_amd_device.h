@@ -29,6 +29,8 @@
 
 #include "rtcore_amd.h"
 #include "../csrc/world.h"
+#include "../csrc/envmap.h"
+#include "../csrc/volume.h"
 
 namespace rtc {
 namespace amd {
@@ -36,6 +38,72 @@ namespace amd {
 struct vec2f {
   float x, y;
 };
+
+// ------------------------------------------------------------------ textures
+// tex2D / tex3D on a TextureObject (the device address of a
+// TextureObjectDD, Texture::getDD()), sampled as the embree backend's
+// TextureSampler does (rtcore/embree/Texture.cpp): tex2D with normalized
+// coordinates and the texture's filter and per-axis addressing (the
+// native renderers' sampler, envmap.h); tex3D with unnormalized
+// coordinates, texel centres at +0.5 and clamping (lerpCoords_notNormalized
+// and the trilinear blend of Texture.cpp:324-390, as the structured-volume
+// sampler of volume.h), FLOAT texels.
+__device__ inline float4 tex2d_sample(TextureObject t, float u, float v) {
+  const TextureObjectDD &o = *(const TextureObjectDD *)t;
+  bnamd::Tex2DDD d;
+  d.texels = (const float4 *)o.texels;
+  d.w = o.w;
+  d.h = o.h;
+  d.filter = o.filter;
+  d.addr[0] = o.addr[0];
+  d.addr[1] = o.addr[1];
+  d.pad = 0;
+  return bnamd::tex2d(d, u, v);
+}
+
+__device__ inline float tex3d_sample(TextureObject t, float x, float y, float z) {
+#pragma clang fp contract(off)
+  const TextureObjectDD &o = *(const TextureObjectDD *)t;
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  bnamd::lerp_coords(o.w, x, x0, x1, fx);
+  bnamd::lerp_coords(o.h, y, y0, y1, fy);
+  bnamd::lerp_coords(o.d, z, z0, z1, fz);
+  const float *s = (const float *)o.texels;
+  const size_t sy = (size_t)o.w, sz = (size_t)o.w * o.h;
+  auto at = [&](int i, int j, int k) { return s[(size_t)i + sy * j + sz * k]; };
+  const float v00 = bnamd::lerp_l(fx, at(x0, y0, z0), at(x1, y0, z0));
+  const float v01 = bnamd::lerp_l(fx, at(x0, y1, z0), at(x1, y1, z0));
+  const float v10 = bnamd::lerp_l(fx, at(x0, y0, z1), at(x1, y0, z1));
+  const float v11 = bnamd::lerp_l(fx, at(x0, y1, z1), at(x1, y1, z1));
+  const float v0 = bnamd::lerp_l(fy, v00, v01);
+  const float v1 = bnamd::lerp_l(fy, v10, v11);
+  return bnamd::lerp_l(fz, v0, v1);
+}
+
+template <class T>
+__device__ T tex2D(TextureObject t, float u, float v);
+template <>
+__device__ inline vec4f tex2D<vec4f>(TextureObject t, float u, float v) {
+  const float4 c = tex2d_sample(t, u, v);
+  return {c.x, c.y, c.z, c.w};
+}
+template <>
+__device__ inline float tex2D<float>(TextureObject t, float u, float v) {
+  return tex2d_sample(t, u, v).x;
+}
+
+template <class T>
+__device__ T tex3D(TextureObject t, float x, float y, float z);
+template <>
+__device__ inline float tex3D<float>(TextureObject t, float x, float y, float z) {
+  return tex3d_sample(t, x, y, z);
+}
+template <>
+__device__ inline vec4f tex3D<vec4f>(TextureObject t, float x, float y, float z) {
+  const float v = tex3d_sample(t, x, y, z);  // FLOAT texels read as (v, v, v, v)
+  return {v, v, v, v};
+}
 
 // ------------------------------------------------------------------ compute
 struct ComputeInterface {

@@ -248,6 +248,10 @@ typedef struct {
 } bnamd_texture2d_desc;
 int bnamd_texture2d_create(const bnamd_texture2d_desc *desc, void *stream, bnamd_texture2d **out);
 int bnamd_texture2d_destroy(bnamd_texture2d *tex);
+/* The device copy tex2D samples: width*height float4 texels (the texels
+ * widened as getTexel does, rtcore/embree/Texture.cpp:166-214), row-major.
+ * For device-side descriptors (rtc::amd's TextureObject). */
+int bnamd_texture2d_device_view(const bnamd_texture2d *tex, const void **texels4, int *width, int *height);
 /* tex2D<float4> at n (u, v) pairs: uv device float2[n], out device float4[n] */
 int bnamd_texture2d_sample(const bnamd_texture2d *tex, const float *uv, float *out, int n, void *stream);
 

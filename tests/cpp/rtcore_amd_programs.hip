@@ -122,8 +122,28 @@ struct ClearOut {
   }
 };
 
+// samples a 2D image and a 3D field through their TextureObjects, as
+// barney's material and structured-volume programs do
+struct TexSample {
+  TextureObject img, field;
+  const float *uvw;  // n x 3: (u, v) for the image, (x, y, z) for the field
+  float *out;        // n x 5: tex2D<vec4f> then tex3D<float>
+  int n;
+  __device__ void run(const ComputeInterface &ci) const {
+    const int i = (int)ci.launchIndex().x;
+    if (i >= n) return;
+    const vec4f c = tex2D<vec4f>(img, uvw[3 * i], uvw[3 * i + 1]);
+    out[5 * i] = c.x;
+    out[5 * i + 1] = c.y;
+    out[5 * i + 2] = c.z;
+    out[5 * i + 3] = c.w;
+    out[5 * i + 4] = tex3D<float>(field, uvw[3 * i] * 7.f, uvw[3 * i + 1] * 6.f, uvw[3 * i + 2] * 5.f);
+  }
+};
+
 }  // namespace
 
+RTC_EXPORT_COMPUTE1D(texSample, TexSample)
 RTC_EXPORT_TRIANGLES_GEOM(TestTriangles, TestTriDD, TriPrograms, true, true)
 RTC_EXPORT_USER_GEOM(TestSpheres, SphereDD, SpherePrograms, false, true)
 RTC_EXPORT_TRACE2D(testTrace, RayGen)
@@ -341,6 +361,82 @@ int main() {
   dev.freeBuffer(vb);
   dev.freeBuffer(ib);
   dev.freeBuffer(sb);
+  // tex2D / tex3D through TextureObjects: the image equals the native 2D
+  // sampler (bnamd_texture2d_sample) bit for bit, the field equals the
+  // embree backend's unnormalized trilinear restated on the host
+  {
+    const int TW = 9, TH = 5;
+    std::vector<float> img(4 * TW * TH);
+    for (size_t i = 0; i < img.size(); ++i) img[i] = 0.1f + 0.37f * (float)((i * 7) % 23);
+    TextureData *itd = dev.createTextureData({TW, TH, 0}, FLOAT4, img.data());
+    TextureDesc idesc;
+    idesc.addressMode[0] = WRAP;
+    idesc.addressMode[1] = MIRROR;
+    Texture *itex = itd->createTexture(idesc);
+    const int FX = 6, FY = 5, FZ = 4;
+    std::vector<float> field(FX * FY * FZ);
+    for (size_t i = 0; i < field.size(); ++i) field[i] = std::sin(0.3f * (float)i) + 0.01f * (float)i;
+    TextureData *ftd = dev.createTextureData({FX, FY, FZ}, FLOAT, field.data());
+    TextureDesc fdesc;
+    fdesc.normalizedCoords = false;
+    Texture *ftex = ftd->createTexture(fdesc);
+    const int m = 4096;
+    std::vector<float> uvw(3 * m);
+    uint32_t st = 12345u;
+    for (float &x : uvw) {
+      st = st * 1664525u + 1013904223u;
+      x = -0.3f + 1.6f * (float)(st >> 8) * (1.f / 16777216.f);
+    }
+    void *dUvw = nullptr, *dTo = nullptr, *dUv = nullptr, *dNat = nullptr;
+    rtc::amd::check(bnamd_device_malloc(&dUvw, uvw.size() * 4), "malloc");
+    rtc::amd::check(bnamd_device_malloc(&dTo, 5 * m * 4), "malloc");
+    rtc::amd::check(bnamd_device_malloc(&dUv, 2 * m * 4), "malloc");
+    rtc::amd::check(bnamd_device_malloc(&dNat, 4 * m * 4), "malloc");
+    rtc::amd::check(bnamd_memcpy_h2d(dUvw, uvw.data(), uvw.size() * 4), "copy");
+    std::vector<float> uv(2 * m);
+    for (int i = 0; i < m; ++i) {
+      uv[2 * i] = uvw[3 * i];
+      uv[2 * i + 1] = uvw[3 * i + 1];
+    }
+    rtc::amd::check(bnamd_memcpy_h2d(dUv, uv.data(), uv.size() * 4), "copy");
+    ComputeKernel1D *ts = createCompute_texSample(&dev);
+    TexSample tsd{itex->getDD(), ftex->getDD(), (const float *)dUvw, (float *)dTo, m};
+    ts->launch((m + 127) / 128, 128, &tsd);
+    dev.sync();
+    rtc::amd::check(bnamd_texture2d_sample(itex->tex2d, (const float *)dUv, (float *)dNat, m, nullptr), "sample");
+    std::vector<float> got(5 * m), nat(4 * m);
+    rtc::amd::check(bnamd_memcpy_d2h(got.data(), dTo, got.size() * 4), "copy");
+    rtc::amd::check(bnamd_memcpy_d2h(nat.data(), dNat, nat.size() * 4), "copy");
+    auto coords = [](int N, float tc, int &i0, int &i1, float &f) {
+      if (tc - .5f <= 0.f) { i0 = i1 = 0; f = 0.f; return; }
+      if (tc - .5f >= N - 1) { i0 = i1 = N - 1; f = 0.f; return; }
+      f = (tc - .5f) - int(tc - .5f);
+      i0 = int(tc - .5f);
+      i1 = i0 + 1;
+    };
+    auto lerp = [](float f, float a, float b) { return (1.f - f) * a + f * b; };
+    for (int i = 0; i < m; ++i) {
+      for (int k = 0; k < 4; ++k)
+        if (std::memcmp(&got[5 * i + k], &nat[4 * i + k], 4) != 0) return fail("tex2D differs from the native sampler", i);
+      int x0, x1, y0, y1, z0, z1;
+      float fx, fy, fz;
+      coords(FX, uvw[3 * i] * 7.f, x0, x1, fx);
+      coords(FY, uvw[3 * i + 1] * 6.f, y0, y1, fy);
+      coords(FZ, uvw[3 * i + 2] * 5.f, z0, z1, fz);
+      auto at = [&](int a, int b, int c) { return field[a + FX * (b + FY * c)]; };
+      const float v0 = lerp(fy, lerp(fx, at(x0, y0, z0), at(x1, y0, z0)), lerp(fx, at(x0, y1, z0), at(x1, y1, z0)));
+      const float v1 = lerp(fy, lerp(fx, at(x0, y0, z1), at(x1, y0, z1)), lerp(fx, at(x0, y1, z1), at(x1, y1, z1)));
+      const float e = lerp(fz, v0, v1);
+      if (std::memcmp(&got[5 * i + 4], &e, 4) != 0) return fail("tex3D differs from the trilinear restatement", i);
+    }
+    std::printf("tex2D / tex3D through TextureObjects: %d samples equal\n", m);
+    delete ts;
+    for (void *b : {dUvw, dTo, dUv, dNat}) bnamd_device_free(b);
+    dev.freeTexture(itex);
+    dev.freeTexture(ftex);
+    dev.freeTextureData(itd);
+    dev.freeTextureData(ftd);
+  }
   std::printf("OK\n");
   return 0;
 }

@@ -272,10 +272,14 @@ int main() {
     TextureDesc desc;
     desc.normalizedCoords = false;
     Texture *tex = td->createTexture(desc);
-    CHECK((void *)tex->getDD() == td->d_texels && td->numBytes == 24 * sizeof(float));
+    CHECK(tex->texels3D() == (const float *)td->d_texels && td->numBytes == 24 * sizeof(float));
     std::vector<float> back(24, -1.f);
-    dev->copy(back.data(), (const void *)tex->getDD(), 24 * sizeof(float));
+    dev->copy(back.data(), tex->texels3D(), 24 * sizeof(float));
     CHECK(back == texels);
+    // getDD(): the device descriptor device code samples with tex3D
+    TextureObjectDD o{};
+    dev->copy(&o, (const void *)tex->getDD(), sizeof o);
+    CHECK(o.dim == 3 && o.w == 4 && o.h == 3 && o.d == 2 && o.texels == td->d_texels);
     threw = false;
     try {
       delete td->createTexture(TextureDesc());  // normalized coordinates on a 3D field
@@ -296,7 +300,10 @@ int main() {
     TextureDesc desc;
     desc.addressMode[0] = WRAP;
     Texture *tex = td->createTexture(desc);
-    CHECK(tex->tex2d != nullptr && (void *)tex->getDD() == (void *)tex->tex2d);
+    CHECK(tex->tex2d != nullptr);
+    TextureObjectDD o{};
+    dev->copy(&o, (const void *)tex->getDD(), sizeof o);
+    CHECK(o.dim == 2 && o.w == 8 && o.h == 4 && o.addr[0] == WRAP && o.addr[1] == CLAMP && o.texels != nullptr);
     const float dir[3] = {1.f, 0.f, 0.f}, up[3] = {0.f, 0.f, 1.f};
     EnvMapLight *env = dev->createEnvMapLight(tex, dir, up, 2.f);
     std::vector<float> cy(4), cx(32);
